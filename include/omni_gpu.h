@@ -1,0 +1,190 @@
+/*
+ * omni_gpu.h -- C ABI of the MI355X (gfx950) hash-aggregation and hash-join hot path.
+ *
+ * The reference library (OmniOperator) runs group-by aggregation and hash join
+ * as single-threaded CPU operators behind
+ *     Operator::AddInput(VectorBatch*) / Operator::GetOutput(VectorBatch**)
+ * (core/src/operator/operator.h:22-264).  This header is the thin, plain-C
+ * boundary under GPU-backed operators with the same surface
+ * (omnioperator_amd/host/gpu_operators.h).  Every entry point takes plain
+ * pointers and sizes; no C++ type and no exception crosses it.
+ *
+ * Conventions
+ *  - Every function returns OMNI_GPU_OK (0) or a negative OMNI_GPU_ERR_* code;
+ *    omni_gpu_last_error() returns the message of the last failure on the
+ *    calling thread.
+ *  - Column pointers passed to *_add / *_probe / *_submit are DEVICE pointers
+ *    (memory from omni_gpu_malloc or any hipMalloc'd buffer of the current
+ *    device).  Null bitmaps use the OmniVec layout: LSB-first, bit = 1 means
+ *    NULL (core/src/vector/nulls_buffer.h:50-84, core/src/util/bit_util.h:62-71);
+ *    a NULL `nulls` pointer means "no NULLs".
+ *  - Type ids are the reference's DataTypeId values (core/src/type/data_type.h:40-65).
+ *  - Aggregate function ids are the reference's FunctionType values
+ *    (core/src/operator/util/function_type.h:11-27).
+ *  - A handle is used by one host thread at a time (the reference's operators
+ *    are single-threaded by contract); distinct handles may be used
+ *    concurrently.  Each handle owns one HIP stream.
+ */
+#ifndef OMNI_GPU_H
+#define OMNI_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------------------- */
+#define OMNI_GPU_OK 0
+#define OMNI_GPU_ERR_INVALID (-1)      /* bad argument or unsupported type/function combination */
+#define OMNI_GPU_ERR_HIP (-2)          /* a HIP runtime call failed */
+#define OMNI_GPU_ERR_NOMEM (-3)        /* device allocation failed */
+#define OMNI_GPU_ERR_OVERFLOW (-4)     /* decimal overflow with overflowAsNull == false */
+#define OMNI_GPU_ERR_OUTPUT_FULL (-5)  /* caller's output capacity too small; *needed reports the size */
+#define OMNI_GPU_ERR_STATE (-6)        /* call out of order (e.g. probe before build finished) */
+
+/* ---- type / function ids (values of the reference enums) -------------------------------------- */
+#define OMNI_TYPE_INT 1
+#define OMNI_TYPE_LONG 2
+#define OMNI_TYPE_DOUBLE 3
+#define OMNI_TYPE_BOOLEAN 4
+#define OMNI_TYPE_SHORT 5
+#define OMNI_TYPE_DECIMAL64 6
+#define OMNI_TYPE_DECIMAL128 7
+#define OMNI_TYPE_DATE32 8
+#define OMNI_TYPE_TIMESTAMP 12
+
+#define OMNI_AGG_SUM 0
+#define OMNI_AGG_COUNT_COLUMN 1
+#define OMNI_AGG_COUNT_ALL 2
+#define OMNI_AGG_AVG 3
+#define OMNI_AGG_MAX 5
+#define OMNI_AGG_MIN 6
+
+/* Aggregation strategies (omni_gpu_agg_spec.strategy). */
+#define OMNI_GPU_STRATEGY_AUTO 0
+#define OMNI_GPU_STRATEGY_LDS 1      /* workgroup-local LDS table, spill to the global table */
+#define OMNI_GPU_STRATEGY_GLOBAL 2   /* global open-addressing table, 64-bit CAS insert */
+#define OMNI_GPU_STRATEGY_RADIX 3    /* radix-partition rows by key hash, then per-partition tables */
+
+#define OMNI_GPU_MAX_KEYS 4
+#define OMNI_GPU_MAX_AGGS 16
+
+/* One input column: mirrors Vector<T> values + NullsBuffer of a flat OmniVec vector
+ * (core/src/vector/vector.h:202-426).  Element i is values[i]. */
+typedef struct omni_gpu_column {
+    int32_t type_id;        /* DataTypeId */
+    int32_t reserved;
+    const void *values;     /* device pointer */
+    const uint8_t *nulls;   /* device pointer to LSB-first null bitmap (bit=1 NULL) or NULL */
+} omni_gpu_column;
+
+/* One output column: the library writes values[0..n) and the null bitmap bits [0..n). */
+typedef struct omni_gpu_out_column {
+    int32_t type_id;
+    int32_t reserved;
+    void *values;           /* device pointer, n * sizeof(type) bytes */
+    uint8_t *nulls;         /* device pointer, (n + 7) / 8 bytes; written in full */
+} omni_gpu_out_column;
+
+/* ---- runtime ----------------------------------------------------------------------------------- */
+int omni_gpu_init(int device);
+const char *omni_gpu_last_error(void);
+int omni_gpu_malloc(void **ptr, size_t bytes);
+int omni_gpu_free(void *ptr);
+int omni_gpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
+int omni_gpu_memcpy_d2h(void *dst, const void *src, size_t bytes);
+int omni_gpu_memcpy_d2d(void *dst, const void *src, size_t bytes);
+int omni_gpu_memset(void *dst, int value, size_t bytes);
+int omni_gpu_device_synchronize(void);
+
+/* Kernel timing: when enabled, every kernel the library launches is bracketed by HIP events on
+ * the stream it runs on; omni_gpu_profile_get reports the launch count and summed duration (ms)
+ * of the kernels whose name starts with `name_prefix`. */
+int omni_gpu_profile_enable(int on);
+int omni_gpu_profile_reset(void);
+int omni_gpu_profile_get(const char *name_prefix, int64_t *launches, double *total_ms);
+
+/* ---- hash aggregation ---------------------------------------------------------------------------
+ * Replaces HashAggregationOperator's Emplace / ProcessGroup / Output path
+ * (core/src/operator/aggregation/group_aggregation.cpp:320-406, 811-890, 892-955, 1486-1538).
+ * The spec mirrors HashAggregationOperatorFactory's constructor arguments
+ * (core/src/operator/aggregation/group_aggregation.h:233-245) for the single-stage
+ * (inputRaw = true, outputPartial = false) form. */
+typedef struct omni_gpu_agg_spec {
+    int32_t num_keys;
+    int32_t key_types[OMNI_GPU_MAX_KEYS];
+    int32_t num_aggs;
+    int32_t agg_funcs[OMNI_GPU_MAX_AGGS];      /* OMNI_AGG_* */
+    int32_t agg_in_types[OMNI_GPU_MAX_AGGS];   /* ignored for COUNT_ALL */
+    int32_t agg_out_types[OMNI_GPU_MAX_AGGS];
+    int32_t agg_in_scales[OMNI_GPU_MAX_AGGS];  /* decimal scales; 0 otherwise */
+    int32_t agg_out_scales[OMNI_GPU_MAX_AGGS];
+    int32_t overflow_as_null;                  /* OverflowConfig: NULL instead of error on overflow */
+    int32_t strategy;                          /* OMNI_GPU_STRATEGY_* */
+    int64_t capacity_hint;                     /* expected number of groups; 0 = unknown */
+} omni_gpu_agg_spec;
+
+int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle);
+/* Adds one VectorBatch: keys[num_keys] and aggs[num_aggs] (the entry of a COUNT_ALL aggregate
+ * is ignored and may be zeroed).  Asynchronous on the handle's stream; the columns must stay
+ * valid until the next synchronizing call on the handle. */
+int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
+                           int64_t rows);
+/* Number of groups so far, NULL-key group included (HashAggregationOperator::GetHashMapUniqueKeys,
+ * group_aggregation.cpp:1168-1180).  Synchronizes the handle's stream. */
+int omni_gpu_agg_num_groups(void *handle, int64_t *groups);
+/* Writes groups [start, start+count) as output columns: keys first, then one column per
+ * aggregate, in spec order (the output layout of Output<>, group_aggregation.cpp:1486-1538).
+ * Synchronizes.  Returns OMNI_GPU_ERR_OVERFLOW if a decimal aggregate overflowed and
+ * overflow_as_null == 0. */
+int omni_gpu_agg_get_output(void *handle, int64_t start, int64_t count, omni_gpu_out_column *outs);
+int omni_gpu_agg_destroy(void *handle);
+
+/* ---- hash join (inner) --------------------------------------------------------------------------
+ * Replaces HashBuilderOperator + JoinHashTableVariants (core/src/operator/join/hash_builder.cpp:169-213,
+ * join_hash_table_variants.cpp:254-283, 695-808) and LookupJoinOperator::ProbeBatchForInnerJoin
+ * (core/src/operator/join/lookup_join.cpp:924-1035) with its output gather (:1657-1779). */
+typedef struct omni_gpu_join_spec {
+    int32_t key_type;                 /* single equality key: INT, DATE32, SHORT, LONG, TIMESTAMP, DECIMAL64 */
+    int32_t num_payloads;             /* build-side output columns */
+    int32_t payload_types[OMNI_GPU_MAX_AGGS];
+    int32_t strategy;                 /* 0 = reference rule (array table if dense), 1 = hash only */
+} omni_gpu_join_spec;
+
+int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle);
+/* Appends one build VectorBatch, copied into device-resident build storage on the handle's stream
+ * (asynchronous: the caller's buffers must stay valid until omni_gpu_join_build_finish returns).
+ * NULL keys are kept but never match. */
+int omni_gpu_join_build_add(void *handle, const omni_gpu_column *key, const omni_gpu_column *payloads,
+                            int64_t rows);
+/* Builds the table (HashBuilderOperator::GetOutput).  Synchronizes. */
+int omni_gpu_join_build_finish(void *handle);
+/* Table facts after finish: *is_array (1 = direct-index array table), *max_dup (largest number of
+ * build rows sharing one key), *table_slots. */
+int omni_gpu_join_build_info(void *handle, int32_t *is_array, int64_t *max_dup, int64_t *table_slots);
+
+/* Probes one batch.  For each match writes out_probe_rows[k] = probe_row_base + i and, for each
+ * payload column p, the build row's payload into out_payloads[p].values[k] (and its NULL bit).
+ * Output order is unspecified (compare as a row set).  *num_matches receives the match count;
+ * if it exceeds out_capacity, nothing beyond out_capacity is written and OMNI_GPU_ERR_OUTPUT_FULL
+ * is returned.  A payload output's `nulls` may be NULL when that build column has no NULLs; otherwise
+ * it must hold 4 * ceil(out_capacity / 32) bytes (it is cleared and written as 32-bit words).
+ * Synchronizes. */
+int omni_gpu_join_probe(void *handle, const omni_gpu_column *probe_key, int64_t rows,
+                        int64_t probe_row_base, int64_t *out_probe_rows, omni_gpu_out_column *out_payloads,
+                        int64_t out_capacity, int64_t *num_matches);
+
+/* Probes many batches in ONE launch.  keys[b] holds batch b's key column with rows[b] rows;
+ * probe row ids are numbered consecutively across batches starting at 0 in batch order. */
+int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const int64_t *rows,
+                                int32_t num_batches, int64_t *out_probe_rows,
+                                omni_gpu_out_column *out_payloads, int64_t out_capacity,
+                                int64_t *num_matches);
+int omni_gpu_join_destroy(void *handle);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMNI_GPU_H */

@@ -1,0 +1,356 @@
+"""ctypes binding of the C ABI in include/omni_gpu.h (libomni_gpu.so).
+
+This is a thin Python face of the native library for tests and benchmarks; every computation runs in
+the HIP kernels behind the C ABI.  There is no CPU fallback: if the library is missing or no GPU is
+present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libomni_gpu.so")
+
+# DataTypeId / FunctionType values shared with the reference (core/src/type/data_type.h:40-65,
+# core/src/operator/util/function_type.h:11-27).
+INT, LONG, DOUBLE, BOOLEAN, SHORT, DECIMAL64, DECIMAL128, DATE32, TIMESTAMP = 1, 2, 3, 4, 5, 6, 7, 8, 12
+SUM, COUNT_COLUMN, COUNT_ALL, AVG, MAX, MIN = 0, 1, 2, 3, 5, 6
+STRATEGY_AUTO, STRATEGY_LDS, STRATEGY_GLOBAL, STRATEGY_RADIX = 0, 1, 2, 3
+MAX_KEYS, MAX_AGGS = 4, 16
+
+ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "OVERFLOW", -5: "OUTPUT_FULL", -6: "STATE"}
+
+WIDTH = {SHORT: 2, INT: 4, DATE32: 4, LONG: 8, TIMESTAMP: 8, DECIMAL64: 8, DOUBLE: 8, DECIMAL128: 16, BOOLEAN: 1}
+NP = {INT: np.int32, DATE32: np.int32, LONG: np.int64, TIMESTAMP: np.int64, DECIMAL64: np.int64,
+      DOUBLE: np.float64, SHORT: np.int16, BOOLEAN: np.uint8, DECIMAL128: np.uint64}
+
+
+class GpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"omni_gpu error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Column(C.Structure):
+    _fields_ = [("type_id", C.c_int32), ("reserved", C.c_int32), ("values", C.c_void_p), ("nulls", C.c_void_p)]
+
+
+class OutColumn(C.Structure):
+    _fields_ = [("type_id", C.c_int32), ("reserved", C.c_int32), ("values", C.c_void_p), ("nulls", C.c_void_p)]
+
+
+class AggSpec(C.Structure):
+    _fields_ = [
+        ("num_keys", C.c_int32), ("key_types", C.c_int32 * MAX_KEYS),
+        ("num_aggs", C.c_int32), ("agg_funcs", C.c_int32 * MAX_AGGS),
+        ("agg_in_types", C.c_int32 * MAX_AGGS), ("agg_out_types", C.c_int32 * MAX_AGGS),
+        ("agg_in_scales", C.c_int32 * MAX_AGGS), ("agg_out_scales", C.c_int32 * MAX_AGGS),
+        ("overflow_as_null", C.c_int32), ("strategy", C.c_int32), ("capacity_hint", C.c_int64),
+    ]
+
+
+class JoinSpec(C.Structure):
+    _fields_ = [("key_type", C.c_int32), ("num_payloads", C.c_int32), ("payload_types", C.c_int32 * MAX_AGGS),
+                ("strategy", C.c_int32)]
+
+
+# Every exported symbol of include/omni_gpu.h with its ctypes signature.
+_I, _I32, _I64, _VP, _SZ, _D = C.c_int, C.c_int32, C.c_int64, C.c_void_p, C.c_size_t, C.c_double
+_P = C.POINTER
+SIGNATURES = {
+    "omni_gpu_init": (_I, [_I]),
+    "omni_gpu_last_error": (C.c_char_p, []),
+    "omni_gpu_malloc": (_I, [_P(_VP), _SZ]),
+    "omni_gpu_free": (_I, [_VP]),
+    "omni_gpu_memcpy_h2d": (_I, [_VP, _VP, _SZ]),
+    "omni_gpu_memcpy_d2h": (_I, [_VP, _VP, _SZ]),
+    "omni_gpu_memcpy_d2d": (_I, [_VP, _VP, _SZ]),
+    "omni_gpu_memset": (_I, [_VP, _I, _SZ]),
+    "omni_gpu_device_synchronize": (_I, []),
+    "omni_gpu_profile_enable": (_I, [_I]),
+    "omni_gpu_profile_reset": (_I, []),
+    "omni_gpu_profile_get": (_I, [C.c_char_p, _P(_I64), _P(_D)]),
+    "omni_gpu_agg_create": (_I, [_P(AggSpec), _P(_VP)]),
+    "omni_gpu_agg_add_batch": (_I, [_VP, _P(Column), _P(Column), _I64]),
+    "omni_gpu_agg_num_groups": (_I, [_VP, _P(_I64)]),
+    "omni_gpu_agg_get_output": (_I, [_VP, _I64, _I64, _P(OutColumn)]),
+    "omni_gpu_agg_destroy": (_I, [_VP]),
+    "omni_gpu_join_build_create": (_I, [_P(JoinSpec), _P(_VP)]),
+    "omni_gpu_join_build_add": (_I, [_VP, _P(Column), _P(Column), _I64]),
+    "omni_gpu_join_build_finish": (_I, [_VP]),
+    "omni_gpu_join_build_info": (_I, [_VP, _P(_I32), _P(_I64), _P(_I64)]),
+    "omni_gpu_join_probe": (_I, [_VP, _P(Column), _I64, _I64, _VP, _P(OutColumn), _I64, _P(_I64)]),
+    "omni_gpu_join_probe_batches": (_I, [_VP, _P(Column), _P(_I64), _I32, _VP, _P(OutColumn), _I64, _P(_I64)]),
+    "omni_gpu_join_destroy": (_I, [_VP]),
+}
+
+_lib = None
+_inited = False
+
+
+def lib():
+    """Loads libomni_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m omnioperator_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != 0:
+        raise GpuError(rc, lib().omni_gpu_last_error().decode(errors="replace"))
+
+
+def init(device: int = 0):
+    global _inited
+    if not _inited:
+        check(lib().omni_gpu_init(device))
+        _inited = True
+
+
+class DeviceArray:
+    """Owning device buffer holding a numpy-shaped array."""
+
+    def __init__(self, nbytes: int, dtype=np.uint8, shape=None):
+        init()
+        self.nbytes = int(nbytes)
+        self.dtype = np.dtype(dtype)
+        self.shape = shape if shape is not None else (self.nbytes // max(1, self.dtype.itemsize),)
+        p = C.c_void_p()
+        check(lib().omni_gpu_malloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes, a.dtype, a.shape)
+        if a.nbytes:
+            check(lib().omni_gpu_memcpy_h2d(d.ptr, a.ctypes.data, a.nbytes))
+        return d
+
+    @classmethod
+    def zeros(cls, shape, dtype) -> "DeviceArray":
+        shape = (shape,) if isinstance(shape, int) else tuple(shape)
+        nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        d = cls(nbytes, dtype, shape)
+        check(lib().omni_gpu_memset(d.ptr, 0, max(1, nbytes)))
+        return d
+
+    def numpy(self, count: Optional[int] = None) -> np.ndarray:
+        shape = self.shape if count is None else (count,) + tuple(self.shape[1:])
+        out = np.empty(shape, self.dtype)
+        if out.nbytes:
+            check(lib().omni_gpu_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().omni_gpu_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def pack_nulls(mask: Optional[np.ndarray]) -> Optional[np.ndarray]:
+    if mask is None:
+        return None
+    return np.packbits(np.asarray(mask, dtype=bool), bitorder="little")
+
+
+def unpack_nulls(bitmap: np.ndarray, n: int) -> np.ndarray:
+    return np.unpackbits(np.asarray(bitmap, dtype=np.uint8), count=n, bitorder="little").astype(bool)
+
+
+class DeviceColumn:
+    """A column resident in device memory (values + optional LSB-first null bitmap)."""
+
+    def __init__(self, type_id: int, values: np.ndarray, nulls: Optional[np.ndarray] = None):
+        self.type_id = type_id
+        self.rows = int(values.shape[0])
+        self.values = DeviceArray.from_numpy(values)
+        bm = pack_nulls(nulls)
+        self.nulls = DeviceArray.from_numpy(bm) if bm is not None and nulls.any() else None
+
+    def c(self, offset_rows: int = 0) -> Column:
+        col = Column()
+        col.type_id = self.type_id
+        col.values = self.values.ptr + offset_rows * WIDTH[self.type_id]
+        col.nulls = self.nulls.ptr if self.nulls is not None else None
+        if offset_rows and self.nulls is not None:
+            raise ValueError("sliced columns with NULLs need an own bitmap")
+        return col
+
+
+def make_agg_spec(key_types: Sequence[int], aggs: Sequence[tuple], overflow_as_null: bool = True,
+                  strategy: int = STRATEGY_AUTO, capacity_hint: int = 0) -> AggSpec:
+    """aggs: tuples (func, in_type, out_type[, in_scale, out_scale])."""
+    s = AggSpec()
+    s.num_keys = len(key_types)
+    for i, t in enumerate(key_types):
+        s.key_types[i] = t
+    s.num_aggs = len(aggs)
+    for i, a in enumerate(aggs):
+        s.agg_funcs[i], s.agg_in_types[i], s.agg_out_types[i] = a[0], a[1], a[2]
+        if len(a) > 3:
+            s.agg_in_scales[i], s.agg_out_scales[i] = a[3], a[4]
+    s.overflow_as_null = 1 if overflow_as_null else 0
+    s.strategy = strategy
+    s.capacity_hint = capacity_hint
+    return s
+
+
+class HashAgg:
+    """GPU group-by aggregation handle (omni_gpu_agg_*)."""
+
+    def __init__(self, spec: AggSpec):
+        init()
+        self.spec = spec
+        h = C.c_void_p()
+        check(lib().omni_gpu_agg_create(C.byref(spec), C.byref(h)))
+        self.h = h.value
+
+    def add_batch(self, keys: Sequence[Column], aggs: Sequence[Optional[Column]], rows: int):
+        karr = (Column * MAX_KEYS)(*keys)
+        aarr = (Column * MAX_AGGS)()
+        for i, a in enumerate(aggs):
+            if a is not None:
+                aarr[i] = a
+        check(lib().omni_gpu_agg_add_batch(self.h, karr, aarr, rows))
+
+    def num_groups(self) -> int:
+        g = C.c_int64()
+        check(lib().omni_gpu_agg_num_groups(self.h, C.byref(g)))
+        return g.value
+
+    def output(self, start: int = 0, count: Optional[int] = None):
+        """Returns [(values ndarray, null mask)] for keys then aggregates."""
+        if count is None:
+            count = self.num_groups() - start
+        types = [self.spec.key_types[i] for i in range(self.spec.num_keys)] + \
+                [self.spec.agg_out_types[i] for i in range(self.spec.num_aggs)]
+        bufs = []
+        oarr = (OutColumn * len(types))()
+        for i, t in enumerate(types):
+            shape = (count, 2) if t == DECIMAL128 else (count,)
+            v = DeviceArray.zeros(shape, NP[t])
+            nb = DeviceArray.zeros(max(1, (count + 7) // 8), np.uint8)
+            bufs.append((v, nb))
+            oarr[i].type_id, oarr[i].values, oarr[i].nulls = t, v.ptr, nb.ptr
+        check(lib().omni_gpu_agg_get_output(self.h, start, count, oarr))
+        return [(v.numpy(), unpack_nulls(nb.numpy(), count)) for v, nb in bufs]
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().omni_gpu_agg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HashJoin:
+    """GPU inner hash join handle (omni_gpu_join_*)."""
+
+    def __init__(self, key_type: int, payload_types: Sequence[int] = (), strategy: int = 0):
+        init()
+        s = JoinSpec()
+        s.key_type = key_type
+        s.num_payloads = len(payload_types)
+        for i, t in enumerate(payload_types):
+            s.payload_types[i] = t
+        s.strategy = strategy
+        self.spec = s
+        self.payload_types = list(payload_types)
+        h = C.c_void_p()
+        check(lib().omni_gpu_join_build_create(C.byref(s), C.byref(h)))
+        self.h = h.value
+
+    def add_build(self, key: Column, payloads: Sequence[Column], rows: int):
+        parr = (Column * max(1, len(payloads)))(*payloads)
+        check(lib().omni_gpu_join_build_add(self.h, C.byref(key), parr, rows))
+
+    def finish(self):
+        check(lib().omni_gpu_join_build_finish(self.h))
+
+    def info(self):
+        a, d, s = C.c_int32(), C.c_int64(), C.c_int64()
+        check(lib().omni_gpu_join_build_info(self.h, C.byref(a), C.byref(d), C.byref(s)))
+        return {"is_array": bool(a.value), "max_dup": d.value, "table_slots": s.value}
+
+    def _outs(self, cap: int, with_nulls: bool):
+        rows = DeviceArray(max(1, cap) * 8, np.int64, (cap,))
+        pays, oarr = [], (OutColumn * max(1, len(self.payload_types)))()
+        for i, t in enumerate(self.payload_types):
+            shape = (cap, 2) if t == DECIMAL128 else (cap,)
+            v = DeviceArray(max(1, cap) * WIDTH[t], NP[t], shape)
+            nb = DeviceArray(max(4, ((cap + 31) // 32) * 4), np.uint8) if with_nulls else None
+            pays.append((v, nb))
+            oarr[i].type_id, oarr[i].values = t, v.ptr
+            oarr[i].nulls = nb.ptr if nb is not None else None
+        return rows, pays, oarr
+
+    def probe(self, key: Column, rows: int, row_base: int = 0, cap: Optional[int] = None, with_nulls: bool = True):
+        """Probes one batch; returns (probe_rows, [(payload values, null mask)])."""
+        if cap is None:
+            cap = rows * max(1, self.info()["max_dup"])
+        out_rows, pays, oarr = self._outs(cap, with_nulls)
+        m = C.c_int64()
+        check(lib().omni_gpu_join_probe(self.h, C.byref(key), rows, row_base, out_rows.ptr, oarr, cap, C.byref(m)))
+        n = m.value
+        res = [(v.numpy(n), unpack_nulls(nb.numpy(), n) if nb is not None else np.zeros(n, bool)) for v, nb in pays]
+        return out_rows.numpy(n), res
+
+    def probe_batches_raw(self, keys: Sequence[Column], rows: Sequence[int], out_rows: DeviceArray,
+                          oarr, cap: int) -> int:
+        """Device-resident batched probe (one launch); returns the match count."""
+        karr = (Column * len(keys))(*keys)
+        rarr = (C.c_int64 * len(rows))(*rows)
+        m = C.c_int64()
+        check(lib().omni_gpu_join_probe_batches(self.h, karr, rarr, len(keys), out_rows.ptr, oarr, cap, C.byref(m)))
+        return m.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().omni_gpu_join_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def profile_enable(on: bool = True):
+    check(lib().omni_gpu_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(lib().omni_gpu_profile_reset())
+
+
+def profile_get(prefix: str):
+    n, ms = C.c_int64(), C.c_double()
+    check(lib().omni_gpu_profile_get(prefix.encode(), C.byref(n), C.byref(ms)))
+    return n.value, ms.value
+
+
+def synchronize():
+    check(lib().omni_gpu_device_synchronize())

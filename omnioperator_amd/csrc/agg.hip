@@ -1,0 +1,943 @@
+// Group-by hash aggregation on gfx950.
+//
+// Replaces the per-row Emplace loop + per-aggregator ProcessGroup scatter of HashAggregationOperator
+// (core/src/operator/aggregation/group_aggregation.cpp:811-890, aggregator/operations_hash_aggregator.h:16-32)
+// and its output traversal (group_aggregation.cpp:892-955, 1486-1538).
+//
+// Device layout (all in HBM, sized by the host):
+//   single-word keys (one key column of width <= 8):
+//     slots[cap]      {u64 key; u32 gid+1; u32 pad}   open addressing, linear probing, 64-bit CAS on key
+//     side cells      special[0] = NULL-key group, special[1] = group of the key equal to the slot sentinel
+//   multi-word keys (several key columns or a DECIMAL128 key), W words per key:
+//     ctrl[cap]       u32: 0 empty, 1 being written, gid+2 published
+//     skeys[cap * W]  key words (word 0 carries the NULL bits of every key column and a <=4-byte column)
+//   per group, indexed by a dense group id handed out in insertion order:
+//     gkeys[G * W], gnull[G] (single-word only), and one structure-of-arrays state block per aggregate
+//
+// Results are independent of the hash function and of insertion order for every integer aggregate
+// (wrapping SUM, COUNT, MIN, MAX and exact 192-bit decimal accumulation); double SUM/AVG depend on the
+// order of atomic additions and are held to a relative tolerance.
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "omni_gpu_internal.h"
+
+namespace omni_gpu {
+
+enum AggKind : int {
+    K_SUM_I64 = 0,
+    K_SUM_F64,
+    K_SUM_DEC,
+    K_COUNT_ALL,
+    K_COUNT_COL,
+    K_AVG,
+    K_MIN_I64,
+    K_MAX_I64,
+    K_MIN_I32,
+    K_MAX_I32,
+    K_MIN_F64,
+    K_MAX_F64,
+};
+
+struct AggDev {
+    int kind;
+    int in_width;     // bytes per input value
+    int in_double;    // input column holds doubles
+    int out_width;    // bytes per output value
+    const void *values;
+    const uint8_t *nulls;
+    void *s0;
+    void *s1;
+    void *s2;
+    uint8_t *flag;    // 1 once a non-NULL value was added
+    int dec_scale_up; // 10^(outScale - inScale) exponent for decimal SUM
+    int dec_out128;   // decimal SUM output is DECIMAL128
+};
+
+struct AggArgs {
+    int n;
+    AggDev a[OMNI_GPU_MAX_AGGS];
+};
+
+struct KeyDev {
+    int num;
+    int W;                      // words per key (multi-word path)
+    int width[OMNI_GPU_MAX_KEYS];
+    int word[OMNI_GPU_MAX_KEYS];   // first word index of each column
+    int in_word0[OMNI_GPU_MAX_KEYS];  // column stored in the high half of word 0
+    const void *values[OMNI_GPU_MAX_KEYS];
+    const uint8_t *nulls[OMNI_GPU_MAX_KEYS];
+};
+
+struct Slot1 {
+    unsigned long long key;
+    unsigned int gid1;
+    unsigned int pad;
+};
+
+struct GroupCtx {
+    unsigned long long *ngroups;   // group counter
+    unsigned long long max_groups; // capacity of the per-group arrays
+    unsigned int *overflow;        // set when a group id would exceed max_groups (host sizing bug guard)
+    unsigned long long *gkeys;
+    uint8_t *gnull;
+    unsigned int *special;         // [0] NULL group, [1] sentinel-key group (0 empty, 1 busy, gid+2)
+};
+
+constexpr int kMaxWords = 6;
+
+__device__ __forceinline__ unsigned int LoadAcquire(unsigned int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long LoadRelaxed64(unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void StoreRelease(unsigned int *p, unsigned int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Claims a fresh group id; returns ~0 if the per-group arrays are full (never expected: the host sizes them).
+__device__ __forceinline__ unsigned int NewGroup(const GroupCtx &g)
+{
+    unsigned long long id = WaveReserveOne(g.ngroups, true);
+    if (id >= g.max_groups) {
+        atomicOr(g.overflow, 1u);
+        return 0xFFFFFFFFu;
+    }
+    return static_cast<unsigned int>(id);
+}
+
+// Group of a side cell (NULL key, or the key equal to the slot sentinel), created on first use.
+__device__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned long long key, bool isNull, int W)
+{
+    unsigned int *cell = g.special + which;
+    while (true) {
+        unsigned int c = LoadAcquire(cell);
+        if (c >= 2) return c - 2;
+        if (c == 0 && atomicCAS(cell, 0u, 1u) == 0u) {
+            unsigned int gid = NewGroup(g);
+            if (gid == 0xFFFFFFFFu) {
+                StoreRelease(cell, 0u);
+                return gid;
+            }
+            g.gkeys[static_cast<size_t>(gid) * W] = key;
+            if (g.gnull) g.gnull[gid] = isNull ? 1 : 0;
+            StoreRelease(cell, gid + 2);
+            return gid;
+        }
+    }
+}
+
+// Find-or-insert of a single-word key; returns its group id.
+__device__ unsigned int InsertSingle(Slot1 *slots, unsigned long long mask, unsigned long long key,
+                                     const GroupCtx &g)
+{
+    unsigned long long pos = Mix64(key) & mask;
+    while (true) {
+        unsigned long long k = slots[pos].key;
+        if (k == kEmptyKey) {
+            k = atomicCAS(&slots[pos].key, kEmptyKey, key);
+            if (k == kEmptyKey) {
+                unsigned int gid = NewGroup(g);
+                if (gid == 0xFFFFFFFFu) return gid;
+                g.gkeys[gid] = key;
+                if (g.gnull) g.gnull[gid] = 0;
+                StoreRelease(&slots[pos].gid1, gid + 1);
+                return gid;
+            }
+        }
+        if (k == key) {
+            unsigned int g1 = LoadAcquire(&slots[pos].gid1);
+            if (g1) return g1 - 1;
+            continue;  // the winning lane publishes its group id; re-read the same slot
+        }
+        pos = (pos + 1) & mask;
+    }
+}
+
+// Find-or-insert of a W-word key (ctrl-word protocol).
+__device__ unsigned int InsertMulti(unsigned int *ctrl, unsigned long long *skeys, unsigned long long mask,
+                                    const unsigned long long *words, int W, unsigned long long hash,
+                                    const GroupCtx &g)
+{
+    unsigned long long pos = hash & mask;
+    while (true) {
+        unsigned int c = LoadAcquire(&ctrl[pos]);
+        if (c == 0) {
+            c = atomicCAS(&ctrl[pos], 0u, 1u);
+            if (c == 0) {
+                unsigned int gid = NewGroup(g);
+                if (gid == 0xFFFFFFFFu) {
+                    StoreRelease(&ctrl[pos], 0u);
+                    return gid;
+                }
+                for (int w = 0; w < W; ++w) {
+                    skeys[pos * W + w] = words[w];
+                    g.gkeys[static_cast<size_t>(gid) * W + w] = words[w];
+                }
+                StoreRelease(&ctrl[pos], gid + 2);
+                return gid;
+            }
+        }
+        if (c == 1) continue;  // another lane is writing this slot's key
+        bool same = true;
+        for (int w = 0; w < W; ++w) same &= LoadRelaxed64(&skeys[pos * W + w]) == words[w];
+        if (same) return c - 2;
+        pos = (pos + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ void EncodeKey(const KeyDev &kd, int64_t row, unsigned long long *words)
+{
+    for (int w = 0; w < kd.W; ++w) words[w] = 0;
+    for (int c = 0; c < kd.num; ++c) {
+        if (NullBit(kd.nulls[c], row)) {
+            words[0] |= 1ULL << c;
+            continue;
+        }
+        if (kd.width[c] == 16) {
+            const unsigned long long *v = static_cast<const unsigned long long *>(kd.values[c]);
+            words[kd.word[c]] = v[2 * row];
+            words[kd.word[c] + 1] = v[2 * row + 1];
+        } else if (kd.in_word0[c]) {
+            words[0] |= (LoadWord(kd.values[c], kd.width[c], row) & 0xFFFFFFFFULL) << 32;
+        } else {
+            words[kd.word[c]] = LoadWord(kd.values[c], kd.width[c], row);
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long HashWords(const unsigned long long *words, int W)
+{
+    unsigned long long h = 0x9E3779B97F4A7C15ULL;
+    for (int w = 0; w < W; ++w) h = Mix64(h ^ (words[w] + 0x632BE59BD9B4E019ULL * (w + 1)));
+    return h;
+}
+
+// 192-bit exact accumulation of a signed 128-bit addend (lo, hi) into (s0 = lo, s1 = hi, s2 = top).
+__device__ __forceinline__ void AddDec(unsigned long long *lo, unsigned long long *hi, long long *top,
+                                       unsigned long long vlo, long long vhi)
+{
+    unsigned long long old = atomicAdd(lo, vlo);
+    unsigned long long carry = (old + vlo) < old ? 1ULL : 0ULL;
+    // add (vhi + carry) to hi, tracking the signed carry-out into top
+    long long parts[2] = {vhi, static_cast<long long>(carry)};
+    if (vhi == 0x7FFFFFFFFFFFFFFFLL && carry) {
+        // vhi + 1 does not fit in 64 bits: add the two parts separately
+    } else {
+        parts[0] = vhi + static_cast<long long>(carry);
+        parts[1] = 0;
+    }
+    for (int p = 0; p < 2; ++p) {
+        long long a = parts[p];
+        if (a == 0) continue;
+        unsigned long long ua = static_cast<unsigned long long>(a);
+        unsigned long long oh = atomicAdd(hi, ua);
+        long long k = ((oh + ua) < oh ? 1 : 0) - (a < 0 ? 1 : 0);
+        if (k) atomicAdd(reinterpret_cast<unsigned long long *>(top), static_cast<unsigned long long>(k));
+    }
+}
+
+__device__ __forceinline__ void UpdateAggs(const AggArgs &args, unsigned int gid, int64_t row)
+{
+    for (int i = 0; i < args.n; ++i) {
+        const AggDev &d = args.a[i];
+        if (d.kind == K_COUNT_ALL) {
+            atomicAdd(static_cast<unsigned long long *>(d.s0) + gid, 1ULL);
+            continue;
+        }
+        if (NullBit(d.nulls, row)) continue;
+        switch (d.kind) {
+            case K_COUNT_COL: atomicAdd(static_cast<unsigned long long *>(d.s0) + gid, 1ULL); break;
+            case K_SUM_I64:
+                atomicAdd(static_cast<unsigned long long *>(d.s0) + gid, LoadWord(d.values, d.in_width, row));
+                d.flag[gid] = 1;
+                break;
+            case K_SUM_F64:
+                atomicAdd(static_cast<double *>(d.s0) + gid, static_cast<const double *>(d.values)[row]);
+                d.flag[gid] = 1;
+                break;
+            case K_AVG: {
+                double v = d.in_double ? static_cast<const double *>(d.values)[row]
+                                       : static_cast<double>(static_cast<long long>(LoadWord(d.values, d.in_width, row)));
+                atomicAdd(static_cast<double *>(d.s0) + gid, v);
+                atomicAdd(static_cast<unsigned long long *>(d.s1) + gid, 1ULL);
+                break;
+            }
+            case K_SUM_DEC: {
+                unsigned long long vlo;
+                long long vhi;
+                if (d.in_width == 16) {
+                    const unsigned long long *v = static_cast<const unsigned long long *>(d.values);
+                    vlo = v[2 * row];
+                    vhi = static_cast<long long>(v[2 * row + 1]);
+                } else {
+                    long long v = static_cast<long long>(LoadWord(d.values, d.in_width, row));
+                    vlo = static_cast<unsigned long long>(v);
+                    vhi = v < 0 ? -1 : 0;
+                }
+                AddDec(static_cast<unsigned long long *>(d.s0) + gid, static_cast<unsigned long long *>(d.s1) + gid,
+                       static_cast<long long *>(d.s2) + gid, vlo, vhi);
+                d.flag[gid] = 1;
+                break;
+            }
+            case K_MIN_I64:
+                atomicMin(static_cast<long long *>(d.s0) + gid, static_cast<long long>(LoadWord(d.values, d.in_width, row)));
+                d.flag[gid] = 1;
+                break;
+            case K_MAX_I64:
+                atomicMax(static_cast<long long *>(d.s0) + gid, static_cast<long long>(LoadWord(d.values, d.in_width, row)));
+                d.flag[gid] = 1;
+                break;
+            case K_MIN_I32:
+                atomicMin(static_cast<int *>(d.s0) + gid, static_cast<int>(LoadWord(d.values, d.in_width, row)));
+                d.flag[gid] = 1;
+                break;
+            case K_MAX_I32:
+                atomicMax(static_cast<int *>(d.s0) + gid, static_cast<int>(LoadWord(d.values, d.in_width, row)));
+                d.flag[gid] = 1;
+                break;
+            case K_MIN_F64:
+                atomicMin(static_cast<double *>(d.s0) + gid, static_cast<const double *>(d.values)[row]);
+                d.flag[gid] = 1;
+                break;
+            case K_MAX_F64:
+                atomicMax(static_cast<double *>(d.s0) + gid, static_cast<const double *>(d.values)[row]);
+                d.flag[gid] = 1;
+                break;
+            default: break;
+        }
+    }
+}
+
+// ---- kernels ------------------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256) agg_insert_single(KeyDev kd, int64_t rows, Slot1 *slots,
+                                                         unsigned long long mask, GroupCtx g, AggArgs args)
+{
+    int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    unsigned int gid;
+    if (NullBit(kd.nulls[0], row)) {
+        gid = SideGroup(g, 0, 0ULL, true, 1);
+    } else {
+        unsigned long long key = LoadWord(kd.values[0], kd.width[0], row);
+        gid = key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g);
+    }
+    if (gid == 0xFFFFFFFFu) return;
+    UpdateAggs(args, gid, row);
+}
+
+__global__ void __launch_bounds__(256) agg_insert_multi(KeyDev kd, int64_t rows, unsigned int *ctrl,
+                                                        unsigned long long *skeys, unsigned long long mask,
+                                                        GroupCtx g, AggArgs args)
+{
+    int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    unsigned long long words[kMaxWords];
+    EncodeKey(kd, row, words);
+    unsigned int gid = InsertMulti(ctrl, skeys, mask, words, kd.W, HashWords(words, kd.W), g);
+    if (gid == 0xFFFFFFFFu) return;
+    UpdateAggs(args, gid, row);
+}
+
+// Rehash after growth: re-inserts groups [0, n) with their existing ids (states do not move).
+__global__ void agg_rehash_single(const unsigned long long *gkeys, const uint8_t *gnull, int64_t n, Slot1 *slots,
+                                  unsigned long long mask, const unsigned int *special)
+{
+    int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    if (special[0] == gid + 2 || special[1] == gid + 2) return;  // side groups are not in the table
+    unsigned long long key = gkeys[gid];
+    unsigned long long pos = Mix64(key) & mask;
+    while (atomicCAS(&slots[pos].key, kEmptyKey, key) != kEmptyKey) pos = (pos + 1) & mask;
+    slots[pos].gid1 = static_cast<unsigned int>(gid) + 1;
+}
+
+__global__ void agg_rehash_multi(const unsigned long long *gkeys, int64_t n, int W, unsigned int *ctrl,
+                                 unsigned long long *skeys, unsigned long long mask)
+{
+    int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= n) return;
+    unsigned long long words[kMaxWords];
+    for (int w = 0; w < W; ++w) words[w] = gkeys[gid * W + w];
+    unsigned long long pos = HashWords(words, W) & mask;
+    while (atomicCAS(&ctrl[pos], 0u, static_cast<unsigned int>(gid) + 2) != 0u) pos = (pos + 1) & mask;
+    for (int w = 0; w < W; ++w) skeys[pos * W + w] = words[w];
+}
+
+__global__ void fill_u64(unsigned long long *p, unsigned long long v, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u32(unsigned int *p, unsigned int v, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void init_slots1(Slot1 *s, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        s[i].key = kEmptyKey;
+        s[i].gid1 = 0;
+        s[i].pad = 0;
+    }
+}
+
+struct OutDev {
+    int width;
+    void *values;
+    uint8_t *nulls;
+};
+
+struct OutArgs {
+    int nkeys;
+    int naggs;
+    OutDev o[OMNI_GPU_MAX_KEYS + OMNI_GPU_MAX_AGGS];
+};
+
+// Writes the null bitmap of 64 consecutive output rows handled by one wave.
+__device__ __forceinline__ void WriteNullBits(uint8_t *nulls, int64_t r, int64_t count, bool isNull)
+{
+    unsigned long long m = __ballot(isNull);
+    int lane = LaneId();
+    int64_t base = r - lane;  // first row of this wave (waves cover aligned groups of 64 rows)
+    if (lane < 8) {
+        int64_t byte = (base >> 3) + lane;
+        if (byte * 8 < count) nulls[byte] = static_cast<uint8_t>(m >> (8 * lane));
+    }
+}
+
+__device__ __forceinline__ void StoreWidth(void *dst, int width, int64_t r, unsigned long long v)
+{
+    switch (width) {
+        case 8: static_cast<unsigned long long *>(dst)[r] = v; break;
+        case 4: static_cast<unsigned int *>(dst)[r] = static_cast<unsigned int>(v); break;
+        case 2: static_cast<unsigned short *>(dst)[r] = static_cast<unsigned short>(v); break;
+        default: static_cast<uint8_t *>(dst)[r] = static_cast<uint8_t>(v); break;
+    }
+}
+
+// Decimal finalisation: range check of the 192-bit total, then x 10^up with overflow checks
+// (SumSparkDecimalAggregator::ExtractValuesBatchInternal, sum_spark_decimal_aggregator.h:246-301).
+__device__ bool FinishDecimal(unsigned long long lo, unsigned long long hi, long long top, int up, bool out128,
+                              __int128 &res)
+{
+    bool fits128 = (top == 0 && static_cast<long long>(hi) >= 0) || (top == -1 && static_cast<long long>(hi) < 0);
+    if (!fits128) return false;
+    __int128 v = static_cast<__int128>((static_cast<unsigned __int128>(hi) << 64) | lo);
+    const __int128 max38 = (static_cast<__int128>(5421010862427522170LL) << 64) + 687399551400673279LL;
+    const __int128 lim = out128 ? max38 : static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL);
+    const __int128 llim = out128 ? -max38 : -static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL) - 1;
+    if (v > lim || v < llim) return false;
+    for (int i = 0; i < up; ++i) {
+        // v * 10 must stay within [-max38, max38]
+        if (v > max38 / 10 || v < -(max38 / 10)) return false;
+        v = (v << 3) + (v << 1);
+    }
+    if (v > lim || v < llim) return false;
+    res = v;
+    return true;
+}
+
+__global__ void __launch_bounds__(256) agg_output(int64_t start, int64_t count, KeyDev kd, int single,
+                                                  const unsigned long long *gkeys, const uint8_t *gnull,
+                                                  AggArgs args, OutArgs out, int overflow_as_null,
+                                                  unsigned int *error_flag)
+{
+    int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool live = r < count;
+    int64_t gid = start + r;
+    // keys
+    for (int c = 0; c < kd.num; ++c) {
+        bool isNull = false;
+        unsigned long long lo = 0, hi = 0;
+        if (live) {
+            if (single) {
+                isNull = gnull[gid] != 0;
+                lo = gkeys[gid];
+            } else {
+                const unsigned long long *w = gkeys + gid * kd.W;
+                isNull = (w[0] >> c) & 1ULL;
+                if (kd.width[c] == 16) {
+                    lo = w[kd.word[c]];
+                    hi = w[kd.word[c] + 1];
+                } else if (kd.in_word0[c]) {
+                    lo = w[0] >> 32;
+                } else {
+                    lo = w[kd.word[c]];
+                }
+            }
+            if (isNull) lo = hi = 0;
+            OutDev &o = out.o[c];
+            if (o.width == 16) {
+                static_cast<unsigned long long *>(o.values)[2 * r] = lo;
+                static_cast<unsigned long long *>(o.values)[2 * r + 1] = hi;
+            } else {
+                StoreWidth(o.values, o.width, r, lo);
+            }
+        }
+        WriteNullBits(out.o[c].nulls, r, count, live && isNull);
+    }
+    // aggregates
+    for (int i = 0; i < args.n; ++i) {
+        const AggDev &d = args.a[i];
+        OutDev &o = out.o[kd.num + i];
+        bool isNull = false;
+        if (live) {
+            switch (d.kind) {
+                case K_COUNT_ALL:
+                case K_COUNT_COL:
+                    static_cast<unsigned long long *>(o.values)[r] = static_cast<unsigned long long *>(d.s0)[gid];
+                    break;
+                case K_AVG: {
+                    unsigned long long cnt = static_cast<unsigned long long *>(d.s1)[gid];
+                    isNull = cnt == 0;
+                    static_cast<double *>(o.values)[r] =
+                        isNull ? 0.0 : static_cast<double *>(d.s0)[gid] / static_cast<double>(cnt);
+                    break;
+                }
+                case K_SUM_DEC: {
+                    isNull = d.flag[gid] == 0;
+                    __int128 v = 0;
+                    if (!isNull) {
+                        bool ok = FinishDecimal(static_cast<unsigned long long *>(d.s0)[gid],
+                                                static_cast<unsigned long long *>(d.s1)[gid],
+                                                static_cast<long long *>(d.s2)[gid], d.dec_scale_up, d.dec_out128, v);
+                        if (!ok) {
+                            isNull = true;
+                            v = 0;
+                            if (!overflow_as_null) atomicOr(error_flag, 1u);
+                        }
+                    }
+                    if (d.dec_out128) {
+                        static_cast<unsigned long long *>(o.values)[2 * r] = static_cast<unsigned long long>(v);
+                        static_cast<unsigned long long *>(o.values)[2 * r + 1] =
+                            static_cast<unsigned long long>(static_cast<unsigned __int128>(v) >> 64);
+                    } else {
+                        static_cast<long long *>(o.values)[r] = static_cast<long long>(v);
+                    }
+                    break;
+                }
+                case K_MIN_I32:
+                case K_MAX_I32: {
+                    isNull = d.flag[gid] == 0;
+                    int v = isNull ? 0 : static_cast<int *>(d.s0)[gid];
+                    StoreWidth(o.values, o.width, r, static_cast<unsigned long long>(static_cast<long long>(v)));
+                    break;
+                }
+                default: {  // SUM_I64, SUM_F64, MIN/MAX 64-bit: copy the 8-byte state
+                    isNull = d.flag[gid] == 0;
+                    static_cast<unsigned long long *>(o.values)[r] =
+                        isNull ? 0ULL : static_cast<unsigned long long *>(d.s0)[gid];
+                    break;
+                }
+            }
+        }
+        WriteNullBits(o.nulls, r, count, live && isNull);
+    }
+}
+
+// ---- host side ----------------------------------------------------------------------------------------------
+
+constexpr int kBlock = 256;
+constexpr double kMaxLoad = 0.5;
+
+inline unsigned int Blocks(int64_t n) { return static_cast<unsigned int>((n + kBlock - 1) / kBlock); }
+
+inline uint64_t NextPow2(uint64_t x)
+{
+    uint64_t p = 1;
+    while (p < x) p <<= 1;
+    return p;
+}
+
+struct AggHandle {
+    omni_gpu_agg_spec spec{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool single = true;
+    KeyDev kd{};
+    AggArgs args{};  // value pointers filled per batch
+    uint64_t cap = 0;
+    int64_t max_groups = 0;
+    DevBuf slots, ctrl, skeys, gkeys, gnull;
+    DevBuf state[OMNI_GPU_MAX_AGGS][4];  // s0, s1, s2, flag
+    DevBuf counters;  // [0] ngroups u64, [8] overflow u32, [12] error u32, [16..24) special u32[2]
+    int64_t known_groups = 0;
+    int64_t pending_rows = 0;
+
+    unsigned long long *ngroups() const { return reinterpret_cast<unsigned long long *>(counters.as<uint8_t>()); }
+    unsigned int *overflow() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 8); }
+    unsigned int *error_flag() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 12); }
+    unsigned int *special() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 16); }
+
+    GroupCtx Ctx() const
+    {
+        GroupCtx g;
+        g.ngroups = ngroups();
+        g.max_groups = static_cast<unsigned long long>(max_groups);
+        g.overflow = overflow();
+        g.gkeys = gkeys.as<unsigned long long>();
+        g.gnull = single ? gnull.as<uint8_t>() : nullptr;
+        g.special = special();
+        return g;
+    }
+};
+
+int Fill64(hipStream_t s, void *p, unsigned long long v, int64_t n)
+{
+    if (n <= 0) return OMNI_GPU_OK;
+    KernelScope ks(s, "fill_u64");
+    fill_u64<<<Blocks(n), kBlock, 0, s>>>(static_cast<unsigned long long *>(p), v, n);
+    return CheckLaunch("fill_u64");
+}
+
+int Fill32(hipStream_t s, void *p, unsigned int v, int64_t n)
+{
+    if (n <= 0) return OMNI_GPU_OK;
+    KernelScope ks(s, "fill_u32");
+    fill_u32<<<Blocks(n), kBlock, 0, s>>>(static_cast<unsigned int *>(p), v, n);
+    return CheckLaunch("fill_u32");
+}
+
+int ClassifyAgg(const omni_gpu_agg_spec &sp, int i, AggDev &d)
+{
+    int f = sp.agg_funcs[i], ti = sp.agg_in_types[i], to = sp.agg_out_types[i];
+    d.in_width = TypeWidth(ti);
+    d.in_double = ti == OMNI_TYPE_DOUBLE;
+    d.out_width = TypeWidth(to);
+    d.dec_scale_up = 0;
+    d.dec_out128 = 0;
+    bool intIn = IsIntegerKeyType(ti) && ti != OMNI_TYPE_DECIMAL64;
+    switch (f) {
+        case OMNI_AGG_COUNT_ALL:
+            OG_CHECK(to == OMNI_TYPE_LONG, OMNI_GPU_ERR_INVALID, "COUNT output must be LONG");
+            d.kind = K_COUNT_ALL;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_COUNT_COLUMN:
+            OG_CHECK(to == OMNI_TYPE_LONG, OMNI_GPU_ERR_INVALID, "COUNT output must be LONG");
+            d.kind = K_COUNT_COL;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_SUM:
+            if (ti == OMNI_TYPE_DECIMAL64 || ti == OMNI_TYPE_DECIMAL128) {
+                OG_CHECK(to == OMNI_TYPE_DECIMAL64 || to == OMNI_TYPE_DECIMAL128, OMNI_GPU_ERR_INVALID,
+                         "decimal SUM output must be decimal");
+                OG_CHECK(sp.agg_out_scales[i] >= sp.agg_in_scales[i] && sp.agg_out_scales[i] - sp.agg_in_scales[i] <= 38,
+                         OMNI_GPU_ERR_INVALID, "decimal SUM output scale must be >= input scale");
+                d.kind = K_SUM_DEC;
+                d.dec_scale_up = sp.agg_out_scales[i] - sp.agg_in_scales[i];
+                d.dec_out128 = to == OMNI_TYPE_DECIMAL128;
+                return OMNI_GPU_OK;
+            }
+            if (ti == OMNI_TYPE_DOUBLE) {
+                OG_CHECK(to == OMNI_TYPE_DOUBLE, OMNI_GPU_ERR_INVALID, "SUM(double) output must be DOUBLE");
+                d.kind = K_SUM_F64;
+                return OMNI_GPU_OK;
+            }
+            OG_CHECK(intIn && to == OMNI_TYPE_LONG, OMNI_GPU_ERR_INVALID, "SUM(int) output must be LONG");
+            d.kind = K_SUM_I64;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_AVG:
+            OG_CHECK((intIn || ti == OMNI_TYPE_DOUBLE) && to == OMNI_TYPE_DOUBLE, OMNI_GPU_ERR_INVALID,
+                     "AVG supports integer/double input with DOUBLE output");
+            d.kind = K_AVG;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_MIN:
+        case OMNI_AGG_MAX: {
+            bool mn = f == OMNI_AGG_MIN;
+            OG_CHECK(ti == to, OMNI_GPU_ERR_INVALID, "MIN/MAX output type must equal input type");
+            if (ti == OMNI_TYPE_SHORT || ti == OMNI_TYPE_INT || ti == OMNI_TYPE_DATE32) d.kind = mn ? K_MIN_I32 : K_MAX_I32;
+            else if (ti == OMNI_TYPE_LONG || ti == OMNI_TYPE_TIMESTAMP || ti == OMNI_TYPE_DECIMAL64) d.kind = mn ? K_MIN_I64 : K_MAX_I64;
+            else if (ti == OMNI_TYPE_DOUBLE) d.kind = mn ? K_MIN_F64 : K_MAX_F64;
+            else return SetError(OMNI_GPU_ERR_INVALID, "MIN/MAX over type %d is not supported on the GPU path", ti);
+            return OMNI_GPU_OK;
+        }
+        default: return SetError(OMNI_GPU_ERR_INVALID, "aggregate function %d is not supported", f);
+    }
+}
+
+// (Re)initialises the per-group state of groups [from, to).
+int InitStates(AggHandle &h, int64_t from, int64_t to)
+{
+    int64_t n = to - from;
+    if (n <= 0) return OMNI_GPU_OK;
+    for (int i = 0; i < h.args.n; ++i) {
+        AggDev &d = h.args.a[i];
+        unsigned long long init = 0;
+        bool is32 = d.kind == K_MIN_I32 || d.kind == K_MAX_I32;
+        switch (d.kind) {
+            case K_MIN_I64: init = 0x7FFFFFFFFFFFFFFFULL; break;
+            case K_MAX_I64: init = 0x8000000000000000ULL; break;
+            case K_MIN_I32: init = 0x7FFFFFFFu; break;
+            case K_MAX_I32: init = 0x80000000u; break;
+            case K_MIN_F64: { double x = 1.7976931348623157e308; std::memcpy(&init, &x, 8); break; }
+            case K_MAX_F64: { double x = -1.7976931348623157e308; std::memcpy(&init, &x, 8); break; }
+            default: init = 0;
+        }
+        if (is32) OG_TRY(Fill32(h.stream, static_cast<unsigned int *>(d.s0) + from, static_cast<unsigned int>(init), n));
+        else OG_TRY(Fill64(h.stream, static_cast<unsigned long long *>(d.s0) + from, init, n));
+        if (d.s1) OG_TRY(Fill64(h.stream, static_cast<unsigned long long *>(d.s1) + from, 0, n));
+        if (d.s2) OG_TRY(Fill64(h.stream, static_cast<unsigned long long *>(d.s2) + from, 0, n));
+        if (d.flag) OG_HIP(hipMemsetAsync(d.flag + from, 0, n, h.stream));
+    }
+    return OMNI_GPU_OK;
+}
+
+// Allocates a table of `cap` slots and group arrays for cap * kMaxLoad groups, keeping groups [0, keep).
+int Resize(AggHandle &h, uint64_t cap, int64_t keep)
+{
+    hipStream_t s = h.stream;
+    int64_t newMax = static_cast<int64_t>(static_cast<double>(cap) * kMaxLoad);
+    int W = h.single ? 1 : h.kd.W;
+    // group arrays (contents kept)
+    OG_TRY(h.gkeys.Grow(sizeof(unsigned long long) * W * newMax, sizeof(unsigned long long) * W * keep, s));
+    if (h.single) OG_TRY(h.gnull.Grow(newMax, keep, s));
+    for (int i = 0; i < h.args.n; ++i) {
+        AggDev &d = h.args.a[i];
+        bool is32 = d.kind == K_MIN_I32 || d.kind == K_MAX_I32;
+        size_t w0 = is32 ? 4 : 8;
+        OG_TRY(h.state[i][0].Grow(w0 * newMax, w0 * keep, s));
+        d.s0 = h.state[i][0].p;
+        if (d.kind == K_AVG || d.kind == K_SUM_DEC) {
+            OG_TRY(h.state[i][1].Grow(8 * newMax, 8 * keep, s));
+            d.s1 = h.state[i][1].p;
+        }
+        if (d.kind == K_SUM_DEC) {
+            OG_TRY(h.state[i][2].Grow(8 * newMax, 8 * keep, s));
+            d.s2 = h.state[i][2].p;
+        }
+        if (d.kind != K_COUNT_ALL && d.kind != K_COUNT_COL && d.kind != K_AVG) {
+            OG_TRY(h.state[i][3].Grow(newMax, keep, s));
+            d.flag = h.state[i][3].as<uint8_t>();
+        }
+    }
+    OG_TRY(InitStates(h, keep, newMax));
+    // fresh table, then re-insert the kept groups under their ids
+    unsigned long long mask = cap - 1;
+    if (h.single) {
+        OG_TRY(h.slots.Reserve(sizeof(Slot1) * cap, s));
+        {
+            KernelScope ks(s, "agg_init_slots");
+            init_slots1<<<Blocks(static_cast<int64_t>(cap)), kBlock, 0, s>>>(h.slots.as<Slot1>(), static_cast<int64_t>(cap));
+            OG_TRY(CheckLaunch("init_slots1"));
+        }
+        if (keep > 0) {
+            KernelScope ks(s, "agg_rehash");
+            agg_rehash_single<<<Blocks(keep), kBlock, 0, s>>>(h.gkeys.as<unsigned long long>(), h.gnull.as<uint8_t>(),
+                                                               keep, h.slots.as<Slot1>(), mask, h.special());
+            OG_TRY(CheckLaunch("agg_rehash_single"));
+        }
+    } else {
+        OG_TRY(h.ctrl.Reserve(sizeof(unsigned int) * cap, s));
+        OG_TRY(h.skeys.Reserve(sizeof(unsigned long long) * W * cap, s));
+        OG_TRY(Fill32(s, h.ctrl.p, 0, static_cast<int64_t>(cap)));
+        if (keep > 0) {
+            KernelScope ks(s, "agg_rehash");
+            agg_rehash_multi<<<Blocks(keep), kBlock, 0, s>>>(h.gkeys.as<unsigned long long>(), keep, W,
+                                                              h.ctrl.as<unsigned int>(), h.skeys.as<unsigned long long>(), mask);
+            OG_TRY(CheckLaunch("agg_rehash_multi"));
+        }
+    }
+    h.cap = cap;
+    h.max_groups = newMax;
+    return OMNI_GPU_OK;
+}
+
+int SyncGroups(AggHandle &h)
+{
+    unsigned long long g = 0;
+    unsigned int ovf = 0;
+    OG_HIP(hipMemcpyAsync(&g, h.ngroups(), 8, hipMemcpyDeviceToHost, h.stream));
+    OG_HIP(hipMemcpyAsync(&ovf, h.overflow(), 4, hipMemcpyDeviceToHost, h.stream));
+    OG_HIP(hipStreamSynchronize(h.stream));
+    OG_CHECK(ovf == 0, OMNI_GPU_ERR_STATE, "aggregation table overflowed its group arrays (internal sizing error)");
+    h.known_groups = static_cast<int64_t>(g);
+    h.pending_rows = 0;
+    return OMNI_GPU_OK;
+}
+
+}  // namespace omni_gpu
+
+using namespace omni_gpu;
+
+extern "C" {
+
+int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
+{
+    OG_CHECK(spec && handle, OMNI_GPU_ERR_INVALID, "null argument");
+    OG_CHECK(spec->num_keys >= 1 && spec->num_keys <= OMNI_GPU_MAX_KEYS, OMNI_GPU_ERR_INVALID,
+             "num_keys must be in [1, %d]", OMNI_GPU_MAX_KEYS);
+    OG_CHECK(spec->num_aggs >= 0 && spec->num_aggs <= OMNI_GPU_MAX_AGGS, OMNI_GPU_ERR_INVALID,
+             "num_aggs must be in [0, %d]", OMNI_GPU_MAX_AGGS);
+    std::unique_ptr<AggHandle> h(new AggHandle());
+    h->spec = *spec;
+    h->device = CurrentDevice();
+    // key layout
+    KeyDev &kd = h->kd;
+    kd.num = spec->num_keys;
+    for (int c = 0; c < kd.num; ++c) {
+        kd.width[c] = TypeWidth(spec->key_types[c]);
+        OG_CHECK(kd.width[c] > 0, OMNI_GPU_ERR_INVALID, "key type %d is not supported", spec->key_types[c]);
+    }
+    h->single = kd.num == 1 && kd.width[0] <= 8;
+    if (h->single) {
+        kd.W = 1;
+        kd.word[0] = 0;
+    } else {
+        int w = 1;  // word 0: NULL bits (low 32) + first <= 4-byte column (high 32)
+        bool word0Free = true;
+        for (int c = 0; c < kd.num; ++c) {
+            if (kd.width[c] <= 4 && word0Free) {
+                kd.in_word0[c] = 1;
+                kd.word[c] = 0;
+                word0Free = false;
+            } else {
+                kd.word[c] = w;
+                w += kd.width[c] == 16 ? 2 : 1;
+            }
+        }
+        kd.W = w;
+        OG_CHECK(kd.W <= kMaxWords, OMNI_GPU_ERR_INVALID, "composite key too wide (%d words)", kd.W);
+    }
+    h->args.n = spec->num_aggs;
+    for (int i = 0; i < spec->num_aggs; ++i) OG_TRY(ClassifyAgg(*spec, i, h->args.a[i]));
+    OG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    OG_TRY(h->counters.Reserve(64, h->stream));
+    OG_HIP(hipMemsetAsync(h->counters.p, 0, 64, h->stream));
+    uint64_t want = spec->capacity_hint > 0 ? static_cast<uint64_t>(spec->capacity_hint) : 16384;
+    uint64_t cap = NextPow2(static_cast<uint64_t>(static_cast<double>(want + 2) / kMaxLoad) + 1);
+    cap = std::max<uint64_t>(cap, 1024);
+    OG_TRY(Resize(*h, cap, 0));
+    *handle = h.release();
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs, int64_t rows)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    AggHandle &h = *static_cast<AggHandle *>(handle);
+    OG_CHECK(rows >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
+    if (rows == 0) return OMNI_GPU_OK;
+    OG_CHECK(keys, OMNI_GPU_ERR_INVALID, "null key columns");
+    OG_CHECK(h.args.n == 0 || aggs, OMNI_GPU_ERR_INVALID, "null aggregate columns");
+    for (int c = 0; c < h.kd.num; ++c) {
+        OG_CHECK(keys[c].type_id == h.spec.key_types[c], OMNI_GPU_ERR_INVALID, "key %d type %d != spec %d", c,
+                 keys[c].type_id, h.spec.key_types[c]);
+        OG_CHECK(keys[c].values, OMNI_GPU_ERR_INVALID, "key %d has no values", c);
+        h.kd.values[c] = keys[c].values;
+        h.kd.nulls[c] = keys[c].nulls;
+    }
+    for (int i = 0; i < h.args.n; ++i) {
+        AggDev &d = h.args.a[i];
+        if (d.kind == K_COUNT_ALL) {
+            d.values = nullptr;
+            d.nulls = nullptr;
+            continue;
+        }
+        OG_CHECK(aggs[i].type_id == h.spec.agg_in_types[i], OMNI_GPU_ERR_INVALID, "aggregate %d input type %d != spec %d",
+                 i, aggs[i].type_id, h.spec.agg_in_types[i]);
+        OG_CHECK(aggs[i].values, OMNI_GPU_ERR_INVALID, "aggregate %d has no values", i);
+        d.values = aggs[i].values;
+        d.nulls = aggs[i].nulls;
+    }
+    // Capacity: groups after this batch <= known + pending + rows + 2 side groups.
+    if (h.known_groups + h.pending_rows + rows + 2 > h.max_groups) {
+        OG_TRY(SyncGroups(h));
+        int64_t need = h.known_groups + rows + 2;
+        if (need > h.max_groups) {
+            uint64_t cap = h.cap;
+            while (static_cast<int64_t>(static_cast<double>(cap) * kMaxLoad) < need) cap *= 4;
+            OG_TRY(Resize(h, cap, h.known_groups));
+        }
+    }
+    GroupCtx g = h.Ctx();
+    {
+        KernelScope ks(h.stream, h.single ? "agg_insert_single" : "agg_insert_multi");
+        if (h.single) {
+            agg_insert_single<<<Blocks(rows), kBlock, 0, h.stream>>>(h.kd, rows, h.slots.as<Slot1>(), h.cap - 1, g, h.args);
+        } else {
+            agg_insert_multi<<<Blocks(rows), kBlock, 0, h.stream>>>(h.kd, rows, h.ctrl.as<unsigned int>(),
+                                                                   h.skeys.as<unsigned long long>(), h.cap - 1, g, h.args);
+        }
+        OG_TRY(CheckLaunch("agg_insert"));
+    }
+    h.pending_rows += rows;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_num_groups(void *handle, int64_t *groups)
+{
+    OG_CHECK(handle && groups, OMNI_GPU_ERR_INVALID, "null argument");
+    AggHandle &h = *static_cast<AggHandle *>(handle);
+    OG_TRY(SyncGroups(h));
+    *groups = h.known_groups;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_get_output(void *handle, int64_t start, int64_t count, omni_gpu_out_column *outs)
+{
+    OG_CHECK(handle && outs, OMNI_GPU_ERR_INVALID, "null argument");
+    AggHandle &h = *static_cast<AggHandle *>(handle);
+    OG_TRY(SyncGroups(h));
+    OG_CHECK(start >= 0 && count >= 0 && start + count <= h.known_groups, OMNI_GPU_ERR_INVALID,
+             "output range [%lld, %lld) outside [0, %lld)", static_cast<long long>(start),
+             static_cast<long long>(start + count), static_cast<long long>(h.known_groups));
+    if (count == 0) return OMNI_GPU_OK;
+    OutArgs oa{};
+    oa.nkeys = h.kd.num;
+    oa.naggs = h.args.n;
+    for (int c = 0; c < h.kd.num + h.args.n; ++c) {
+        int expect = c < h.kd.num ? h.spec.key_types[c] : h.spec.agg_out_types[c - h.kd.num];
+        OG_CHECK(outs[c].type_id == expect, OMNI_GPU_ERR_INVALID, "output column %d type %d != %d", c, outs[c].type_id, expect);
+        OG_CHECK(outs[c].values && outs[c].nulls, OMNI_GPU_ERR_INVALID, "output column %d lacks buffers", c);
+        oa.o[c].width = TypeWidth(outs[c].type_id);
+        oa.o[c].values = outs[c].values;
+        oa.o[c].nulls = outs[c].nulls;
+    }
+    OG_HIP(hipMemsetAsync(h.error_flag(), 0, 4, h.stream));
+    {
+        KernelScope ks(h.stream, "agg_output");
+        agg_output<<<Blocks(count), kBlock, 0, h.stream>>>(start, count, h.kd, h.single ? 1 : 0,
+                                                          h.gkeys.as<unsigned long long>(), h.gnull.as<uint8_t>(),
+                                                          h.args, oa, h.spec.overflow_as_null, h.error_flag());
+        OG_TRY(CheckLaunch("agg_output"));
+    }
+    unsigned int err = 0;
+    OG_HIP(hipMemcpyAsync(&err, h.error_flag(), 4, hipMemcpyDeviceToHost, h.stream));
+    OG_HIP(hipStreamSynchronize(h.stream));
+    if (err) return SetError(OMNI_GPU_ERR_OVERFLOW, "decimal overflow in SUM (overflowAsNull is false)");
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_destroy(void *handle)
+{
+    if (!handle) return OMNI_GPU_OK;
+    AggHandle *h = static_cast<AggHandle *>(handle);
+    hipStream_t s = h->stream;
+    h->slots.Release(s);
+    h->ctrl.Release(s);
+    h->skeys.Release(s);
+    h->gkeys.Release(s);
+    h->gnull.Release(s);
+    for (auto &st : h->state)
+        for (auto &b : st) b.Release(s);
+    h->counters.Release(s);
+    if (s) {
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    }
+    delete h;
+    return OMNI_GPU_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,343 @@
+"""CPU oracle for the hash-aggregation / hash-join hot path (test infrastructure only).
+
+`oracle/omni_ref_cpu.cpp` restates the reference operators (Swiss-table BaseHashMap, CRC32C and
+xxHash64-style hashing, HashAggregationOperator, HashBuilderOperator + LookupJoinOperator) in C++;
+this module loads it (`oracle/lib/liboracle.so`, built by `make -C oracle`) through ctypes and
+offers numpy-level entry points.  `oracle.brute` holds an independent sort-based restatement used
+to pin this one.
+
+Only tests, `__graft_entry__.smoke()` and bench.py's CPU-baseline leg use this package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
+
+# DataTypeId values (core/src/type/data_type.h:40-65) and FunctionType values
+# (core/src/operator/util/function_type.h:11-27).
+INT, LONG, DOUBLE, BOOLEAN, SHORT, DECIMAL64, DECIMAL128, DATE32, TIMESTAMP = 1, 2, 3, 4, 5, 6, 7, 8, 12
+SUM, COUNT_COLUMN, COUNT_ALL, AVG, MAX, MIN = 0, 1, 2, 3, 5, 6
+MAX_KEYS, MAX_AGGS = 4, 16
+
+NP_DTYPE = {
+    INT: np.int32, DATE32: np.int32, LONG: np.int64, TIMESTAMP: np.int64, DECIMAL64: np.int64,
+    DOUBLE: np.float64, SHORT: np.int16, BOOLEAN: np.uint8,
+}
+
+
+def np_dtype(type_id: int):
+    """numpy storage type of a column; DECIMAL128 is stored as (n, 2) uint64 [low, high]."""
+    return np.uint64 if type_id == DECIMAL128 else NP_DTYPE[type_id]
+
+
+@dataclass
+class Col:
+    """A flat column: values + optional boolean NULL mask (True = NULL)."""
+    type_id: int
+    values: np.ndarray
+    nulls: Optional[np.ndarray] = None
+    scale: int = 0
+
+    def __len__(self):
+        return int(self.values.shape[0])
+
+    def slice(self, a: int, b: int) -> "Col":
+        return Col(self.type_id, self.values[a:b], None if self.nulls is None else self.nulls[a:b], self.scale)
+
+
+def pack_nulls(mask: Optional[np.ndarray]) -> Optional[np.ndarray]:
+    """Boolean mask -> OmniVec LSB-first bitmap (bit = 1 means NULL)."""
+    if mask is None:
+        return None
+    return np.packbits(np.asarray(mask, dtype=bool), bitorder="little")
+
+
+def unpack_nulls(bitmap: np.ndarray, n: int) -> np.ndarray:
+    return np.unpackbits(bitmap, count=n, bitorder="little").astype(bool)
+
+
+class AggSpec(C.Structure):
+    """Layout of omni_gpu_agg_spec (include/omni_gpu.h)."""
+    _fields_ = [
+        ("num_keys", C.c_int32), ("key_types", C.c_int32 * MAX_KEYS),
+        ("num_aggs", C.c_int32), ("agg_funcs", C.c_int32 * MAX_AGGS),
+        ("agg_in_types", C.c_int32 * MAX_AGGS), ("agg_out_types", C.c_int32 * MAX_AGGS),
+        ("agg_in_scales", C.c_int32 * MAX_AGGS), ("agg_out_scales", C.c_int32 * MAX_AGGS),
+        ("overflow_as_null", C.c_int32), ("strategy", C.c_int32), ("capacity_hint", C.c_int64),
+    ]
+
+
+def make_agg_spec(key_types: Sequence[int], aggs: Sequence[tuple], overflow_as_null: bool = True,
+                  strategy: int = 0, capacity_hint: int = 0) -> AggSpec:
+    """aggs: tuples (func, in_type, out_type[, in_scale, out_scale])."""
+    s = AggSpec()
+    s.num_keys = len(key_types)
+    for i, t in enumerate(key_types):
+        s.key_types[i] = t
+    s.num_aggs = len(aggs)
+    for i, a in enumerate(aggs):
+        f, ti, to = a[0], a[1], a[2]
+        s.agg_funcs[i], s.agg_in_types[i], s.agg_out_types[i] = f, ti, to
+        if len(a) > 3:
+            s.agg_in_scales[i], s.agg_out_scales[i] = a[3], a[4]
+    s.overflow_as_null = 1 if overflow_as_null else 0
+    s.strategy = strategy
+    s.capacity_hint = capacity_hint
+    return s
+
+
+class _OColumn(C.Structure):
+    _fields_ = [("type_id", C.c_int32), ("scale", C.c_int32), ("values", C.c_void_p), ("nulls", C.c_void_p)]
+
+
+class _OutColumn(C.Structure):
+    _fields_ = [("type_id", C.c_int32), ("reserved", C.c_int32), ("values", C.c_void_p), ("nulls", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        u64, i64, i32, u32, vp = C.c_uint64, C.c_int64, C.c_int32, C.c_uint32, C.c_void_p
+        sig = {
+            "oracle_hash_uint64": (u64, [u64]),
+            "oracle_crc32c_step_software": (u32, [u32, u64]),
+            "oracle_crc_hash_int": (u64, [i64, i32]),
+            "oracle_crc_hash_decimal128": (u64, [u64, u64]),
+            "oracle_crc32c_extend": (u32, [u32, vp, C.c_size_t]),
+            "oracle_join_hash_int": (u64, [i64]),
+            "oracle_hash_value_i64": (i64, [i64]),
+            "oracle_hash_value_double": (i64, [C.c_double]),
+            "oracle_hash_value_decimal128": (i64, [i64, i64]),
+            "oracle_xxhash64": (i64, [vp, i32]),
+            "oracle_raw_hash_partition": (u32, [i64, u32]),
+            "oracle_raw_hash_position": (u32, [i64, u64]),
+            "oracle_serialize_row": (i32, [vp, i32, i64, vp, i32]),
+            "oracle_map_create": (vp, [i32]),
+            "oracle_map_emplace": (i32, [vp, i64, i32, u32]),
+            "oracle_map_size": (i64, [vp]),
+            "oracle_map_capacity": (i64, [vp]),
+            "oracle_map_values_in_order": (i64, [vp, vp]),
+            "oracle_map_destroy": (None, [vp]),
+            "oracle_agg_create": (vp, [vp]),
+            "oracle_agg_add_batch": (i32, [vp, vp, vp, i64]),
+            "oracle_agg_num_groups": (i64, [vp]),
+            "oracle_agg_get_output": (i32, [vp, vp]),
+            "oracle_agg_destroy": (None, [vp]),
+            "oracle_join_create": (vp, [i32]),
+            "oracle_join_add_batch": (i32, [vp, vp, vp, i32, i64]),
+            "oracle_join_finish": (i32, [vp]),
+            "oracle_join_is_array": (i32, [vp]),
+            "oracle_join_probe": (i64, [vp, vp, i64, i64, vp, vp, i64]),
+            "oracle_join_gather": (None, [vp, i32, vp, i64, vp, vp]),
+            "oracle_join_destroy": (None, [vp]),
+            "oracle_last_error": (C.c_char_p, []),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype, fn.argtypes = res, args
+        _lib = L
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data
+
+
+def _ocols(cols: Sequence[Optional[Col]], n: int):
+    arr = (_OColumn * n)()
+    keep = []
+    for i, c in enumerate(cols):
+        if c is None:
+            continue
+        v = np.ascontiguousarray(c.values)
+        nb = pack_nulls(c.nulls)
+        keep += [v, nb]
+        arr[i].type_id, arr[i].scale, arr[i].values, arr[i].nulls = c.type_id, c.scale, _ptr(v), _ptr(nb)
+    return arr, keep
+
+
+# ---- hashes -------------------------------------------------------------------------------------------
+def hash_uint64(x: int) -> int:
+    return lib().oracle_hash_uint64(x & (2**64 - 1))
+
+
+def crc32c_step_software(crc: int, x: int) -> int:
+    return lib().oracle_crc32c_step_software(crc, x & (2**64 - 1))
+
+
+def crc_hash_int(v: int, type_id: int = LONG) -> int:
+    return lib().oracle_crc_hash_int(v, type_id)
+
+
+def crc_hash_decimal128(lo: int, hi: int) -> int:
+    return lib().oracle_crc_hash_decimal128(lo & (2**64 - 1), hi & (2**64 - 1))
+
+
+def crc32c_extend(crc: int, data: bytes) -> int:
+    b = np.frombuffer(data, dtype=np.uint8).copy()
+    return lib().oracle_crc32c_extend(crc, _ptr(b), len(data))
+
+
+def join_hash_int(v: int) -> int:
+    return lib().oracle_join_hash_int(v)
+
+
+def hash_value_i64(v: int) -> int:
+    return lib().oracle_hash_value_i64(v)
+
+
+def hash_value_double(v: float) -> int:
+    return lib().oracle_hash_value_double(v)
+
+
+def hash_value_decimal128(low: int, high: int) -> int:
+    return lib().oracle_hash_value_decimal128(low, high)
+
+
+def xxhash64(data: bytes) -> int:
+    b = np.frombuffer(data, dtype=np.uint8).copy() if data else np.zeros(1, np.uint8)
+    return lib().oracle_xxhash64(_ptr(b), len(data))
+
+
+def raw_hash_partition(h: int, mask: int) -> int:
+    return lib().oracle_raw_hash_partition(h, mask)
+
+
+def raw_hash_position(h: int, mask: int) -> int:
+    return lib().oracle_raw_hash_position(h, mask)
+
+
+def serialize_row(cols: Sequence[Col], row: int) -> bytes:
+    arr, keep = _ocols(cols, len(cols))
+    out = np.zeros(256, np.uint8)
+    n = lib().oracle_serialize_row(arr, len(cols), row, _ptr(out), 256)
+    return bytes(out[:n])
+
+
+class SwissMap:
+    """BaseHashMap<int64, uint32> with the aggregation hasher (CRC32C), for hash-map behaviour tests."""
+
+    def __init__(self, degree: int = 15):
+        self.h = lib().oracle_map_create(degree)
+
+    def emplace(self, key: Optional[int], value: int) -> bool:
+        return bool(lib().oracle_map_emplace(self.h, 0 if key is None else key, 1 if key is None else 0, value))
+
+    def size(self) -> int:
+        return lib().oracle_map_size(self.h)
+
+    def capacity(self) -> int:
+        return lib().oracle_map_capacity(self.h)
+
+    def values_in_order(self) -> np.ndarray:
+        out = np.zeros(self.size(), np.uint32)
+        n = lib().oracle_map_values_in_order(self.h, _ptr(out))
+        return out[:n]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_map_destroy(self.h)
+            self.h = None
+
+
+# ---- aggregation ------------------------------------------------------------------------------------------
+def type_width(type_id: int) -> int:
+    return {SHORT: 2, INT: 4, DATE32: 4, LONG: 8, TIMESTAMP: 8, DECIMAL64: 8, DOUBLE: 8, DECIMAL128: 16,
+            BOOLEAN: 1}[type_id]
+
+
+def alloc_out(type_id: int, n: int) -> Col:
+    shape = (n, 2) if type_id == DECIMAL128 else (n,)
+    return Col(type_id, np.zeros(shape, dtype=np_dtype(type_id)), np.zeros(n, dtype=bool))
+
+
+def hash_agg(spec: AggSpec, key_cols: Sequence[Col], agg_cols: Sequence[Optional[Col]], batch_rows: int) -> List[Col]:
+    """Runs the restated HashAggregationOperator over the input cut into VectorBatches of batch_rows
+    rows; returns the output columns (keys, then one per aggregate) in the reference's slot order."""
+    L = lib()
+    h = L.oracle_agg_create(C.byref(spec))
+    try:
+        n = len(key_cols[0])
+        for a in range(0, max(n, 1) if n else 0, batch_rows):
+            b = min(n, a + batch_rows)
+            karr, k1 = _ocols([c.slice(a, b) for c in key_cols], MAX_KEYS)
+            aarr, k2 = _ocols([None if c is None else c.slice(a, b) for c in agg_cols], MAX_AGGS)
+            L.oracle_agg_add_batch(h, karr, aarr, b - a)
+        g = L.oracle_agg_num_groups(h)
+        out_types = [spec.key_types[i] for i in range(spec.num_keys)] + \
+                    [spec.agg_out_types[i] for i in range(spec.num_aggs)]
+        outs = [alloc_out(t, g) for t in out_types]
+        bitmaps = [np.zeros((g + 7) // 8 + 1, np.uint8) for _ in outs]
+        oarr = (_OutColumn * len(outs))()
+        for i, (o, bm) in enumerate(zip(outs, bitmaps)):
+            oarr[i].type_id, oarr[i].values, oarr[i].nulls = o.type_id, _ptr(o.values), _ptr(bm)
+        rc = L.oracle_agg_get_output(h, oarr)
+        if rc != 0:
+            raise OverflowError(L.oracle_last_error().decode())
+        for o, bm in zip(outs, bitmaps):
+            o.nulls = unpack_nulls(bm, g)
+        return outs
+    finally:
+        L.oracle_agg_destroy(h)
+
+
+# ---- join ---------------------------------------------------------------------------------------------------
+@dataclass
+class JoinResult:
+    probe_rows: np.ndarray
+    build_rows: np.ndarray
+    payloads: List[Col]
+    is_array: bool
+
+
+def hash_join(build_key: Col, build_payloads: Sequence[Col], probe_key: Col, build_batch: int,
+              probe_batch: int) -> JoinResult:
+    """Restated HashBuilderOperator + LookupJoinOperator (inner join, single key)."""
+    L = lib()
+    h = L.oracle_join_create(build_key.type_id)
+    try:
+        nb = len(build_key)
+        for a in range(0, nb, build_batch):
+            b = min(nb, a + build_batch)
+            karr, k1 = _ocols([build_key.slice(a, b)], 1)
+            parr, k2 = _ocols([p.slice(a, b) for p in build_payloads], max(1, len(build_payloads)))
+            L.oracle_join_add_batch(h, karr, parr, len(build_payloads), b - a)
+        L.oracle_join_finish(h)
+        is_array = bool(L.oracle_join_is_array(h))
+        np_rows, nb_rows = [], []
+        npr = len(probe_key)
+        for a in range(0, npr, probe_batch):
+            b = min(npr, a + probe_batch)
+            karr, k1 = _ocols([probe_key.slice(a, b)], 1)
+            m = L.oracle_join_probe(h, karr, b - a, a, None, None, 0)
+            pr = np.zeros(max(m, 1), np.int64)
+            br = np.zeros(max(m, 1), np.int64)
+            L.oracle_join_probe(h, karr, b - a, a, _ptr(pr), _ptr(br), m)
+            np_rows.append(pr[:m])
+            nb_rows.append(br[:m])
+        probe_rows = np.concatenate(np_rows) if np_rows else np.zeros(0, np.int64)
+        build_rows = np.concatenate(nb_rows) if nb_rows else np.zeros(0, np.int64)
+        pays = []
+        m = len(build_rows)
+        for p, bp in enumerate(build_payloads):
+            o = alloc_out(bp.type_id, m)
+            bm = np.zeros((m + 7) // 8 + 1, np.uint8)
+            L.oracle_join_gather(h, p, _ptr(build_rows), m, _ptr(o.values), _ptr(bm))
+            o.nulls = unpack_nulls(bm, m)
+            pays.append(o)
+        return JoinResult(probe_rows, build_rows, pays, is_array)
+    finally:
+        L.oracle_join_destroy(h)

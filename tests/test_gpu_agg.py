@@ -1,0 +1,175 @@
+"""GPU hash aggregation (HIP kernels through the C ABI) against the CPU oracle: bit-exact for integer,
+decimal, COUNT, MIN and MAX results; double SUM/AVG within 1e-12 relative (order of atomic additions)."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import (AVG, COUNT_ALL, COUNT_COLUMN, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, MIN, SUM, Col)
+from omnioperator_amd import capi, workloads
+from tests.helpers import assert_row_sets_equal, load_golden, rows_of
+from tests.test_oracle_agg import STD_AGGS, TYPES, _random_case
+
+pytestmark = pytest.mark.gpu
+FIX = load_golden("operator_fixtures.json")
+DOUBLE_RTOL = 1e-12
+
+
+def run_gpu(key_cols, aggs, agg_cols, batch_rows, strategy=capi.STRATEGY_AUTO, capacity_hint=0,
+            overflow_as_null=True):
+    spec = capi.make_agg_spec([c.type_id for c in key_cols], aggs, overflow_as_null, strategy, capacity_hint)
+    h = capi.HashAgg(spec)
+    n = len(key_cols[0])
+    keep = []
+    for a, b in workloads.batches(n, batch_rows):
+        kd = [capi.DeviceColumn(c.type_id, c.values[a:b], None if c.nulls is None else c.nulls[a:b]) for c in key_cols]
+        ad = [None if c is None else capi.DeviceColumn(c.type_id, c.values[a:b], None if c.nulls is None else c.nulls[a:b])
+              for c in agg_cols]
+        keep.append((kd, ad))
+        h.add_batch([d.c() for d in kd], [None if d is None else d.c() for d in ad], b - a)
+    outs = h.output()
+    types = [c.type_id for c in key_cols] + [a[2] for a in aggs]
+    h.close()
+    return rows_of([(t, v, nl) for t, (v, nl) in zip(types, outs)])
+
+
+def run_oracle(key_cols, aggs, agg_cols, batch_rows, overflow_as_null=True):
+    spec = oracle.make_agg_spec([c.type_id for c in key_cols], aggs, overflow_as_null)
+    outs = oracle.hash_agg(spec, key_cols, agg_cols, batch_rows)
+    return rows_of([(c.type_id, c.values, c.nulls) for c in outs])
+
+
+def float_cols(nkeys, aggs):
+    return [nkeys + i for i, a in enumerate(aggs) if a[2] == DOUBLE]
+
+
+def test_verify_correctness_fixture():
+    f = FIX["agg_verify_correctness"]
+    n = f["batches"] * f["rows_per_batch"]
+    j = np.tile(np.arange(f["rows_per_batch"], dtype=np.int64) % f["cardinality"], f["batches"])
+    got = run_gpu([Col(LONG, j), Col(LONG, j.copy())], STD_AGGS, [Col(LONG, np.ones(n, np.int64)) for _ in range(5)],
+                  f["rows_per_batch"])
+    e = f["expected"]
+    want = [(k, k, e["sum"], e["avg"], e["count"], e["min"], e["max"]) for k in e["keys"]]
+    assert_row_sets_equal(got, want, float_cols=[3], rtol=DOUBLE_RTOL)
+
+
+@pytest.mark.parametrize("key_type", ["SHORT", "INT", "DATE32", "LONG", "TIMESTAMP", "DECIMAL64"])
+def test_null_key_fixture(key_type):
+    f = FIX["agg_null_keys"]
+    t, r = TYPES[key_type], f["rows_per_batch"]
+    k = np.tile(np.arange(r) % f["cardinality"], 2).astype(oracle.np_dtype(t))
+    kn = np.zeros(2 * r, bool)
+    for b in range(2):
+        kn[[b * r + i for i in f["key_null_rows"][b]]] = True
+    ac = []
+    for a in range(5):
+        m = np.zeros(2 * r, bool)
+        for b in range(2):
+            m[[b * r + i for i in f["agg_null_rows"][str(a + 1)][b]]] = True
+        ac.append(Col(LONG, np.ones(2 * r, np.int64), m))
+    got = run_gpu([Col(t, k, kn)], STD_AGGS, ac, r)
+    assert_row_sets_equal(got, [tuple(x) for x in f["expected_rows"]], float_cols=[2], rtol=DOUBLE_RTOL)
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_sum_long_wraps(case):
+    c = FIX["sum_long"]["cases"][case]
+    v = np.array(c["values"], np.int64)
+    got = run_gpu([Col(LONG, np.zeros(len(v), np.int64))], [(SUM, LONG, LONG)], [Col(LONG, v)], 1024)
+    assert got == [(0, c["sum"])]
+
+
+def test_sum_decimal64_fixture():
+    f = FIX["sum_decimal64"]
+    got = run_gpu([Col(LONG, np.zeros(3, np.int64))], [(SUM, DECIMAL64, DECIMAL128, 6, 6)],
+                  [Col(DECIMAL64, np.array(f["values"], np.int64), scale=6)], 1024)
+    assert got == [(0, f["sum"])]
+
+
+def test_decimal_overflow_null_and_error():
+    v = np.array([2**62, 2**62, 5, 7, 10**18], np.int64)
+    k = np.array([0, 0, 1, 1, 2], np.int64)
+    aggs = [(SUM, DECIMAL64, DECIMAL64, 0, 2)]
+    got = run_gpu([Col(LONG, k)], aggs, [Col(DECIMAL64, v)], 16)
+    assert sorted(got, key=lambda r: r[0]) == [(0, None), (1, 1200), (2, None)]
+    assert_row_sets_equal(got, run_oracle([Col(LONG, k)], aggs, [Col(DECIMAL64, v)], 16))
+    with pytest.raises(capi.GpuError) as e:
+        run_gpu([Col(LONG, k)], aggs, [Col(DECIMAL64, v)], 16, overflow_as_null=False)
+    assert e.value.code == -4
+
+
+@pytest.mark.parametrize("key_kind", ["long", "int", "composite", "double"])
+@pytest.mark.parametrize("with_nulls", [False, True])
+def test_random_parity(key_kind, with_nulls):
+    keys, aggs, cols = _random_case(hash((key_kind, with_nulls, "gpu")) % 2**32, 20000, key_kind, with_nulls)
+    got = run_gpu(keys, aggs, cols, 3000)
+    want = run_oracle(keys, aggs, cols, 3000)
+    assert_row_sets_equal(got, want, float_cols=float_cols(len(keys), aggs), rtol=DOUBLE_RTOL)
+
+
+def test_growth_and_rehash():
+    """A tiny capacity hint forces several grow-and-rehash rounds; results must not change."""
+    rng = np.random.default_rng(5)
+    n = 300_000
+    k = rng.integers(0, 120_000, n, dtype=np.int64) * 1_000_003
+    v = rng.integers(-10**9, 10**9, n, dtype=np.int64)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, LONG, LONG)]
+    cols = [Col(LONG, v), None, Col(LONG, v)]
+    got = run_gpu([Col(LONG, k)], aggs, cols, 65536, capacity_hint=16)
+    want = run_oracle([Col(LONG, k)], aggs, cols, 65536)
+    assert_row_sets_equal(got, want)
+
+
+def test_composite_growth():
+    rng = np.random.default_rng(6)
+    n = 200_000
+    k1 = rng.integers(0, 300, n, dtype=np.int64).astype(np.int32)
+    k2 = rng.integers(0, 500, n, dtype=np.int64) * 7
+    aggs = [(COUNT_ALL, LONG, LONG), (MAX, INT, INT)]
+    cols = [None, Col(INT, k1)]
+    got = run_gpu([Col(INT, k1), Col(LONG, k2)], aggs, cols, 40000, capacity_hint=8)
+    want = run_oracle([Col(INT, k1), Col(LONG, k2)], aggs, cols, 40000)
+    assert_row_sets_equal(got, want)
+
+
+def test_sentinel_and_extreme_keys():
+    """INT64_MIN is the slot sentinel internally; as a real key it must still form its own group."""
+    k = np.array([-2**63, 2**63 - 1, -2**63, 0, -1, 2**63 - 1], np.int64)
+    n = np.array([False, False, False, True, False, False])
+    aggs = [(COUNT_ALL, LONG, LONG), (SUM, LONG, LONG)]
+    cols = [None, Col(LONG, np.arange(6, dtype=np.int64))]
+    got = run_gpu([Col(LONG, k, n)], aggs, cols, 4)
+    want = run_oracle([Col(LONG, k, n)], aggs, cols, 4)
+    assert_row_sets_equal(got, want)
+    assert len(got) == 4
+
+
+def test_empty_input():
+    spec = capi.make_agg_spec([LONG], [(SUM, LONG, LONG)])
+    h = capi.HashAgg(spec)
+    assert h.num_groups() == 0
+    h.close()
+
+
+def test_plain_agg_config_full_size():
+    """1M rows in batches of 4096, int64 key over 1,000 values; SUM/COUNT/MIN/MAX bit-exact."""
+    w = workloads.agg_plain()
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, LONG, LONG), (MAX, LONG, LONG)]
+    cols = [Col(LONG, w.values), None, Col(LONG, w.values), Col(LONG, w.values)]
+    got = run_gpu([Col(LONG, w.keys)], aggs, cols, w.batch_rows)
+    want = run_oracle([Col(LONG, w.keys)], aggs, cols, w.batch_rows)
+    assert len(got) == 1000
+    assert_row_sets_equal(got, want)
+
+
+def test_skewed_agg_reduced_size():
+    """The skewed composite-key configuration at 2M rows (Zipf over 5e5 ranks): decimal64 -> decimal128 SUM,
+    double SUM/AVG, COUNT, MAX(int32)."""
+    w = workloads.agg_skewed(rows=2_000_000, k1_distinct=1000, k2_ranks=500_000, seed=0xC3)
+    keys = [Col(INT, w.k1), Col(LONG, w.k2)]
+    aggs = [(SUM, DECIMAL64, DECIMAL128, 2, 2), (SUM, DOUBLE, DOUBLE), (AVG, DOUBLE, DOUBLE), (COUNT_ALL, LONG, LONG),
+            (MAX, INT, INT)]
+    cols = [Col(DECIMAL64, w.v_dec, scale=2), Col(DOUBLE, w.v_dbl), Col(DOUBLE, w.v_dbl), None, Col(INT, w.v_i32)]
+    got = run_gpu(keys, aggs, cols, 65536)
+    want = run_oracle(keys, aggs, cols, 65536)
+    assert_row_sets_equal(got, want, float_cols=float_cols(2, aggs), rtol=DOUBLE_RTOL)

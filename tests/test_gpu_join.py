@@ -1,0 +1,198 @@
+"""GPU inner hash join (HIP kernels through the C ABI) against the CPU oracle and the reference fixtures.
+Join matches are bit-exact as (probe row, payload) row sets."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import INT, LONG, SHORT, Col
+from omnioperator_amd import capi, workloads
+from tests.helpers import load_golden
+
+pytestmark = pytest.mark.gpu
+FIX = load_golden("operator_fixtures.json")
+
+
+def gpu_join(bk: Col, bpay, pk: Col, build_batch, probe_batch, strategy=0, batched=False):
+    j = capi.HashJoin(bk.type_id, [p.type_id for p in bpay], strategy)
+    keep = []  # build batches stay alive until finish (the copies are asynchronous)
+    for a, b in workloads.batches(len(bk), build_batch):
+        kd = capi.DeviceColumn(bk.type_id, bk.values[a:b], None if bk.nulls is None else bk.nulls[a:b])
+        pd = [capi.DeviceColumn(p.type_id, p.values[a:b], None if p.nulls is None else p.nulls[a:b]) for p in bpay]
+        keep.append((kd, pd))
+        j.add_build(kd.c(), [d.c() for d in pd], b - a)
+    j.finish()
+    del keep
+    info = j.info()
+    rows_out, pays_out = [], [[] for _ in bpay]
+    if batched:
+        cols = [capi.DeviceColumn(pk.type_id, pk.values[a:b], None if pk.nulls is None else pk.nulls[a:b])
+                for a, b in workloads.batches(len(pk), probe_batch)]
+        cap = len(pk) * max(1, info["max_dup"])
+        out_rows, pays, oarr = j._outs(cap, True)
+        m = j.probe_batches_raw([c.c() for c in cols], [c.rows for c in cols], out_rows, oarr, cap)
+        rows_out = [out_rows.numpy(m)]
+        for i, (v, nb) in enumerate(pays):
+            pays_out[i].append((v.numpy(m), capi.unpack_nulls(nb.numpy(), m)))
+    else:
+        for a, b in workloads.batches(len(pk), probe_batch):
+            kd = capi.DeviceColumn(pk.type_id, pk.values[a:b], None if pk.nulls is None else pk.nulls[a:b])
+            r, p = j.probe(kd.c(), b - a, row_base=a)
+            rows_out.append(r)
+            for i, x in enumerate(p):
+                pays_out[i].append(x)
+    j.close()
+    rows = np.concatenate(rows_out) if rows_out else np.zeros(0, np.int64)
+    pays = [(np.concatenate([x[0] for x in pl]) if pl else np.zeros(0),
+             np.concatenate([x[1] for x in pl]) if pl else np.zeros(0, bool)) for pl in pays_out]
+    return rows, pays, info
+
+
+def as_set(rows, pays):
+    cols = [rows.tolist()] + [[None if n else int(v) for v, n in zip(pv, pn)] for pv, pn in pays]
+    return sorted(zip(*cols), key=lambda r: tuple((x is None, x or 0) for x in r))
+
+
+def oracle_set(bk, bpay, pk, bb, pb):
+    r = oracle.hash_join(bk, bpay, pk, bb, pb)
+    return as_set(r.probe_rows, [(c.values, c.nulls) for c in r.payloads]), r.is_array
+
+
+def test_join_simple_fixture():
+    f = FIX["join_simple"]
+    bk = Col(LONG, np.array(f["build_keys"], np.int64))
+    bp = Col(LONG, np.array(f["build_payload"], np.int64))
+    pk = Col(LONG, np.array(f["probe_keys"], np.int64))
+    rows, pays, info = gpu_join(bk, [bp], pk, 1024, 1024)
+    pp = np.array(f["probe_payload"], np.int64)
+    got = sorted(zip(pp[rows].tolist(), pays[0][0].tolist()))
+    want = sorted(zip(pp[np.array(f["expected_probe_rows"])].tolist(), f["expected_build_payload"]))
+    assert got == want
+    assert info["max_dup"] == 3
+
+
+@pytest.mark.parametrize("strategy", [0, 1])
+def test_join_array_table_fixture(strategy):
+    f = FIX["join_array_table"]
+    bk = Col(SHORT, np.array(f["build_keys"], np.int16))
+    bp = Col(SHORT, np.array(f["build_payload"], np.int16))
+    pk = np.array(f["probe_keys"], np.int16)
+    pp = np.array(f["probe_payload"], np.int16)
+    rows, pays, info = gpu_join(bk, [bp], Col(SHORT, pk), 1024, 1024, strategy=strategy)
+    assert info["is_array"] == (strategy == 0)
+    got = sorted(zip(pk[rows].tolist(), pp[rows].tolist(), pk[rows].tolist(), pays[0][0].tolist()))
+    assert got == sorted(tuple(r) for r in f["expected_rows"])
+
+
+@pytest.mark.parametrize("dense", [False, True])
+@pytest.mark.parametrize("nulls", [False, True])
+@pytest.mark.parametrize("batched", [False, True])
+def test_random_parity(dense, nulls, batched):
+    rng = np.random.default_rng(100 + dense + 2 * nulls + 4 * batched)
+    nb, npr = 50_000, 300_000
+    if dense:
+        bkv = rng.integers(0, 30_000, nb, dtype=np.int64)
+        pkv = rng.integers(-1000, 32_000, npr, dtype=np.int64)
+    else:
+        pool = rng.integers(-2**63, 2**63 - 1, 40_000, dtype=np.int64)
+        bkv = pool[rng.integers(0, 40_000, nb)]
+        pkv = np.where(rng.random(npr) < 0.8, pool[rng.integers(0, 40_000, npr)],
+                       rng.integers(-2**63, 2**63 - 1, npr, dtype=np.int64))
+    bk = Col(LONG, bkv, rng.random(nb) < 0.03 if nulls else None)
+    pk = Col(LONG, pkv, rng.random(npr) < 0.03 if nulls else None)
+    pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32), rng.random(nb) < 0.1 if nulls else None),
+            Col(LONG, rng.integers(-2**63, 2**63 - 1, nb, dtype=np.int64))]
+    rows, gp, info = gpu_join(bk, pays, pk, 8192, 65536, batched=batched)
+    want, is_array = oracle_set(bk, pays, pk, 8192, 65536)
+    assert info["is_array"] == is_array
+    assert as_set(rows, gp) == want
+
+
+@pytest.mark.parametrize("key_type", [INT, SHORT])
+def test_narrow_keys_and_sentinel(key_type):
+    dt = oracle.np_dtype(key_type)
+    info_ = np.iinfo(dt)
+    bkv = np.array([info_.min, info_.max, 0, -1, info_.min, 5], dt)
+    pkv = np.array([info_.min, 5, 7, info_.max, -1, -1], dt)
+    bk, pk = Col(key_type, bkv), Col(key_type, pkv)
+    pay = [Col(INT, np.arange(6, dtype=np.int32))]
+    for strategy in (0, 1):
+        rows, gp, _ = gpu_join(bk, pay, pk, 4, 3, strategy=strategy)
+        want, _ = oracle_set(bk, pay, pk, 4, 3)
+        assert as_set(rows, gp) == want
+
+
+def test_int64_sentinel_key():
+    """INT64_MIN is the internal empty-slot marker; as a real key it must still join."""
+    bk = Col(LONG, np.array([-2**63, 3, -2**63, 2**63 - 1], np.int64))
+    pk = Col(LONG, np.array([-2**63, 2**63 - 1, 4, -2**63], np.int64))
+    pay = [Col(LONG, np.array([10, 11, 12, 13], np.int64))]
+    rows, gp, info = gpu_join(bk, pay, pk, 2, 2, strategy=1)
+    want, _ = oracle_set(bk, pay, pk, 2, 2)
+    assert as_set(rows, gp) == want
+    assert info["max_dup"] == 2
+
+
+def test_output_full_reports_needed_size():
+    bk = Col(LONG, np.array([1, 1, 1, 2], np.int64))
+    pk = Col(LONG, np.array([1, 2, 1], np.int64))
+    j = capi.HashJoin(LONG, [LONG])
+    kd = capi.DeviceColumn(LONG, bk.values)
+    pd = capi.DeviceColumn(LONG, np.arange(4, dtype=np.int64))
+    j.add_build(kd.c(), [pd.c()], 4)
+    j.finish()
+    pcol = capi.DeviceColumn(LONG, pk.values)
+    with pytest.raises(capi.GpuError) as e:
+        j.probe(pcol.c(), 3, cap=3)
+    assert e.value.code == -5
+    r, p = j.probe(pcol.c(), 3, cap=7)
+    assert len(r) == 7
+    j.close()
+
+
+def test_probe_before_finish_is_an_error():
+    j = capi.HashJoin(LONG, [])
+    pcol = capi.DeviceColumn(LONG, np.array([1], np.int64))
+    with pytest.raises(capi.GpuError) as e:
+        j.probe(pcol.c(), 1, cap=1)
+    assert e.value.code == -6
+    j.close()
+
+
+@pytest.mark.parametrize("dense", [False, True])
+def test_inner_join_config_full_size(dense):
+    """Build 2M unique int64 keys + int32 payload, probe 64M keys (80% hits) in batches of 65,536, all
+    batches in one launch.  Size-independent checks: exact match count, every hit probe row exactly once,
+    every payload equal to its build row's payload; plus bit-exact parity with the oracle on a 1M-row slice."""
+    w = workloads.join_inner(dense=dense)
+    bk = Col(LONG, w.build_keys)
+    j = capi.HashJoin(LONG, [INT])
+    keep = []
+    for a, b in workloads.batches(len(w.build_keys), w.batch_rows):
+        kd, pd = capi.DeviceColumn(LONG, w.build_keys[a:b]), capi.DeviceColumn(INT, w.build_payload[a:b])
+        keep.append((kd, pd))
+        j.add_build(kd.c(), [pd.c()], b - a)
+    j.finish()
+    info = j.info()
+    assert info["is_array"] == dense and info["max_dup"] == 1
+    probe = capi.DeviceColumn(LONG, w.probe_keys)
+    spans = workloads.batches(len(w.probe_keys), w.batch_rows)
+    cols = [probe.c(a) for a, _ in spans]
+    cap = len(w.probe_keys)
+    out_rows, pays, oarr = j._outs(cap, False)
+    m = j.probe_batches_raw(cols, [b - a for a, b in spans], out_rows, oarr, cap)
+    assert m == w.hits
+    rows = out_rows.numpy(m)
+    pay = pays[0][0].numpy(m)
+    order = np.argsort(w.build_keys, kind="stable")
+    pos = order[np.searchsorted(w.build_keys[order], w.probe_keys[rows])]
+    assert np.array_equal(w.build_keys[pos], w.probe_keys[rows])
+    assert np.array_equal(pay, w.build_payload[pos])
+    hit_rows = np.sort(rows)
+    assert np.all(np.diff(hit_rows) > 0)  # each probe row at most once (unique build keys)
+    j.close()
+    # bit-exact against the oracle on a slice
+    sl = slice(0, 1_000_000)
+    r = oracle.hash_join(bk, [Col(INT, w.build_payload)], Col(LONG, w.probe_keys[sl]), w.batch_rows, w.batch_rows)
+    sel = rows < 1_000_000
+    assert sorted(zip(rows[sel].tolist(), pay[sel].tolist())) == sorted(zip(r.probe_rows.tolist(),
+                                                                             r.payloads[0].values.tolist()))

@@ -1,0 +1,156 @@
+"""Pins the CPU oracle's hash aggregation: the reference's own fixtures, closed forms, and an
+independent sort-based brute force (oracle.brute) on seeded random inputs."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import brute
+from oracle import (AVG, COUNT_ALL, COUNT_COLUMN, DATE32, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, MIN, SHORT,
+                    SUM, TIMESTAMP, Col)
+from tests.helpers import assert_row_sets_equal, load_golden, rows_of
+
+FIX = load_golden("operator_fixtures.json")
+TYPES = {"SHORT": SHORT, "INT": INT, "DATE32": DATE32, "LONG": LONG, "TIMESTAMP": TIMESTAMP, "DECIMAL64": DECIMAL64}
+STD_AGGS = [(SUM, LONG, LONG), (AVG, LONG, DOUBLE), (COUNT_COLUMN, LONG, LONG), (MIN, LONG, LONG), (MAX, LONG, LONG)]
+
+
+def cols_rows(outs):
+    return rows_of([(c.type_id, c.values, c.nulls) for c in outs])
+
+
+def test_verify_correctness_fixture():
+    f = FIX["agg_verify_correctness"]
+    n = f["batches"] * f["rows_per_batch"]
+    j = np.tile(np.arange(f["rows_per_batch"], dtype=np.int64) % f["cardinality"], f["batches"])
+    keys = [Col(LONG, j), Col(LONG, j.copy())]
+    aggs = [Col(LONG, np.ones(n, np.int64)) for _ in range(5)]
+    spec = oracle.make_agg_spec([LONG, LONG], STD_AGGS)
+    out = oracle.hash_agg(spec, keys, aggs, f["rows_per_batch"])
+    e = f["expected"]
+    want = [(k, k, e["sum"], e["avg"], e["count"], e["min"], e["max"]) for k in e["keys"]]
+    assert_row_sets_equal(cols_rows(out), want, float_cols=[3])
+
+
+@pytest.mark.parametrize("key_type", ["SHORT", "INT", "DATE32", "LONG", "TIMESTAMP", "DECIMAL64"])
+def test_null_key_fixture(key_type):
+    f = FIX["agg_null_keys"]
+    t = TYPES[key_type]
+    r = f["rows_per_batch"]
+    keys, knull = [], []
+    aggs_v = [[] for _ in range(5)]
+    aggs_n = [[] for _ in range(5)]
+    for b in range(2):
+        k = np.arange(r) % f["cardinality"]
+        nl = np.zeros(r, bool)
+        nl[f["key_null_rows"][b]] = True
+        keys.append(k)
+        knull.append(nl)
+        for a in range(5):
+            aggs_v[a].append(np.ones(r, np.int64))
+            m = np.zeros(r, bool)
+            m[f["agg_null_rows"][str(a + 1)][b]] = True
+            aggs_n[a].append(m)
+    kc = Col(t, np.concatenate(keys).astype(oracle.np_dtype(t)), np.concatenate(knull))
+    ac = [Col(LONG, np.concatenate(aggs_v[a]), np.concatenate(aggs_n[a])) for a in range(5)]
+    spec = oracle.make_agg_spec([t], STD_AGGS)
+    out = oracle.hash_agg(spec, [kc], ac, r)
+    assert_row_sets_equal(cols_rows(out), [tuple(x) for x in f["expected_rows"]], float_cols=[2])
+    # the brute force agrees on the same input
+    assert_row_sets_equal(cols_rows(brute.group_by([kc], STD_AGGS, ac)), [tuple(x) for x in f["expected_rows"]],
+                          float_cols=[2])
+
+
+@pytest.mark.parametrize("case", range(4))
+def test_sum_long_wraps(case):
+    c = FIX["sum_long"]["cases"][case]
+    v = np.array(c["values"], np.int64)
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, LONG, LONG)]), [Col(LONG, np.zeros(len(v), np.int64))],
+                          [Col(LONG, v)], 1024)
+    assert int(out[1].values[0]) == c["sum"]
+
+
+def test_sum_decimal64_to_decimal128_fixture():
+    f = FIX["sum_decimal64"]
+    v = np.array(f["values"], np.int64)
+    spec = oracle.make_agg_spec([LONG], [(SUM, DECIMAL64, DECIMAL128, f["in_scale"], f["out_scale"])])
+    out = oracle.hash_agg(spec, [Col(LONG, np.zeros(3, np.int64))], [Col(DECIMAL64, v, scale=f["in_scale"])], 1024)
+    got = rows_of([(c.type_id, c.values, c.nulls) for c in out])
+    assert got == [(0, f["sum"])]
+    out1 = oracle.hash_agg(spec, [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL64, v[:1], scale=6)], 1024)
+    assert rows_of([(c.type_id, c.values, c.nulls) for c in out1]) == [(0, f["sum_first_row"])]
+
+
+def test_decimal_overflow_semantics():
+    """SUM(decimal64 -> decimal64): a row-order partial sum leaving int64 makes the group NULL (or raises)."""
+    v = np.array([2**62, 2**62, -2**62, 5, 7], np.int64)
+    k = np.array([0, 0, 0, 1, 1], np.int64)
+    aggs = [(SUM, DECIMAL64, DECIMAL64, 0, 0)]
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], aggs, overflow_as_null=True), [Col(LONG, k)],
+                          [Col(DECIMAL64, v)], 1024)
+    assert sorted(cols_rows(out), key=lambda r: r[0]) == [(0, None), (1, 12)]
+    with pytest.raises(OverflowError):
+        oracle.hash_agg(oracle.make_agg_spec([LONG], aggs, overflow_as_null=False), [Col(LONG, k)],
+                        [Col(DECIMAL64, v)], 1024)
+    # scale-up overflow at extraction: 10^18 * 10^2 does not fit a decimal64
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, DECIMAL64, DECIMAL64, 0, 2)]),
+                          [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL64, np.array([10**18], np.int64))], 16)
+    assert cols_rows(out) == [(0, None)]
+
+
+def _random_case(seed, n, key_kind, with_nulls):
+    rng = np.random.default_rng(seed)
+    if key_kind == "long":
+        keys = [Col(LONG, rng.integers(-50, 50, n, dtype=np.int64))]
+    elif key_kind == "int":
+        keys = [Col(INT, rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32) % 97)]
+    elif key_kind == "composite":
+        keys = [Col(INT, rng.integers(0, 7, n, dtype=np.int64).astype(np.int32)),
+                Col(LONG, rng.integers(-2**63, 2**63 - 1, 13, dtype=np.int64)[rng.integers(0, 13, n)])]
+    else:  # double key (serialize path, bit-pattern equality)
+        keys = [Col(DOUBLE, rng.integers(0, 20, n).astype(np.float64) / 4)]
+    if with_nulls:
+        for c in keys:
+            c.nulls = rng.random(n) < 0.1
+    vals = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    i32 = rng.integers(-2**31, 2**31, n, dtype=np.int64).astype(np.int32)
+    dec = rng.integers(-10**12, 10**12, n, dtype=np.int64)
+    dbl = rng.normal(100, 50, n)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (COUNT_COLUMN, LONG, LONG), (MIN, LONG, LONG),
+            (MAX, LONG, LONG), (MAX, INT, INT), (MIN, INT, INT), (SUM, DECIMAL64, DECIMAL128, 2, 2),
+            (SUM, DECIMAL64, DECIMAL64, 2, 3), (SUM, DOUBLE, DOUBLE), (AVG, DOUBLE, DOUBLE), (AVG, LONG, DOUBLE),
+            (SUM, INT, LONG)]
+    nl = (lambda: rng.random(n) < 0.2) if with_nulls else (lambda: None)
+    cols = [Col(LONG, vals, nl()), None, Col(LONG, vals, nl()), Col(LONG, vals, nl()), Col(LONG, vals, nl()),
+            Col(INT, i32, nl()), Col(INT, i32, nl()), Col(DECIMAL64, dec, nl(), 2), Col(DECIMAL64, dec, nl(), 2),
+            Col(DOUBLE, dbl, nl()), Col(DOUBLE, dbl, nl()), Col(LONG, rng.integers(-1000, 1000, n, dtype=np.int64), nl()),
+            Col(INT, i32, nl())]
+    return keys, aggs, cols
+
+
+@pytest.mark.parametrize("key_kind", ["long", "int", "composite", "double"])
+@pytest.mark.parametrize("with_nulls", [False, True])
+def test_oracle_matches_brute_force(key_kind, with_nulls):
+    keys, aggs, cols = _random_case(hash((key_kind, with_nulls)) % 2**32, 5000, key_kind, with_nulls)
+    out = oracle.hash_agg(oracle.make_agg_spec([c.type_id for c in keys], aggs), keys, cols, 777)
+    ref = brute.group_by(keys, aggs, cols)
+    nk = len(keys)
+    float_cols = [nk + i for i, a in enumerate(aggs) if a[2] == DOUBLE]
+    assert_row_sets_equal(cols_rows(out), cols_rows(ref), float_cols=float_cols, rtol=1e-9)
+
+
+def test_swiss_map_growth_and_null_slot():
+    """BaseHashMap: degree 15 start, rehash once size exceeds ceil(0.9 * cap) before an insert, +2 degrees
+    below 23 (base_hash_map.h:255-292, 528-547); the NULL key is a separate slot counted in the size."""
+    m = oracle.SwissMap(15)
+    assert m.capacity() == 2**15
+    thr = int(np.ceil(2**15 * 0.9))
+    for i in range(thr + 1):
+        assert m.emplace(i * 7919, i)
+    assert m.capacity() == 2**15
+    assert m.emplace(-1, 10**6)
+    assert m.capacity() == 2**17
+    assert not m.emplace(7919 * 5, 0)
+    assert m.emplace(None, 123) and not m.emplace(None, 5)
+    assert m.size() == thr + 3
+    vals = m.values_in_order()
+    assert vals[-1] == 123 and sorted(vals[:-1].tolist()) == sorted(list(range(thr + 1)) + [10**6])
