@@ -132,6 +132,8 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle);
  * valid until the next synchronizing call on the handle. */
 int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
                            int64_t rows);
+/* Blocks until every batch added so far has been consumed (its columns may then be reused). */
+int omni_gpu_agg_wait(void *handle);
 /* Number of groups so far, NULL-key group included (HashAggregationOperator::GetHashMapUniqueKeys,
  * group_aggregation.cpp:1168-1180).  Synchronizes the handle's stream. */
 int omni_gpu_agg_num_groups(void *handle, int64_t *groups);
@@ -141,6 +143,33 @@ int omni_gpu_agg_num_groups(void *handle, int64_t *groups);
  * overflow_as_null == 0. */
 int omni_gpu_agg_get_output(void *handle, int64_t start, int64_t count, omni_gpu_out_column *outs);
 int omni_gpu_agg_destroy(void *handle);
+
+/* ---- batched stream of independent group-bys ----------------------------------------------------------
+ * Replaces one HashAggregationOperator per VectorBatch (AddInput + GetOutput per batch,
+ * group_aggregation.cpp:320-406, 454-474) for a stream of many small, uneven batches: the whole stream is
+ * aggregated by ONE persistent launch that drains a device queue of batches (each batch in an LDS table,
+ * spilling exactly to a per-workgroup scratch table in HBM when its keys do not fit).  No reference
+ * counterpart exists for the batching itself; per-batch results equal the reference operator's.
+ * Single key column of <= 8 bytes; any aggregate the per-batch path supports. */
+int omni_gpu_agg_stream_create(const omni_gpu_agg_spec *spec, int64_t max_batch_rows, void **handle);
+/* Launch tuning (the autotuner's knobs): threads per workgroup (256 or 512), LDS table bytes
+ * ([8 KiB, 64 KiB]), LDS probe limit before spilling, resident workgroups per CU (0 = occupancy limit). */
+int omni_gpu_agg_stream_tune(void *handle, int32_t block_threads, int32_t lds_bytes, int32_t max_probe,
+                             int32_t wgs_per_cu);
+int omni_gpu_agg_stream_info(void *handle, int32_t *lds_slots, int32_t *workgroups, int64_t *scratch_bytes);
+/* Runs the stream.  keys[0] / aggs[i] are the concatenated device columns of all batches; batch b owns rows
+ * [batch_offsets[b], batch_offsets[b+1]) (host array, batch_offsets[0] == 0).  batch_modes (host, optional):
+ * 0 = LDS table first, 2 = scratch table directly (e.g. from a predicted group count).  flags: bit 0 = one
+ * launch per batch instead of the persistent launch; bit 1 = capture the steady state in a hipGraph and
+ * replay it while pointers and sizes stay the same.  Batch b's groups are written to output rows
+ * [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key, then aggregates); outs hold
+ * batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.  Synchronizes. */
+int omni_gpu_agg_stream_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
+                            const int64_t *batch_offsets, int32_t num_batches, const int32_t *batch_modes,
+                            int32_t flags, omni_gpu_out_column *outs, int64_t *groups_per_batch);
+/* Batches that stayed in LDS / used the scratch table since the last call. */
+int omni_gpu_agg_stream_stats(void *handle, int64_t *lds_batches, int64_t *scratch_batches);
+int omni_gpu_agg_stream_destroy(void *handle);
 
 /* ---- hash join (inner) --------------------------------------------------------------------------
  * Replaces HashBuilderOperator + JoinHashTableVariants (core/src/operator/join/hash_builder.cpp:169-213,
@@ -169,8 +198,9 @@ int omni_gpu_join_build_info(void *handle, int32_t *is_array, int64_t *max_dup, 
  * payload column p, the build row's payload into out_payloads[p].values[k] (and its NULL bit).
  * Output order is unspecified (compare as a row set).  *num_matches receives the match count;
  * if it exceeds out_capacity, nothing beyond out_capacity is written and OMNI_GPU_ERR_OUTPUT_FULL
- * is returned.  A payload output's `nulls` may be NULL when that build column has no NULLs; otherwise
- * it must hold 4 * ceil(out_capacity / 32) bytes (it is cleared and written as 32-bit words).
+ * is returned.  A payload output whose `values` is NULL is skipped.  A payload output's `nulls` may
+ * be NULL when that build column has no NULLs; otherwise it must hold 4 * ceil(out_capacity / 32)
+ * bytes (it is cleared and written as 32-bit words).
  * Synchronizes. */
 int omni_gpu_join_probe(void *handle, const omni_gpu_column *probe_key, int64_t rows,
                         int64_t probe_row_base, int64_t *out_probe_rows, omni_gpu_out_column *out_payloads,

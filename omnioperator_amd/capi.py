@@ -76,9 +76,17 @@ SIGNATURES = {
     "omni_gpu_profile_get": (_I, [C.c_char_p, _P(_I64), _P(_D)]),
     "omni_gpu_agg_create": (_I, [_P(AggSpec), _P(_VP)]),
     "omni_gpu_agg_add_batch": (_I, [_VP, _P(Column), _P(Column), _I64]),
+    "omni_gpu_agg_wait": (_I, [_VP]),
     "omni_gpu_agg_num_groups": (_I, [_VP, _P(_I64)]),
     "omni_gpu_agg_get_output": (_I, [_VP, _I64, _I64, _P(OutColumn)]),
     "omni_gpu_agg_destroy": (_I, [_VP]),
+    "omni_gpu_agg_stream_create": (_I, [_P(AggSpec), _I64, _P(_VP)]),
+    "omni_gpu_agg_stream_tune": (_I, [_VP, _I32, _I32, _I32, _I32]),
+    "omni_gpu_agg_stream_info": (_I, [_VP, _P(_I32), _P(_I32), _P(_I64)]),
+    "omni_gpu_agg_stream_run": (_I, [_VP, _P(Column), _P(Column), _P(_I64), _I32, _VP, _I32, _P(OutColumn),
+                                     _P(_I64)]),
+    "omni_gpu_agg_stream_stats": (_I, [_VP, _P(_I64), _P(_I64)]),
+    "omni_gpu_agg_stream_destroy": (_I, [_VP]),
     "omni_gpu_join_build_create": (_I, [_P(JoinSpec), _P(_VP)]),
     "omni_gpu_join_build_add": (_I, [_VP, _P(Column), _P(Column), _I64]),
     "omni_gpu_join_build_finish": (_I, [_VP]),
@@ -256,6 +264,83 @@ class HashAgg:
     def close(self):
         if getattr(self, "h", None):
             lib().omni_gpu_agg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class AggStream:
+    """Batched stream of independent group-bys (omni_gpu_agg_stream_*)."""
+
+    FLAG_PER_BATCH, FLAG_GRAPH = 1, 2
+
+    def __init__(self, spec: AggSpec, max_batch_rows: int):
+        init()
+        self.spec = spec
+        h = C.c_void_p()
+        check(lib().omni_gpu_agg_stream_create(C.byref(spec), max_batch_rows, C.byref(h)))
+        self.h = h.value
+        self._outs = None
+
+    def tune(self, block: int = 256, lds_bytes: int = 32768, max_probe: int = 32, wgs_per_cu: int = 0):
+        check(lib().omni_gpu_agg_stream_tune(self.h, block, lds_bytes, max_probe, wgs_per_cu))
+
+    def info(self):
+        s, w, b = C.c_int32(), C.c_int32(), C.c_int64()
+        check(lib().omni_gpu_agg_stream_info(self.h, C.byref(s), C.byref(w), C.byref(b)))
+        return {"lds_slots": s.value, "workgroups": w.value, "scratch_bytes": b.value}
+
+    def alloc_outputs(self, total_rows: int):
+        """Device output columns (key, then one per aggregate) for a stream of total_rows rows."""
+        types = [self.spec.key_types[0]] + [self.spec.agg_out_types[i] for i in range(self.spec.num_aggs)]
+        bufs, oarr = [], (OutColumn * len(types))()
+        for i, t in enumerate(types):
+            shape = (total_rows, 2) if t == DECIMAL128 else (total_rows,)
+            v = DeviceArray(max(1, total_rows) * WIDTH[t], NP[t], shape)
+            nb = DeviceArray(max(4, ((total_rows + 31) // 32) * 4), np.uint8)
+            bufs.append((v, nb))
+            oarr[i].type_id, oarr[i].values, oarr[i].nulls = t, v.ptr, nb.ptr
+        self._outs = (types, bufs, oarr, total_rows)
+        return self._outs
+
+    def run(self, keys: Column, aggs: Sequence[Optional[Column]], offsets: np.ndarray, modes=None, flags: int = 0):
+        """Runs every batch; returns groups per batch (numpy int64).  Outputs stay on the device (see outputs())."""
+        offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        nb = len(offsets) - 1
+        total = int(offsets[-1])
+        if self._outs is None or self._outs[3] != total:
+            self.alloc_outputs(total)
+        aarr = (Column * MAX_AGGS)()
+        for i, a in enumerate(aggs):
+            if a is not None:
+                aarr[i] = a
+        groups = np.zeros(nb, np.int64)
+        mp = None
+        if modes is not None:
+            modes = np.ascontiguousarray(modes, dtype=np.int32)
+            mp = modes.ctypes.data
+        karr = (Column * 1)(keys)
+        check(lib().omni_gpu_agg_stream_run(self.h, karr, aarr, offsets.ctypes.data_as(C.POINTER(C.c_int64)), nb, mp,
+                                            flags, self._outs[2], groups.ctypes.data_as(C.POINTER(C.c_int64))))
+        return groups
+
+    def outputs(self):
+        """[(values, null mask)] of the whole output buffer (rows beyond each batch's group count are junk)."""
+        types, bufs, _, total = self._outs
+        return [(v.numpy(), unpack_nulls(nb.numpy(), total)) for v, nb in bufs]
+
+    def stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        check(lib().omni_gpu_agg_stream_stats(self.h, C.byref(a), C.byref(b)))
+        return {"lds_batches": a.value, "scratch_batches": b.value}
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().omni_gpu_agg_stream_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -22,298 +22,9 @@
 #include <memory>
 #include <vector>
 
-#include "omni_gpu_internal.h"
+#include "agg_device.h"
 
 namespace omni_gpu {
-
-enum AggKind : int {
-    K_SUM_I64 = 0,
-    K_SUM_F64,
-    K_SUM_DEC,
-    K_COUNT_ALL,
-    K_COUNT_COL,
-    K_AVG,
-    K_MIN_I64,
-    K_MAX_I64,
-    K_MIN_I32,
-    K_MAX_I32,
-    K_MIN_F64,
-    K_MAX_F64,
-};
-
-struct AggDev {
-    int kind;
-    int in_width;     // bytes per input value
-    int in_double;    // input column holds doubles
-    int out_width;    // bytes per output value
-    const void *values;
-    const uint8_t *nulls;
-    void *s0;
-    void *s1;
-    void *s2;
-    uint8_t *flag;    // 1 once a non-NULL value was added
-    int dec_scale_up; // 10^(outScale - inScale) exponent for decimal SUM
-    int dec_out128;   // decimal SUM output is DECIMAL128
-};
-
-struct AggArgs {
-    int n;
-    AggDev a[OMNI_GPU_MAX_AGGS];
-};
-
-struct KeyDev {
-    int num;
-    int W;                      // words per key (multi-word path)
-    int width[OMNI_GPU_MAX_KEYS];
-    int word[OMNI_GPU_MAX_KEYS];   // first word index of each column
-    int in_word0[OMNI_GPU_MAX_KEYS];  // column stored in the high half of word 0
-    const void *values[OMNI_GPU_MAX_KEYS];
-    const uint8_t *nulls[OMNI_GPU_MAX_KEYS];
-};
-
-struct Slot1 {
-    unsigned long long key;
-    unsigned int gid1;
-    unsigned int pad;
-};
-
-struct GroupCtx {
-    unsigned long long *ngroups;   // group counter
-    unsigned long long max_groups; // capacity of the per-group arrays
-    unsigned int *overflow;        // set when a group id would exceed max_groups (host sizing bug guard)
-    unsigned long long *gkeys;
-    uint8_t *gnull;
-    unsigned int *special;         // [0] NULL group, [1] sentinel-key group (0 empty, 1 busy, gid+2)
-};
-
-constexpr int kMaxWords = 6;
-
-__device__ __forceinline__ unsigned int LoadAcquire(unsigned int *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned long long LoadRelaxed64(unsigned long long *p)
-{
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void StoreRelease(unsigned int *p, unsigned int v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Claims a fresh group id; returns ~0 if the per-group arrays are full (never expected: the host sizes them).
-__device__ __forceinline__ unsigned int NewGroup(const GroupCtx &g)
-{
-    unsigned long long id = WaveReserveOne(g.ngroups, true);
-    if (id >= g.max_groups) {
-        atomicOr(g.overflow, 1u);
-        return 0xFFFFFFFFu;
-    }
-    return static_cast<unsigned int>(id);
-}
-
-// Group of a side cell (NULL key, or the key equal to the slot sentinel), created on first use.
-__device__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned long long key, bool isNull, int W)
-{
-    unsigned int *cell = g.special + which;
-    while (true) {
-        unsigned int c = LoadAcquire(cell);
-        if (c >= 2) return c - 2;
-        if (c == 0 && atomicCAS(cell, 0u, 1u) == 0u) {
-            unsigned int gid = NewGroup(g);
-            if (gid == 0xFFFFFFFFu) {
-                StoreRelease(cell, 0u);
-                return gid;
-            }
-            g.gkeys[static_cast<size_t>(gid) * W] = key;
-            if (g.gnull) g.gnull[gid] = isNull ? 1 : 0;
-            StoreRelease(cell, gid + 2);
-            return gid;
-        }
-    }
-}
-
-// Find-or-insert of a single-word key; returns its group id.
-__device__ unsigned int InsertSingle(Slot1 *slots, unsigned long long mask, unsigned long long key,
-                                     const GroupCtx &g)
-{
-    unsigned long long pos = Mix64(key) & mask;
-    while (true) {
-        unsigned long long k = slots[pos].key;
-        if (k == kEmptyKey) {
-            k = atomicCAS(&slots[pos].key, kEmptyKey, key);
-            if (k == kEmptyKey) {
-                unsigned int gid = NewGroup(g);
-                if (gid == 0xFFFFFFFFu) return gid;
-                g.gkeys[gid] = key;
-                if (g.gnull) g.gnull[gid] = 0;
-                StoreRelease(&slots[pos].gid1, gid + 1);
-                return gid;
-            }
-        }
-        if (k == key) {
-            unsigned int g1 = LoadAcquire(&slots[pos].gid1);
-            if (g1) return g1 - 1;
-            continue;  // the winning lane publishes its group id; re-read the same slot
-        }
-        pos = (pos + 1) & mask;
-    }
-}
-
-// Find-or-insert of a W-word key (ctrl-word protocol).
-__device__ unsigned int InsertMulti(unsigned int *ctrl, unsigned long long *skeys, unsigned long long mask,
-                                    const unsigned long long *words, int W, unsigned long long hash,
-                                    const GroupCtx &g)
-{
-    unsigned long long pos = hash & mask;
-    while (true) {
-        unsigned int c = LoadAcquire(&ctrl[pos]);
-        if (c == 0) {
-            c = atomicCAS(&ctrl[pos], 0u, 1u);
-            if (c == 0) {
-                unsigned int gid = NewGroup(g);
-                if (gid == 0xFFFFFFFFu) {
-                    StoreRelease(&ctrl[pos], 0u);
-                    return gid;
-                }
-                for (int w = 0; w < W; ++w) {
-                    skeys[pos * W + w] = words[w];
-                    g.gkeys[static_cast<size_t>(gid) * W + w] = words[w];
-                }
-                StoreRelease(&ctrl[pos], gid + 2);
-                return gid;
-            }
-        }
-        if (c == 1) continue;  // another lane is writing this slot's key
-        bool same = true;
-        for (int w = 0; w < W; ++w) same &= LoadRelaxed64(&skeys[pos * W + w]) == words[w];
-        if (same) return c - 2;
-        pos = (pos + 1) & mask;
-    }
-}
-
-__device__ __forceinline__ void EncodeKey(const KeyDev &kd, int64_t row, unsigned long long *words)
-{
-    for (int w = 0; w < kd.W; ++w) words[w] = 0;
-    for (int c = 0; c < kd.num; ++c) {
-        if (NullBit(kd.nulls[c], row)) {
-            words[0] |= 1ULL << c;
-            continue;
-        }
-        if (kd.width[c] == 16) {
-            const unsigned long long *v = static_cast<const unsigned long long *>(kd.values[c]);
-            words[kd.word[c]] = v[2 * row];
-            words[kd.word[c] + 1] = v[2 * row + 1];
-        } else if (kd.in_word0[c]) {
-            words[0] |= (LoadWord(kd.values[c], kd.width[c], row) & 0xFFFFFFFFULL) << 32;
-        } else {
-            words[kd.word[c]] = LoadWord(kd.values[c], kd.width[c], row);
-        }
-    }
-}
-
-__device__ __forceinline__ unsigned long long HashWords(const unsigned long long *words, int W)
-{
-    unsigned long long h = 0x9E3779B97F4A7C15ULL;
-    for (int w = 0; w < W; ++w) h = Mix64(h ^ (words[w] + 0x632BE59BD9B4E019ULL * (w + 1)));
-    return h;
-}
-
-// 192-bit exact accumulation of a signed 128-bit addend (lo, hi) into (s0 = lo, s1 = hi, s2 = top).
-__device__ __forceinline__ void AddDec(unsigned long long *lo, unsigned long long *hi, long long *top,
-                                       unsigned long long vlo, long long vhi)
-{
-    unsigned long long old = atomicAdd(lo, vlo);
-    unsigned long long carry = (old + vlo) < old ? 1ULL : 0ULL;
-    // add (vhi + carry) to hi, tracking the signed carry-out into top
-    long long parts[2] = {vhi, static_cast<long long>(carry)};
-    if (vhi == 0x7FFFFFFFFFFFFFFFLL && carry) {
-        // vhi + 1 does not fit in 64 bits: add the two parts separately
-    } else {
-        parts[0] = vhi + static_cast<long long>(carry);
-        parts[1] = 0;
-    }
-    for (int p = 0; p < 2; ++p) {
-        long long a = parts[p];
-        if (a == 0) continue;
-        unsigned long long ua = static_cast<unsigned long long>(a);
-        unsigned long long oh = atomicAdd(hi, ua);
-        long long k = ((oh + ua) < oh ? 1 : 0) - (a < 0 ? 1 : 0);
-        if (k) atomicAdd(reinterpret_cast<unsigned long long *>(top), static_cast<unsigned long long>(k));
-    }
-}
-
-__device__ __forceinline__ void UpdateAggs(const AggArgs &args, unsigned int gid, int64_t row)
-{
-    for (int i = 0; i < args.n; ++i) {
-        const AggDev &d = args.a[i];
-        if (d.kind == K_COUNT_ALL) {
-            atomicAdd(static_cast<unsigned long long *>(d.s0) + gid, 1ULL);
-            continue;
-        }
-        if (NullBit(d.nulls, row)) continue;
-        switch (d.kind) {
-            case K_COUNT_COL: atomicAdd(static_cast<unsigned long long *>(d.s0) + gid, 1ULL); break;
-            case K_SUM_I64:
-                atomicAdd(static_cast<unsigned long long *>(d.s0) + gid, LoadWord(d.values, d.in_width, row));
-                d.flag[gid] = 1;
-                break;
-            case K_SUM_F64:
-                atomicAdd(static_cast<double *>(d.s0) + gid, static_cast<const double *>(d.values)[row]);
-                d.flag[gid] = 1;
-                break;
-            case K_AVG: {
-                double v = d.in_double ? static_cast<const double *>(d.values)[row]
-                                       : static_cast<double>(static_cast<long long>(LoadWord(d.values, d.in_width, row)));
-                atomicAdd(static_cast<double *>(d.s0) + gid, v);
-                atomicAdd(static_cast<unsigned long long *>(d.s1) + gid, 1ULL);
-                break;
-            }
-            case K_SUM_DEC: {
-                unsigned long long vlo;
-                long long vhi;
-                if (d.in_width == 16) {
-                    const unsigned long long *v = static_cast<const unsigned long long *>(d.values);
-                    vlo = v[2 * row];
-                    vhi = static_cast<long long>(v[2 * row + 1]);
-                } else {
-                    long long v = static_cast<long long>(LoadWord(d.values, d.in_width, row));
-                    vlo = static_cast<unsigned long long>(v);
-                    vhi = v < 0 ? -1 : 0;
-                }
-                AddDec(static_cast<unsigned long long *>(d.s0) + gid, static_cast<unsigned long long *>(d.s1) + gid,
-                       static_cast<long long *>(d.s2) + gid, vlo, vhi);
-                d.flag[gid] = 1;
-                break;
-            }
-            case K_MIN_I64:
-                atomicMin(static_cast<long long *>(d.s0) + gid, static_cast<long long>(LoadWord(d.values, d.in_width, row)));
-                d.flag[gid] = 1;
-                break;
-            case K_MAX_I64:
-                atomicMax(static_cast<long long *>(d.s0) + gid, static_cast<long long>(LoadWord(d.values, d.in_width, row)));
-                d.flag[gid] = 1;
-                break;
-            case K_MIN_I32:
-                atomicMin(static_cast<int *>(d.s0) + gid, static_cast<int>(LoadWord(d.values, d.in_width, row)));
-                d.flag[gid] = 1;
-                break;
-            case K_MAX_I32:
-                atomicMax(static_cast<int *>(d.s0) + gid, static_cast<int>(LoadWord(d.values, d.in_width, row)));
-                d.flag[gid] = 1;
-                break;
-            case K_MIN_F64:
-                atomicMin(static_cast<double *>(d.s0) + gid, static_cast<const double *>(d.values)[row]);
-                d.flag[gid] = 1;
-                break;
-            case K_MAX_F64:
-                atomicMax(static_cast<double *>(d.s0) + gid, static_cast<const double *>(d.values)[row]);
-                d.flag[gid] = 1;
-                break;
-            default: break;
-        }
-    }
-}
 
 // ---- kernels ------------------------------------------------------------------------------------------------
 
@@ -344,6 +55,79 @@ __global__ void __launch_bounds__(256) agg_insert_multi(KeyDev kd, int64_t rows,
     unsigned int gid = InsertMulti(ctrl, skeys, mask, words, kd.W, HashWords(words, kd.W), g);
     if (gid == 0xFFFFFFFFu) return;
     UpdateAggs(args, gid, row);
+}
+
+constexpr int kLdsBlock = 1024;
+constexpr int kLdsMaxProbe = 32;
+
+// LDS-local partial aggregation (single-word keys).  Each workgroup aggregates a contiguous chunk of rows
+// in a private LDS table of L.S slots (linear probing, 64-bit LDS CAS), then merges every occupied slot into
+// the global table with one global insert and one atomic per aggregate.  Rows whose key is NULL or the
+// slot sentinel, or that find no LDS slot within kLdsMaxProbe steps, go straight to the global table.
+__global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t rows, int64_t rows_per_block,
+                                                             Slot1 *slots, unsigned long long mask, GroupCtx g,
+                                                             AggArgs args, LdsLayout L,
+                                                             unsigned long long *spill_counter)
+{
+    extern __shared__ unsigned long long lds_words[];
+    uint8_t *base = reinterpret_cast<uint8_t *>(lds_words);
+    unsigned long long *lkeys = reinterpret_cast<unsigned long long *>(base + L.keys_off);
+    const int S = L.S;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        lkeys[i] = kEmptyKey;
+        for (int a = 0; a < args.n; ++a) {
+            int kind = args.a[a].kind;
+            if (Is32(kind)) {
+                reinterpret_cast<int *>(base + L.off[a][0])[i] = static_cast<int>(InitWord(kind));
+            } else {
+                reinterpret_cast<unsigned long long *>(base + L.off[a][0])[i] = InitWord(kind);
+            }
+            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = 0;
+            if (L.off[a][2] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][2])[i] = 0;
+            if (L.flag_off[a] >= 0) base[L.flag_off[a] + i] = 0;
+        }
+    }
+    __syncthreads();
+    int64_t start = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    int64_t end = start + rows_per_block < rows ? start + rows_per_block : rows;
+    unsigned int spilled = 0;
+    for (int64_t row = start + threadIdx.x; row < end; row += blockDim.x) {
+        bool isNull = NullBit(kd.nulls[0], row);
+        unsigned long long key = isNull ? 0ULL : LoadWord(kd.values[0], kd.width[0], row);
+        int slot = -1;
+        if (!isNull && key != kEmptyKey) {
+            unsigned long long h = Mix64(key);
+            unsigned int pos = static_cast<unsigned int>(((h >> 32) * static_cast<unsigned long long>(S)) >> 32);
+            for (int step = 0; step < kLdsMaxProbe; ++step) {
+                unsigned long long k = lkeys[pos];
+                if (k == kEmptyKey) {
+                    k = atomicCAS(&lkeys[pos], kEmptyKey, key);
+                    if (k == kEmptyKey) k = key;
+                }
+                if (k == key) {
+                    slot = static_cast<int>(pos);
+                    break;
+                }
+                pos = pos + 1 == static_cast<unsigned int>(S) ? 0 : pos + 1;
+            }
+        }
+        if (slot >= 0) {
+            UpdateAggsT(args, LdsRef{base, L, slot}, row);
+        } else {
+            ++spilled;
+            unsigned int gid = isNull ? SideGroup(g, 0, 0ULL, true, 1)
+                                      : (key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g));
+            if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        unsigned long long k = lkeys[i];
+        if (k == kEmptyKey) continue;
+        unsigned int gid = InsertSingle(slots, mask, k, g);
+        if (gid != 0xFFFFFFFFu) MergeAggs(args, gid, LdsRef{base, L, i});
+    }
+    if (spilled) atomicAdd(spill_counter, static_cast<unsigned long long>(spilled));
 }
 
 // Rehash after growth: re-inserts groups [0, n) with their existing ids (states do not move).
@@ -393,62 +177,6 @@ __global__ void init_slots1(Slot1 *s, int64_t n)
     }
 }
 
-struct OutDev {
-    int width;
-    void *values;
-    uint8_t *nulls;
-};
-
-struct OutArgs {
-    int nkeys;
-    int naggs;
-    OutDev o[OMNI_GPU_MAX_KEYS + OMNI_GPU_MAX_AGGS];
-};
-
-// Writes the null bitmap of 64 consecutive output rows handled by one wave.
-__device__ __forceinline__ void WriteNullBits(uint8_t *nulls, int64_t r, int64_t count, bool isNull)
-{
-    unsigned long long m = __ballot(isNull);
-    int lane = LaneId();
-    int64_t base = r - lane;  // first row of this wave (waves cover aligned groups of 64 rows)
-    if (lane < 8) {
-        int64_t byte = (base >> 3) + lane;
-        if (byte * 8 < count) nulls[byte] = static_cast<uint8_t>(m >> (8 * lane));
-    }
-}
-
-__device__ __forceinline__ void StoreWidth(void *dst, int width, int64_t r, unsigned long long v)
-{
-    switch (width) {
-        case 8: static_cast<unsigned long long *>(dst)[r] = v; break;
-        case 4: static_cast<unsigned int *>(dst)[r] = static_cast<unsigned int>(v); break;
-        case 2: static_cast<unsigned short *>(dst)[r] = static_cast<unsigned short>(v); break;
-        default: static_cast<uint8_t *>(dst)[r] = static_cast<uint8_t>(v); break;
-    }
-}
-
-// Decimal finalisation: range check of the 192-bit total, then x 10^up with overflow checks
-// (SumSparkDecimalAggregator::ExtractValuesBatchInternal, sum_spark_decimal_aggregator.h:246-301).
-__device__ bool FinishDecimal(unsigned long long lo, unsigned long long hi, long long top, int up, bool out128,
-                              __int128 &res)
-{
-    bool fits128 = (top == 0 && static_cast<long long>(hi) >= 0) || (top == -1 && static_cast<long long>(hi) < 0);
-    if (!fits128) return false;
-    __int128 v = static_cast<__int128>((static_cast<unsigned __int128>(hi) << 64) | lo);
-    const __int128 max38 = (static_cast<__int128>(5421010862427522170LL) << 64) + 687399551400673279LL;
-    const __int128 lim = out128 ? max38 : static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL);
-    const __int128 llim = out128 ? -max38 : -static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL) - 1;
-    if (v > lim || v < llim) return false;
-    for (int i = 0; i < up; ++i) {
-        // v * 10 must stay within [-max38, max38]
-        if (v > max38 / 10 || v < -(max38 / 10)) return false;
-        v = (v << 3) + (v << 1);
-    }
-    if (v > lim || v < llim) return false;
-    res = v;
-    return true;
-}
-
 __global__ void __launch_bounds__(256) agg_output(int64_t start, int64_t count, KeyDev kd, int single,
                                                   const unsigned long long *gkeys, const uint8_t *gnull,
                                                   AggArgs args, OutArgs out, int overflow_as_null,
@@ -494,54 +222,8 @@ __global__ void __launch_bounds__(256) agg_output(int64_t start, int64_t count, 
         OutDev &o = out.o[kd.num + i];
         bool isNull = false;
         if (live) {
-            switch (d.kind) {
-                case K_COUNT_ALL:
-                case K_COUNT_COL:
-                    static_cast<unsigned long long *>(o.values)[r] = static_cast<unsigned long long *>(d.s0)[gid];
-                    break;
-                case K_AVG: {
-                    unsigned long long cnt = static_cast<unsigned long long *>(d.s1)[gid];
-                    isNull = cnt == 0;
-                    static_cast<double *>(o.values)[r] =
-                        isNull ? 0.0 : static_cast<double *>(d.s0)[gid] / static_cast<double>(cnt);
-                    break;
-                }
-                case K_SUM_DEC: {
-                    isNull = d.flag[gid] == 0;
-                    __int128 v = 0;
-                    if (!isNull) {
-                        bool ok = FinishDecimal(static_cast<unsigned long long *>(d.s0)[gid],
-                                                static_cast<unsigned long long *>(d.s1)[gid],
-                                                static_cast<long long *>(d.s2)[gid], d.dec_scale_up, d.dec_out128, v);
-                        if (!ok) {
-                            isNull = true;
-                            v = 0;
-                            if (!overflow_as_null) atomicOr(error_flag, 1u);
-                        }
-                    }
-                    if (d.dec_out128) {
-                        static_cast<unsigned long long *>(o.values)[2 * r] = static_cast<unsigned long long>(v);
-                        static_cast<unsigned long long *>(o.values)[2 * r + 1] =
-                            static_cast<unsigned long long>(static_cast<unsigned __int128>(v) >> 64);
-                    } else {
-                        static_cast<long long *>(o.values)[r] = static_cast<long long>(v);
-                    }
-                    break;
-                }
-                case K_MIN_I32:
-                case K_MAX_I32: {
-                    isNull = d.flag[gid] == 0;
-                    int v = isNull ? 0 : static_cast<int *>(d.s0)[gid];
-                    StoreWidth(o.values, o.width, r, static_cast<unsigned long long>(static_cast<long long>(v)));
-                    break;
-                }
-                default: {  // SUM_I64, SUM_F64, MIN/MAX 64-bit: copy the 8-byte state
-                    isNull = d.flag[gid] == 0;
-                    static_cast<unsigned long long *>(o.values)[r] =
-                        isNull ? 0ULL : static_cast<unsigned long long *>(d.s0)[gid];
-                    break;
-                }
-            }
+            isNull = FinalizeAgg(d, i, GlobalRef{args, static_cast<unsigned int>(gid)}, o, r, overflow_as_null,
+                                 error_flag);
         }
         WriteNullBits(o.nulls, r, count, live && isNull);
     }
@@ -572,14 +254,19 @@ struct AggHandle {
     int64_t max_groups = 0;
     DevBuf slots, ctrl, skeys, gkeys, gnull;
     DevBuf state[OMNI_GPU_MAX_AGGS][4];  // s0, s1, s2, flag
-    DevBuf counters;  // [0] ngroups u64, [8] overflow u32, [12] error u32, [16..24) special u32[2]
+    DevBuf counters;  // [0] ngroups u64, [8] overflow u32, [12] error u32, [16..24) special u32[2], [24] spills u64
     int64_t known_groups = 0;
     int64_t pending_rows = 0;
+    bool use_lds = false;
+    LdsLayout lds{};
+    int64_t lds_rows = 0;    // rows sent through the LDS kernel since the last spill check
+    int64_t spills_seen = 0;
 
     unsigned long long *ngroups() const { return reinterpret_cast<unsigned long long *>(counters.as<uint8_t>()); }
     unsigned int *overflow() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 8); }
     unsigned int *error_flag() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 12); }
     unsigned int *special() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 16); }
+    unsigned long long *spills() const { return reinterpret_cast<unsigned long long *>(counters.as<uint8_t>() + 24); }
 
     GroupCtx Ctx() const
     {
@@ -673,18 +360,8 @@ int InitStates(AggHandle &h, int64_t from, int64_t to)
     if (n <= 0) return OMNI_GPU_OK;
     for (int i = 0; i < h.args.n; ++i) {
         AggDev &d = h.args.a[i];
-        unsigned long long init = 0;
-        bool is32 = d.kind == K_MIN_I32 || d.kind == K_MAX_I32;
-        switch (d.kind) {
-            case K_MIN_I64: init = 0x7FFFFFFFFFFFFFFFULL; break;
-            case K_MAX_I64: init = 0x8000000000000000ULL; break;
-            case K_MIN_I32: init = 0x7FFFFFFFu; break;
-            case K_MAX_I32: init = 0x80000000u; break;
-            case K_MIN_F64: { double x = 1.7976931348623157e308; std::memcpy(&init, &x, 8); break; }
-            case K_MAX_F64: { double x = -1.7976931348623157e308; std::memcpy(&init, &x, 8); break; }
-            default: init = 0;
-        }
-        if (is32) OG_TRY(Fill32(h.stream, static_cast<unsigned int *>(d.s0) + from, static_cast<unsigned int>(init), n));
+        unsigned long long init = InitWord(d.kind);
+        if (Is32(d.kind)) OG_TRY(Fill32(h.stream, static_cast<unsigned int *>(d.s0) + from, static_cast<unsigned int>(init), n));
         else OG_TRY(Fill64(h.stream, static_cast<unsigned long long *>(d.s0) + from, init, n));
         if (d.s1) OG_TRY(Fill64(h.stream, static_cast<unsigned long long *>(d.s1) + from, 0, n));
         if (d.s2) OG_TRY(Fill64(h.stream, static_cast<unsigned long long *>(d.s2) + from, 0, n));
@@ -763,8 +440,54 @@ int SyncGroups(AggHandle &h)
     OG_CHECK(ovf == 0, OMNI_GPU_ERR_STATE, "aggregation table overflowed its group arrays (internal sizing error)");
     h.known_groups = static_cast<int64_t>(g);
     h.pending_rows = 0;
+    if (h.use_lds && h.spec.strategy == OMNI_GPU_STRATEGY_AUTO && h.lds_rows > 0) {
+        // Adaptive fallback: when most rows miss the LDS table (too many groups per workgroup chunk),
+        // later batches go straight to the global table.
+        unsigned long long sp = 0;
+        OG_HIP(hipMemcpy(&sp, h.spills(), 8, hipMemcpyDeviceToHost));
+        int64_t delta = static_cast<int64_t>(sp) - h.spills_seen;
+        h.spills_seen = static_cast<int64_t>(sp);
+        if (delta * 2 > h.lds_rows) h.use_lds = false;
+        h.lds_rows = 0;
+    }
     return OMNI_GPU_OK;
 }
+
+// LDS table layout for the aggregates of a handle within `budget` bytes: per slot one u64 key plus each
+// aggregate's state words (4 bytes for 32-bit MIN/MAX) and a flag byte where the aggregate needs one.
+LdsLayout MakeLdsLayout(const AggArgs &args, int budget)
+{
+    LdsLayout L{};
+    int perSlot = 8;
+    for (int i = 0; i < args.n; ++i) {
+        int k = args.a[i].kind;
+        perSlot += Is32(k) ? 4 : 8;
+        if (k == K_AVG || k == K_SUM_DEC) perSlot += 8;
+        if (k == K_SUM_DEC) perSlot += 8;
+        if (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) perSlot += 1;
+    }
+    int S = (budget - 64 * (2 * OMNI_GPU_MAX_AGGS + 1)) / perSlot;  // room for 8-byte alignment padding
+    S = S - S % 64;
+    L.S = S;
+    int off = 0;
+    auto take = [&](int bytes) {
+        int o = off;
+        off += (bytes + 7) & ~7;
+        return o;
+    };
+    L.keys_off = take(8 * S);
+    for (int i = 0; i < args.n; ++i) {
+        int k = args.a[i].kind;
+        L.off[i][0] = take((Is32(k) ? 4 : 8) * S);
+        L.off[i][1] = (k == K_AVG || k == K_SUM_DEC) ? take(8 * S) : -1;
+        L.off[i][2] = k == K_SUM_DEC ? take(8 * S) : -1;
+        L.flag_off[i] = (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) ? take(S) : -1;
+    }
+    L.bytes = off;
+    return L;
+}
+
+constexpr int kLdsBudget = 64 * 1024;
 
 }  // namespace omni_gpu
 
@@ -818,6 +541,16 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     uint64_t cap = NextPow2(static_cast<uint64_t>(static_cast<double>(want + 2) / kMaxLoad) + 1);
     cap = std::max<uint64_t>(cap, 1024);
     OG_TRY(Resize(*h, cap, 0));
+    // Strategy: the LDS-local table serves single-word keys whose group count fits it (a capacity hint
+    // above half the LDS slots, or a multi-word key, selects the global table directly).
+    h->lds = MakeLdsLayout(h->args, kLdsBudget);
+    bool ldsFits = h->single && h->lds.S >= 256;
+    if (spec->strategy == OMNI_GPU_STRATEGY_LDS) {
+        OG_CHECK(ldsFits, OMNI_GPU_ERR_INVALID, "the LDS strategy needs one key column of <= 8 bytes");
+        h->use_lds = true;
+    } else if (spec->strategy == OMNI_GPU_STRATEGY_AUTO) {
+        h->use_lds = ldsFits && (spec->capacity_hint <= 0 || spec->capacity_hint * 2 <= h->lds.S);
+    }
     *handle = h.release();
     return OMNI_GPU_OK;
 }
@@ -861,6 +594,19 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         }
     }
     GroupCtx g = h.Ctx();
+    if (h.use_lds) {
+        // chunks of >= 8 rows per thread so that the per-workgroup merge stays small against the rows
+        int64_t blocks = (rows + kLdsBlock * 8 - 1) / (kLdsBlock * 8);
+        blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, 4096));
+        int64_t per = (rows + blocks - 1) / blocks;
+        KernelScope ks(h.stream, "agg_insert_lds");
+        agg_insert_lds<<<static_cast<unsigned int>(blocks), kLdsBlock, h.lds.bytes, h.stream>>>(
+            h.kd, rows, per, h.slots.as<Slot1>(), h.cap - 1, g, h.args, h.lds, h.spills());
+        OG_TRY(CheckLaunch("agg_insert_lds"));
+        h.lds_rows += rows;
+        h.pending_rows += rows;
+        return OMNI_GPU_OK;
+    }
     {
         KernelScope ks(h.stream, h.single ? "agg_insert_single" : "agg_insert_multi");
         if (h.single) {
@@ -872,6 +618,13 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         OG_TRY(CheckLaunch("agg_insert"));
     }
     h.pending_rows += rows;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_wait(void *handle)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    OG_HIP(hipStreamSynchronize(static_cast<AggHandle *>(handle)->stream));
     return OMNI_GPU_OK;
 }
 

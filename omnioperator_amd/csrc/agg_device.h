@@ -1,0 +1,511 @@
+// Device-side data structures and table operations of the hash aggregation, shared by the
+// per-batch kernels (agg.hip) and the batched persistent kernel (batched.hip).
+#pragma once
+
+#include "omni_gpu_internal.h"
+
+namespace omni_gpu {
+
+
+enum AggKind : int {
+    K_SUM_I64 = 0,
+    K_SUM_F64,
+    K_SUM_DEC,
+    K_COUNT_ALL,
+    K_COUNT_COL,
+    K_AVG,
+    K_MIN_I64,
+    K_MAX_I64,
+    K_MIN_I32,
+    K_MAX_I32,
+    K_MIN_F64,
+    K_MAX_F64,
+};
+
+struct AggDev {
+    int kind;
+    int in_width;     // bytes per input value
+    int in_double;    // input column holds doubles
+    int out_width;    // bytes per output value
+    const void *values;
+    const uint8_t *nulls;
+    void *s0;
+    void *s1;
+    void *s2;
+    uint8_t *flag;    // 1 once a non-NULL value was added
+    int dec_scale_up; // 10^(outScale - inScale) exponent for decimal SUM
+    int dec_out128;   // decimal SUM output is DECIMAL128
+};
+
+struct AggArgs {
+    int n;
+    AggDev a[OMNI_GPU_MAX_AGGS];
+};
+
+struct KeyDev {
+    int num;
+    int W;                      // words per key (multi-word path)
+    int width[OMNI_GPU_MAX_KEYS];
+    int word[OMNI_GPU_MAX_KEYS];   // first word index of each column
+    int in_word0[OMNI_GPU_MAX_KEYS];  // column stored in the high half of word 0
+    const void *values[OMNI_GPU_MAX_KEYS];
+    const uint8_t *nulls[OMNI_GPU_MAX_KEYS];
+};
+
+struct Slot1 {
+    unsigned long long key;
+    unsigned int gid1;
+    unsigned int pad;
+};
+
+struct GroupCtx {
+    unsigned long long *ngroups;   // group counter
+    unsigned long long max_groups; // capacity of the per-group arrays
+    unsigned int *overflow;        // set when a group id would exceed max_groups (host sizing bug guard)
+    unsigned long long *gkeys;
+    uint8_t *gnull;
+    unsigned int *special;         // [0] NULL group, [1] sentinel-key group (0 empty, 1 busy, gid+2)
+};
+
+constexpr int kMaxWords = 6;
+
+__device__ __forceinline__ unsigned int LoadAcquire(unsigned int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long LoadRelaxed64(unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void StoreRelease(unsigned int *p, unsigned int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Claims a fresh group id; returns ~0 if the per-group arrays are full (never expected: the host sizes them).
+__device__ __forceinline__ unsigned int NewGroup(const GroupCtx &g)
+{
+    unsigned long long id = WaveReserveOne(g.ngroups, true);
+    if (id >= g.max_groups) {
+        atomicOr(g.overflow, 1u);
+        return 0xFFFFFFFFu;
+    }
+    return static_cast<unsigned int>(id);
+}
+
+// Group of a side cell (NULL key, or the key equal to the slot sentinel), created on first use.
+static __device__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned long long key, bool isNull, int W)
+{
+    unsigned int *cell = g.special + which;
+    while (true) {
+        unsigned int c = LoadAcquire(cell);
+        if (c >= 2) return c - 2;
+        if (c == 0 && atomicCAS(cell, 0u, 1u) == 0u) {
+            unsigned int gid = NewGroup(g);
+            if (gid == 0xFFFFFFFFu) {
+                StoreRelease(cell, 0u);
+                return gid;
+            }
+            g.gkeys[static_cast<size_t>(gid) * W] = key;
+            if (g.gnull) g.gnull[gid] = isNull ? 1 : 0;
+            StoreRelease(cell, gid + 2);
+            return gid;
+        }
+    }
+}
+
+// Find-or-insert of a single-word key; returns its group id.
+static __device__ unsigned int InsertSingle(Slot1 *slots, unsigned long long mask, unsigned long long key,
+                                     const GroupCtx &g)
+{
+    unsigned long long pos = Mix64(key) & mask;
+    while (true) {
+        unsigned long long k = slots[pos].key;
+        if (k == kEmptyKey) {
+            k = atomicCAS(&slots[pos].key, kEmptyKey, key);
+            if (k == kEmptyKey) {
+                unsigned int gid = NewGroup(g);
+                if (gid == 0xFFFFFFFFu) return gid;
+                g.gkeys[gid] = key;
+                if (g.gnull) g.gnull[gid] = 0;
+                StoreRelease(&slots[pos].gid1, gid + 1);
+                return gid;
+            }
+        }
+        if (k == key) {
+            unsigned int g1 = LoadAcquire(&slots[pos].gid1);
+            if (g1) return g1 - 1;
+            continue;  // the winning lane publishes its group id; re-read the same slot
+        }
+        pos = (pos + 1) & mask;
+    }
+}
+
+// Find-or-insert of a W-word key (ctrl-word protocol).
+static __device__ unsigned int InsertMulti(unsigned int *ctrl, unsigned long long *skeys, unsigned long long mask,
+                                    const unsigned long long *words, int W, unsigned long long hash,
+                                    const GroupCtx &g)
+{
+    unsigned long long pos = hash & mask;
+    while (true) {
+        unsigned int c = LoadAcquire(&ctrl[pos]);
+        if (c == 0) {
+            c = atomicCAS(&ctrl[pos], 0u, 1u);
+            if (c == 0) {
+                unsigned int gid = NewGroup(g);
+                if (gid == 0xFFFFFFFFu) {
+                    StoreRelease(&ctrl[pos], 0u);
+                    return gid;
+                }
+                for (int w = 0; w < W; ++w) {
+                    skeys[pos * W + w] = words[w];
+                    g.gkeys[static_cast<size_t>(gid) * W + w] = words[w];
+                }
+                StoreRelease(&ctrl[pos], gid + 2);
+                return gid;
+            }
+        }
+        if (c == 1) continue;  // another lane is writing this slot's key
+        bool same = true;
+        for (int w = 0; w < W; ++w) same &= LoadRelaxed64(&skeys[pos * W + w]) == words[w];
+        if (same) return c - 2;
+        pos = (pos + 1) & mask;
+    }
+}
+
+__device__ __forceinline__ void EncodeKey(const KeyDev &kd, int64_t row, unsigned long long *words)
+{
+    for (int w = 0; w < kd.W; ++w) words[w] = 0;
+    for (int c = 0; c < kd.num; ++c) {
+        if (NullBit(kd.nulls[c], row)) {
+            words[0] |= 1ULL << c;
+            continue;
+        }
+        if (kd.width[c] == 16) {
+            const unsigned long long *v = static_cast<const unsigned long long *>(kd.values[c]);
+            words[kd.word[c]] = v[2 * row];
+            words[kd.word[c] + 1] = v[2 * row + 1];
+        } else if (kd.in_word0[c]) {
+            words[0] |= (LoadWord(kd.values[c], kd.width[c], row) & 0xFFFFFFFFULL) << 32;
+        } else {
+            words[kd.word[c]] = LoadWord(kd.values[c], kd.width[c], row);
+        }
+    }
+}
+
+__device__ __forceinline__ unsigned long long HashWords(const unsigned long long *words, int W)
+{
+    unsigned long long h = 0x9E3779B97F4A7C15ULL;
+    for (int w = 0; w < W; ++w) h = Mix64(h ^ (words[w] + 0x632BE59BD9B4E019ULL * (w + 1)));
+    return h;
+}
+
+// Exact 192-bit accumulation: adds top*2^128 + hi*2^64 + lo (hi, lo unsigned words) into the
+// accumulator (s0 = lo, s1 = hi, s2 = top).  Every atomic's carry-out is derived from the value it
+// returned, so concurrent additions in any order give the exact sum.
+__device__ __forceinline__ void AddDec3(unsigned long long *lo, unsigned long long *hi, long long *top,
+                                        unsigned long long vlo, unsigned long long vhi, long long vtop)
+{
+    unsigned long long old = atomicAdd(lo, vlo);
+    unsigned long long c = (old + vlo) < old ? 1ULL : 0ULL;
+    unsigned long long a = vhi + c;
+    long long t = vtop + ((c && a == 0) ? 1 : 0);  // vhi == 2^64-1 plus a carry wraps into top
+    if (a) {
+        unsigned long long oh = atomicAdd(hi, a);
+        t += (oh + a) < oh ? 1 : 0;
+    }
+    if (t) atomicAdd(reinterpret_cast<unsigned long long *>(top), static_cast<unsigned long long>(t));
+}
+
+// Signed 128-bit input value (lo, hi) of a decimal SUM as (lo, hi, top) words.
+__device__ __forceinline__ void DecInput(const AggDev &d, int64_t row, unsigned long long &vlo,
+                                         unsigned long long &vhi, long long &vtop)
+{
+    long long h;
+    if (d.in_width == 16) {
+        const unsigned long long *v = static_cast<const unsigned long long *>(d.values);
+        vlo = v[2 * row];
+        h = static_cast<long long>(v[2 * row + 1]);
+    } else {
+        long long v = static_cast<long long>(LoadWord(d.values, d.in_width, row));
+        vlo = static_cast<unsigned long long>(v);
+        h = v < 0 ? -1 : 0;
+    }
+    vhi = static_cast<unsigned long long>(h);
+    vtop = h < 0 ? -1 : 0;
+}
+
+// Per-group state addressing: GlobalRef indexes the HBM state arrays by group id, LdsRef the
+// workgroup's LDS table by slot.  The same update code runs against either.
+struct GlobalRef {
+    const AggArgs &args;
+    unsigned int gid;
+    template <typename T> __device__ __forceinline__ T *p(int i, int w) const
+    {
+        const AggDev &d = args.a[i];
+        return static_cast<T *>(w == 0 ? d.s0 : (w == 1 ? d.s1 : d.s2)) + gid;
+    }
+    __device__ __forceinline__ uint8_t *flag(int i) const { return args.a[i].flag + gid; }
+};
+
+struct LdsLayout {
+    int S;                              // LDS slots
+    int keys_off;                       // byte offset of the u64 key array
+    int off[OMNI_GPU_MAX_AGGS][3];      // byte offsets of each aggregate's state arrays (-1 unused)
+    int flag_off[OMNI_GPU_MAX_AGGS];    // byte offsets of the flag arrays (-1 unused)
+    int bytes;                          // total LDS bytes
+};
+
+struct LdsRef {
+    uint8_t *base;
+    const LdsLayout &L;
+    int slot;
+    template <typename T> __device__ __forceinline__ T *p(int i, int w) const
+    {
+        return reinterpret_cast<T *>(base + L.off[i][w]) + slot;
+    }
+    __device__ __forceinline__ uint8_t *flag(int i) const { return base + L.flag_off[i] + slot; }
+};
+
+template <typename Ref> __device__ __forceinline__ void UpdateAggsT(const AggArgs &args, const Ref &r, int64_t row)
+{
+    typedef unsigned long long u64;
+    for (int i = 0; i < args.n; ++i) {
+        const AggDev &d = args.a[i];
+        if (d.kind == K_COUNT_ALL) {
+            atomicAdd(r.template p<u64>(i, 0), 1ULL);
+            continue;
+        }
+        if (NullBit(d.nulls, row)) continue;
+        switch (d.kind) {
+            case K_COUNT_COL: atomicAdd(r.template p<u64>(i, 0), 1ULL); break;
+            case K_SUM_I64:
+                atomicAdd(r.template p<u64>(i, 0), LoadWord(d.values, d.in_width, row));
+                *r.flag(i) = 1;
+                break;
+            case K_SUM_F64:
+                atomicAdd(r.template p<double>(i, 0), static_cast<const double *>(d.values)[row]);
+                *r.flag(i) = 1;
+                break;
+            case K_AVG: {
+                double v = d.in_double ? static_cast<const double *>(d.values)[row]
+                                       : static_cast<double>(static_cast<long long>(LoadWord(d.values, d.in_width, row)));
+                atomicAdd(r.template p<double>(i, 0), v);
+                atomicAdd(r.template p<u64>(i, 1), 1ULL);
+                break;
+            }
+            case K_SUM_DEC: {
+                u64 vlo, vhi;
+                long long vtop;
+                DecInput(d, row, vlo, vhi, vtop);
+                AddDec3(r.template p<u64>(i, 0), r.template p<u64>(i, 1), r.template p<long long>(i, 2), vlo, vhi, vtop);
+                *r.flag(i) = 1;
+                break;
+            }
+            case K_MIN_I64:
+                atomicMin(r.template p<long long>(i, 0), static_cast<long long>(LoadWord(d.values, d.in_width, row)));
+                *r.flag(i) = 1;
+                break;
+            case K_MAX_I64:
+                atomicMax(r.template p<long long>(i, 0), static_cast<long long>(LoadWord(d.values, d.in_width, row)));
+                *r.flag(i) = 1;
+                break;
+            case K_MIN_I32:
+                atomicMin(r.template p<int>(i, 0), static_cast<int>(LoadWord(d.values, d.in_width, row)));
+                *r.flag(i) = 1;
+                break;
+            case K_MAX_I32:
+                atomicMax(r.template p<int>(i, 0), static_cast<int>(LoadWord(d.values, d.in_width, row)));
+                *r.flag(i) = 1;
+                break;
+            case K_MIN_F64:
+                atomicMin(r.template p<double>(i, 0), static_cast<const double *>(d.values)[row]);
+                *r.flag(i) = 1;
+                break;
+            case K_MAX_F64:
+                atomicMax(r.template p<double>(i, 0), static_cast<const double *>(d.values)[row]);
+                *r.flag(i) = 1;
+                break;
+            default: break;
+        }
+    }
+}
+
+__device__ __forceinline__ void UpdateAggs(const AggArgs &args, unsigned int gid, int64_t row)
+{
+    UpdateAggsT(args, GlobalRef{args, gid}, row);
+}
+
+// Merges one LDS partial state into the global group `gid`.
+__device__ __forceinline__ void MergeAggs(const AggArgs &args, unsigned int gid, const LdsRef &l)
+{
+    typedef unsigned long long u64;
+    GlobalRef g{args, gid};
+    for (int i = 0; i < args.n; ++i) {
+        const AggDev &d = args.a[i];
+        switch (d.kind) {
+            case K_COUNT_ALL:
+            case K_COUNT_COL: {
+                u64 v = *l.p<u64>(i, 0);
+                if (v) atomicAdd(g.p<u64>(i, 0), v);
+                break;
+            }
+            case K_AVG: {
+                u64 c = *l.p<u64>(i, 1);
+                if (c) {
+                    atomicAdd(g.p<double>(i, 0), *l.p<double>(i, 0));
+                    atomicAdd(g.p<u64>(i, 1), c);
+                }
+                break;
+            }
+            default: {
+                if (!*l.flag(i)) break;
+                *g.flag(i) = 1;
+                switch (d.kind) {
+                    case K_SUM_I64: atomicAdd(g.p<u64>(i, 0), *l.p<u64>(i, 0)); break;
+                    case K_SUM_F64: atomicAdd(g.p<double>(i, 0), *l.p<double>(i, 0)); break;
+                    case K_SUM_DEC:
+                        AddDec3(g.p<u64>(i, 0), g.p<u64>(i, 1), g.p<long long>(i, 2), *l.p<u64>(i, 0), *l.p<u64>(i, 1),
+                                *l.p<long long>(i, 2));
+                        break;
+                    case K_MIN_I64: atomicMin(g.p<long long>(i, 0), *l.p<long long>(i, 0)); break;
+                    case K_MAX_I64: atomicMax(g.p<long long>(i, 0), *l.p<long long>(i, 0)); break;
+                    case K_MIN_I32: atomicMin(g.p<int>(i, 0), *l.p<int>(i, 0)); break;
+                    case K_MAX_I32: atomicMax(g.p<int>(i, 0), *l.p<int>(i, 0)); break;
+                    case K_MIN_F64: atomicMin(g.p<double>(i, 0), *l.p<double>(i, 0)); break;
+                    case K_MAX_F64: atomicMax(g.p<double>(i, 0), *l.p<double>(i, 0)); break;
+                    default: break;
+                }
+            }
+        }
+    }
+}
+
+// Initial value of an aggregate's first state word (MIN/MAX start at the type's extreme).
+__host__ __device__ __forceinline__ unsigned long long InitWord(int kind)
+{
+    switch (kind) {
+        case K_MIN_I64: return 0x7FFFFFFFFFFFFFFFULL;
+        case K_MAX_I64: return 0x8000000000000000ULL;
+        case K_MIN_I32: return 0x7FFFFFFFULL;
+        case K_MAX_I32: return 0x80000000ULL;
+        case K_MIN_F64: return 0x7FEFFFFFFFFFFFFFULL;  // DBL_MAX
+        case K_MAX_F64: return 0xFFEFFFFFFFFFFFFFULL;  // -DBL_MAX
+        default: return 0;
+    }
+}
+
+__host__ __device__ __forceinline__ bool Is32(int kind) { return kind == K_MIN_I32 || kind == K_MAX_I32; }
+
+
+struct OutDev {
+    int width;
+    void *values;
+    uint8_t *nulls;
+};
+
+struct OutArgs {
+    int nkeys;
+    int naggs;
+    OutDev o[OMNI_GPU_MAX_KEYS + OMNI_GPU_MAX_AGGS];
+};
+
+// Writes the null bitmap of 64 consecutive output rows handled by one wave.
+__device__ __forceinline__ void WriteNullBits(uint8_t *nulls, int64_t r, int64_t count, bool isNull)
+{
+    unsigned long long m = __ballot(isNull);
+    int lane = LaneId();
+    int64_t base = r - lane;  // first row of this wave (waves cover aligned groups of 64 rows)
+    if (lane < 8) {
+        int64_t byte = (base >> 3) + lane;
+        if (byte * 8 < count) nulls[byte] = static_cast<uint8_t>(m >> (8 * lane));
+    }
+}
+
+__device__ __forceinline__ void StoreWidth(void *dst, int width, int64_t r, unsigned long long v)
+{
+    switch (width) {
+        case 8: static_cast<unsigned long long *>(dst)[r] = v; break;
+        case 4: static_cast<unsigned int *>(dst)[r] = static_cast<unsigned int>(v); break;
+        case 2: static_cast<unsigned short *>(dst)[r] = static_cast<unsigned short>(v); break;
+        default: static_cast<uint8_t *>(dst)[r] = static_cast<uint8_t>(v); break;
+    }
+}
+
+// Decimal finalisation: range check of the 192-bit total, then x 10^up with overflow checks
+// (SumSparkDecimalAggregator::ExtractValuesBatchInternal, sum_spark_decimal_aggregator.h:246-301).
+static __device__ bool FinishDecimal(unsigned long long lo, unsigned long long hi, long long top, int up, bool out128,
+                              __int128 &res)
+{
+    bool fits128 = (top == 0 && static_cast<long long>(hi) >= 0) || (top == -1 && static_cast<long long>(hi) < 0);
+    if (!fits128) return false;
+    __int128 v = static_cast<__int128>((static_cast<unsigned __int128>(hi) << 64) | lo);
+    const __int128 max38 = (static_cast<__int128>(5421010862427522170LL) << 64) + 687399551400673279LL;
+    const __int128 lim = out128 ? max38 : static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL);
+    const __int128 llim = out128 ? -max38 : -static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL) - 1;
+    if (v > lim || v < llim) return false;
+    for (int i = 0; i < up; ++i) {
+        // v * 10 must stay within [-max38, max38]
+        if (v > max38 / 10 || v < -(max38 / 10)) return false;
+        v = (v << 3) + (v << 1);
+    }
+    if (v > lim || v < llim) return false;
+    res = v;
+    return true;
+}
+
+
+// Writes aggregate i of one group (state addressed through `s`) into output row r of `o` and returns whether
+// the value is NULL (ExtractValuesBatch of each aggregator: sum_flatim_aggregator.h:243-257,
+// sum_spark_decimal_aggregator.h:246-301, max_aggregator.cpp:34-55, average_flatim_aggregator.h:275-290).
+template <typename Ref>
+__device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s, const OutDev &o, int64_t r,
+                                            int overflow_as_null, unsigned int *error_flag)
+{
+    typedef unsigned long long u64;
+    bool isNull = false;
+    switch (d.kind) {
+        case K_COUNT_ALL:
+        case K_COUNT_COL: static_cast<u64 *>(o.values)[r] = *s.template p<u64>(i, 0); break;
+        case K_AVG: {
+            u64 cnt = *s.template p<u64>(i, 1);
+            isNull = cnt == 0;
+            static_cast<double *>(o.values)[r] = isNull ? 0.0 : *s.template p<double>(i, 0) / static_cast<double>(cnt);
+            break;
+        }
+        case K_SUM_DEC: {
+            isNull = *s.flag(i) == 0;
+            __int128 v = 0;
+            if (!isNull) {
+                bool ok = FinishDecimal(*s.template p<u64>(i, 0), *s.template p<u64>(i, 1),
+                                        *s.template p<long long>(i, 2), d.dec_scale_up, d.dec_out128, v);
+                if (!ok) {
+                    isNull = true;
+                    v = 0;
+                    if (!overflow_as_null) atomicOr(error_flag, 1u);
+                }
+            }
+            if (d.dec_out128) {
+                static_cast<u64 *>(o.values)[2 * r] = static_cast<u64>(v);
+                static_cast<u64 *>(o.values)[2 * r + 1] = static_cast<u64>(static_cast<unsigned __int128>(v) >> 64);
+            } else {
+                static_cast<long long *>(o.values)[r] = static_cast<long long>(v);
+            }
+            break;
+        }
+        case K_MIN_I32:
+        case K_MAX_I32: {
+            isNull = *s.flag(i) == 0;
+            int v = isNull ? 0 : *s.template p<int>(i, 0);
+            StoreWidth(o.values, o.width, r, static_cast<u64>(static_cast<long long>(v)));
+            break;
+        }
+        default:  // SUM_I64, SUM_F64, MIN/MAX 64-bit: the 8-byte state is the value
+            isNull = *s.flag(i) == 0;
+            static_cast<u64 *>(o.values)[r] = isNull ? 0ULL : *s.template p<u64>(i, 0);
+            break;
+    }
+    return isNull;
+}
+
+}  // namespace omni_gpu

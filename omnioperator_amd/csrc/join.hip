@@ -238,6 +238,7 @@ __global__ void __launch_bounds__(kBlock) join_probe(ProbeArgs a, TableDev t)
         a.out_rows[o] = probeRow;
         for (int p = 0; p < a.npay; ++p) {
             const PayloadDev &pd = a.pay[p];
+            if (pd.out == nullptr) continue;
             if (INLINE) {
                 if (pd.width == 4) static_cast<unsigned int *>(pd.out)[o] = pay;
                 else if (pd.width == 2) static_cast<unsigned short *>(pd.out)[o] = static_cast<unsigned short>(pay);
@@ -318,15 +319,14 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
     a.counter = reinterpret_cast<unsigned long long *>(h.scratch.as<uint8_t>() + 32);
     for (int p = 0; p < a.npay; ++p) {
         OG_CHECK(outs && outs[p].type_id == h.spec.payload_types[p], OMNI_GPU_ERR_INVALID, "payload output %d type mismatch", p);
-        OG_CHECK(cap == 0 || outs[p].values, OMNI_GPU_ERR_INVALID, "payload output %d has no buffer", p);
-        OG_CHECK(!h.pay_nullable[p] || cap == 0 || outs[p].nulls, OMNI_GPU_ERR_INVALID,
+        OG_CHECK(!h.pay_nullable[p] || cap == 0 || !outs[p].values || outs[p].nulls, OMNI_GPU_ERR_INVALID,
                  "payload %d has NULLs: its output needs a null bitmap", p);
         a.pay[p].width = TypeWidth(h.spec.payload_types[p]);
         a.pay[p].values = h.pay[p].p;
         a.pay[p].nulls = h.pay_nullable[p] ? h.pay_null[p].as<uint8_t>() : nullptr;
         a.pay[p].out = outs[p].values;
         a.pay[p].out_nulls = outs[p].nulls;
-        if (outs[p].nulls && cap > 0) {
+        if (outs[p].values && outs[p].nulls && cap > 0) {
             int64_t words = (cap + 31) / 32;
             KernelScope ks(s, "join_clear_nulls");
             join_fill_i32<<<Blocks(words), kBlock, 0, s>>>(reinterpret_cast<int *>(outs[p].nulls), 0, words);

@@ -1,0 +1,424 @@
+// GPU-backed HashAggregation / HashBuilder / LookupJoin operators over the C ABI of include/omni_gpu.h.
+#include "gpu_operators.h"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace omniruntime {
+namespace op {
+
+namespace {
+constexpr int32_t kMaxVecBatchBytes = 1024 * 1024;  // operator_util.h:33
+
+void Check(int rc, const char *what)
+{
+    if (rc != OMNI_GPU_OK) {
+        std::string code = rc == OMNI_GPU_ERR_OVERFLOW ? "OPERATOR_RUNTIME_ERROR" : "GPU_ERROR";
+        throw OmniException(code, std::string(what) + ": " + omni_gpu_last_error());
+    }
+}
+
+void EnsureInit()
+{
+    static bool done = false;
+    if (!done) {
+        Check(omni_gpu_init(0), "omni_gpu_init");
+        done = true;
+    }
+}
+
+// Copies a flat vector to device memory: values into `vals`, the null bitmap into `nulls` when it has NULLs.
+omni_gpu_column Upload(const vec::BaseVector *v, int32_t rows, DeviceBuffer &vals, DeviceBuffer &nulls)
+{
+    omni_gpu_column c{};
+    c.type_id = v->GetTypeId();
+    size_t bytes = static_cast<size_t>(rows) * v->Width();
+    vals.Reserve(bytes);
+    Check(omni_gpu_memcpy_h2d(vals.Get(), v->RawValues(), bytes), "upload values");
+    c.values = vals.Get();
+    if (v->HasNull()) {
+        size_t nb = (rows + 7) / 8;
+        nulls.Reserve(nb);
+        Check(omni_gpu_memcpy_h2d(nulls.Get(), v->RawNulls(), nb), "upload nulls");
+        c.nulls = static_cast<const uint8_t *>(nulls.Get());
+    }
+    return c;
+}
+}  // namespace
+
+DeviceBuffer::~DeviceBuffer()
+{
+    if (ptr_) omni_gpu_free(ptr_);
+}
+
+void DeviceBuffer::Reserve(size_t bytes)
+{
+    if (bytes <= bytes_ && ptr_) return;
+    if (ptr_) omni_gpu_free(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+    Check(omni_gpu_malloc(&ptr_, bytes ? bytes : 1), "omni_gpu_malloc");
+    bytes_ = bytes ? bytes : 1;
+}
+
+int32_t MaxRowCount(int32_t rowSize) { return (kMaxVecBatchBytes + rowSize - 1) / rowSize; }
+
+// ================================================================================================
+// Hash aggregation
+// ================================================================================================
+
+GpuHashAggregationOperatorFactory::GpuHashAggregationOperatorFactory(
+    std::vector<uint32_t> &groupByCol, const type::DataTypes &groupInputTypes,
+    std::vector<std::vector<uint32_t>> &aggsCols, std::vector<type::DataTypes> &aggInputTypes,
+    std::vector<type::DataTypes> &aggOutputTypes, std::vector<uint32_t> &aggFuncTypes,
+    std::vector<uint32_t> &maskColsVector, std::vector<bool> inputRaws, std::vector<bool> outputPartials,
+    bool overflowAsNull, bool isStatisticalAggregate)
+    : groupByCols_(groupByCol), groupTypes_(groupInputTypes)
+{
+    if (!maskColsVector.empty()) throw OmniException("UNSUPPORTED_ERROR", "aggregate masks are not on the GPU path");
+    if (groupByCol.empty() || groupByCol.size() > OMNI_GPU_MAX_KEYS) {
+        throw OmniException("UNSUPPORTED_ERROR", "1 to 4 group-by columns are supported on the GPU path");
+    }
+    if (aggFuncTypes.size() > OMNI_GPU_MAX_AGGS) throw OmniException("UNSUPPORTED_ERROR", "too many aggregates");
+    spec_.num_keys = static_cast<int32_t>(groupByCol.size());
+    for (size_t i = 0; i < groupByCol.size(); ++i) spec_.key_types[i] = groupInputTypes.GetType(i)->GetId();
+    spec_.num_aggs = static_cast<int32_t>(aggFuncTypes.size());
+    spec_.overflow_as_null = overflowAsNull ? 1 : 0;
+    // COUNT_ALL aggregates take no input column: aggsCols / aggInputTypes list only the others
+    // (HashAggregationOperatorFactory::CreateOperator, group_aggregation.cpp:104-121).
+    size_t inputIdx = 0;
+    for (size_t i = 0; i < aggFuncTypes.size(); ++i) {
+        bool raw = i < inputRaws.size() ? inputRaws[i] : true;
+        bool partial = i < outputPartials.size() ? outputPartials[i] : false;
+        if (!raw || partial) throw OmniException("UNSUPPORTED_ERROR", "only single-stage aggregation is on the GPU path");
+        spec_.agg_funcs[i] = static_cast<int32_t>(aggFuncTypes[i]);
+        const type::DataTypePtr &out = aggOutputTypes.at(i).GetType(0);
+        spec_.agg_out_types[i] = out->GetId();
+        spec_.agg_out_scales[i] = out->GetScale();
+        aggOutTypes_.push_back(out);
+        if (aggFuncTypes[i] == OMNI_AGG_COUNT_ALL) {
+            spec_.agg_in_types[i] = type::OMNI_LONG;
+            aggInputCol_.push_back(-1);
+            continue;
+        }
+        const type::DataTypePtr &in = aggInputTypes.at(inputIdx).GetType(0);
+        spec_.agg_in_types[i] = in->GetId();
+        spec_.agg_in_scales[i] = in->GetScale();
+        aggInputCol_.push_back(static_cast<int32_t>(aggsCols.at(inputIdx).at(0)));
+        ++inputIdx;
+    }
+}
+
+OmniStatus GpuHashAggregationOperatorFactory::Init() { return OMNI_STATUS_NORMAL; }
+
+Operator *GpuHashAggregationOperatorFactory::CreateOperator() { return new GpuHashAggregationOperator(*this); }
+
+GpuHashAggregationOperator::GpuHashAggregationOperator(const GpuHashAggregationOperatorFactory &f)
+    : groupByCols_(f.groupByCols_), aggInputCol_(f.aggInputCol_), spec_(f.spec_)
+{
+    EnsureInit();
+    Check(omni_gpu_agg_create(&spec_, &handle_), "omni_gpu_agg_create");
+    int32_t rowSize = 0;
+    for (int i = 0; i < spec_.num_keys; ++i) {
+        outTypes_.push_back(spec_.key_types[i]);
+        rowSize += type::TypeWidth(spec_.key_types[i]);
+    }
+    for (int i = 0; i < spec_.num_aggs; ++i) {
+        outTypes_.push_back(spec_.agg_out_types[i]);
+        rowSize += type::TypeWidth(spec_.agg_out_types[i]);
+    }
+    rowsPerBatch_ = MaxRowCount(std::max(rowSize, 1));
+}
+
+GpuHashAggregationOperator::~GpuHashAggregationOperator() { Close(); }
+
+OmniStatus GpuHashAggregationOperator::Close()
+{
+    if (handle_) {
+        omni_gpu_agg_destroy(handle_);
+        handle_ = nullptr;
+    }
+    return OMNI_STATUS_NORMAL;
+}
+
+// AddInput: group_aggregation.cpp:320-406 -- the batch is consumed and freed here.
+int32_t GpuHashAggregationOperator::AddInput(vec::VectorBatch *vecBatch)
+{
+    hasInput_ = true;
+    int32_t rows = vecBatch->GetRowCount();
+    if (rows <= 0) {
+        vec::VectorHelper::FreeVecBatch(vecBatch);
+        return 0;
+    }
+    if (pending_) Check(omni_gpu_agg_wait(handle_), "omni_gpu_agg_wait");
+    size_t nk = groupByCols_.size(), na = aggInputCol_.size();
+    while (staging_.size() < 2 * (nk + na)) staging_.emplace_back(new DeviceBuffer());
+    omni_gpu_column keys[OMNI_GPU_MAX_KEYS] = {};
+    omni_gpu_column aggs[OMNI_GPU_MAX_AGGS] = {};
+    for (size_t i = 0; i < nk; ++i) {
+        keys[i] = Upload(vecBatch->Get(groupByCols_[i]), rows, *staging_[2 * i], *staging_[2 * i + 1]);
+    }
+    for (size_t i = 0; i < na; ++i) {
+        if (aggInputCol_[i] < 0) {
+            aggs[i].type_id = type::OMNI_LONG;
+            continue;
+        }
+        aggs[i] = Upload(vecBatch->Get(aggInputCol_[i]), rows, *staging_[2 * (nk + i)], *staging_[2 * (nk + i) + 1]);
+    }
+    vec::VectorHelper::FreeVecBatch(vecBatch);
+    Check(omni_gpu_agg_add_batch(handle_, keys, aggs, rows), "omni_gpu_agg_add_batch");
+    pending_ = true;
+    return 0;
+}
+
+// GetOutput: group_aggregation.cpp:454-474, 1486-1538 -- batches of ceil(1 MiB / row size) rows.
+int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
+{
+    *result = nullptr;
+    if (!hasInput_) return 0;
+    if (groups_ < 0) {
+        Check(omni_gpu_agg_num_groups(handle_, &groups_), "omni_gpu_agg_num_groups");
+        pending_ = false;
+    }
+    if (emitted_ >= groups_) {
+        SetStatus(OMNI_STATUS_FINISHED);
+        return 0;
+    }
+    int32_t count = static_cast<int32_t>(std::min<int64_t>(rowsPerBatch_, groups_ - emitted_));
+    size_t ncols = outTypes_.size();
+    std::vector<std::unique_ptr<DeviceBuffer>> vals(ncols), nulls(ncols);
+    std::vector<omni_gpu_out_column> outs(ncols);
+    for (size_t c = 0; c < ncols; ++c) {
+        vals[c].reset(new DeviceBuffer());
+        nulls[c].reset(new DeviceBuffer());
+        vals[c]->Reserve(static_cast<size_t>(count) * type::TypeWidth(outTypes_[c]));
+        nulls[c]->Reserve((count + 7) / 8);
+        outs[c].type_id = outTypes_[c];
+        outs[c].values = vals[c]->Get();
+        outs[c].nulls = static_cast<uint8_t *>(nulls[c]->Get());
+    }
+    Check(omni_gpu_agg_get_output(handle_, emitted_, count, outs.data()), "omni_gpu_agg_get_output");
+    auto *batch = new vec::VectorBatch(count);
+    for (size_t c = 0; c < ncols; ++c) {
+        auto *v = vec::VectorHelper::CreateFlatVector(outTypes_[c], count);
+        Check(omni_gpu_memcpy_d2h(v->RawValues(), vals[c]->Get(), static_cast<size_t>(count) * v->Width()), "download");
+        Check(omni_gpu_memcpy_d2h(v->RawNulls(), nulls[c]->Get(), (count + 7) / 8), "download nulls");
+        batch->Append(v);
+    }
+    emitted_ += count;
+    if (emitted_ >= groups_) SetStatus(OMNI_STATUS_FINISHED);
+    *result = batch;
+    return 1;
+}
+
+uint64_t GpuHashAggregationOperator::GetHashMapUniqueKeys()
+{
+    int64_t g = 0;
+    Check(omni_gpu_agg_num_groups(handle_, &g), "omni_gpu_agg_num_groups");
+    pending_ = false;
+    return static_cast<uint64_t>(g);
+}
+
+// ================================================================================================
+// Hash join: builder
+// ================================================================================================
+
+GpuHashBuilderOperatorFactory *GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(
+    JoinType joinType, const type::DataTypes &buildTypes, const int32_t *buildHashCols, int32_t buildHashColsCount,
+    int32_t operatorCount)
+{
+    if (joinType != OMNI_JOIN_TYPE_INNER) throw OmniException("UNSUPPORTED_ERROR", "only inner joins are on the GPU path");
+    if (buildHashColsCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one join key column is supported");
+    if (operatorCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one build partition is supported");
+    return new GpuHashBuilderOperatorFactory(buildTypes, buildHashCols[0]);
+}
+
+GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(const type::DataTypes &buildTypes, int32_t keyCol)
+    : buildTypes_(buildTypes), keyCol_(keyCol)
+{
+    EnsureInit();
+    omni_gpu_join_spec spec{};
+    spec.key_type = buildTypes.GetType(keyCol)->GetId();
+    spec.num_payloads = buildTypes.GetSize();  // every build column is retained, like the reference's batches
+    if (spec.num_payloads > OMNI_GPU_MAX_AGGS) throw OmniException("UNSUPPORTED_ERROR", "too many build columns");
+    for (int32_t i = 0; i < spec.num_payloads; ++i) {
+        spec.payload_types[i] = buildTypes.GetType(i)->GetId();
+        if (type::TypeWidth(spec.payload_types[i]) == 0) {
+            throw OmniException("UNSUPPORTED_ERROR", "variable-width build columns are not on the GPU path");
+        }
+    }
+    spec.strategy = 0;
+    Check(omni_gpu_join_build_create(&spec, &handle_), "omni_gpu_join_build_create");
+}
+
+GpuHashBuilderOperatorFactory::~GpuHashBuilderOperatorFactory()
+{
+    if (handle_) omni_gpu_join_destroy(handle_);
+}
+
+Operator *GpuHashBuilderOperatorFactory::CreateOperator() { return new GpuHashBuilderOperator(*this); }
+
+// HashBuilderOperator::AddInput (hash_builder.cpp:169-180): the batch is retained -- here its columns move to
+// device memory and the host batch is freed.
+int32_t GpuHashBuilderOperator::AddInput(vec::VectorBatch *vecBatch)
+{
+    int32_t rows = vecBatch->GetRowCount();
+    int32_t ncols = f_.buildTypes_.GetSize();
+    if (rows > 0) {
+        std::vector<omni_gpu_column> cols(ncols);
+        for (int32_t c = 0; c < ncols; ++c) {
+            f_.retained_.emplace_back(new DeviceBuffer());
+            f_.retained_.emplace_back(new DeviceBuffer());
+            auto &vb = *f_.retained_[f_.retained_.size() - 2];
+            auto &nb = *f_.retained_.back();
+            cols[c] = Upload(vecBatch->Get(c), rows, vb, nb);
+        }
+        Check(omni_gpu_join_build_add(f_.handle_, &cols[f_.keyCol_], cols.data(), rows), "omni_gpu_join_build_add");
+    }
+    vec::VectorHelper::FreeVecBatch(vecBatch);
+    return 0;
+}
+
+// HashBuilderOperator::GetOutput (hash_builder.cpp:182-213): builds the table once input is done.
+int32_t GpuHashBuilderOperator::GetOutput(vec::VectorBatch **result)
+{
+    *result = nullptr;
+    if (!f_.finished_) {
+        Check(omni_gpu_join_build_finish(f_.handle_), "omni_gpu_join_build_finish");
+        f_.finished_ = true;
+        f_.retained_.clear();
+    }
+    SetStatus(OMNI_STATUS_FINISHED);
+    return 0;
+}
+
+// ================================================================================================
+// Hash join: lookup (inner)
+// ================================================================================================
+
+GpuLookupJoinOperatorFactory *GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+    const type::DataTypes &probeTypes, int32_t *probeOutputCols, int32_t probeOutputColsCount, int32_t *probeHashCols,
+    int32_t probeHashColsCount, int32_t *buildOutputCols, int32_t buildOutputColsCount,
+    const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr, void *filterExpr,
+    bool isShuffleExchangeBuildPlan, void *overflowConfig)
+{
+    if (filterExpr != nullptr) throw OmniException("UNSUPPORTED_ERROR", "join filters are not on the GPU path");
+    if (probeHashColsCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one join key column is supported");
+    auto *f = new GpuLookupJoinOperatorFactory();
+    f->probeTypes_ = probeTypes;
+    f->probeOutputCols_.assign(probeOutputCols, probeOutputCols + probeOutputColsCount);
+    f->probeKeyCol_ = probeHashCols[0];
+    f->buildOutputCols_.assign(buildOutputCols, buildOutputCols + buildOutputColsCount);
+    // the build side is shared by raw factory address (lookup_join.cpp:84-95)
+    f->builder_ = reinterpret_cast<GpuHashBuilderOperatorFactory *>(hashBuilderFactoryAddr);
+    for (int32_t i = 0; i < buildOutputColsCount; ++i) {
+        if (buildOutputTypes.GetType(i)->GetId() != f->builder_->BuildTypes().GetType(buildOutputCols[i])->GetId()) {
+            throw OmniException("INVALID_ARGUMENT", "build output type does not match the build column");
+        }
+    }
+    return f;
+}
+
+Operator *GpuLookupJoinOperatorFactory::CreateOperator() { return new GpuLookupJoinOperator(*this); }
+
+GpuLookupJoinOperator::~GpuLookupJoinOperator()
+{
+    if (probe_) vec::VectorHelper::FreeVecBatch(probe_);
+}
+
+int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
+{
+    if (!f_.builder_->Finished()) throw OmniException("OPERATOR_RUNTIME_ERROR", "probe before the build finished");
+    if (probe_) vec::VectorHelper::FreeVecBatch(probe_);
+    probe_ = vecBatch;
+    SetStatus(OMNI_STATUS_NORMAL);
+    int32_t rows = vecBatch->GetRowCount();
+    matches_ = emitted_ = 0;
+    if (rows <= 0) return 0;
+    int32_t is_array = 0;
+    int64_t max_dup = 0, slots = 0;
+    Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
+    int64_t cap = std::max<int64_t>(1, rows * std::max<int64_t>(1, max_dup));
+    omni_gpu_column key = Upload(vecBatch->Get(f_.probeKeyCol_), rows, keyBuf_, nullBuf_);
+    rowsBuf_.Reserve(sizeof(int64_t) * cap);
+    const auto &btypes = f_.builder_->BuildTypes();
+    int32_t nb = btypes.GetSize();
+    std::vector<omni_gpu_out_column> outs(nb);
+    while (payBufs_.size() < static_cast<size_t>(nb)) {
+        payBufs_.emplace_back(new DeviceBuffer());
+        payNullBufs_.emplace_back(new DeviceBuffer());
+    }
+    for (int32_t c = 0; c < nb; ++c) outs[c].type_id = btypes.GetType(c)->GetId();
+    for (int32_t col : f_.buildOutputCols_) {
+        payBufs_[col]->Reserve(static_cast<size_t>(cap) * type::TypeWidth(btypes.GetType(col)->GetId()));
+        payNullBufs_[col]->Reserve(4 * ((cap + 31) / 32));
+        outs[col].values = payBufs_[col]->Get();
+        outs[col].nulls = static_cast<uint8_t *>(payNullBufs_[col]->Get());
+    }
+    Check(omni_gpu_join_probe(f_.builder_->Table(), &key, rows, 0, static_cast<int64_t *>(rowsBuf_.Get()), outs.data(),
+                              cap, &matches_),
+          "omni_gpu_join_probe");
+    matchRows_.resize(matches_);
+    Check(omni_gpu_memcpy_d2h(matchRows_.data(), rowsBuf_.Get(), sizeof(int64_t) * matches_), "download rows");
+    matchPay_.assign(f_.buildOutputCols_.size(), {});
+    matchPayNulls_.assign(f_.buildOutputCols_.size(), {});
+    for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
+        int32_t col = f_.buildOutputCols_[i];
+        int w = type::TypeWidth(btypes.GetType(col)->GetId());
+        matchPay_[i].resize(static_cast<size_t>(matches_) * w);
+        matchPayNulls_[i].resize((matches_ + 7) / 8 + 1);
+        Check(omni_gpu_memcpy_d2h(matchPay_[i].data(), payBufs_[col]->Get(), matchPay_[i].size()), "download payload");
+        Check(omni_gpu_memcpy_d2h(matchPayNulls_[i].data(), payNullBufs_[col]->Get(), (matches_ + 7) / 8),
+              "download payload nulls");
+    }
+    return 0;
+}
+
+// Output batch = probe output columns (gathered by probe row) ++ build output columns
+// (LookupJoinOutputBuilder::BuildOutput, lookup_join.cpp:1924-1985), at most ceil(1 MiB / row size) rows.
+int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
+{
+    *result = nullptr;
+    if (!probe_) return 0;
+    if (emitted_ >= matches_) {
+        vec::VectorHelper::FreeVecBatch(probe_);
+        probe_ = nullptr;
+        if (noMoreInput_) SetStatus(OMNI_STATUS_FINISHED);
+        return 0;
+    }
+    const auto &btypes = f_.builder_->BuildTypes();
+    int32_t rowSize = 0;
+    for (int32_t c : f_.probeOutputCols_) rowSize += f_.probeTypes_.GetType(c)->GetId() ? type::TypeWidth(f_.probeTypes_.GetType(c)->GetId()) : 0;
+    for (int32_t c : f_.buildOutputCols_) rowSize += type::TypeWidth(btypes.GetType(c)->GetId());
+    int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(std::max(rowSize, 1)), matches_ - emitted_));
+    auto *out = new vec::VectorBatch(count);
+    for (int32_t c : f_.probeOutputCols_) {
+        const vec::BaseVector *src = probe_->Get(c);
+        auto *v = vec::VectorHelper::CreateFlatVector(src->GetTypeId(), count);
+        int w = src->Width();
+        for (int32_t r = 0; r < count; ++r) {
+            int64_t pr = matchRows_[emitted_ + r];
+            std::memcpy(v->RawValues() + static_cast<size_t>(w) * r, src->RawValues() + static_cast<size_t>(w) * pr, w);
+            if (src->IsNull(static_cast<int32_t>(pr))) v->SetNull(r);
+        }
+        out->Append(v);
+    }
+    for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
+        int32_t tid = btypes.GetType(f_.buildOutputCols_[i])->GetId();
+        auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
+        int w = v->Width();
+        std::memcpy(v->RawValues(), matchPay_[i].data() + static_cast<size_t>(w) * emitted_, static_cast<size_t>(w) * count);
+        for (int32_t r = 0; r < count; ++r) {
+            int64_t m = emitted_ + r;
+            if ((matchPayNulls_[i][m >> 3] >> (m & 7)) & 1) v->SetNull(r);
+        }
+        out->Append(v);
+    }
+    emitted_ += count;
+    *result = out;
+    return 0;
+}
+
+}  // namespace op
+}  // namespace omniruntime

@@ -1,0 +1,163 @@
+// GPU-backed drop-ins for the reference's hash-aggregation and inner hash-join operators.
+//
+// The factories keep the reference constructors' argument meaning, so a caller (JNI, the native driver,
+// tests) switches by constructing the Gpu* factory instead of the CPU one:
+//   HashAggregationOperatorFactory  core/src/operator/aggregation/group_aggregation.h:233-245
+//   HashBuilderOperatorFactory      core/src/operator/join/hash_builder.h:28-31
+//   LookupJoinOperatorFactory       core/src/operator/join/lookup_join.h:148-152
+// Operators follow the reference's ownership rules: aggregation consumes and frees each input batch in
+// AddInput; the builder takes its batches (their data moves to device memory); lookup keeps the current
+// probe batch until all its matches are emitted; the caller owns every output batch.
+// Errors reported by the C ABI become OmniException (the reference's error channel).
+#pragma once
+
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "omni_gpu.h"
+#include "omni_vector.h"
+
+namespace omniruntime {
+namespace op {
+
+// Owning device buffer allocated through the C ABI.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    ~DeviceBuffer();
+    DeviceBuffer(const DeviceBuffer &) = delete;
+    DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+    void Reserve(size_t bytes);  // contents not kept
+    void *Get() const { return ptr_; }
+
+private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Rows per output batch: ceil(1 MiB / row size) (core/src/operator/util/operator_util.h:111-115).
+int32_t MaxRowCount(int32_t rowSize);
+
+class GpuHashAggregationOperatorFactory : public OperatorFactory {
+public:
+    GpuHashAggregationOperatorFactory(std::vector<uint32_t> &groupByCol, const type::DataTypes &groupInputTypes,
+                                      std::vector<std::vector<uint32_t>> &aggsCols,
+                                      std::vector<type::DataTypes> &aggInputTypes,
+                                      std::vector<type::DataTypes> &aggOutputTypes, std::vector<uint32_t> &aggFuncTypes,
+                                      std::vector<uint32_t> &maskColsVector, std::vector<bool> inputRaws,
+                                      std::vector<bool> outputPartials, bool overflowAsNull = false,
+                                      bool isStatisticalAggregate = false);
+    OmniStatus Init();
+    Operator *CreateOperator() override;
+
+    // Learned capacity hint (expected groups) forwarded to every operator this factory creates.
+    void SetCapacityHint(int64_t groups) { spec_.capacity_hint = groups; }
+    void SetStrategy(int32_t strategy) { spec_.strategy = strategy; }
+
+private:
+    friend class GpuHashAggregationOperator;
+    std::vector<uint32_t> groupByCols_;
+    type::DataTypes groupTypes_;
+    std::vector<int32_t> aggInputCol_;  // -1 for COUNT_ALL
+    std::vector<type::DataTypePtr> aggOutTypes_;
+    omni_gpu_agg_spec spec_{};
+};
+
+class GpuHashAggregationOperator : public Operator {
+public:
+    explicit GpuHashAggregationOperator(const GpuHashAggregationOperatorFactory &f);
+    ~GpuHashAggregationOperator() override;
+    int32_t AddInput(vec::VectorBatch *vecBatch) override;
+    int32_t GetOutput(vec::VectorBatch **result) override;
+    OmniStatus Close() override;
+    uint64_t GetHashMapUniqueKeys() override;
+
+private:
+    std::vector<uint32_t> groupByCols_;
+    std::vector<int32_t> aggInputCol_;
+    std::vector<int32_t> outTypes_;
+    omni_gpu_agg_spec spec_;
+    void *handle_ = nullptr;
+    bool hasInput_ = false;
+    bool pending_ = false;  // a batch may still be reading the staging buffers
+    int64_t groups_ = -1;
+    int64_t emitted_ = 0;
+    int32_t rowsPerBatch_ = 0;
+    std::vector<std::unique_ptr<DeviceBuffer>> staging_;
+};
+
+class GpuHashBuilderOperatorFactory : public OperatorFactory {
+public:
+    static GpuHashBuilderOperatorFactory *CreateHashBuilderOperatorFactory(JoinType joinType,
+                                                                           const type::DataTypes &buildTypes,
+                                                                           const int32_t *buildHashCols,
+                                                                           int32_t buildHashColsCount,
+                                                                           int32_t operatorCount);
+    ~GpuHashBuilderOperatorFactory() override;
+    Operator *CreateOperator() override;
+    void *Table() const { return handle_; }
+    const type::DataTypes &BuildTypes() const { return buildTypes_; }
+    bool Finished() const { return finished_; }
+
+private:
+    friend class GpuHashBuilderOperator;
+    GpuHashBuilderOperatorFactory(const type::DataTypes &buildTypes, int32_t keyCol);
+    type::DataTypes buildTypes_;
+    int32_t keyCol_;
+    void *handle_ = nullptr;
+    bool finished_ = false;
+    std::vector<std::unique_ptr<DeviceBuffer>> retained_;  // build batches in device memory until finish
+};
+
+class GpuHashBuilderOperator : public Operator {
+public:
+    explicit GpuHashBuilderOperator(GpuHashBuilderOperatorFactory &f) : f_(f) {}
+    int32_t AddInput(vec::VectorBatch *vecBatch) override;
+    int32_t GetOutput(vec::VectorBatch **result) override;
+
+private:
+    GpuHashBuilderOperatorFactory &f_;
+};
+
+class GpuLookupJoinOperatorFactory : public OperatorFactory {
+public:
+    // filterExpr must be null (join filters are not on the GPU path); isShuffle and overflowConfig are
+    // accepted for signature compatibility.
+    static GpuLookupJoinOperatorFactory *CreateLookupJoinOperatorFactory(
+        const type::DataTypes &probeTypes, int32_t *probeOutputCols, int32_t probeOutputColsCount,
+        int32_t *probeHashCols, int32_t probeHashColsCount, int32_t *buildOutputCols, int32_t buildOutputColsCount,
+        const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr, void *filterExpr,
+        bool isShuffleExchangeBuildPlan, void *overflowConfig);
+    Operator *CreateOperator() override;
+
+private:
+    friend class GpuLookupJoinOperator;
+    type::DataTypes probeTypes_;
+    std::vector<int32_t> probeOutputCols_;
+    int32_t probeKeyCol_ = 0;
+    std::vector<int32_t> buildOutputCols_;
+    GpuHashBuilderOperatorFactory *builder_ = nullptr;
+};
+
+class GpuLookupJoinOperator : public Operator {
+public:
+    explicit GpuLookupJoinOperator(const GpuLookupJoinOperatorFactory &f) : f_(f) {}
+    ~GpuLookupJoinOperator() override;
+    int32_t AddInput(vec::VectorBatch *vecBatch) override;
+    int32_t GetOutput(vec::VectorBatch **result) override;
+
+private:
+    const GpuLookupJoinOperatorFactory &f_;
+    vec::VectorBatch *probe_ = nullptr;
+    std::vector<int64_t> matchRows_;
+    std::vector<std::vector<uint8_t>> matchPay_;       // per build output column: values
+    std::vector<std::vector<uint8_t>> matchPayNulls_;  // per build output column: bitmap
+    int64_t matches_ = 0;
+    int64_t emitted_ = 0;
+    DeviceBuffer keyBuf_, nullBuf_, rowsBuf_;
+    std::vector<std::unique_ptr<DeviceBuffer>> payBufs_, payNullBufs_;
+};
+
+}  // namespace op
+}  // namespace omniruntime

@@ -1,0 +1,315 @@
+// Operator-level tests of the GPU drop-ins, written like the reference's gtest cases
+// (core/test/operator/join_test.cpp, aggregation_test.cpp): build VectorBatches, push them through
+// AddInput, pull GetOutput, compare as row sets (VecBatchMatchIgnoreOrder semantics).
+// Exit status 0 when every case passes.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "gpu_operators.h"
+
+using namespace omniruntime;
+using namespace omniruntime::op;
+using namespace omniruntime::type;
+using namespace omniruntime::vec;
+
+static int g_failures = 0;
+#define EXPECT_TRUE(c)                                                               \
+    do {                                                                             \
+        if (!(c)) {                                                                  \
+            std::printf("  expectation failed: %s (%s:%d)\n", #c, __FILE__, __LINE__); \
+            ++g_failures;                                                            \
+        }                                                                            \
+    } while (0)
+
+// A cell rendered as text ("null" for NULL) so rows of mixed types compare as tuples.
+static std::string Cell(const BaseVector *v, int32_t r)
+{
+    if (v->IsNull(r)) return "null";
+    char buf[64];
+    switch (v->GetTypeId()) {
+        case OMNI_SHORT: std::snprintf(buf, sizeof buf, "%d", static_cast<const Vector<int16_t> *>(v)->GetValue(r)); break;
+        case OMNI_INT:
+        case OMNI_DATE32: std::snprintf(buf, sizeof buf, "%d", static_cast<const Vector<int32_t> *>(v)->GetValue(r)); break;
+        case OMNI_DOUBLE: std::snprintf(buf, sizeof buf, "%.12g", static_cast<const Vector<double> *>(v)->GetValue(r)); break;
+        default: std::snprintf(buf, sizeof buf, "%lld", static_cast<long long>(static_cast<const Vector<int64_t> *>(v)->GetValue(r)));
+    }
+    return buf;
+}
+
+using Rows = std::vector<std::vector<std::string>>;
+
+static void AppendRows(Rows &rows, const VectorBatch *b)
+{
+    for (int32_t r = 0; r < b->GetRowCount(); ++r) {
+        std::vector<std::string> row;
+        for (int32_t c = 0; c < b->GetVectorCount(); ++c) row.push_back(Cell(b->Get(c), r));
+        rows.push_back(row);
+    }
+}
+
+static bool SameRowSet(Rows a, Rows b)
+{
+    std::sort(a.begin(), a.end());
+    std::sort(b.begin(), b.end());
+    return a == b;
+}
+
+template <typename T> static Vector<T> *MakeVector(DataTypeId id, const std::vector<T> &vals)
+{
+    auto *v = new Vector<T>(static_cast<int32_t>(vals.size()), id);
+    for (size_t i = 0; i < vals.size(); ++i) v->SetValue(static_cast<int32_t>(i), vals[i]);
+    return v;
+}
+
+static Rows Drain(Operator *op)
+{
+    Rows rows;
+    for (int guard = 0; guard < 100000; ++guard) {
+        VectorBatch *out = nullptr;
+        op->GetOutput(&out);
+        if (!out) break;
+        AppendRows(rows, out);
+        VectorHelper::FreeVecBatch(out);
+    }
+    return rows;
+}
+
+// join_test.cpp:283-308 (TestInnerEqualityJoinWithOneBuildOp)
+static void TestInnerJoinOneBuildOp()
+{
+    DataTypes buildTypes({LongType(), LongType()});
+    auto *build = new VectorBatch(10);
+    build->Append(MakeVector<int64_t>(OMNI_LONG, {1, 2, 1, 2, 3, 4, 5, 6, 7, 1}));
+    build->Append(MakeVector<int64_t>(OMNI_LONG, {79, 79, 70, 70, 70, 70, 70, 70, 70, 70}));
+    int32_t hashCols[1] = {0};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(OMNI_JOIN_TYPE_INNER, buildTypes, hashCols, 1, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    EXPECT_TRUE(none == nullptr && builder->GetStatus() == OMNI_STATUS_FINISHED);
+
+    DataTypes probeTypes({LongType(), LongType()});
+    auto *probe = new VectorBatch(10);
+    probe->Append(MakeVector<int64_t>(OMNI_LONG, {1, 2, 3, 4, 5, 6, 1, 1, 2, 3}));
+    probe->Append(MakeVector<int64_t>(OMNI_LONG, {78, 78, 78, 78, 78, 78, 78, 82, 82, 65}));
+    int32_t probeOut[1] = {1}, probeHash[1] = {0}, buildOut[1] = {1};
+    DataTypes buildOutTypes({LongType()});
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        probeTypes, probeOut, 1, probeHash, 1, buildOut, 1, buildOutTypes, reinterpret_cast<int64_t>(bf), nullptr,
+        false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    Rows got = Drain(lookup);
+    std::vector<int64_t> e0 = {78, 78, 78, 78, 78, 78, 78, 82, 82, 65};
+    std::vector<int> ids = {0, 0, 0, 1, 1, 2, 3, 4, 5, 6, 6, 6, 7, 7, 7, 8, 8, 9};
+    std::vector<int64_t> e1 = {79, 70, 70, 79, 70, 70, 70, 70, 70, 79, 70, 70, 79, 70, 70, 79, 70, 70};
+    Rows want;
+    for (size_t i = 0; i < ids.size(); ++i) want.push_back({std::to_string(e0[ids[i]]), std::to_string(e1[i])});
+    EXPECT_TRUE(got.size() == 18);
+    EXPECT_TRUE(SameRowSet(got, want));
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete lf;
+    delete bf;
+}
+
+// join_test.cpp:4265-4321 (TestInnerEqualityJoinOfArrayTable)
+static void TestInnerJoinArrayTable()
+{
+    DataTypes types({ShortType(), ShortType()});
+    auto *build = new VectorBatch(6);
+    build->Append(MakeVector<int16_t>(OMNI_SHORT, {1, 2, 3, 4, 1, 2}));
+    build->Append(MakeVector<int16_t>(OMNI_SHORT, {111, 11, 333, 33, 20, 16}));
+    int32_t hashCols[1] = {0};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(OMNI_JOIN_TYPE_INNER, types, hashCols, 1, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    auto *probe = new VectorBatch(4);
+    probe->Append(MakeVector<int16_t>(OMNI_SHORT, {1, 2, 3, 4}));
+    probe->Append(MakeVector<int16_t>(OMNI_SHORT, {11, 22, 33, 44}));
+    int32_t probeOut[2] = {0, 1}, probeHash[1] = {0}, buildOut[2] = {0, 1};
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        types, probeOut, 2, probeHash, 1, buildOut, 2, types, reinterpret_cast<int64_t>(bf), nullptr, false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    Rows got = Drain(lookup);
+    Rows want = {{"1", "11", "1", "111"}, {"1", "11", "1", "20"}, {"2", "22", "2", "11"},
+                 {"2", "22", "2", "16"},  {"3", "33", "3", "333"}, {"4", "44", "4", "33"}};
+    EXPECT_TRUE(SameRowSet(got, want));
+    int32_t is_array = 0;
+    int64_t dup = 0, slots = 0;
+    omni_gpu_join_build_info(bf->Table(), &is_array, &dup, &slots);
+    EXPECT_TRUE(is_array == 1 && dup == 2);
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete lf;
+    delete bf;
+}
+
+static GpuHashAggregationOperatorFactory *StdAggFactory(std::vector<DataTypePtr> groupTypes, uint32_t firstAggCol)
+{
+    std::vector<uint32_t> groupCols;
+    for (uint32_t i = 0; i < groupTypes.size(); ++i) groupCols.push_back(i);
+    std::vector<std::vector<uint32_t>> aggCols;
+    std::vector<DataTypes> inTypes, outTypes;
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM, OMNI_AGG_AVG, OMNI_AGG_COUNT_COLUMN, OMNI_AGG_MIN, OMNI_AGG_MAX};
+    std::vector<DataTypePtr> outs = {LongType(), DoubleType(), LongType(), LongType(), LongType()};
+    for (uint32_t i = 0; i < 5; ++i) {
+        aggCols.push_back({firstAggCol + i});
+        inTypes.push_back(DataTypes({LongType()}));
+        outTypes.push_back(DataTypes({outs[i]}));
+    }
+    std::vector<uint32_t> masks;
+    return new GpuHashAggregationOperatorFactory(groupCols, DataTypes(groupTypes), aggCols, inTypes, outTypes, funcs, masks,
+                                                 std::vector<bool>(5, true), std::vector<bool>(5, false), false);
+}
+
+// aggregation_test.cpp:544-630 (verify_correctness), single stage
+static void TestAggVerifyCorrectness()
+{
+    auto *f = StdAggFactory({LongType(), LongType()}, 2);
+    Operator *op = f->CreateOperator();
+    for (int b = 0; b < 20; ++b) {
+        auto *batch = new VectorBatch(2000);
+        std::vector<int64_t> k(2000), one(2000, 1);
+        for (int j = 0; j < 2000; ++j) k[j] = j % 10;
+        batch->Append(MakeVector<int64_t>(OMNI_LONG, k));
+        batch->Append(MakeVector<int64_t>(OMNI_LONG, k));
+        for (int a = 0; a < 5; ++a) batch->Append(MakeVector<int64_t>(OMNI_LONG, one));
+        op->AddInput(batch);
+    }
+    EXPECT_TRUE(op->GetHashMapUniqueKeys() == 10);
+    Rows got = Drain(op);
+    Rows want;
+    for (int k = 0; k < 10; ++k) want.push_back({std::to_string(k), std::to_string(k), "4000", "1", "4000", "1", "1"});
+    EXPECT_TRUE(SameRowSet(got, want));
+    EXPECT_TRUE(op->GetStatus() == OMNI_STATUS_FINISHED);
+    Operator::DeleteOperator(op);
+    delete f;
+}
+
+// aggregation_test.cpp:632-740 (NULL keys form their own group), LONG key
+static void TestAggNullKeys()
+{
+    auto *f = StdAggFactory({LongType()}, 1);
+    Operator *op = f->CreateOperator();
+    std::vector<std::vector<int>> keyNulls = {{0, 11}, {0, 11, 20}};
+    for (int b = 0; b < 2; ++b) {
+        auto *batch = new VectorBatch(21);
+        std::vector<int64_t> k(21), one(21, 1);
+        for (int j = 0; j < 21; ++j) k[j] = j % 3;
+        auto *kv = MakeVector<int64_t>(OMNI_LONG, k);
+        for (int r : keyNulls[b]) kv->SetNull(r);
+        batch->Append(kv);
+        for (int a = 0; a < 5; ++a) {
+            auto *v = MakeVector<int64_t>(OMNI_LONG, one);
+            if (a == 1)
+                for (int r = 0; r < 5; ++r) v->SetNull(r);
+            if (a >= 2) {
+                v->SetNull(0);
+                v->SetNull(1);
+            }
+            batch->Append(v);
+        }
+        op->AddInput(batch);
+    }
+    Rows got = Drain(op);
+    Rows want = {{"null", "5", "1", "3", "1", "1"}, {"0", "12", "1", "12", "1", "1"},
+                 {"1", "14", "1", "12", "1", "1"}, {"2", "11", "1", "11", "1", "1"}};
+    EXPECT_TRUE(SameRowSet(got, want));
+    Operator::DeleteOperator(op);
+    delete f;
+}
+
+// Output is cut into batches of ceil(1 MiB / row size) rows (operator_util.h:111-115).
+static void TestAggOutputBatching()
+{
+    auto *f = StdAggFactory({LongType()}, 1);
+    Operator *op = f->CreateOperator();
+    const int n = 100000;
+    auto *batch = new VectorBatch(n);
+    std::vector<int64_t> k(n), v(n);
+    for (int j = 0; j < n; ++j) {
+        k[j] = static_cast<int64_t>(j) * 2654435761LL;
+        v[j] = j;
+    }
+    batch->Append(MakeVector<int64_t>(OMNI_LONG, k));
+    for (int a = 0; a < 5; ++a) batch->Append(MakeVector<int64_t>(OMNI_LONG, v));
+    op->AddInput(batch);
+    int batches = 0;
+    int64_t rows = 0, sum = 0;
+    while (true) {
+        VectorBatch *out = nullptr;
+        op->GetOutput(&out);
+        if (!out) break;
+        ++batches;
+        EXPECT_TRUE(out->GetRowCount() <= MaxRowCount(48));
+        rows += out->GetRowCount();
+        for (int r = 0; r < out->GetRowCount(); ++r) sum += static_cast<Vector<int64_t> *>(out->Get(1))->GetValue(r);
+        VectorHelper::FreeVecBatch(out);
+    }
+    EXPECT_TRUE(rows == n);
+    EXPECT_TRUE(batches == (n + MaxRowCount(48) - 1) / MaxRowCount(48));
+    EXPECT_TRUE(sum == static_cast<int64_t>(n) * (n - 1) / 2);
+    Operator::DeleteOperator(op);
+    delete f;
+}
+
+// A decimal SUM overflow raises OmniException when overflowAsNull is false.
+static void TestAggDecimalOverflowThrows()
+{
+    std::vector<uint32_t> groupCols = {0};
+    std::vector<std::vector<uint32_t>> aggCols = {{1}};
+    std::vector<DataTypes> inTypes = {DataTypes({Decimal64Type(18, 0)})}, outTypes = {DataTypes({Decimal64Type(18, 0)})};
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM}, masks;
+    GpuHashAggregationOperatorFactory f(groupCols, DataTypes({LongType()}), aggCols, inTypes, outTypes, funcs, masks,
+                                        {true}, {false}, false);
+    Operator *op = f.CreateOperator();
+    auto *batch = new VectorBatch(2);
+    batch->Append(MakeVector<int64_t>(OMNI_LONG, {1, 1}));
+    batch->Append(MakeVector<int64_t>(OMNI_DECIMAL64, {INT64_MAX, 1}));
+    op->AddInput(batch);
+    bool threw = false;
+    try {
+        Drain(op);
+    } catch (const OmniException &e) {
+        threw = e.Code() == "OPERATOR_RUNTIME_ERROR";
+    }
+    EXPECT_TRUE(threw);
+    Operator::DeleteOperator(op);
+}
+
+int main()
+{
+    std::vector<std::pair<const char *, std::function<void()>>> tests = {
+        {"InnerJoinOneBuildOp", TestInnerJoinOneBuildOp},
+        {"InnerJoinArrayTable", TestInnerJoinArrayTable},
+        {"AggVerifyCorrectness", TestAggVerifyCorrectness},
+        {"AggNullKeys", TestAggNullKeys},
+        {"AggOutputBatching", TestAggOutputBatching},
+        {"AggDecimalOverflowThrows", TestAggDecimalOverflowThrows},
+    };
+    for (auto &t : tests) {
+        int before = g_failures;
+        std::printf("[ RUN  ] %s\n", t.first);
+        std::fflush(stdout);
+        try {
+            t.second();
+        } catch (const std::exception &e) {
+            std::printf("  exception: %s\n", e.what());
+            ++g_failures;
+        }
+        std::printf("[ %s ] %s\n", g_failures == before ? " OK " : "FAIL", t.first);
+        std::fflush(stdout);
+    }
+    std::printf("%d failure(s)\n", g_failures);
+    return g_failures == 0 ? 0 : 1;
+}

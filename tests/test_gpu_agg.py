@@ -173,3 +173,27 @@ def test_skewed_agg_reduced_size():
     got = run_gpu(keys, aggs, cols, 65536)
     want = run_oracle(keys, aggs, cols, 65536)
     assert_row_sets_equal(got, want, float_cols=float_cols(2, aggs), rtol=DOUBLE_RTOL)
+
+
+@pytest.mark.parametrize("strategy", [capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL])
+@pytest.mark.parametrize("key_kind", ["long", "int", "double"])
+@pytest.mark.parametrize("with_nulls", [False, True])
+def test_strategy_parity(strategy, key_kind, with_nulls):
+    """Every strategy gives the oracle's results (LDS-local partial tables merge into the global table)."""
+    keys, aggs, cols = _random_case(hash((key_kind, with_nulls, "strategy")) % 2**32, 60000, key_kind, with_nulls)
+    got = run_gpu(keys, aggs, cols, 20000, strategy=strategy)
+    want = run_oracle(keys, aggs, cols, 20000)
+    assert_row_sets_equal(got, want, float_cols=float_cols(len(keys), aggs), rtol=DOUBLE_RTOL)
+
+
+def test_lds_strategy_spills_when_groups_exceed_lds():
+    """More distinct keys than LDS slots: rows that find no LDS slot go to the global table."""
+    rng = np.random.default_rng(9)
+    n = 400_000
+    k = rng.integers(0, 50_000, n, dtype=np.int64)
+    v = rng.integers(-10**12, 10**12, n, dtype=np.int64)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MAX, LONG, LONG), (SUM, DECIMAL64, DECIMAL128, 0, 0)]
+    cols = [Col(LONG, v), None, Col(LONG, v), Col(DECIMAL64, v)]
+    got = run_gpu([Col(LONG, k)], aggs, cols, 100_000, strategy=capi.STRATEGY_LDS)
+    want = run_oracle([Col(LONG, k)], aggs, cols, 100_000)
+    assert_row_sets_equal(got, want)
