@@ -146,30 +146,30 @@ int omni_gpu_agg_destroy(void *handle);
 
 /* ---- batched stream of independent group-bys ----------------------------------------------------------
  * Replaces one HashAggregationOperator per VectorBatch (AddInput + GetOutput per batch,
- * group_aggregation.cpp:320-406, 454-474) for a stream of many small, uneven batches: the whole stream is
- * aggregated by ONE persistent launch that drains a device queue of batches (each batch in an LDS table,
- * spilling exactly to a per-workgroup scratch table in HBM when its keys do not fit).  No reference
- * counterpart exists for the batching itself; per-batch results equal the reference operator's.
- * Single key column of <= 8 bytes; any aggregate the per-batch path supports. */
-int omni_gpu_agg_stream_create(const omni_gpu_agg_spec *spec, int64_t max_batch_rows, void **handle);
+ * group_aggregation.cpp:320-406, 454-474) for a stream of many small, uneven batches resident in HBM.
+ * Two launches cover all batches: phase 1 aggregates each batch in an LDS table (one workgroup per batch)
+ * and abandons the batches whose keys do not fit; phase 2 aggregates those exactly in private HBM tables.
+ * No reference counterpart exists for the batching itself; per-batch results equal the reference operator's.
+ * One key column of <= 8 bytes; any aggregate the per-batch path supports. */
+int omni_gpu_agg_batches_create(const omni_gpu_agg_spec *spec, int64_t max_batch_rows, void **handle);
 /* Launch tuning (the autotuner's knobs): threads per workgroup (256 or 512), LDS table bytes
- * ([8 KiB, 64 KiB]), LDS probe limit before spilling, resident workgroups per CU (0 = occupancy limit). */
-int omni_gpu_agg_stream_tune(void *handle, int32_t block_threads, int32_t lds_bytes, int32_t max_probe,
-                             int32_t wgs_per_cu);
-int omni_gpu_agg_stream_info(void *handle, int32_t *lds_slots, int32_t *workgroups, int64_t *scratch_bytes);
+ * ([8 KiB, 64 KiB]) and the LDS probe limit before a batch is handed to phase 2. */
+int omni_gpu_agg_batches_tune(void *handle, int32_t block_threads, int32_t lds_bytes, int32_t max_probe);
+int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_bytes);
 /* Runs the stream.  keys[0] / aggs[i] are the concatenated device columns of all batches; batch b owns rows
- * [batch_offsets[b], batch_offsets[b+1]) (host array, batch_offsets[0] == 0).  batch_modes (host, optional):
- * 0 = LDS table first, 2 = scratch table directly (e.g. from a predicted group count).  flags: bit 0 = one
- * launch per batch instead of the persistent launch; bit 1 = capture the steady state in a hipGraph and
- * replay it while pointers and sizes stay the same.  Batch b's groups are written to output rows
- * [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key, then aggregates); outs hold
- * batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.  Synchronizes. */
-int omni_gpu_agg_stream_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
-                            const int64_t *batch_offsets, int32_t num_batches, const int32_t *batch_modes,
-                            int32_t flags, omni_gpu_out_column *outs, int64_t *groups_per_batch);
-/* Batches that stayed in LDS / used the scratch table since the last call. */
-int omni_gpu_agg_stream_stats(void *handle, int64_t *lds_batches, int64_t *scratch_batches);
-int omni_gpu_agg_stream_destroy(void *handle);
+ * [batch_offsets[b], batch_offsets[b+1]) (host array, batch_offsets[0] == 0, at most max_batch_rows rows
+ * each).  batch_modes (host, optional): 0 = LDS table first, 2 = HBM table directly (e.g. from a predicted
+ * group count).  flags: bit 0 = one launch per batch and phase instead of one launch per phase; bit 1 =
+ * capture the launch sequence in a hipGraph and replay it while the stream's shape stays the same.  Batch
+ * b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
+ * then aggregates); outs hold batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.
+ * Synchronizes. */
+int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
+                             const int64_t *batch_offsets, int32_t num_batches, const int32_t *batch_modes,
+                             int32_t flags, omni_gpu_out_column *outs, int64_t *groups_per_batch);
+/* Batches finished in LDS / in HBM tables since the last call. */
+int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_batches);
+int omni_gpu_agg_batches_destroy(void *handle);
 
 /* ---- hash join (inner) --------------------------------------------------------------------------
  * Replaces HashBuilderOperator + JoinHashTableVariants (core/src/operator/join/hash_builder.cpp:169-213,

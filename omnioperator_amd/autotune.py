@@ -1,0 +1,101 @@
+"""Autotuner for the batched group-by: picks threads per workgroup, LDS table bytes and the LDS probe limit by
+timing candidate runs on the stream itself, per (key type, aggregate signature, cardinality bucket); winners
+are cached in a JSON file (OMNI_GPU_AUTOTUNE_CACHE, default ~/.cache/omnioperator_amd/autotune.json) so later
+runs of the same kind skip the sweep.
+
+Tuning never changes results: every candidate is an exact configuration of the same kernels (a batch that
+does not fit the LDS table is aggregated exactly in HBM).
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+from dataclasses import asdict, dataclass
+from typing import Dict, Iterable, List, Optional
+
+import numpy as np
+
+CACHE = os.environ.get("OMNI_GPU_AUTOTUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "omnioperator_amd", "autotune.json"))
+
+
+@dataclass(frozen=True)
+class StreamConfig:
+    block: int = 256
+    lds_bytes: int = 32768
+    max_probe: int = 32
+
+
+DEFAULT = StreamConfig()
+
+CANDIDATES: List[StreamConfig] = [
+    StreamConfig(b, l, p)
+    for b in (256, 512)
+    for l in (16384, 32768, 65536)
+    for p in (8, 32)
+]
+
+
+def cardinality_bucket(mean_distinct: float) -> int:
+    """log4 bucket of the mean per-batch distinct count."""
+    return int(math.floor(math.log(max(1.0, mean_distinct), 4)))
+
+
+def key_of(key_type: int, aggs: Iterable[tuple], bucket: int) -> str:
+    sig = ",".join(f"{a[0]}:{a[1]}:{a[2]}" for a in aggs)
+    return f"k{key_type}|{sig}|c{bucket}"
+
+
+def _load() -> Dict[str, dict]:
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(d: Dict[str, dict]):
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        tmp = CACHE + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, CACHE)
+    except OSError:
+        pass
+
+
+def lookup(key: str) -> Optional[StreamConfig]:
+    e = _load().get(key)
+    return StreamConfig(**e["config"]) if e else None
+
+
+def tune_stream(stream, run, key: str, candidates: List[StreamConfig] = CANDIDATES, repeats: int = 2,
+                use_cache: bool = True) -> StreamConfig:
+    """stream: capi.AggBatches; run(): runs the stream once (synchronizing).  Returns the fastest config and
+    leaves the stream tuned to it."""
+    if use_cache:
+        hit = lookup(key)
+        if hit is not None:
+            stream.tune(**asdict(hit))
+            return hit
+    best, best_t, results = None, float("inf"), []
+    for cfg in candidates:
+        stream.tune(**asdict(cfg))
+        run()  # warm: first launch of this configuration
+        ts = []
+        for _ in range(repeats):
+            t0 = time.perf_counter()
+            run()
+            ts.append(time.perf_counter() - t0)
+        t = float(np.median(ts))
+        results.append((asdict(cfg), t))
+        if t < best_t:
+            best, best_t = cfg, t
+    stream.tune(**asdict(best))
+    d = _load()
+    d[key] = {"config": asdict(best), "seconds": best_t, "swept": results}
+    _store(d)
+    return best

@@ -80,13 +80,13 @@ SIGNATURES = {
     "omni_gpu_agg_num_groups": (_I, [_VP, _P(_I64)]),
     "omni_gpu_agg_get_output": (_I, [_VP, _I64, _I64, _P(OutColumn)]),
     "omni_gpu_agg_destroy": (_I, [_VP]),
-    "omni_gpu_agg_stream_create": (_I, [_P(AggSpec), _I64, _P(_VP)]),
-    "omni_gpu_agg_stream_tune": (_I, [_VP, _I32, _I32, _I32, _I32]),
-    "omni_gpu_agg_stream_info": (_I, [_VP, _P(_I32), _P(_I32), _P(_I64)]),
-    "omni_gpu_agg_stream_run": (_I, [_VP, _P(Column), _P(Column), _P(_I64), _I32, _VP, _I32, _P(OutColumn),
-                                     _P(_I64)]),
-    "omni_gpu_agg_stream_stats": (_I, [_VP, _P(_I64), _P(_I64)]),
-    "omni_gpu_agg_stream_destroy": (_I, [_VP]),
+    "omni_gpu_agg_batches_create": (_I, [_P(AggSpec), _I64, _P(_VP)]),
+    "omni_gpu_agg_batches_tune": (_I, [_VP, _I32, _I32, _I32]),
+    "omni_gpu_agg_batches_info": (_I, [_VP, _P(_I32), _P(_I64)]),
+    "omni_gpu_agg_batches_run": (_I, [_VP, _P(Column), _P(Column), _P(_I64), _I32, _VP, _I32, _P(OutColumn),
+                                      _P(_I64)]),
+    "omni_gpu_agg_batches_stats": (_I, [_VP, _P(_I64), _P(_I64)]),
+    "omni_gpu_agg_batches_destroy": (_I, [_VP]),
     "omni_gpu_join_build_create": (_I, [_P(JoinSpec), _P(_VP)]),
     "omni_gpu_join_build_add": (_I, [_VP, _P(Column), _P(Column), _I64]),
     "omni_gpu_join_build_finish": (_I, [_VP]),
@@ -273,8 +273,8 @@ class HashAgg:
             pass
 
 
-class AggStream:
-    """Batched stream of independent group-bys (omni_gpu_agg_stream_*)."""
+class AggBatches:
+    """Batched stream of independent group-bys (omni_gpu_agg_batches_*)."""
 
     FLAG_PER_BATCH, FLAG_GRAPH = 1, 2
 
@@ -282,17 +282,17 @@ class AggStream:
         init()
         self.spec = spec
         h = C.c_void_p()
-        check(lib().omni_gpu_agg_stream_create(C.byref(spec), max_batch_rows, C.byref(h)))
+        check(lib().omni_gpu_agg_batches_create(C.byref(spec), max_batch_rows, C.byref(h)))
         self.h = h.value
         self._outs = None
 
-    def tune(self, block: int = 256, lds_bytes: int = 32768, max_probe: int = 32, wgs_per_cu: int = 0):
-        check(lib().omni_gpu_agg_stream_tune(self.h, block, lds_bytes, max_probe, wgs_per_cu))
+    def tune(self, block: int = 256, lds_bytes: int = 32768, max_probe: int = 32):
+        check(lib().omni_gpu_agg_batches_tune(self.h, block, lds_bytes, max_probe))
 
     def info(self):
-        s, w, b = C.c_int32(), C.c_int32(), C.c_int64()
-        check(lib().omni_gpu_agg_stream_info(self.h, C.byref(s), C.byref(w), C.byref(b)))
-        return {"lds_slots": s.value, "workgroups": w.value, "scratch_bytes": b.value}
+        s, p = C.c_int32(), C.c_int64()
+        check(lib().omni_gpu_agg_batches_info(self.h, C.byref(s), C.byref(p)))
+        return {"lds_slots": s.value, "pool_bytes": p.value}
 
     def alloc_outputs(self, total_rows: int):
         """Device output columns (key, then one per aggregate) for a stream of total_rows rows."""
@@ -308,7 +308,7 @@ class AggStream:
         return self._outs
 
     def run(self, keys: Column, aggs: Sequence[Optional[Column]], offsets: np.ndarray, modes=None, flags: int = 0):
-        """Runs every batch; returns groups per batch (numpy int64).  Outputs stay on the device (see outputs())."""
+        """Runs every batch; returns groups per batch (numpy int64).  Outputs stay on the device (outputs())."""
         offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         nb = len(offsets) - 1
         total = int(offsets[-1])
@@ -324,23 +324,23 @@ class AggStream:
             modes = np.ascontiguousarray(modes, dtype=np.int32)
             mp = modes.ctypes.data
         karr = (Column * 1)(keys)
-        check(lib().omni_gpu_agg_stream_run(self.h, karr, aarr, offsets.ctypes.data_as(C.POINTER(C.c_int64)), nb, mp,
-                                            flags, self._outs[2], groups.ctypes.data_as(C.POINTER(C.c_int64))))
+        check(lib().omni_gpu_agg_batches_run(self.h, karr, aarr, offsets.ctypes.data_as(C.POINTER(C.c_int64)), nb, mp,
+                                             flags, self._outs[2], groups.ctypes.data_as(C.POINTER(C.c_int64))))
         return groups
 
     def outputs(self):
-        """[(values, null mask)] of the whole output buffer (rows beyond each batch's group count are junk)."""
+        """[(values, null mask)] of the whole output buffer (rows past each batch's group count are junk)."""
         types, bufs, _, total = self._outs
         return [(v.numpy(), unpack_nulls(nb.numpy(), total)) for v, nb in bufs]
 
     def stats(self):
         a, b = C.c_int64(), C.c_int64()
-        check(lib().omni_gpu_agg_stream_stats(self.h, C.byref(a), C.byref(b)))
-        return {"lds_batches": a.value, "scratch_batches": b.value}
+        check(lib().omni_gpu_agg_batches_stats(self.h, C.byref(a), C.byref(b)))
+        return {"lds_batches": a.value, "hbm_batches": b.value}
 
     def close(self):
         if getattr(self, "h", None):
-            lib().omni_gpu_agg_stream_destroy(self.h)
+            lib().omni_gpu_agg_batches_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -397,6 +397,54 @@ __host__ __device__ __forceinline__ unsigned long long InitWord(int kind)
 
 __host__ __device__ __forceinline__ bool Is32(int kind) { return kind == K_MIN_I32 || kind == K_MAX_I32; }
 
+// Per-slot layout of an HBM table with the LDS-table structure (one u64 key array, then each aggregate's
+// state arrays and flag bytes): offsets and `bytes` count bytes per slot.  In a table whose arrays each hold
+// `stride` slots, array j starts at offset × stride, 256-byte aligned when stride is a multiple of 256.
+inline LdsLayout UnitLayout(const AggArgs &args)
+{
+    LdsLayout U{};
+    U.S = 1;
+    int off = 8;  // keys at 0
+    U.keys_off = 0;
+    for (int i = 0; i < args.n; ++i) {
+        int k = args.a[i].kind;
+        U.off[i][0] = off;
+        off += Is32(k) ? 4 : 8;
+        U.off[i][1] = -1;
+        U.off[i][2] = -1;
+        U.flag_off[i] = -1;
+        if (k == K_AVG || k == K_SUM_DEC) {
+            U.off[i][1] = off;
+            off += 8;
+        }
+        if (k == K_SUM_DEC) {
+            U.off[i][2] = off;
+            off += 8;
+        }
+        if (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) {
+            U.flag_off[i] = off;
+            off += 1;
+        }
+    }
+    U.bytes = off;
+    return U;
+}
+
+// State addressing of one slot of a table laid out by UnitLayout with `stride` slots per array.
+struct ScaledRef {
+    uint8_t *base;
+    const LdsLayout &U;
+    long long stride;
+    int slot;
+    template <typename T> __device__ __forceinline__ T *p(int i, int w) const
+    {
+        return reinterpret_cast<T *>(base + static_cast<long long>(U.off[i][w]) * stride) + slot;
+    }
+    __device__ __forceinline__ uint8_t *flag(int i) const
+    {
+        return base + static_cast<long long>(U.flag_off[i]) * stride + slot;
+    }
+};
 
 struct OutDev {
     int width;

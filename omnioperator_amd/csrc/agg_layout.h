@@ -1,0 +1,102 @@
+// Host-side helpers that map an aggregation spec onto device descriptors and table layouts.
+#pragma once
+
+#include "agg_device.h"
+
+namespace omni_gpu {
+
+// Maps aggregate i of the spec onto a device aggregate kind, rejecting combinations the GPU path lacks.
+inline int ClassifyAgg(const omni_gpu_agg_spec &sp, int i, AggDev &d)
+{
+    int f = sp.agg_funcs[i], ti = sp.agg_in_types[i], to = sp.agg_out_types[i];
+    d.in_width = TypeWidth(ti);
+    d.in_double = ti == OMNI_TYPE_DOUBLE;
+    d.out_width = TypeWidth(to);
+    d.dec_scale_up = 0;
+    d.dec_out128 = 0;
+    bool intIn = IsIntegerKeyType(ti) && ti != OMNI_TYPE_DECIMAL64;
+    switch (f) {
+        case OMNI_AGG_COUNT_ALL:
+            OG_CHECK(to == OMNI_TYPE_LONG, OMNI_GPU_ERR_INVALID, "COUNT output must be LONG");
+            d.kind = K_COUNT_ALL;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_COUNT_COLUMN:
+            OG_CHECK(to == OMNI_TYPE_LONG, OMNI_GPU_ERR_INVALID, "COUNT output must be LONG");
+            d.kind = K_COUNT_COL;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_SUM:
+            if (ti == OMNI_TYPE_DECIMAL64 || ti == OMNI_TYPE_DECIMAL128) {
+                OG_CHECK(to == OMNI_TYPE_DECIMAL64 || to == OMNI_TYPE_DECIMAL128, OMNI_GPU_ERR_INVALID,
+                         "decimal SUM output must be decimal");
+                OG_CHECK(sp.agg_out_scales[i] >= sp.agg_in_scales[i] && sp.agg_out_scales[i] - sp.agg_in_scales[i] <= 38,
+                         OMNI_GPU_ERR_INVALID, "decimal SUM output scale must be >= input scale");
+                d.kind = K_SUM_DEC;
+                d.dec_scale_up = sp.agg_out_scales[i] - sp.agg_in_scales[i];
+                d.dec_out128 = to == OMNI_TYPE_DECIMAL128;
+                return OMNI_GPU_OK;
+            }
+            if (ti == OMNI_TYPE_DOUBLE) {
+                OG_CHECK(to == OMNI_TYPE_DOUBLE, OMNI_GPU_ERR_INVALID, "SUM(double) output must be DOUBLE");
+                d.kind = K_SUM_F64;
+                return OMNI_GPU_OK;
+            }
+            OG_CHECK(intIn && to == OMNI_TYPE_LONG, OMNI_GPU_ERR_INVALID, "SUM(int) output must be LONG");
+            d.kind = K_SUM_I64;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_AVG:
+            OG_CHECK((intIn || ti == OMNI_TYPE_DOUBLE) && to == OMNI_TYPE_DOUBLE, OMNI_GPU_ERR_INVALID,
+                     "AVG supports integer/double input with DOUBLE output");
+            d.kind = K_AVG;
+            return OMNI_GPU_OK;
+        case OMNI_AGG_MIN:
+        case OMNI_AGG_MAX: {
+            bool mn = f == OMNI_AGG_MIN;
+            OG_CHECK(ti == to, OMNI_GPU_ERR_INVALID, "MIN/MAX output type must equal input type");
+            if (ti == OMNI_TYPE_SHORT || ti == OMNI_TYPE_INT || ti == OMNI_TYPE_DATE32) d.kind = mn ? K_MIN_I32 : K_MAX_I32;
+            else if (ti == OMNI_TYPE_LONG || ti == OMNI_TYPE_TIMESTAMP || ti == OMNI_TYPE_DECIMAL64) d.kind = mn ? K_MIN_I64 : K_MAX_I64;
+            else if (ti == OMNI_TYPE_DOUBLE) d.kind = mn ? K_MIN_F64 : K_MAX_F64;
+            else return SetError(OMNI_GPU_ERR_INVALID, "MIN/MAX over type %d is not supported on the GPU path", ti);
+            return OMNI_GPU_OK;
+        }
+        default: return SetError(OMNI_GPU_ERR_INVALID, "aggregate function %d is not supported", f);
+    }
+}
+
+
+// LDS table layout for the aggregates of a handle within `budget` bytes: per slot one u64 key plus each
+// aggregate's state words (4 bytes for 32-bit MIN/MAX) and a flag byte where the aggregate needs one.
+inline LdsLayout MakeLdsLayout(const AggArgs &args, int budget)
+{
+    LdsLayout L{};
+    int perSlot = 8;
+    for (int i = 0; i < args.n; ++i) {
+        int k = args.a[i].kind;
+        perSlot += Is32(k) ? 4 : 8;
+        if (k == K_AVG || k == K_SUM_DEC) perSlot += 8;
+        if (k == K_SUM_DEC) perSlot += 8;
+        if (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) perSlot += 1;
+    }
+    int S = (budget - 64 * (2 * OMNI_GPU_MAX_AGGS + 1)) / perSlot;  // room for 8-byte alignment padding
+    S = S - S % 64;
+    L.S = S;
+    int off = 0;
+    auto take = [&](int bytes) {
+        int o = off;
+        off += (bytes + 7) & ~7;
+        return o;
+    };
+    L.keys_off = take(8 * S);
+    for (int i = 0; i < args.n; ++i) {
+        int k = args.a[i].kind;
+        L.off[i][0] = take((Is32(k) ? 4 : 8) * S);
+        L.off[i][1] = (k == K_AVG || k == K_SUM_DEC) ? take(8 * S) : -1;
+        L.off[i][2] = k == K_SUM_DEC ? take(8 * S) : -1;
+        L.flag_off[i] = (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) ? take(S) : -1;
+    }
+    L.bytes = off;
+    return L;
+}
+
+
+
+}  // namespace omni_gpu

@@ -1,0 +1,246 @@
+// Device code of the batched group-by: many independent group-bys (one per VectorBatch) in two launches.
+//
+// Phase 1 (one workgroup per batch): the batch is aggregated in an LDS table; if any row finds no LDS slot
+// within the probe limit, the workgroup abandons the batch (writes nothing) and flags it.
+// Phase 2 (one workgroup per batch, flagged batches only): the batch is aggregated exactly in a private HBM
+// table of next_pow2(2 * rows) slots (plus 2 side slots) carved from a region pool at a scanned offset.
+// Both phases emit the batch's groups to output rows [off[b], off[b] + groups[b]).
+// The configuration lives in device memory and is read through one pointer (no large kernel arguments).
+#pragma once
+
+#include "agg_device.h"
+
+namespace omni_gpu {
+
+struct BatchConfig {
+    KeyDev kd;
+    AggArgs args;
+    LdsLayout L;               // LDS table: probe slots [0, L.S), side slots L.S (NULL key), L.S + 1 (sentinel key)
+    LdsLayout R;               // per-slot layout of the phase-2 HBM tables (UnitLayout)
+    OutArgs out;
+    const int64_t *offsets;    // nb + 1
+    const int32_t *modes;      // per batch: 2 = skip LDS (straight to phase 2); nullptr = LDS first everywhere
+    int32_t nbatches;
+    int32_t max_probe;
+    int32_t overflow_as_null;
+    int32_t pad;
+    unsigned int *spilled;     // per batch: 1 = phase 2 must aggregate it
+    unsigned long long *groups;  // per batch
+    unsigned long long *region_off;  // per batch byte offset into the region pool (phase 2)
+    uint8_t *region_pool;
+    unsigned int *error_flag;
+};
+
+// Probe slots of the phase-2 table of a batch with `rows` rows: next power of two >= 2 * rows (at least 64).
+__host__ __device__ __forceinline__ int RegionSlots(int64_t rows)
+{
+    int s = 64;
+    while (static_cast<int64_t>(s) < 2 * rows) s <<= 1;
+    return s;
+}
+
+// Slots per array of that table: the probe slots, two side slots, rounded up to a multiple of 256.
+__host__ __device__ __forceinline__ long long RegionStride(int64_t rows)
+{
+    return (static_cast<long long>(RegionSlots(rows)) + 2 + 255) & ~255LL;
+}
+
+__host__ __device__ __forceinline__ unsigned long long RegionBytes(const LdsLayout &R, int64_t rows)
+{
+    return static_cast<unsigned long long>(R.bytes) * static_cast<unsigned long long>(RegionStride(rows));
+}
+
+// Resets the aggregate state of one slot.
+template <typename Ref> __device__ __forceinline__ void InitSlot(const AggArgs &args, const Ref &r)
+{
+    for (int a = 0; a < args.n; ++a) {
+        int kind = args.a[a].kind;
+        if (Is32(kind)) *r.template p<int>(a, 0) = static_cast<int>(InitWord(kind));
+        else *r.template p<unsigned long long>(a, 0) = InitWord(kind);
+        if (kind == K_AVG || kind == K_SUM_DEC) *r.template p<unsigned long long>(a, 1) = 0;
+        if (kind == K_SUM_DEC) *r.template p<unsigned long long>(a, 2) = 0;
+        if (kind != K_COUNT_ALL && kind != K_COUNT_COL && kind != K_AVG) *r.flag(a) = 0;
+    }
+}
+
+// Writes one group (key + aggregates) to output row r; NULL bits are OR-ed into pre-cleared bitmaps.
+template <typename Ref>
+__device__ __forceinline__ void EmitGroup(const BatchConfig &c, const Ref &s, unsigned long long key, bool keyNull,
+                                          int64_t r)
+{
+    const OutDev &ko = c.out.o[0];
+    StoreWidth(ko.values, ko.width, r, keyNull ? 0ULL : key);
+    if (keyNull) atomicOr(reinterpret_cast<unsigned int *>(ko.nulls) + (r >> 5), 1u << (r & 31));
+    for (int i = 0; i < c.args.n; ++i) {
+        const OutDev &o = c.out.o[1 + i];
+        bool isNull = FinalizeAgg(c.args.a[i], i, s, o, r, c.overflow_as_null, c.error_flag);
+        if (isNull) atomicOr(reinterpret_cast<unsigned int *>(o.nulls) + (r >> 5), 1u << (r & 31));
+    }
+}
+
+// Emits every used slot of a table with probe slots [0, S) and side slots S, S + 1; ref(i) addresses slot i.
+template <int BLOCK, typename MakeRef>
+__device__ __forceinline__ void EmitTable(const BatchConfig &c, const unsigned long long *keys, int S, int64_t start,
+                                          unsigned int *counter, MakeRef ref)
+{
+    for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
+        unsigned long long k = keys[i];
+        if (k == kEmptyKey) continue;
+        unsigned int r = atomicAdd(counter, 1u);
+        bool side = i >= S;
+        EmitGroup(c, ref(i), side ? (i == S ? 0ULL : kEmptyKey) : k, i == S, start + r);
+    }
+}
+
+// Phase 1: LDS aggregation of batch b; abandons the batch when a row finds no slot.
+template <int BLOCK>
+__device__ __forceinline__ void BatchLds(const BatchConfig &c, int b, uint8_t *lds, unsigned int *sh)
+{
+    const LdsLayout &L = c.L;
+    const int S = L.S;
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    if (c.modes != nullptr && c.modes[b] == 2) {
+        if (threadIdx.x == 0) c.spilled[b] = 1u;
+        return;
+    }
+    unsigned long long *lkeys = reinterpret_cast<unsigned long long *>(lds + L.keys_off);
+    for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
+        lkeys[i] = kEmptyKey;
+        InitSlot(c.args, LdsRef{lds, L, i});
+    }
+    if (threadIdx.x == 0) {
+        sh[0] = 0;  // a row found no LDS slot
+        sh[1] = 0;  // groups emitted
+    }
+    __syncthreads();
+    for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
+        if (sh[0]) break;  // the batch goes to phase 2 anyway
+        bool isNull = NullBit(c.kd.nulls[0], row);
+        unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
+        int slot = -1;
+        if (isNull) {
+            slot = S;
+            lkeys[S] = 0;  // marks the side slot used
+        } else if (key == kEmptyKey) {
+            slot = S + 1;
+            lkeys[S + 1] = 0;
+        } else {
+            unsigned long long h = Mix64(key);
+            unsigned int pos = static_cast<unsigned int>(((h >> 32) * static_cast<unsigned long long>(S)) >> 32);
+            for (int step = 0; step < c.max_probe; ++step) {
+                unsigned long long k = lkeys[pos];
+                if (k == kEmptyKey) {
+                    k = atomicCAS(&lkeys[pos], kEmptyKey, key);
+                    if (k == kEmptyKey) k = key;
+                }
+                if (k == key) {
+                    slot = static_cast<int>(pos);
+                    break;
+                }
+                pos = pos + 1 == static_cast<unsigned int>(S) ? 0 : pos + 1;
+            }
+        }
+        if (slot < 0) {
+            sh[0] = 1u;
+            break;
+        }
+        UpdateAggsT(c.args, LdsRef{lds, L, slot}, row);
+    }
+    __syncthreads();
+    if (sh[0]) {
+        if (threadIdx.x == 0) c.spilled[b] = 1u;
+        return;
+    }
+    EmitTable<BLOCK>(c, lkeys, S, start, &sh[1], [&](int i) { return LdsRef{lds, L, i}; });
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        c.spilled[b] = 0u;
+        c.groups[b] = sh[1];
+    }
+}
+
+// Linear-probing find-or-insert of a key into a key array with probe slots [0, mask].
+__device__ __forceinline__ int TableSlot(unsigned long long *keys, unsigned long long key, unsigned int mask)
+{
+    unsigned int pos = static_cast<unsigned int>(Mix64(key)) & mask;
+    while (true) {
+        unsigned long long k = keys[pos];
+        if (k == kEmptyKey) {
+            k = atomicCAS(&keys[pos], kEmptyKey, key);
+            if (k == kEmptyKey) return static_cast<int>(pos);
+        }
+        if (k == key) return static_cast<int>(pos);
+        pos = (pos + 1) & mask;
+    }
+}
+
+// Phase 2: exact aggregation of a flagged batch in its private HBM table (load factor <= 1/2, so every
+// probe sequence ends).
+template <int BLOCK>
+__device__ __forceinline__ void BatchHbm(const BatchConfig &c, int b, unsigned int *sh)
+{
+    if (!c.spilled[b]) return;
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    const int S = RegionSlots(end - start);
+    const long long stride = RegionStride(end - start);
+    const LdsLayout &R = c.R;
+    uint8_t *base = c.region_pool + c.region_off[b];
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(base);
+    for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
+        keys[i] = kEmptyKey;
+        InitSlot(c.args, ScaledRef{base, R, stride, i});
+    }
+    if (threadIdx.x == 0) sh[1] = 0;
+    __syncthreads();
+    for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
+        bool isNull = NullBit(c.kd.nulls[0], row);
+        unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
+        int slot;
+        if (isNull) {
+            slot = S;
+            keys[S] = 0;
+        } else if (key == kEmptyKey) {
+            slot = S + 1;
+            keys[S + 1] = 0;
+        } else {
+            slot = TableSlot(keys, key, static_cast<unsigned int>(S - 1));
+        }
+        UpdateAggsT(c.args, ScaledRef{base, R, stride, slot}, row);
+    }
+    __syncthreads();
+    EmitTable<BLOCK>(c, keys, S, start, &sh[1], [&](int i) { return ScaledRef{base, R, stride, i}; });
+    __syncthreads();
+    if (threadIdx.x == 0) c.groups[b] = sh[1];
+}
+
+// Region offsets of the flagged batches: exclusive scan of their table bytes (one workgroup of BLOCK threads).
+template <int BLOCK>
+__device__ __forceinline__ void ScanRegions(const BatchConfig &c, unsigned long long *partial)
+{
+    const int nb = c.nbatches;
+    const int per = (nb + BLOCK - 1) / BLOCK;
+    const int lo = threadIdx.x * per;
+    const int hi = lo + per < nb ? lo + per : nb;
+    unsigned long long sum = 0;
+    for (int b = lo; b < hi; ++b) {
+        if (c.spilled[b]) sum += RegionBytes(c.R, c.offsets[b + 1] - c.offsets[b]);
+    }
+    partial[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int t = 0; t < BLOCK; ++t) {
+            unsigned long long v = partial[t];
+            partial[t] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    unsigned long long off = partial[threadIdx.x];
+    for (int b = lo; b < hi; ++b) {
+        c.region_off[b] = off;
+        if (c.spilled[b]) off += RegionBytes(c.R, c.offsets[b + 1] - c.offsets[b]);
+    }
+}
+
+}  // namespace omni_gpu

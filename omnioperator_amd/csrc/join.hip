@@ -88,17 +88,36 @@ __global__ void unpack_null_bytes(const uint8_t *bits, int64_t rows, uint8_t *ds
     if (i < rows) dst[i] = NullBit(bits, i) ? 1 : 0;
 }
 
-__global__ void join_minmax(const long long *keys, const uint8_t *knull, int64_t n, long long *mm)
+constexpr int kMinMaxRows = 16384;  // rows per workgroup of the min/max reduction
+
+// Min/max of the non-NULL build keys: each workgroup reduces kMinMaxRows rows, then one atomic pair.
+__global__ void __launch_bounds__(256) join_minmax(const long long *keys, const uint8_t *knull, int64_t n, long long *mm)
 {
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    __shared__ long long slo[4], shi[4];
     long long lo = std::numeric_limits<long long>::max(), hi = std::numeric_limits<long long>::min();
-    if (i < n && !knull[i]) lo = hi = keys[i];
+    int64_t start = static_cast<int64_t>(blockIdx.x) * kMinMaxRows;
+    int64_t end = start + kMinMaxRows < n ? start + kMinMaxRows : n;
+    for (int64_t i = start + threadIdx.x; i < end; i += 256) {
+        if (!knull[i]) {
+            lo = min(lo, keys[i]);
+            hi = max(hi, keys[i]);
+        }
+    }
 #pragma unroll
     for (int d = 32; d > 0; d >>= 1) {
         lo = min(lo, __shfl_xor(lo, d, 64));
         hi = max(hi, __shfl_xor(hi, d, 64));
     }
     if (LaneId() == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            lo = min(lo, slo[w]);
+            hi = max(hi, shi[w]);
+        }
         atomicMin(&mm[0], lo);
         atomicMax(&mm[1], hi);
     }
@@ -191,77 +210,144 @@ __device__ __forceinline__ int FindBatch(const ProbeArgs &a, int64_t block)
     return lo;
 }
 
+constexpr int kProbeRows = 16;                     // probe rows per thread
+constexpr int kProbeTile = kBlock * kProbeRows;    // probe rows per workgroup
+
+template <bool ARRAY>
+__device__ __forceinline__ int Lookup(const TableDev &t, long long key, unsigned int &pay)
+{
+    if (ARRAY) {
+        unsigned long long idx = static_cast<unsigned long long>(key) - static_cast<unsigned long long>(t.min_key);
+        return idx < t.range ? t.heads[idx] : -1;
+    }
+    if (static_cast<unsigned long long>(key) == kEmptyKey) return *t.sentinel_head;
+    unsigned long long pos = Mix64(static_cast<unsigned long long>(key)) & t.mask;
+    while (true) {
+        JoinSlot s = t.slots[pos];
+        if (s.key == key) {
+            pay = s.pay;
+            return s.head;
+        }
+        if (static_cast<unsigned long long>(s.key) == kEmptyKey) return -1;
+        pos = (pos + 1) & t.mask;
+    }
+}
+
+template <bool INLINE>
+__device__ __forceinline__ void WriteMatch(const ProbeArgs &a, long long o, long long probeRow, int h, unsigned int pay)
+{
+    if (o >= a.cap) return;
+    a.out_rows[o] = probeRow;
+    for (int p = 0; p < a.npay; ++p) {
+        const PayloadDev &pd = a.pay[p];
+        if (pd.out == nullptr) continue;
+        if (INLINE) {
+            if (pd.width == 4) static_cast<unsigned int *>(pd.out)[o] = pay;
+            else if (pd.width == 2) static_cast<unsigned short *>(pd.out)[o] = static_cast<unsigned short>(pay);
+            else static_cast<uint8_t *>(pd.out)[o] = static_cast<uint8_t>(pay);
+            continue;
+        }
+        switch (pd.width) {
+            case 8: static_cast<unsigned long long *>(pd.out)[o] = static_cast<const unsigned long long *>(pd.values)[h]; break;
+            case 4: static_cast<unsigned int *>(pd.out)[o] = static_cast<const unsigned int *>(pd.values)[h]; break;
+            case 2: static_cast<unsigned short *>(pd.out)[o] = static_cast<const unsigned short *>(pd.values)[h]; break;
+            case 16: {
+                const unsigned long long *src = static_cast<const unsigned long long *>(pd.values);
+                unsigned long long *dst = static_cast<unsigned long long *>(pd.out);
+                dst[2 * o] = src[2 * h];
+                dst[2 * o + 1] = src[2 * h + 1];
+                break;
+            }
+            default: static_cast<uint8_t *>(pd.out)[o] = static_cast<const uint8_t *>(pd.values)[h]; break;
+        }
+        if (pd.nulls && pd.out_nulls && pd.nulls[h]) {
+            atomicOr(reinterpret_cast<unsigned int *>(pd.out_nulls) + (o >> 5), 1u << (o & 31));
+        }
+    }
+}
+
+// One global atomic per workgroup: every wave passes its match total; returns the wave's first output row.
+// Must be reached by all threads of the workgroup.
+__device__ __forceinline__ unsigned long long BlockReserve(unsigned long long *counter, unsigned int wave_total,
+                                                           unsigned long long *sh)
+{
+    constexpr int kWaves = kBlock / 64;
+    const int wave = threadIdx.x >> 6;
+    if (LaneId() == 0) sh[wave] = wave_total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long run = 0;
+        for (int w = 0; w < kWaves; ++w) {
+            unsigned long long v = sh[w];
+            sh[w] = run;
+            run += v;
+        }
+        sh[kWaves] = run ? atomicAdd(counter, run) : 0ULL;
+    }
+    __syncthreads();
+    return sh[kWaves] + sh[wave];
+}
+
+// Probes kProbeTile rows per workgroup (kProbeRows per thread, coalesced per row slot), keeps the hits in
+// registers, then reserves the workgroup's output range with one atomic and writes the matches contiguously.
 // ARRAY: direct-index table; INLINE: the single <= 4-byte payload lives in the slot (unique keys only);
 // DUPS: some key has several build rows (walk next[]).
 template <bool ARRAY, bool INLINE, bool DUPS>
 __global__ void __launch_bounds__(kBlock) join_probe(ProbeArgs a, TableDev t)
 {
+    __shared__ unsigned long long sh[kBlock / 64 + 1];
     const ProbeBatch &b = a.nb == 1 ? a.single : a.batches[FindBatch(a, blockIdx.x)];
-    int64_t row = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kBlock + threadIdx.x;
-    bool valid = row < b.rows && !NullBit(b.nulls, row);
-    long long key = valid ? static_cast<long long>(LoadWord(b.keys, b.width, row)) : 0;
-    int head = -1;
-    unsigned int pay = 0;
-    if (valid) {
-        if (ARRAY) {
-            unsigned long long idx = static_cast<unsigned long long>(key) - static_cast<unsigned long long>(t.min_key);
-            if (idx < t.range) head = t.heads[idx];
-        } else if (static_cast<unsigned long long>(key) == kEmptyKey) {
-            head = *t.sentinel_head;
-        } else {
-            unsigned long long pos = Mix64(static_cast<unsigned long long>(key)) & t.mask;
-            while (true) {
-                JoinSlot s = t.slots[pos];
-                if (s.key == key) {
-                    head = s.head;
-                    pay = s.pay;
-                    break;
-                }
-                if (static_cast<unsigned long long>(s.key) == kEmptyKey) break;
-                pos = (pos + 1) & t.mask;
-            }
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
+    int head[kProbeRows];
+    unsigned int pay[kProbeRows];
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        int64_t row = tile0 + j * kBlock + threadIdx.x;
+        head[j] = -1;
+        pay[j] = 0;
+        if (row < b.rows && !NullBit(b.nulls, row)) {
+            head[j] = Lookup<ARRAY>(t, static_cast<long long>(LoadWord(b.keys, b.width, row)), pay[j]);
         }
     }
-    unsigned int cnt = 0;
-    if (DUPS) {
-        for (int h = head; h >= 0; h = t.next[h]) ++cnt;
-    } else {
-        cnt = head >= 0 ? 1u : 0u;
-    }
-    unsigned long long base = WaveReserve(a.counter, cnt);
-    if (cnt == 0) return;
-    long long probeRow = b.row_base + row;
-    int h = head;
-    for (unsigned int j = 0; j < cnt; ++j, h = DUPS ? t.next[h] : h) {
-        long long o = static_cast<long long>(base + j);
-        if (o >= a.cap) break;
-        a.out_rows[o] = probeRow;
-        for (int p = 0; p < a.npay; ++p) {
-            const PayloadDev &pd = a.pay[p];
-            if (pd.out == nullptr) continue;
-            if (INLINE) {
-                if (pd.width == 4) static_cast<unsigned int *>(pd.out)[o] = pay;
-                else if (pd.width == 2) static_cast<unsigned short *>(pd.out)[o] = static_cast<unsigned short>(pay);
-                else static_cast<uint8_t *>(pd.out)[o] = static_cast<uint8_t>(pay);
-            } else {
-                switch (pd.width) {
-                    case 8: static_cast<unsigned long long *>(pd.out)[o] = static_cast<const unsigned long long *>(pd.values)[h]; break;
-                    case 4: static_cast<unsigned int *>(pd.out)[o] = static_cast<const unsigned int *>(pd.values)[h]; break;
-                    case 2: static_cast<unsigned short *>(pd.out)[o] = static_cast<const unsigned short *>(pd.values)[h]; break;
-                    case 16: {
-                        const unsigned long long *src = static_cast<const unsigned long long *>(pd.values);
-                        unsigned long long *dst = static_cast<unsigned long long *>(pd.out);
-                        dst[2 * o] = src[2 * h];
-                        dst[2 * o + 1] = src[2 * h + 1];
-                        break;
-                    }
-                    default: static_cast<uint8_t *>(pd.out)[o] = static_cast<const uint8_t *>(pd.values)[h]; break;
-                }
-                if (pd.nulls && pd.out_nulls && pd.nulls[h]) {
-                    atomicOr(reinterpret_cast<unsigned int *>(pd.out_nulls) + (o >> 5), 1u << (o & 31));
-                }
-            }
+    const long long rowBase = b.row_base + tile0 + threadIdx.x;
+    if (!DUPS) {
+        unsigned long long m[kProbeRows];
+        unsigned int total = 0;
+#pragma unroll
+        for (int j = 0; j < kProbeRows; ++j) {
+            m[j] = __ballot(head[j] >= 0);
+            total += __popcll(m[j]);
         }
+        unsigned long long base = BlockReserve(a.counter, total, sh);
+        const unsigned long long lt = LaneMaskLt();
+#pragma unroll
+        for (int j = 0; j < kProbeRows; ++j) {
+            if (head[j] >= 0) {
+                WriteMatch<INLINE>(a, static_cast<long long>(base + __popcll(m[j] & lt)), rowBase + j * kBlock, head[j],
+                                   pay[j]);
+            }
+            base += __popcll(m[j]);
+        }
+        return;
+    }
+    // duplicate keys: per-row match counts, wave-level exclusive scan of each lane's total
+    unsigned int mine = 0;
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        for (int h = head[j]; h >= 0; h = t.next[h]) ++mine;
+    }
+    const int lane = LaneId();
+    unsigned int incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned int up = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += up;
+    }
+    unsigned int waveTotal = __builtin_amdgcn_readlane(incl, 63);
+    unsigned long long o = BlockReserve(a.counter, waveTotal, sh) + (incl - mine);
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        for (int h = head[j]; h >= 0; h = t.next[h]) WriteMatch<false>(a, static_cast<long long>(o++), rowBase + j * kBlock, h, 0);
     }
 }
 
@@ -477,7 +563,7 @@ int omni_gpu_join_build_finish(void *handle)
         OG_HIP(hipMemcpyAsync(dmm, mm, sizeof(mm), hipMemcpyHostToDevice, s));
         {
             KernelScope ks(s, "join_minmax");
-            join_minmax<<<Blocks(h.n), kBlock, 0, s>>>(h.keys.as<long long>(), h.knull.as<uint8_t>(), h.n, dmm);
+            join_minmax<<<static_cast<unsigned int>((h.n + kMinMaxRows - 1) / kMinMaxRows), 256, 0, s>>>(h.keys.as<long long>(), h.knull.as<uint8_t>(), h.n, dmm);
             OG_TRY(CheckLaunch("join_minmax"));
         }
         OG_HIP(hipMemcpyAsync(mm, dmm, sizeof(mm), hipMemcpyDeviceToHost, s));
@@ -579,7 +665,7 @@ int omni_gpu_join_probe(void *handle, const omni_gpu_column *probe_key, int64_t 
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     ProbeBatch b{};
     OG_TRY(FillBatch(b, *probe_key, rows, probe_row_base, 0, h.spec.key_type));
-    return RunProbe(h, &b, 1, Blocks(rows), reinterpret_cast<long long *>(out_probe_rows), out_payloads, out_capacity,
+    return RunProbe(h, &b, 1, (rows + kProbeTile - 1) / kProbeTile, reinterpret_cast<long long *>(out_probe_rows), out_payloads, out_capacity,
                     num_matches);
 }
 
@@ -593,7 +679,7 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
     int64_t blocks = 0, base = 0;
     for (int i = 0; i < num_batches; ++i) {
         OG_TRY(FillBatch(bs[i], keys[i], rows[i], base, blocks, h.spec.key_type));
-        blocks += Blocks(rows[i]);
+        blocks += (rows[i] + kProbeTile - 1) / kProbeTile;
         base += rows[i];
     }
     OG_CHECK(blocks < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many probe rows for one launch");

@@ -9,7 +9,13 @@ from oracle import AVG, COUNT_ALL, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX
 from omnioperator_amd import capi, workloads
 from tests.helpers import assert_row_sets_equal, rows_of
 
-pytestmark = pytest.mark.gpu
+pytestmark = [
+    pytest.mark.gpu,
+    pytest.mark.skip(reason="the first persistent-queue stream kernel (agg_stream) took a GPU memory fault on its "
+                            "first run; its indexing is clean in a concurrent CPU build under AddressSanitizer and no "
+                            "cause is shown, so that kernel is switched off (omni_gpu_agg_stream_run returns an "
+                            "error); the batched path is rebuilt as agg_batch_* (tests/test_gpu_batches.py)"),
+]
 
 AGGS = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, INT, INT), (AVG, DOUBLE, DOUBLE),
         (SUM, DECIMAL64, DECIMAL128, 2, 2), (MAX, LONG, LONG)]
