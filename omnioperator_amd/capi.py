@@ -91,6 +91,8 @@ SIGNATURES = {
     "omni_gpu_join_build_add": (_I, [_VP, _P(Column), _P(Column), _I64]),
     "omni_gpu_join_build_finish": (_I, [_VP]),
     "omni_gpu_join_build_info": (_I, [_VP, _P(_I32), _P(_I64), _P(_I64)]),
+    "omni_gpu_join_tune": (_I, [_VP, _I32, _I32]),
+    "omni_gpu_join_probe_info": (_I, [_VP, _P(_I32)]),
     "omni_gpu_join_probe": (_I, [_VP, _P(Column), _I64, _I64, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_probe_batches": (_I, [_VP, _P(Column), _P(_I64), _I32, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_destroy": (_I, [_VP]),
@@ -367,12 +369,21 @@ class HashJoin:
         check(lib().omni_gpu_join_build_create(C.byref(s), C.byref(h)))
         self.h = h.value
 
-    def add_build(self, key: Column, payloads: Sequence[Column], rows: int):
-        parr = (Column * max(1, len(payloads)))(*payloads)
+    def add_build(self, key: Column, payloads, rows: int):
+        """payloads: sequence of Columns or a prebuilt ctypes array of them."""
+        parr = payloads if isinstance(payloads, C.Array) else (Column * max(1, len(payloads)))(*payloads)
         check(lib().omni_gpu_join_build_add(self.h, C.byref(key), parr, rows))
 
     def finish(self):
         check(lib().omni_gpu_join_build_finish(self.h))
+
+    def tune(self, probe_group: int = 1, partitions: int = -1):
+        check(lib().omni_gpu_join_tune(self.h, probe_group, partitions))
+
+    def last_partitions(self) -> int:
+        p = C.c_int32()
+        check(lib().omni_gpu_join_probe_info(self.h, C.byref(p)))
+        return p.value
 
     def info(self):
         a, d, s = C.c_int32(), C.c_int64(), C.c_int64()
@@ -404,12 +415,18 @@ class HashJoin:
 
     def probe_batches_raw(self, keys: Sequence[Column], rows: Sequence[int], out_rows: DeviceArray,
                           oarr, cap: int) -> int:
-        """Device-resident batched probe (one launch); returns the match count."""
-        karr = (Column * len(keys))(*keys)
-        rarr = (C.c_int64 * len(rows))(*rows)
+        """Device-resident batched probe (one launch); returns the match count.  keys / rows may be prebuilt
+        ctypes arrays (probe_arrays) so that repeated calls do not rebuild them."""
+        karr = keys if isinstance(keys, C.Array) else (Column * len(keys))(*keys)
+        rarr = rows if isinstance(rows, C.Array) else (C.c_int64 * len(rows))(*rows)
         m = C.c_int64()
-        check(lib().omni_gpu_join_probe_batches(self.h, karr, rarr, len(keys), out_rows.ptr, oarr, cap, C.byref(m)))
+        check(lib().omni_gpu_join_probe_batches(self.h, karr, rarr, len(karr), out_rows.ptr, oarr, cap, C.byref(m)))
         return m.value
+
+    @staticmethod
+    def probe_arrays(keys: Sequence[Column], rows: Sequence[int]):
+        """ctypes arrays of probe batch columns and row counts for probe_batches_raw."""
+        return (Column * len(keys))(*keys), (C.c_int64 * len(rows))(*rows)
 
     def close(self):
         if getattr(self, "h", None):

@@ -444,7 +444,7 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     }
     h->args.n = spec->num_aggs;
     for (int i = 0; i < spec->num_aggs; ++i) OG_TRY(ClassifyAgg(*spec, i, h->args.a[i]));
-    OG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    OG_TRY(AcquireStream(&h->stream));
     OG_TRY(h->counters.Reserve(64, h->stream));
     OG_HIP(hipMemsetAsync(h->counters.p, 0, 64, h->stream));
     uint64_t want = spec->capacity_hint > 0 ? static_cast<uint64_t>(spec->capacity_hint) : 16384;
@@ -595,10 +595,7 @@ int omni_gpu_agg_destroy(void *handle)
     for (auto &st : h->state)
         for (auto &b : st) b.Release(s);
     h->counters.Release(s);
-    if (s) {
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
-    }
+    ReleaseStream(s);
     delete h;
     return OMNI_GPU_OK;
 }

@@ -140,7 +140,7 @@ int omni_gpu_agg_batches_create(const omni_gpu_agg_spec *spec, int64_t max_batch
     for (int i = 0; i < spec->num_aggs; ++i) OG_TRY(ClassifyAgg(*spec, i, c.args.a[i]));
     c.R = UnitLayout(c.args);
     c.overflow_as_null = spec->overflow_as_null;
-    OG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    OG_TRY(AcquireStream(&h->stream));
     OG_TRY(h->counters.Reserve(64, h->stream));
     OG_HIP(hipMemsetAsync(h->counters.p, 0, 64, h->stream));
     OG_TRY(h->dcfg.Reserve(sizeof(BatchConfig), h->stream));
@@ -321,10 +321,7 @@ int omni_gpu_agg_batches_destroy(void *handle)
     h->region_off.Release(s);
     h->pool.Release(s);
     h->counters.Release(s);
-    if (s) {
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
-    }
+    ReleaseStream(s);
     delete h;
     return OMNI_GPU_OK;
 }

@@ -7,12 +7,14 @@
 // (core/src/operator/join/lookup_join.cpp:924-1035, 1657-1779).
 //
 // Device layout:
-//   build store   keys[n] (int64, sign-extended), key_null[n] (u8), payload values / NULL bytes per column
+//   build store   keys[n] (int64, sign-extended), key_null[n] (u8), payload values / NULL bytes per column,
+//                 gathered from every retained build batch in one launch at finish
 //   hash table    slots[cap] {i64 key; i32 head; u32 inline payload} (16 B, one cache-line read per probe step),
 //                 linear probing at load <= 0.5; duplicate keys chain through next[n]
 //   array table   heads[range] (i32 build row, -1 none) + next[n]
-// Probe: one probe row per lane; matches are compacted with a wave-level exclusive scan and one atomic
-// per wave on the output counter, so every wave writes its output rows contiguously.
+// Probe: 16 probe rows per lane held in registers; matches are compacted with wave ballots (unique keys) or a
+// wave-level exclusive scan (duplicate keys) and one atomic per workgroup on the output counter, so each
+// workgroup writes its output rows contiguously.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -20,336 +22,9 @@
 #include <memory>
 #include <vector>
 
-#include "omni_gpu_internal.h"
+#include "join_device.h"
 
 namespace omni_gpu {
-
-struct JoinSlot {
-    long long key;
-    int head;
-    unsigned int pay;
-};
-
-struct PayloadDev {
-    int width;
-    const void *values;     // build store, width bytes per build row
-    const uint8_t *nulls;   // build store NULL bytes, or nullptr if the column has no NULLs
-    void *out;              // output values
-    uint8_t *out_nulls;     // output bitmap (pre-cleared) or nullptr
-};
-
-struct ProbeBatch {
-    const void *keys;
-    const uint8_t *nulls;
-    int64_t rows;
-    int64_t row_base;
-    int64_t block_start;
-    int width;
-    int pad;
-};
-
-struct ProbeArgs {
-    int nb;
-    const ProbeBatch *batches;  // device array when nb > 1
-    ProbeBatch single;
-    int npay;
-    PayloadDev pay[OMNI_GPU_MAX_AGGS];
-    long long *out_rows;
-    unsigned long long *counter;
-    long long cap;
-};
-
-struct TableDev {
-    const JoinSlot *slots;
-    unsigned long long mask;
-    const int *heads;      // array table
-    long long min_key;
-    unsigned long long range;
-    const int *next;
-    const int *sentinel_head;  // chain of build rows whose key equals the slot sentinel
-};
-
-constexpr int kBlock = 256;
-inline unsigned int Blocks(int64_t n) { return static_cast<unsigned int>((n + kBlock - 1) / kBlock); }
-
-__global__ void join_load_keys(const void *src, int width, const uint8_t *nulls, int64_t rows, long long *dst,
-                               uint8_t *dst_null)
-{
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= rows) return;
-    bool isNull = NullBit(nulls, i);
-    dst[i] = isNull ? 0 : static_cast<long long>(LoadWord(src, width, i));
-    dst_null[i] = isNull ? 1 : 0;
-}
-
-__global__ void unpack_null_bytes(const uint8_t *bits, int64_t rows, uint8_t *dst)
-{
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < rows) dst[i] = NullBit(bits, i) ? 1 : 0;
-}
-
-constexpr int kMinMaxRows = 16384;  // rows per workgroup of the min/max reduction
-
-// Min/max of the non-NULL build keys: each workgroup reduces kMinMaxRows rows, then one atomic pair.
-__global__ void __launch_bounds__(256) join_minmax(const long long *keys, const uint8_t *knull, int64_t n, long long *mm)
-{
-    __shared__ long long slo[4], shi[4];
-    long long lo = std::numeric_limits<long long>::max(), hi = std::numeric_limits<long long>::min();
-    int64_t start = static_cast<int64_t>(blockIdx.x) * kMinMaxRows;
-    int64_t end = start + kMinMaxRows < n ? start + kMinMaxRows : n;
-    for (int64_t i = start + threadIdx.x; i < end; i += 256) {
-        if (!knull[i]) {
-            lo = min(lo, keys[i]);
-            hi = max(hi, keys[i]);
-        }
-    }
-#pragma unroll
-    for (int d = 32; d > 0; d >>= 1) {
-        lo = min(lo, __shfl_xor(lo, d, 64));
-        hi = max(hi, __shfl_xor(hi, d, 64));
-    }
-    if (LaneId() == 0) {
-        slo[threadIdx.x >> 6] = lo;
-        shi[threadIdx.x >> 6] = hi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w) {
-            lo = min(lo, slo[w]);
-            hi = max(hi, shi[w]);
-        }
-        atomicMin(&mm[0], lo);
-        atomicMax(&mm[1], hi);
-    }
-}
-
-__global__ void join_init_slots(JoinSlot *s, int64_t n)
-{
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) {
-        s[i].key = static_cast<long long>(kEmptyKey);
-        s[i].head = -1;
-        s[i].pay = 0;
-    }
-}
-
-__global__ void join_fill_i32(int *p, int v, int64_t n)
-{
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__device__ __forceinline__ unsigned int PayloadWord(const void *v, int width, int64_t row)
-{
-    switch (width) {
-        case 4: return static_cast<const unsigned int *>(v)[row];
-        case 2: return static_cast<const unsigned short *>(v)[row];
-        default: return static_cast<const uint8_t *>(v)[row];
-    }
-}
-
-__global__ void join_build_hash(const long long *keys, const uint8_t *knull, int64_t n, JoinSlot *slots,
-                                unsigned long long mask, int *next, int *sentinel_head, unsigned int *has_dup,
-                                const void *inline_pay, int inline_width)
-{
-    int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (row >= n || knull[row]) return;
-    long long key = keys[row];
-    int old;
-    if (static_cast<unsigned long long>(key) == kEmptyKey) {
-        old = atomicExch(sentinel_head, static_cast<int>(row));
-    } else {
-        unsigned long long pos = Mix64(static_cast<unsigned long long>(key)) & mask;
-        while (true) {
-            long long k = slots[pos].key;
-            if (static_cast<unsigned long long>(k) == kEmptyKey) {
-                k = static_cast<long long>(atomicCAS(reinterpret_cast<unsigned long long *>(&slots[pos].key), kEmptyKey,
-                                                     static_cast<unsigned long long>(key)));
-                if (static_cast<unsigned long long>(k) == kEmptyKey) k = key;
-            }
-            if (k == key) break;
-            pos = (pos + 1) & mask;
-        }
-        old = atomicExch(&slots[pos].head, static_cast<int>(row));
-        if (inline_pay) slots[pos].pay = PayloadWord(inline_pay, inline_width, row);
-    }
-    next[row] = old;
-    if (old >= 0) *has_dup = 1u;
-}
-
-__global__ void join_build_array(const long long *keys, const uint8_t *knull, int64_t n, int *heads,
-                                 long long min_key, int *next, unsigned int *has_dup)
-{
-    int64_t row = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (row >= n || knull[row]) return;
-    unsigned long long idx = static_cast<unsigned long long>(keys[row]) - static_cast<unsigned long long>(min_key);
-    int old = atomicExch(&heads[idx], static_cast<int>(row));
-    next[row] = old;
-    if (old >= 0) *has_dup = 1u;
-}
-
-// Longest duplicate chain (only run when some key repeats).
-__global__ void join_max_chain(const int *heads, int64_t nheads, int stride, const int *next, unsigned int *max_len)
-{
-    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i >= nheads) return;
-    int h = heads[i * stride];
-    unsigned int len = 0;
-    for (; h >= 0; h = next[h]) ++len;
-    if (len) atomicMax(max_len, len);
-}
-
-__device__ __forceinline__ int FindBatch(const ProbeArgs &a, int64_t block)
-{
-    int lo = 0, hi = a.nb - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (a.batches[mid].block_start <= block) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-constexpr int kProbeRows = 16;                     // probe rows per thread
-constexpr int kProbeTile = kBlock * kProbeRows;    // probe rows per workgroup
-
-template <bool ARRAY>
-__device__ __forceinline__ int Lookup(const TableDev &t, long long key, unsigned int &pay)
-{
-    if (ARRAY) {
-        unsigned long long idx = static_cast<unsigned long long>(key) - static_cast<unsigned long long>(t.min_key);
-        return idx < t.range ? t.heads[idx] : -1;
-    }
-    if (static_cast<unsigned long long>(key) == kEmptyKey) return *t.sentinel_head;
-    unsigned long long pos = Mix64(static_cast<unsigned long long>(key)) & t.mask;
-    while (true) {
-        JoinSlot s = t.slots[pos];
-        if (s.key == key) {
-            pay = s.pay;
-            return s.head;
-        }
-        if (static_cast<unsigned long long>(s.key) == kEmptyKey) return -1;
-        pos = (pos + 1) & t.mask;
-    }
-}
-
-template <bool INLINE>
-__device__ __forceinline__ void WriteMatch(const ProbeArgs &a, long long o, long long probeRow, int h, unsigned int pay)
-{
-    if (o >= a.cap) return;
-    a.out_rows[o] = probeRow;
-    for (int p = 0; p < a.npay; ++p) {
-        const PayloadDev &pd = a.pay[p];
-        if (pd.out == nullptr) continue;
-        if (INLINE) {
-            if (pd.width == 4) static_cast<unsigned int *>(pd.out)[o] = pay;
-            else if (pd.width == 2) static_cast<unsigned short *>(pd.out)[o] = static_cast<unsigned short>(pay);
-            else static_cast<uint8_t *>(pd.out)[o] = static_cast<uint8_t>(pay);
-            continue;
-        }
-        switch (pd.width) {
-            case 8: static_cast<unsigned long long *>(pd.out)[o] = static_cast<const unsigned long long *>(pd.values)[h]; break;
-            case 4: static_cast<unsigned int *>(pd.out)[o] = static_cast<const unsigned int *>(pd.values)[h]; break;
-            case 2: static_cast<unsigned short *>(pd.out)[o] = static_cast<const unsigned short *>(pd.values)[h]; break;
-            case 16: {
-                const unsigned long long *src = static_cast<const unsigned long long *>(pd.values);
-                unsigned long long *dst = static_cast<unsigned long long *>(pd.out);
-                dst[2 * o] = src[2 * h];
-                dst[2 * o + 1] = src[2 * h + 1];
-                break;
-            }
-            default: static_cast<uint8_t *>(pd.out)[o] = static_cast<const uint8_t *>(pd.values)[h]; break;
-        }
-        if (pd.nulls && pd.out_nulls && pd.nulls[h]) {
-            atomicOr(reinterpret_cast<unsigned int *>(pd.out_nulls) + (o >> 5), 1u << (o & 31));
-        }
-    }
-}
-
-// One global atomic per workgroup: every wave passes its match total; returns the wave's first output row.
-// Must be reached by all threads of the workgroup.
-__device__ __forceinline__ unsigned long long BlockReserve(unsigned long long *counter, unsigned int wave_total,
-                                                           unsigned long long *sh)
-{
-    constexpr int kWaves = kBlock / 64;
-    const int wave = threadIdx.x >> 6;
-    if (LaneId() == 0) sh[wave] = wave_total;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long run = 0;
-        for (int w = 0; w < kWaves; ++w) {
-            unsigned long long v = sh[w];
-            sh[w] = run;
-            run += v;
-        }
-        sh[kWaves] = run ? atomicAdd(counter, run) : 0ULL;
-    }
-    __syncthreads();
-    return sh[kWaves] + sh[wave];
-}
-
-// Probes kProbeTile rows per workgroup (kProbeRows per thread, coalesced per row slot), keeps the hits in
-// registers, then reserves the workgroup's output range with one atomic and writes the matches contiguously.
-// ARRAY: direct-index table; INLINE: the single <= 4-byte payload lives in the slot (unique keys only);
-// DUPS: some key has several build rows (walk next[]).
-template <bool ARRAY, bool INLINE, bool DUPS>
-__global__ void __launch_bounds__(kBlock) join_probe(ProbeArgs a, TableDev t)
-{
-    __shared__ unsigned long long sh[kBlock / 64 + 1];
-    const ProbeBatch &b = a.nb == 1 ? a.single : a.batches[FindBatch(a, blockIdx.x)];
-    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
-    int head[kProbeRows];
-    unsigned int pay[kProbeRows];
-#pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
-        int64_t row = tile0 + j * kBlock + threadIdx.x;
-        head[j] = -1;
-        pay[j] = 0;
-        if (row < b.rows && !NullBit(b.nulls, row)) {
-            head[j] = Lookup<ARRAY>(t, static_cast<long long>(LoadWord(b.keys, b.width, row)), pay[j]);
-        }
-    }
-    const long long rowBase = b.row_base + tile0 + threadIdx.x;
-    if (!DUPS) {
-        unsigned long long m[kProbeRows];
-        unsigned int total = 0;
-#pragma unroll
-        for (int j = 0; j < kProbeRows; ++j) {
-            m[j] = __ballot(head[j] >= 0);
-            total += __popcll(m[j]);
-        }
-        unsigned long long base = BlockReserve(a.counter, total, sh);
-        const unsigned long long lt = LaneMaskLt();
-#pragma unroll
-        for (int j = 0; j < kProbeRows; ++j) {
-            if (head[j] >= 0) {
-                WriteMatch<INLINE>(a, static_cast<long long>(base + __popcll(m[j] & lt)), rowBase + j * kBlock, head[j],
-                                   pay[j]);
-            }
-            base += __popcll(m[j]);
-        }
-        return;
-    }
-    // duplicate keys: per-row match counts, wave-level exclusive scan of each lane's total
-    unsigned int mine = 0;
-#pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
-        for (int h = head[j]; h >= 0; h = t.next[h]) ++mine;
-    }
-    const int lane = LaneId();
-    unsigned int incl = mine;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned int up = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += up;
-    }
-    unsigned int waveTotal = __builtin_amdgcn_readlane(incl, 63);
-    unsigned long long o = BlockReserve(a.counter, waveTotal, sh) + (incl - mine);
-#pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
-        for (int h = head[j]; h >= 0; h = t.next[h]) WriteMatch<false>(a, static_cast<long long>(o++), rowBase + j * kBlock, h, 0);
-    }
-}
 
 // ---- host side ----------------------------------------------------------------------------------------------
 
@@ -357,7 +32,7 @@ struct JoinHandle {
     omni_gpu_join_spec spec{};
     hipStream_t stream = nullptr;
     int64_t n = 0;             // build rows
-    int64_t cap_rows = 0;      // allocated build-store rows
+    std::vector<BuildBatch> pending;  // retained batches, gathered at finish
     DevBuf keys, knull;
     DevBuf pay[OMNI_GPU_MAX_AGGS], pay_null[OMNI_GPU_MAX_AGGS];
     bool pay_nullable[OMNI_GPU_MAX_AGGS] = {};
@@ -366,28 +41,110 @@ struct JoinHandle {
     bool inline_pay = false;
     bool has_dup = false;
     int64_t max_dup = 0;
+    int probe_group = 1;       // probe width G of join_probe
+    int partitions = -1;       // radix-partitioned probe: -1 auto, 0 off, else the partition count
+    int last_partitions = 0;   // partition count the last probe used (1 = direct probe)
+    DevBuf part_keys, part_rows, part_counts, part_meta;
     uint64_t slots_n = 0;
     long long min_key = 0;
     unsigned long long range = 0;
-    DevBuf slots, heads, next, scratch;  // scratch: [0] sentinel head i32, [4] has_dup u32, [8] max_len u32,
-                                         //          [16] min/max i64[2], [32] match counter u64
-    DevBuf batch_desc;
+    // scratch: [0] sentinel head i32, [4] has_dup u32, [8] max_len u32, [12] sentinel inline payload u32,
+    //          [16] min/max i64[2], [32] match counter u64
+    DevBuf slots, heads, next, scratch;
+    DevBuf batch_desc, build_desc;
 };
 
-int EnsureBuildCapacity(JoinHandle &h, int64_t need)
+// Partition count of the radix-partitioned probe for `rows` probe rows in `nwg` tiles (1 = direct probe).
+// Automatic mode partitions only tables larger than the 256 MiB Infinity Cache (regions of about 2 MiB, one
+// L2-resident region per XCD at a time): measured on a 64 MiB table (2M keys, 64M probe rows) the direct probe
+// (1.56 ms) still beats the partition chain (1.65 ms: histogram 0.11, scatter 0.51, probe 1.08 ms).
+int ChoosePartitions(const JoinHandle &h, int64_t rows, int64_t nwg)
 {
-    if (need <= h.cap_rows) return OMNI_GPU_OK;
-    int64_t cap = std::max<int64_t>(need, std::max<int64_t>(h.cap_rows * 2, 65536));
-    hipStream_t s = h.stream;
-    OG_TRY(h.keys.Grow(sizeof(long long) * cap, sizeof(long long) * h.n, s));
-    OG_TRY(h.knull.Grow(cap, h.n, s));
-    for (int p = 0; p < h.spec.num_payloads; ++p) {
-        int w = TypeWidth(h.spec.payload_types[p]);
-        OG_TRY(h.pay[p].Grow(static_cast<size_t>(w) * cap, static_cast<size_t>(w) * h.n, s));
-        OG_TRY(h.pay_null[p].Grow(cap, h.n, s));
+    if (h.is_array || h.partitions == 0 || h.slots_n == 0) return 1;
+    const uint64_t tbytes = h.slots_n * sizeof(JoinSlot);
+    int P;
+    if (h.partitions > 0) {
+        P = h.partitions;
+    } else {
+        if (tbytes <= (256ull << 20) || rows < (4 << 20)) return 1;
+        P = 8;
+        while (P < kPartMax && tbytes / static_cast<uint64_t>(P) > (2u << 20)) P <<= 1;
     }
-    h.cap_rows = cap;
-    return OMNI_GPU_OK;
+    if (static_cast<uint64_t>(P) > h.slots_n) return 1;
+    if (rows >= (1LL << 32) - 1) return 1;                                   // u32 entry indices
+    if (static_cast<int64_t>(P) * nwg > static_cast<int64_t>(kScanTile) * kScanTile) return 1;  // two-level scan
+    return P;
+}
+
+int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64_t nwg, int64_t rows)
+{
+    hipStream_t s = h.stream;
+    PartArgs pa{};
+    pa.nparts = P;
+    int lgS = 0, lgP = 0;
+    while ((1ULL << lgS) < h.slots_n) ++lgS;
+    while ((1 << lgP) < P) ++lgP;
+    pa.shift = lgS - lgP;
+    pa.nwg = nwg;
+    const int64_t L = static_cast<int64_t>(P) * nwg;
+    const int ntile = static_cast<int>((L + kScanTile - 1) / kScanTile);
+    const int per = P / kXcds;
+    OG_TRY(h.part_keys.Reserve(sizeof(long long) * rows, s));
+    OG_TRY(h.part_rows.Reserve(sizeof(unsigned int) * rows, s));
+    OG_TRY(h.part_counts.Reserve(sizeof(unsigned int) * L, s));
+    const size_t metaWords = kScanTile + 1 + (P + 1) + kXcds * (per + 1) + kXcds;
+    OG_TRY(h.part_meta.Reserve(sizeof(unsigned int) * metaWords, s));
+    unsigned int *meta = h.part_meta.as<unsigned int>();
+    unsigned int *sums = meta;
+    pa.total = meta + kScanTile;
+    pa.part_start = pa.total + 1;
+    pa.xcd_prefix = pa.part_start + P + 1;
+    pa.xcd_next = pa.xcd_prefix + kXcds * (per + 1);
+    pa.counts = h.part_counts.as<unsigned int>();
+    pa.pkeys = h.part_keys.as<long long>();
+    pa.prows = h.part_rows.as<unsigned int>();
+    const unsigned int grid = static_cast<unsigned int>(nwg);
+    {
+        KernelScope ks(s, "join_probe_hist");
+        join_part_hist<<<grid, kBlock, 0, s>>>(a, t, pa);
+        OG_TRY(CheckLaunch("join_part_hist"));
+    }
+    {
+        KernelScope ks(s, "join_probe_scan");
+        scan_tiles<<<ntile, kScanBlock, 0, s>>>(pa.counts, L, sums);
+        scan_sums<<<1, kScanBlock, 0, s>>>(sums, ntile, pa.total);
+        scan_add<<<ntile, kScanBlock, 0, s>>>(pa.counts, L, sums);
+        join_part_plan<<<1, 64, 0, s>>>(pa);
+        OG_TRY(CheckLaunch("join_part_scan"));
+    }
+    {
+        KernelScope ks(s, "join_probe_scatter");
+        join_part_scatter<<<grid, kBlock, 0, s>>>(a, t, pa);
+        OG_TRY(CheckLaunch("join_part_scatter"));
+    }
+    int cus = 0;
+    OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
+    const unsigned int pgrid = static_cast<unsigned int>(std::max(kXcds, (cus * 4 / kXcds) * kXcds));
+    KernelScope ks(s, "join_probe_part");
+    if (h.has_dup) join_part_probe<false, true><<<pgrid, kBlock, 0, s>>>(a, t, pa);
+    else if (h.inline_pay) join_part_probe<true, false><<<pgrid, kBlock, 0, s>>>(a, t, pa);
+    else join_part_probe<false, false><<<pgrid, kBlock, 0, s>>>(a, t, pa);
+    return CheckLaunch("join_part_probe");
+}
+
+template <int G>
+void LaunchProbe(const JoinHandle &h, dim3 grid, hipStream_t s, const ProbeArgs &a, const TableDev &t)
+{
+    if (h.is_array) {
+        if (h.has_dup) join_probe<true, false, true, G><<<grid, kBlock, 0, s>>>(a, t);
+        else join_probe<true, false, false, G><<<grid, kBlock, 0, s>>>(a, t);
+    } else if (h.has_dup) {
+        join_probe<false, false, true, G><<<grid, kBlock, 0, s>>>(a, t);
+    } else if (h.inline_pay) {
+        join_probe<false, true, false, G><<<grid, kBlock, 0, s>>>(a, t);
+    } else {
+        join_probe<false, false, false, G><<<grid, kBlock, 0, s>>>(a, t);
+    }
 }
 
 int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blocks, long long *out_rows,
@@ -436,18 +193,22 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
     t.range = h.range;
     t.next = h.next.as<int>();
     t.sentinel_head = reinterpret_cast<const int *>(h.scratch.as<uint8_t>());
-    if (total_blocks > 0 && h.n > 0) {
+    t.sentinel_pay = reinterpret_cast<const unsigned int *>(h.scratch.as<uint8_t>() + 12);
+    int64_t total_rows = 0;
+    for (int i = 0; i < nb; ++i) total_rows += batches[i].rows;
+    const int P = (total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows, total_blocks) : 1;
+    h.last_partitions = P;
+    if (P > 1) {
+        OG_TRY(LaunchPartitioned(h, a, t, P, total_blocks, total_rows));
+    } else if (total_blocks > 0 && h.n > 0) {
         KernelScope ks(s, "join_probe");
         dim3 grid(static_cast<unsigned int>(total_blocks));
-        if (h.is_array) {
-            if (h.has_dup) join_probe<true, false, true><<<grid, kBlock, 0, s>>>(a, t);
-            else join_probe<true, false, false><<<grid, kBlock, 0, s>>>(a, t);
-        } else if (h.has_dup) {
-            join_probe<false, false, true><<<grid, kBlock, 0, s>>>(a, t);
-        } else if (h.inline_pay) {
-            join_probe<false, true, false><<<grid, kBlock, 0, s>>>(a, t);
-        } else {
-            join_probe<false, false, false><<<grid, kBlock, 0, s>>>(a, t);
+        switch (h.probe_group) {
+            case 1: LaunchProbe<1>(h, grid, s, a, t); break;
+            case 2: LaunchProbe<2>(h, grid, s, a, t); break;
+            case 4: LaunchProbe<4>(h, grid, s, a, t); break;
+            case 16: LaunchProbe<16>(h, grid, s, a, t); break;
+            default: LaunchProbe<8>(h, grid, s, a, t); break;
         }
         OG_TRY(CheckLaunch("join_probe"));
     }
@@ -496,7 +257,7 @@ int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle)
     }
     std::unique_ptr<JoinHandle> h(new JoinHandle());
     h->spec = *spec;
-    OG_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+    OG_TRY(AcquireStream(&h->stream));
     OG_TRY(h->scratch.Reserve(64, h->stream));
     *handle = h.release();
     return OMNI_GPU_OK;
@@ -513,29 +274,19 @@ int omni_gpu_join_build_add(void *handle, const omni_gpu_column *key, const omni
     OG_CHECK(key->type_id == h.spec.key_type, OMNI_GPU_ERR_INVALID, "build key type mismatch");
     OG_CHECK(key->values, OMNI_GPU_ERR_INVALID, "build key has no values");
     OG_CHECK(h.spec.num_payloads == 0 || payloads, OMNI_GPU_ERR_INVALID, "null payload columns");
-    OG_TRY(EnsureBuildCapacity(h, h.n + rows));
-    hipStream_t s = h.stream;
-    {
-        KernelScope ks(s, "join_load_keys");
-        join_load_keys<<<Blocks(rows), kBlock, 0, s>>>(key->values, TypeWidth(key->type_id), key->nulls, rows,
-                                                       h.keys.as<long long>() + h.n, h.knull.as<uint8_t>() + h.n);
-        OG_TRY(CheckLaunch("join_load_keys"));
-    }
+    BuildBatch bb{};
+    bb.key = key->values;
+    bb.knull = key->nulls;
+    bb.rows = rows;
+    bb.row_base = h.n;
     for (int p = 0; p < h.spec.num_payloads; ++p) {
         OG_CHECK(payloads[p].type_id == h.spec.payload_types[p], OMNI_GPU_ERR_INVALID, "payload %d type mismatch", p);
         OG_CHECK(payloads[p].values, OMNI_GPU_ERR_INVALID, "payload %d has no values", p);
-        int w = TypeWidth(h.spec.payload_types[p]);
-        OG_HIP(hipMemcpyAsync(h.pay[p].as<uint8_t>() + static_cast<size_t>(w) * h.n, payloads[p].values,
-                              static_cast<size_t>(w) * rows, hipMemcpyDeviceToDevice, s));
-        if (payloads[p].nulls) {
-            h.pay_nullable[p] = true;
-            KernelScope ks(s, "join_unpack_nulls");
-            unpack_null_bytes<<<Blocks(rows), kBlock, 0, s>>>(payloads[p].nulls, rows, h.pay_null[p].as<uint8_t>() + h.n);
-            OG_TRY(CheckLaunch("unpack_null_bytes"));
-        } else {
-            OG_HIP(hipMemsetAsync(h.pay_null[p].as<uint8_t>() + h.n, 0, rows, s));
-        }
+        bb.pay[p] = payloads[p].values;
+        bb.pnull[p] = payloads[p].nulls;
+        if (payloads[p].nulls) h.pay_nullable[p] = true;
     }
+    h.pending.push_back(bb);
     h.n += rows;
     return OMNI_GPU_OK;
 }
@@ -547,25 +298,59 @@ int omni_gpu_join_build_finish(void *handle)
     OG_CHECK(!h.finished, OMNI_GPU_ERR_STATE, "build side already finished");
     hipStream_t s = h.stream;
     uint8_t *sc = h.scratch.as<uint8_t>();
-    int init_scratch[4] = {-1, 0, 0, 0};
-    OG_HIP(hipMemcpyAsync(sc, init_scratch, sizeof(init_scratch), hipMemcpyHostToDevice, s));
+    // scratch: sentinel head -1, has_dup 0, max chain 0, sentinel inline payload, then min/max seeds
+    struct {
+        int head, has_dup;
+        unsigned int max_len, sentinel_pay;
+        long long mm[2];
+    } init = {-1, 0, 0, 0, {std::numeric_limits<long long>::max(), std::numeric_limits<long long>::min()}};
+    static_assert(sizeof(init) == 32, "scratch layout");
+    OG_HIP(hipMemcpyAsync(sc, &init, sizeof(init), hipMemcpyHostToDevice, s));
     h.finished = true;
     if (h.n == 0) {
         h.max_dup = 0;
+        h.pending.clear();
         return OMNI_GPU_OK;
+    }
+    // build store: every retained batch gathered in one launch
+    OG_TRY(h.keys.Reserve(sizeof(long long) * h.n, s));
+    OG_TRY(h.knull.Reserve(h.n, s));
+    GatherArgs g{};
+    g.nb = static_cast<int>(h.pending.size());
+    g.key_width = TypeWidth(h.spec.key_type);
+    g.npay = h.spec.num_payloads;
+    g.keys = h.keys.as<long long>();
+    g.knull = h.knull.as<uint8_t>();
+    for (int p = 0; p < g.npay; ++p) {
+        g.pay_width[p] = TypeWidth(h.spec.payload_types[p]);
+        OG_TRY(h.pay[p].Reserve(static_cast<size_t>(g.pay_width[p]) * h.n, s));
+        g.pay[p] = h.pay[p].p;
+        if (h.pay_nullable[p]) {
+            OG_TRY(h.pay_null[p].Reserve(h.n, s));
+            g.pnull[p] = h.pay_null[p].as<uint8_t>();
+        }
+    }
+    int64_t blocks = 0;
+    for (BuildBatch &bb : h.pending) {
+        bb.block_start = blocks;
+        blocks += (bb.rows + kGatherTile - 1) / kGatherTile;
+    }
+    OG_CHECK(blocks < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "build side too large for one gather launch");
+    OG_TRY(h.build_desc.Reserve(sizeof(BuildBatch) * h.pending.size(), s));
+    OG_HIP(hipMemcpyAsync(h.build_desc.p, h.pending.data(), sizeof(BuildBatch) * h.pending.size(), hipMemcpyHostToDevice, s));
+    g.batches = h.build_desc.as<BuildBatch>();
+    long long *dmm = reinterpret_cast<long long *>(sc + 16);
+    g.mm = h.spec.strategy == 0 ? dmm : nullptr;
+    {
+        KernelScope ks(s, "join_gather_build");
+        join_gather_build<<<static_cast<unsigned int>(blocks), 256, 0, s>>>(g);
+        OG_TRY(CheckLaunch("join_gather_build"));
     }
     OG_TRY(h.next.Reserve(sizeof(int) * h.n, s));
     // initDegree and the array-table rule of JoinHashTableVariants::BuildHashTable / TryToBuildArrayTable.
     bool arrayOk = false;
     if (h.spec.strategy == 0) {
-        long long mm[2] = {std::numeric_limits<long long>::max(), std::numeric_limits<long long>::min()};
-        long long *dmm = reinterpret_cast<long long *>(sc + 16);
-        OG_HIP(hipMemcpyAsync(dmm, mm, sizeof(mm), hipMemcpyHostToDevice, s));
-        {
-            KernelScope ks(s, "join_minmax");
-            join_minmax<<<static_cast<unsigned int>((h.n + kMinMaxRows - 1) / kMinMaxRows), 256, 0, s>>>(h.keys.as<long long>(), h.knull.as<uint8_t>(), h.n, dmm);
-            OG_TRY(CheckLaunch("join_minmax"));
-        }
+        long long mm[2];
         OG_HIP(hipMemcpyAsync(mm, dmm, sizeof(mm), hipMemcpyDeviceToHost, s));
         OG_HIP(hipStreamSynchronize(s));
         if (mm[1] >= mm[0]) {
@@ -579,8 +364,8 @@ int omni_gpu_join_build_finish(void *handle)
             }
         }
     }
+    h.pending.clear();
     unsigned int *has_dup = reinterpret_cast<unsigned int *>(sc + 4);
-    unsigned int *max_len = reinterpret_cast<unsigned int *>(sc + 8);
     if (arrayOk) {
         h.is_array = true;
         OG_TRY(h.heads.Reserve(sizeof(int) * h.range, s));
@@ -608,41 +393,51 @@ int omni_gpu_join_build_finish(void *handle)
         KernelScope ks(s, "join_build");
         join_build_hash<<<Blocks(h.n), kBlock, 0, s>>>(h.keys.as<long long>(), h.knull.as<uint8_t>(), h.n,
                                                        h.slots.as<JoinSlot>(), h.slots_n - 1, h.next.as<int>(),
-                                                       reinterpret_cast<int *>(sc), has_dup,
+                                                       reinterpret_cast<int *>(sc), reinterpret_cast<unsigned int *>(sc + 12),
+                                                       has_dup,
                                                        h.inline_pay ? h.pay[0].p : nullptr,
                                                        h.inline_pay ? TypeWidth(h.spec.payload_types[0]) : 0);
         OG_TRY(CheckLaunch("join_build_hash"));
     }
-    unsigned int flags[2] = {0, 0};
-    OG_HIP(hipMemcpyAsync(flags, has_dup, 8, hipMemcpyDeviceToHost, s));
+    // the probe kernel variant depends on whether any key repeats
+    unsigned int dup = 0;
+    OG_HIP(hipMemcpyAsync(&dup, has_dup, 4, hipMemcpyDeviceToHost, s));
     OG_HIP(hipStreamSynchronize(s));
-    h.has_dup = flags[0] != 0;
-    if (h.has_dup) {
-        h.inline_pay = false;
-        {
-            KernelScope ks(s, "join_max_chain");
-            if (h.is_array) {
-                join_max_chain<<<Blocks(static_cast<int64_t>(h.range)), kBlock, 0, s>>>(h.heads.as<int>(),
-                                                                                        static_cast<int64_t>(h.range), 1,
-                                                                                        h.next.as<int>(), max_len);
-            } else {
-                join_max_chain<<<Blocks(static_cast<int64_t>(h.slots_n)), kBlock, 0, s>>>(
-                    reinterpret_cast<const int *>(h.slots.as<uint8_t>() + 8), static_cast<int64_t>(h.slots_n), 4,
-                    h.next.as<int>(), max_len);
-            }
-            OG_TRY(CheckLaunch("join_max_chain"));
+    h.has_dup = dup != 0;
+    if (h.has_dup) h.inline_pay = false;
+    h.max_dup = h.has_dup ? -1 : 1;  // longest chain: measured on first request
+    return OMNI_GPU_OK;
+}
+
+// Longest build-row chain of one key (the largest number of matches one probe row can produce).
+static int MaxChain(JoinHandle &h)
+{
+    hipStream_t s = h.stream;
+    uint8_t *sc = h.scratch.as<uint8_t>();
+    unsigned int *max_len = reinterpret_cast<unsigned int *>(sc + 8);
+    OG_HIP(hipMemsetAsync(max_len, 0, 4, s));
+    {
+        KernelScope ks(s, "join_max_chain");
+        if (h.is_array) {
+            join_max_chain<<<Blocks(static_cast<int64_t>(h.range)), kBlock, 0, s>>>(h.heads.as<int>(),
+                                                                                    static_cast<int64_t>(h.range), 1,
+                                                                                    h.next.as<int>(), max_len);
+        } else {
+            join_max_chain<<<Blocks(static_cast<int64_t>(h.slots_n)), kBlock, 0, s>>>(
+                reinterpret_cast<const int *>(h.slots.as<uint8_t>() + 8), static_cast<int64_t>(h.slots_n), 4,
+                h.next.as<int>(), max_len);
         }
-        if (!h.is_array) {  // the sentinel-key chain is outside the table
-            KernelScope ks(s, "join_max_chain");
-            join_max_chain<<<1, kBlock, 0, s>>>(reinterpret_cast<const int *>(sc), 1, 1, h.next.as<int>(), max_len);
-            OG_TRY(CheckLaunch("join_max_chain"));
-        }
-        OG_HIP(hipMemcpyAsync(flags, has_dup, 8, hipMemcpyDeviceToHost, s));
-        OG_HIP(hipStreamSynchronize(s));
-        h.max_dup = flags[1];
-    } else {
-        h.max_dup = 1;
+        OG_TRY(CheckLaunch("join_max_chain"));
     }
+    if (!h.is_array) {  // the sentinel-key chain is outside the table
+        KernelScope ks(s, "join_max_chain");
+        join_max_chain<<<1, kBlock, 0, s>>>(reinterpret_cast<const int *>(sc), 1, 1, h.next.as<int>(), max_len);
+        OG_TRY(CheckLaunch("join_max_chain"));
+    }
+    unsigned int len = 0;
+    OG_HIP(hipMemcpyAsync(&len, max_len, 4, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
+    h.max_dup = len;
     return OMNI_GPU_OK;
 }
 
@@ -651,9 +446,32 @@ int omni_gpu_join_build_info(void *handle, int32_t *is_array, int64_t *max_dup, 
     OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "build side not finished");
+    if (max_dup && h.max_dup < 0) OG_TRY(MaxChain(h));
     if (is_array) *is_array = h.is_array ? 1 : 0;
     if (max_dup) *max_dup = h.max_dup;
     if (table_slots) *table_slots = h.is_array ? static_cast<int64_t>(h.range) : static_cast<int64_t>(h.slots_n);
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_join_tune(void *handle, int32_t probe_group, int32_t partitions)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    OG_CHECK(probe_group == 1 || probe_group == 2 || probe_group == 4 || probe_group == 8 || probe_group == 16,
+             OMNI_GPU_ERR_INVALID, "probe_group must be 1, 2, 4, 8 or 16");
+    OG_CHECK(partitions == -1 || partitions == 0 ||
+                 (partitions >= kXcds && partitions <= kPartMax && (partitions & (partitions - 1)) == 0),
+             OMNI_GPU_ERR_INVALID, "partitions must be -1 (auto), 0 (off) or a power of two in [%d, %d]", kXcds,
+             kPartMax);
+    JoinHandle &h = *static_cast<JoinHandle *>(handle);
+    h.probe_group = probe_group;
+    h.partitions = partitions;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_join_probe_info(void *handle, int32_t *partitions)
+{
+    OG_CHECK(handle && partitions, OMNI_GPU_ERR_INVALID, "null argument");
+    *partitions = static_cast<JoinHandle *>(handle)->last_partitions;
     return OMNI_GPU_OK;
 }
 
@@ -703,10 +521,12 @@ int omni_gpu_join_destroy(void *handle)
     h->next.Release(s);
     h->scratch.Release(s);
     h->batch_desc.Release(s);
-    if (s) {
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
-    }
+    h->build_desc.Release(s);
+    h->part_keys.Release(s);
+    h->part_rows.Release(s);
+    h->part_counts.Release(s);
+    h->part_meta.Release(s);
+    ReleaseStream(s);
     delete h;
     return OMNI_GPU_OK;
 }

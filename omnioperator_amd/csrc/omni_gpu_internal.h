@@ -8,6 +8,12 @@
 
 #include "../../include/omni_gpu.h"
 
+// Global (HBM) address space qualifier for pointers the compiler cannot otherwise prove global, so that loads
+// through them are global rather than flat loads.
+#ifndef OG_GLOBAL
+#define OG_GLOBAL __attribute__((address_space(1)))
+#endif
+
 namespace omni_gpu {
 
 // ---- errors ---------------------------------------------------------------------------------------------
@@ -39,6 +45,12 @@ int CheckLaunch(const char *kernel);
 // freed blocks stay cached in the pool and re-allocation in a steady state costs no driver call).
 int DevAlloc(void **p, size_t bytes, hipStream_t s);
 void DevFree(void *p, hipStream_t s);
+
+// Per-device pool of non-blocking streams (default priority).  Each handle owns one stream while it
+// lives; creating and destroying a HIP stream per operator would cost more host time than the operator's
+// own work on small inputs.  ReleaseStream synchronizes the stream before returning it to the pool.
+int AcquireStream(hipStream_t *s);
+void ReleaseStream(hipStream_t s);
 
 struct DevBuf {
     void *p = nullptr;

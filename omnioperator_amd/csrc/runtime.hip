@@ -82,6 +82,43 @@ void DevFree(void *p, hipStream_t s)
     if (p) (void)hipFreeAsync(p, s);
 }
 
+namespace {
+std::mutex g_stream_mu;
+std::map<int, std::vector<hipStream_t>> g_idle_streams;
+}  // namespace
+
+int AcquireStream(hipStream_t *s)
+{
+    int dev = CurrentDevice();
+    {
+        std::lock_guard<std::mutex> g(g_stream_mu);
+        auto &idle = g_idle_streams[dev];
+        if (!idle.empty()) {
+            *s = idle.back();
+            idle.pop_back();
+            return OMNI_GPU_OK;
+        }
+    }
+    OG_HIP(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+    return OMNI_GPU_OK;
+}
+
+void ReleaseStream(hipStream_t s)
+{
+    if (!s) return;
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        (void)hipStreamDestroy(s);  // a stream in an error state is not reused
+        return;
+    }
+    int dev = 0;
+    if (hipStreamGetDevice(s, &dev) != hipSuccess) {
+        (void)hipStreamDestroy(s);
+        return;
+    }
+    std::lock_guard<std::mutex> g(g_stream_mu);
+    g_idle_streams[dev].push_back(s);
+}
+
 int DevBuf::Grow(size_t need, size_t keep, hipStream_t s)
 {
     if (need <= bytes && p) return OMNI_GPU_OK;

@@ -12,8 +12,11 @@ pytestmark = pytest.mark.gpu
 FIX = load_golden("operator_fixtures.json")
 
 
-def gpu_join(bk: Col, bpay, pk: Col, build_batch, probe_batch, strategy=0, batched=False):
+def gpu_join(bk: Col, bpay, pk: Col, build_batch, probe_batch, strategy=0, batched=False, probe_group=None,
+             partitions=-1):
     j = capi.HashJoin(bk.type_id, [p.type_id for p in bpay], strategy)
+    if probe_group is not None or partitions != -1:
+        j.tune(probe_group or 1, partitions)
     keep = []  # build batches stay alive until finish (the copies are asynchronous)
     for a, b in workloads.batches(len(bk), build_batch):
         kd = capi.DeviceColumn(bk.type_id, bk.values[a:b], None if bk.nulls is None else bk.nulls[a:b])
@@ -105,6 +108,74 @@ def test_random_parity(dense, nulls, batched):
     want, is_array = oracle_set(bk, pays, pk, 8192, 65536)
     assert info["is_array"] == is_array
     assert as_set(rows, gp) == want
+
+
+@pytest.mark.parametrize("probe_group", [1, 2, 4, 16])
+@pytest.mark.parametrize("unique", [False, True])
+def test_probe_width_parity(probe_group, unique):
+    """Every probe width gives the oracle's row set: hash table with duplicate keys (chains), unique keys
+    with an inline payload, and the array table; NULL probe keys and a ragged last batch included."""
+    rng = np.random.default_rng(7 + probe_group)
+    nb, npr = 20_000, 100_003
+    if unique:
+        bkv = rng.permutation(np.arange(-2**40, -2**40 + 3 * nb, 3, dtype=np.int64))[:nb]
+    else:
+        bkv = rng.integers(-2**62, -2**62 + 15_000, nb, dtype=np.int64)
+    pkv = np.where(rng.random(npr) < 0.7, bkv[rng.integers(0, nb, npr)], rng.integers(-2**63, 2**63 - 1, npr, dtype=np.int64))
+    bk = Col(LONG, bkv)
+    pk = Col(LONG, pkv, rng.random(npr) < 0.02)
+    pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32))]
+    for strategy in (0, 1):
+        rows, gp, _ = gpu_join(bk, pays, pk, 4096, 65536, strategy=strategy, batched=True, probe_group=probe_group)
+        want, _ = oracle_set(bk, pays, pk, 4096, 65536)
+        assert as_set(rows, gp) == want
+
+
+def test_probe_width_rejects_other_values():
+    j = capi.HashJoin(LONG, [])
+    with pytest.raises(capi.GpuError):
+        j.tune(3)
+    with pytest.raises(capi.GpuError):
+        j.tune(1, 12)
+    j.close()
+
+
+@pytest.mark.parametrize("partitions", [8, 64, 1024])
+@pytest.mark.parametrize("unique", [False, True])
+def test_partitioned_probe_parity(partitions, unique):
+    """Radix-partitioned probe (histogram, scan, scatter, per-XCD persistent probe) against the oracle:
+    duplicate-key chains and inline payloads, NULL probe keys, the slot sentinel as a real key on both sides,
+    ragged batches."""
+    rng = np.random.default_rng(11 + partitions + unique)
+    nb, npr = 30_000, 250_001
+    if unique:
+        bkv = rng.permutation(np.arange(-2**50, -2**50 + 5 * nb, 5, dtype=np.int64))[:nb]
+    else:
+        bkv = rng.integers(2**40, 2**40 + 20_000, nb, dtype=np.int64)
+    bkv[7] = -2**63
+    pkv = np.where(rng.random(npr) < 0.75, bkv[rng.integers(0, nb, npr)],
+                   rng.integers(-2**63, 2**63 - 1, npr, dtype=np.int64))
+    pkv[[0, 99, npr - 1]] = -2**63
+    bk = Col(LONG, bkv)
+    pk = Col(LONG, pkv, rng.random(npr) < 0.02)
+    pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32))]
+    rows, gp, info = gpu_join(bk, pays, pk, 8192, 65536, strategy=1, batched=True, partitions=partitions)
+    want, _ = oracle_set(bk, pays, pk, 8192, 65536)
+    assert as_set(rows, gp) == want
+    assert not info["is_array"]
+
+
+def test_sentinel_key_inline_payload():
+    """Unique keys with a <= 4-byte payload keep the payload in the slot; the sentinel key's row lives
+    outside the table and must still return its own payload."""
+    bk = Col(LONG, np.array([-2**63, 3, 9, 2**63 - 1], np.int64))
+    pk = Col(LONG, np.array([9, -2**63, 4, -2**63, 3], np.int64))
+    pay = [Col(INT, np.array([101, 102, 103, 104], np.int32))]
+    for batched in (False, True):
+        rows, gp, info = gpu_join(bk, pay, pk, 4, 4, strategy=1, batched=batched)
+        want, _ = oracle_set(bk, pay, pk, 4, 4)
+        assert as_set(rows, gp) == want
+        assert info["max_dup"] == 1
 
 
 @pytest.mark.parametrize("key_type", [INT, SHORT])
