@@ -71,10 +71,11 @@ def bench_join(args):
         j.finish()
         m = j.probe_batches_raw(pcols, prows, out_rows, oarr, cap)
         parts[0] = j.last_partitions()
+        is_array[0] = j.info()["is_array"]
         j.close()
         return m
 
-    parts = [0]
+    parts, is_array = [0], [False]
 
     for _ in range(args.warmup):
         m = step()
@@ -99,8 +100,11 @@ def bench_join(args):
     avg_probe_ms = probe_ms / args.steps
     probe_bytes = npr * 8 + m * (8 + 4)
     achieved = probe_bytes / (avg_probe_ms / 1e3) / 1e9
+    variant = (f"join_probe<{'true' if is_array[0] else 'false'}, {'false' if is_array[0] else 'true'}, false, "
+               f"{args.probe_group}>") if parts[0] <= 1 else None
+    traffic = profiled_traffic(variant, "dense" if args.dense else "sparse") if variant else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "join_probe" if parts[0] <= 1 else
                           f"join_probe_* (radix-partitioned, {parts[0]} partitions: hist+scan+scatter+probe)",
                 "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
@@ -116,6 +120,21 @@ def bench_join(args):
     cfg.update(extra)
     return {"value": value, "ms_per_step": ms_per_step, "config": cfg, "roofline": roofline,
             "cpu_baseline": cpu, "dtype": "int64"}
+
+
+def profiled_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
+    profiles/join_probe_c2.md (copy-calibrated); None when that kernel variant was not profiled."""
+    path = os.path.join(ROOT, "profiles", "join_probe_c2.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 6 and cells[0] == kernel and cells[1] == workload:
+                    return int(cells[4])
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline_join(w, args):

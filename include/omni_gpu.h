@@ -167,7 +167,7 @@ int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_by
 int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
                              const int64_t *batch_offsets, int32_t num_batches, const int32_t *batch_modes,
                              int32_t flags, omni_gpu_out_column *outs, int64_t *groups_per_batch);
-/* Batches finished in LDS / in HBM tables since the last call. */
+/* Batches of the last run aggregated in LDS tables / in HBM tables. */
 int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_batches);
 int omni_gpu_agg_batches_destroy(void *handle);
 
