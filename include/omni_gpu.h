@@ -226,6 +226,45 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
                                 int64_t *num_matches);
 int omni_gpu_join_destroy(void *handle);
 
+/* ---- learned strategy / capacity advisor ----------------------------------------------------------------
+ * No reference counterpart: the reference sizes its tables by growth only (BaseHashMap's Grower,
+ * core/src/operator/hashmap/base_hash_map.h:255-292) and picks the table kind by a fixed rule.  These entry
+ * points compute a sample sketch of key columns, and run / train a 16-64-64-4 fp32 MLP that predicts log2 of
+ * the group count (output 0) and the fastest strategy (outputs 1..3: LDS-local, global-atomic,
+ * radix-partitioned).  The prediction only sets a capacity hint and a strategy: results never depend on it. */
+typedef struct omni_gpu_sketch {
+    float features[16];     /* MLP input (see DESIGN) */
+    int64_t rows, sampled;  /* rows of the column, rows in the 1-in-k sample */
+    int64_t distinct, f1, f2, max_count, heavy, nulls;  /* over the sample: distinct keys, keys seen once / twice,
+                                                          the top key's count, rows of keys seen >= 4 times */
+    int64_t key_min, key_max;  /* range of the first key column over the sample */
+} omni_gpu_sketch;
+
+/* Sketch of one or two integer-like key columns of `rows` rows (device columns). */
+int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
+                         omni_gpu_sketch *out);
+/* MLP handle with He-uniform weights drawn from `seed`.  Parameters are the flattened W1[16x64] b1[64]
+ * W2[64x64] b2[64] W3[64x4] b3[4] (row-major, y = x @ W + b); host arrays below. */
+int omni_gpu_mlp_create(uint64_t seed, void **handle);
+int omni_gpu_mlp_num_params(void);
+int omni_gpu_mlp_set_params(void *handle, const float *params);
+int omni_gpu_mlp_get_params(void *handle, float *params);
+/* Y[n][4] = MLP(X[n][16]). */
+int omni_gpu_mlp_forward(void *handle, const float *X, int32_t n, float *Y);
+/* Gradient of loss = mean (y0 - t)^2 + mean cross-entropy(softmax(y1..3), c) over the n samples (c in 0..2). */
+int omni_gpu_mlp_gradient(void *handle, const float *X, const float *t, const int32_t *c, int32_t n, float *grad,
+                          float *loss);
+/* Adam training (beta 0.9 / 0.999) on the GPU: `epochs` passes over the n samples in shuffled mini-batches;
+ * epoch_loss (optional, `epochs` floats) receives each epoch's mean loss. */
+int omni_gpu_mlp_train(void *handle, const float *X, const float *t, const int32_t *c, int32_t n, int32_t batch,
+                       int32_t epochs, float lr, uint64_t seed, float *epoch_loss);
+int omni_gpu_mlp_destroy(void *handle);
+/* Seeded synthetic keys on the device (the advisor's training runs): `rows` keys whose rank follows a Zipf(s)
+ * law over `distinct` ranks (s = 0: uniform), mapped through a bijection.  kind 0: int64 into k64; 1: int32
+ * into k32; 2: (int32, int64) pairs into (k32, k64).  v64 (optional): int64 values in [-1e6, 1e6]. */
+int omni_gpu_generate_keys(int32_t kind, int64_t rows, double distinct, double zipf_s, uint64_t seed, void *k64,
+                           void *k32, void *v64);
+
 #ifdef __cplusplus
 }
 #endif

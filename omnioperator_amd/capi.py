@@ -58,6 +58,12 @@ class JoinSpec(C.Structure):
                 ("strategy", C.c_int32)]
 
 
+class Sketch(C.Structure):
+    _fields_ = [("features", C.c_float * 16), ("rows", C.c_int64), ("sampled", C.c_int64),
+                ("distinct", C.c_int64), ("f1", C.c_int64), ("f2", C.c_int64), ("max_count", C.c_int64),
+                ("heavy", C.c_int64), ("nulls", C.c_int64), ("key_min", C.c_int64), ("key_max", C.c_int64)]
+
+
 # Every exported symbol of include/omni_gpu.h with its ctypes signature.
 _I, _I32, _I64, _VP, _SZ, _D = C.c_int, C.c_int32, C.c_int64, C.c_void_p, C.c_size_t, C.c_double
 _P = C.POINTER
@@ -96,6 +102,17 @@ SIGNATURES = {
     "omni_gpu_join_probe": (_I, [_VP, _P(Column), _I64, _I64, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_probe_batches": (_I, [_VP, _P(Column), _P(_I64), _I32, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_destroy": (_I, [_VP]),
+    "omni_gpu_sketch_keys": (_I, [_P(Column), _I32, _I64, _I32, _P(Sketch)]),
+    "omni_gpu_mlp_create": (_I, [C.c_uint64, _P(_VP)]),
+    "omni_gpu_mlp_num_params": (_I, []),
+    "omni_gpu_mlp_set_params": (_I, [_VP, _P(C.c_float)]),
+    "omni_gpu_mlp_get_params": (_I, [_VP, _P(C.c_float)]),
+    "omni_gpu_mlp_forward": (_I, [_VP, _P(C.c_float), _I32, _P(C.c_float)]),
+    "omni_gpu_mlp_gradient": (_I, [_VP, _P(C.c_float), _P(C.c_float), _P(_I32), _I32, _P(C.c_float), _P(C.c_float)]),
+    "omni_gpu_mlp_train": (_I, [_VP, _P(C.c_float), _P(C.c_float), _P(_I32), _I32, _I32, _I32, C.c_float, C.c_uint64,
+                                _P(C.c_float)]),
+    "omni_gpu_mlp_destroy": (_I, [_VP]),
+    "omni_gpu_generate_keys": (_I, [_I32, _I64, _D, _D, C.c_uint64, _VP, _VP, _VP]),
 }
 
 _lib = None
@@ -438,6 +455,89 @@ class HashJoin:
             self.close()
         except Exception:
             pass
+
+
+def sketch_keys(keys: Sequence[Column], rows: int, is_join: bool = False) -> Sketch:
+    """Sample sketch of one or two device key columns (omni_gpu_sketch_keys)."""
+    init()
+    karr = (Column * len(keys))(*keys)
+    out = Sketch()
+    check(lib().omni_gpu_sketch_keys(karr, len(keys), rows, 1 if is_join else 0, C.byref(out)))
+    return out
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Mlp:
+    """The advisor's 16-64-64-4 fp32 MLP on the GPU (omni_gpu_mlp_*)."""
+
+    def __init__(self, seed: int = 0):
+        init()
+        h = C.c_void_p()
+        check(lib().omni_gpu_mlp_create(seed, C.byref(h)))
+        self.h = h.value
+        self.n_params = lib().omni_gpu_mlp_num_params()
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.n_params, np.float32)
+        check(lib().omni_gpu_mlp_get_params(self.h, _fptr(p)))
+        return p
+
+    def set_params(self, p: np.ndarray):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.shape == (self.n_params,)
+        check(lib().omni_gpu_mlp_set_params(self.h, _fptr(p)))
+
+    def forward(self, X: np.ndarray) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), 4), np.float32)
+        check(lib().omni_gpu_mlp_forward(self.h, _fptr(X), len(X), _fptr(Y)))
+        return Y
+
+    def gradient(self, X: np.ndarray, t: np.ndarray, c: np.ndarray):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        c = np.ascontiguousarray(c, dtype=np.int32)
+        g = np.zeros(self.n_params, np.float32)
+        loss = C.c_float()
+        check(lib().omni_gpu_mlp_gradient(self.h, _fptr(X), _fptr(t), c.ctypes.data_as(C.POINTER(C.c_int32)), len(X),
+                                          _fptr(g), C.byref(loss)))
+        return g, loss.value
+
+    def train(self, X, t, c, batch: int = 128, epochs: int = 50, lr: float = 3e-3, seed: int = 0) -> np.ndarray:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        c = np.ascontiguousarray(c, dtype=np.int32)
+        losses = np.zeros(epochs, np.float32)
+        check(lib().omni_gpu_mlp_train(self.h, _fptr(X), _fptr(t), c.ctypes.data_as(C.POINTER(C.c_int32)), len(X),
+                                       batch, epochs, lr, seed, _fptr(losses)))
+        return losses
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().omni_gpu_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def generate_keys(kind: int, rows: int, distinct: float, zipf_s: float, seed: int, with_values: bool = True):
+    """Device key columns from the seeded generator: kind 0 int64, 1 int32, 2 (int32, int64); returns
+    ([key DeviceArrays...], value DeviceArray or None)."""
+    init()
+    k64 = DeviceArray(max(1, rows) * 8, np.int64, (rows,)) if kind != 1 else None
+    k32 = DeviceArray(max(1, rows) * 4, np.int32, (rows,)) if kind != 0 else None
+    v = DeviceArray(max(1, rows) * 8, np.int64, (rows,)) if with_values else None
+    check(lib().omni_gpu_generate_keys(kind, rows, float(distinct), float(zipf_s), seed, k64.ptr if k64 else None,
+                                       k32.ptr if k32 else None, v.ptr if v else None))
+    keys = [k64] if kind == 0 else ([k32] if kind == 1 else [k32, k64])
+    return keys, v
 
 
 def profile_enable(on: bool = True):

@@ -29,16 +29,16 @@ struct AggDev {
     int out_width;    // bytes per output value
     const void *values;
     const uint8_t *nulls;
-    void *s0;
-    void *s1;
-    void *s2;
-    uint8_t *flag;    // 1 once a non-NULL value was added
+    int off[3];       // byte offsets of the state words s0, s1, s2 within a group's state row (-1 unused)
+    int flag_off;     // byte offset of the "a non-NULL value was added" flag (-1 unused)
     int dec_scale_up; // 10^(outScale - inScale) exponent for decimal SUM
     int dec_out128;   // decimal SUM output is DECIMAL128
 };
 
 struct AggArgs {
     int n;
+    int row_bytes;        // bytes of one group's state row (a multiple of 8)
+    uint8_t *states;      // state rows, indexed by group id (array of structures: one row per group)
     AggDev a[OMNI_GPU_MAX_AGGS];
 };
 
@@ -59,33 +59,90 @@ struct Slot1 {
 };
 
 struct GroupCtx {
-    unsigned long long *ngroups;   // group counter
+    unsigned long long *ngroups;   // group ids handed out so far (in chunks)
     unsigned long long max_groups; // capacity of the per-group arrays
     unsigned int *overflow;        // set when a group id would exceed max_groups (host sizing bug guard)
     unsigned long long *gkeys;
     uint8_t *gnull;
+    uint8_t *gvalid;               // 1 once a group id holds a group (ids reserved but unused stay 0)
     unsigned int *special;         // [0] NULL group, [1] sentinel-key group (0 empty, 1 busy, gid+2)
+    unsigned long long *wave;      // this wave's id chunk in LDS: [0] next id, [1] end of the chunk
+    unsigned long long chunk;      // ids a wave takes from ngroups at a time
 };
 
 constexpr int kMaxWords = 6;
 
-__device__ __forceinline__ unsigned int LoadAcquire(unsigned int *p)
+// Table words shared by all workgroups are read and written with agent-scope atomics.  On MI355X each of the
+// eight XCDs has its own L2, so an agent-scope acquire invalidates the XCD's L2 and a release writes it back;
+// the hot paths therefore use relaxed accesses, which go to the coherence point past the L2.
+__device__ __forceinline__ unsigned int LoadRelaxed32(unsigned int *p)
 {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ unsigned long long LoadRelaxed64(unsigned long long *p)
 {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void StoreRelease(unsigned int *p, unsigned int v)
+__device__ __forceinline__ void StoreRelaxed32(unsigned int *p, unsigned int v)
 {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void StoreRelaxed64(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Claims a fresh group id; returns ~0 if the per-group arrays are full (never expected: the host sizes them).
+// Publishes `v` at *p after every earlier store of the calling wave has completed.  The key words a reader
+// compares are written with agent-scope (write-through) stores, so waiting for their completion (vmcnt 0)
+// before the flag store orders them for any later agent-scope reader without the agent-scope release fence,
+// which would write back this XCD's whole L2 for every new group.
+__device__ __forceinline__ void PublishAfterStores(unsigned int *p, unsigned int v)
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0); expcnt and lgkmcnt left at their maximum
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long LdsLoad(unsigned long long *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+__device__ __forceinline__ void LdsStore(unsigned long long *p, unsigned long long v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
+// Claims a fresh group id for every calling lane.  Each wave owns a chunk of ids [next, end) in LDS and takes a
+// new chunk of g.chunk ids from the global counter only when the calling lanes need more than it holds, so the
+// counter sees one atomic per chunk instead of one per wave.  The chunk cursor is read and written through
+// relaxed atomics because lanes of the wave read what its leader wrote.  Returns ~0 if the per-group arrays are
+// full (never expected: the host sizes them).
 __device__ __forceinline__ unsigned int NewGroup(const GroupCtx &g)
 {
-    unsigned long long id = WaveReserveOne(g.ngroups, true);
+    const unsigned long long act = __ballot(1);
+    const unsigned int k = static_cast<unsigned int>(__popcll(act));
+    const unsigned int r = static_cast<unsigned int>(__popcll(act & LaneMaskLt()));
+    const int leader = __ffsll(static_cast<long long>(act)) - 1;
+    const bool lead = LaneId() == leader;
+    const unsigned long long next = LdsLoad(&g.wave[0]), end = LdsLoad(&g.wave[1]);
+    const unsigned long long have = end - next;
+    unsigned long long id;
+    if (have >= k) {
+        id = next + r;
+        if (lead) LdsStore(&g.wave[0], next + k);
+    } else {
+        const unsigned long long need = k - have;
+        const unsigned long long take = need > g.chunk ? need : g.chunk;
+        unsigned long long base = 0;
+        if (lead) base = atomicAdd(g.ngroups, take);
+        base = __shfl(base, leader, 64);
+        id = r < have ? next + r : base + (r - have);
+        if (lead) {
+            LdsStore(&g.wave[0], base + need);
+            LdsStore(&g.wave[1], base + take);
+        }
+    }
     if (id >= g.max_groups) {
         atomicOr(g.overflow, 1u);
         return 0xFFFFFFFFu;
@@ -93,22 +150,40 @@ __device__ __forceinline__ unsigned int NewGroup(const GroupCtx &g)
     return static_cast<unsigned int>(id);
 }
 
+// Loads (at kernel start) and stores back (at kernel end) the id chunk of wave slot `slot`.  All lanes of the
+// wave call both; between them the chunk lives in `lds` (two words per wave).
+__device__ __forceinline__ void LoadWaveChunk(unsigned long long *lds, const unsigned long long *cursors, int slot)
+{
+    if (LaneId() == 0) {
+        LdsStore(&lds[0], cursors[2 * slot]);
+        LdsStore(&lds[1], cursors[2 * slot + 1]);
+    }
+}
+__device__ __forceinline__ void StoreWaveChunk(const unsigned long long *lds, unsigned long long *cursors, int slot)
+{
+    if (LaneId() == 0) {
+        cursors[2 * slot] = LdsLoad(const_cast<unsigned long long *>(&lds[0]));
+        cursors[2 * slot + 1] = LdsLoad(const_cast<unsigned long long *>(&lds[1]));
+    }
+}
+
 // Group of a side cell (NULL key, or the key equal to the slot sentinel), created on first use.
 static __device__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned long long key, bool isNull, int W)
 {
     unsigned int *cell = g.special + which;
     while (true) {
-        unsigned int c = LoadAcquire(cell);
+        unsigned int c = LoadRelaxed32(cell);
         if (c >= 2) return c - 2;
         if (c == 0 && atomicCAS(cell, 0u, 1u) == 0u) {
             unsigned int gid = NewGroup(g);
             if (gid == 0xFFFFFFFFu) {
-                StoreRelease(cell, 0u);
+                StoreRelaxed32(cell, 0u);
                 return gid;
             }
             g.gkeys[static_cast<size_t>(gid) * W] = key;
             if (g.gnull) g.gnull[gid] = isNull ? 1 : 0;
-            StoreRelease(cell, gid + 2);
+            g.gvalid[gid] = 1;
+            StoreRelaxed32(cell, gid + 2);  // readers only need the id (states were initialised before)
             return gid;
         }
     }
@@ -125,15 +200,20 @@ static __device__ unsigned int InsertSingle(Slot1 *slots, unsigned long long mas
             k = atomicCAS(&slots[pos].key, kEmptyKey, key);
             if (k == kEmptyKey) {
                 unsigned int gid = NewGroup(g);
-                if (gid == 0xFFFFFFFFu) return gid;
+                if (gid == 0xFFFFFFFFu) {
+                    StoreRelaxed32(&slots[pos].gid1, 0xFFFFFFFFu);  // lanes waiting on this slot give up too
+                    return gid;
+                }
                 g.gkeys[gid] = key;
                 if (g.gnull) g.gnull[gid] = 0;
-                StoreRelease(&slots[pos].gid1, gid + 1);
+                g.gvalid[gid] = 1;
+                StoreRelaxed32(&slots[pos].gid1, gid + 1);  // readers only need the id
                 return gid;
             }
         }
         if (k == key) {
-            unsigned int g1 = LoadAcquire(&slots[pos].gid1);
+            unsigned int g1 = LoadRelaxed32(&slots[pos].gid1);
+            if (g1 == 0xFFFFFFFFu) return g1;  // the arrays were full when this key arrived
             if (g1) return g1 - 1;
             continue;  // the winning lane publishes its group id; re-read the same slot
         }
@@ -148,20 +228,21 @@ static __device__ unsigned int InsertMulti(unsigned int *ctrl, unsigned long lon
 {
     unsigned long long pos = hash & mask;
     while (true) {
-        unsigned int c = LoadAcquire(&ctrl[pos]);
+        unsigned int c = LoadRelaxed32(&ctrl[pos]);
         if (c == 0) {
             c = atomicCAS(&ctrl[pos], 0u, 1u);
             if (c == 0) {
                 unsigned int gid = NewGroup(g);
                 if (gid == 0xFFFFFFFFu) {
-                    StoreRelease(&ctrl[pos], 0u);
+                    StoreRelaxed32(&ctrl[pos], 0u);
                     return gid;
                 }
                 for (int w = 0; w < W; ++w) {
-                    skeys[pos * W + w] = words[w];
-                    g.gkeys[static_cast<size_t>(gid) * W + w] = words[w];
+                    StoreRelaxed64(&skeys[pos * W + w], words[w]);
+                    g.gkeys[static_cast<size_t>(gid) * W + w] = words[w];  // read by later kernels only
                 }
-                StoreRelease(&ctrl[pos], gid + 2);
+                g.gvalid[gid] = 1;
+                PublishAfterStores(&ctrl[pos], gid + 2);
                 return gid;
             }
         }
@@ -240,12 +321,15 @@ __device__ __forceinline__ void DecInput(const AggDev &d, int64_t row, unsigned 
 struct GlobalRef {
     const AggArgs &args;
     unsigned int gid;
+    __device__ __forceinline__ uint8_t *row() const
+    {
+        return args.states + static_cast<size_t>(gid) * static_cast<size_t>(args.row_bytes);
+    }
     template <typename T> __device__ __forceinline__ T *p(int i, int w) const
     {
-        const AggDev &d = args.a[i];
-        return static_cast<T *>(w == 0 ? d.s0 : (w == 1 ? d.s1 : d.s2)) + gid;
+        return reinterpret_cast<T *>(row() + args.a[i].off[w]);
     }
-    __device__ __forceinline__ uint8_t *flag(int i) const { return args.a[i].flag + gid; }
+    __device__ __forceinline__ uint8_t *flag(int i) const { return row() + args.a[i].flag_off; }
 };
 
 struct LdsLayout {

@@ -1,0 +1,246 @@
+// Kernels of the group-by hash aggregation (included by agg.hip).
+#pragma once
+
+#include "agg_device.h"
+
+namespace omni_gpu {
+
+// ---- kernels ------------------------------------------------------------------------------------------------
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kWaveSize;
+constexpr int kCursorSlots = 16384;       // wave slots of the persistent insert launches (id chunks live there)
+constexpr int kMaxInsertBlocks = kCursorSlots / kWavesPerBlock;
+
+// Rows are visited grid-stride by a fixed grid, so wave slot w keeps its id chunk across launches.
+__global__ void __launch_bounds__(kBlock) agg_insert_single(KeyDev kd, int64_t rows, Slot1 *slots,
+                                                            unsigned long long mask, GroupCtx g, AggArgs args,
+                                                            unsigned long long *cursors)
+{
+    __shared__ unsigned long long wc[kWavesPerBlock][2];
+    const int slot = blockIdx.x * kWavesPerBlock + (WaveInBlock());
+    LoadWaveChunk(wc[WaveInBlock()], cursors, slot);
+    g.wave = wc[WaveInBlock()];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < rows; row += stride) {
+        unsigned int gid;
+        if (NullBit(kd.nulls[0], row)) {
+            gid = SideGroup(g, 0, 0ULL, true, 1);
+        } else {
+            unsigned long long key = LoadWord(kd.values[0], kd.width[0], row);
+            gid = key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g);
+        }
+        if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+    }
+    StoreWaveChunk(wc[WaveInBlock()], cursors, slot);
+}
+
+__global__ void __launch_bounds__(kBlock) agg_insert_multi(KeyDev kd, int64_t rows, unsigned int *ctrl,
+                                                           unsigned long long *skeys, unsigned long long mask,
+                                                           GroupCtx g, AggArgs args, unsigned long long *cursors)
+{
+    __shared__ unsigned long long wc[kWavesPerBlock][2];
+    const int slot = blockIdx.x * kWavesPerBlock + (WaveInBlock());
+    LoadWaveChunk(wc[WaveInBlock()], cursors, slot);
+    g.wave = wc[WaveInBlock()];
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < rows; row += stride) {
+        unsigned long long words[kMaxWords];
+        EncodeKey(kd, row, words);
+        unsigned int gid = InsertMulti(ctrl, skeys, mask, words, kd.W, HashWords(words, kd.W), g);
+        if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+    }
+    StoreWaveChunk(wc[WaveInBlock()], cursors, slot);
+}
+
+constexpr int kLdsBlock = 1024;
+constexpr int kLdsWaves = kLdsBlock / kWaveSize;
+constexpr int kLdsMaxProbe = 32;
+constexpr int kMaxLdsBlocks = kCursorSlots / kLdsWaves;
+
+// LDS-local partial aggregation (single-word keys).  Each workgroup aggregates a contiguous chunk of rows
+// in a private LDS table of L.S slots (linear probing, 64-bit LDS CAS), then merges every occupied slot into
+// the global table with one global insert and one atomic per aggregate.  Rows whose key is NULL or the
+// slot sentinel, or that find no LDS slot within kLdsMaxProbe steps, go straight to the global table.
+__global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t rows, int64_t rows_per_block,
+                                                             Slot1 *slots, unsigned long long mask, GroupCtx g,
+                                                             AggArgs args, LdsLayout L,
+                                                             unsigned long long *spill_counter,
+                                                             unsigned long long *cursors)
+{
+    OG_DYNAMIC_LDS(unsigned long long, lds_words);
+    __shared__ unsigned long long wc[kLdsWaves][2];
+    const int wslot = blockIdx.x * kLdsWaves + (WaveInBlock());
+    LoadWaveChunk(wc[WaveInBlock()], cursors, wslot);
+    g.wave = wc[WaveInBlock()];
+    uint8_t *base = reinterpret_cast<uint8_t *>(lds_words);
+    unsigned long long *lkeys = reinterpret_cast<unsigned long long *>(base + L.keys_off);
+    const int S = L.S;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        lkeys[i] = kEmptyKey;
+        for (int a = 0; a < args.n; ++a) {
+            int kind = args.a[a].kind;
+            if (Is32(kind)) {
+                reinterpret_cast<int *>(base + L.off[a][0])[i] = static_cast<int>(InitWord(kind));
+            } else {
+                reinterpret_cast<unsigned long long *>(base + L.off[a][0])[i] = InitWord(kind);
+            }
+            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = 0;
+            if (L.off[a][2] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][2])[i] = 0;
+            if (L.flag_off[a] >= 0) base[L.flag_off[a] + i] = 0;
+        }
+    }
+    __syncthreads();
+    int64_t start = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    int64_t end = start + rows_per_block < rows ? start + rows_per_block : rows;
+    unsigned int spilled = 0;
+    for (int64_t row = start + threadIdx.x; row < end; row += blockDim.x) {
+        bool isNull = NullBit(kd.nulls[0], row);
+        unsigned long long key = isNull ? 0ULL : LoadWord(kd.values[0], kd.width[0], row);
+        int slot = -1;
+        if (!isNull && key != kEmptyKey) {
+            unsigned long long h = Mix64(key);
+            unsigned int pos = static_cast<unsigned int>(((h >> 32) * static_cast<unsigned long long>(S)) >> 32);
+            for (int step = 0; step < kLdsMaxProbe; ++step) {
+                unsigned long long k = lkeys[pos];
+                if (k == kEmptyKey) {
+                    k = atomicCAS(&lkeys[pos], kEmptyKey, key);
+                    if (k == kEmptyKey) k = key;
+                }
+                if (k == key) {
+                    slot = static_cast<int>(pos);
+                    break;
+                }
+                pos = pos + 1 == static_cast<unsigned int>(S) ? 0 : pos + 1;
+            }
+        }
+        if (slot >= 0) {
+            UpdateAggsT(args, LdsRef{base, L, slot}, row);
+        } else {
+            ++spilled;
+            unsigned int gid = isNull ? SideGroup(g, 0, 0ULL, true, 1)
+                                      : (key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g));
+            if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        unsigned long long k = lkeys[i];
+        if (k == kEmptyKey) continue;
+        unsigned int gid = InsertSingle(slots, mask, k, g);
+        if (gid != 0xFFFFFFFFu) MergeAggs(args, gid, LdsRef{base, L, i});
+    }
+    if (spilled) atomicAdd(spill_counter, static_cast<unsigned long long>(spilled));
+    StoreWaveChunk(wc[WaveInBlock()], cursors, wslot);
+}
+
+// Rehash after growth: re-inserts the live groups of [0, n) under their ids (states do not move).
+__global__ void agg_rehash_single(const unsigned long long *gkeys, const uint8_t *gvalid, int64_t n, Slot1 *slots,
+                                  unsigned long long mask, const unsigned int *special)
+{
+    int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= n || !gvalid[gid]) return;
+    if (special[0] == gid + 2 || special[1] == gid + 2) return;  // side groups are not in the table
+    unsigned long long key = gkeys[gid];
+    unsigned long long pos = Mix64(key) & mask;
+    while (atomicCAS(&slots[pos].key, kEmptyKey, key) != kEmptyKey) pos = (pos + 1) & mask;
+    slots[pos].gid1 = static_cast<unsigned int>(gid) + 1;
+}
+
+__global__ void agg_rehash_multi(const unsigned long long *gkeys, const uint8_t *gvalid, int64_t n, int W,
+                                 unsigned int *ctrl, unsigned long long *skeys, unsigned long long mask)
+{
+    int64_t gid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (gid >= n || !gvalid[gid]) return;
+    unsigned long long words[kMaxWords];
+    for (int w = 0; w < W; ++w) words[w] = gkeys[gid * W + w];
+    unsigned long long pos = HashWords(words, W) & mask;
+    while (atomicCAS(&ctrl[pos], 0u, static_cast<unsigned int>(gid) + 2) != 0u) pos = (pos + 1) & mask;
+    for (int w = 0; w < W; ++w) skeys[pos * W + w] = words[w];
+}
+
+__global__ void fill_u32(unsigned int *p, unsigned int v, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void fill_u8(uint8_t *p, uint8_t v, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Writes the initial state row image `tmpl` (words_per_row 8-byte words) into rows [from, from + n).
+__global__ void init_state_rows(unsigned long long *states, const unsigned long long *tmpl, int words_per_row,
+                                int64_t from, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n * words_per_row) return;
+    states[from * words_per_row + i] = tmpl[i % words_per_row];
+}
+
+__global__ void init_slots1(Slot1 *s, int64_t n)
+{
+    int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) {
+        s[i].key = kEmptyKey;
+        s[i].gid1 = 0;
+        s[i].pad = 0;
+    }
+}
+
+// Output rows [start, start + count) of the live groups, in the order of `index` (live group ids ascending).
+__global__ void __launch_bounds__(256) agg_output(int64_t start, int64_t count, KeyDev kd, int single,
+                                                  const unsigned int *index, const unsigned long long *gkeys,
+                                                  const uint8_t *gnull, AggArgs args, OutArgs out,
+                                                  int overflow_as_null, unsigned int *error_flag)
+{
+    int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    bool live = r < count;
+    int64_t gid = live ? static_cast<int64_t>(index[start + r]) : 0;
+    // keys
+    for (int c = 0; c < kd.num; ++c) {
+        bool isNull = false;
+        unsigned long long lo = 0, hi = 0;
+        if (live) {
+            if (single) {
+                isNull = gnull[gid] != 0;
+                lo = gkeys[gid];
+            } else {
+                const unsigned long long *w = gkeys + gid * kd.W;
+                isNull = (w[0] >> c) & 1ULL;
+                if (kd.width[c] == 16) {
+                    lo = w[kd.word[c]];
+                    hi = w[kd.word[c] + 1];
+                } else if (kd.in_word0[c]) {
+                    lo = w[0] >> 32;
+                } else {
+                    lo = w[kd.word[c]];
+                }
+            }
+            if (isNull) lo = hi = 0;
+            OutDev &o = out.o[c];
+            if (o.width == 16) {
+                static_cast<unsigned long long *>(o.values)[2 * r] = lo;
+                static_cast<unsigned long long *>(o.values)[2 * r + 1] = hi;
+            } else {
+                StoreWidth(o.values, o.width, r, lo);
+            }
+        }
+        WriteNullBits(out.o[c].nulls, r, count, live && isNull);
+    }
+    // aggregates
+    for (int i = 0; i < args.n; ++i) {
+        const AggDev &d = args.a[i];
+        OutDev &o = out.o[kd.num + i];
+        bool isNull = false;
+        if (live) {
+            isNull = FinalizeAgg(d, i, GlobalRef{args, static_cast<unsigned int>(gid)}, o, r, overflow_as_null,
+                                 error_flag);
+        }
+        WriteNullBits(o.nulls, r, count, live && isNull);
+    }
+}
+
+}  // namespace omni_gpu

@@ -1,6 +1,9 @@
 // Host-side helpers that map an aggregation spec onto device descriptors and table layouts.
 #pragma once
 
+#include <cstring>
+#include <vector>
+
 #include "agg_device.h"
 
 namespace omni_gpu {
@@ -62,6 +65,62 @@ inline int ClassifyAgg(const omni_gpu_agg_spec &sp, int i, AggDev &d)
     }
 }
 
+
+// Group state row layout (array of structures, one row per group id): every aggregate's 8-byte words first,
+// then the 4-byte MIN/MAX words, then the flag bytes, so each atomic word is naturally aligned; the row is
+// padded to a multiple of 8 bytes.  Fills args.a[i].off / flag_off and args.row_bytes.
+inline void AssignRowLayout(AggArgs &args)
+{
+    int off = 0;
+    for (int i = 0; i < args.n; ++i) {
+        AggDev &d = args.a[i];
+        int k = d.kind;
+        d.off[0] = d.off[1] = d.off[2] = -1;
+        d.flag_off = -1;
+        if (!Is32(k)) {
+            d.off[0] = off;
+            off += 8;
+        }
+        if (k == K_AVG || k == K_SUM_DEC) {
+            d.off[1] = off;
+            off += 8;
+        }
+        if (k == K_SUM_DEC) {
+            d.off[2] = off;
+            off += 8;
+        }
+    }
+    for (int i = 0; i < args.n; ++i) {
+        if (Is32(args.a[i].kind)) {
+            args.a[i].off[0] = off;
+            off += 4;
+        }
+    }
+    for (int i = 0; i < args.n; ++i) {
+        int k = args.a[i].kind;
+        if (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) args.a[i].flag_off = off++;
+    }
+    args.row_bytes = (off + 7) & ~7;
+    if (args.row_bytes == 0) args.row_bytes = 8;
+}
+
+// Initial image of one state row (MIN/MAX at the type's extreme, everything else zero).
+inline std::vector<unsigned long long> RowTemplate(const AggArgs &args)
+{
+    std::vector<unsigned long long> w(static_cast<size_t>(args.row_bytes / 8), 0ULL);
+    uint8_t *b = reinterpret_cast<uint8_t *>(w.data());
+    for (int i = 0; i < args.n; ++i) {
+        const AggDev &d = args.a[i];
+        unsigned long long init = InitWord(d.kind);
+        if (Is32(d.kind)) {
+            unsigned int v = static_cast<unsigned int>(init);
+            std::memcpy(b + d.off[0], &v, 4);
+        } else {
+            std::memcpy(b + d.off[0], &init, 8);
+        }
+    }
+    return w;
+}
 
 // LDS table layout for the aggregates of a handle within `budget` bytes: per slot one u64 key plus each
 // aggregate's state words (4 bytes for 32-bit MIN/MAX) and a flag byte where the aggregate needs one.

@@ -3,7 +3,7 @@
 
 #include <limits>
 
-#include "omni_gpu_internal.h"
+#include "scan_device.h"
 
 namespace omni_gpu {
 
@@ -489,8 +489,6 @@ __global__ void __launch_bounds__(kBlock) join_probe(ProbeArgs a, TableDev t)
 
 constexpr int kPartMax = 1024;
 constexpr int kXcds = 8;
-constexpr int kScanBlock = 1024;                 // threads of the scan kernels
-constexpr int kScanTile = 2 * kScanBlock;        // elements per scan workgroup
 
 struct PartArgs {
     int nparts;                  // P, a power of two in [8, kPartMax]
@@ -529,79 +527,6 @@ __global__ void __launch_bounds__(kBlock) join_part_hist(ProbeArgs a, TableDev t
     }
     __syncthreads();
     for (int i = threadIdx.x; i < pa.nparts; i += kBlock) pa.counts[static_cast<long long>(i) * pa.nwg + blockIdx.x] = hist[i];
-}
-
-// Exclusive scan of kScanTile consecutive u32 values per workgroup (in place); the workgroup's sum goes to
-// sums[blockIdx.x].
-__global__ void __launch_bounds__(kScanBlock) scan_tiles(unsigned int *v, long long n, unsigned int *sums)
-{
-    __shared__ unsigned int wsum[kScanBlock / 64];
-    const long long i0 = static_cast<long long>(blockIdx.x) * kScanTile + 2 * threadIdx.x;
-    unsigned int x0 = i0 < n ? v[i0] : 0u;
-    unsigned int x1 = i0 + 1 < n ? v[i0 + 1] : 0u;
-    unsigned int mine = x0 + x1;
-    const int lane = LaneId(), wave = threadIdx.x >> 6;
-    unsigned int incl = mine;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned int up = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += up;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int run = 0;
-        for (int w = 0; w < kScanBlock / 64; ++w) {
-            unsigned int t = wsum[w];
-            wsum[w] = run;
-            run += t;
-        }
-        sums[blockIdx.x] = run;
-    }
-    __syncthreads();
-    unsigned int excl = wsum[wave] + incl - mine;
-    if (i0 < n) v[i0] = excl;
-    if (i0 + 1 < n) v[i0 + 1] = excl + x0;
-}
-
-// Exclusive scan of the tile sums (at most kScanTile of them, one workgroup); *total receives their sum.
-__global__ void __launch_bounds__(kScanBlock) scan_sums(unsigned int *sums, int n, unsigned int *total)
-{
-    __shared__ unsigned int wsum[kScanBlock / 64];
-    const int i0 = 2 * threadIdx.x;
-    unsigned int x0 = i0 < n ? sums[i0] : 0u;
-    unsigned int x1 = i0 + 1 < n ? sums[i0 + 1] : 0u;
-    unsigned int mine = x0 + x1;
-    const int lane = LaneId(), wave = threadIdx.x >> 6;
-    unsigned int incl = mine;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        unsigned int up = __shfl_up(incl, d, 64);
-        if (lane >= d) incl += up;
-    }
-    if (lane == 63) wsum[wave] = incl;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int run = 0;
-        for (int w = 0; w < kScanBlock / 64; ++w) {
-            unsigned int t = wsum[w];
-            wsum[w] = run;
-            run += t;
-        }
-        *total = run;
-    }
-    __syncthreads();
-    unsigned int excl = wsum[wave] + incl - mine;
-    if (i0 < n) sums[i0] = excl;
-    if (i0 + 1 < n) sums[i0 + 1] = excl + x0;
-}
-
-__global__ void __launch_bounds__(kScanBlock) scan_add(unsigned int *v, long long n, const unsigned int *sums)
-{
-    const long long i0 = static_cast<long long>(blockIdx.x) * kScanTile + 2 * threadIdx.x;
-    const unsigned int add = sums[blockIdx.x];
-    if (i0 < n) v[i0] += add;
-    if (i0 + 1 < n) v[i0 + 1] += add;
 }
 
 // Partition starts and, per XCD, the tile prefix over its partitions; resets the XCD work counters.

@@ -1,0 +1,329 @@
+// Learned strategy / capacity advisor: key sketch, MLP inference / gradient / training, seeded key generator.
+//
+// The plain path sizes hash tables by growth (grow x4 and rehash, as BaseHashMap's Grower does,
+// core/src/operator/hashmap/base_hash_map.h:255-292) and picks its table kind by a fixed rule.  The advisor
+// predicts the group count and the fastest strategy from a 1% sample sketch, so the table can be sized once and
+// the kernel chosen up front.  Results never depend on it: a prediction that is too small only costs a growth.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <random>
+#include <vector>
+
+#include "learned_device.h"
+
+namespace omni_gpu {
+
+struct MlpHandle {
+    hipStream_t stream = nullptr;
+    DevBuf params, m, v, grad, partial, loss_parts, loss, X, T, C, idx, Y;
+    int step = 0;
+};
+
+int UploadTraining(MlpHandle &h, const float *X, const float *T, const int32_t *C, int64_t n)
+{
+    hipStream_t s = h.stream;
+    OG_TRY(h.X.Reserve(sizeof(float) * kIn * n, s));
+    OG_TRY(h.T.Reserve(sizeof(float) * n, s));
+    OG_TRY(h.C.Reserve(sizeof(int32_t) * n, s));
+    OG_HIP(hipMemcpyAsync(h.X.p, X, sizeof(float) * kIn * n, hipMemcpyHostToDevice, s));
+    OG_HIP(hipMemcpyAsync(h.T.p, T, sizeof(float) * n, hipMemcpyHostToDevice, s));
+    OG_HIP(hipMemcpyAsync(h.C.p, C, sizeof(int32_t) * n, hipMemcpyHostToDevice, s));
+    return OMNI_GPU_OK;
+}
+
+// One gradient evaluation over `B` samples (rows idx[0..B) of the uploaded set, or rows [0, B) when idx is
+// null), followed by an Adam step when lr > 0.
+int GradStep(MlpHandle &h, const int *idx, int B, float lr, bool keep_grad)
+{
+    hipStream_t s = h.stream;
+    const int nparts = (B + kMlpBlock - 1) / kMlpBlock;
+    OG_TRY(h.partial.Reserve(sizeof(float) * kParams * nparts, s));
+    OG_TRY(h.loss_parts.Reserve(sizeof(float) * nparts, s));
+    {
+        KernelScope ks(s, "mlp_backward");
+        mlp_backward<<<nparts, kMlpBlock, 0, s>>>(h.params.as<float>(), h.X.as<float>(), h.T.as<float>(),
+                                                  h.C.as<int>(), idx, B, h.partial.as<float>(),
+                                                  h.loss_parts.as<float>());
+        OG_TRY(CheckLaunch("mlp_backward"));
+    }
+    if (lr > 0.f) ++h.step;
+    KernelScope ks(s, "mlp_adam");
+    mlp_reduce_adam<<<(kParams + 255) / 256, 256, 0, s>>>(h.partial.as<float>(), nparts, h.loss_parts.as<float>(),
+                                                         h.params.as<float>(), h.m.as<float>(), h.v.as<float>(),
+                                                         keep_grad ? h.grad.as<float>() : nullptr,
+                                                         h.loss.as<float>(), lr, 0.9f, 0.999f, 1e-8f,
+                                                         std::max(1, h.step));
+    return CheckLaunch("mlp_reduce_adam");
+}
+
+}  // namespace omni_gpu
+
+using namespace omni_gpu;
+
+extern "C" {
+
+int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
+                         omni_gpu_sketch *out)
+{
+    OG_CHECK(keys && out, OMNI_GPU_ERR_INVALID, "null argument");
+    OG_CHECK(num_keys >= 1 && num_keys <= 2, OMNI_GPU_ERR_INVALID, "the sketch reads one or two key columns");
+    OG_CHECK(rows >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
+    std::memset(out, 0, sizeof(*out));
+    SketchArgs a{};
+    a.ncols = num_keys;
+    int keyBytes = 0;
+    for (int c = 0; c < num_keys; ++c) {
+        a.width[c] = TypeWidth(keys[c].type_id);
+        OG_CHECK(a.width[c] > 0 && a.width[c] <= 8 && keys[c].values, OMNI_GPU_ERR_INVALID,
+                 "sketch key %d must be an integer-like column of <= 8 bytes", c);
+        a.values[c] = keys[c].values;
+        a.nulls[c] = keys[c].nulls;
+        keyBytes += a.width[c];
+    }
+    a.rows = rows;
+    // about 1% of the rows, at least 4096 (or all of them), at most 2^21
+    int64_t m = std::min<int64_t>(rows, std::max<int64_t>(4096, rows / 100));
+    m = std::min<int64_t>(m, 1 << 21);
+    a.stride = m > 0 ? std::max<int64_t>(1, rows / m) : 1;
+    a.m = m;
+    SketchStats st{};
+    if (m > 0) {
+        hipStream_t s;
+        OG_TRY(AcquireStream(&s));
+        uint64_t cap = 1024;
+        while (cap < static_cast<uint64_t>(2 * m)) cap <<= 1;
+        DevBuf hk, cnt, sb;
+        int rc = OMNI_GPU_OK;
+        do {
+            if ((rc = hk.Reserve(8 * cap, s)) != OMNI_GPU_OK) break;
+            if ((rc = cnt.Reserve(4 * cap, s)) != OMNI_GPU_OK) break;
+            if ((rc = sb.Reserve(sizeof(SketchStats), s)) != OMNI_GPU_OK) break;
+            SketchStats init{};
+            init.kmin = std::numeric_limits<long long>::max();
+            init.kmax = std::numeric_limits<long long>::min();
+            if (hipMemsetAsync(hk.p, 0, 8 * cap, s) != hipSuccess || hipMemsetAsync(cnt.p, 0, 4 * cap, s) != hipSuccess ||
+                hipMemcpyAsync(sb.p, &init, sizeof(init), hipMemcpyHostToDevice, s) != hipSuccess) {
+                rc = SetError(OMNI_GPU_ERR_HIP, "sketch buffer initialisation failed");
+                break;
+            }
+            a.hkeys = hk.as<unsigned long long>();
+            a.counts = cnt.as<unsigned int>();
+            a.mask = cap - 1;
+            a.stats = sb.as<SketchStats>();
+            {
+                KernelScope ks(s, "sketch_insert");
+                sketch_insert<<<static_cast<unsigned int>((m + 255) / 256), 256, 0, s>>>(a);
+                if ((rc = CheckLaunch("sketch_insert")) != OMNI_GPU_OK) break;
+            }
+            {
+                KernelScope ks(s, "sketch_reduce");
+                sketch_reduce<<<static_cast<unsigned int>((cap + 255) / 256), 256, 0, s>>>(a, cap);
+                if ((rc = CheckLaunch("sketch_reduce")) != OMNI_GPU_OK) break;
+            }
+            if (hipMemcpyAsync(&st, sb.p, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                hipStreamSynchronize(s) != hipSuccess) {
+                rc = SetError(OMNI_GPU_ERR_HIP, "sketch readback failed");
+            }
+        } while (false);
+        hk.Release(s);
+        cnt.Release(s);
+        sb.Release(s);
+        ReleaseStream(s);
+        OG_TRY(rc);
+    }
+    out->rows = rows;
+    out->sampled = m;
+    out->distinct = static_cast<int64_t>(st.distinct);
+    out->f1 = static_cast<int64_t>(st.f1);
+    out->f2 = static_cast<int64_t>(st.f2);
+    out->max_count = static_cast<int64_t>(st.maxc);
+    out->heavy = static_cast<int64_t>(st.heavy);
+    out->nulls = static_cast<int64_t>(st.nulls);
+    out->key_min = st.kmin;
+    out->key_max = st.kmax;
+    // features (all roughly in [0, 1])
+    const double md = std::max<double>(1.0, static_cast<double>(m));
+    const double d = static_cast<double>(st.distinct), f1 = static_cast<double>(st.f1), f2 = static_cast<double>(st.f2);
+    const double gee = std::sqrt(static_cast<double>(std::max<int64_t>(rows, 1)) / md) * f1 + (d - f1);
+    const double chao = d + f1 * f1 / (2.0 * std::max(f2, 1.0));
+    const double range = st.kmax >= st.kmin ? static_cast<double>(st.kmax) - static_cast<double>(st.kmin) : 0.0;
+    float *f = out->features;
+    f[0] = static_cast<float>(std::log2(static_cast<double>(rows) + 1.0) / 32.0);
+    f[1] = static_cast<float>(std::log2(md + 1.0) / 32.0);
+    f[2] = static_cast<float>(std::log2(d + 1.0) / 32.0);
+    f[3] = static_cast<float>(f1 / md);
+    f[4] = static_cast<float>(f2 / md);
+    f[5] = static_cast<float>(static_cast<double>(st.maxc) / md);
+    f[6] = static_cast<float>(static_cast<double>(st.heavy) / md);
+    f[7] = static_cast<float>(d / md);
+    f[8] = static_cast<float>(std::log2(gee + 1.0) / 32.0);
+    f[9] = static_cast<float>(std::log2(chao + 1.0) / 32.0);
+    f[10] = static_cast<float>(std::log2(range + 1.0) / 64.0);
+    f[11] = static_cast<float>(static_cast<double>(st.nulls) / md);
+    f[12] = static_cast<float>(keyBytes / 16.0);
+    f[13] = static_cast<float>(num_keys / 4.0);
+    f[14] = is_join ? 1.f : 0.f;
+    f[15] = 1.f;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_create(uint64_t seed, void **handle)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null argument");
+    std::unique_ptr<MlpHandle> h(new MlpHandle());
+    OG_TRY(AcquireStream(&h->stream));
+    hipStream_t s = h->stream;
+    OG_TRY(h->params.Reserve(sizeof(float) * kParams, s));
+    OG_TRY(h->m.Reserve(sizeof(float) * kParams, s));
+    OG_TRY(h->v.Reserve(sizeof(float) * kParams, s));
+    OG_TRY(h->grad.Reserve(sizeof(float) * kParams, s));
+    OG_TRY(h->loss.Reserve(sizeof(float), s));
+    // He-uniform weights, zero biases
+    std::vector<float> p(kParams, 0.f);
+    std::mt19937_64 rng(seed);
+    auto fill = [&](int off, int fan_in, int n) {
+        const float lim = std::sqrt(6.0f / static_cast<float>(fan_in));
+        std::uniform_real_distribution<float> u(-lim, lim);
+        for (int i = 0; i < n; ++i) p[off + i] = u(rng);
+    };
+    fill(kOffW1, kIn, kIn * kH);
+    fill(kOffW2, kH, kH * kH);
+    fill(kOffW3, kH, kH * kOut);
+    OG_HIP(hipMemcpyAsync(h->params.p, p.data(), sizeof(float) * kParams, hipMemcpyHostToDevice, s));
+    OG_HIP(hipMemsetAsync(h->m.p, 0, sizeof(float) * kParams, s));
+    OG_HIP(hipMemsetAsync(h->v.p, 0, sizeof(float) * kParams, s));
+    OG_HIP(hipStreamSynchronize(s));
+    *handle = h.release();
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_num_params(void) { return kParams; }
+
+int omni_gpu_mlp_set_params(void *handle, const float *params)
+{
+    OG_CHECK(handle && params, OMNI_GPU_ERR_INVALID, "null argument");
+    MlpHandle &h = *static_cast<MlpHandle *>(handle);
+    OG_HIP(hipMemcpyAsync(h.params.p, params, sizeof(float) * kParams, hipMemcpyHostToDevice, h.stream));
+    OG_HIP(hipMemsetAsync(h.m.p, 0, sizeof(float) * kParams, h.stream));
+    OG_HIP(hipMemsetAsync(h.v.p, 0, sizeof(float) * kParams, h.stream));
+    h.step = 0;
+    OG_HIP(hipStreamSynchronize(h.stream));
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_get_params(void *handle, float *params)
+{
+    OG_CHECK(handle && params, OMNI_GPU_ERR_INVALID, "null argument");
+    MlpHandle &h = *static_cast<MlpHandle *>(handle);
+    OG_HIP(hipMemcpyAsync(params, h.params.p, sizeof(float) * kParams, hipMemcpyDeviceToHost, h.stream));
+    OG_HIP(hipStreamSynchronize(h.stream));
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_forward(void *handle, const float *X, int32_t n, float *Y)
+{
+    OG_CHECK(handle && X && Y && n >= 0, OMNI_GPU_ERR_INVALID, "bad argument");
+    if (n == 0) return OMNI_GPU_OK;
+    MlpHandle &h = *static_cast<MlpHandle *>(handle);
+    hipStream_t s = h.stream;
+    OG_TRY(h.X.Reserve(sizeof(float) * kIn * n, s));
+    OG_TRY(h.Y.Reserve(sizeof(float) * kOut * n, s));
+    OG_HIP(hipMemcpyAsync(h.X.p, X, sizeof(float) * kIn * n, hipMemcpyHostToDevice, s));
+    {
+        KernelScope ks(s, "mlp_forward");
+        mlp_forward<<<(n + kMlpFwdBlock - 1) / kMlpFwdBlock, kMlpFwdBlock, 0, s>>>(h.params.as<float>(), h.X.as<float>(),
+                                                                                  n, h.Y.as<float>());
+        OG_TRY(CheckLaunch("mlp_forward"));
+    }
+    OG_HIP(hipMemcpyAsync(Y, h.Y.p, sizeof(float) * kOut * n, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_gradient(void *handle, const float *X, const float *t, const int32_t *c, int32_t n, float *grad,
+                          float *loss)
+{
+    OG_CHECK(handle && X && t && c && grad && n > 0, OMNI_GPU_ERR_INVALID, "bad argument");
+    MlpHandle &h = *static_cast<MlpHandle *>(handle);
+    for (int32_t i = 0; i < n; ++i) OG_CHECK(c[i] >= 0 && c[i] < 3, OMNI_GPU_ERR_INVALID, "class %d out of range", c[i]);
+    OG_TRY(UploadTraining(h, X, t, c, n));
+    OG_TRY(GradStep(h, nullptr, n, 0.f, true));
+    OG_HIP(hipMemcpyAsync(grad, h.grad.p, sizeof(float) * kParams, hipMemcpyDeviceToHost, h.stream));
+    float l = 0.f;
+    OG_HIP(hipMemcpyAsync(&l, h.loss.p, sizeof(float), hipMemcpyDeviceToHost, h.stream));
+    OG_HIP(hipStreamSynchronize(h.stream));
+    if (loss) *loss = l;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_train(void *handle, const float *X, const float *t, const int32_t *c, int32_t n, int32_t batch,
+                       int32_t epochs, float lr, uint64_t seed, float *epoch_loss)
+{
+    OG_CHECK(handle && X && t && c && n > 0 && batch > 0 && epochs > 0 && lr > 0.f, OMNI_GPU_ERR_INVALID,
+             "bad argument");
+    MlpHandle &h = *static_cast<MlpHandle *>(handle);
+    for (int32_t i = 0; i < n; ++i) OG_CHECK(c[i] >= 0 && c[i] < 3, OMNI_GPU_ERR_INVALID, "class %d out of range", c[i]);
+    hipStream_t s = h.stream;
+    OG_TRY(UploadTraining(h, X, t, c, n));
+    OG_TRY(h.idx.Reserve(sizeof(int) * n, s));
+    std::vector<int> perm(n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::mt19937_64 rng(seed);
+    for (int e = 0; e < epochs; ++e) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        OG_HIP(hipMemcpyAsync(h.idx.p, perm.data(), sizeof(int) * n, hipMemcpyHostToDevice, s));
+        // the copy reads `perm` asynchronously: wait before the next epoch reshuffles it
+        double sum = 0.0;
+        int steps = 0;
+        for (int b0 = 0; b0 < n; b0 += batch) {
+            const int B = std::min(batch, n - b0);
+            OG_TRY(GradStep(h, h.idx.as<int>() + b0, B, lr, false));
+            if (epoch_loss) {
+                float l = 0.f;
+                OG_HIP(hipMemcpyAsync(&l, h.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+                OG_HIP(hipStreamSynchronize(s));
+                sum += l * B;
+                ++steps;
+            }
+        }
+        OG_HIP(hipStreamSynchronize(s));
+        if (epoch_loss) epoch_loss[e] = static_cast<float>(sum / n);
+    }
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_mlp_destroy(void *handle)
+{
+    if (!handle) return OMNI_GPU_OK;
+    MlpHandle *h = static_cast<MlpHandle *>(handle);
+    hipStream_t s = h->stream;
+    for (DevBuf *b : {&h->params, &h->m, &h->v, &h->grad, &h->partial, &h->loss_parts, &h->loss, &h->X, &h->T, &h->C,
+                      &h->idx, &h->Y}) {
+        b->Release(s);
+    }
+    ReleaseStream(s);
+    delete h;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_generate_keys(int32_t kind, int64_t rows, double distinct, double zipf_s, uint64_t seed, void *k64,
+                           void *k32, void *v64)
+{
+    OG_CHECK(kind >= 0 && kind <= 2 && rows >= 0 && distinct >= 1.0 && zipf_s >= 0.0, OMNI_GPU_ERR_INVALID,
+             "bad generator arguments");
+    OG_CHECK((kind == 1 || k64) && (kind == 0 || k32), OMNI_GPU_ERR_INVALID, "missing output column");
+    if (rows == 0) return OMNI_GPU_OK;
+    hipStream_t s;
+    OG_TRY(AcquireStream(&s));
+    gen_keys<<<static_cast<unsigned int>((rows + 255) / 256), 256, 0, s>>>(
+        kind, rows, distinct, zipf_s, seed, static_cast<long long *>(k64), static_cast<int *>(k32),
+        static_cast<long long *>(v64));
+    int rc = CheckLaunch("gen_keys");
+    ReleaseStream(s);
+    return rc;
+}
+
+}  // extern "C"

@@ -14,7 +14,20 @@
 #define OG_GLOBAL __attribute__((address_space(1)))
 #endif
 
+// Lanes per wavefront (CDNA: 64).  Overridable only so that a CPU debugging build can run each thread as its
+// own wave.
+#ifndef OG_WAVE_SIZE
+#define OG_WAVE_SIZE 64
+#endif
+
+// Dynamic LDS array of a kernel (its size is the launch's dynamic shared-memory bytes).
+#ifndef OG_DYNAMIC_LDS
+#define OG_DYNAMIC_LDS(T, name) extern __shared__ T name[]
+#endif
+
 namespace omni_gpu {
+
+constexpr int kWaveSize = OG_WAVE_SIZE;
 
 // ---- errors ---------------------------------------------------------------------------------------------
 int SetError(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -150,6 +163,8 @@ __device__ __forceinline__ unsigned long long Mix64(unsigned long long k)
 }
 
 __device__ __forceinline__ int LaneId() { return static_cast<int>(__lane_id()); }
+// Wave of the calling thread within its workgroup.
+__device__ __forceinline__ int WaveInBlock() { return static_cast<int>(threadIdx.x) / kWaveSize; }
 
 __device__ __forceinline__ unsigned long long LaneMaskLt()
 {

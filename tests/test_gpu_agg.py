@@ -175,6 +175,24 @@ def test_skewed_agg_reduced_size():
     assert_row_sets_equal(got, want, float_cols=float_cols(2, aggs), rtol=DOUBLE_RTOL)
 
 
+@pytest.mark.parametrize("composite", [False, True])
+def test_many_groups_chunked_ids(composite):
+    """About 1M groups over 3M rows in 12 batches: waves take group ids in chunks once many groups exist, the
+    table grows several times, and ids reserved but unused must not surface as groups."""
+    rng = np.random.default_rng(21 + composite)
+    n = 3_000_000
+    k1 = rng.integers(0, 1_000_000, n, dtype=np.int64) * 2654435761 - 5
+    k2 = rng.integers(0, 3, n, dtype=np.int64).astype(np.int32)
+    v = rng.integers(-10**9, 10**9, n, dtype=np.int64)
+    keys = [Col(LONG, k1, rng.random(n) < 0.001)] + ([Col(INT, k2)] if composite else [])
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, LONG, LONG), (MAX, LONG, LONG)]
+    cols = [Col(LONG, v, rng.random(n) < 0.05), None, Col(LONG, v), Col(LONG, v)]
+    got = run_gpu(keys, aggs, cols, 250_000, strategy=capi.STRATEGY_GLOBAL)
+    want = run_oracle(keys, aggs, cols, 250_000)
+    assert len(got) == len(want)
+    assert_row_sets_equal(got, want)
+
+
 @pytest.mark.parametrize("strategy", [capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL])
 @pytest.mark.parametrize("key_kind", ["long", "int", "double"])
 @pytest.mark.parametrize("with_nulls", [False, True])
