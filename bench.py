@@ -122,6 +122,155 @@ def bench_join(args):
             "cpu_baseline": cpu, "dtype": "int64"}
 
 
+def timed_steps(step, args, capi):
+    for _ in range(args.warmup):
+        step()
+    capi.synchronize()
+    capi.profile_reset()
+    capi.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+    capi.synchronize()
+    t1 = time.perf_counter()
+    capi.profile_enable(False)
+    return r, (t1 - t0) * 1e3 / args.steps
+
+
+def kernel_roofline(capi, prefix: str, algorithmic_bytes: float, launches_per_step: float, args):
+    n, ms = capi.profile_get(prefix)
+    avg = ms / max(1, n)
+    per_launch = algorithmic_bytes / max(1.0, launches_per_step)
+    achieved = per_launch / (avg / 1e3) / 1e9 if avg > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": prefix,
+            "kernel_ms": round(avg, 4), "algorithmic_bytes": int(per_launch)}
+
+
+def bench_agg_plain(args):
+    """1M rows, int64 key over 1,000 values, VectorBatches of 4096; SUM/COUNT/MIN/MAX."""
+    from omnioperator_amd import capi, workloads
+    from oracle import COUNT_ALL, LONG, MAX, MIN, SUM
+    capi.init(0)
+    w = workloads.agg_plain()
+    n = len(w.keys)
+    kd, vd = capi.DeviceColumn(LONG, w.keys), capi.DeviceColumn(LONG, w.values)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, LONG, LONG), (MAX, LONG, LONG)]
+    spans = workloads.batches(n, w.batch_rows)
+    cols = [([kd.c(a)], [vd.c(a), None, vd.c(a), vd.c(a)], b - a) for a, b in spans]
+
+    def step():
+        h = capi.HashAgg(capi.make_agg_spec([LONG], aggs))
+        for kc, ac, m in cols:
+            h.add_batch(kc, ac, m)
+        out = h.output()
+        h.close()
+        return len(out[0][0])
+
+    groups, ms = timed_steps(step, args, capi)
+    alg = n * 16 + groups * 40
+    roof = kernel_roofline(capi, "agg_insert", n * 16, len(spans), args)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        spec = oracle.make_agg_spec([LONG], aggs)
+        t0 = time.perf_counter()
+        oracle.hash_agg(spec, [oracle.Col(LONG, w.keys)], [oracle.Col(LONG, w.values), None, oracle.Col(LONG, w.values),
+                                                            oracle.Col(LONG, w.values)], w.batch_rows)
+        t1 = time.perf_counter()
+        cpu = {"value": n / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"the full workload ({n} rows); host CPU {cpu_model()}"}
+    cfg = {"workload": "group-by: 1M rows, int64 key over 1,000 values, int64 value, VectorBatches of 4096; "
+                       "SUM/COUNT(*)/MIN/MAX", "rows": n, "groups": groups, "batch_rows": w.batch_rows,
+           "algorithmic_bytes_per_step": alg}
+    return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
+            "dtype": "int64"}
+
+
+def bench_agg_skewed(args):
+    """200M rows, key (int32 U[1e4], int64 Zipf 1.1 over 5e7); SUM(dec64->dec128), SUM/AVG(double), COUNT, MAX(int32)."""
+    from omnioperator_amd import capi, workloads
+    from oracle import AVG, COUNT_ALL, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, SUM
+    capi.init(0)
+    w = workloads.agg_skewed(rows=args.skewed_rows)
+    n = len(w.k1)
+    k1, k2 = capi.DeviceColumn(INT, w.k1), capi.DeviceColumn(LONG, w.k2)
+    vd, vf, vi = capi.DeviceColumn(DECIMAL64, w.v_dec), capi.DeviceColumn(DOUBLE, w.v_dbl), capi.DeviceColumn(INT, w.v_i32)
+    aggs = [(SUM, DECIMAL64, DECIMAL128, 2, 2), (SUM, DOUBLE, DOUBLE), (AVG, DOUBLE, DOUBLE), (COUNT_ALL, LONG, LONG),
+            (MAX, INT, INT)]
+    batch = 1 << 20
+    spans = workloads.batches(n, batch)
+    cols = [([k1.c(a), k2.c(a)], [vd.c(a), vf.c(a), vf.c(a), None, vi.c(a)], b - a) for a, b in spans]
+    outs = {}
+
+    def step():
+        h = capi.HashAgg(capi.make_agg_spec([INT, LONG], aggs))
+        for kc, ac, m in cols:
+            h.add_batch(kc, ac, m)
+        g = h.num_groups()
+        if "bufs" not in outs:
+            outs["bufs"] = h.alloc_output(g)
+        h.output_into(outs["bufs"], g)
+        h.close()
+        return g
+
+    groups, ms = timed_steps(step, args, capi)
+    in_bytes = n * (4 + 8 + 8 + 8 + 4)
+    out_bytes = groups * (4 + 8 + 16 + 8 + 8 + 8 + 4)
+    roof = kernel_roofline(capi, "agg_insert_multi", in_bytes, len(spans), args)
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        sample = min(n, args.cpu_skewed_rows)
+        spec = oracle.make_agg_spec([INT, LONG], aggs)
+        t0 = time.perf_counter()
+        oracle.hash_agg(spec, [oracle.Col(INT, w.k1[:sample]), oracle.Col(LONG, w.k2[:sample])],
+                        [oracle.Col(DECIMAL64, w.v_dec[:sample], None, 2), oracle.Col(DOUBLE, w.v_dbl[:sample]),
+                         oracle.Col(DOUBLE, w.v_dbl[:sample]), None, oracle.Col(INT, w.v_i32[:sample])], batch)
+        t1 = time.perf_counter()
+        cpu = {"value": sample / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample} rows of the workload; host CPU {cpu_model()}"}
+    cfg = {"workload": f"skewed group-by: {n} rows, key (int32 U[0,1e4), int64 Zipf(1.1) over 5e7), VectorBatches of "
+                       f"{batch}; SUM(decimal64->decimal128), SUM/AVG(double), COUNT(*), MAX(int32); output of "
+                       "every group included", "rows": n, "groups": groups,
+           "algorithmic_bytes_per_step": in_bytes + out_bytes}
+    return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
+            "dtype": "mixed: int32/int64 keys, decimal64->decimal128, f64, int32"}
+
+
+def bench_batched(args):
+    """20,000 independent group-bys (lognormal rows, median 2048), SUM + COUNT(*), one run over the stream."""
+    from omnioperator_amd import capi, workloads
+    from oracle import COUNT_ALL, LONG, SUM
+    capi.init(0)
+    w = workloads.batched_stream()
+    offsets = np.concatenate([[0], np.cumsum(w.row_counts)]).astype(np.int64)
+    n = int(offsets[-1])
+    kd, vd = capi.DeviceColumn(LONG, w.keys), capi.DeviceColumn(LONG, w.values)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
+    s = capi.AggBatches(capi.make_agg_spec([LONG], aggs), 65536)
+    flags = {"graph": capi.AggBatches.FLAG_GRAPH, "two_launches": 0, "per_batch": capi.AggBatches.FLAG_PER_BATCH}[args.batched_form]
+
+    def step():
+        g = s.run(kd.c(), [vd.c(), None], offsets, None, flags)
+        return int(g.sum())
+
+    groups, ms = timed_steps(step, args, capi)
+    st = s.stats()
+    s.close()
+    alg = n * 16 + groups * 24
+    cfg = {"workload": f"batched group-by: {len(w.row_counts)} independent VectorBatches, {n} rows (lognormal rows, "
+                       "median 2048, [64, 65536]), per-batch distinct keys log-uniform in [8, 1e6]; SUM + COUNT(*)",
+           "form": args.batched_form, "rows": n, "groups": groups, "lds_batches": st["lds_batches"],
+           "hbm_batches": st["hbm_batches"], "algorithmic_bytes_per_step": alg}
+    achieved = alg / (ms / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "batch_lds + batch_hbm (whole run)",
+            "kernel_ms": round(ms, 4), "algorithmic_bytes": alg}
+    return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": None,
+            "dtype": "int64"}
+
+
 def profiled_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
     profiles/join_probe_c2.md (copy-calibrated); None when that kernel variant was not profiled."""
@@ -158,7 +307,12 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="join", choices=["join"])
+    p.add_argument("--workload", default="join", choices=["join", "agg", "skewed", "batched"],
+                   help="join: the headline inner-join configuration; agg: 1M-row plain group-by; skewed: 200M-row "
+                        "composite-key group-by; batched: 20,000-batch stream")
+    p.add_argument("--skewed-rows", type=int, default=200_000_000)
+    p.add_argument("--cpu-skewed-rows", type=int, default=4_000_000)
+    p.add_argument("--batched-form", default="graph", choices=["graph", "two_launches", "per_batch"])
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
     p.add_argument("--build-rows", type=int, default=2_000_000)
     p.add_argument("--probe-rows", type=int, default=64_000_000)
@@ -183,7 +337,8 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
               "described in DESIGN.md and is not run.", file=sys.stderr)
         sys.exit(2)
-    res = bench_join(args)
+    res = {"join": bench_join, "agg": bench_agg_plain, "skewed": bench_agg_skewed,
+           "batched": bench_batched}[args.workload](args)
     line = {"metric": METRIC, "value": res["value"], "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
             "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": res["dtype"], "data": "synthetic (seeded)", "config": res["config"],

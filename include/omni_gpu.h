@@ -243,6 +243,10 @@ typedef struct omni_gpu_sketch {
 /* Sketch of one or two integer-like key columns of `rows` rows (device columns). */
 int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
                          omni_gpu_sketch *out);
+/* Sketch + MLP inference in one call on the MLP handle's stream (one synchronisation): y[4] receives the network's
+ * outputs for the key columns (y[0] * 32 = predicted log2(groups + 1); y[1..3] strategy logits). */
+int omni_gpu_advise(void *mlp, const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
+                    float *y);
 /* MLP handle with He-uniform weights drawn from `seed`.  Parameters are the flattened W1[16x64] b1[64]
  * W2[64x64] b2[64] W3[64x4] b3[4] (row-major, y = x @ W + b); host arrays below. */
 int omni_gpu_mlp_create(uint64_t seed, void **handle);

@@ -103,6 +103,7 @@ SIGNATURES = {
     "omni_gpu_join_probe_batches": (_I, [_VP, _P(Column), _P(_I64), _I32, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_destroy": (_I, [_VP]),
     "omni_gpu_sketch_keys": (_I, [_P(Column), _I32, _I64, _I32, _P(Sketch)]),
+    "omni_gpu_advise": (_I, [_VP, _P(Column), _I32, _I64, _I32, _P(C.c_float)]),
     "omni_gpu_mlp_create": (_I, [C.c_uint64, _P(_VP)]),
     "omni_gpu_mlp_num_params": (_I, []),
     "omni_gpu_mlp_set_params": (_I, [_VP, _P(C.c_float)]),
@@ -279,6 +280,24 @@ class HashAgg:
             oarr[i].type_id, oarr[i].values, oarr[i].nulls = t, v.ptr, nb.ptr
         check(lib().omni_gpu_agg_get_output(self.h, start, count, oarr))
         return [(v.numpy(), unpack_nulls(nb.numpy(), count)) for v, nb in bufs]
+
+    def alloc_output(self, count: int):
+        """Device output buffers for `count` groups (keys, then aggregates)."""
+        types = [self.spec.key_types[i] for i in range(self.spec.num_keys)] + \
+                [self.spec.agg_out_types[i] for i in range(self.spec.num_aggs)]
+        bufs, oarr = [], (OutColumn * len(types))()
+        for i, t in enumerate(types):
+            shape = (count, 2) if t == DECIMAL128 else (count,)
+            v = DeviceArray(max(1, count) * WIDTH[t], NP[t], shape)
+            nb = DeviceArray(max(8, (count + 63) // 64 * 8), np.uint8)
+            bufs.append((v, nb))
+            oarr[i].type_id, oarr[i].values, oarr[i].nulls = t, v.ptr, nb.ptr
+        return (bufs, oarr, count)
+
+    def output_into(self, bufs, count: int, start: int = 0):
+        """Writes groups [start, start + count) into buffers from alloc_output (device-resident)."""
+        assert count <= bufs[2]
+        check(lib().omni_gpu_agg_get_output(self.h, start, count, bufs[1]))
 
     def close(self):
         if getattr(self, "h", None):
@@ -489,6 +508,13 @@ class Mlp:
         p = np.ascontiguousarray(p, dtype=np.float32)
         assert p.shape == (self.n_params,)
         check(lib().omni_gpu_mlp_set_params(self.h, _fptr(p)))
+
+    def advise(self, keys: Sequence[Column], rows: int, is_join: bool = False) -> np.ndarray:
+        """Sketch of the device key columns + inference in one call; returns the 4 outputs."""
+        karr = (Column * len(keys))(*keys)
+        y = np.zeros(4, np.float32)
+        check(lib().omni_gpu_advise(self.h, karr, len(keys), rows, 1 if is_join else 0, _fptr(y)))
+        return y
 
     def forward(self, X: np.ndarray) -> np.ndarray:
         X = np.ascontiguousarray(X, dtype=np.float32)

@@ -16,9 +16,80 @@
 
 namespace omni_gpu {
 
+// Device buffers of one sketch: the sample's hash table (sized for the largest sample), statistics, features.
+struct SketchBuffers {
+    DevBuf hkeys, counts, stats, features;
+    void Release(hipStream_t s)
+    {
+        hkeys.Release(s);
+        counts.Release(s);
+        stats.Release(s);
+        features.Release(s);
+    }
+};
+
+constexpr int64_t kMaxSample = 1 << 21;
+
+// Launches the sketch of `keys` on stream s: a 1-in-k sample of about 1% of the rows (at least 4096 or all of
+// them, at most 2^21) counted in a hash table, reduced to statistics and turned into the 16 MLP features.
+int RunSketch(hipStream_t s, SketchBuffers &b, const omni_gpu_column *keys, int num_keys, int64_t rows, int is_join,
+              int64_t *sampled)
+{
+    OG_CHECK(keys, OMNI_GPU_ERR_INVALID, "null key columns");
+    OG_CHECK(num_keys >= 1 && num_keys <= 2, OMNI_GPU_ERR_INVALID, "the sketch reads one or two key columns");
+    OG_CHECK(rows >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
+    SketchArgs a{};
+    a.ncols = num_keys;
+    int keyBytes = 0;
+    for (int c = 0; c < num_keys; ++c) {
+        a.width[c] = TypeWidth(keys[c].type_id);
+        OG_CHECK(a.width[c] > 0 && a.width[c] <= 8 && (rows == 0 || keys[c].values), OMNI_GPU_ERR_INVALID,
+                 "sketch key %d must be an integer-like column of <= 8 bytes", c);
+        a.values[c] = keys[c].values;
+        a.nulls[c] = keys[c].nulls;
+        keyBytes += a.width[c];
+    }
+    a.rows = rows;
+    int64_t m = std::min<int64_t>(rows, std::max<int64_t>(4096, rows / 100));
+    m = std::min<int64_t>(m, kMaxSample);
+    a.stride = m > 0 ? std::max<int64_t>(1, rows / m) : 1;
+    a.m = m;
+    *sampled = m;
+    uint64_t cap = 1024;
+    while (cap < static_cast<uint64_t>(2 * m)) cap <<= 1;
+    OG_TRY(b.hkeys.Reserve(8 * cap, s));
+    OG_TRY(b.counts.Reserve(4 * cap, s));
+    OG_TRY(b.stats.Reserve(sizeof(SketchStats), s));
+    OG_TRY(b.features.Reserve(sizeof(float) * kIn, s));
+    SketchStats init{};
+    init.kmin = std::numeric_limits<long long>::max();
+    init.kmax = std::numeric_limits<long long>::min();
+    OG_HIP(hipMemsetAsync(b.hkeys.p, 0, 8 * cap, s));
+    OG_HIP(hipMemsetAsync(b.counts.p, 0, 4 * cap, s));
+    OG_HIP(hipMemcpyAsync(b.stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    a.hkeys = b.hkeys.as<unsigned long long>();
+    a.counts = b.counts.as<unsigned int>();
+    a.mask = cap - 1;
+    a.stats = b.stats.as<SketchStats>();
+    if (m > 0) {
+        KernelScope ks(s, "sketch_insert");
+        sketch_insert<<<static_cast<unsigned int>((m + 255) / 256), 256, 0, s>>>(a);
+        OG_TRY(CheckLaunch("sketch_insert"));
+    }
+    {
+        KernelScope ks(s, "sketch_reduce");
+        sketch_reduce<<<static_cast<unsigned int>((cap + 255) / 256), 256, 0, s>>>(a, cap);
+        OG_TRY(CheckLaunch("sketch_reduce"));
+    }
+    sketch_features<<<1, 64, 0, s>>>(b.stats.as<SketchStats>(), rows, m, keyBytes, num_keys, is_join,
+                                     b.features.as<float>());
+    return CheckLaunch("sketch_features");
+}
+
 struct MlpHandle {
     hipStream_t stream = nullptr;
     DevBuf params, m, v, grad, partial, loss_parts, loss, X, T, C, idx, Y;
+    SketchBuffers sketch;   // the advisor's sketch buffers, kept across calls
     int step = 0;
 };
 
@@ -68,72 +139,23 @@ extern "C" {
 int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
                          omni_gpu_sketch *out)
 {
-    OG_CHECK(keys && out, OMNI_GPU_ERR_INVALID, "null argument");
-    OG_CHECK(num_keys >= 1 && num_keys <= 2, OMNI_GPU_ERR_INVALID, "the sketch reads one or two key columns");
-    OG_CHECK(rows >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
+    OG_CHECK(out, OMNI_GPU_ERR_INVALID, "null argument");
     std::memset(out, 0, sizeof(*out));
-    SketchArgs a{};
-    a.ncols = num_keys;
-    int keyBytes = 0;
-    for (int c = 0; c < num_keys; ++c) {
-        a.width[c] = TypeWidth(keys[c].type_id);
-        OG_CHECK(a.width[c] > 0 && a.width[c] <= 8 && keys[c].values, OMNI_GPU_ERR_INVALID,
-                 "sketch key %d must be an integer-like column of <= 8 bytes", c);
-        a.values[c] = keys[c].values;
-        a.nulls[c] = keys[c].nulls;
-        keyBytes += a.width[c];
-    }
-    a.rows = rows;
-    // about 1% of the rows, at least 4096 (or all of them), at most 2^21
-    int64_t m = std::min<int64_t>(rows, std::max<int64_t>(4096, rows / 100));
-    m = std::min<int64_t>(m, 1 << 21);
-    a.stride = m > 0 ? std::max<int64_t>(1, rows / m) : 1;
-    a.m = m;
+    hipStream_t s;
+    OG_TRY(AcquireStream(&s));
+    SketchBuffers b;
+    int64_t m = 0;
+    int rc = RunSketch(s, b, keys, num_keys, rows, is_join, &m);
     SketchStats st{};
-    if (m > 0) {
-        hipStream_t s;
-        OG_TRY(AcquireStream(&s));
-        uint64_t cap = 1024;
-        while (cap < static_cast<uint64_t>(2 * m)) cap <<= 1;
-        DevBuf hk, cnt, sb;
-        int rc = OMNI_GPU_OK;
-        do {
-            if ((rc = hk.Reserve(8 * cap, s)) != OMNI_GPU_OK) break;
-            if ((rc = cnt.Reserve(4 * cap, s)) != OMNI_GPU_OK) break;
-            if ((rc = sb.Reserve(sizeof(SketchStats), s)) != OMNI_GPU_OK) break;
-            SketchStats init{};
-            init.kmin = std::numeric_limits<long long>::max();
-            init.kmax = std::numeric_limits<long long>::min();
-            if (hipMemsetAsync(hk.p, 0, 8 * cap, s) != hipSuccess || hipMemsetAsync(cnt.p, 0, 4 * cap, s) != hipSuccess ||
-                hipMemcpyAsync(sb.p, &init, sizeof(init), hipMemcpyHostToDevice, s) != hipSuccess) {
-                rc = SetError(OMNI_GPU_ERR_HIP, "sketch buffer initialisation failed");
-                break;
-            }
-            a.hkeys = hk.as<unsigned long long>();
-            a.counts = cnt.as<unsigned int>();
-            a.mask = cap - 1;
-            a.stats = sb.as<SketchStats>();
-            {
-                KernelScope ks(s, "sketch_insert");
-                sketch_insert<<<static_cast<unsigned int>((m + 255) / 256), 256, 0, s>>>(a);
-                if ((rc = CheckLaunch("sketch_insert")) != OMNI_GPU_OK) break;
-            }
-            {
-                KernelScope ks(s, "sketch_reduce");
-                sketch_reduce<<<static_cast<unsigned int>((cap + 255) / 256), 256, 0, s>>>(a, cap);
-                if ((rc = CheckLaunch("sketch_reduce")) != OMNI_GPU_OK) break;
-            }
-            if (hipMemcpyAsync(&st, sb.p, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess) {
-                rc = SetError(OMNI_GPU_ERR_HIP, "sketch readback failed");
-            }
-        } while (false);
-        hk.Release(s);
-        cnt.Release(s);
-        sb.Release(s);
-        ReleaseStream(s);
-        OG_TRY(rc);
+    if (rc == OMNI_GPU_OK && (hipMemcpyAsync(&st, b.stats.p, sizeof(st), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                              hipMemcpyAsync(out->features, b.features.p, sizeof(float) * kIn, hipMemcpyDeviceToHost,
+                                             s) != hipSuccess ||
+                              hipStreamSynchronize(s) != hipSuccess)) {
+        rc = SetError(OMNI_GPU_ERR_HIP, "sketch readback failed");
     }
+    b.Release(s);
+    ReleaseStream(s);
+    OG_TRY(rc);
     out->rows = rows;
     out->sampled = m;
     out->distinct = static_cast<int64_t>(st.distinct);
@@ -144,29 +166,25 @@ int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t 
     out->nulls = static_cast<int64_t>(st.nulls);
     out->key_min = st.kmin;
     out->key_max = st.kmax;
-    // features (all roughly in [0, 1])
-    const double md = std::max<double>(1.0, static_cast<double>(m));
-    const double d = static_cast<double>(st.distinct), f1 = static_cast<double>(st.f1), f2 = static_cast<double>(st.f2);
-    const double gee = std::sqrt(static_cast<double>(std::max<int64_t>(rows, 1)) / md) * f1 + (d - f1);
-    const double chao = d + f1 * f1 / (2.0 * std::max(f2, 1.0));
-    const double range = st.kmax >= st.kmin ? static_cast<double>(st.kmax) - static_cast<double>(st.kmin) : 0.0;
-    float *f = out->features;
-    f[0] = static_cast<float>(std::log2(static_cast<double>(rows) + 1.0) / 32.0);
-    f[1] = static_cast<float>(std::log2(md + 1.0) / 32.0);
-    f[2] = static_cast<float>(std::log2(d + 1.0) / 32.0);
-    f[3] = static_cast<float>(f1 / md);
-    f[4] = static_cast<float>(f2 / md);
-    f[5] = static_cast<float>(static_cast<double>(st.maxc) / md);
-    f[6] = static_cast<float>(static_cast<double>(st.heavy) / md);
-    f[7] = static_cast<float>(d / md);
-    f[8] = static_cast<float>(std::log2(gee + 1.0) / 32.0);
-    f[9] = static_cast<float>(std::log2(chao + 1.0) / 32.0);
-    f[10] = static_cast<float>(std::log2(range + 1.0) / 64.0);
-    f[11] = static_cast<float>(static_cast<double>(st.nulls) / md);
-    f[12] = static_cast<float>(keyBytes / 16.0);
-    f[13] = static_cast<float>(num_keys / 4.0);
-    f[14] = is_join ? 1.f : 0.f;
-    f[15] = 1.f;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_advise(void *mlp, const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
+                    float *y)
+{
+    OG_CHECK(mlp && y, OMNI_GPU_ERR_INVALID, "null argument");
+    MlpHandle &h = *static_cast<MlpHandle *>(mlp);
+    hipStream_t s = h.stream;
+    int64_t m = 0;
+    OG_TRY(RunSketch(s, h.sketch, keys, num_keys, rows, is_join, &m));
+    OG_TRY(h.Y.Reserve(sizeof(float) * kOut, s));
+    {
+        KernelScope ks(s, "mlp_forward");
+        mlp_forward<<<1, kMlpFwdBlock, 0, s>>>(h.params.as<float>(), h.sketch.features.as<float>(), 1, h.Y.as<float>());
+        OG_TRY(CheckLaunch("mlp_forward"));
+    }
+    OG_HIP(hipMemcpyAsync(y, h.Y.p, sizeof(float) * kOut, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipStreamSynchronize(s));
     return OMNI_GPU_OK;
 }
 
@@ -304,6 +322,7 @@ int omni_gpu_mlp_destroy(void *handle)
                       &h->idx, &h->Y}) {
         b->Release(s);
     }
+    h->sketch.Release(s);
     ReleaseStream(s);
     delete h;
     return OMNI_GPU_OK;

@@ -312,6 +312,36 @@ __global__ void __launch_bounds__(256) sketch_reduce(SketchArgs a, unsigned long
     }
 }
 
+// The MLP's 16 input features from the sketch statistics (all roughly in [0, 1]).  One thread; shared by the
+// sketch entry point and the advisor so that training and inference see identical features.
+__global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, int key_bytes, int num_keys,
+                                int is_join, float *f)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double md = m > 0 ? static_cast<double>(m) : 1.0;
+    const double d = static_cast<double>(st->distinct), f1 = static_cast<double>(st->f1),
+                 f2 = static_cast<double>(st->f2);
+    const double gee = sqrt(static_cast<double>(rows > 0 ? rows : 1) / md) * f1 + (d - f1);
+    const double chao = d + f1 * f1 / (2.0 * (f2 > 1.0 ? f2 : 1.0));
+    const double range = st->kmax >= st->kmin ? static_cast<double>(st->kmax) - static_cast<double>(st->kmin) : 0.0;
+    f[0] = static_cast<float>(log2(static_cast<double>(rows) + 1.0) / 32.0);
+    f[1] = static_cast<float>(log2(md + 1.0) / 32.0);
+    f[2] = static_cast<float>(log2(d + 1.0) / 32.0);
+    f[3] = static_cast<float>(f1 / md);
+    f[4] = static_cast<float>(f2 / md);
+    f[5] = static_cast<float>(static_cast<double>(st->maxc) / md);
+    f[6] = static_cast<float>(static_cast<double>(st->heavy) / md);
+    f[7] = static_cast<float>(d / md);
+    f[8] = static_cast<float>(log2(gee + 1.0) / 32.0);
+    f[9] = static_cast<float>(log2(chao + 1.0) / 32.0);
+    f[10] = static_cast<float>(log2(range + 1.0) / 64.0);
+    f[11] = static_cast<float>(static_cast<double>(st->nulls) / md);
+    f[12] = static_cast<float>(key_bytes / 16.0);
+    f[13] = static_cast<float>(num_keys / 4.0);
+    f[14] = is_join ? 1.f : 0.f;
+    f[15] = 1.f;
+}
+
 // ---- seeded key generator ----------------------------------------------------------------------------------
 // Row i draws a rank r in [1, D] from a Zipf(s) law by inverting its continuous approximation (s = 0 is
 // uniform), then maps r through splitmix64, a bijection, so the number of distinct keys is that of ranks.

@@ -97,18 +97,18 @@ def run_agg(data: TaskData, strategy: int, capacity_hint: int, batch_rows: int =
 
 
 def label_task(data: TaskData, repeats: int = 2) -> Tuple[int, int, List[float]]:
-    """(true groups, fastest strategy class, seconds per class) with the exact group count as capacity hint,
-    so that the label reflects the kernel choice and not the growth schedule."""
+    """(true groups, fastest strategy class, seconds per class).  A first run measures the group count; each
+    strategy is then timed with that exact capacity, so that the label reflects the kernel choice and not the
+    growth schedule."""
+    groups, _ = run_agg(data, capi.STRATEGY_GLOBAL, 0)
     times = []
-    groups = None
     for s in STRATEGIES:
         if s == capi.STRATEGY_LDS and data.task.kind == 2:
             times.append(float("inf"))  # the LDS table holds single-word keys only
             continue
         best = float("inf")
         for _ in range(repeats):
-            g, dt = run_agg(data, s, 0 if groups is None else groups)
-            groups = g
+            _, dt = run_agg(data, s, groups)
             best = min(best, dt)
         times.append(best)
     return groups, int(np.argmin(times)), times
@@ -169,8 +169,9 @@ class Advisor:
         return groups, cls
 
     def plan(self, data: TaskData) -> Tuple[int, int]:
-        sk = data.sketch()
-        groups, cls = self.predict(np.frombuffer(bytes(sk.features), np.float32))
+        y = self.mlp.advise(data.key_cols(), data.task.rows)
+        groups = [2.0 ** (min(max(float(y[0]), 0.0), 1.0) * T_SCALE) - 1.0]
+        cls = [int(np.argmax(y[1:3]))]  # the radix class has no aggregation kernel yet
         strategy = STRATEGIES[int(cls[0])]
         if strategy == capi.STRATEGY_LDS and data.task.kind == 2:
             strategy = capi.STRATEGY_GLOBAL

@@ -1,0 +1,172 @@
+"""The learned strategy/capacity advisor on the GPU: MLP kernels against the float64 CPU restatement
+(oracle/mlp_ref.py), the sample sketch against exact counts, the seeded key generator, and the advisor's gain
+over the plain method with identical results."""
+import math
+import time
+
+import numpy as np
+import pytest
+
+from oracle import mlp_ref
+from omnioperator_amd import capi, learned
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(seed, B):
+    rng = np.random.default_rng(seed)
+    p = rng.normal(0, 0.2, mlp_ref.N_PARAMS).astype(np.float32)
+    X = rng.normal(0, 1, (B, 16)).astype(np.float32)
+    t = rng.normal(0, 1, B).astype(np.float32)
+    c = rng.integers(0, 3, B).astype(np.int32)
+    return p, X, t, c
+
+
+def test_mlp_forward_matches_oracle():
+    p, X, _, _ = _case(1, 200)
+    m = capi.Mlp(0)
+    m.set_params(p)
+    y = m.forward(X)
+    m.close()
+    want, _ = mlp_ref.forward(p.astype(np.float64), X.astype(np.float64))
+    np.testing.assert_allclose(y, want, rtol=1e-4, atol=1e-4)
+
+
+@pytest.mark.parametrize("B", [1, 31, 32, 100, 257])
+def test_mlp_gradient_matches_oracle(B):
+    """fp32 GPU gradient (per-workgroup partials summed in a fixed order) against the float64 analytic gradient,
+    itself pinned by finite differences; relative error of the whole gradient vector below 1e-4."""
+    p, X, t, c = _case(10 + B, B)
+    m = capi.Mlp(0)
+    m.set_params(p)
+    g, loss = m.gradient(X, t, c)
+    m.close()
+    want = mlp_ref.grad(p.astype(np.float64), X.astype(np.float64), t.astype(np.float64), c)
+    assert np.linalg.norm(g - want) / np.linalg.norm(want) < 1e-4
+    assert abs(loss - mlp_ref.loss(p.astype(np.float64), X, t, c)) < 1e-4 * max(1.0, abs(loss))
+
+
+def test_mlp_adam_step_matches_reference():
+    """One training step of batch B equals one bias-corrected Adam step on the oracle gradient."""
+    p, X, t, c = _case(5, 64)
+    m = capi.Mlp(0)
+    m.set_params(p)
+    lr = 1e-3
+    m.train(X, t, c, batch=64, epochs=1, lr=lr, seed=0)
+    got = m.params()
+    m.close()
+    g = mlp_ref.grad(p.astype(np.float64), X.astype(np.float64), t.astype(np.float64), c)
+    mh = (0.1 * g) / (1 - 0.9)
+    vh = (0.001 * g * g) / (1 - 0.999)
+    want = p - lr * mh / (np.sqrt(vh) + 1e-8)
+    big = np.abs(g) > 1e-5   # where |g| is near the epsilon the fp32 and fp64 steps may differ in size
+    np.testing.assert_allclose(got[big], want[big], rtol=1e-3, atol=2e-6)
+    assert np.all(np.abs(got - p) <= lr * 1.001 + 1e-7)
+
+
+def test_mlp_training_fits_a_rule():
+    """Training on the GPU fits a target the network can represent: log2-like regression and a 3-way class."""
+    rng = np.random.default_rng(3)
+    X = rng.uniform(0, 1, (2048, 16)).astype(np.float32)
+    t = (0.5 * X[:, 0] + 0.25 * X[:, 1]).astype(np.float32)
+    c = np.where(X[:, 2] < 0.33, 0, np.where(X[:, 2] < 0.66, 1, 2)).astype(np.int32)
+    m = capi.Mlp(7)
+    losses = m.train(X, t, c, batch=128, epochs=60, lr=3e-3, seed=1)
+    y = m.forward(X)
+    m.close()
+    assert losses[-1] < 0.25 * losses[0]
+    assert np.mean(np.argmax(y[:, 1:], axis=1) == c) > 0.9
+
+
+def test_sketch_is_exact_when_every_row_is_sampled():
+    """Up to 4096 rows the sample is the whole column: distinct, singleton, doubleton and top counts are exact."""
+    rng = np.random.default_rng(4)
+    n = 4000
+    k = rng.zipf(1.3, n).astype(np.int64) % 700
+    nulls = rng.random(n) < 0.05
+    col = capi.DeviceColumn(capi.LONG, k, nulls)
+    sk = capi.sketch_keys([col.c()], n)
+    # NULL rows form one key of their own
+    nn = int(nulls.sum())
+    vals_nn, counts_nn = np.unique(k[~nulls], return_counts=True)
+    counts_all = np.concatenate([counts_nn, [nn]]) if nn else counts_nn
+    assert sk.sampled == n
+    assert sk.distinct == len(counts_all)
+    assert sk.f1 == int((counts_all == 1).sum())
+    assert sk.f2 == int((counts_all == 2).sum())
+    assert sk.max_count == int(counts_all.max())
+    assert sk.heavy == int(counts_all[counts_all >= 4].sum())
+    assert sk.nulls == nn
+    assert sk.key_min == int(k[~nulls].min()) and sk.key_max == int(k[~nulls].max())
+
+
+def test_generator_is_seeded_and_bounded():
+    keys1, v1 = capi.generate_keys(0, 200_000, 1000, 0.8, seed=42)
+    keys2, _ = capi.generate_keys(0, 200_000, 1000, 0.8, seed=42)
+    a, b = keys1[0].numpy(), keys2[0].numpy()
+    assert np.array_equal(a, b)
+    assert len(np.unique(a)) <= 1000
+    v = v1.numpy()
+    assert v.min() >= -1_000_000 and v.max() <= 1_000_000
+    kk, _ = capi.generate_keys(2, 50_000, 5000, 0.0, seed=7, with_values=False)
+    pairs = np.unique(np.stack([kk[0].numpy().astype(np.int64), kk[1].numpy()], axis=1), axis=0)
+    assert len(pairs) == len(np.unique(kk[1].numpy())) <= 5000
+
+
+def test_advisor_training_pipeline_reduces_loss():
+    """Runs of the aggregation itself (40 seeded tasks) labelled and fitted on the GPU: the loss falls."""
+    tasks = learned.sample_tasks(40, seed=11, min_rows=2e4, max_rows=5e5, max_distinct=5e5)
+    X, t, c, groups = learned.collect(tasks, repeats=1)
+    assert np.all(groups >= 1) and np.all(groups <= [tk.rows for tk in tasks])
+    _, losses = learned.train(X, t, c, seed=3, epochs=60)
+    assert losses[-1] < 0.5 * losses[0]
+
+
+def test_advisor_gains_over_the_plain_method():
+    """The committed advisor (trained on the GPU from 4096 runs by omnioperator_amd.train_advisor) against the plain
+    fixed-strategy method on 32 held-out tasks (seeds never used in training): the global-atomic kernel without a
+    capacity (grow-and-rehash) versus the advisor's capacity + strategy, sketch and inference time included.  Each
+    configuration runs once untimed first.  Every task's groups are identical both ways; the advisor's total time
+    is lower.  The adaptive automatic rule is timed alongside for reference."""
+    adv = learned.Advisor()
+    test_tasks = learned.sample_tasks(32, seed=777, min_rows=2e5, max_rows=8e6, max_distinct=8e6)
+    plain_t = adv_t = auto_t = 0.0
+    errs = []
+    for task in test_tasks:
+        data = learned.TaskData(task)
+        cap, strategy = adv.plan(data)
+        learned.run_agg(data, capi.STRATEGY_GLOBAL, 0)
+        learned.run_agg(data, strategy, cap)
+        learned.run_agg(data, capi.STRATEGY_AUTO, 0)
+        g0, dt0, out0 = learned.run_agg(data, capi.STRATEGY_GLOBAL, 0, output=True)
+        t0 = time.perf_counter()
+        cap, strategy = adv.plan(data)  # sketch + inference, charged to the advisor
+        t_plan = time.perf_counter() - t0
+        g1, dt1, out1 = learned.run_agg(data, strategy, cap, output=True)
+        dt1 += t_plan
+        _, dta = learned.run_agg(data, capi.STRATEGY_AUTO, 0)
+        assert g0 == g1
+        order0 = np.lexsort([o[0] for o in out0[:len(data.types)]][::-1])
+        order1 = np.lexsort([o[0] for o in out1[:len(data.types)]][::-1])
+        for (v0, n0), (v1, n1) in zip(out0, out1):
+            assert np.array_equal(v0[order0], v1[order1]) and np.array_equal(n0[order0], n1[order1])
+        plain_t += dt0
+        adv_t += dt1
+        auto_t += dta
+        errs.append(abs(math.log2((cap / learned.CAPACITY_MARGIN) + 1) - math.log2(g0 + 1)))
+    adv.close()
+    print(f"plain fixed global {plain_t * 1e3:.1f} ms, advisor {adv_t * 1e3:.1f} ms ({plain_t / adv_t:.2f}x), "
+          f"automatic rule {auto_t * 1e3:.1f} ms, median |log2 error| {np.median(errs):.2f}")
+    assert np.median(errs) < 1.0
+    assert adv_t < plain_t
+
+
+def test_capacity_below_the_groups_falls_back_to_growth():
+    """A capacity hint far below the true group count (a wrong prediction) still gives the exact groups."""
+    data = learned.TaskData(learned.Task(0, 500_000, 300_000, 0.3, 99))
+    g0, _, out0 = learned.run_agg(data, capi.STRATEGY_GLOBAL, 0, output=True)
+    g1, _, out1 = learned.run_agg(data, capi.STRATEGY_GLOBAL, 1000, output=True)
+    assert g0 == g1
+    o0, o1 = np.argsort(out0[0][0]), np.argsort(out1[0][0])
+    for (v0, _), (v1, _) in zip(out0, out1):
+        assert np.array_equal(v0[o0], v1[o1])
