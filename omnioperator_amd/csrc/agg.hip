@@ -287,10 +287,10 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     OG_TRY(Resize(*h, cap, 0));
     // Strategy: the LDS-local table serves single-word keys whose group count fits it (a capacity hint
     // above half the LDS slots, or a multi-word key, selects the global table directly).
-    h->lds = MakeLdsLayout(h->args, kLdsBudget);
-    bool ldsFits = h->single && h->lds.S >= 256;
+    h->lds = MakeLdsLayout(h->args, kLdsBudget, h->single ? 1 : h->kd.W);
+    bool ldsFits = h->lds.S >= 256;
     if (spec->strategy == OMNI_GPU_STRATEGY_LDS) {
-        OG_CHECK(ldsFits, OMNI_GPU_ERR_INVALID, "the LDS strategy needs one key column of <= 8 bytes");
+        OG_CHECK(ldsFits, OMNI_GPU_ERR_INVALID, "the aggregates leave too few LDS slots for the LDS strategy");
         h->use_lds = true;
     } else if (spec->strategy == OMNI_GPU_STRATEGY_AUTO) {
         h->use_lds = ldsFits && (spec->capacity_hint <= 0 || spec->capacity_hint * 2 <= h->lds.S);
@@ -349,8 +349,14 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         blocks = std::max<int64_t>(1, std::min<int64_t>(blocks, kMaxLdsBlocks));
         int64_t per = (rows + blocks - 1) / blocks;
         KernelScope ks(h.stream, "agg_insert_lds");
-        agg_insert_lds<<<static_cast<unsigned int>(blocks), kLdsBlock, h.lds.bytes, h.stream>>>(
-            h.kd, rows, per, h.slots.as<Slot1>(), h.cap - 1, g, h.args, h.lds, h.spills(), cur);
+        if (h.single) {
+            agg_insert_lds<<<static_cast<unsigned int>(blocks), kLdsBlock, h.lds.bytes, h.stream>>>(
+                h.kd, rows, per, h.slots.as<Slot1>(), h.cap - 1, g, h.args, h.lds, h.spills(), cur);
+        } else {
+            agg_insert_lds_multi<<<static_cast<unsigned int>(blocks), kLdsBlock, h.lds.bytes, h.stream>>>(
+                h.kd, rows, per, h.ctrl.as<unsigned int>(), h.skeys.as<unsigned long long>(), h.cap - 1, g, h.args,
+                h.lds, h.spills(), cur);
+        }
         OG_TRY(CheckLaunch("agg_insert_lds"));
         h.lds_rows += rows;
         h.pending_rows += rows;

@@ -334,7 +334,8 @@ struct GlobalRef {
 
 struct LdsLayout {
     int S;                              // LDS slots
-    int keys_off;                       // byte offset of the u64 key array
+    int keys_off;                       // byte offset of the u64 key array (W words per slot)
+    int ctrl_off;                       // byte offset of the u32 slot states of multi-word keys (-1: one word)
     int off[OMNI_GPU_MAX_AGGS][3];      // byte offsets of each aggregate's state arrays (-1 unused)
     int flag_off[OMNI_GPU_MAX_AGGS];    // byte offsets of the flag arrays (-1 unused)
     int bytes;                          // total LDS bytes

@@ -134,6 +134,92 @@ __global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t r
     StoreWaveChunk(wc[WaveInBlock()], cursors, wslot);
 }
 
+// LDS-local partial aggregation of multi-word keys (several key columns or a DECIMAL128 key): as
+// agg_insert_lds, with an LDS slot state per slot (0 empty, 1 being written, 2 ready) guarding the W key words.
+// Hot keys of a skewed composite key are aggregated in LDS instead of contending on one global state row.
+__global__ void __launch_bounds__(kLdsBlock) agg_insert_lds_multi(KeyDev kd, int64_t rows, int64_t rows_per_block,
+                                                                   unsigned int *ctrl, unsigned long long *skeys,
+                                                                   unsigned long long mask, GroupCtx g, AggArgs args,
+                                                                   LdsLayout L, unsigned long long *spill_counter,
+                                                                   unsigned long long *cursors)
+{
+    OG_DYNAMIC_LDS(unsigned long long, lds_words);
+    __shared__ unsigned long long wc[kLdsWaves][2];
+    const int wslot = blockIdx.x * kLdsWaves + WaveInBlock();
+    LoadWaveChunk(wc[WaveInBlock()], cursors, wslot);
+    g.wave = wc[WaveInBlock()];
+    uint8_t *base = reinterpret_cast<uint8_t *>(lds_words);
+    unsigned long long *lkeys = reinterpret_cast<unsigned long long *>(base + L.keys_off);
+    unsigned int *lctrl = reinterpret_cast<unsigned int *>(base + L.ctrl_off);
+    const int S = L.S, W = kd.W;
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        lctrl[i] = 0;
+        for (int a = 0; a < args.n; ++a) {
+            int kind = args.a[a].kind;
+            if (Is32(kind)) {
+                reinterpret_cast<int *>(base + L.off[a][0])[i] = static_cast<int>(InitWord(kind));
+            } else {
+                reinterpret_cast<unsigned long long *>(base + L.off[a][0])[i] = InitWord(kind);
+            }
+            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = 0;
+            if (L.off[a][2] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][2])[i] = 0;
+            if (L.flag_off[a] >= 0) base[L.flag_off[a] + i] = 0;
+        }
+    }
+    __syncthreads();
+    int64_t start = static_cast<int64_t>(blockIdx.x) * rows_per_block;
+    int64_t end = start + rows_per_block < rows ? start + rows_per_block : rows;
+    unsigned int spilled = 0;
+    for (int64_t row = start + threadIdx.x; row < end; row += blockDim.x) {
+        unsigned long long words[kMaxWords];
+        EncodeKey(kd, row, words);
+        const unsigned long long h = HashWords(words, W);
+        int slot = -1;
+        unsigned int pos = static_cast<unsigned int>(((h >> 32) * static_cast<unsigned long long>(S)) >> 32);
+        int step = 0;
+        while (step < kLdsMaxProbe) {
+            unsigned int c = __hip_atomic_load(&lctrl[pos], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if (c == 0) {
+                unsigned int exp = 0;
+                if (__hip_atomic_compare_exchange_strong(&lctrl[pos], &exp, 1u, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                         __HIP_MEMORY_SCOPE_WORKGROUP)) {
+                    for (int w = 0; w < W; ++w) lkeys[pos * W + w] = words[w];
+                    __hip_atomic_store(&lctrl[pos], 2u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+                    slot = static_cast<int>(pos);
+                    break;
+                }
+                c = exp;
+            }
+            if (c == 1) continue;  // another lane is writing this slot's key words: read the state again
+            bool same = true;
+            for (int w = 0; w < W; ++w) same &= lkeys[pos * W + w] == words[w];
+            if (same) {
+                slot = static_cast<int>(pos);
+                break;
+            }
+            pos = pos + 1 == static_cast<unsigned int>(S) ? 0 : pos + 1;
+            ++step;
+        }
+        if (slot >= 0) {
+            UpdateAggsT(args, LdsRef{base, L, slot}, row);
+        } else {
+            ++spilled;
+            unsigned int gid = InsertMulti(ctrl, skeys, mask, words, W, h, g);
+            if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < S; i += blockDim.x) {
+        if (lctrl[i] != 2u) continue;
+        unsigned long long words[kMaxWords];
+        for (int w = 0; w < W; ++w) words[w] = lkeys[i * W + w];
+        unsigned int gid = InsertMulti(ctrl, skeys, mask, words, W, HashWords(words, W), g);
+        if (gid != 0xFFFFFFFFu) MergeAggs(args, gid, LdsRef{base, L, i});
+    }
+    if (spilled) atomicAdd(spill_counter, static_cast<unsigned long long>(spilled));
+    StoreWaveChunk(wc[WaveInBlock()], cursors, wslot);
+}
+
 // Rehash after growth: re-inserts the live groups of [0, n) under their ids (states do not move).
 __global__ void agg_rehash_single(const unsigned long long *gkeys, const uint8_t *gvalid, int64_t n, Slot1 *slots,
                                   unsigned long long mask, const unsigned int *special)

@@ -122,12 +122,13 @@ inline std::vector<unsigned long long> RowTemplate(const AggArgs &args)
     return w;
 }
 
-// LDS table layout for the aggregates of a handle within `budget` bytes: per slot one u64 key plus each
-// aggregate's state words (4 bytes for 32-bit MIN/MAX) and a flag byte where the aggregate needs one.
-inline LdsLayout MakeLdsLayout(const AggArgs &args, int budget)
+// LDS table layout for the aggregates of a handle within `budget` bytes: per slot `key_words` u64 key words (and a
+// u32 slot state for multi-word keys) plus each aggregate's state words (4 bytes for 32-bit MIN/MAX) and a flag
+// byte where the aggregate needs one.
+inline LdsLayout MakeLdsLayout(const AggArgs &args, int budget, int key_words = 1)
 {
     LdsLayout L{};
-    int perSlot = 8;
+    int perSlot = 8 * key_words + (key_words > 1 ? 4 : 0);
     for (int i = 0; i < args.n; ++i) {
         int k = args.a[i].kind;
         perSlot += Is32(k) ? 4 : 8;
@@ -144,7 +145,8 @@ inline LdsLayout MakeLdsLayout(const AggArgs &args, int budget)
         off += (bytes + 7) & ~7;
         return o;
     };
-    L.keys_off = take(8 * S);
+    L.keys_off = take(8 * S * key_words);
+    L.ctrl_off = key_words > 1 ? take(4 * S) : -1;
     for (int i = 0; i < args.n; ++i) {
         int k = args.a[i].kind;
         L.off[i][0] = take((Is32(k) ? 4 : 8) * S);

@@ -103,9 +103,6 @@ def label_task(data: TaskData, repeats: int = 2) -> Tuple[int, int, List[float]]
     groups, _ = run_agg(data, capi.STRATEGY_GLOBAL, 0)
     times = []
     for s in STRATEGIES:
-        if s == capi.STRATEGY_LDS and data.task.kind == 2:
-            times.append(float("inf"))  # the LDS table holds single-word keys only
-            continue
         best = float("inf")
         for _ in range(repeats):
             _, dt = run_agg(data, s, groups)
@@ -173,8 +170,6 @@ class Advisor:
         groups = [2.0 ** (min(max(float(y[0]), 0.0), 1.0) * T_SCALE) - 1.0]
         cls = [int(np.argmax(y[1:3]))]  # the radix class has no aggregation kernel yet
         strategy = STRATEGIES[int(cls[0])]
-        if strategy == capi.STRATEGY_LDS and data.task.kind == 2:
-            strategy = capi.STRATEGY_GLOBAL
         cap = int(min(float(data.task.rows), groups[0] * CAPACITY_MARGIN)) + 16
         return cap, strategy
 

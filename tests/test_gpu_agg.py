@@ -193,6 +193,27 @@ def test_many_groups_chunked_ids(composite):
     assert_row_sets_equal(got, want)
 
 
+@pytest.mark.parametrize("strategy", [capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL, capi.STRATEGY_AUTO])
+@pytest.mark.parametrize("groups", [40, 30_000])
+def test_composite_key_strategy_parity(strategy, groups):
+    """Composite (int32, int64) keys with NULLs in both columns through the LDS table of multi-word keys (hot
+    keys aggregated in LDS, the rest spilled to the global table) and through the global table."""
+    rng = np.random.default_rng(groups + strategy)
+    n = 300_000
+    r = np.minimum(rng.zipf(1.3, n), groups) - 1
+    k1 = (r % 97).astype(np.int32)
+    k2 = (r // 97).astype(np.int64) * 1_000_003 - 11
+    keys = [Col(INT, k1, rng.random(n) < 0.01), Col(LONG, k2, rng.random(n) < 0.01)]
+    v = rng.integers(-10**12, 10**12, n, dtype=np.int64)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MAX, LONG, LONG), (SUM, DECIMAL64, DECIMAL128, 0, 0),
+            (AVG, DOUBLE, DOUBLE)]
+    # doubles of one sign (N(100, 50)): sums without cancellation, so a relative tolerance applies
+    cols = [Col(LONG, v, rng.random(n) < 0.05), None, Col(LONG, v), Col(DECIMAL64, v), Col(DOUBLE, rng.normal(100, 50, n))]
+    got = run_gpu(keys, aggs, cols, 100_000, strategy=strategy)
+    want = run_oracle(keys, aggs, cols, 100_000)
+    assert_row_sets_equal(got, want, float_cols=float_cols(2, aggs), rtol=DOUBLE_RTOL)
+
+
 @pytest.mark.parametrize("strategy", [capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL])
 @pytest.mark.parametrize("key_kind", ["long", "int", "double"])
 @pytest.mark.parametrize("with_nulls", [False, True])
