@@ -24,10 +24,20 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_lds(const Ba
     BatchLds<BLOCK>(*cfg, first + static_cast<int>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
 }
 
-template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm(const BatchConfig *__restrict__ cfg, int first)
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_init(const BatchConfig *__restrict__ cfg, int64_t first)
+{
+    BatchHbmInit<BLOCK>(*cfg, first + static_cast<int64_t>(blockIdx.x));
+}
+
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_insert(const BatchConfig *__restrict__ cfg, int64_t first)
+{
+    BatchHbmInsert<BLOCK>(*cfg, first + static_cast<int64_t>(blockIdx.x));
+}
+
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_emit(const BatchConfig *__restrict__ cfg, int first)
 {
     __shared__ unsigned int sh[2];
-    BatchHbm<BLOCK>(*cfg, first + static_cast<int>(blockIdx.x), sh);
+    BatchHbmEmit<BLOCK>(*cfg, first + static_cast<int>(blockIdx.x), sh);
 }
 
 __global__ void __launch_bounds__(kScanBlock) batch_scan(const BatchConfig *__restrict__ cfg)
@@ -50,7 +60,8 @@ struct BatchesHandle {
     int lds_budget = 32 * 1024;
     int max_probe = 32;
     int64_t max_batch_rows = 0;
-    DevBuf dcfg, offsets, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error flag
+    DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error flag
+    std::vector<int64_t> h_tiles;  // phase-2 tile prefix over the batches
     size_t pool_bytes = 0;
     std::vector<int64_t> h_offsets;
     std::vector<int32_t> h_modes;
@@ -105,14 +116,32 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     OG_TRY(CheckLaunch("batch_lds"));
     batch_scan<<<1, kScanBlock, 0, s>>>(dc);
     OG_TRY(CheckLaunch("batch_scan"));
+    const int64_t tiles = h.cfg.total_tiles;
     if (per_batch) {
         for (int b = 0; b < nb; ++b) {
-            if (h.block == 512) batch_hbm<512><<<1, 512, 0, s>>>(dc, b);
-            else batch_hbm<256><<<1, 256, 0, s>>>(dc, b);
+            const int64_t t0 = h.h_tiles[b], nt = h.h_tiles[b + 1] - h.h_tiles[b];
+            if (nt == 0) continue;
+            if (h.block == 512) {
+                batch_hbm_init<512><<<static_cast<unsigned int>(nt), 512, 0, s>>>(dc, t0);
+                batch_hbm_insert<512><<<static_cast<unsigned int>(nt), 512, 0, s>>>(dc, t0);
+                batch_hbm_emit<512><<<1, 512, 0, s>>>(dc, b);
+            } else {
+                batch_hbm_init<256><<<static_cast<unsigned int>(nt), 256, 0, s>>>(dc, t0);
+                batch_hbm_insert<256><<<static_cast<unsigned int>(nt), 256, 0, s>>>(dc, t0);
+                batch_hbm_emit<256><<<1, 256, 0, s>>>(dc, b);
+            }
         }
-    } else {
-        if (h.block == 512) batch_hbm<512><<<nb, 512, 0, s>>>(dc, 0);
-        else batch_hbm<256><<<nb, 256, 0, s>>>(dc, 0);
+    } else if (tiles > 0) {
+        const unsigned int g = static_cast<unsigned int>(tiles);
+        if (h.block == 512) {
+            batch_hbm_init<512><<<g, 512, 0, s>>>(dc, 0);
+            batch_hbm_insert<512><<<g, 512, 0, s>>>(dc, 0);
+            batch_hbm_emit<512><<<nb, 512, 0, s>>>(dc, 0);
+        } else {
+            batch_hbm_init<256><<<g, 256, 0, s>>>(dc, 0);
+            batch_hbm_insert<256><<<g, 256, 0, s>>>(dc, 0);
+            batch_hbm_emit<256><<<nb, 256, 0, s>>>(dc, 0);
+        }
     }
     return CheckLaunch("batch_hbm");
 }
@@ -229,6 +258,15 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         OG_TRY(h.groups.Reserve(sizeof(unsigned long long) * num_batches, s));
         OG_TRY(h.region_off.Reserve(sizeof(unsigned long long) * num_batches, s));
         OG_HIP(hipMemcpyAsync(h.offsets.p, batch_offsets, sizeof(int64_t) * (num_batches + 1), hipMemcpyHostToDevice, s));
+        h.h_tiles.assign(num_batches + 1, 0);
+        for (int b = 0; b < num_batches; ++b) {
+            // at least one tile per batch: an empty batch sent to phase 2 still initialises its table
+            h.h_tiles[b + 1] = h.h_tiles[b] +
+                               std::max<int64_t>(1, (batch_offsets[b + 1] - batch_offsets[b] + kHbmTile - 1) / kHbmTile);
+        }
+        OG_CHECK(h.h_tiles[num_batches] < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many rows for one run");
+        OG_TRY(h.tiles.Reserve(sizeof(int64_t) * (num_batches + 1), s));
+        OG_HIP(hipMemcpyAsync(h.tiles.p, h.h_tiles.data(), sizeof(int64_t) * (num_batches + 1), hipMemcpyHostToDevice, s));
         if (batch_modes) {
             OG_TRY(h.modes.Reserve(sizeof(int32_t) * num_batches, s));
             OG_HIP(hipMemcpyAsync(h.modes.p, batch_modes, sizeof(int32_t) * num_batches, hipMemcpyHostToDevice, s));
@@ -243,6 +281,8 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         h.have_desc = true;
     }
     c.offsets = h.offsets.as<int64_t>();
+    c.tile_start = h.tiles.as<int64_t>();
+    c.total_tiles = h.h_tiles.back();
     c.modes = batch_modes ? h.modes.as<int32_t>() : nullptr;
     c.nbatches = num_batches;
     c.spilled = h.spilled.as<unsigned int>();
@@ -315,6 +355,7 @@ int omni_gpu_agg_batches_destroy(void *handle)
     hipStream_t s = h->stream;
     h->dcfg.Release(s);
     h->offsets.Release(s);
+    h->tiles.Release(s);
     h->modes.Release(s);
     h->spilled.Release(s);
     h->groups.Release(s);

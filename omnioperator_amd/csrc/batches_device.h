@@ -2,8 +2,10 @@
 //
 // Phase 1 (one workgroup per batch): the batch is aggregated in an LDS table; if any row finds no LDS slot
 // within the probe limit, the workgroup abandons the batch (writes nothing) and flags it.
-// Phase 2 (one workgroup per batch, flagged batches only): the batch is aggregated exactly in a private HBM
-// table of next_pow2(2 * rows) slots (plus 2 side slots) carved from a region pool at a scanned offset.
+// Phase 2 (flagged batches only): the batch is aggregated exactly in its own HBM table of next_pow2(2 * rows)
+// slots (plus 2 side slots) carved from a region pool at a scanned offset.  Three launches: the tables are
+// initialised and filled by workgroups of kHbmTile rows each (a large batch is spread over many workgroups; the
+// table words are updated with device-wide atomics), then one workgroup per batch emits its groups.
 // Both phases emit the batch's groups to output rows [off[b], off[b] + groups[b]).
 // The configuration lives in device memory and is read through one pointer (no large kernel arguments).
 #pragma once
@@ -24,6 +26,9 @@ struct BatchConfig {
     int32_t max_probe;
     int32_t overflow_as_null;
     int32_t pad;
+    const int64_t *tile_start; // nb + 1: first phase-2 tile of each batch (kHbmTile rows per tile, at least one
+                               // tile per batch, all batches)
+    int64_t total_tiles;
     unsigned int *spilled;     // per batch: 1 = phase 2 must aggregate it
     unsigned long long *groups;  // per batch
     unsigned long long *region_off;  // per batch byte offset into the region pool (phase 2)
@@ -174,41 +179,84 @@ __device__ __forceinline__ int TableSlot(unsigned long long *keys, unsigned long
     }
 }
 
-// Phase 2: exact aggregation of a flagged batch in its private HBM table (load factor <= 1/2, so every
-// probe sequence ends).
-template <int BLOCK>
-__device__ __forceinline__ void BatchHbm(const BatchConfig &c, int b, unsigned int *sh)
+constexpr int kHbmTile = 4096;  // rows (and table slots) per phase-2 workgroup
+
+// Batch owning phase-2 tile `t` (binary search over the tile prefix of all batches).
+__device__ __forceinline__ int TileBatch(const BatchConfig &c, int64_t t)
 {
+    int lo = 0, hi = c.nbatches - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (c.tile_start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Phase 2a: tile t of a flagged batch initialises its share of the batch's table slots (and side slots).
+template <int BLOCK>
+__device__ __forceinline__ void BatchHbmInit(const BatchConfig &c, int64_t t)
+{
+    const int b = TileBatch(c, t);
     if (!c.spilled[b]) return;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
     const int S = RegionSlots(end - start);
     const long long stride = RegionStride(end - start);
-    const LdsLayout &R = c.R;
+    const int64_t ntiles = c.tile_start[b + 1] - c.tile_start[b];
+    const int64_t k = t - c.tile_start[b];
+    const int64_t span = (static_cast<int64_t>(S) + 2 + ntiles - 1) / ntiles;
+    const int64_t s0 = k * span, s1 = s0 + span < S + 2 ? s0 + span : S + 2;
     uint8_t *base = c.region_pool + c.region_off[b];
     unsigned long long *keys = reinterpret_cast<unsigned long long *>(base);
-    for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
+    for (int64_t i = s0 + threadIdx.x; i < s1; i += BLOCK) {
         keys[i] = kEmptyKey;
-        InitSlot(c.args, ScaledRef{base, R, stride, i});
+        InitSlot(c.args, ScaledRef{base, c.R, stride, static_cast<int>(i)});
     }
-    if (threadIdx.x == 0) sh[1] = 0;
-    __syncthreads();
-    for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
+}
+
+// Phase 2b: tile t of a flagged batch inserts its kHbmTile rows (load factor <= 1/2: every probe ends).
+template <int BLOCK>
+__device__ __forceinline__ void BatchHbmInsert(const BatchConfig &c, int64_t t)
+{
+    const int b = TileBatch(c, t);
+    if (!c.spilled[b]) return;
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    const int S = RegionSlots(end - start);
+    const long long stride = RegionStride(end - start);
+    const int64_t r0 = start + (t - c.tile_start[b]) * kHbmTile;
+    const int64_t r1 = r0 + kHbmTile < end ? r0 + kHbmTile : end;
+    uint8_t *base = c.region_pool + c.region_off[b];
+    unsigned long long *keys = reinterpret_cast<unsigned long long *>(base);
+    for (int64_t row = r0 + threadIdx.x; row < r1; row += BLOCK) {
         bool isNull = NullBit(c.kd.nulls[0], row);
         unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
         int slot;
         if (isNull) {
             slot = S;
-            keys[S] = 0;
+            keys[S] = 0;  // marks the side slot used
         } else if (key == kEmptyKey) {
             slot = S + 1;
             keys[S + 1] = 0;
         } else {
             slot = TableSlot(keys, key, static_cast<unsigned int>(S - 1));
         }
-        UpdateAggsT(c.args, ScaledRef{base, R, stride, slot}, row);
+        UpdateAggsT(c.args, ScaledRef{base, c.R, stride, slot}, row);
     }
+}
+
+// Phase 2c: emits the groups of flagged batch b (a later launch, so every insert is visible).
+template <int BLOCK>
+__device__ __forceinline__ void BatchHbmEmit(const BatchConfig &c, int b, unsigned int *sh)
+{
+    if (!c.spilled[b]) return;
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    const int S = RegionSlots(end - start);
+    const long long stride = RegionStride(end - start);
+    uint8_t *base = c.region_pool + c.region_off[b];
+    const unsigned long long *keys = reinterpret_cast<const unsigned long long *>(base);
+    if (threadIdx.x == 0) sh[1] = 0;
     __syncthreads();
-    EmitTable<BLOCK>(c, keys, S, start, &sh[1], [&](int i) { return ScaledRef{base, R, stride, i}; });
+    EmitTable<BLOCK>(c, keys, S, start, &sh[1], [&](int i) { return ScaledRef{base, c.R, stride, i}; });
     __syncthreads();
     if (threadIdx.x == 0) c.groups[b] = sh[1];
 }
