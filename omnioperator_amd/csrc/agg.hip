@@ -171,19 +171,15 @@ int Resize(AggHandle &h, uint64_t cap, int64_t keep)
 
 int SyncGroups(AggHandle &h)
 {
-    unsigned long long g = 0;
+    unsigned long long g = 0, sp = 0;
     unsigned int ovf = 0;
-    OG_HIP(hipMemcpyAsync(&g, h.ngroups(), 8, hipMemcpyDeviceToHost, h.stream));
-    OG_HIP(hipMemcpyAsync(&ovf, h.overflow(), 4, hipMemcpyDeviceToHost, h.stream));
-    OG_HIP(hipStreamSynchronize(h.stream));
+    OG_TRY(CopyToHost(h.stream, {{&g, h.ngroups(), 8}, {&ovf, h.overflow(), 4}, {&sp, h.spills(), 8}}));
     OG_CHECK(ovf == 0, OMNI_GPU_ERR_STATE, "aggregation table overflowed its group arrays (internal sizing error)");
     h.known_ids = static_cast<int64_t>(g);
     h.pending_rows = 0;
     if (h.use_lds && h.spec.strategy == OMNI_GPU_STRATEGY_AUTO && h.lds_rows > 0) {
         // Adaptive fallback: when most rows miss the LDS table (too many groups per workgroup chunk),
         // later batches go straight to the global table.
-        unsigned long long sp = 0;
-        OG_HIP(hipMemcpy(&sp, h.spills(), 8, hipMemcpyDeviceToHost));
         int64_t delta = static_cast<int64_t>(sp) - h.spills_seen;
         h.spills_seen = static_cast<int64_t>(sp);
         if (delta * 2 > h.lds_rows) h.use_lds = false;
@@ -216,8 +212,7 @@ int Compact(AggHandle &h)
         OG_TRY(CheckLaunch("agg_compact"));
     }
     unsigned int live = 0;
-    OG_HIP(hipMemcpyAsync(&live, h.live(), 4, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipStreamSynchronize(s));
+    OG_TRY(CopyToHost(s, {{&live, h.live(), 4}}));
     h.live_groups = live;
     return OMNI_GPU_OK;
 }
@@ -279,8 +274,7 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     OG_HIP(hipMemsetAsync(h->cursors.p, 0, sizeof(unsigned long long) * 2 * kCursorSlots, h->stream));
     std::vector<unsigned long long> tmpl = RowTemplate(h->args);
     OG_TRY(h->row_tmpl.Reserve(sizeof(unsigned long long) * tmpl.size(), h->stream));
-    OG_HIP(hipMemcpyAsync(h->row_tmpl.p, tmpl.data(), sizeof(unsigned long long) * tmpl.size(),
-                          hipMemcpyHostToDevice, h->stream));
+    OG_TRY(CopyToDevice(h->row_tmpl.p, tmpl.data(), sizeof(unsigned long long) * tmpl.size(), h->stream));
     uint64_t want = spec->capacity_hint > 0 ? static_cast<uint64_t>(spec->capacity_hint) : 16384;
     uint64_t cap = NextPow2(static_cast<uint64_t>(static_cast<double>(want + 2) / kMaxLoad) + 1);
     cap = std::max<uint64_t>(cap, 1024);
@@ -295,7 +289,6 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     } else if (spec->strategy == OMNI_GPU_STRATEGY_AUTO) {
         h->use_lds = ldsFits && (spec->capacity_hint <= 0 || spec->capacity_hint * 2 <= h->lds.S);
     }
-    OG_HIP(hipStreamSynchronize(h->stream));
     *handle = h.release();
     return OMNI_GPU_OK;
 }
@@ -380,7 +373,7 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
 int omni_gpu_agg_wait(void *handle)
 {
     OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
-    OG_HIP(hipStreamSynchronize(static_cast<AggHandle *>(handle)->stream));
+    OG_TRY(SyncStream(static_cast<AggHandle *>(handle)->stream));
     return OMNI_GPU_OK;
 }
 
@@ -423,8 +416,7 @@ int omni_gpu_agg_get_output(void *handle, int64_t start, int64_t count, omni_gpu
         OG_TRY(CheckLaunch("agg_output"));
     }
     unsigned int err = 0;
-    OG_HIP(hipMemcpyAsync(&err, h.error_flag(), 4, hipMemcpyDeviceToHost, h.stream));
-    OG_HIP(hipStreamSynchronize(h.stream));
+    OG_TRY(CopyToHost(h.stream, {{&err, h.error_flag(), 4}}));
     if (err) return SetError(OMNI_GPU_ERR_OVERFLOW, "decimal overflow in SUM (overflowAsNull is false)");
     return OMNI_GPU_OK;
 }

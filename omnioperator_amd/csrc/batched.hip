@@ -174,7 +174,7 @@ int omni_gpu_agg_batches_create(const omni_gpu_agg_spec *spec, int64_t max_batch
     OG_HIP(hipMemsetAsync(h->counters.p, 0, 64, h->stream));
     OG_TRY(h->dcfg.Reserve(sizeof(BatchConfig), h->stream));
     OG_TRY(Layout(*h));
-    OG_HIP(hipStreamSynchronize(h->stream));
+    OG_TRY(SyncStream(h->stream));
     *handle = h.release();
     return OMNI_GPU_OK;
 }
@@ -257,7 +257,7 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         OG_TRY(h.spilled.Reserve(sizeof(unsigned int) * num_batches, s));
         OG_TRY(h.groups.Reserve(sizeof(unsigned long long) * num_batches, s));
         OG_TRY(h.region_off.Reserve(sizeof(unsigned long long) * num_batches, s));
-        OG_HIP(hipMemcpyAsync(h.offsets.p, batch_offsets, sizeof(int64_t) * (num_batches + 1), hipMemcpyHostToDevice, s));
+        OG_TRY(CopyToDevice(h.offsets.p, batch_offsets, sizeof(int64_t) * (num_batches + 1), s));
         h.h_tiles.assign(num_batches + 1, 0);
         for (int b = 0; b < num_batches; ++b) {
             // at least one tile per batch: an empty batch sent to phase 2 still initialises its table
@@ -266,10 +266,10 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         }
         OG_CHECK(h.h_tiles[num_batches] < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many rows for one run");
         OG_TRY(h.tiles.Reserve(sizeof(int64_t) * (num_batches + 1), s));
-        OG_HIP(hipMemcpyAsync(h.tiles.p, h.h_tiles.data(), sizeof(int64_t) * (num_batches + 1), hipMemcpyHostToDevice, s));
+        OG_TRY(CopyToDevice(h.tiles.p, h.h_tiles.data(), sizeof(int64_t) * (num_batches + 1), s));
         if (batch_modes) {
             OG_TRY(h.modes.Reserve(sizeof(int32_t) * num_batches, s));
-            OG_HIP(hipMemcpyAsync(h.modes.p, batch_modes, sizeof(int32_t) * num_batches, hipMemcpyHostToDevice, s));
+            OG_TRY(CopyToDevice(h.modes.p, batch_modes, sizeof(int32_t) * num_batches, s));
         }
         // region pool: room for every batch to need its HBM table
         size_t pool = 0;
@@ -290,7 +290,7 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.region_off = h.region_off.as<unsigned long long>();
     c.region_pool = h.pool.as<uint8_t>();
     c.error_flag = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>());
-    OG_HIP(hipMemcpyAsync(h.dcfg.p, &c, sizeof(BatchConfig), hipMemcpyHostToDevice, s));
+    OG_TRY(CopyToDevice(h.dcfg.p, &c, sizeof(BatchConfig), s));
     OG_HIP(hipMemsetAsync(c.error_flag, 0, 4, s));
     const bool per_batch = flags & 1;
     const bool use_graph = (flags & 2) && !per_batch;
@@ -321,9 +321,7 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     }
     std::vector<unsigned long long> g(num_batches);
     unsigned int err = 0;
-    OG_HIP(hipMemcpyAsync(g.data(), h.groups.p, sizeof(unsigned long long) * num_batches, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipMemcpyAsync(&err, c.error_flag, 4, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipStreamSynchronize(s));
+    OG_TRY(CopyToHost(s, {{g.data(), h.groups.p, sizeof(unsigned long long) * num_batches}, {&err, c.error_flag, 4}}));
     for (int b = 0; b < num_batches; ++b) groups_per_batch[b] = static_cast<int64_t>(g[b]);
     if (err) return SetError(OMNI_GPU_ERR_OVERFLOW, "decimal overflow in SUM (overflowAsNull is false)");
     return OMNI_GPU_OK;
@@ -337,8 +335,7 @@ int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_
     int64_t nb = h.have_desc ? static_cast<int64_t>(h.h_offsets.size()) - 1 : 0;
     std::vector<unsigned int> sp(static_cast<size_t>(std::max<int64_t>(nb, 1)));
     if (nb > 0) {
-        OG_HIP(hipMemcpyAsync(sp.data(), h.spilled.p, sizeof(unsigned int) * nb, hipMemcpyDeviceToHost, h.stream));
-        OG_HIP(hipStreamSynchronize(h.stream));
+        OG_TRY(CopyToHost(h.stream, {{sp.data(), h.spilled.p, sizeof(unsigned int) * nb}}));
     }
     int64_t hbm = 0;
     for (int64_t b = 0; b < nb; ++b) hbm += sp[b] ? 1 : 0;

@@ -181,7 +181,7 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
         a.batches = nullptr;
     } else {
         OG_TRY(h.batch_desc.Reserve(sizeof(ProbeBatch) * nb, s));
-        OG_HIP(hipMemcpyAsync(h.batch_desc.p, batches, sizeof(ProbeBatch) * nb, hipMemcpyHostToDevice, s));
+        OG_TRY(CopyToDevice(h.batch_desc.p, batches, sizeof(ProbeBatch) * nb, s));
         a.batches = h.batch_desc.as<ProbeBatch>();
     }
     OG_HIP(hipMemsetAsync(a.counter, 0, 8, s));
@@ -213,8 +213,7 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
         OG_TRY(CheckLaunch("join_probe"));
     }
     unsigned long long m = 0;
-    OG_HIP(hipMemcpyAsync(&m, a.counter, 8, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipStreamSynchronize(s));
+    OG_TRY(CopyToHost(s, {{&m, a.counter, 8}}));
     *num_matches = static_cast<int64_t>(m);
     if (static_cast<int64_t>(m) > cap) {
         return SetError(OMNI_GPU_ERR_OUTPUT_FULL, "join produced %llu matches for an output capacity of %lld", m,
@@ -305,7 +304,7 @@ int omni_gpu_join_build_finish(void *handle)
         long long mm[2];
     } init = {-1, 0, 0, 0, {std::numeric_limits<long long>::max(), std::numeric_limits<long long>::min()}};
     static_assert(sizeof(init) == 32, "scratch layout");
-    OG_HIP(hipMemcpyAsync(sc, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    OG_TRY(CopyToDevice(sc, &init, sizeof(init), s));
     h.finished = true;
     if (h.n == 0) {
         h.max_dup = 0;
@@ -337,7 +336,7 @@ int omni_gpu_join_build_finish(void *handle)
     }
     OG_CHECK(blocks < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "build side too large for one gather launch");
     OG_TRY(h.build_desc.Reserve(sizeof(BuildBatch) * h.pending.size(), s));
-    OG_HIP(hipMemcpyAsync(h.build_desc.p, h.pending.data(), sizeof(BuildBatch) * h.pending.size(), hipMemcpyHostToDevice, s));
+    OG_TRY(CopyToDevice(h.build_desc.p, h.pending.data(), sizeof(BuildBatch) * h.pending.size(), s));
     g.batches = h.build_desc.as<BuildBatch>();
     long long *dmm = reinterpret_cast<long long *>(sc + 16);
     g.mm = h.spec.strategy == 0 ? dmm : nullptr;
@@ -351,8 +350,7 @@ int omni_gpu_join_build_finish(void *handle)
     bool arrayOk = false;
     if (h.spec.strategy == 0) {
         long long mm[2];
-        OG_HIP(hipMemcpyAsync(mm, dmm, sizeof(mm), hipMemcpyDeviceToHost, s));
-        OG_HIP(hipStreamSynchronize(s));
+        OG_TRY(CopyToHost(s, {{mm, dmm, sizeof(mm)}}));
         if (mm[1] >= mm[0]) {
             double degree = std::ceil(std::log2(static_cast<double>(h.n) / 0.75));
             long double len = std::pow(2.0L, static_cast<long double>(degree));
@@ -401,8 +399,7 @@ int omni_gpu_join_build_finish(void *handle)
     }
     // the probe kernel variant depends on whether any key repeats
     unsigned int dup = 0;
-    OG_HIP(hipMemcpyAsync(&dup, has_dup, 4, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipStreamSynchronize(s));
+    OG_TRY(CopyToHost(s, {{&dup, has_dup, 4}}));
     h.has_dup = dup != 0;
     if (h.has_dup) h.inline_pay = false;
     h.max_dup = h.has_dup ? -1 : 1;  // longest chain: measured on first request
@@ -435,8 +432,7 @@ static int MaxChain(JoinHandle &h)
         OG_TRY(CheckLaunch("join_max_chain"));
     }
     unsigned int len = 0;
-    OG_HIP(hipMemcpyAsync(&len, max_len, 4, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipStreamSynchronize(s));
+    OG_TRY(CopyToHost(s, {{&len, max_len, 4}}));
     h.max_dup = len;
     return OMNI_GPU_OK;
 }

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <string>
 
 #include "../../include/omni_gpu.h"
@@ -54,8 +55,9 @@ int HipFail(hipError_t e, const char *what, const char *file, int line);
 int CheckLaunch(const char *kernel);
 
 // ---- device memory ---------------------------------------------------------------------------------------
-// Stream-ordered allocations from the device's default memory pool (release threshold raised at init, so
-// freed blocks stay cached in the pool and re-allocation in a steady state costs no driver call).
+// Stream-ordered device blocks.  Allocation rounds up to a size class and first reuses a cached block (one
+// freed earlier on the same stream, or on any stream that has since been synchronized); only a miss reaches
+// hipMallocAsync.  DevFree keeps the block for reuse instead of returning it to the runtime.
 int DevAlloc(void **p, size_t bytes, hipStream_t s);
 void DevFree(void *p, hipStream_t s);
 
@@ -64,6 +66,18 @@ void DevFree(void *p, hipStream_t s);
 // own work on small inputs.  ReleaseStream synchronizes the stream before returning it to the pool.
 int AcquireStream(hipStream_t *s);
 void ReleaseStream(hipStream_t s);
+
+// Small transfers through the stream's pinned staging block.  CopyToDevice is stream-ordered and the host
+// source may be reused at once; CopyToHost enqueues every read, synchronizes the stream once and then fills
+// the destinations.  SyncStream is hipStreamSynchronize that also recycles the staging space.
+struct HostRead {
+    void *dst;
+    const void *src;
+    size_t n;
+};
+int CopyToDevice(void *dst, const void *src, size_t n, hipStream_t s);
+int CopyToHost(hipStream_t s, std::initializer_list<HostRead> reads);
+int SyncStream(hipStream_t s);
 
 struct DevBuf {
     void *p = nullptr;
