@@ -110,6 +110,9 @@ def bench_join(args):
                 "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
     extra = {"build_kernel_ms": round(build_ms / max(1, args.steps), 4), "matches": m, "probe_group": args.probe_group,
              "probe_partitions": parts[0]}
+    if parts[0] > 1:
+        extra["probe_stage_ms"] = {k: round(capi.profile_get("join_probe_" + k)[1] / args.steps, 4)
+                                   for k in ("hist", "scan", "scatter", "window", "part")}
     cpu = cpu_baseline_join(w, args) if not args.no_cpu else None
     cfg = {"workload": "inner hash join: build 2M unique int64 keys + int32 payload, probe 64M int64 keys "
                        "(80% hit), VectorBatches of 65536" if (nb, npr) == (2_000_000, 64_000_000) else

@@ -58,21 +58,25 @@ struct JoinHandle {
 // Automatic mode partitions only tables larger than the 256 MiB Infinity Cache (regions of about 2 MiB, one
 // L2-resident region per XCD at a time): measured on a 64 MiB table (2M keys, 64M probe rows) the direct probe
 // (1.56 ms) still beats the partition chain (1.65 ms: histogram 0.11, scatter 0.51, probe 1.08 ms).
-int ChoosePartitions(const JoinHandle &h, int64_t rows, int64_t nwg)
+// Partition count of the radix-partitioned probe, 1 for the direct probe.  Automatic mode partitions tables
+// above kAutoPartBytes when the probe side has at least half as many rows as the table has slots (so the
+// partitioning pays for itself), with regions of kWinSlots slots for the LDS window probe; tables too big for
+// kPartMax such regions use the L2 probe with kPartMax partitions.
+constexpr uint64_t kAutoPartBytes = 8ull << 20;
+
+int ChoosePartitions(const JoinHandle &h, int64_t rows)
 {
     if (h.is_array || h.partitions == 0 || h.slots_n == 0) return 1;
-    const uint64_t tbytes = h.slots_n * sizeof(JoinSlot);
+    if (rows >= (1LL << 32) - 1) return 1;  // u32 entry indices
     int P;
     if (h.partitions > 0) {
         P = h.partitions;
     } else {
-        if (tbytes <= (256ull << 20) || rows < (4 << 20)) return 1;
+        if (h.slots_n * sizeof(JoinSlot) < kAutoPartBytes || static_cast<uint64_t>(rows) * 2 < h.slots_n) return 1;
         P = 8;
-        while (P < kPartMax && tbytes / static_cast<uint64_t>(P) > (2u << 20)) P <<= 1;
+        while (P < kPartMax && h.slots_n / static_cast<uint64_t>(P) > static_cast<uint64_t>(kWinSlots)) P <<= 1;
     }
-    if (static_cast<uint64_t>(P) > h.slots_n) return 1;
-    if (rows >= (1LL << 32) - 1) return 1;                                   // u32 entry indices
-    if (static_cast<int64_t>(P) * nwg > static_cast<int64_t>(kScanTile) * kScanTile) return 1;  // two-level scan
+    if (static_cast<uint64_t>(P) * 64 > h.slots_n) return 1;  // regions of at least 64 slots
     return P;
 }
 
@@ -85,18 +89,20 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     while ((1ULL << lgS) < h.slots_n) ++lgS;
     while ((1 << lgP) < P) ++lgP;
     pa.shift = lgS - lgP;
-    pa.nwg = nwg;
-    const int64_t L = static_cast<int64_t>(P) * nwg;
+    pa.xcd_cols = (nwg + kXcds - 1) / kXcds;
+    pa.ncol = pa.xcd_cols * kXcds;
+    const int64_t L = static_cast<int64_t>(P) * pa.ncol;
+    OG_CHECK(L < 0x7FFFFFFFLL * kScanTile, OMNI_GPU_ERR_INVALID, "partition count matrix too large");
     const int ntile = static_cast<int>((L + kScanTile - 1) / kScanTile);
     const int per = P / kXcds;
     OG_TRY(h.part_keys.Reserve(sizeof(long long) * rows, s));
     OG_TRY(h.part_rows.Reserve(sizeof(unsigned int) * rows, s));
     OG_TRY(h.part_counts.Reserve(sizeof(unsigned int) * L, s));
-    const size_t metaWords = kScanTile + 1 + (P + 1) + kXcds * (per + 1) + kXcds;
+    const size_t metaWords = static_cast<size_t>(ntile) + 1 + (P + 1) + kXcds * (per + 1) + kXcds;
     OG_TRY(h.part_meta.Reserve(sizeof(unsigned int) * metaWords, s));
     unsigned int *meta = h.part_meta.as<unsigned int>();
     unsigned int *sums = meta;
-    pa.total = meta + kScanTile;
+    pa.total = meta + ntile;
     pa.part_start = pa.total + 1;
     pa.xcd_prefix = pa.part_start + P + 1;
     pa.xcd_next = pa.xcd_prefix + kXcds * (per + 1);
@@ -104,6 +110,11 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     pa.pkeys = h.part_keys.as<long long>();
     pa.prows = h.part_rows.as<unsigned int>();
     const unsigned int grid = static_cast<unsigned int>(nwg);
+    if (pa.ncol != nwg) {  // columns of the missing workgroups of the last XCD group stay zero
+        KernelScope ks(s, "join_probe_hist");
+        join_fill_i32<<<Blocks(L), kBlock, 0, s>>>(reinterpret_cast<int *>(pa.counts), 0, L);
+        OG_TRY(CheckLaunch("join_fill_i32"));
+    }
     {
         KernelScope ks(s, "join_probe_hist");
         join_part_hist<<<grid, kBlock, 0, s>>>(a, t, pa);
@@ -114,16 +125,41 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
         scan_tiles<<<ntile, kScanBlock, 0, s>>>(pa.counts, L, sums);
         scan_sums<<<1, kScanBlock, 0, s>>>(sums, ntile, pa.total);
         scan_add<<<ntile, kScanBlock, 0, s>>>(pa.counts, L, sums);
-        join_part_plan<<<1, 64, 0, s>>>(pa);
+        join_part_plan<<<1, kBlock, 0, s>>>(pa);
         OG_TRY(CheckLaunch("join_part_scan"));
     }
     {
         KernelScope ks(s, "join_probe_scatter");
-        join_part_scatter<<<grid, kBlock, 0, s>>>(a, t, pa);
+        const size_t lds = ScatterLdsBytes(P);
+        OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_part_scatter),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+        join_part_scatter<<<grid, kScatterBlock, lds, s>>>(a, t, pa);
         OG_TRY(CheckLaunch("join_part_scatter"));
     }
     int cus = 0;
     OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
+    const uint64_t region = h.slots_n >> lgP;
+    if (region <= static_cast<uint64_t>(kWinSlots)) {
+        // window probe: about 4 workgroups per CU in all, each partition split evenly
+        const int splits = std::max(1, std::min(64, (cus * 4 + P - 1) / P));
+        const size_t lds = kWinLdsBytes;
+        const unsigned int wgrid = static_cast<unsigned int>(P) * static_cast<unsigned int>(splits);
+        KernelScope ks(s, "join_probe_window");
+        if (h.has_dup) {
+            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_probe<false, true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            join_window_probe<false, true><<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits);
+        } else if (h.inline_pay) {
+            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_probe<true, false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            join_window_probe<true, false><<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits);
+        } else {
+            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_probe<false, false>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+            join_window_probe<false, false><<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits);
+        }
+        return CheckLaunch("join_window_probe");
+    }
     const unsigned int pgrid = static_cast<unsigned int>(std::max(kXcds, (cus * 4 / kXcds) * kXcds));
     KernelScope ks(s, "join_probe_part");
     if (h.has_dup) join_part_probe<false, true><<<pgrid, kBlock, 0, s>>>(a, t, pa);
@@ -147,7 +183,8 @@ void LaunchProbe(const JoinHandle &h, dim3 grid, hipStream_t s, const ProbeArgs 
     }
 }
 
-int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blocks, long long *out_rows,
+int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blocks, int64_t part_blocks,
+             long long *out_rows,
              omni_gpu_out_column *outs, int64_t cap, int64_t *num_matches)
 {
     OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "probe before the build side finished");
@@ -196,10 +233,10 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
     t.sentinel_pay = reinterpret_cast<const unsigned int *>(h.scratch.as<uint8_t>() + 12);
     int64_t total_rows = 0;
     for (int i = 0; i < nb; ++i) total_rows += batches[i].rows;
-    const int P = (total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows, total_blocks) : 1;
+    const int P = (total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows) : 1;
     h.last_partitions = P;
     if (P > 1) {
-        OG_TRY(LaunchPartitioned(h, a, t, P, total_blocks, total_rows));
+        OG_TRY(LaunchPartitioned(h, a, t, P, part_blocks, total_rows));
     } else if (total_blocks > 0 && h.n > 0) {
         KernelScope ks(s, "join_probe");
         dim3 grid(static_cast<unsigned int>(total_blocks));
@@ -223,7 +260,7 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
 }
 
 int FillBatch(ProbeBatch &b, const omni_gpu_column &c, int64_t rows, int64_t row_base, int64_t block_start,
-              int32_t key_type)
+              int64_t pblock_start, int32_t key_type)
 {
     OG_CHECK(c.type_id == key_type, OMNI_GPU_ERR_INVALID, "probe key type %d != build key type %d", c.type_id, key_type);
     OG_CHECK(rows == 0 || c.values, OMNI_GPU_ERR_INVALID, "probe key has no values");
@@ -233,6 +270,7 @@ int FillBatch(ProbeBatch &b, const omni_gpu_column &c, int64_t rows, int64_t row
     b.rows = rows;
     b.row_base = row_base;
     b.block_start = block_start;
+    b.pblock_start = pblock_start;
     b.width = TypeWidth(c.type_id);
     return OMNI_GPU_OK;
 }
@@ -478,9 +516,9 @@ int omni_gpu_join_probe(void *handle, const omni_gpu_column *probe_key, int64_t 
     OG_CHECK(handle && probe_key, OMNI_GPU_ERR_INVALID, "null argument");
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     ProbeBatch b{};
-    OG_TRY(FillBatch(b, *probe_key, rows, probe_row_base, 0, h.spec.key_type));
-    return RunProbe(h, &b, 1, (rows + kProbeTile - 1) / kProbeTile, reinterpret_cast<long long *>(out_probe_rows), out_payloads, out_capacity,
-                    num_matches);
+    OG_TRY(FillBatch(b, *probe_key, rows, probe_row_base, 0, 0, h.spec.key_type));
+    return RunProbe(h, &b, 1, (rows + kProbeTile - 1) / kProbeTile, (rows + kPartTile - 1) / kPartTile,
+                    reinterpret_cast<long long *>(out_probe_rows), out_payloads, out_capacity, num_matches);
 }
 
 int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const int64_t *rows, int32_t num_batches,
@@ -490,15 +528,16 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
     OG_CHECK(handle && keys && rows && num_batches > 0, OMNI_GPU_ERR_INVALID, "null argument or no batches");
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     std::vector<ProbeBatch> bs(num_batches);
-    int64_t blocks = 0, base = 0;
+    int64_t blocks = 0, pblocks = 0, base = 0;
     for (int i = 0; i < num_batches; ++i) {
-        OG_TRY(FillBatch(bs[i], keys[i], rows[i], base, blocks, h.spec.key_type));
+        OG_TRY(FillBatch(bs[i], keys[i], rows[i], base, blocks, pblocks, h.spec.key_type));
         blocks += (rows[i] + kProbeTile - 1) / kProbeTile;
+        pblocks += (rows[i] + kPartTile - 1) / kPartTile;
         base += rows[i];
     }
     OG_CHECK(blocks < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many probe rows for one launch");
-    return RunProbe(h, bs.data(), num_batches, blocks, reinterpret_cast<long long *>(out_probe_rows), out_payloads,
-                    out_capacity, num_matches);
+    return RunProbe(h, bs.data(), num_batches, blocks, pblocks, reinterpret_cast<long long *>(out_probe_rows),
+                    out_payloads, out_capacity, num_matches);
 }
 
 int omni_gpu_join_destroy(void *handle)

@@ -26,7 +26,8 @@ struct ProbeBatch {
     const uint8_t *nulls;
     int64_t rows;
     int64_t row_base;
-    int64_t block_start;
+    int64_t block_start;   // first workgroup of this batch in a launch tiled by kProbeTile rows
+    int64_t pblock_start;  // first workgroup of this batch in a launch tiled by kPartTile rows
     int width;
     int pad;
 };
@@ -252,7 +253,7 @@ constexpr int kProbeTile = kBlock * kProbeRows;    // probe rows per workgroup
 
 // Keys of probe rows first, first + kBlock, ... (G rows, coalesced across the wave); the key width
 // and the presence of a NULL bitmap are uniform, so each is one branch around the unrolled loads.
-template <int G, typename T>
+template <int G, typename T, int STRIDE = kBlock>
 __device__ __forceinline__ void LoadKeysT(const void *base, int64_t rows, int64_t first, long long *key, bool *live)
 {
     // rows >= 1 for every batch that owns a workgroup: out-of-range lanes re-read the last row (no branch
@@ -261,21 +262,21 @@ __device__ __forceinline__ void LoadKeysT(const void *base, int64_t rows, int64_
         reinterpret_cast<uintptr_t>(base));
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        int64_t row = first + j * kBlock;
+        int64_t row = first + j * STRIDE;
         live[j] = row < rows;
         key[j] = static_cast<long long>(v[live[j] ? row : rows - 1]);
     }
 }
 
-template <int G>
+template <int G, int STRIDE = kBlock>
 __device__ __forceinline__ void LoadProbeKeys(const ProbeBatch &b, int64_t first, long long *key, bool *live)
 {
     switch (b.width) {
-        case 8: LoadKeysT<G, long long>(b.keys, b.rows, first, key, live); break;
-        case 4: LoadKeysT<G, int>(b.keys, b.rows, first, key, live); break;
-        case 2: LoadKeysT<G, short>(b.keys, b.rows, first, key, live); break;
+        case 8: LoadKeysT<G, long long, STRIDE>(b.keys, b.rows, first, key, live); break;
+        case 4: LoadKeysT<G, int, STRIDE>(b.keys, b.rows, first, key, live); break;
+        case 2: LoadKeysT<G, short, STRIDE>(b.keys, b.rows, first, key, live); break;
         default:  // BOOLEAN: any non-zero byte is true (1), as LoadWord reads it on the build side
-            LoadKeysT<G, uint8_t>(b.keys, b.rows, first, key, live);
+            LoadKeysT<G, uint8_t, STRIDE>(b.keys, b.rows, first, key, live);
 #pragma unroll
             for (int j = 0; j < G; ++j) key[j] = key[j] != 0 ? 1 : 0;
             break;
@@ -283,7 +284,7 @@ __device__ __forceinline__ void LoadProbeKeys(const ProbeBatch &b, int64_t first
     if (b.nulls != nullptr) {
 #pragma unroll
         for (int j = 0; j < G; ++j) {
-            int64_t row = first + j * kBlock;
+            int64_t row = first + j * STRIDE;
             if (live[j] && NullBit(b.nulls, row)) live[j] = false;
         }
     }
@@ -381,10 +382,11 @@ __device__ __forceinline__ void WriteMatch(const ProbeArgs &a, long long o, long
 
 // One global atomic per workgroup: every wave passes its match total; returns the wave's first output row.
 // Must be reached by all threads of the workgroup.
+template <int BLOCK = kBlock>
 __device__ __forceinline__ unsigned long long BlockReserve(unsigned long long *counter, unsigned int wave_total,
                                                            unsigned long long *sh)
 {
-    constexpr int kWaves = kBlock / 64;
+    constexpr int kWaves = BLOCK / 64;
     const int wave = threadIdx.x >> 6;
     if (LaneId() == 0) sh[wave] = wave_total;
     __syncthreads();
@@ -408,7 +410,7 @@ __device__ __forceinline__ unsigned long long BlockReserve(unsigned long long *c
 // Writes the matches of one tile: lane row j has first build row head[j] (-1: none), inline payload pay[j] and
 // output probe row id rowid(j).  Reserves the workgroup's output range with one atomic.  All threads of the
 // workgroup must call it.
-template <bool INLINE, bool DUPS, typename RowId>
+template <bool INLINE, bool DUPS, int BLOCK = kBlock, typename RowId>
 __device__ __forceinline__ void EmitTile(const ProbeArgs &a, const TableDev &t, const int *head, const unsigned int *pay,
                                          RowId rowid, unsigned long long *sh)
 {
@@ -420,7 +422,7 @@ __device__ __forceinline__ void EmitTile(const ProbeArgs &a, const TableDev &t, 
             m[j] = __ballot(head[j] >= 0);
             total += __popcll(m[j]);
         }
-        unsigned long long base = BlockReserve(a.counter, total, sh);
+        unsigned long long base = BlockReserve<BLOCK>(a.counter, total, sh);
         const unsigned long long lt = LaneMaskLt();
 #pragma unroll
         for (int j = 0; j < kProbeRows; ++j) {
@@ -445,7 +447,7 @@ __device__ __forceinline__ void EmitTile(const ProbeArgs &a, const TableDev &t, 
         if (lane >= d) incl += up;
     }
     unsigned int waveTotal = __builtin_amdgcn_readlane(incl, 63);
-    unsigned long long o = BlockReserve(a.counter, waveTotal, sh) + (incl - mine);
+    unsigned long long o = BlockReserve<BLOCK>(a.counter, waveTotal, sh) + (incl - mine);
 #pragma unroll
     for (int j = 0; j < kProbeRows; ++j) {
         for (int h = head[j]; h >= 0; h = t.next[h]) WriteMatch<false>(a, static_cast<long long>(o++), rowid(j), h, 0);
@@ -483,24 +485,43 @@ __global__ void __launch_bounds__(kBlock) join_probe(ProbeArgs a, TableDev t)
 // ---- radix-partitioned probe (hash tables larger than the L2 caches) -------------------------------------
 // Partition p of a probe key is the top bits of its home slot (slot >> shift), so partition p only reads
 // table region [p << shift, (p + 1) << shift) (plus the rare probe run past its end).  The probe rows are
-// histogrammed and scattered into partition order, then a persistent probe walks the partitions with each
-// XCD (workgroup id mod 8) owning partitions p = x, x + 8, ... one after another, so the region it reads stays
-// in its own L2 instead of every lookup fetching a 128-byte line from the Infinity Cache or HBM.
+// histogrammed and scattered into partition order (key + row id), then one of two probes runs:
+//  - window probe: the region is staged in LDS by each workgroup serving the partition, and lookups read
+//    LDS (the table is read once per workgroup instead of one 128-byte line per lookup);
+//  - L2 probe (regions too big for LDS): a persistent probe in which XCD x (workgroup id mod 8) walks
+//    partitions x, x + 8, ... in order, so the region it reads stays in its own L2.
 
-constexpr int kPartMax = 1024;
+constexpr int kPartMax = 4096;
 constexpr int kXcds = 8;
+constexpr int kPartTile = 8192;                     // probe rows per workgroup of the histogram / scatter launches
+constexpr int kHistRows = kPartTile / kBlock;       // rows per thread of the histogram
+constexpr int kScatterBlock = 1024;                 // threads of the scatter
+constexpr int kScatterRows = kPartTile / kScatterBlock;
+constexpr int kWinSlots = 4096;                     // largest region the window probe stages in LDS
+constexpr int kWinExtra = 256;                      // slots past the region end staged with it
+constexpr int kWinBlock = 512;                      // threads of the window probe
+constexpr int kWinTile = kWinBlock * kProbeRows;    // entries per window-probe iteration
+constexpr int kWinStage = (kWinSlots + kWinExtra + kWinBlock - 1) / kWinBlock;  // staged slots per thread
+constexpr size_t kWinLdsBytes = sizeof(JoinSlot) * kWinStage * kWinBlock;        // window probe dynamic LDS
+
+// Dynamic LDS bytes of the scatter for P partitions.
+inline size_t ScatterLdsBytes(int P)
+{
+    return static_cast<size_t>(kPartTile) * (sizeof(long long) + sizeof(unsigned int)) + 2 * sizeof(unsigned int) * P;
+}
 
 struct PartArgs {
     int nparts;                  // P, a power of two in [8, kPartMax]
     int shift;                   // partition = home slot >> shift
-    long long nwg;               // workgroups of the histogram / scatter launches (same tiling as join_probe)
-    unsigned int *counts;        // [P][nwg] counts, scanned in place into output offsets
+    long long ncol;              // columns of the count matrix: histogram workgroups rounded up to 8
+    long long xcd_cols;          // ncol / 8
+    unsigned int *counts;        // [P][ncol] counts, scanned in place into output offsets
     long long *pkeys;            // probe keys in partition order
     unsigned int *prows;         // their probe row ids
     unsigned int *total;         // rows partitioned (non-NULL probe rows)
     unsigned int *part_start;    // [P + 1] first entry of each partition
-    unsigned int *xcd_prefix;    // [kXcds][P / kXcds + 1] tile prefix over each XCD's partitions
-    unsigned int *xcd_next;      // [kXcds] next tile to take (work counters)
+    unsigned int *xcd_prefix;    // [kXcds][P / kXcds + 1] tile prefix over each XCD's partitions (L2 probe)
+    unsigned int *xcd_next;      // [kXcds] next tile to take (work counters, L2 probe)
 };
 
 __device__ __forceinline__ unsigned int PartOf(const TableDev &t, const PartArgs &pa, long long key)
@@ -508,15 +529,35 @@ __device__ __forceinline__ unsigned int PartOf(const TableDev &t, const PartArgs
     return static_cast<unsigned int>((Mix64(static_cast<unsigned long long>(key)) & t.mask) >> pa.shift);
 }
 
+// Count-matrix column of a histogram / scatter workgroup.  Workgroup w runs on XCD w mod 8, and the columns
+// of one XCD are adjacent, so within a partition the rows written by one XCD's workgroups are contiguous and
+// their partial lines meet in that XCD's L2 before being written back.
+__device__ __forceinline__ long long PartCol(const PartArgs &pa, unsigned int block)
+{
+    return static_cast<long long>(block % kXcds) * pa.xcd_cols + block / kXcds;
+}
+
+// The batch owning workgroup `block` of a launch tiled by kPartTile rows (uniform per workgroup).
+__device__ __forceinline__ ProbeBatch PartBatchOf(const ProbeArgs &a, unsigned int block)
+{
+    if (a.nb == 1) return a.single;
+    int lo = 0, hi = a.nb - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.batches[mid].pblock_start <= block) lo = mid;
+        else hi = mid - 1;
+    }
+    return a.batches[lo];
+}
+
 __global__ void __launch_bounds__(kBlock) join_part_hist(ProbeArgs a, TableDev t, PartArgs pa)
 {
     __shared__ unsigned int hist[kPartMax];
     for (int i = threadIdx.x; i < pa.nparts; i += kBlock) hist[i] = 0;
     __syncthreads();
-    const ProbeBatch b = BatchOf(a, blockIdx.x);
-    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
-#pragma unroll
-    for (int g = 0; g < kProbeRows; g += 8) {
+    const ProbeBatch b = PartBatchOf(a, blockIdx.x);
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.pblock_start) * kPartTile;
+    for (int g = 0; g < kHistRows; g += 8) {
         long long key[8];
         bool live[8];
         LoadProbeKeys<8>(b, tile0 + g * kBlock + threadIdx.x, key, live);
@@ -526,53 +567,124 @@ __global__ void __launch_bounds__(kBlock) join_part_hist(ProbeArgs a, TableDev t
         }
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < pa.nparts; i += kBlock) pa.counts[static_cast<long long>(i) * pa.nwg + blockIdx.x] = hist[i];
+    const long long col = PartCol(pa, blockIdx.x);
+    for (int i = threadIdx.x; i < pa.nparts; i += kBlock) pa.counts[static_cast<long long>(i) * pa.ncol + col] = hist[i];
 }
 
 // Partition starts and, per XCD, the tile prefix over its partitions; resets the XCD work counters.
-__global__ void join_part_plan(PartArgs pa)
+// One workgroup of kBlock threads: the starts are gathered in parallel into LDS, then one thread per XCD
+// walks that XCD's partitions.
+__global__ void __launch_bounds__(kBlock) join_part_plan(PartArgs pa)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ unsigned int st[kPartMax + 1];
     const int P = pa.nparts, per = P / kXcds;
-    for (int p = 0; p < P; ++p) pa.part_start[p] = pa.counts[static_cast<long long>(p) * pa.nwg];
-    pa.part_start[P] = *pa.total;
-    for (int x = 0; x < kXcds; ++x) {
+    for (int p = threadIdx.x; p < P; p += kBlock) {
+        st[p] = pa.counts[static_cast<long long>(p) * pa.ncol];
+        pa.part_start[p] = st[p];
+    }
+    if (threadIdx.x == 0) {
+        st[P] = *pa.total;
+        pa.part_start[P] = st[P];
+    }
+    __syncthreads();
+    if (threadIdx.x < kXcds) {
+        const int x = threadIdx.x;
         unsigned int run = 0;
         for (int i = 0; i < per; ++i) {
             int p = x + i * kXcds;
             pa.xcd_prefix[x * (per + 1) + i] = run;
-            run += (pa.part_start[p + 1] - pa.part_start[p] + kProbeTile - 1) / kProbeTile;
+            run += (st[p + 1] - st[p] + kProbeTile - 1) / kProbeTile;
         }
         pa.xcd_prefix[x * (per + 1) + per] = run;
         pa.xcd_next[x] = 0;
     }
 }
 
-__global__ void __launch_bounds__(kBlock) join_part_scatter(ProbeArgs a, TableDev t, PartArgs pa)
+// LDS-staged scatter of one tile of kPartTile probe rows: the rows are ranked within their partition (LDS
+// histogram), the partition counts are scanned, the rows are placed in partition order in LDS, and then
+// written out in that order, so consecutive lanes store consecutive entries of one partition's run instead of
+// every store of a wave touching a different partition.  Dynamic LDS: ScatterLdsBytes(P).
+__global__ void __launch_bounds__(kScatterBlock) join_part_scatter(ProbeArgs a, TableDev t, PartArgs pa)
 {
-    __shared__ unsigned int base[kPartMax];
-    for (int i = threadIdx.x; i < pa.nparts; i += kBlock) base[i] = pa.counts[static_cast<long long>(i) * pa.nwg + blockIdx.x];
+    OG_DYNAMIC_LDS(unsigned long long, lds);
+    __shared__ unsigned int wsum[kScatterBlock / 64];
+    __shared__ unsigned int ntot;
+    long long *skey = reinterpret_cast<long long *>(lds);
+    unsigned int *srid = reinterpret_cast<unsigned int *>(skey + kPartTile);
+    unsigned int *loff = srid + kPartTile;  // partition counts of the tile, then their exclusive offsets
+    unsigned int *gbase = loff + pa.nparts;  // the tile's first output entry in each partition
+    const int P = pa.nparts;
+    const long long col = PartCol(pa, blockIdx.x);
+    for (int i = threadIdx.x; i < P; i += kScatterBlock) {
+        loff[i] = 0;
+        gbase[i] = pa.counts[static_cast<long long>(i) * pa.ncol + col];
+    }
     __syncthreads();
-    const ProbeBatch b = BatchOf(a, blockIdx.x);
-    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
+    const ProbeBatch b = PartBatchOf(a, blockIdx.x);
+    const int64_t first = (static_cast<int64_t>(blockIdx.x) - b.pblock_start) * kPartTile + threadIdx.x;
+    long long key[kScatterRows];
+    bool live[kScatterRows];
+    LoadProbeKeys<kScatterRows, kScatterBlock>(b, first, key, live);
+    unsigned int part[kScatterRows], rank[kScatterRows];
 #pragma unroll
-    for (int g = 0; g < kProbeRows; g += 8) {
-        long long key[8];
-        bool live[8];
-        const int64_t first = tile0 + g * kBlock + threadIdx.x;
-        LoadProbeKeys<8>(b, first, key, live);
+    for (int j = 0; j < kScatterRows; ++j) {
+        part[j] = live[j] ? PartOf(t, pa, key[j]) : 0u;
+        rank[j] = live[j] ? atomicAdd(&loff[part[j]], 1u) : 0u;
+    }
+    __syncthreads();
+    // exclusive scan of the P counts: thread i owns counts [i * per, (i + 1) * per)
+    const int per = (P + kScatterBlock - 1) / kScatterBlock;
+    const int c0 = threadIdx.x * per;
+    unsigned int mine = 0;
+    for (int i = 0; i < per; ++i) mine += (c0 + i < P) ? loff[c0 + i] : 0u;
+    const int lane = LaneId(), wave = threadIdx.x / 64;
+    unsigned int incl = mine;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            if (live[j]) {
-                unsigned int dst = atomicAdd(&base[PartOf(t, pa, key[j])], 1u);
-                pa.pkeys[dst] = key[j];
-                pa.prows[dst] = static_cast<unsigned int>(b.row_base + first + j * kBlock);
-            }
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned int up = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += up;
+    }
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int run = 0;
+        for (int w = 0; w < kScatterBlock / 64; ++w) {
+            unsigned int v = wsum[w];
+            wsum[w] = run;
+            run += v;
         }
+        ntot = run;
+    }
+    __syncthreads();
+    unsigned int run = wsum[wave] + incl - mine;
+    for (int i = 0; i < per; ++i) {
+        if (c0 + i < P) {
+            unsigned int c = loff[c0 + i];
+            loff[c0 + i] = run;
+            run += c;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kScatterRows; ++j) {
+        if (live[j]) {
+            const unsigned int pos = loff[part[j]] + rank[j];
+            skey[pos] = key[j];
+            srid[pos] = static_cast<unsigned int>(b.row_base + first + j * kScatterBlock);
+        }
+    }
+    __syncthreads();
+    const unsigned int n = ntot;
+    for (unsigned int i = threadIdx.x; i < n; i += kScatterBlock) {
+        const long long k = skey[i];
+        const unsigned int p = PartOf(t, pa, k);
+        const unsigned int dst = gbase[p] + (i - loff[p]);
+        pa.pkeys[dst] = k;
+        pa.prows[dst] = srid[i];
     }
 }
 
-// Persistent probe over the partitioned rows: workgroup w serves XCD w % 8 and takes that XCD's tiles in
+// Persistent L2 probe over the partitioned rows: workgroup w serves XCD w % 8 and takes that XCD's tiles in
 // partition order from its work counter until none is left.
 template <bool INLINE, bool DUPS>
 __global__ void __launch_bounds__(kBlock) join_part_probe(ProbeArgs a, TableDev t, PartArgs pa)
@@ -616,6 +728,98 @@ __global__ void __launch_bounds__(kBlock) join_part_probe(ProbeArgs a, TableDev 
             LookupGroup<false, 8>(t, key, live, head + g, pay + g);
         }
         EmitTile<INLINE, DUPS>(a, t, head, pay, [&](int j) { return rid[j]; }, sh);
+    }
+}
+
+// Continues a probe run that left the LDS window, in the global table from slot `pos`.
+__device__ __forceinline__ int WalkGlobal(const TableDev &t, long long key, unsigned long long pos, unsigned int *pay)
+{
+    while (true) {
+        const JoinSlot sl = t.slots[pos & t.mask];
+        if (sl.key == key) {
+            *pay = sl.pay;
+            return sl.head;
+        }
+        if (static_cast<unsigned long long>(sl.key) == kEmptyKey) return -1;
+        ++pos;
+    }
+}
+
+// Window probe: `splits` workgroups serve each partition (workgroup w: partition w / splits, its share
+// w % splits of the partition's entries).  Each stages the partition's region and the kWinExtra slots after it
+// (wrapping at the table end) in LDS, then resolves its entries against LDS; a probe run that leaves the
+// window continues in the global table.  Dynamic LDS: kWinLdsBytes (shift <= log2 kWinSlots).
+template <bool INLINE, bool DUPS>
+__global__ void __launch_bounds__(kWinBlock) join_window_probe(ProbeArgs a, TableDev t, PartArgs pa, int splits)
+{
+    OG_DYNAMIC_LDS(JoinSlot, win);
+    __shared__ unsigned long long sh[kWinBlock / 64 + 1];
+    const unsigned int p = blockIdx.x / static_cast<unsigned int>(splits);
+    const unsigned int sp = blockIdx.x % static_cast<unsigned int>(splits);
+    const unsigned int W = (1u << pa.shift) + kWinExtra;
+    const unsigned long long wbase = static_cast<unsigned long long>(p) << pa.shift;
+    {
+        // every staging load is issued before the first LDS store; the LDS window holds kWinStage * kWinBlock
+        // slots, so the stores need no bound (slots past W repeat slot W - 1 and are never read)
+        JoinSlot tmp[kWinStage];
+#pragma unroll
+        for (int u = 0; u < kWinStage; ++u) {
+            const unsigned int i = u * kWinBlock + threadIdx.x;
+            tmp[u] = t.slots[(wbase + (i < W ? i : W - 1)) & t.mask];
+        }
+#pragma unroll
+        for (int u = 0; u < kWinStage; ++u) win[u * kWinBlock + threadIdx.x] = tmp[u];
+    }
+    const int sentHead = *t.sentinel_head;
+    const unsigned int sentPay = *t.sentinel_pay;
+    __syncthreads();
+    const unsigned int lo = pa.part_start[p], n = pa.part_start[p + 1] - lo;
+    const unsigned int s0 = lo + static_cast<unsigned int>(static_cast<unsigned long long>(n) * sp / splits);
+    const unsigned int s1 = lo + static_cast<unsigned int>(static_cast<unsigned long long>(n) * (sp + 1) / splits);
+    for (unsigned int e0 = s0; e0 < s1; e0 += kWinTile) {  // bounds uniform across the workgroup
+        long long key[kProbeRows];
+        unsigned int rid[kProbeRows];
+        bool live[kProbeRows];
+#pragma unroll
+        for (int j = 0; j < kProbeRows; ++j) {
+            const unsigned int e = e0 + j * kWinBlock + threadIdx.x;
+            live[j] = e < s1;
+            const unsigned int ec = live[j] ? e : s1 - 1;  // s1 > e0: the share holds this entry or an earlier one
+            key[j] = pa.pkeys[ec];
+            rid[j] = pa.prows[ec];
+        }
+        int head[kProbeRows];
+        unsigned int pay[kProbeRows];
+#pragma unroll
+        for (int j = 0; j < kProbeRows; ++j) {
+            int hd = -1;
+            unsigned int py = 0;
+            if (live[j] && static_cast<unsigned long long>(key[j]) == kEmptyKey) {
+                hd = sentHead;
+                py = sentPay;
+            } else if (live[j]) {
+                // home - wbase lies in [0, 1 << shift) for every entry of this partition
+                unsigned int pos = static_cast<unsigned int>(
+                    ((Mix64(static_cast<unsigned long long>(key[j])) & t.mask) - wbase) & t.mask);
+                bool open = true;
+                while (open && pos < W) {
+                    const JoinSlot sl = win[pos];
+                    if (sl.key == key[j]) {
+                        hd = sl.head;
+                        py = sl.pay;
+                        open = false;
+                    } else if (static_cast<unsigned long long>(sl.key) == kEmptyKey) {
+                        open = false;
+                    } else {
+                        ++pos;
+                    }
+                }
+                if (open) hd = WalkGlobal(t, key[j], wbase + pos, &py);
+            }
+            head[j] = hd;
+            pay[j] = py;
+        }
+        EmitTile<INLINE, DUPS, kWinBlock>(a, t, head, pay, [&](int j) { return static_cast<long long>(rid[j]); }, sh);
     }
 }
 

@@ -140,13 +140,15 @@ def test_probe_width_rejects_other_values():
     j.close()
 
 
-@pytest.mark.parametrize("partitions", [8, 64, 1024])
+@pytest.mark.parametrize("partitions", [8, 16, 64, 1024])
 @pytest.mark.parametrize("unique", [False, True])
-def test_partitioned_probe_parity(partitions, unique):
-    """Radix-partitioned probe (histogram, scan, scatter, per-XCD persistent probe) against the oracle:
-    duplicate-key chains and inline payloads, NULL probe keys, the slot sentinel as a real key on both sides,
-    ragged batches."""
-    rng = np.random.default_rng(11 + partitions + unique)
+@pytest.mark.parametrize("wide", [False, True])
+def test_partitioned_probe_parity(partitions, unique, wide):
+    """Radix-partitioned probe against the oracle.  The table has 65,536 slots: 8 partitions (regions of 8,192
+    slots) take the per-XCD persistent L2 probe, 16 / 64 / 1024 (regions of 4,096 / 1,024 / 64 slots) the LDS
+    window probe.  Duplicate-key chains and inline payloads (one int32) or gathered payloads (int64 + int16 with
+    NULLs), NULL probe keys, the slot sentinel as a real key on both sides, ragged batches."""
+    rng = np.random.default_rng(11 + partitions + unique + 5 * wide)
     nb, npr = 30_000, 250_001
     if unique:
         bkv = rng.permutation(np.arange(-2**50, -2**50 + 5 * nb, 5, dtype=np.int64))[:nb]
@@ -158,11 +160,15 @@ def test_partitioned_probe_parity(partitions, unique):
     pkv[[0, 99, npr - 1]] = -2**63
     bk = Col(LONG, bkv)
     pk = Col(LONG, pkv, rng.random(npr) < 0.02)
-    pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32))]
+    if wide:
+        pays = [Col(LONG, rng.integers(-2**63, 2**63 - 1, nb, dtype=np.int64)),
+                Col(SHORT, rng.integers(-2**15, 2**15, nb).astype(np.int16), rng.random(nb) < 0.1)]
+    else:
+        pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32))]
     rows, gp, info = gpu_join(bk, pays, pk, 8192, 65536, strategy=1, batched=True, partitions=partitions)
     want, _ = oracle_set(bk, pays, pk, 8192, 65536)
     assert as_set(rows, gp) == want
-    assert not info["is_array"]
+    assert not info["is_array"] and info["table_slots"] == 65536
 
 
 def test_sentinel_key_inline_payload():
@@ -230,13 +236,15 @@ def test_probe_before_finish_is_an_error():
 
 
 @pytest.mark.parametrize("dense", [False, True])
-def test_inner_join_config_full_size(dense):
+@pytest.mark.parametrize("partitions", [-1, 0])
+def test_inner_join_config_full_size(dense, partitions):
     """Build 2M unique int64 keys + int32 payload, probe 64M keys (80% hits) in batches of 65,536, all
     batches in one launch.  Size-independent checks: exact match count, every hit probe row exactly once,
     every payload equal to its build row's payload; plus bit-exact parity with the oracle on a 1M-row slice."""
     w = workloads.join_inner(dense=dense)
     bk = Col(LONG, w.build_keys)
     j = capi.HashJoin(LONG, [INT])
+    j.tune(1, partitions)
     keep = []
     for a, b in workloads.batches(len(w.build_keys), w.batch_rows):
         kd, pd = capi.DeviceColumn(LONG, w.build_keys[a:b]), capi.DeviceColumn(INT, w.build_payload[a:b])
@@ -252,6 +260,8 @@ def test_inner_join_config_full_size(dense):
     out_rows, pays, oarr = j._outs(cap, False)
     m = j.probe_batches_raw(cols, [b - a for a, b in spans], out_rows, oarr, cap)
     assert m == w.hits
+    # automatic mode partitions the 64 MiB table into 1024 regions of 4096 slots (LDS window probe)
+    assert j.last_partitions() == (1 if dense or partitions == 0 else 1024)
     rows = out_rows.numpy(m)
     pay = pays[0][0].numpy(m)
     order = np.argsort(w.build_keys, kind="stable")
