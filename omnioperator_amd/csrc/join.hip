@@ -89,6 +89,7 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     while ((1ULL << lgS) < h.slots_n) ++lgS;
     while ((1 << lgP) < P) ++lgP;
     pa.shift = lgS - lgP;
+    pa.ntiles = nwg;
     pa.xcd_cols = (nwg + kXcds - 1) / kXcds;
     pa.ncol = pa.xcd_cols * kXcds;
     const int64_t L = static_cast<int64_t>(P) * pa.ncol;
@@ -109,15 +110,11 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     pa.counts = h.part_counts.as<unsigned int>();
     pa.pkeys = h.part_keys.as<long long>();
     pa.prows = h.part_rows.as<unsigned int>();
-    const unsigned int grid = static_cast<unsigned int>(nwg);
-    if (pa.ncol != nwg) {  // columns of the missing workgroups of the last XCD group stay zero
-        KernelScope ks(s, "join_probe_hist");
-        join_fill_i32<<<Blocks(L), kBlock, 0, s>>>(reinterpret_cast<int *>(pa.counts), 0, L);
-        OG_TRY(CheckLaunch("join_fill_i32"));
-    }
+    int cus = 0;
+    OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
     {
         KernelScope ks(s, "join_probe_hist");
-        join_part_hist<<<grid, kBlock, 0, s>>>(a, t, pa);
+        join_part_hist<<<static_cast<unsigned int>(pa.ncol), kBlock, 0, s>>>(a, t, pa);
         OG_TRY(CheckLaunch("join_part_hist"));
     }
     {
@@ -133,11 +130,9 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
         const size_t lds = ScatterLdsBytes(P);
         OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_part_scatter),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-        join_part_scatter<<<grid, kScatterBlock, lds, s>>>(a, t, pa);
+        join_part_scatter<<<static_cast<unsigned int>(nwg), kScatterBlock, lds, s>>>(a, t, pa);
         OG_TRY(CheckLaunch("join_part_scatter"));
     }
-    int cus = 0;
-    OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
     const uint64_t region = h.slots_n >> lgP;
     if (region <= static_cast<uint64_t>(kWinSlots)) {
         // window probe: about 4 workgroups per CU in all, each partition split evenly
@@ -416,8 +411,10 @@ int omni_gpu_join_build_finish(void *handle)
                                                         h.heads.as<int>(), h.min_key, h.next.as<int>(), has_dup);
         OG_TRY(CheckLaunch("join_build_array"));
     } else {
+        // load <= 1/4: linear-probing runs stay short (the window probe's LDS rounds and the direct probe's
+        // line crossings both scale with the longest runs a wave meets)
         h.slots_n = 1024;
-        while (h.slots_n < static_cast<uint64_t>(2 * h.n)) h.slots_n <<= 1;
+        while (h.slots_n < static_cast<uint64_t>(4 * h.n)) h.slots_n <<= 1;
         OG_TRY(h.slots.Reserve(sizeof(JoinSlot) * h.slots_n, s));
         {
             KernelScope ks(s, "join_init_table");

@@ -410,22 +410,22 @@ __device__ __forceinline__ unsigned long long BlockReserve(unsigned long long *c
 // Writes the matches of one tile: lane row j has first build row head[j] (-1: none), inline payload pay[j] and
 // output probe row id rowid(j).  Reserves the workgroup's output range with one atomic.  All threads of the
 // workgroup must call it.
-template <bool INLINE, bool DUPS, int BLOCK = kBlock, typename RowId>
+template <bool INLINE, bool DUPS, int BLOCK = kBlock, int ROWS = kProbeRows, typename RowId>
 __device__ __forceinline__ void EmitTile(const ProbeArgs &a, const TableDev &t, const int *head, const unsigned int *pay,
                                          RowId rowid, unsigned long long *sh)
 {
     if (!DUPS) {
-        unsigned long long m[kProbeRows];
+        unsigned long long m[ROWS];
         unsigned int total = 0;
 #pragma unroll
-        for (int j = 0; j < kProbeRows; ++j) {
+        for (int j = 0; j < ROWS; ++j) {
             m[j] = __ballot(head[j] >= 0);
             total += __popcll(m[j]);
         }
         unsigned long long base = BlockReserve<BLOCK>(a.counter, total, sh);
         const unsigned long long lt = LaneMaskLt();
 #pragma unroll
-        for (int j = 0; j < kProbeRows; ++j) {
+        for (int j = 0; j < ROWS; ++j) {
             if (head[j] >= 0) {
                 WriteMatch<INLINE>(a, static_cast<long long>(base + __popcll(m[j] & lt)), rowid(j), head[j], pay[j]);
             }
@@ -436,7 +436,7 @@ __device__ __forceinline__ void EmitTile(const ProbeArgs &a, const TableDev &t, 
     // duplicate keys: per-row match counts, wave-level exclusive scan of each lane's total
     unsigned int mine = 0;
 #pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
+    for (int j = 0; j < ROWS; ++j) {
         for (int h = head[j]; h >= 0; h = t.next[h]) ++mine;
     }
     const int lane = LaneId();
@@ -449,7 +449,7 @@ __device__ __forceinline__ void EmitTile(const ProbeArgs &a, const TableDev &t, 
     unsigned int waveTotal = __builtin_amdgcn_readlane(incl, 63);
     unsigned long long o = BlockReserve<BLOCK>(a.counter, waveTotal, sh) + (incl - mine);
 #pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
+    for (int j = 0; j < ROWS; ++j) {
         for (int h = head[j]; h >= 0; h = t.next[h]) WriteMatch<false>(a, static_cast<long long>(o++), rowid(j), h, 0);
     }
 }
@@ -497,9 +497,9 @@ constexpr int kPartTile = 8192;                     // probe rows per workgroup 
 constexpr int kHistRows = kPartTile / kBlock;       // rows per thread of the histogram
 constexpr int kScatterBlock = 1024;                 // threads of the scatter
 constexpr int kScatterRows = kPartTile / kScatterBlock;
-constexpr int kWinSlots = 4096;                     // largest region the window probe stages in LDS
+constexpr int kWinSlots = 8192;                     // largest region the window probe stages in LDS
 constexpr int kWinExtra = 256;                      // slots past the region end staged with it
-constexpr int kWinBlock = 512;                      // threads of the window probe
+constexpr int kWinBlock = 1024;                     // threads of the window probe
 constexpr int kWinTile = kWinBlock * kProbeRows;    // entries per window-probe iteration
 constexpr int kWinStage = (kWinSlots + kWinExtra + kWinBlock - 1) / kWinBlock;  // staged slots per thread
 constexpr size_t kWinLdsBytes = sizeof(JoinSlot) * kWinStage * kWinBlock;        // window probe dynamic LDS
@@ -513,7 +513,8 @@ inline size_t ScatterLdsBytes(int P)
 struct PartArgs {
     int nparts;                  // P, a power of two in [8, kPartMax]
     int shift;                   // partition = home slot >> shift
-    long long ncol;              // columns of the count matrix: histogram workgroups rounded up to 8
+    long long ntiles;            // tiles of kPartTile probe rows (histogram workgroups with rows)
+    long long ncol;              // columns of the count matrix: ntiles rounded up to 8
     long long xcd_cols;          // ncol / 8
     unsigned int *counts;        // [P][ncol] counts, scanned in place into output offsets
     long long *pkeys;            // probe keys in partition order
@@ -550,9 +551,15 @@ __device__ __forceinline__ ProbeBatch PartBatchOf(const ProbeArgs &a, unsigned i
     return a.batches[lo];
 }
 
+// Launched with ncol workgroups: those past ntiles only write the zero counts of their padding column.
 __global__ void __launch_bounds__(kBlock) join_part_hist(ProbeArgs a, TableDev t, PartArgs pa)
 {
     __shared__ unsigned int hist[kPartMax];
+    if (blockIdx.x >= pa.ntiles) {
+        const long long col = PartCol(pa, blockIdx.x);
+        for (int i = threadIdx.x; i < pa.nparts; i += kBlock) pa.counts[static_cast<long long>(i) * pa.ncol + col] = 0;
+        return;
+    }
     for (int i = threadIdx.x; i < pa.nparts; i += kBlock) hist[i] = 0;
     __syncthreads();
     const ProbeBatch b = PartBatchOf(a, blockIdx.x);

@@ -144,8 +144,8 @@ def test_probe_width_rejects_other_values():
 @pytest.mark.parametrize("unique", [False, True])
 @pytest.mark.parametrize("wide", [False, True])
 def test_partitioned_probe_parity(partitions, unique, wide):
-    """Radix-partitioned probe against the oracle.  The table has 65,536 slots: 8 partitions (regions of 8,192
-    slots) take the per-XCD persistent L2 probe, 16 / 64 / 1024 (regions of 4,096 / 1,024 / 64 slots) the LDS
+    """Radix-partitioned probe against the oracle.  The table has 131,072 slots: 8 partitions (regions of 16,384
+    slots) take the per-XCD persistent L2 probe, 16 / 64 / 1024 (regions of 8,192 / 2,048 / 128 slots) the LDS
     window probe.  Duplicate-key chains and inline payloads (one int32) or gathered payloads (int64 + int16 with
     NULLs), NULL probe keys, the slot sentinel as a real key on both sides, ragged batches."""
     rng = np.random.default_rng(11 + partitions + unique + 5 * wide)
@@ -168,7 +168,7 @@ def test_partitioned_probe_parity(partitions, unique, wide):
     rows, gp, info = gpu_join(bk, pays, pk, 8192, 65536, strategy=1, batched=True, partitions=partitions)
     want, _ = oracle_set(bk, pays, pk, 8192, 65536)
     assert as_set(rows, gp) == want
-    assert not info["is_array"] and info["table_slots"] == 65536
+    assert not info["is_array"] and info["table_slots"] == 131072
 
 
 def test_sentinel_key_inline_payload():
@@ -260,7 +260,7 @@ def test_inner_join_config_full_size(dense, partitions):
     out_rows, pays, oarr = j._outs(cap, False)
     m = j.probe_batches_raw(cols, [b - a for a, b in spans], out_rows, oarr, cap)
     assert m == w.hits
-    # automatic mode partitions the 64 MiB table into 1024 regions of 4096 slots (LDS window probe)
+    # automatic mode partitions the 128 MiB table into 1024 regions of 8192 slots (LDS window probe)
     assert j.last_partitions() == (1 if dense or partitions == 0 else 1024)
     rows = out_rows.numpy(m)
     pay = pays[0][0].numpy(m)
