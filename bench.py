@@ -162,17 +162,24 @@ def bench_agg_plain(args):
     spans = workloads.batches(n, w.batch_rows)
     cols = [([kd.c(a)], [vd.c(a), None, vd.c(a), vd.c(a)], b - a) for a, b in spans]
 
+    spec = capi.make_agg_spec([LONG], aggs)
+    per_batch = args.agg_form == "per_batch"
+    arrays = capi.HashAgg.batch_arrays_for(spec, cols)
+
     def step():
-        h = capi.HashAgg(capi.make_agg_spec([LONG], aggs))
-        for kc, ac, m in cols:
-            h.add_batch(kc, ac, m)
+        h = capi.HashAgg(spec)
+        if per_batch:
+            for kc, ac, m in cols:
+                h.add_batch(kc, ac, m)
+        else:  # all VectorBatches in one omni_gpu_agg_add_batches call: one staging launch + one insert
+            h.add_batches_raw(*arrays)
         out = h.output()
         h.close()
         return len(out[0][0])
 
     groups, ms = timed_steps(step, args, capi)
     alg = n * 16 + groups * 40
-    roof = kernel_roofline(capi, "agg_insert", n * 16, len(spans), args)
+    roof = kernel_roofline(capi, "agg_insert", n * 16, len(spans) if per_batch else 1, args)
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -185,6 +192,8 @@ def bench_agg_plain(args):
                "sample": f"the full workload ({n} rows); host CPU {cpu_model()}"}
     cfg = {"workload": "group-by: 1M rows, int64 key over 1,000 values, int64 value, VectorBatches of 4096; "
                        "SUM/COUNT(*)/MIN/MAX", "rows": n, "groups": groups, "batch_rows": w.batch_rows,
+           "form": "one omni_gpu_agg_add_batch per VectorBatch" if per_batch else
+                   "all VectorBatches in one omni_gpu_agg_add_batches call",
            "algorithmic_bytes_per_step": alg}
     return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
             "dtype": "int64"}
@@ -316,6 +325,8 @@ def main():
     p.add_argument("--skewed-rows", type=int, default=200_000_000)
     p.add_argument("--cpu-skewed-rows", type=int, default=4_000_000)
     p.add_argument("--batched-form", default="graph", choices=["graph", "two_launches", "per_batch"])
+    p.add_argument("--agg-form", default="batches", choices=["batches", "per_batch"],
+                   help="plain group-by: all batches in one add_batches call, or one add_batch per batch")
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
     p.add_argument("--build-rows", type=int, default=2_000_000)
     p.add_argument("--probe-rows", type=int, default=64_000_000)

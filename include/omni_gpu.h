@@ -132,6 +132,15 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle);
  * valid until the next synchronizing call on the handle. */
 int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
                            int64_t rows);
+/* Adds num_batches VectorBatches in one call: batch b's key columns are keys[b * num_keys ..], its aggregate
+ * columns aggs[b * num_aggs ..] and its row count rows[b].  The batches are copied into one staging run of
+ * columns by one launch (an aggregate input that is the same column as a key or an earlier aggregate in
+ * every batch is staged once) and aggregated by one insert, instead of one launch per batch -- the
+ * AddInput loop of HashAggregationOperator (group_aggregation.cpp:320-406) over many small batches.
+ * Results are those of adding the batches one by one.  Same asynchrony and column lifetime as
+ * omni_gpu_agg_add_batch. */
+int omni_gpu_agg_add_batches(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
+                             const int64_t *rows, int32_t num_batches);
 /* Blocks until every batch added so far has been consumed (its columns may then be reused). */
 int omni_gpu_agg_wait(void *handle);
 /* Number of groups so far, NULL-key group included (HashAggregationOperator::GetHashMapUniqueKeys,

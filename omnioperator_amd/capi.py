@@ -82,6 +82,7 @@ SIGNATURES = {
     "omni_gpu_profile_get": (_I, [C.c_char_p, _P(_I64), _P(_D)]),
     "omni_gpu_agg_create": (_I, [_P(AggSpec), _P(_VP)]),
     "omni_gpu_agg_add_batch": (_I, [_VP, _P(Column), _P(Column), _I64]),
+    "omni_gpu_agg_add_batches": (_I, [_VP, _P(Column), _P(Column), _P(_I64), _I32]),
     "omni_gpu_agg_wait": (_I, [_VP]),
     "omni_gpu_agg_num_groups": (_I, [_VP, _P(_I64)]),
     "omni_gpu_agg_get_output": (_I, [_VP, _I64, _I64, _P(OutColumn)]),
@@ -258,6 +259,32 @@ class HashAgg:
             if a is not None:
                 aarr[i] = a
         check(lib().omni_gpu_agg_add_batch(self.h, karr, aarr, rows))
+
+    def batch_arrays(self, batches):
+        """ctypes arrays for add_batches_raw from [(keys, aggs, rows)] (aggs entries may be None)."""
+        return HashAgg.batch_arrays_for(self.spec, batches)
+
+    @staticmethod
+    def batch_arrays_for(spec, batches):
+        nk, na = spec.num_keys, spec.num_aggs
+        karr = (Column * max(1, len(batches) * nk))()
+        aarr = (Column * max(1, len(batches) * na))()
+        rarr = (C.c_int64 * max(1, len(batches)))()
+        for b, (keys, aggs, rows) in enumerate(batches):
+            for c, k in enumerate(keys):
+                karr[b * nk + c] = k
+            for i, a in enumerate(aggs):
+                if a is not None:
+                    aarr[b * na + i] = a
+            rarr[b] = rows
+        return karr, aarr, rarr, len(batches)
+
+    def add_batches(self, batches):
+        """Adds [(keys, aggs, rows)] with one omni_gpu_agg_add_batches call."""
+        self.add_batches_raw(*self.batch_arrays(batches))
+
+    def add_batches_raw(self, karr, aarr, rarr, n: int):
+        check(lib().omni_gpu_agg_add_batches(self.h, karr, aarr, rarr, n))
 
     def num_groups(self) -> int:
         g = C.c_int64()

@@ -55,6 +55,7 @@ struct AggHandle {
     int64_t max_groups = 0;
     DevBuf slots, ctrl, skeys, gkeys, gnull, gvalid, states, row_tmpl, cursors;
     DevBuf index, tile_counts;      // output compaction: live group ids in ascending order
+    DevBuf stage_vals[kStageCols], stage_nulls[kStageCols], stage_src, stage_rows;  // omni_gpu_agg_add_batches
     DevBuf counters;  // [0] ids handed out u64, [8] overflow u32, [12] error u32, [16..24) special u32[2],
                       // [24] spills u64, [32] live groups u32
     int64_t known_ids = 0;          // ids handed out at the last synchronisation
@@ -370,6 +371,105 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
     return OMNI_GPU_OK;
 }
 
+int omni_gpu_agg_add_batches(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
+                             const int64_t *rows, int32_t num_batches)
+{
+    OG_CHECK(handle && rows && num_batches >= 0, OMNI_GPU_ERR_INVALID, "null argument");
+    AggHandle &h = *static_cast<AggHandle *>(handle);
+    if (num_batches == 0) return OMNI_GPU_OK;
+    OG_CHECK(keys, OMNI_GPU_ERR_INVALID, "null key columns");
+    OG_CHECK(h.args.n == 0 || aggs, OMNI_GPU_ERR_INVALID, "null aggregate columns");
+    const int nk = h.kd.num, na = h.args.n;
+    if (num_batches == 1) return omni_gpu_agg_add_batch(handle, keys, aggs, rows[0]);
+    std::vector<long long> start(static_cast<size_t>(num_batches) + 1, 0);
+    for (int b = 0; b < num_batches; ++b) {
+        OG_CHECK(rows[b] >= 0, OMNI_GPU_ERR_INVALID, "negative row count in batch %d", b);
+        start[b + 1] = start[b] + rows[b];
+    }
+    const long long total = start[num_batches];
+    if (total == 0) return OMNI_GPU_OK;
+    // staged columns: the keys, then each aggregate input that is not COUNT(*) and is not the same column
+    // (same pointers in every batch) as an earlier one
+    std::vector<const omni_gpu_column *> col0;      // the column in batch 0 (type, pointers)
+    std::vector<std::pair<const omni_gpu_column *, int>> cols;  // (array, stride) per staged column
+    int keyCol[OMNI_GPU_MAX_KEYS], aggCol[OMNI_GPU_MAX_AGGS];
+    for (int c = 0; c < nk; ++c) {
+        OG_CHECK(keys[c].type_id == h.spec.key_types[c], OMNI_GPU_ERR_INVALID, "key %d type %d != spec %d", c,
+                 keys[c].type_id, h.spec.key_types[c]);
+        keyCol[c] = static_cast<int>(cols.size());
+        cols.push_back({keys + c, nk});
+    }
+    for (int i = 0; i < na; ++i) {
+        aggCol[i] = -1;
+        if (h.args.a[i].kind == K_COUNT_ALL) continue;
+        OG_CHECK(aggs[i].type_id == h.spec.agg_in_types[i], OMNI_GPU_ERR_INVALID,
+                 "aggregate %d input type %d != spec %d", i, aggs[i].type_id, h.spec.agg_in_types[i]);
+        for (int j = 0; j < static_cast<int>(cols.size()) && aggCol[i] < 0; ++j) {
+            bool same = TypeWidth(cols[j].first[0].type_id) == TypeWidth(aggs[i].type_id);
+            for (int b = 0; same && b < num_batches; ++b) {
+                const omni_gpu_column &x = cols[j].first[static_cast<size_t>(b) * cols[j].second];
+                const omni_gpu_column &y = aggs[static_cast<size_t>(b) * na + i];
+                same = x.values == y.values && x.nulls == y.nulls;
+            }
+            if (same) aggCol[i] = j;
+        }
+        if (aggCol[i] < 0) {
+            aggCol[i] = static_cast<int>(cols.size());
+            cols.push_back({aggs + i, na});
+        }
+    }
+    const int nc = static_cast<int>(cols.size());
+    hipStream_t s = h.stream;
+    StageArgs a{};
+    a.nb = num_batches;
+    a.ncols = nc;
+    a.rows = total;
+    std::vector<StageSrc> src(static_cast<size_t>(num_batches) * nc);
+    for (int c = 0; c < nc; ++c) {
+        const int w = TypeWidth(cols[c].first[0].type_id);
+        bool anyNull = false;
+        for (int b = 0; b < num_batches; ++b) {
+            const omni_gpu_column &x = cols[c].first[static_cast<size_t>(b) * cols[c].second];
+            OG_CHECK(rows[b] == 0 || x.values, OMNI_GPU_ERR_INVALID, "batch %d column %d has no values", b, c);
+            src[static_cast<size_t>(b) * nc + c] = {x.values, x.nulls};
+            anyNull = anyNull || (x.nulls != nullptr && rows[b] > 0);
+        }
+        OG_TRY(h.stage_vals[c].Reserve(static_cast<size_t>(w) * total, s));
+        a.width[c] = w;
+        a.dst[c] = h.stage_vals[c].p;
+        if (anyNull) {
+            OG_TRY(h.stage_nulls[c].Reserve(sizeof(unsigned int) * ((total + 31) / 32), s));
+            a.dst_nulls[c] = h.stage_nulls[c].as<unsigned int>();
+        }
+    }
+    OG_TRY(h.stage_src.Reserve(sizeof(StageSrc) * src.size(), s));
+    OG_TRY(h.stage_rows.Reserve(sizeof(long long) * start.size(), s));
+    OG_TRY(CopyToDevice(h.stage_src.p, src.data(), sizeof(StageSrc) * src.size(), s));
+    OG_TRY(CopyToDevice(h.stage_rows.p, start.data(), sizeof(long long) * start.size(), s));
+    a.src = h.stage_src.as<StageSrc>();
+    a.row_start = h.stage_rows.as<long long>();
+    {
+        KernelScope ks(s, "agg_stage");
+        const long long wblocks = ((total + 31) / 32 + kBlock - 1) / kBlock;
+        agg_stage_values<<<dim3(static_cast<unsigned int>(Blocks(total)), nc), kBlock, 0, s>>>(a);
+        agg_stage_nulls<<<dim3(static_cast<unsigned int>(wblocks), nc), kBlock, 0, s>>>(a);
+        OG_TRY(CheckLaunch("agg_stage"));
+    }
+    omni_gpu_column sk[OMNI_GPU_MAX_KEYS] = {}, sa[OMNI_GPU_MAX_AGGS] = {};
+    for (int c = 0; c < nk; ++c) {
+        sk[c].type_id = keys[c].type_id;
+        sk[c].values = a.dst[keyCol[c]];
+        sk[c].nulls = reinterpret_cast<const uint8_t *>(a.dst_nulls[keyCol[c]]);
+    }
+    for (int i = 0; i < na; ++i) {
+        sa[i].type_id = h.spec.agg_in_types[i];
+        if (aggCol[i] < 0) continue;
+        sa[i].values = a.dst[aggCol[i]];
+        sa[i].nulls = reinterpret_cast<const uint8_t *>(a.dst_nulls[aggCol[i]]);
+    }
+    return omni_gpu_agg_add_batch(handle, sk, sa, total);
+}
+
 int omni_gpu_agg_wait(void *handle)
 {
     OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
@@ -427,8 +527,12 @@ int omni_gpu_agg_destroy(void *handle)
     AggHandle *h = static_cast<AggHandle *>(handle);
     hipStream_t s = h->stream;
     for (DevBuf *b : {&h->slots, &h->ctrl, &h->skeys, &h->gkeys, &h->gnull, &h->gvalid, &h->states, &h->row_tmpl,
-                      &h->cursors, &h->index, &h->tile_counts, &h->counters}) {
+                      &h->cursors, &h->index, &h->tile_counts, &h->counters, &h->stage_src, &h->stage_rows}) {
         b->Release(s);
+    }
+    for (int c = 0; c < kStageCols; ++c) {
+        h->stage_vals[c].Release(s);
+        h->stage_nulls[c].Release(s);
     }
     ReleaseStream(s);
     delete h;

@@ -276,6 +276,81 @@ __global__ void init_slots1(Slot1 *s, int64_t n)
     }
 }
 
+// ---- staging of many batches (omni_gpu_agg_add_batches) ------------------------------------------------------
+constexpr int kStageCols = OMNI_GPU_MAX_KEYS + OMNI_GPU_MAX_AGGS;
+
+struct StageSrc {
+    const void *values;
+    const uint8_t *nulls;  // LSB-first bitmap or nullptr
+};
+
+struct StageArgs {
+    int nb;                              // batches
+    int ncols;                           // staged columns
+    long long rows;                      // total rows
+    const long long *row_start;          // [nb + 1] first staged row of each batch
+    const StageSrc *src;                 // [nb][ncols]
+    int width[kStageCols];
+    void *dst[kStageCols];               // staged values
+    unsigned int *dst_nulls[kStageCols]; // staged bitmap, or nullptr when no batch has NULLs in the column
+};
+
+__device__ __forceinline__ int StageBatch(const StageArgs &a, long long row)
+{
+    int lo = 0, hi = a.nb - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (a.row_start[mid] <= row) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Column blockIdx.y, one row per thread: copies the value of staged row r from its batch.
+__global__ void __launch_bounds__(kBlock) agg_stage_values(StageArgs a)
+{
+    const int c = blockIdx.y;
+    const long long r = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    if (r >= a.rows) return;
+    const int b = StageBatch(a, r);
+    const long long i = r - a.row_start[b];
+    const void *src = a.src[static_cast<long long>(b) * a.ncols + c].values;
+    switch (a.width[c]) {
+        case 8: static_cast<unsigned long long *>(a.dst[c])[r] = static_cast<const unsigned long long *>(src)[i]; break;
+        case 4: static_cast<unsigned int *>(a.dst[c])[r] = static_cast<const unsigned int *>(src)[i]; break;
+        case 2: static_cast<unsigned short *>(a.dst[c])[r] = static_cast<const unsigned short *>(src)[i]; break;
+        case 16: {
+            const unsigned long long *s2 = static_cast<const unsigned long long *>(src);
+            unsigned long long *d2 = static_cast<unsigned long long *>(a.dst[c]);
+            d2[2 * r] = s2[2 * i];
+            d2[2 * r + 1] = s2[2 * i + 1];
+            break;
+        }
+        default: static_cast<uint8_t *>(a.dst[c])[r] = static_cast<const uint8_t *>(src)[i]; break;
+    }
+}
+
+// Column blockIdx.y, one 32-row bitmap word per thread (columns without NULLs return at once).
+__global__ void __launch_bounds__(kBlock) agg_stage_nulls(StageArgs a)
+{
+    const int c = blockIdx.y;
+    if (a.dst_nulls[c] == nullptr) return;
+    const long long w = static_cast<long long>(blockIdx.x) * kBlock + threadIdx.x;
+    const long long r0 = w * 32;
+    if (r0 >= a.rows) return;
+    int b = StageBatch(a, r0);
+    unsigned int word = 0;
+    for (int k = 0; k < 32; ++k) {
+        const long long r = r0 + k;
+        if (r >= a.rows) break;
+        while (a.row_start[b + 1] <= r) ++b;  // empty batches are skipped
+        const uint8_t *nb = a.src[static_cast<long long>(b) * a.ncols + c].nulls;
+        if (nb != nullptr && NullBit(nb, r - a.row_start[b])) word |= 1u << k;
+    }
+    a.dst_nulls[c][w] = word;
+}
+
+
 // Output rows [start, start + count) of the live groups, in the order of `index` (live group ids ascending).
 __global__ void __launch_bounds__(256) agg_output(int64_t start, int64_t count, KeyDev kd, int single,
                                                   const unsigned int *index, const unsigned long long *gkeys,

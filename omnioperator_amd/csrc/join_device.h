@@ -57,6 +57,16 @@ struct TableDev {
 constexpr int kBlock = 256;
 inline unsigned int Blocks(int64_t n) { return static_cast<unsigned int>((n + kBlock - 1) / kBlock); }
 
+// Home slot of a join key in a table of mask + 1 = 2^b slots: the top b bits of a multiplicative (Fibonacci)
+// hash of the key, after folding its high half onto its low half so that keys differing only in high bits
+// still spread.  One 64-bit multiply per key; the mask is uniform, so the shift is a scalar value.
+__device__ __forceinline__ unsigned long long JoinHome(long long key, unsigned long long mask)
+{
+    unsigned long long k = static_cast<unsigned long long>(key);
+    k ^= k >> 29;
+    return (k * 0x9E3779B97F4A7C15ULL) >> (64 - __popcll(mask));
+}
+
 // One retained build batch (HashBuilderOperator::AddInput keeps the VectorBatch; here its device columns).
 struct BuildBatch {
     const void *key;
@@ -195,7 +205,7 @@ __global__ void join_build_hash(const long long *keys, const uint8_t *knull, int
         // are only used when every key is unique)
         if (inline_pay) *sentinel_pay = PayloadWord(inline_pay, inline_width, row);
     } else {
-        unsigned long long pos = Mix64(static_cast<unsigned long long>(key)) & mask;
+        unsigned long long pos = JoinHome(key, mask);
         while (true) {
             long long k = slots[pos].key;
             if (static_cast<unsigned long long>(k) == kEmptyKey) {
@@ -312,7 +322,7 @@ __device__ __forceinline__ void LookupGroup(const TableDev &t, const long long *
     unsigned int fp[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) {
-        pos[j] = Mix64(static_cast<unsigned long long>(key[j])) & t.mask;
+        pos[j] = JoinHome(key[j], t.mask);
         JoinSlot sl = t.slots[pos[j]];  // read for every lane (pos is always in the table): no branch
         fk[j] = sl.key;
         fh[j] = sl.head;
@@ -352,6 +362,14 @@ __device__ __forceinline__ void WriteMatch(const ProbeArgs &a, long long o, long
 {
     if (o >= a.cap) return;
     a.out_rows[o] = probeRow;
+    if (INLINE) {  // inline payloads: exactly one <= 4-byte payload column without NULLs
+        const PayloadDev &pd = a.pay[0];
+        if (pd.out == nullptr) return;
+        if (pd.width == 4) static_cast<unsigned int *>(pd.out)[o] = pay;
+        else if (pd.width == 2) static_cast<unsigned short *>(pd.out)[o] = static_cast<unsigned short>(pay);
+        else static_cast<uint8_t *>(pd.out)[o] = static_cast<uint8_t>(pay);
+        return;
+    }
     for (int p = 0; p < a.npay; ++p) {
         const PayloadDev &pd = a.pay[p];
         if (pd.out == nullptr) continue;
@@ -527,7 +545,7 @@ struct PartArgs {
 
 __device__ __forceinline__ unsigned int PartOf(const TableDev &t, const PartArgs &pa, long long key)
 {
-    return static_cast<unsigned int>((Mix64(static_cast<unsigned long long>(key)) & t.mask) >> pa.shift);
+    return static_cast<unsigned int>(JoinHome(key, t.mask) >> pa.shift);
 }
 
 // Count-matrix column of a histogram / scatter workgroup.  Workgroup w runs on XCD w mod 8, and the columns
@@ -807,7 +825,7 @@ __global__ void __launch_bounds__(kWinBlock) join_window_probe(ProbeArgs a, Tabl
             } else if (live[j]) {
                 // home - wbase lies in [0, 1 << shift) for every entry of this partition
                 unsigned int pos = static_cast<unsigned int>(
-                    ((Mix64(static_cast<unsigned long long>(key[j])) & t.mask) - wbase) & t.mask);
+                    (JoinHome(key[j], t.mask) - wbase) & t.mask);
                 bool open = true;
                 while (open && pos < W) {
                     const JoinSlot sl = win[pos];

@@ -236,3 +236,64 @@ def test_lds_strategy_spills_when_groups_exceed_lds():
     got = run_gpu([Col(LONG, k)], aggs, cols, 100_000, strategy=capi.STRATEGY_LDS)
     want = run_oracle([Col(LONG, k)], aggs, cols, 100_000)
     assert_row_sets_equal(got, want)
+
+
+def _uneven_spans(n, seed):
+    """Cuts n rows into uneven batches, empty ones included."""
+    rng = np.random.default_rng(seed)
+    cuts = np.sort(rng.integers(0, n + 1, 40))
+    cuts[5:8] = cuts[5]  # a run of empty batches
+    edges = np.concatenate([[0], cuts, [n]])
+    return [(int(a), int(b)) for a, b in zip(edges[:-1], edges[1:])]
+
+
+def _dev_slice(c, a, b):
+    """Device copy of rows [a, b) of column c; a slice without NULLs gets no bitmap."""
+    nl = None if c.nulls is None or not c.nulls[a:b].any() else c.nulls[a:b]
+    return capi.DeviceColumn(c.type_id, c.values[a:b], nl)
+
+
+@pytest.mark.parametrize("key_kind", ["long", "int", "composite", "double"])
+@pytest.mark.parametrize("with_nulls", [False, True])
+def test_add_batches_parity(key_kind, with_nulls):
+    """omni_gpu_agg_add_batches (one staging launch + one insert for 41 uneven batches, empty ones and batches
+    with and without NULL bitmaps mixed) against the oracle: bit-exact, doubles within 1e-12."""
+    keys, aggs, cols = _random_case(hash((key_kind, with_nulls, "batches")) % 2**32, 30000, key_kind, with_nulls)
+    spec = capi.make_agg_spec([c.type_id for c in keys], aggs, True)
+    h = capi.HashAgg(spec)
+    keep, batches = [], []
+    for a, b in _uneven_spans(len(keys[0]), 7):
+        kd = [_dev_slice(c, a, b) for c in keys]
+        ad = [None if c is None else _dev_slice(c, a, b) for c in cols]
+        keep.append((kd, ad))
+        batches.append(([d.c() for d in kd], [None if d is None else d.c() for d in ad], b - a))
+    h.add_batches(batches)
+    outs = h.output()
+    types = [c.type_id for c in keys] + [x[2] for x in aggs]
+    got = rows_of([(t, v, nl) for t, (v, nl) in zip(types, outs)])
+    h.close()
+    want = run_oracle(keys, aggs, cols, 4096)
+    assert_row_sets_equal(got, want, float_cols=float_cols(len(keys), aggs), rtol=DOUBLE_RTOL)
+
+
+def test_add_batches_shared_value_column():
+    """The plain group-by shape: SUM, COUNT(*), MIN and MAX over one int64 value column passed for every
+    aggregate (staged once), 100,000 rows in batches of 4096 added in two calls; equals the per-batch path."""
+    rng = np.random.default_rng(5)
+    n = 100_000
+    k = rng.integers(0, 1000, n, dtype=np.int64)
+    v = rng.integers(-10**6, 10**6 + 1, n, dtype=np.int64)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, LONG, LONG), (MAX, LONG, LONG)]
+    spec = capi.make_agg_spec([LONG], aggs, True)
+    kd, vd = capi.DeviceColumn(LONG, k), capi.DeviceColumn(LONG, v)
+    spans = workloads.batches(n, 4096)
+    batches = [([kd.c(a)], [vd.c(a), None, vd.c(a), vd.c(a)], b - a) for a, b in spans]
+    h = capi.HashAgg(spec)
+    h.add_batches(batches[:10])
+    h.add_batches(batches[10:])
+    outs = h.output()
+    got = rows_of([(t, v_, nl) for t, (v_, nl) in zip([LONG] + [a[2] for a in aggs], outs)])
+    h.close()
+    want = run_oracle([Col(LONG, k)], aggs, [Col(LONG, v), None, Col(LONG, v), Col(LONG, v)], 4096)
+    assert len(got) == 1000
+    assert_row_sets_equal(got, want)
