@@ -267,14 +267,25 @@ def bench_batched(args):
         g = s.run(kd.c(), [vd.c(), None], offsets, None, flags)
         return int(g.sum())
 
+    tuned = None
+    if args.batched_form != "per_batch":  # the plain one-launch-per-batch form keeps the default tiles
+        from omnioperator_amd import autotune
+        key = autotune.key_of(LONG, aggs, autotune.cardinality_bucket(float(np.mean(w.distinct))))
+        tuned = autotune.tune_stream(s, lambda: (step(), capi.synchronize()), key, use_cache=False)
+
     groups, ms = timed_steps(step, args, capi)
     st = s.stats()
+    ph = s.phases()
     s.close()
     alg = n * 16 + groups * 24
     cfg = {"workload": f"batched group-by: {len(w.row_counts)} independent VectorBatches, {n} rows (lognormal rows, "
                        "median 2048, [64, 65536]), per-batch distinct keys log-uniform in [8, 1e6]; SUM + COUNT(*)",
            "form": args.batched_form, "rows": n, "groups": groups, "lds_batches": st["lds_batches"],
-           "hbm_batches": st["hbm_batches"], "algorithmic_bytes_per_step": alg}
+           "hbm_batches": st["hbm_batches"],
+           "batches_by_phase": {"lds_table": ph[0], "lds_partitions": ph[1], "hbm_table": ph[2]},
+           "tiles": None if tuned is None else {"block": tuned.block, "lds_bytes": tuned.lds_bytes,
+                                                "max_probe": tuned.max_probe, "part_lds_bytes": tuned.part_lds_bytes},
+           "algorithmic_bytes_per_step": alg}
     achieved = alg / (ms / 1e3) / 1e9
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "batch_lds + batch_hbm (whole run)",

@@ -156,19 +156,25 @@ int omni_gpu_agg_destroy(void *handle);
 /* ---- batched stream of independent group-bys ----------------------------------------------------------
  * Replaces one HashAggregationOperator per VectorBatch (AddInput + GetOutput per batch,
  * group_aggregation.cpp:320-406, 454-474) for a stream of many small, uneven batches resident in HBM.
- * Two launches cover all batches: phase 1 aggregates each batch in an LDS table (one workgroup per batch)
- * and abandons the batches whose keys do not fit; phase 2 aggregates those exactly in private HBM tables.
+ * One launch per phase covers all batches: phase 1 aggregates each batch in an LDS table (one workgroup per
+ * batch) and abandons the batches whose keys do not fit; phase 2 splits each abandoned batch by key hash into
+ * partitions that fit an LDS table (one workgroup per partition); phase 3 aggregates exactly, in a private
+ * HBM table, any batch one of whose partitions still overflowed.
  * No reference counterpart exists for the batching itself; per-batch results equal the reference operator's.
  * One key column of <= 8 bytes; any aggregate the per-batch path supports. */
 int omni_gpu_agg_batches_create(const omni_gpu_agg_spec *spec, int64_t max_batch_rows, void **handle);
 /* Launch tuning (the autotuner's knobs): threads per workgroup (256 or 512), LDS table bytes
  * ([8 KiB, 64 KiB]) and the LDS probe limit before a batch is handed to phase 2. */
 int omni_gpu_agg_batches_tune(void *handle, int32_t block_threads, int32_t lds_bytes, int32_t max_probe);
+/* Phase-2 tuning: LDS bytes of a partition table ([8 KiB, 64 KiB], default 64 KiB) and rows per partition
+ * (0, the default: half the table's slots, so a partition overflows only far above its mean share; more
+ * rows mean fewer passes over each batch and more batches left to phase 3). */
+int omni_gpu_agg_batches_tune_partitions(void *handle, int32_t part_lds_bytes, int32_t part_rows);
 int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_bytes);
 /* Runs the stream.  keys[0] / aggs[i] are the concatenated device columns of all batches; batch b owns rows
  * [batch_offsets[b], batch_offsets[b+1]) (host array, batch_offsets[0] == 0, at most max_batch_rows rows
- * each).  batch_modes (host, optional): 0 = LDS table first, 2 = HBM table directly (e.g. from a predicted
- * group count).  flags: bit 0 = one launch per batch and phase instead of one launch per phase; bit 1 =
+ * each).  batch_modes (host, optional): 0 = LDS table first, 2 = skip the per-batch LDS table and start
+ * at phase 2 (e.g. from a predicted group count).  flags: bit 0 = one launch per batch and phase instead of one launch per phase; bit 1 =
  * capture the launch sequence in a hipGraph and replay it while the stream's shape stays the same.  Batch
  * b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
  * then aggregates); outs hold batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.
@@ -176,8 +182,12 @@ int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_by
 int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
                              const int64_t *batch_offsets, int32_t num_batches, const int32_t *batch_modes,
                              int32_t flags, omni_gpu_out_column *outs, int64_t *groups_per_batch);
-/* Batches of the last run aggregated in LDS tables / in HBM tables. */
+/* Batches of the last run finished by the per-batch LDS table (phase 1) / by a later phase. */
 int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_batches);
+/* Batches of the last run finished by each phase: 1 the per-batch LDS table, 2 the hash-partitioned LDS
+ * passes, 3 the per-batch HBM table (the exact fallback; in the one-launch-per-batch form every batch that
+ * overflowed phase 1). */
+int omni_gpu_agg_batches_phases(void *handle, int64_t *phase1, int64_t *phase2, int64_t *phase3);
 int omni_gpu_agg_batches_destroy(void *handle);
 
 /* ---- hash join (inner) --------------------------------------------------------------------------

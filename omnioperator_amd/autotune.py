@@ -1,4 +1,5 @@
-"""Autotuner for the batched group-by: picks threads per workgroup, LDS table bytes and the LDS probe limit by
+"""Autotuner for the batched group-by: picks threads per workgroup, LDS table bytes, the LDS probe limit and the
+phase-2 partition table bytes by
 timing candidate runs on the stream itself, per (key type, aggregate signature, cardinality bucket); winners
 are cached in a JSON file (OMNI_GPU_AUTOTUNE_CACHE, default ~/.cache/omnioperator_amd/autotune.json) so later
 runs of the same kind skip the sweep.
@@ -26,16 +27,24 @@ class StreamConfig:
     block: int = 256
     lds_bytes: int = 32768
     max_probe: int = 32
+    part_lds_bytes: int = 65536  # phase-2 partition table
 
 
 DEFAULT = StreamConfig()
 
 CANDIDATES: List[StreamConfig] = [
-    StreamConfig(b, l, p)
+    StreamConfig(b, l, p, q)
     for b in (256, 512)
     for l in (16384, 32768, 65536)
     for p in (8, 32)
+    for q in (32768, 65536)
 ]
+
+
+def apply(stream, cfg: StreamConfig):
+    """Sets every knob of cfg on a capi.AggBatches stream."""
+    stream.tune(block=cfg.block, lds_bytes=cfg.lds_bytes, max_probe=cfg.max_probe)
+    stream.tune_partitions(part_lds_bytes=cfg.part_lds_bytes, part_rows=0)
 
 
 def cardinality_bucket(mean_distinct: float) -> int:
@@ -79,11 +88,11 @@ def tune_stream(stream, run, key: str, candidates: List[StreamConfig] = CANDIDAT
     if use_cache:
         hit = lookup(key)
         if hit is not None:
-            stream.tune(**asdict(hit))
+            apply(stream, hit)
             return hit
     best, best_t, results = None, float("inf"), []
     for cfg in candidates:
-        stream.tune(**asdict(cfg))
+        apply(stream, cfg)
         run()  # warm: first launch of this configuration
         ts = []
         for _ in range(repeats):
@@ -94,7 +103,7 @@ def tune_stream(stream, run, key: str, candidates: List[StreamConfig] = CANDIDAT
         results.append((asdict(cfg), t))
         if t < best_t:
             best, best_t = cfg, t
-    stream.tune(**asdict(best))
+    apply(stream, best)
     d = _load()
     d[key] = {"config": asdict(best), "seconds": best_t, "swept": results}
     _store(d)

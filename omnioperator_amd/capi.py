@@ -93,6 +93,8 @@ SIGNATURES = {
     "omni_gpu_agg_batches_run": (_I, [_VP, _P(Column), _P(Column), _P(_I64), _I32, _VP, _I32, _P(OutColumn),
                                       _P(_I64)]),
     "omni_gpu_agg_batches_stats": (_I, [_VP, _P(_I64), _P(_I64)]),
+    "omni_gpu_agg_batches_phases": (_I, [_VP, _P(_I64), _P(_I64), _P(_I64)]),
+    "omni_gpu_agg_batches_tune_partitions": (_I, [_VP, _I32, _I32]),
     "omni_gpu_agg_batches_destroy": (_I, [_VP]),
     "omni_gpu_join_build_create": (_I, [_P(JoinSpec), _P(_VP)]),
     "omni_gpu_join_build_add": (_I, [_VP, _P(Column), _P(Column), _I64]),
@@ -354,6 +356,9 @@ class AggBatches:
     def tune(self, block: int = 256, lds_bytes: int = 32768, max_probe: int = 32):
         check(lib().omni_gpu_agg_batches_tune(self.h, block, lds_bytes, max_probe))
 
+    def tune_partitions(self, part_lds_bytes: int = 65536, part_rows: int = 0):
+        check(lib().omni_gpu_agg_batches_tune_partitions(self.h, part_lds_bytes, part_rows))
+
     def info(self):
         s, p = C.c_int32(), C.c_int64()
         check(lib().omni_gpu_agg_batches_info(self.h, C.byref(s), C.byref(p)))
@@ -402,6 +407,12 @@ class AggBatches:
         a, b = C.c_int64(), C.c_int64()
         check(lib().omni_gpu_agg_batches_stats(self.h, C.byref(a), C.byref(b)))
         return {"lds_batches": a.value, "hbm_batches": b.value}
+
+    def phases(self):
+        """Batches of the last run finished by phase 1 / 2 / 3."""
+        a, b, c = C.c_int64(), C.c_int64(), C.c_int64()
+        check(lib().omni_gpu_agg_batches_phases(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -16,12 +16,26 @@
 namespace omni_gpu {
 
 constexpr int kScanBlock = 1024;
+constexpr int kXcdCount = 8;  // XCDs of an MI355X; workgroup w runs on XCD w mod 8
 
 template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_lds(const BatchConfig *__restrict__ cfg, int first)
 {
     extern __shared__ unsigned long long lds_words[];
     __shared__ unsigned int sh[2];
     BatchLds<BLOCK>(*cfg, first + static_cast<int>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
+}
+
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_split(const BatchConfig *__restrict__ cfg)
+{
+    __shared__ unsigned int hist[kMaxParts];
+    BatchSplit<BLOCK>(*cfg, static_cast<int>(blockIdx.x), hist);
+}
+
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const BatchConfig *__restrict__ cfg)
+{
+    extern __shared__ unsigned long long lds_words[];
+    __shared__ unsigned int sh[4];
+    BatchPart<BLOCK>(*cfg, static_cast<int64_t>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
 }
 
 template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_init(const BatchConfig *__restrict__ cfg, int64_t first)
@@ -60,8 +74,13 @@ struct BatchesHandle {
     int lds_budget = 32 * 1024;
     int max_probe = 32;
     int64_t max_batch_rows = 0;
+    int part_budget = 64 * 1024;  // LDS bytes of a phase-2 partition table
+    int part_rows = 0;            // rows per phase-2 partition (0: half the partition table's slots)
     DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error flag
-    std::vector<int64_t> h_tiles;  // phase-2 tile prefix over the batches
+    DevBuf items, parts;           // phase-2 work items, partitions per batch
+    int64_t nitems = 0;
+    bool last_per_batch = false;
+    std::vector<int64_t> h_tiles;  // phase-3 tile prefix over the batches
     size_t pool_bytes = 0;
     std::vector<int64_t> h_offsets;
     std::vector<int32_t> h_modes;
@@ -88,8 +107,39 @@ int Layout(BatchesHandle &h)
     OG_CHECK(h.cfg.L.S >= 62, OMNI_GPU_ERR_INVALID, "LDS budget %d bytes holds only %d slots for these aggregates",
              h.lds_budget, h.cfg.L.S);
     h.cfg.max_probe = h.max_probe;
+    h.cfg.L2 = MakeLdsLayout(h.cfg.args, h.part_budget);
+    h.cfg.L2.S -= 2;
+    OG_CHECK(h.cfg.L2.S >= 62, OMNI_GPU_ERR_INVALID, "LDS budget %d bytes holds only %d slots for these aggregates",
+             h.part_budget, h.cfg.L2.S);
+    h.have_desc = false;  // the phase-2 items depend on the partition table size
     DropGraph(h);
     return OMNI_GPU_OK;
+}
+
+// Phase-2 work items for batches of the given row counts: batch b gets P_b partitions, the smallest power of
+// two with rows / P_b <= half the partition table (so a partition overflows only if its hash share is far
+// above the mean).  Items of batch b go to workgroup ids == b (mod 8), i.e. to one XCD, consecutively, so the
+// batch's rows stay in that XCD's L2 while its partitions re-read them; the per-XCD lists are interleaved
+// and padded with empty items.
+std::vector<int4> PartItems(const int64_t *offsets, int nb, int table_slots, int part_rows, std::vector<int32_t> &parts)
+{
+    const int64_t cpart = part_rows > 0 ? part_rows : std::max(1, table_slots / 2);
+    std::vector<std::vector<int4>> lists(kXcdCount);
+    parts.assign(nb, 1);
+    for (int b = 0; b < nb; ++b) {
+        const int64_t rows = offsets[b + 1] - offsets[b];
+        int P = 1;
+        while (P < kMaxParts && static_cast<int64_t>(P) * cpart < rows) P <<= 1;
+        parts[b] = P;
+        for (int p = 0; p < P; ++p) lists[b % kXcdCount].push_back(make_int4(b, p, P, 0));
+    }
+    size_t longest = 0;
+    for (auto &l : lists) longest = std::max(longest, l.size());
+    std::vector<int4> items(longest * kXcdCount, make_int4(-1, 0, 1, 0));
+    for (int x = 0; x < kXcdCount; ++x) {
+        for (size_t k = 0; k < lists[x].size(); ++k) items[k * kXcdCount + x] = lists[x][k];
+    }
+    return items;
 }
 
 // Launches the whole sequence on the handle's stream (no timing events: it may be under graph capture).
@@ -116,6 +166,17 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     OG_TRY(CheckLaunch("batch_lds"));
     batch_scan<<<1, kScanBlock, 0, s>>>(dc);
     OG_TRY(CheckLaunch("batch_scan"));
+    if (!per_batch && h.nitems > 0) {
+        const unsigned int g = static_cast<unsigned int>(h.nitems);
+        if (h.block == 512) {
+            batch_split<512><<<nb, 512, 0, s>>>(dc);
+            batch_part<512><<<g, 512, h.cfg.L2.bytes, s>>>(dc);
+        } else {
+            batch_split<256><<<nb, 256, 0, s>>>(dc);
+            batch_part<256><<<g, 256, h.cfg.L2.bytes, s>>>(dc);
+        }
+        OG_TRY(CheckLaunch("batch_part"));
+    }
     const int64_t tiles = h.cfg.total_tiles;
     if (per_batch) {
         for (int b = 0; b < nb; ++b) {
@@ -189,6 +250,18 @@ int omni_gpu_agg_batches_tune(void *handle, int32_t block_threads, int32_t lds_b
     h.block = block_threads;
     h.lds_budget = lds_bytes;
     h.max_probe = max_probe;
+    return Layout(h);
+}
+
+int omni_gpu_agg_batches_tune_partitions(void *handle, int32_t part_lds_bytes, int32_t part_rows)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    BatchesHandle &h = *static_cast<BatchesHandle *>(handle);
+    OG_CHECK(part_lds_bytes >= 8 * 1024 && part_lds_bytes <= 64 * 1024, OMNI_GPU_ERR_INVALID,
+             "part_lds_bytes must be in [8K, 64K]");
+    OG_CHECK(part_rows >= 0, OMNI_GPU_ERR_INVALID, "part_rows must be >= 0");
+    h.part_budget = part_lds_bytes;
+    h.part_rows = part_rows;
     return Layout(h);
 }
 
@@ -278,6 +351,14 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         }
         OG_TRY(h.pool.Reserve(pool, s));
         h.pool_bytes = pool;
+        std::vector<int32_t> parts;
+        const std::vector<int4> items = PartItems(batch_offsets, num_batches, c.L2.S, h.part_rows, parts);
+        h.nitems = static_cast<int64_t>(items.size());
+        OG_CHECK(h.nitems < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many phase-2 items for one run");
+        OG_TRY(h.items.Reserve(sizeof(int4) * items.size(), s));
+        OG_TRY(CopyToDevice(h.items.p, items.data(), sizeof(int4) * items.size(), s));
+        OG_TRY(h.parts.Reserve(sizeof(int32_t) * parts.size(), s));
+        OG_TRY(CopyToDevice(h.parts.p, parts.data(), sizeof(int32_t) * parts.size(), s));
         h.have_desc = true;
     }
     c.offsets = h.offsets.as<int64_t>();
@@ -289,6 +370,11 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.groups = h.groups.as<unsigned long long>();
     c.region_off = h.region_off.as<unsigned long long>();
     c.region_pool = h.pool.as<uint8_t>();
+    c.items = h.items.as<int4>();
+    c.parts = h.parts.as<int32_t>();
+    c.nitems = h.nitems;
+    c.hbm_flag = (flags & 1) ? 1 : 2;  // the per-batch form has no phase 2
+    h.last_per_batch = (flags & 1) != 0;
     c.error_flag = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>());
     OG_TRY(CopyToDevice(h.dcfg.p, &c, sizeof(BatchConfig), s));
     OG_HIP(hipMemsetAsync(c.error_flag, 0, 4, s));
@@ -344,6 +430,29 @@ int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_
     return OMNI_GPU_OK;
 }
 
+int omni_gpu_agg_batches_phases(void *handle, int64_t *phase1, int64_t *phase2, int64_t *phase3)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    BatchesHandle &h = *static_cast<BatchesHandle *>(handle);
+    int64_t nb = h.have_desc ? static_cast<int64_t>(h.h_offsets.size()) - 1 : 0;
+    std::vector<unsigned int> sp(static_cast<size_t>(std::max<int64_t>(nb, 1)));
+    if (nb > 0) {
+        OG_TRY(CopyToHost(h.stream, {{sp.data(), h.spilled.p, sizeof(unsigned int) * nb}}));
+    }
+    // flags of the last run: 0 finished in phase 1; 1 finished in phase 2 (or, in the per-batch form, in
+    // HBM tables); 2 finished in HBM tables after a phase-2 partition overflowed
+    int64_t n[3] = {0, 0, 0};
+    for (int64_t b = 0; b < nb; ++b) n[sp[b] < 3 ? sp[b] : 2] += 1;
+    if (h.last_per_batch) {
+        n[2] += n[1];
+        n[1] = 0;
+    }
+    if (phase1) *phase1 = n[0];
+    if (phase2) *phase2 = n[1];
+    if (phase3) *phase3 = n[2];
+    return OMNI_GPU_OK;
+}
+
 int omni_gpu_agg_batches_destroy(void *handle)
 {
     if (!handle) return OMNI_GPU_OK;
@@ -351,6 +460,8 @@ int omni_gpu_agg_batches_destroy(void *handle)
     DropGraph(*h);
     hipStream_t s = h->stream;
     h->dcfg.Release(s);
+    h->items.Release(s);
+    h->parts.Release(s);
     h->offsets.Release(s);
     h->tiles.Release(s);
     h->modes.Release(s);

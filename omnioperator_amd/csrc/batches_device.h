@@ -1,12 +1,18 @@
-// Device code of the batched group-by: many independent group-bys (one per VectorBatch) in two launches.
+// Device code of the batched group-by: many independent group-bys (one per VectorBatch).
 //
 // Phase 1 (one workgroup per batch): the batch is aggregated in an LDS table; if any row finds no LDS slot
-// within the probe limit, the workgroup abandons the batch (writes nothing) and flags it.
-// Phase 2 (flagged batches only): the batch is aggregated exactly in its own HBM table of next_pow2(2 * rows)
-// slots (plus 2 side slots) carved from a region pool at a scanned offset.  Three launches: the tables are
-// initialised and filled by workgroups of kHbmTile rows each (a large batch is spread over many workgroups; the
-// table words are updated with device-wide atomics), then one workgroup per batch emits its groups.
-// Both phases emit the batch's groups to output rows [off[b], off[b] + groups[b]).
+// within the probe limit, the workgroup abandons the batch (writes nothing) and flags it (spilled = 1).
+// Phase 2 (flagged batches): the batch's keys are split by hash into P partitions, P a power of two sized
+// so that a partition's rows fit half an LDS table.  2a: one workgroup per batch writes each row's key and
+// row number to its partition's bucket (in the batch's region of the pool).  2b: one workgroup per (batch,
+// partition) item aggregates its bucket in LDS and emits its groups.  A batch's 2a workgroup and all its 2b
+// items run on the same XCD (workgroup id mod 8), so the buckets are read back from that XCD's L2.  A
+// partition that still overflows its table flags the batch (spilled = 2) for phase 3.
+// Phase 3 (batches flagged 2; batches flagged 1 when phase 2 is not used): the batch is aggregated exactly in
+// its own HBM table of next_pow2(2 * rows) slots (plus 2 side slots) carved from a region pool at a scanned
+// offset, by workgroups of kHbmTile rows (device-wide atomics); then one workgroup per batch emits all its
+// groups again from output row off[b], setting and clearing NULL bits, so partial phase-2 output is replaced.
+// Every phase emits batch b's groups to output rows [off[b], off[b] + groups[b]).
 // The configuration lives in device memory and is read through one pointer (no large kernel arguments).
 #pragma once
 
@@ -26,9 +32,15 @@ struct BatchConfig {
     int32_t max_probe;
     int32_t overflow_as_null;
     int32_t pad;
-    const int64_t *tile_start; // nb + 1: first phase-2 tile of each batch (kHbmTile rows per tile, at least one
+    const int64_t *tile_start; // nb + 1: first phase-3 tile of each batch (kHbmTile rows per tile, at least one
                                // tile per batch, all batches)
     int64_t total_tiles;
+    LdsLayout L2;              // phase-2 LDS table (probe slots [0, L2.S), side slots L2.S, L2.S + 1)
+    const int4 *items;         // phase-2 work items {batch, partition, partitions, 0}; batch -1: no work
+    int64_t nitems;
+    const int32_t *parts;      // per batch: phase-2 partitions (a power of two, <= kMaxParts)
+    int32_t hbm_flag;          // phase 3 aggregates the batches whose spilled flag equals this value
+    int32_t pad2;
     unsigned int *spilled;     // per batch: 1 = phase 2 must aggregate it
     unsigned long long *groups;  // per batch
     unsigned long long *region_off;  // per batch byte offset into the region pool (phase 2)
@@ -69,22 +81,27 @@ template <typename Ref> __device__ __forceinline__ void InitSlot(const AggArgs &
 }
 
 // Writes one group (key + aggregates) to output row r; NULL bits are OR-ed into pre-cleared bitmaps.
-template <typename Ref>
+// CLEAR: also clears the NULL bits of non-NULL results (the row may hold an earlier partial emission).
+template <bool CLEAR = false, typename Ref>
 __device__ __forceinline__ void EmitGroup(const BatchConfig &c, const Ref &s, unsigned long long key, bool keyNull,
                                           int64_t r)
 {
     const OutDev &ko = c.out.o[0];
     StoreWidth(ko.values, ko.width, r, keyNull ? 0ULL : key);
-    if (keyNull) atomicOr(reinterpret_cast<unsigned int *>(ko.nulls) + (r >> 5), 1u << (r & 31));
+    unsigned int *kw = reinterpret_cast<unsigned int *>(ko.nulls) + (r >> 5);
+    if (keyNull) atomicOr(kw, 1u << (r & 31));
+    else if (CLEAR) atomicAnd(kw, ~(1u << (r & 31)));
     for (int i = 0; i < c.args.n; ++i) {
         const OutDev &o = c.out.o[1 + i];
         bool isNull = FinalizeAgg(c.args.a[i], i, s, o, r, c.overflow_as_null, c.error_flag);
-        if (isNull) atomicOr(reinterpret_cast<unsigned int *>(o.nulls) + (r >> 5), 1u << (r & 31));
+        unsigned int *w = reinterpret_cast<unsigned int *>(o.nulls) + (r >> 5);
+        if (isNull) atomicOr(w, 1u << (r & 31));
+        else if (CLEAR) atomicAnd(w, ~(1u << (r & 31)));
     }
 }
 
 // Emits every used slot of a table with probe slots [0, S) and side slots S, S + 1; ref(i) addresses slot i.
-template <int BLOCK, typename MakeRef>
+template <int BLOCK, bool CLEAR = false, typename MakeRef>
 __device__ __forceinline__ void EmitTable(const BatchConfig &c, const unsigned long long *keys, int S, int64_t start,
                                           unsigned int *counter, MakeRef ref)
 {
@@ -93,7 +110,7 @@ __device__ __forceinline__ void EmitTable(const BatchConfig &c, const unsigned l
         if (k == kEmptyKey) continue;
         unsigned int r = atomicAdd(counter, 1u);
         bool side = i >= S;
-        EmitGroup(c, ref(i), side ? (i == S ? 0ULL : kEmptyKey) : k, i == S, start + r);
+        EmitGroup<CLEAR>(c, ref(i), side ? (i == S ? 0ULL : kEmptyKey) : k, i == S, start + r);
     }
 }
 
@@ -105,7 +122,10 @@ __device__ __forceinline__ void BatchLds(const BatchConfig &c, int b, uint8_t *l
     const int S = L.S;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
     if (c.modes != nullptr && c.modes[b] == 2) {
-        if (threadIdx.x == 0) c.spilled[b] = 1u;
+        if (threadIdx.x == 0) {
+            c.spilled[b] = 1u;
+            c.groups[b] = 0;
+        }
         return;
     }
     unsigned long long *lkeys = reinterpret_cast<unsigned long long *>(lds + L.keys_off);
@@ -153,7 +173,10 @@ __device__ __forceinline__ void BatchLds(const BatchConfig &c, int b, uint8_t *l
     }
     __syncthreads();
     if (sh[0]) {
-        if (threadIdx.x == 0) c.spilled[b] = 1u;
+        if (threadIdx.x == 0) {
+            c.spilled[b] = 1u;
+            c.groups[b] = 0;  // phase 2 items add their group counts
+        }
         return;
     }
     EmitTable<BLOCK>(c, lkeys, S, start, &sh[1], [&](int i) { return LdsRef{lds, L, i}; });
@@ -162,6 +185,146 @@ __device__ __forceinline__ void BatchLds(const BatchConfig &c, int b, uint8_t *l
         c.spilled[b] = 0u;
         c.groups[b] = sh[1];
     }
+}
+
+// Partition of a key among P = 2^k partitions of a phase-2 batch: the low bits of its hash (the LDS slot uses
+// the high bits).  The NULL key and the slot sentinel belong to partition 0.
+__device__ __forceinline__ int PartOfKey(unsigned long long key, bool isNull, int P)
+{
+    if (isNull || key == kEmptyKey) return 0;
+    return static_cast<int>(Mix64(key) & static_cast<unsigned long long>(P - 1));
+}
+
+constexpr int kMaxParts = 1024;  // phase-2 partitions of one batch
+
+// Phase-2 buckets of batch b live in its region of the pool (the region is not used by phase 3 before every
+// phase-2 launch has finished): u32 bucket starts [P + 1], then the keys (u64) and the batch-relative row
+// numbers (u32) of the batch's rows in partition order.
+struct Buckets {
+    unsigned int *start;
+    unsigned long long *keys;
+    unsigned int *rows;
+};
+
+__device__ __forceinline__ Buckets BucketsOf(const BatchConfig &c, int b, int P, int64_t rows)
+{
+    uint8_t *base = c.region_pool + c.region_off[b];
+    Buckets k;
+    k.start = reinterpret_cast<unsigned int *>(base);
+    k.keys = reinterpret_cast<unsigned long long *>(base + ((static_cast<size_t>(P + 1) * 4 + 255) & ~size_t(255)));
+    k.rows = reinterpret_cast<unsigned int *>(k.keys + rows);
+    return k;
+}
+
+// Phase 2a (one workgroup per batch; batches that phase 1 finished return): splits the batch's rows into its
+// P partitions -- LDS histogram, scan, then each row's key and row number written to its bucket.  Workgroup b
+// runs on the XCD of batch b's phase-2b items, so the buckets are read back from that XCD's L2.
+template <int BLOCK>
+__device__ __forceinline__ void BatchSplit(const BatchConfig &c, int b, unsigned int *hist)
+{
+    if (c.spilled[b] == 0u) return;
+    const int P = c.parts[b];
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    const Buckets k = BucketsOf(c, b, P, end - start);
+    for (int i = threadIdx.x; i < P; i += BLOCK) hist[i] = 0;
+    __syncthreads();
+    for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
+        const bool isNull = NullBit(c.kd.nulls[0], row);
+        const unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
+        atomicAdd(&hist[PartOfKey(key, isNull, P)], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // P <= kMaxParts: a serial scan is short
+        unsigned int run = 0;
+        for (int i = 0; i < P; ++i) {
+            const unsigned int v = hist[i];
+            hist[i] = run;
+            k.start[i] = run;
+            run += v;
+        }
+        k.start[P] = run;
+    }
+    __syncthreads();
+    for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
+        const bool isNull = NullBit(c.kd.nulls[0], row);
+        const unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
+        const unsigned int at = atomicAdd(&hist[PartOfKey(key, isNull, P)], 1u);
+        k.keys[at] = key;
+        k.rows[at] = static_cast<unsigned int>(row - start);
+    }
+}
+
+// Phase 2b: work item `it` = (batch b, partition p of P) aggregates bucket p of batch b in an LDS table of
+// L2.S slots (probing the whole table), then reserves its output rows with one atomic on groups[b] and
+// emits.  A partition whose keys do not fit flags the batch for phase 3 and emits nothing.
+template <int BLOCK>
+__device__ __forceinline__ void BatchPart(const BatchConfig &c, int64_t it, uint8_t *lds, unsigned int *sh)
+{
+    const int4 item = c.items[it];
+    const int b = item.x;
+    if (b < 0 || c.spilled[b] == 0u) return;
+    const int p = item.y, P = item.z;
+    const LdsLayout &L = c.L2;
+    const int S = L.S;
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    const Buckets k = BucketsOf(c, b, P, end - start);
+    unsigned long long *lkeys = reinterpret_cast<unsigned long long *>(lds + L.keys_off);
+    for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
+        lkeys[i] = kEmptyKey;
+        InitSlot(c.args, LdsRef{lds, L, i});
+    }
+    if (threadIdx.x == 0) {
+        sh[0] = 0;  // a key found no slot
+        sh[1] = 0;  // groups emitted
+        sh[2] = 0;  // occupied slots
+    }
+    __syncthreads();
+    const unsigned int e0 = k.start[p], e1 = k.start[p + 1];
+    for (unsigned int e = e0 + threadIdx.x; e < e1; e += BLOCK) {
+        const unsigned long long key = k.keys[e];
+        const int64_t row = start + k.rows[e];
+        const bool isNull = NullBit(c.kd.nulls[0], row);
+        int slot = -1;
+        if (isNull) {
+            slot = S;
+            lkeys[S] = 0;
+        } else if (key == kEmptyKey) {
+            slot = S + 1;
+            lkeys[S + 1] = 0;
+        } else {
+            unsigned long long h = Mix64(key);
+            unsigned int pos = static_cast<unsigned int>(((h >> 32) * static_cast<unsigned long long>(S)) >> 32);
+            for (int step = 0; step < S; ++step) {
+                unsigned long long kk = lkeys[pos];
+                if (kk == kEmptyKey) {
+                    kk = atomicCAS(&lkeys[pos], kEmptyKey, key);
+                    if (kk == kEmptyKey) kk = key;
+                }
+                if (kk == key) {
+                    slot = static_cast<int>(pos);
+                    break;
+                }
+                pos = pos + 1 == static_cast<unsigned int>(S) ? 0 : pos + 1;
+            }
+        }
+        if (slot < 0) {
+            sh[0] = 1u;
+            break;
+        }
+        UpdateAggsT(c.args, LdsRef{lds, L, slot}, row);
+    }
+    __syncthreads();
+    if (sh[0]) {
+        if (threadIdx.x == 0) c.spilled[b] = 2u;
+        return;
+    }
+    unsigned int mine = 0;
+    for (int i = threadIdx.x; i < S + 2; i += BLOCK) mine += lkeys[i] != kEmptyKey ? 1u : 0u;
+    if (mine) atomicAdd(&sh[2], mine);
+    __syncthreads();
+    if (threadIdx.x == 0) sh[3] = sh[2] ? static_cast<unsigned int>(atomicAdd(&c.groups[b], static_cast<unsigned long long>(sh[2]))) : 0u;
+    __syncthreads();
+    EmitTable<BLOCK>(c, lkeys, S, start + sh[3], &sh[1], [&](int i) { return LdsRef{lds, L, i}; });
 }
 
 // Linear-probing find-or-insert of a key into a key array with probe slots [0, mask].
@@ -198,7 +361,7 @@ template <int BLOCK>
 __device__ __forceinline__ void BatchHbmInit(const BatchConfig &c, int64_t t)
 {
     const int b = TileBatch(c, t);
-    if (!c.spilled[b]) return;
+    if (c.spilled[b] != static_cast<unsigned int>(c.hbm_flag)) return;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
     const int S = RegionSlots(end - start);
     const long long stride = RegionStride(end - start);
@@ -219,7 +382,7 @@ template <int BLOCK>
 __device__ __forceinline__ void BatchHbmInsert(const BatchConfig &c, int64_t t)
 {
     const int b = TileBatch(c, t);
-    if (!c.spilled[b]) return;
+    if (c.spilled[b] != static_cast<unsigned int>(c.hbm_flag)) return;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
     const int S = RegionSlots(end - start);
     const long long stride = RegionStride(end - start);
@@ -248,7 +411,7 @@ __device__ __forceinline__ void BatchHbmInsert(const BatchConfig &c, int64_t t)
 template <int BLOCK>
 __device__ __forceinline__ void BatchHbmEmit(const BatchConfig &c, int b, unsigned int *sh)
 {
-    if (!c.spilled[b]) return;
+    if (c.spilled[b] != static_cast<unsigned int>(c.hbm_flag)) return;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
     const int S = RegionSlots(end - start);
     const long long stride = RegionStride(end - start);
@@ -256,7 +419,7 @@ __device__ __forceinline__ void BatchHbmEmit(const BatchConfig &c, int b, unsign
     const unsigned long long *keys = reinterpret_cast<const unsigned long long *>(base);
     if (threadIdx.x == 0) sh[1] = 0;
     __syncthreads();
-    EmitTable<BLOCK>(c, keys, S, start, &sh[1], [&](int i) { return ScaledRef{base, c.R, stride, i}; });
+    EmitTable<BLOCK, true>(c, keys, S, start, &sh[1], [&](int i) { return ScaledRef{base, c.R, stride, i}; });
     __syncthreads();
     if (threadIdx.x == 0) c.groups[b] = sh[1];
 }

@@ -65,11 +65,13 @@ def per_batch_rows(offsets, groups, outs):
     return res
 
 
-def run_batches(offsets, key, cols, modes=None, flags=0, tune=None, repeat=1):
+def run_batches(offsets, key, cols, modes=None, flags=0, tune=None, repeat=1, parts=None):
     spec = capi.make_agg_spec([LONG], AGGS)
     s = capi.AggBatches(spec, int(np.max(np.diff(offsets))))
     if tune:
         s.tune(**tune)
+    if parts:
+        s.tune_partitions(**parts)
     kd = capi.DeviceColumn(LONG, key.values, key.nulls)
     ad = [None if c is None else capi.DeviceColumn(c.type_id, c.values, c.nulls) for c in cols]
     ac = [None if d is None else d.c() for d in ad]
@@ -78,12 +80,17 @@ def run_batches(offsets, key, cols, modes=None, flags=0, tune=None, repeat=1):
         groups = s.run(kd.c(), ac, offsets, modes, flags)
         results.append((groups, s.outputs()))
     stats = s.stats()
+    stats["phases"] = s.phases()
     s.close()
     return results, stats
 
 
-@pytest.mark.parametrize("form", ["two_launches", "per_batch", "graph", "hbm_modes", "small_lds", "block512"])
+@pytest.mark.parametrize("form", ["two_launches", "per_batch", "graph", "hbm_modes", "small_lds", "block512",
+                                  "small_partitions", "partition_overflow"])
 def test_batches_match_oracle(form):
+    """Every form of the batched group-by against the oracle, batch by batch.  small_partitions: 8 KiB phase-2
+    tables (many partitions per batch); partition_overflow: partitions sized 16x above their table, so phase-2
+    partitions overflow, the batch's partial phase-2 output is replaced by phase 3 (NULL bits included)."""
     offsets, key, cols = make_stream(1234)
     nb = len(offsets) - 1
     kwargs = {}
@@ -98,6 +105,12 @@ def test_batches_match_oracle(form):
         kwargs["tune"] = {"block": 256, "lds_bytes": 8192, "max_probe": 4}
     elif form == "block512":
         kwargs["tune"] = {"block": 512, "lds_bytes": 65536, "max_probe": 64}
+    elif form == "small_partitions":
+        kwargs["tune"] = {"block": 256, "lds_bytes": 8192, "max_probe": 4}
+        kwargs["parts"] = {"part_lds_bytes": 8192, "part_rows": 0}
+    elif form == "partition_overflow":
+        kwargs["tune"] = {"block": 256, "lds_bytes": 8192, "max_probe": 4}
+        kwargs["parts"] = {"part_lds_bytes": 8192, "part_rows": 4096}
     results, stats = run_batches(offsets, key, cols, **kwargs)
     want = oracle_batches(offsets, key, cols, range(nb))
     for groups, outs in results:
@@ -105,8 +118,18 @@ def test_batches_match_oracle(form):
         for b in range(nb):
             assert_row_sets_equal(got[b], want[b], float_cols=FLOAT_COLS, rtol=1e-12)
     assert stats["lds_batches"] + stats["hbm_batches"] == nb
+    p1, p2, p3 = stats["phases"]
+    assert p1 + p2 + p3 == nb and p1 == stats["lds_batches"]
     if form == "small_lds":
         assert stats["hbm_batches"] > 0  # 8 KiB tables cannot hold the larger batches
+    if form in ("small_lds", "small_partitions", "two_launches"):
+        assert p3 == 0  # partitions sized to half a table do not overflow on this stream
+    if form == "small_partitions":
+        assert p2 > 0
+    if form == "partition_overflow":
+        assert p3 > 0
+    if form == "per_batch":
+        assert p2 == 0
     if form == "hbm_modes":
         assert stats["hbm_batches"] >= (nb + 1) // 2
     if form == "block512":
