@@ -28,7 +28,9 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_lds(const Ba
 template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_split(const BatchConfig *__restrict__ cfg)
 {
     __shared__ unsigned int hist[kMaxParts];
-    BatchSplit<BLOCK>(*cfg, static_cast<int>(blockIdx.x), hist);
+    __shared__ unsigned int bits[kLcWords];
+    __shared__ unsigned int sh[2];
+    BatchSplit<BLOCK>(*cfg, static_cast<int>(blockIdx.x), hist, bits, sh);
 }
 
 template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const BatchConfig *__restrict__ cfg)
@@ -38,20 +40,31 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const B
     BatchPart<BLOCK>(*cfg, static_cast<int64_t>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
 }
 
-template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_init(const BatchConfig *__restrict__ cfg, int64_t first)
+// Phase 3 launches are grid-stride over their tiles / batches (grid capped at kPhase3Grid) and return at once
+// when phase 2 ran and no partition overflowed, so an idle phase 3 costs a few microseconds.
+constexpr int64_t kPhase3Grid = 2048;
+
+__device__ __forceinline__ bool Phase3Idle(const BatchConfig &c) { return c.hbm_flag == 2 && *c.redo == 0u; }
+
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_init(const BatchConfig *__restrict__ cfg, int64_t first,
+                                                                              int64_t count)
 {
-    BatchHbmInit<BLOCK>(*cfg, first + static_cast<int64_t>(blockIdx.x));
+    if (Phase3Idle(*cfg)) return;
+    for (int64_t t = blockIdx.x; t < count; t += gridDim.x) BatchHbmInit<BLOCK>(*cfg, first + t);
 }
 
-template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_insert(const BatchConfig *__restrict__ cfg, int64_t first)
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_insert(const BatchConfig *__restrict__ cfg, int64_t first,
+                                                                                int64_t count)
 {
-    BatchHbmInsert<BLOCK>(*cfg, first + static_cast<int64_t>(blockIdx.x));
+    if (Phase3Idle(*cfg)) return;
+    for (int64_t t = blockIdx.x; t < count; t += gridDim.x) BatchHbmInsert<BLOCK>(*cfg, first + t);
 }
 
-template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_emit(const BatchConfig *__restrict__ cfg, int first)
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_emit(const BatchConfig *__restrict__ cfg, int first, int count)
 {
     __shared__ unsigned int sh[2];
-    BatchHbmEmit<BLOCK>(*cfg, first + static_cast<int>(blockIdx.x), sh);
+    if (Phase3Idle(*cfg)) return;
+    for (int b = blockIdx.x; b < count; b += gridDim.x) BatchHbmEmit<BLOCK>(*cfg, first + b, sh);
 }
 
 __global__ void __launch_bounds__(kScanBlock) batch_scan(const BatchConfig *__restrict__ cfg)
@@ -76,8 +89,8 @@ struct BatchesHandle {
     int64_t max_batch_rows = 0;
     int part_budget = 64 * 1024;  // LDS bytes of a phase-2 partition table
     int part_rows = 0;            // rows per phase-2 partition (0: half the partition table's slots)
-    DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error flag
-    DevBuf items, parts;           // phase-2 work items, partitions per batch
+    DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error, [4] redo
+    DevBuf items, parts, parts_used;  // phase-2 work items, partitions launched / used per batch
     int64_t nitems = 0;
     bool last_per_batch = false;
     std::vector<int64_t> h_tiles;  // phase-3 tile prefix over the batches
@@ -178,30 +191,30 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
         OG_TRY(CheckLaunch("batch_part"));
     }
     const int64_t tiles = h.cfg.total_tiles;
+    auto grid = [](int64_t n) { return static_cast<unsigned int>(std::max<int64_t>(1, std::min(n, kPhase3Grid))); };
     if (per_batch) {
         for (int b = 0; b < nb; ++b) {
             const int64_t t0 = h.h_tiles[b], nt = h.h_tiles[b + 1] - h.h_tiles[b];
             if (nt == 0) continue;
             if (h.block == 512) {
-                batch_hbm_init<512><<<static_cast<unsigned int>(nt), 512, 0, s>>>(dc, t0);
-                batch_hbm_insert<512><<<static_cast<unsigned int>(nt), 512, 0, s>>>(dc, t0);
-                batch_hbm_emit<512><<<1, 512, 0, s>>>(dc, b);
+                batch_hbm_init<512><<<grid(nt), 512, 0, s>>>(dc, t0, nt);
+                batch_hbm_insert<512><<<grid(nt), 512, 0, s>>>(dc, t0, nt);
+                batch_hbm_emit<512><<<1, 512, 0, s>>>(dc, b, 1);
             } else {
-                batch_hbm_init<256><<<static_cast<unsigned int>(nt), 256, 0, s>>>(dc, t0);
-                batch_hbm_insert<256><<<static_cast<unsigned int>(nt), 256, 0, s>>>(dc, t0);
-                batch_hbm_emit<256><<<1, 256, 0, s>>>(dc, b);
+                batch_hbm_init<256><<<grid(nt), 256, 0, s>>>(dc, t0, nt);
+                batch_hbm_insert<256><<<grid(nt), 256, 0, s>>>(dc, t0, nt);
+                batch_hbm_emit<256><<<1, 256, 0, s>>>(dc, b, 1);
             }
         }
     } else if (tiles > 0) {
-        const unsigned int g = static_cast<unsigned int>(tiles);
         if (h.block == 512) {
-            batch_hbm_init<512><<<g, 512, 0, s>>>(dc, 0);
-            batch_hbm_insert<512><<<g, 512, 0, s>>>(dc, 0);
-            batch_hbm_emit<512><<<nb, 512, 0, s>>>(dc, 0);
+            batch_hbm_init<512><<<grid(tiles), 512, 0, s>>>(dc, 0, tiles);
+            batch_hbm_insert<512><<<grid(tiles), 512, 0, s>>>(dc, 0, tiles);
+            batch_hbm_emit<512><<<grid(nb), 512, 0, s>>>(dc, 0, nb);
         } else {
-            batch_hbm_init<256><<<g, 256, 0, s>>>(dc, 0);
-            batch_hbm_insert<256><<<g, 256, 0, s>>>(dc, 0);
-            batch_hbm_emit<256><<<nb, 256, 0, s>>>(dc, 0);
+            batch_hbm_init<256><<<grid(tiles), 256, 0, s>>>(dc, 0, tiles);
+            batch_hbm_insert<256><<<grid(tiles), 256, 0, s>>>(dc, 0, tiles);
+            batch_hbm_emit<256><<<grid(nb), 256, 0, s>>>(dc, 0, nb);
         }
     }
     return CheckLaunch("batch_hbm");
@@ -358,6 +371,7 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         OG_TRY(h.items.Reserve(sizeof(int4) * items.size(), s));
         OG_TRY(CopyToDevice(h.items.p, items.data(), sizeof(int4) * items.size(), s));
         OG_TRY(h.parts.Reserve(sizeof(int32_t) * parts.size(), s));
+        OG_TRY(h.parts_used.Reserve(sizeof(int32_t) * parts.size(), s));
         OG_TRY(CopyToDevice(h.parts.p, parts.data(), sizeof(int32_t) * parts.size(), s));
         h.have_desc = true;
     }
@@ -372,12 +386,14 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.region_pool = h.pool.as<uint8_t>();
     c.items = h.items.as<int4>();
     c.parts = h.parts.as<int32_t>();
+    c.parts_used = h.parts_used.as<int32_t>();
+    c.redo = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>() + 4);
     c.nitems = h.nitems;
     c.hbm_flag = (flags & 1) ? 1 : 2;  // the per-batch form has no phase 2
     h.last_per_batch = (flags & 1) != 0;
     c.error_flag = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>());
     OG_TRY(CopyToDevice(h.dcfg.p, &c, sizeof(BatchConfig), s));
-    OG_HIP(hipMemsetAsync(c.error_flag, 0, 4, s));
+    OG_HIP(hipMemsetAsync(c.error_flag, 0, 8, s));  // error flag and phase-3 redo flag
     const bool per_batch = flags & 1;
     const bool use_graph = (flags & 2) && !per_batch;
     if (use_graph) {
@@ -462,6 +478,7 @@ int omni_gpu_agg_batches_destroy(void *handle)
     h->dcfg.Release(s);
     h->items.Release(s);
     h->parts.Release(s);
+    h->parts_used.Release(s);
     h->offsets.Release(s);
     h->tiles.Release(s);
     h->modes.Release(s);

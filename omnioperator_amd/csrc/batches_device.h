@@ -38,7 +38,9 @@ struct BatchConfig {
     LdsLayout L2;              // phase-2 LDS table (probe slots [0, L2.S), side slots L2.S, L2.S + 1)
     const int4 *items;         // phase-2 work items {batch, partition, partitions, 0}; batch -1: no work
     int64_t nitems;
-    const int32_t *parts;      // per batch: phase-2 partitions (a power of two, <= kMaxParts)
+    const int32_t *parts;      // per batch: phase-2 partition items launched (a power of two, <= kMaxParts)
+    int32_t *parts_used;       // per batch: partitions phase 2a chose (<= parts[b], from its distinct estimate)
+    unsigned int *redo;        // set when some phase-2 partition overflowed (phase 3 has work)
     int32_t hbm_flag;          // phase 3 aggregates the batches whose spilled flag equals this value
     int32_t pad2;
     unsigned int *spilled;     // per batch: 1 = phase 2 must aggregate it
@@ -216,35 +218,61 @@ __device__ __forceinline__ Buckets BucketsOf(const BatchConfig &c, int b, int P,
     return k;
 }
 
-// Phase 2a (one workgroup per batch; batches that phase 1 finished return): splits the batch's rows into its
-// P partitions -- LDS histogram, scan, then each row's key and row number written to its bucket.  Workgroup b
-// runs on the XCD of batch b's phase-2b items, so the buckets are read back from that XCD's L2.
+constexpr int kLcWords = 1024;  // linear-counting bitmap of phase 2a: 32,768 bits
+
+// Phase 2a (one workgroup per batch; batches that phase 1 finished return): splits the batch's rows into
+// partitions -- one pass builds an LDS histogram over the launched partitions (Pmax) and a linear-counting
+// bitmap of the keys; the distinct estimate picks the partitions used, P <= Pmax, so that a partition holds
+// at most 40% of the table's slots (P divides Pmax and partitions are the low hash bits, so P's histogram
+// is Pmax's folded); a second pass writes each row's key and row number to its bucket.  Workgroup b runs on
+// the XCD of batch b's phase-2b items, so the buckets are read back from that XCD's L2.
 template <int BLOCK>
-__device__ __forceinline__ void BatchSplit(const BatchConfig &c, int b, unsigned int *hist)
+__device__ __forceinline__ void BatchSplit(const BatchConfig &c, int b, unsigned int *hist, unsigned int *bits,
+                                           unsigned int *sh)
 {
     if (c.spilled[b] == 0u) return;
-    const int P = c.parts[b];
+    const int Pmax = c.parts[b];
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
-    const Buckets k = BucketsOf(c, b, P, end - start);
-    for (int i = threadIdx.x; i < P; i += BLOCK) hist[i] = 0;
+    for (int i = threadIdx.x; i < Pmax; i += BLOCK) hist[i] = 0;
+    for (int i = threadIdx.x; i < kLcWords; i += BLOCK) bits[i] = 0;
+    if (threadIdx.x == 0) sh[0] = 0;
     __syncthreads();
     for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
         const bool isNull = NullBit(c.kd.nulls[0], row);
         const unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
-        atomicAdd(&hist[PartOfKey(key, isNull, P)], 1u);
+        atomicAdd(&hist[PartOfKey(key, isNull, Pmax)], 1u);
+        const unsigned int bit = static_cast<unsigned int>(Mix64(key ^ (isNull ? 0x5bd1e995ULL : 0ULL)) >> 20) &
+                                 (kLcWords * 32 - 1);
+        atomicOr(&bits[bit >> 5], 1u << (bit & 31));
     }
     __syncthreads();
-    if (threadIdx.x == 0) {  // P <= kMaxParts: a serial scan is short
-        unsigned int run = 0;
+    unsigned int ones = 0;
+    for (int i = threadIdx.x; i < kLcWords; i += BLOCK) ones += __popc(bits[i]);
+    if (ones) atomicAdd(&sh[0], ones);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double m = kLcWords * 32.0;
+        const double zeros = m - static_cast<double>(sh[0]);
+        const double rows = static_cast<double>(end - start);
+        const double est = zeros >= 1.0 ? fmin(rows, -m * log(zeros / m)) : rows;
+        int P = 1;
+        while (P < Pmax && static_cast<double>(P) * 0.4 * c.L2.S < est) P <<= 1;
+        c.parts_used[b] = P;
+        const Buckets k = BucketsOf(c, b, P, end - start);
+        unsigned int run = 0;  // fold Pmax's histogram onto P partitions, then scan (P <= kMaxParts: short)
         for (int i = 0; i < P; ++i) {
-            const unsigned int v = hist[i];
+            unsigned int v = 0;
+            for (int j = i; j < Pmax; j += P) v += hist[j];
             hist[i] = run;
             k.start[i] = run;
             run += v;
         }
         k.start[P] = run;
+        sh[1] = static_cast<unsigned int>(P);
     }
     __syncthreads();
+    const int P = static_cast<int>(sh[1]);
+    const Buckets k = BucketsOf(c, b, P, end - start);
     for (int64_t row = start + threadIdx.x; row < end; row += BLOCK) {
         const bool isNull = NullBit(c.kd.nulls[0], row);
         const unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
@@ -263,7 +291,8 @@ __device__ __forceinline__ void BatchPart(const BatchConfig &c, int64_t it, uint
     const int4 item = c.items[it];
     const int b = item.x;
     if (b < 0 || c.spilled[b] == 0u) return;
-    const int p = item.y, P = item.z;
+    const int p = item.y, P = c.parts_used[b];
+    if (p >= P) return;  // phase 2a chose fewer partitions than launched
     const LdsLayout &L = c.L2;
     const int S = L.S;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
@@ -315,7 +344,10 @@ __device__ __forceinline__ void BatchPart(const BatchConfig &c, int64_t it, uint
     }
     __syncthreads();
     if (sh[0]) {
-        if (threadIdx.x == 0) c.spilled[b] = 2u;
+        if (threadIdx.x == 0) {
+            c.spilled[b] = 2u;
+            atomicOr(c.redo, 1u);
+        }
         return;
     }
     unsigned int mine = 0;
@@ -342,9 +374,9 @@ __device__ __forceinline__ int TableSlot(unsigned long long *keys, unsigned long
     }
 }
 
-constexpr int kHbmTile = 4096;  // rows (and table slots) per phase-2 workgroup
+constexpr int kHbmTile = 4096;  // rows (and table slots) per phase-3 tile
 
-// Batch owning phase-2 tile `t` (binary search over the tile prefix of all batches).
+// Batch owning phase-3 tile `t` (binary search over the tile prefix of all batches).
 __device__ __forceinline__ int TileBatch(const BatchConfig &c, int64_t t)
 {
     int lo = 0, hi = c.nbatches - 1;
@@ -356,7 +388,7 @@ __device__ __forceinline__ int TileBatch(const BatchConfig &c, int64_t t)
     return lo;
 }
 
-// Phase 2a: tile t of a flagged batch initialises its share of the batch's table slots (and side slots).
+// Phase 3a: tile t of a flagged batch initialises its share of the batch's table slots (and side slots).
 template <int BLOCK>
 __device__ __forceinline__ void BatchHbmInit(const BatchConfig &c, int64_t t)
 {
@@ -377,7 +409,7 @@ __device__ __forceinline__ void BatchHbmInit(const BatchConfig &c, int64_t t)
     }
 }
 
-// Phase 2b: tile t of a flagged batch inserts its kHbmTile rows (load factor <= 1/2: every probe ends).
+// Phase 3b: tile t of a flagged batch inserts its kHbmTile rows (load factor <= 1/2: every probe ends).
 template <int BLOCK>
 __device__ __forceinline__ void BatchHbmInsert(const BatchConfig &c, int64_t t)
 {
@@ -407,7 +439,7 @@ __device__ __forceinline__ void BatchHbmInsert(const BatchConfig &c, int64_t t)
     }
 }
 
-// Phase 2c: emits the groups of flagged batch b (a later launch, so every insert is visible).
+// Phase 3c: emits the groups of flagged batch b (a later launch, so every insert is visible).
 template <int BLOCK>
 __device__ __forceinline__ void BatchHbmEmit(const BatchConfig &c, int b, unsigned int *sh)
 {
