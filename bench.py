@@ -100,9 +100,9 @@ def bench_join(args):
     avg_probe_ms = probe_ms / args.steps
     probe_bytes = npr * 8 + m * (8 + 4)
     achieved = probe_bytes / (avg_probe_ms / 1e3) / 1e9
-    variant = (f"join_probe<{'true' if is_array[0] else 'false'}, {'false' if is_array[0] else 'true'}, false, "
-               f"{args.probe_group}>") if parts[0] <= 1 else None
-    traffic = profiled_traffic(variant, "dense" if args.dense else "sparse") if variant else None
+    # the profiled configuration is the default one: sparse keys, automatic partitioning (1024 partitions)
+    profiled = not args.dense and parts[0] == 1024 and (nb, npr) == (2_000_000, 64_000_000)
+    traffic = profiled_traffic("probe stage (chain)") if profiled else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "join_probe" if parts[0] <= 1 else
@@ -294,16 +294,16 @@ def bench_batched(args):
             "dtype": "int64"}
 
 
-def profiled_traffic(kernel: str, workload: str):
-    """HBM bytes per launch of `kernel` on `workload` from the rocprofv3 FETCH_SIZE / WRITE_SIZE summary in
-    profiles/join_probe_c2.md (copy-calibrated); None when that kernel variant was not profiled."""
+def profiled_traffic(row_name: str):
+    """HBM bytes per launch of the probe stage from the copy-calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE table in
+    profiles/join_probe_c2.md (the row named `row_name`); None when it was not profiled."""
     path = os.path.join(ROOT, "profiles", "join_probe_c2.md")
     try:
         with open(path) as f:
             for line in f:
                 cells = [c.strip() for c in line.strip().strip("|").split("|")]
-                if len(cells) == 6 and cells[0] == kernel and cells[1] == workload:
-                    return int(cells[4])
+                if len(cells) == 4 and cells[0] == row_name and cells[3].isdigit():
+                    return int(cells[3])
     except OSError:
         pass
     return None
