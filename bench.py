@@ -294,6 +294,52 @@ def bench_batched(args):
             "dtype": "int64"}
 
 
+def bench_learned(args):
+    """The learned advisor on the C5 task set: 512 seeded group-by and join tasks (rows log-uniform in
+    [1e5, 2e8]), each run with every fixed strategy, the library's automatic rule and the advisor."""
+    from omnioperator_amd import capi, learned
+    capi.init(0)
+    tasks = learned.sample_tasks(args.learned_tasks, seed=0xC5, min_rows=1e5, max_rows=args.learned_max_rows,
+                                 join_share=0.5)
+    adv = learned.Advisor()
+    t0 = time.perf_counter()
+    secs, rows, recs = learned.evaluate(tasks, adv, log=lambda m: print(m, file=sys.stderr, flush=True))
+    wall = time.perf_counter() - t0
+    adv.close()
+    if args.learned_records:
+        with open(args.learned_records, "w") as f:
+            json.dump(recs, f)
+    by_op = {op: {k: round(sum(r[k] for r in recs if r["op"] == op), 4) for k in learned.CONFIGS}
+             for op in ("agg", "join")}
+    fixed = {k: secs[k] for k in ("lds", "global", "radix")}
+    best_fixed = min(fixed, key=fixed.get)
+    alg = 0
+    for r in recs:
+        if r["op"] == "join":
+            alg += int(r["distinct"]) * 16 + r["rows"] * 8
+        else:
+            alg += r["rows"] * 16 + r["groups"] * 24
+    achieved = alg / secs["advisor"] / 1e9
+    n_join = sum(1 for r in recs if r["op"] == "join")
+    cfg = {"workload": f"learned strategy/capacity: {len(tasks)} seeded tasks ({len(tasks) - n_join} group-by, {n_join} "
+                       f"inner join), rows log-uniform [1e5, {args.learned_max_rows:.0e}], distinct log-uniform "
+                       "[10, 1e8] capped at rows, Zipf s U[0, 1.5], keys int32 / int64 / (int32, int64)",
+           "tasks": len(tasks), "rows": rows, "seconds": {k: round(v, 4) for k, v in secs.items()},
+           "best_fixed_strategy": best_fixed,
+           "speedup_vs_best_fixed": round(secs[best_fixed] / secs["advisor"], 3),
+           "speedup_vs_each_fixed": {k: round(v / secs["advisor"], 3) for k, v in fixed.items()},
+           "speedup_vs_auto_rule": round(secs["auto"] / secs["advisor"], 3),
+           "seconds_by_operator": by_op,
+           "advisor_plan_seconds": round(sum(r.get("plan", 0.0) for r in recs), 4),
+           "wall_seconds_incl_generation_and_checks": round(wall, 1),
+           "algorithmic_bytes": alg}
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "whole task set (advisor)",
+            "kernel_ms": round(secs["advisor"] * 1e3, 3), "algorithmic_bytes": alg}
+    return {"value": rows / secs["advisor"], "ms_per_step": secs["advisor"] * 1e3, "config": cfg, "roofline": roof,
+            "cpu_baseline": None, "dtype": "int64"}
+
+
 def profiled_traffic(row_name: str):
     """HBM bytes per launch of the probe stage from the copy-calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE table in
     profiles/join_probe_c2.md (the row named `row_name`); None when it was not profiled."""
@@ -330,12 +376,15 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="join", choices=["join", "agg", "skewed", "batched"],
+    p.add_argument("--workload", default="join", choices=["join", "agg", "skewed", "batched", "learned"],
                    help="join: the headline inner-join configuration; agg: 1M-row plain group-by; skewed: 200M-row "
-                        "composite-key group-by; batched: 20,000-batch stream")
+                        "composite-key group-by; batched: 20,000-batch stream; learned: the 512-task advisor set")
     p.add_argument("--skewed-rows", type=int, default=200_000_000)
     p.add_argument("--cpu-skewed-rows", type=int, default=4_000_000)
     p.add_argument("--batched-form", default="graph", choices=["graph", "two_launches", "per_batch"])
+    p.add_argument("--learned-tasks", type=int, default=512)
+    p.add_argument("--learned-max-rows", type=float, default=2e8)
+    p.add_argument("--learned-records", default=None, help="write the per-task records of --workload learned here")
     p.add_argument("--agg-form", default="batches", choices=["batches", "per_batch"],
                    help="plain group-by: all batches in one add_batches call, or one add_batch per batch")
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
@@ -363,9 +412,10 @@ def main():
               "described in DESIGN.md and is not run.", file=sys.stderr)
         sys.exit(2)
     res = {"join": bench_join, "agg": bench_agg_plain, "skewed": bench_agg_skewed,
-           "batched": bench_batched}[args.workload](args)
-    line = {"metric": METRIC, "value": res["value"], "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
-            "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+           "batched": bench_batched, "learned": bench_learned}[args.workload](args)
+    single = args.workload == "learned"  # one pass over the task set; each task warmed by an untimed run
+    line = {"metric": METRIC, "value": res["value"], "unit": "rows/s", "n_gpus": 1,
+            "steps": 1 if single else args.steps, "warmup": 0 if single else args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": res["dtype"], "data": "synthetic (seeded)", "config": res["config"],
             "roofline": res["roofline"], "cpu_baseline": res["cpu_baseline"]}
     print(json.dumps(line), flush=True)

@@ -215,10 +215,11 @@ int omni_gpu_join_build_finish(void *handle);
 int omni_gpu_join_build_info(void *handle, int32_t *is_array, int64_t *max_dup, int64_t *table_slots);
 /* Probe tuning (results do not depend on either knob).
  *   probe_group: table lookups each lane issues together before resolving any (1, 2, 4, 8 or 16; default 1).
- *   partitions:  radix-partitioned probe of hash tables: -1 = automatic (tables above 256 MiB probed with at
- *                least 4M rows are cut into regions of about 2 MiB), 0 = never, or a power of two in
- *                [8, 1024].  Probe rows are scattered into partition order and each XCD probes its own
- *                partitions, so table reads hit its L2.
+ *   partitions:  radix-partitioned probe of hash tables: -1 = automatic (tables of at least 8 MiB probed
+ *                with at least half as many rows as the table has slots are cut into regions of 8,192 slots),
+ *                -2 = always with that region size, 0 = never, or a power of two in [8, 4096].  Probe rows
+ *                are scattered into partition order; a region of <= 8,192 slots is staged in LDS by each
+ *                workgroup that probes it, a larger one is probed from the L2 of the XCD that owns it.
  * No reference counterpart: the CPU probe loop (lookup_join.cpp:924-1035) looks rows up one at a time, and
  * its partitions (hash_builder.cpp:155-160) split the build side across operator instances instead. */
 int omni_gpu_join_tune(void *handle, int32_t probe_group, int32_t partitions);
@@ -259,7 +260,8 @@ typedef struct omni_gpu_sketch {
     int64_t key_min, key_max;  /* range of the first key column over the sample */
 } omni_gpu_sketch;
 
-/* Sketch of one or two integer-like key columns of `rows` rows (device columns). */
+/* Sketch of one or two integer-like key columns of `rows` rows (device columns): evenly strided sample of 1% of
+ * the rows, at least 4,096 (all rows below that) and at most 65,536. */
 int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
                          omni_gpu_sketch *out);
 /* Sketch + MLP inference in one call on the MLP handle's stream (one synchronisation): y[4] receives the network's
@@ -282,9 +284,11 @@ int omni_gpu_mlp_gradient(void *handle, const float *X, const float *t, const in
 int omni_gpu_mlp_train(void *handle, const float *X, const float *t, const int32_t *c, int32_t n, int32_t batch,
                        int32_t epochs, float lr, uint64_t seed, float *epoch_loss);
 int omni_gpu_mlp_destroy(void *handle);
-/* Seeded synthetic keys on the device (the advisor's training runs): `rows` keys whose rank follows a Zipf(s)
- * law over `distinct` ranks (s = 0: uniform), mapped through a bijection.  kind 0: int64 into k64; 1: int32
- * into k32; 2: (int32, int64) pairs into (k32, k64).  v64 (optional): int64 values in [-1e6, 1e6]. */
+/* Seeded synthetic keys on the device (the advisor's training and evaluation runs): `rows` keys whose rank
+ * follows a Zipf(s) law over `distinct` ranks (s = 0: uniform), mapped through a bijection.  kind & 7: 0 int64
+ * into k64; 1 int32 into k32; 2 (int32, int64) pairs into (k32, k64).  kind & 8: row i gets rank i + 1 (a join
+ * build side of unique keys); kind & 16: about 20% of rows get keys outside the rank range (probe misses).
+ * v64 (optional): int64 values in [-1e6, 1e6]. */
 int omni_gpu_generate_keys(int32_t kind, int64_t rows, double distinct, double zipf_s, uint64_t seed, void *k64,
                            void *k32, void *v64);
 

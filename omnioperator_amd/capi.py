@@ -591,16 +591,21 @@ class Mlp:
             pass
 
 
+GEN_SEQUENTIAL = 8   # row i gets rank i + 1 (unique keys: a join build side)
+GEN_MISSES = 16      # about 20% of rows get keys outside the rank range (join probe misses)
+
+
 def generate_keys(kind: int, rows: int, distinct: float, zipf_s: float, seed: int, with_values: bool = True):
-    """Device key columns from the seeded generator: kind 0 int64, 1 int32, 2 (int32, int64); returns
-    ([key DeviceArrays...], value DeviceArray or None)."""
+    """Device key columns from the seeded generator: kind & 7 = 0 int64, 1 int32, 2 (int32, int64), optionally
+    | GEN_SEQUENTIAL / GEN_MISSES; returns ([key DeviceArrays...], value DeviceArray or None)."""
     init()
-    k64 = DeviceArray(max(1, rows) * 8, np.int64, (rows,)) if kind != 1 else None
-    k32 = DeviceArray(max(1, rows) * 4, np.int32, (rows,)) if kind != 0 else None
+    t = kind & 7
+    k64 = DeviceArray(max(1, rows) * 8, np.int64, (rows,)) if t != 1 else None
+    k32 = DeviceArray(max(1, rows) * 4, np.int32, (rows,)) if t != 0 else None
     v = DeviceArray(max(1, rows) * 8, np.int64, (rows,)) if with_values else None
     check(lib().omni_gpu_generate_keys(kind, rows, float(distinct), float(zipf_s), seed, k64.ptr if k64 else None,
                                        k32.ptr if k32 else None, v.ptr if v else None))
-    keys = [k64] if kind == 0 else ([k32] if kind == 1 else [k32, k64])
+    keys = [k64] if t == 0 else ([k32] if t == 1 else [k32, k64])
     return keys, v
 
 

@@ -73,12 +73,14 @@ struct AggHandle {
     unsigned long long *spills() const { return reinterpret_cast<unsigned long long *>(counters.as<uint8_t>() + 24); }
     unsigned int *live() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 32); }
 
-    // Ids a wave takes at a time: about the groups seen so far over the wave slots, a power of two in [1, 512],
-    // so that small group counts leave no unused reserved ids and large ones need few counter atomics.
+    // Ids a wave takes at a time: about the groups expected (seen so far, or the capacity hint if larger) over
+    // the wave slots, a power of two in [1, 512], so that small group counts leave no unused reserved ids and
+    // large ones need few counter atomics -- also before the first synchronisation of a presized table.
     unsigned long long Chunk() const
     {
+        const int64_t expect = std::max<int64_t>(known_ids, spec.capacity_hint);
         unsigned long long c = 1;
-        while (c < 512 && static_cast<int64_t>(c) * 2 * kCursorSlots <= known_ids) c <<= 1;
+        while (c < 512 && static_cast<int64_t>(c) * 2 * kCursorSlots <= expect) c <<= 1;
         return c;
     }
 

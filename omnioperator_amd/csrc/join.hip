@@ -72,7 +72,11 @@ int ChoosePartitions(const JoinHandle &h, int64_t rows)
     if (h.partitions > 0) {
         P = h.partitions;
     } else {
-        if (h.slots_n * sizeof(JoinSlot) < kAutoPartBytes || static_cast<uint64_t>(rows) * 2 < h.slots_n) return 1;
+        // -1 (automatic): only tables that pay for it; -2: always, with the automatic region size
+        if (h.partitions == -1 &&
+            (h.slots_n * sizeof(JoinSlot) < kAutoPartBytes || static_cast<uint64_t>(rows) * 2 < h.slots_n)) {
+            return 1;
+        }
         P = 8;
         while (P < kPartMax && h.slots_n / static_cast<uint64_t>(P) > static_cast<uint64_t>(kWinSlots)) P <<= 1;
     }
@@ -489,10 +493,10 @@ int omni_gpu_join_tune(void *handle, int32_t probe_group, int32_t partitions)
     OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
     OG_CHECK(probe_group == 1 || probe_group == 2 || probe_group == 4 || probe_group == 8 || probe_group == 16,
              OMNI_GPU_ERR_INVALID, "probe_group must be 1, 2, 4, 8 or 16");
-    OG_CHECK(partitions == -1 || partitions == 0 ||
+    OG_CHECK(partitions == -2 || partitions == -1 || partitions == 0 ||
                  (partitions >= kXcds && partitions <= kPartMax && (partitions & (partitions - 1)) == 0),
-             OMNI_GPU_ERR_INVALID, "partitions must be -1 (auto), 0 (off) or a power of two in [%d, %d]", kXcds,
-             kPartMax);
+             OMNI_GPU_ERR_INVALID,
+             "partitions must be -1 (auto), -2 (always), 0 (off) or a power of two in [%d, %d]", kXcds, kPartMax);
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     h.probe_group = probe_group;
     h.partitions = partitions;

@@ -2,7 +2,8 @@
 //
 // The plain path sizes hash tables by growth (grow x4 and rehash, as BaseHashMap's Grower does,
 // core/src/operator/hashmap/base_hash_map.h:255-292) and picks its table kind by a fixed rule.  The advisor
-// predicts the group count and the fastest strategy from a 1% sample sketch, so the table can be sized once and
+// predicts the group count and the fastest strategy from a sample sketch (1% of the rows, at least 4,096 and at
+// most 65,536 of them), so the table can be sized once and
 // the kernel chosen up front.  Results never depend on it: a prediction that is too small only costs a growth.
 #include <algorithm>
 #include <cmath>
@@ -28,7 +29,7 @@ struct SketchBuffers {
     }
 };
 
-constexpr int64_t kMaxSample = 1 << 21;
+constexpr int64_t kMaxSample = 1 << 16;  // sampled rows at most: the sketch's cost stays flat
 
 // Launches the sketch of `keys` on stream s: a 1-in-k sample of about 1% of the rows (at least 4096 or all of
 // them, at most 2^21) counted in a hash table, reduced to statistics and turned into the 16 MLP features.
@@ -66,7 +67,7 @@ int RunSketch(hipStream_t s, SketchBuffers &b, const omni_gpu_column *keys, int 
     init.kmax = std::numeric_limits<long long>::min();
     OG_HIP(hipMemsetAsync(b.hkeys.p, 0, 8 * cap, s));
     OG_HIP(hipMemsetAsync(b.counts.p, 0, 4 * cap, s));
-    OG_HIP(hipMemcpyAsync(b.stats.p, &init, sizeof(init), hipMemcpyHostToDevice, s));
+    OG_TRY(CopyToDevice(b.stats.p, &init, sizeof(init), s));
     a.hkeys = b.hkeys.as<unsigned long long>();
     a.counts = b.counts.as<unsigned int>();
     a.mask = cap - 1;
@@ -183,8 +184,7 @@ int omni_gpu_advise(void *mlp, const omni_gpu_column *keys, int32_t num_keys, in
         mlp_forward<<<1, kMlpFwdBlock, 0, s>>>(h.params.as<float>(), h.sketch.features.as<float>(), 1, h.Y.as<float>());
         OG_TRY(CheckLaunch("mlp_forward"));
     }
-    OG_HIP(hipMemcpyAsync(y, h.Y.p, sizeof(float) * kOut, hipMemcpyDeviceToHost, s));
-    OG_HIP(hipStreamSynchronize(s));
+    OG_TRY(CopyToHost(s, {{y, h.Y.p, sizeof(float) * kOut}}));
     return OMNI_GPU_OK;
 }
 
@@ -331,9 +331,10 @@ int omni_gpu_mlp_destroy(void *handle)
 int omni_gpu_generate_keys(int32_t kind, int64_t rows, double distinct, double zipf_s, uint64_t seed, void *k64,
                            void *k32, void *v64)
 {
-    OG_CHECK(kind >= 0 && kind <= 2 && rows >= 0 && distinct >= 1.0 && zipf_s >= 0.0, OMNI_GPU_ERR_INVALID,
+    const int32_t type = kind & 7;
+    OG_CHECK(type <= 2 && (kind & ~31) == 0 && rows >= 0 && distinct >= 1.0 && zipf_s >= 0.0, OMNI_GPU_ERR_INVALID,
              "bad generator arguments");
-    OG_CHECK((kind == 1 || k64) && (kind == 0 || k32), OMNI_GPU_ERR_INVALID, "missing output column");
+    OG_CHECK((type == 1 || k64) && (type == 0 || k32), OMNI_GPU_ERR_INVALID, "missing output column");
     if (rows == 0) return OMNI_GPU_OK;
     hipStream_t s;
     OG_TRY(AcquireStream(&s));

@@ -373,7 +373,11 @@ __global__ void gen_keys(int kind, int64_t rows, double D, double s, unsigned lo
     unsigned long long r = static_cast<unsigned long long>(x);
     if (r < 1) r = 1;
     if (r > static_cast<unsigned long long>(D)) r = static_cast<unsigned long long>(D);
-    const unsigned long long key = SplitMix(r + (seed << 20));
+    if (kind & 8) r = static_cast<unsigned long long>(i) + 1;  // sequential ranks: row i is rank i + 1
+    unsigned long long in = r + (seed << 20);
+    if ((kind & 16) && (u0 & 1023ULL) < 205ULL) in += 1ULL << 62;  // ~20% of rows outside the rank range
+    const unsigned long long key = SplitMix(in);
+    kind &= 7;
     if (kind == 0) {
         k64[i] = static_cast<long long>(key);
     } else if (kind == 1) {

@@ -1,16 +1,22 @@
-"""Learned strategy / capacity advisor for the GPU group-by.
+"""Learned strategy / capacity advisor for the GPU group-by and join.
 
-A 1% sample sketch of the key columns (omni_gpu_sketch_keys) feeds a 16-64-64-4 fp32 MLP (omni_gpu_mlp_*, trained
-and run by the library's own HIP kernels).  Output 0 predicts log2 of the group count, which sizes the hash table
-once instead of growing it x4 several times; outputs 1..3 pick the strategy (LDS-local, global-atomic,
-radix-partitioned).  The plain method is the fallback and the judge: a capacity that is too small only costs a
-grow-and-rehash, and every strategy gives the same groups, so results never depend on the prediction.
+A sample sketch of the key columns (omni_gpu_sketch_keys: 1% of the rows, 4,096 to 65,536 of them) feeds a 16-64-64-4 fp32 MLP (omni_gpu_mlp_*, trained
+and run by the library's own HIP kernels).  Output 0 predicts log2 of the group count, which sizes a group-by's
+hash table once instead of growing it x4 several times; outputs 1..3 pick the strategy:
+  class 0  LDS-local       group-by: LDS partial tables merged into the global table;
+  class 1  global-atomic   group-by: global table only; join: direct probe;
+  class 2  radix-partitioned  join: partitioned probe with LDS-staged table regions.
+A group-by chooses between classes 0 and 1 (it has no radix kernel), a join between 1 and 2 (its table is sized
+from the exact build row count at finish, so only the strategy is predicted).  The plain method is the fallback
+and the judge: a capacity that is too small only costs a grow-and-rehash, and every strategy gives the same
+result rows, so results never depend on the prediction.
 
-Training data come from runs of the aggregation itself over seeded key distributions generated on the device
+Training data come from runs of the operators themselves over seeded key distributions generated on the device
 (omni_gpu_generate_keys): for each run the sketch features, the true group count and the fastest strategy.
 """
 from __future__ import annotations
 
+import ctypes as C
 import math
 import os
 import time
@@ -23,7 +29,7 @@ from . import capi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "models", "advisor_mlp.npz")
-STRATEGIES = (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL)   # classes 0 and 1; class 2 (radix) has no agg kernel yet
+STRATEGIES = (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL)   # group-by kernels of classes 0 and 1
 KIND_TYPES = {0: [capi.LONG], 1: [capi.INT], 2: [capi.INT, capi.LONG]}
 T_SCALE = 32.0          # the regression target is log2(groups + 1) / T_SCALE
 CAPACITY_MARGIN = 1.25  # capacity hint = predicted groups x margin
@@ -31,24 +37,32 @@ CAPACITY_MARGIN = 1.25  # capacity hint = predicted groups x margin
 
 @dataclass
 class Task:
-    kind: int          # 0 int64 key, 1 int32 key, 2 (int32, int64) key
-    rows: int
-    distinct: float    # ranks of the Zipf law (an upper bound of the group count)
+    kind: int          # 0 int64 key, 1 int32 key, 2 (int32, int64) key (group-by only)
+    rows: int          # group-by rows, or join probe rows
+    distinct: float    # ranks of the Zipf law (an upper bound of the group count), or join build rows
     zipf_s: float
     seed: int
+    op: str = "agg"    # "agg" (SUM + COUNT(*) GROUP BY key) or "join" (inner join, unique build keys)
 
 
 def sample_tasks(n: int, seed: int, min_rows: float = 1e5, max_rows: float = 2e8, min_distinct: float = 10,
-                 max_distinct: float = 1e8) -> List[Task]:
+                 max_distinct: float = 1e8, join_share: float = 0.0) -> List[Task]:
     """Tasks of the advisor's workload: rows and distinct counts log-uniform, Zipf s uniform in [0, 1.5],
-    key width int32, int64 or (int32, int64); the distinct count is capped at the rows."""
+    key width int32, int64 or (int32, int64); the distinct count is capped at the rows.  A share `join_share`
+    of the tasks are joins (build = `distinct` unique keys, probe = `rows` keys by the Zipf law over them with
+    about 20% misses); the join has single keys, so a drawn (int32, int64) width becomes int64."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
         rows = int(math.exp(rng.uniform(math.log(min_rows), math.log(max_rows))))
         d = min(float(rows), math.exp(rng.uniform(math.log(min_distinct), math.log(max_distinct))))
-        out.append(Task(int(rng.integers(0, 3)), rows, max(1.0, math.floor(d)), float(rng.uniform(0.0, 1.5)),
-                        int(rng.integers(1, 2**62))))
+        kind = int(rng.integers(0, 3))
+        s = float(rng.uniform(0.0, 1.5))
+        sd = int(rng.integers(1, 2**62))
+        op = "join" if rng.random() < join_share else "agg"
+        if op == "join" and kind == 2:
+            kind = 0
+        out.append(Task(kind, rows, max(1.0, math.floor(d)), s, sd, op))
     return out
 
 
@@ -61,24 +75,67 @@ def _col(type_id: int, arr: capi.DeviceArray, offset: int = 0) -> capi.Column:
 
 
 class TaskData:
-    """Device columns of one task (keys + an int64 value column)."""
+    """Device columns of one task: group-by keys + an int64 value column, or a join's probe keys (`keys`) and
+    build keys + int64 payload (`build_keys`, `build_values`)."""
 
     def __init__(self, task: Task):
         self.task = task
-        self.keys, self.values = capi.generate_keys(task.kind, task.rows, task.distinct, task.zipf_s, task.seed)
         self.types = KIND_TYPES[task.kind]
+        if task.op == "join":
+            nb = int(task.distinct)
+            self.build_keys, self.build_values = capi.generate_keys(task.kind | capi.GEN_SEQUENTIAL, nb,
+                                                                    task.distinct, 0.0, task.seed)
+            self.keys, self.values = capi.generate_keys(task.kind | capi.GEN_MISSES, task.rows, task.distinct,
+                                                        task.zipf_s, task.seed, with_values=False)
+        else:
+            self.keys, self.values = capi.generate_keys(task.kind, task.rows, task.distinct, task.zipf_s, task.seed)
 
     def key_cols(self, offset: int = 0):
         return [_col(t, k, offset) for t, k in zip(self.types, self.keys)]
 
     def sketch(self) -> capi.Sketch:
-        return capi.sketch_keys(self.key_cols(), self.task.rows)
+        return capi.sketch_keys(self.key_cols(), self.task.rows, self.task.op == "join")
+
+    def features(self) -> np.ndarray:
+        """The advisor's 16 inputs: the device sketch of a group-by's keys; a join's exact sizes (join_features)."""
+        if self.task.op == "join":
+            return join_features(self.task.rows, int(self.task.distinct), 4 if self.task.kind == 1 else 8)
+        return np.frombuffer(bytes(self.sketch().features), np.float32).copy()
+
+
+def join_features(probe_rows: int, build_rows: int, key_bytes: int) -> np.ndarray:
+    """Advisor inputs of a join, from sizes the operator knows exactly when the build side is finished (no
+    sampling pass): log probe rows, log build rows (the table's keys), their ratio, key width, join flag; the
+    slots of the group-by sketch statistics stay 0.  Same scaling as sketch_features."""
+    f = np.zeros(16, np.float32)
+    f[0] = math.log2(probe_rows + 1) / 32
+    f[1] = math.log2(build_rows + 1) / 32
+    f[2] = math.log2(build_rows + 1) / 32
+    f[7] = min(1.0, build_rows / max(1, probe_rows))
+    f[12] = key_bytes / 16
+    f[13] = 1 / 4
+    f[14] = 1.0
+    f[15] = 1.0
+    return f
 
 
 AGGS = [(capi.SUM, capi.LONG, capi.LONG), (capi.COUNT_ALL, capi.LONG, capi.LONG)]
+CLASS_LDS, CLASS_GLOBAL, CLASS_RADIX = 0, 1, 2
+AGG_CLASSES = (CLASS_LDS, CLASS_GLOBAL)
+JOIN_CLASSES = (CLASS_GLOBAL, CLASS_RADIX)
+BATCH_ROWS = 1 << 22
 
 
-def run_agg(data: TaskData, strategy: int, capacity_hint: int, batch_rows: int = 1 << 22, output: bool = False):
+def _checksum(cols, n) -> int:
+    """Order-independent checksum of n result rows (sum of per-row mixes of every column's values)."""
+    acc = np.zeros(n, np.uint64)
+    for v in cols:
+        x = np.asarray(v[:n]).astype(np.int64).view(np.uint64)
+        acc = acc * np.uint64(0x9E3779B97F4A7C15) ^ x
+    return int(np.sum(acc * np.uint64(0xBF58476D1CE4E5B9), dtype=np.uint64))
+
+
+def run_agg(data: TaskData, strategy: int, capacity_hint: int, batch_rows: int = BATCH_ROWS, output: bool = False):
     """Runs SUM(value), COUNT(*) GROUP BY key over the task in VectorBatches of batch_rows rows; returns
     (groups, seconds[, outputs])."""
     spec = capi.make_agg_spec(data.types, AGGS, True, strategy, int(capacity_hint))
@@ -96,10 +153,69 @@ def run_agg(data: TaskData, strategy: int, capacity_hint: int, batch_rows: int =
     return (g, dt, outs) if output else (g, dt)
 
 
+def run_agg_full(data: TaskData, strategy: int, capacity_hint: int, bufs):
+    """As run_agg, with every group written to the preallocated device outputs `bufs` (capi.HashAgg.alloc_output
+    of at least the group count) inside the timed region; returns (groups, seconds)."""
+    spec = capi.make_agg_spec(data.types, AGGS, True, strategy, int(capacity_hint))
+    capi.synchronize()
+    t0 = time.perf_counter()
+    h = capi.HashAgg(spec)
+    n = data.task.rows
+    for a in range(0, n, BATCH_ROWS):
+        h.add_batch(data.key_cols(a), [_col(capi.LONG, data.values, a), None], min(BATCH_ROWS, n - a))
+    g = h.num_groups()
+    h.output_into(bufs, g)
+    capi.synchronize()
+    dt = time.perf_counter() - t0
+    h.close()
+    return g, dt
+
+
+def join_partitions(strategy: int) -> int:
+    """omni_gpu_join_tune partitions of a join strategy class: direct probe, or always partitioned."""
+    return -2 if strategy == CLASS_RADIX else 0
+
+
+def run_join(data: TaskData, strategy: Optional[int], outs=None):
+    """Inner join of the task (build = its unique keys with an int64 payload, probe = its keys), probe batches of
+    BATCH_ROWS rows, all probed in one call; returns (matches, seconds, outs) where outs = (row ids, payload
+    buffers, ctypes outputs, capacity) can be passed back to reuse the output buffers."""
+    kt = data.types[0]
+    j = capi.HashJoin(kt, [capi.LONG])
+    nb, n = int(data.task.distinct), data.task.rows
+    if outs is None:
+        cap = max(1, n)
+        outs = j._outs(cap, False) + (cap,)
+    out_rows, pays, oarr, cap = outs
+    spans = [(a, min(n, a + BATCH_ROWS)) for a in range(0, n, BATCH_ROWS)]
+    karr = (capi.Column * max(1, len(spans)))(*[_col(kt, data.keys[0], a) for a, _ in spans])
+    rarr = (C.c_int64 * max(1, len(spans)))(*[b - a for a, b in spans])
+    capi.synchronize()
+    t0 = time.perf_counter()
+    j.tune(1, -1 if strategy is None else join_partitions(strategy))  # None: the library's automatic rule
+    for a in range(0, nb, BATCH_ROWS):
+        j.add_build(_col(kt, data.build_keys[0], a), [_col(capi.LONG, data.build_values, a)], min(BATCH_ROWS, nb - a))
+    j.finish()
+    m = j.probe_batches_raw(karr, rarr, out_rows, oarr, cap)
+    capi.synchronize()
+    dt = time.perf_counter() - t0
+    j.close()
+    return m, dt, outs
+
+
 def label_task(data: TaskData, repeats: int = 2) -> Tuple[int, int, List[float]]:
-    """(true groups, fastest strategy class, seconds per class).  A first run measures the group count; each
-    strategy is then timed with that exact capacity, so that the label reflects the kernel choice and not the
-    growth schedule."""
+    """(true groups, fastest strategy class, seconds per class of the operator).  A group-by's first run measures
+    the group count; each strategy is then timed with that exact capacity, so that the label reflects the kernel
+    choice and not the growth schedule.  A join's "groups" is its build row count."""
+    if data.task.op == "join":
+        times, outs = [], None
+        for s in JOIN_CLASSES:
+            best = float("inf")
+            for _ in range(repeats):
+                _, dt, outs = run_join(data, s, outs)
+                best = min(best, dt)
+            times.append(best)
+        return int(data.task.distinct), JOIN_CLASSES[int(np.argmin(times))], times
     groups, _ = run_agg(data, capi.STRATEGY_GLOBAL, 0)
     times = []
     for s in STRATEGIES:
@@ -111,7 +227,7 @@ def label_task(data: TaskData, repeats: int = 2) -> Tuple[int, int, List[float]]
     return groups, int(np.argmin(times)), times
 
 
-def collect(tasks: Sequence[Task], repeats: int = 2):
+def collect(tasks: Sequence[Task], repeats: int = 2, log=None):
     """Features, regression targets and class labels of runs over `tasks` (the training set)."""
     X = np.zeros((len(tasks), 16), np.float32)
     t = np.zeros(len(tasks), np.float32)
@@ -119,11 +235,13 @@ def collect(tasks: Sequence[Task], repeats: int = 2):
     groups = np.zeros(len(tasks), np.int64)
     for i, task in enumerate(tasks):
         data = TaskData(task)
-        X[i] = np.frombuffer(bytes(data.sketch().features), np.float32)
+        X[i] = data.features()
         g, cls, _ = label_task(data, repeats)
         groups[i] = g
         t[i] = math.log2(g + 1) / T_SCALE
         c[i] = cls
+        if log and (i + 1) % 256 == 0:
+            log(f"collected {i + 1}/{len(tasks)} runs")
     return X, t, c, groups
 
 
@@ -159,19 +277,114 @@ class Advisor:
         self.mlp = capi.Mlp(0)
         self.mlp.set_params(params)
 
+    @staticmethod
+    def _classes(y: np.ndarray, is_join: np.ndarray) -> np.ndarray:
+        """Strategy class per row: argmax over the classes the operator has (group-by 0/1, join 1/2)."""
+        agg = np.argmax(y[:, 1:3], axis=1)
+        join = 1 + np.argmax(y[:, 2:4], axis=1)
+        return np.where(is_join, join, agg)
+
     def predict(self, features: np.ndarray):
-        y = self.mlp.forward(np.asarray(features, np.float32).reshape(-1, 16))
+        X = np.asarray(features, np.float32).reshape(-1, 16)
+        y = self.mlp.forward(X)
         groups = np.exp2(np.clip(y[:, 0], 0.0, 1.0) * T_SCALE) - 1.0
-        cls = np.argmax(y[:, 1:3], axis=1)  # the radix class has no aggregation kernel yet
-        return groups, cls
+        return groups, self._classes(y, X[:, 14] > 0.5)
+
+    def plan_joins(self, tasks: Sequence[Task]) -> List[int]:
+        """Strategy classes of many joins from their exact sizes, in one inference call."""
+        X = np.stack([join_features(t.rows, int(t.distinct), 4 if t.kind == 1 else 8) for t in tasks])
+        y = self.mlp.forward(X)
+        return [int(c) for c in self._classes(y, np.ones(len(tasks), bool))]
 
     def plan(self, data: TaskData) -> Tuple[int, int]:
-        y = self.mlp.advise(data.key_cols(), data.task.rows)
-        groups = [2.0 ** (min(max(float(y[0]), 0.0), 1.0) * T_SCALE) - 1.0]
-        cls = [int(np.argmax(y[1:3]))]  # the radix class has no aggregation kernel yet
-        strategy = STRATEGIES[int(cls[0])]
-        cap = int(min(float(data.task.rows), groups[0] * CAPACITY_MARGIN)) + 16
-        return cap, strategy
+        """(capacity hint, strategy) of a group-by (the capi strategy; sketch and inference run on the device in
+        one call), or (0, class) of a join (inference on its exact sizes)."""
+        is_join = data.task.op == "join"
+        if is_join:  # exact sizes, no sketch: one forward pass
+            y = self.mlp.forward(data.features().reshape(1, 16))[0]
+            return 0, int(self._classes(y.reshape(1, 4), np.array([True]))[0])
+        y = self.mlp.advise(data.key_cols(), data.task.rows, False)
+        cls = int(self._classes(y.reshape(1, 4), np.array([is_join]))[0])
+        groups = 2.0 ** (min(max(float(y[0]), 0.0), 1.0) * T_SCALE) - 1.0
+        cap = int(min(float(data.task.rows), groups * CAPACITY_MARGIN)) + 16
+        return cap, STRATEGIES[cls]
 
     def close(self):
         self.mlp.close()
+
+
+# ---- evaluation on the task set (every strategy and the advisor, outputs cross-checked) ------------------------
+CONFIGS = ("lds", "global", "radix", "auto", "advisor")
+CHECK_ROWS = 20_000_000  # result sets up to this size are compared by checksum, larger ones by count
+
+
+def evaluate(tasks: Sequence[Task], advisor: "Advisor", log=None):
+    """Runs every task with each fixed strategy, the library's automatic rule and the advisor (its sketch and
+    inference timed with it).  Fixed strategies per operator: lds = LDS-local group-by / direct join, global =
+    global-atomic group-by / direct join, radix = global-atomic group-by (no radix group-by kernel) /
+    partitioned join.  Group-by results are written to device outputs inside the timed region.  Returns
+    {config: seconds}, input rows, and per-task records; raises if any configuration's result rows differ."""
+    secs = {k: 0.0 for k in CONFIGS}
+    rows_total = 0
+    records = []
+    # join strategies depend only on sizes, so all joins are planned in one inference call, timed and charged to
+    # the advisor in equal shares
+    joins = [i for i, tk in enumerate(tasks) if tk.op == "join"]
+    join_cls, join_plan = {}, 0.0
+    if joins:
+        t = time.perf_counter()
+        for i, c in zip(joins, advisor.plan_joins([tasks[i] for i in joins])):
+            join_cls[i] = c
+        join_plan = (time.perf_counter() - t) / len(joins)
+    for i, task in enumerate(tasks):
+        data = TaskData(task)
+        rec = {"op": task.op, "rows": task.rows, "distinct": task.distinct}
+        if task.op == "join":
+            rows_total += task.rows + int(task.distinct)
+            m0, _, outs = run_join(data, CLASS_GLOBAL)  # untimed first run: allocations, warm caches
+            results = {}
+            cls, t_plan = join_cls[i], join_plan
+            for k, s in (("lds", CLASS_GLOBAL), ("global", CLASS_GLOBAL), ("radix", CLASS_RADIX),
+                         ("auto", None), ("advisor", cls)):
+                m, dt, outs = run_join(data, s, outs)
+                if k == "advisor":
+                    dt += t_plan
+                secs[k] += dt
+                rec[k] = dt
+                results[k] = m if m > CHECK_ROWS else (m, _checksum([outs[0].numpy(m), outs[1][0][0].numpy(m)], m))
+            rec["choice"] = int(cls)
+            rec["plan"] = t_plan
+            rec["matches"] = m0
+        else:
+            rows_total += task.rows
+            g0, _ = run_agg(data, capi.STRATEGY_AUTO, 0)  # untimed first run: group count, warm caches
+            h = capi.HashAgg(capi.make_agg_spec(data.types, AGGS, True))
+            bufs = h.alloc_output(g0)
+            h.close()
+            t = time.perf_counter()
+            cap, strat = advisor.plan(data)
+            t_plan = time.perf_counter() - t
+            results = {}
+            for k, s, c in (("lds", capi.STRATEGY_LDS, 0), ("global", capi.STRATEGY_GLOBAL, 0),
+                            ("radix", capi.STRATEGY_GLOBAL, 0), ("auto", capi.STRATEGY_AUTO, 0),
+                            ("advisor", strat, cap)):
+                g, dt = run_agg_full(data, s, c, bufs)
+                if k == "advisor":
+                    dt += t_plan
+                secs[k] += dt
+                rec[k] = dt
+                results[k] = g if g > CHECK_ROWS else (g, _checksum([b[0].numpy(g) for b in bufs[0]], g))
+            rec["choice"] = int(strat)
+            rec["plan"] = t_plan
+            rec["capacity"] = cap
+            rec["groups"] = g0
+        first = results["global"]
+        for k, r in results.items():
+            if r != first:
+                raise RuntimeError(f"task {i} ({task}): configuration {k} gave {r}, global gave {first}")
+        records.append(rec)
+        if log and (i + 1) % 16 == 0:
+            log(f"{i + 1}/{len(tasks)} tasks; seconds so far " +
+                ", ".join(f"{k} {v:.2f}" for k, v in secs.items()))
+    return secs, rows_total, records
+

@@ -113,11 +113,40 @@ def test_generator_is_seeded_and_bounded():
     assert len(pairs) == len(np.unique(kk[1].numpy())) <= 5000
 
 
+def test_generator_join_modes():
+    """Join build side: row i is rank i + 1, so its keys are unique and are exactly the keys a probe over the same
+    ranks draws; the probe's miss mode sends about 20% of rows outside them."""
+    (b,), _ = capi.generate_keys(capi.GEN_SEQUENTIAL, 5000, 5000, 0.0, seed=9)
+    build = b.numpy()
+    assert len(np.unique(build)) == 5000
+    (p,), _ = capi.generate_keys(capi.GEN_MISSES, 200_000, 5000, 0.7, seed=9, with_values=False)
+    (q,), _ = capi.generate_keys(0, 200_000, 5000, 0.7, seed=9, with_values=False)
+    hit = np.isin(p.numpy(), build)
+    assert abs(hit.mean() - 0.8) < 0.01
+    assert np.array_equal(p.numpy()[hit], q.numpy()[hit])  # hits keep their rank's key
+    assert np.isin(q.numpy(), build).all()
+
+
+def test_join_strategies_agree():
+    """A join task gives the same match rows with the direct probe, the always-partitioned probe and the
+    automatic rule (match count and an order-independent checksum of (probe row, payload))."""
+    data = learned.TaskData(learned.Task(1, 3_000_000, 700_000, 0.9, 1234, "join"))
+    res, outs = [], None
+    for s in (learned.CLASS_GLOBAL, learned.CLASS_RADIX, None):
+        m, _, outs = learned.run_join(data, s, outs)
+        res.append((m, learned._checksum([outs[0].numpy(m), outs[1][0][0].numpy(m)], m)))
+    assert res[0] == res[1] == res[2]
+    assert 0.75 * 3_000_000 < res[0][0] < 0.85 * 3_000_000
+
+
 def test_advisor_training_pipeline_reduces_loss():
     """Runs of the aggregation itself (40 seeded tasks) labelled and fitted on the GPU: the loss falls."""
-    tasks = learned.sample_tasks(40, seed=11, min_rows=2e4, max_rows=5e5, max_distinct=5e5)
+    tasks = learned.sample_tasks(40, seed=11, min_rows=2e4, max_rows=5e5, max_distinct=5e5, join_share=0.3)
     X, t, c, groups = learned.collect(tasks, repeats=1)
     assert np.all(groups >= 1) and np.all(groups <= [tk.rows for tk in tasks])
+    is_join = np.array([tk.op == "join" for tk in tasks])
+    assert is_join.any() and np.all(np.isin(c[is_join], learned.JOIN_CLASSES))
+    assert np.all(np.isin(c[~is_join], learned.AGG_CLASSES))
     _, losses = learned.train(X, t, c, seed=3, epochs=60)
     assert losses[-1] < 0.5 * losses[0]
 
@@ -130,6 +159,7 @@ def test_advisor_gains_over_the_plain_method():
     is lower.  The adaptive automatic rule is timed alongside for reference."""
     adv = learned.Advisor()
     test_tasks = learned.sample_tasks(32, seed=777, min_rows=2e5, max_rows=8e6, max_distinct=8e6)
+    assert all(tk.op == "agg" for tk in test_tasks)
     plain_t = adv_t = auto_t = 0.0
     errs = []
     for task in test_tasks:
@@ -170,3 +200,18 @@ def test_capacity_below_the_groups_falls_back_to_growth():
     o0, o1 = np.argsort(out0[0][0]), np.argsort(out1[0][0])
     for (v0, _), (v1, _) in zip(out0, out1):
         assert np.array_equal(v0[o0], v1[o1])
+
+
+def test_advisor_evaluation_small_set():
+    """The evaluation routine on 24 small mixed tasks: every configuration's results agree (checked inside), the
+    advisor picks group-by classes for group-bys and join classes for joins."""
+    adv = learned.Advisor()
+    tasks = learned.sample_tasks(24, seed=0xC5, min_rows=1e5, max_rows=2e6, join_share=0.5)
+    secs, rows, recs = learned.evaluate(tasks, adv)
+    adv.close()
+    assert rows > 0 and all(v > 0 for v in secs.values())
+    for r in recs:
+        if r["op"] == "join":
+            assert r["choice"] in learned.JOIN_CLASSES
+        else:
+            assert r["choice"] in (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL)
