@@ -268,8 +268,12 @@ def bench_batched(args):
         return int(g.sum())
 
     tuned = None
-    if args.batched_form != "per_batch":  # the plain one-launch-per-batch form keeps the default tiles
-        from omnioperator_amd import autotune
+    from omnioperator_amd import autotune
+    if args.tiles:  # fixed tiles, e.g. for a profile of one configuration
+        b, l, p_, q = (int(x) for x in args.tiles.split(","))
+        tuned = autotune.StreamConfig(b, l, p_, q)
+        autotune.apply(s, tuned)
+    elif args.batched_form != "per_batch":  # the plain one-launch-per-batch form keeps the default tiles
         key = autotune.key_of(LONG, aggs, autotune.cardinality_bucket(float(np.mean(w.distinct))))
         tuned = autotune.tune_stream(s, lambda: (step(), capi.synchronize()), key, use_cache=False)
 
@@ -382,6 +386,8 @@ def main():
     p.add_argument("--skewed-rows", type=int, default=200_000_000)
     p.add_argument("--cpu-skewed-rows", type=int, default=4_000_000)
     p.add_argument("--batched-form", default="graph", choices=["graph", "two_launches", "per_batch"])
+    p.add_argument("--tiles", default=None, help="batched: fixed block,lds_bytes,max_probe,part_lds_bytes "
+                                                 "instead of the autotuner")
     p.add_argument("--learned-tasks", type=int, default=512)
     p.add_argument("--learned-max-rows", type=float, default=2e8)
     p.add_argument("--learned-records", default=None, help="write the per-task records of --workload learned here")

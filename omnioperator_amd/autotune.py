@@ -37,7 +37,7 @@ CANDIDATES: List[StreamConfig] = [
     for b in (256, 512)
     for l in (16384, 32768, 65536)
     for p in (8, 32)
-    for q in (32768, 65536)
+    for q in (16384, 32768, 65536)
 ]
 
 
