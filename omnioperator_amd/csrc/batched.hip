@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cstring>
 #include <memory>
+#include <numeric>
 #include <vector>
 
 #include "agg_layout.h"
@@ -18,19 +19,24 @@ namespace omni_gpu {
 constexpr int kScanBlock = 1024;
 constexpr int kXcdCount = 8;  // XCDs of an MI355X; workgroup w runs on XCD w mod 8
 
-template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_lds(const BatchConfig *__restrict__ cfg, int first)
+// order (optional): workgroup i takes batch order[i] -- the batches by decreasing rows, so that the longest
+// workgroups start first and the launch ends with short ones (longest-processing-time-first).
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_lds(const BatchConfig *__restrict__ cfg, int first,
+                                                                         const int32_t *__restrict__ order)
 {
     extern __shared__ unsigned long long lds_words[];
     __shared__ unsigned int sh[2];
-    BatchLds<BLOCK>(*cfg, first + static_cast<int>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
+    const int i = first + static_cast<int>(blockIdx.x);
+    BatchLds<BLOCK>(*cfg, order ? order[i] : i, reinterpret_cast<uint8_t *>(lds_words), sh);
 }
 
-template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_split(const BatchConfig *__restrict__ cfg)
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_split(const BatchConfig *__restrict__ cfg,
+                                                                           const int32_t *__restrict__ order)
 {
     __shared__ unsigned int hist[kMaxParts];
     __shared__ unsigned int bits[kLcWords];
     __shared__ unsigned int sh[2];
-    BatchSplit<BLOCK>(*cfg, static_cast<int>(blockIdx.x), hist, bits, sh);
+    BatchSplit<BLOCK>(*cfg, order[blockIdx.x], hist, bits, sh);
 }
 
 template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const BatchConfig *__restrict__ cfg)
@@ -91,6 +97,7 @@ struct BatchesHandle {
     int part_rows = 0;            // rows per phase-2 partition (0: half the partition table's slots)
     DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error, [4] redo
     DevBuf items, parts, parts_used;  // phase-2 work items, partitions launched / used per batch
+    DevBuf order;                     // batches by decreasing rows (phase 1 and 2a workgroup order)
     int64_t nitems = 0;
     bool last_per_batch = false;
     std::vector<int64_t> h_tiles;  // phase-3 tile prefix over the batches
@@ -134,17 +141,19 @@ int Layout(BatchesHandle &h)
 // above the mean).  Items of batch b go to workgroup ids == b (mod 8), i.e. to one XCD, consecutively, so the
 // batch's rows stay in that XCD's L2 while its partitions re-read them; the per-XCD lists are interleaved
 // and padded with empty items.
-std::vector<int4> PartItems(const int64_t *offsets, int nb, int table_slots, int part_rows, std::vector<int32_t> &parts)
+std::vector<int4> PartItems(const int64_t *offsets, int nb, int table_slots, int part_rows, std::vector<int32_t> &parts,
+                            const std::vector<int32_t> &order)
 {
     const int64_t cpart = part_rows > 0 ? part_rows : std::max(1, table_slots / 2);
     std::vector<std::vector<int4>> lists(kXcdCount);
     parts.assign(nb, 1);
-    for (int b = 0; b < nb; ++b) {
+    for (int i = 0; i < nb; ++i) {  // batch order[i]'s split workgroup is workgroup i, on XCD i mod 8
+        const int b = order[i];
         const int64_t rows = offsets[b + 1] - offsets[b];
         int P = 1;
         while (P < kMaxParts && static_cast<int64_t>(P) * cpart < rows) P <<= 1;
         parts[b] = P;
-        for (int p = 0; p < P; ++p) lists[b % kXcdCount].push_back(make_int4(b, p, P, 0));
+        for (int p = 0; p < P; ++p) lists[i % kXcdCount].push_back(make_int4(b, p, P, 0));
     }
     size_t longest = 0;
     for (auto &l : lists) longest = std::max(longest, l.size());
@@ -169,12 +178,13 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     int lds_bytes = h.cfg.L.bytes;
     if (per_batch) {
         for (int b = 0; b < nb; ++b) {
-            if (h.block == 512) batch_lds<512><<<1, 512, lds_bytes, s>>>(dc, b);
-            else batch_lds<256><<<1, 256, lds_bytes, s>>>(dc, b);
+            if (h.block == 512) batch_lds<512><<<1, 512, lds_bytes, s>>>(dc, b, nullptr);
+            else batch_lds<256><<<1, 256, lds_bytes, s>>>(dc, b, nullptr);
         }
     } else {
-        if (h.block == 512) batch_lds<512><<<nb, 512, lds_bytes, s>>>(dc, 0);
-        else batch_lds<256><<<nb, 256, lds_bytes, s>>>(dc, 0);
+        const int32_t *order = h.order.as<int32_t>();
+        if (h.block == 512) batch_lds<512><<<nb, 512, lds_bytes, s>>>(dc, 0, order);
+        else batch_lds<256><<<nb, 256, lds_bytes, s>>>(dc, 0, order);
     }
     OG_TRY(CheckLaunch("batch_lds"));
     batch_scan<<<1, kScanBlock, 0, s>>>(dc);
@@ -182,10 +192,10 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     if (!per_batch && h.nitems > 0) {
         const unsigned int g = static_cast<unsigned int>(h.nitems);
         if (h.block == 512) {
-            batch_split<512><<<nb, 512, 0, s>>>(dc);
+            batch_split<512><<<nb, 512, 0, s>>>(dc, h.order.as<int32_t>());
             batch_part<512><<<g, 512, h.cfg.L2.bytes, s>>>(dc);
         } else {
-            batch_split<256><<<nb, 256, 0, s>>>(dc);
+            batch_split<256><<<nb, 256, 0, s>>>(dc, h.order.as<int32_t>());
             batch_part<256><<<g, 256, h.cfg.L2.bytes, s>>>(dc);
         }
         OG_TRY(CheckLaunch("batch_part"));
@@ -364,8 +374,15 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         }
         OG_TRY(h.pool.Reserve(pool, s));
         h.pool_bytes = pool;
+        std::vector<int32_t> order(num_batches);
+        std::iota(order.begin(), order.end(), 0);
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return batch_offsets[a + 1] - batch_offsets[a] > batch_offsets[b + 1] - batch_offsets[b];
+        });
+        OG_TRY(h.order.Reserve(sizeof(int32_t) * order.size(), s));
+        OG_TRY(CopyToDevice(h.order.p, order.data(), sizeof(int32_t) * order.size(), s));
         std::vector<int32_t> parts;
-        const std::vector<int4> items = PartItems(batch_offsets, num_batches, c.L2.S, h.part_rows, parts);
+        const std::vector<int4> items = PartItems(batch_offsets, num_batches, c.L2.S, h.part_rows, parts, order);
         h.nitems = static_cast<int64_t>(items.size());
         OG_CHECK(h.nitems < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many phase-2 items for one run");
         OG_TRY(h.items.Reserve(sizeof(int4) * items.size(), s));
@@ -479,6 +496,7 @@ int omni_gpu_agg_batches_destroy(void *handle)
     h->items.Release(s);
     h->parts.Release(s);
     h->parts_used.Release(s);
+    h->order.Release(s);
     h->offsets.Release(s);
     h->tiles.Release(s);
     h->modes.Release(s);

@@ -6,7 +6,8 @@
 // so that a partition's rows fit half an LDS table.  2a: one workgroup per batch writes each row's key and
 // row number to its partition's bucket (in the batch's region of the pool).  2b: one workgroup per (batch,
 // partition) item aggregates its bucket in LDS and emits its groups.  A batch's 2a workgroup and all its 2b
-// items run on the same XCD (workgroup id mod 8), so the buckets are read back from that XCD's L2.  A
+// items run on the same XCD (workgroup id mod 8), so the buckets are read back from that XCD's L2.  Phases 1
+// and 2a take the batches by decreasing rows.  A
 // partition that still overflows its table flags the batch (spilled = 2) for phase 3.
 // Phase 3 (batches flagged 2; batches flagged 1 when phase 2 is not used): the batch is aggregated exactly in
 // its own HBM table of next_pow2(2 * rows) slots (plus 2 side slots) carved from a region pool at a scanned
@@ -224,8 +225,8 @@ constexpr int kLcWords = 1024;  // linear-counting bitmap of phase 2a: 32,768 bi
 // partitions -- one pass builds an LDS histogram over the launched partitions (Pmax) and a linear-counting
 // bitmap of the keys; the distinct estimate picks the partitions used, P <= Pmax, so that a partition holds
 // at most 40% of the table's slots (P divides Pmax and partitions are the low hash bits, so P's histogram
-// is Pmax's folded); a second pass writes each row's key and row number to its bucket.  Workgroup b runs on
-// the XCD of batch b's phase-2b items, so the buckets are read back from that XCD's L2.
+// is Pmax's folded); a second pass writes each row's key and row number to its bucket.  The workgroup runs on
+// the XCD of the batch's phase-2b items, so the buckets are read back from that XCD's L2.
 template <int BLOCK>
 __device__ __forceinline__ void BatchSplit(const BatchConfig &c, int b, unsigned int *hist, unsigned int *bits,
                                            unsigned int *sh)
