@@ -174,8 +174,10 @@ int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_by
 /* Runs the stream.  keys[0] / aggs[i] are the concatenated device columns of all batches; batch b owns rows
  * [batch_offsets[b], batch_offsets[b+1]) (host array, batch_offsets[0] == 0, at most max_batch_rows rows
  * each).  batch_modes (host, optional): 0 = LDS table first, 2 = skip the per-batch LDS table and start
- * at phase 2 (e.g. from a predicted group count).  flags: bit 0 = one launch per batch and phase instead of one launch per phase; bit 1 =
- * capture the launch sequence in a hipGraph and replay it while the stream's shape stays the same.  Batch
+ * at phase 2 (e.g. from a predicted group count).  flags: bit 0 = one launch per batch and phase instead of
+ * one launch per phase; bit 1 = capture the launch sequence in a hipGraph and replay it while the stream's
+ * shape stays the same; bit 2 (without batch_modes, not with bit 0) = a sketch launch estimates each batch's
+ * distinct keys from its first 1,024 rows and starts the batches estimated above the LDS table at phase 2.  Batch
  * b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
  * then aggregates); outs hold batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.
  * Synchronizes. */

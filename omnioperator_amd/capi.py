@@ -343,7 +343,7 @@ class HashAgg:
 class AggBatches:
     """Batched stream of independent group-bys (omni_gpu_agg_batches_*)."""
 
-    FLAG_PER_BATCH, FLAG_GRAPH = 1, 2
+    FLAG_PER_BATCH, FLAG_GRAPH, FLAG_SKETCH = 1, 2, 4
 
     def __init__(self, spec: AggSpec, max_batch_rows: int):
         init()

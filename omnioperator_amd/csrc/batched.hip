@@ -30,6 +30,13 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_lds(const Ba
     BatchLds<BLOCK>(*cfg, order ? order[i] : i, reinterpret_cast<uint8_t *>(lds_words), sh);
 }
 
+template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_sketch(const BatchConfig *__restrict__ cfg)
+{
+    __shared__ unsigned int bits[kSketchWords];
+    __shared__ unsigned int sh[1];
+    BatchSketch<BLOCK>(*cfg, static_cast<int>(blockIdx.x), bits, sh);
+}
+
 template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_split(const BatchConfig *__restrict__ cfg,
                                                                            const int32_t *__restrict__ order)
 {
@@ -98,6 +105,7 @@ struct BatchesHandle {
     DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error, [4] redo
     DevBuf items, parts, parts_used;  // phase-2 work items, partitions launched / used per batch
     DevBuf order;                     // batches by decreasing rows (phase 1 and 2a workgroup order)
+    DevBuf sketch_modes;              // per-batch first phase from the sketch launch
     int64_t nitems = 0;
     bool last_per_batch = false;
     std::vector<int64_t> h_tiles;  // phase-3 tile prefix over the batches
@@ -183,6 +191,10 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
         }
     } else {
         const int32_t *order = h.order.as<int32_t>();
+        if (h.cfg.sketch_modes) {
+            batch_sketch<256><<<nb, 256, 0, s>>>(dc);
+            OG_TRY(CheckLaunch("batch_sketch"));
+        }
         if (h.block == 512) batch_lds<512><<<nb, 512, lds_bytes, s>>>(dc, 0, order);
         else batch_lds<256><<<nb, 256, lds_bytes, s>>>(dc, 0, order);
     }
@@ -407,6 +419,12 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.redo = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>() + 4);
     c.nitems = h.nitems;
     c.hbm_flag = (flags & 1) ? 1 : 2;  // the per-batch form has no phase 2
+    // flag bit 2: the per-batch sketch picks each batch's first phase (not with caller modes or the per-batch
+    // form).  Off by default: on the C4 stream it moved as much work into phase 2 as it saved in phase 1.
+    const bool sketch = !(flags & 1) && (flags & 4) && batch_modes == nullptr;
+    if (sketch) OG_TRY(h.sketch_modes.Reserve(sizeof(int32_t) * num_batches, s));
+    c.sketch_modes = sketch ? h.sketch_modes.as<int32_t>() : nullptr;
+    h.cfg.sketch_modes = c.sketch_modes;
     h.last_per_batch = (flags & 1) != 0;
     c.error_flag = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>());
     OG_TRY(CopyToDevice(h.dcfg.p, &c, sizeof(BatchConfig), s));
@@ -418,7 +436,7 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         // the launch shapes; it is re-captured when any of them changes
         std::vector<uint64_t> key = {static_cast<uint64_t>(num_batches), static_cast<uint64_t>(total),
                                      static_cast<uint64_t>(h.block), static_cast<uint64_t>(c.L.bytes),
-                                     reinterpret_cast<uint64_t>(h.dcfg.p)};
+                                     reinterpret_cast<uint64_t>(h.dcfg.p), c.sketch_modes ? 1ULL : 0ULL};
         for (int k = 0; k < 1 + c.args.n; ++k) key.push_back(reinterpret_cast<uint64_t>(outs[k].nulls));
         if (!h.graph_exec || key != h.graph_key) {
             DropGraph(h);
@@ -497,6 +515,7 @@ int omni_gpu_agg_batches_destroy(void *handle)
     h->parts.Release(s);
     h->parts_used.Release(s);
     h->order.Release(s);
+    h->sketch_modes.Release(s);
     h->offsets.Release(s);
     h->tiles.Release(s);
     h->modes.Release(s);

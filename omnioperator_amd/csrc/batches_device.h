@@ -29,6 +29,7 @@ struct BatchConfig {
     OutArgs out;
     const int64_t *offsets;    // nb + 1
     const int32_t *modes;      // per batch: 2 = skip LDS (straight to phase 2); nullptr = LDS first everywhere
+    int32_t *sketch_modes;     // per batch, written by the sketch launch (same meaning); nullptr = no sketch
     int32_t nbatches;
     int32_t max_probe;
     int32_t overflow_as_null;
@@ -124,7 +125,7 @@ __device__ __forceinline__ void BatchLds(const BatchConfig &c, int b, uint8_t *l
     const LdsLayout &L = c.L;
     const int S = L.S;
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
-    if (c.modes != nullptr && c.modes[b] == 2) {
+    if ((c.modes != nullptr && c.modes[b] == 2) || (c.sketch_modes != nullptr && c.sketch_modes[b] == 2)) {
         if (threadIdx.x == 0) {
             c.spilled[b] = 1u;
             c.groups[b] = 0;
@@ -358,6 +359,60 @@ __device__ __forceinline__ void BatchPart(const BatchConfig &c, int64_t it, uint
     if (threadIdx.x == 0) sh[3] = sh[2] ? static_cast<unsigned int>(atomicAdd(&c.groups[b], static_cast<unsigned long long>(sh[2]))) : 0u;
     __syncthreads();
     EmitTable<BLOCK>(c, lkeys, S, start + sh[3], &sh[1], [&](int i) { return LdsRef{lds, L, i}; });
+}
+
+// Per-batch sketch (before phase 1): the distinct keys among the batch's first kSketchRows rows, by linear
+// counting in a 4,096-bit LDS bitmap, give an estimate of the batch's distinct keys (the d solving
+// d (1 - exp(-m / d)) = u for m sampled rows with u distinct); a batch estimated above the phase-1 table's
+// slots starts at phase 2.  Batches with at most L.S rows always fit and are not sketched.  A wrong estimate
+// only moves work between phases; results do not depend on it.
+constexpr int kSketchRows = 1024;
+constexpr int kSketchWords = 128;
+
+template <int BLOCK>
+__device__ __forceinline__ void BatchSketch(const BatchConfig &c, int b, unsigned int *bits, unsigned int *sh)
+{
+    const int64_t start = c.offsets[b], end = c.offsets[b + 1];
+    const int S = c.L.S;
+    if (end - start <= S) {
+        if (threadIdx.x == 0) c.sketch_modes[b] = 0;
+        return;
+    }
+    for (int i = threadIdx.x; i < kSketchWords; i += BLOCK) bits[i] = 0;
+    if (threadIdx.x == 0) sh[0] = 0;
+    __syncthreads();
+    const int64_t m = end - start < kSketchRows ? end - start : kSketchRows;
+    for (int64_t r = threadIdx.x; r < m; r += BLOCK) {
+        const int64_t row = start + r;
+        const bool isNull = NullBit(c.kd.nulls[0], row);
+        const unsigned long long key = isNull ? 0ULL : LoadWord(c.kd.values[0], c.kd.width[0], row);
+        const unsigned int bit = static_cast<unsigned int>(Mix64(key) >> 40) & (kSketchWords * 32 - 1);
+        atomicOr(&bits[bit >> 5], 1u << (bit & 31));
+    }
+    __syncthreads();
+    unsigned int ones = 0;
+    for (int i = threadIdx.x; i < kSketchWords; i += BLOCK) ones += __popc(bits[i]);
+    if (ones) atomicAdd(&sh[0], ones);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double nb = kSketchWords * 32.0;
+        const double zeros = nb - static_cast<double>(sh[0]);
+        const double u = zeros >= 1.0 ? -nb * log(zeros / nb) : nb;  // distinct among the sampled rows
+        const double md = static_cast<double>(m);
+        double d = u;  // solve d (1 - exp(-m / d)) = u by bisection; u close to m means "far more than m"
+        if (u >= 0.97 * md) {
+            d = 1e18;
+        } else if (u > 1.0) {
+            double lo = u, hi = 1e9;
+            for (int it = 0; it < 60; ++it) {
+                const double mid = 0.5 * (lo + hi);
+                if (mid * (1.0 - exp(-md / mid)) < u) lo = mid;
+                else hi = mid;
+            }
+            d = lo;
+        }
+        c.sketch_modes[b] = d > static_cast<double>(S) ? 2 : 0;
+    }
 }
 
 // Linear-probing find-or-insert of a key into a key array with probe slots [0, mask].

@@ -85,8 +85,8 @@ def run_batches(offsets, key, cols, modes=None, flags=0, tune=None, repeat=1, pa
     return results, stats
 
 
-@pytest.mark.parametrize("form", ["two_launches", "per_batch", "graph", "hbm_modes", "small_lds", "block512",
-                                  "small_partitions", "partition_overflow"])
+@pytest.mark.parametrize("form", ["two_launches", "sketch", "per_batch", "graph", "hbm_modes", "small_lds",
+                                  "block512", "small_partitions", "partition_overflow"])
 def test_batches_match_oracle(form):
     """Every form of the batched group-by against the oracle, batch by batch.  small_partitions: 8 KiB phase-2
     tables (many partitions per batch); partition_overflow: partitions sized 16x above their table, so phase-2
@@ -96,6 +96,8 @@ def test_batches_match_oracle(form):
     kwargs = {}
     if form == "per_batch":
         kwargs["flags"] = capi.AggBatches.FLAG_PER_BATCH
+    elif form == "sketch":
+        kwargs["flags"] = capi.AggBatches.FLAG_SKETCH
     elif form == "graph":
         kwargs["flags"] = capi.AggBatches.FLAG_GRAPH
         kwargs["repeat"] = 3  # captured on the first run, replayed on the next two
@@ -122,7 +124,7 @@ def test_batches_match_oracle(form):
     assert p1 + p2 + p3 == nb and p1 == stats["lds_batches"]
     if form == "small_lds":
         assert stats["hbm_batches"] > 0  # 8 KiB tables cannot hold the larger batches
-    if form in ("small_lds", "small_partitions", "two_launches"):
+    if form in ("small_lds", "small_partitions", "two_launches", "sketch"):
         assert p3 == 0  # partitions sized to half a table do not overflow on this stream
     if form == "small_partitions":
         assert p2 > 0
