@@ -9,8 +9,9 @@
 // Device layout:
 //   build store   keys[n] (int64, sign-extended), key_null[n] (u8), payload values / NULL bytes per column,
 //                 gathered from every retained build batch in one launch at finish
-//   hash table    slots[cap] {i64 key; i32 head; u32 inline payload} (16 B, one cache-line read per probe step),
-//                 linear probing at load <= 0.5; duplicate keys chain through next[n]
+//   hash table    slots[cap] {i64 key; i32 head; u32 inline payload} (16 B), linear probing at load <= 0.25
+//                 from a multiplicative home slot; duplicate keys chain through next[n].  Tables of 8 to
+//                 4,096 regions of 8,192 slots are built region by region in LDS (partitioned build).
 //   array table   heads[range] (i32 build row, -1 none) + next[n]
 // Probe: 16 probe rows per lane held in registers; matches are compacted with wave ballots (unique keys) or a
 // wave-level exclusive scan (duplicate keys) and one atomic per workgroup on the output counter, so each
@@ -45,19 +46,16 @@ struct JoinHandle {
     int partitions = -1;       // radix-partitioned probe: -1 auto, 0 off, else the partition count
     int last_partitions = 0;   // partition count the last probe used (1 = direct probe)
     DevBuf part_keys, part_rows, part_counts, part_meta;
+    DevBuf ovf_keys, ovf_rows;  // partitioned build: keys whose probe run leaves their region
     uint64_t slots_n = 0;
     long long min_key = 0;
     unsigned long long range = 0;
     // scratch: [0] sentinel head i32, [4] has_dup u32, [8] max_len u32, [12] sentinel inline payload u32,
-    //          [16] min/max i64[2], [32] match counter u64
+    //          [16] min/max i64[2], [32] match counter u64, [40] build overflow count u32
     DevBuf slots, heads, next, scratch;
     DevBuf batch_desc, build_desc;
 };
 
-// Partition count of the radix-partitioned probe for `rows` probe rows in `nwg` tiles (1 = direct probe).
-// Automatic mode partitions only tables larger than the 256 MiB Infinity Cache (regions of about 2 MiB, one
-// L2-resident region per XCD at a time): measured on a 64 MiB table (2M keys, 64M probe rows) the direct probe
-// (1.56 ms) still beats the partition chain (1.65 ms: histogram 0.11, scatter 0.51, probe 1.08 ms).
 // Partition count of the radix-partitioned probe, 1 for the direct probe.  Automatic mode partitions tables
 // above kAutoPartBytes when the probe side has at least half as many rows as the table has slots (so the
 // partitioning pays for itself), with regions of kWinSlots slots for the LDS window probe; tables too big for
@@ -84,10 +82,13 @@ int ChoosePartitions(const JoinHandle &h, int64_t rows)
     return P;
 }
 
-int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64_t nwg, int64_t rows)
+// Histogram, scan, plan and scatter of `rows` rows (the batches of `a`, nwg tiles of kPartTile) into P
+// partitions of the table; fills pa.  `tag` names the stage for the kernel timers.
+int Partition(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64_t nwg, int64_t rows, PartArgs &pa,
+              const char *hist_tag, const char *scan_tag, const char *scatter_tag)
 {
     hipStream_t s = h.stream;
-    PartArgs pa{};
+    pa = PartArgs{};
     pa.nparts = P;
     int lgS = 0, lgP = 0;
     while ((1ULL << lgS) < h.slots_n) ++lgS;
@@ -114,15 +115,13 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     pa.counts = h.part_counts.as<unsigned int>();
     pa.pkeys = h.part_keys.as<long long>();
     pa.prows = h.part_rows.as<unsigned int>();
-    int cus = 0;
-    OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
     {
-        KernelScope ks(s, "join_probe_hist");
+        KernelScope ks(s, hist_tag);
         join_part_hist<<<static_cast<unsigned int>(pa.ncol), kBlock, 0, s>>>(a, t, pa);
         OG_TRY(CheckLaunch("join_part_hist"));
     }
     {
-        KernelScope ks(s, "join_probe_scan");
+        KernelScope ks(s, scan_tag);
         scan_tiles<<<ntile, kScanBlock, 0, s>>>(pa.counts, L, sums);
         scan_sums<<<1, kScanBlock, 0, s>>>(sums, ntile, pa.total);
         scan_add<<<ntile, kScanBlock, 0, s>>>(pa.counts, L, sums);
@@ -130,13 +129,26 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
         OG_TRY(CheckLaunch("join_part_scan"));
     }
     {
-        KernelScope ks(s, "join_probe_scatter");
+        KernelScope ks(s, scatter_tag);
         const size_t lds = ScatterLdsBytes(P);
         OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_part_scatter),
                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
         join_part_scatter<<<static_cast<unsigned int>(nwg), kScatterBlock, lds, s>>>(a, t, pa);
         OG_TRY(CheckLaunch("join_part_scatter"));
     }
+    return OMNI_GPU_OK;
+}
+
+int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64_t nwg, int64_t rows)
+{
+    hipStream_t s = h.stream;
+    PartArgs pa;
+    OG_TRY(Partition(h, a, t, P, nwg, rows, pa, "join_probe_hist", "join_probe_scan", "join_probe_scatter"));
+    int lgP = 0;
+    while ((1 << lgP) < P) ++lgP;
+    const int per = P / kXcds;
+    int cus = 0;
+    OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
     const uint64_t region = h.slots_n >> lgP;
     if (region <= static_cast<uint64_t>(kWinSlots)) {
         // window probe: about 4 workgroups per CU in all, each partition split evenly
@@ -165,6 +177,48 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     else if (h.inline_pay) join_part_probe<true, false><<<pgrid, kBlock, 0, s>>>(a, t, pa);
     else join_part_probe<false, false><<<pgrid, kBlock, 0, s>>>(a, t, pa);
     return CheckLaunch("join_part_probe");
+}
+
+// Partitioned build of the hash table (P regions of kWinSlots slots): the build store's rows are partitioned
+// like probe rows, each region is built in LDS by one workgroup and written out whole, then the keys whose
+// probe run left their region are inserted into the global table.
+int BuildPartitioned(JoinHandle &h, int P, unsigned int *has_dup)
+{
+    hipStream_t s = h.stream;
+    uint8_t *sc = h.scratch.as<uint8_t>();
+    ProbeArgs a{};
+    a.nb = 1;
+    a.single.keys = h.keys.p;
+    a.single.nulls = nullptr;  // NULL build keys are partitioned and skipped by the region build
+    a.single.rows = h.n;
+    a.single.width = 8;
+    TableDev t{};
+    t.slots = h.slots.as<JoinSlot>();
+    t.mask = h.slots_n - 1;
+    const int64_t nwg = (h.n + kPartTile - 1) / kPartTile;
+    PartArgs pa;
+    OG_TRY(Partition(h, a, t, P, nwg, h.n, pa, "join_build_hist", "join_build_scan", "join_build_scatter"));
+    OG_TRY(h.ovf_keys.Reserve(sizeof(long long) * h.n, s));
+    OG_TRY(h.ovf_rows.Reserve(sizeof(unsigned int) * h.n, s));
+    BuildArgs b{};
+    b.knull = h.knull.as<uint8_t>();
+    b.next = h.next.as<int>();
+    b.sentinel_head = reinterpret_cast<int *>(sc);
+    b.sentinel_pay = reinterpret_cast<unsigned int *>(sc + 12);
+    b.has_dup = has_dup;
+    b.inline_pay = h.inline_pay ? h.pay[0].p : nullptr;
+    b.inline_width = h.inline_pay ? TypeWidth(h.spec.payload_types[0]) : 0;
+    b.ovf_keys = h.ovf_keys.as<long long>();
+    b.ovf_rows = h.ovf_rows.as<unsigned int>();
+    b.ovf_count = reinterpret_cast<unsigned int *>(sc + 40);
+    const size_t lds = sizeof(JoinSlot) * (static_cast<size_t>(1) << pa.shift);
+    KernelScope ks(s, "join_build_window");
+    OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_build),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    join_window_build<<<static_cast<unsigned int>(P), kWinBlock, lds, s>>>(t, pa, b, h.slots.as<JoinSlot>());
+    OG_TRY(CheckLaunch("join_window_build"));
+    join_build_overflow<<<256, kBlock, 0, s>>>(t, b, h.slots.as<JoinSlot>());
+    return CheckLaunch("join_build_overflow");
 }
 
 template <int G>
@@ -339,8 +393,10 @@ int omni_gpu_join_build_finish(void *handle)
         int head, has_dup;
         unsigned int max_len, sentinel_pay;
         long long mm[2];
-    } init = {-1, 0, 0, 0, {std::numeric_limits<long long>::max(), std::numeric_limits<long long>::min()}};
-    static_assert(sizeof(init) == 32, "scratch layout");
+        unsigned long long counter;
+        unsigned int ovf, pad;
+    } init = {-1, 0, 0, 0, {std::numeric_limits<long long>::max(), std::numeric_limits<long long>::min()}, 0, 0, 0};
+    static_assert(sizeof(init) == 48, "scratch layout");
     OG_TRY(CopyToDevice(sc, &init, sizeof(init), s));
     h.finished = true;
     if (h.n == 0) {
@@ -420,21 +476,26 @@ int omni_gpu_join_build_finish(void *handle)
         h.slots_n = 1024;
         while (h.slots_n < static_cast<uint64_t>(4 * h.n)) h.slots_n <<= 1;
         OG_TRY(h.slots.Reserve(sizeof(JoinSlot) * h.slots_n, s));
-        {
-            KernelScope ks(s, "join_init_table");
-            join_init_slots<<<Blocks(static_cast<int64_t>(h.slots_n)), kBlock, 0, s>>>(h.slots.as<JoinSlot>(),
-                                                                                      static_cast<int64_t>(h.slots_n));
-            OG_TRY(CheckLaunch("join_init_slots"));
-        }
         h.inline_pay = h.spec.num_payloads == 1 && TypeWidth(h.spec.payload_types[0]) <= 4 && !h.pay_nullable[0];
-        KernelScope ks(s, "join_build");
-        join_build_hash<<<Blocks(h.n), kBlock, 0, s>>>(h.keys.as<long long>(), h.knull.as<uint8_t>(), h.n,
-                                                       h.slots.as<JoinSlot>(), h.slots_n - 1, h.next.as<int>(),
-                                                       reinterpret_cast<int *>(sc), reinterpret_cast<unsigned int *>(sc + 12),
-                                                       has_dup,
-                                                       h.inline_pay ? h.pay[0].p : nullptr,
-                                                       h.inline_pay ? TypeWidth(h.spec.payload_types[0]) : 0);
-        OG_TRY(CheckLaunch("join_build_hash"));
+        const uint64_t bP = h.slots_n / kWinSlots;
+        if (bP >= static_cast<uint64_t>(kXcds) && bP <= static_cast<uint64_t>(kPartMax) && h.n < (1LL << 32) - 1) {
+            OG_TRY(BuildPartitioned(h, static_cast<int>(bP), has_dup));
+        } else {
+            {
+                KernelScope ks(s, "join_init_table");
+                join_init_slots<<<Blocks(static_cast<int64_t>(h.slots_n)), kBlock, 0, s>>>(
+                    h.slots.as<JoinSlot>(), static_cast<int64_t>(h.slots_n));
+                OG_TRY(CheckLaunch("join_init_slots"));
+            }
+            KernelScope ks(s, "join_build");
+            join_build_hash<<<Blocks(h.n), kBlock, 0, s>>>(h.keys.as<long long>(), h.knull.as<uint8_t>(), h.n,
+                                                           h.slots.as<JoinSlot>(), h.slots_n - 1, h.next.as<int>(),
+                                                           reinterpret_cast<int *>(sc),
+                                                           reinterpret_cast<unsigned int *>(sc + 12), has_dup,
+                                                           h.inline_pay ? h.pay[0].p : nullptr,
+                                                           h.inline_pay ? TypeWidth(h.spec.payload_types[0]) : 0);
+            OG_TRY(CheckLaunch("join_build_hash"));
+        }
     }
     // the probe kernel variant depends on whether any key repeats
     unsigned int dup = 0;

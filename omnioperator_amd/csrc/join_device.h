@@ -756,6 +756,104 @@ __global__ void __launch_bounds__(kBlock) join_part_probe(ProbeArgs a, TableDev 
     }
 }
 
+// ---- partitioned build ----------------------------------------------------------------------------------------
+// The build rows are histogrammed and scattered into partitions like probe rows (same kernels, the build store
+// as one batch), then one workgroup per partition builds its whole table region in LDS and writes it out with
+// coalesced stores, instead of one device-wide CAS per row into HBM.  A key whose probe run reaches the end of
+// its region is appended to an overflow list and inserted into the global table by a second launch (the run
+// continues into the next region, exactly where the global build would have put it).  Every slot of the table
+// is written by its region's workgroup, so the table needs no separate initialisation.
+
+struct BuildArgs {
+    const uint8_t *knull;           // build NULL flags (NULL keys are not inserted)
+    int *next;                      // build row chains
+    int *sentinel_head;             // chain of rows whose key is the slot sentinel
+    unsigned int *sentinel_pay;
+    unsigned int *has_dup;
+    const void *inline_pay;         // the payload kept in the slot, or nullptr
+    int inline_width;
+    long long *ovf_keys;            // overflow list: keys and rows, count in *ovf_count
+    unsigned int *ovf_rows;
+    unsigned int *ovf_count;
+};
+
+// Window build: workgroup p builds region p (1 << shift slots) of the table in LDS from partition p's
+// entries.  Dynamic LDS: (1 << shift) slots.
+__global__ void __launch_bounds__(kWinBlock) join_window_build(TableDev t, PartArgs pa, BuildArgs ba, JoinSlot *slots)
+{
+    OG_DYNAMIC_LDS(JoinSlot, win);
+    const unsigned int p = blockIdx.x;
+    const unsigned int R = 1u << pa.shift;
+    const unsigned long long wbase = static_cast<unsigned long long>(p) << pa.shift;
+    for (unsigned int i = threadIdx.x; i < R; i += kWinBlock) {
+        win[i].key = static_cast<long long>(kEmptyKey);
+        win[i].head = -1;
+        win[i].pay = 0;
+    }
+    __syncthreads();
+    const unsigned int lo = pa.part_start[p], hi = pa.part_start[p + 1];
+    for (unsigned int e = lo + threadIdx.x; e < hi; e += kWinBlock) {
+        const long long key = pa.pkeys[e];
+        const unsigned int row = pa.prows[e];
+        if (ba.knull[row]) continue;
+        int old;
+        if (static_cast<unsigned long long>(key) == kEmptyKey) {
+            old = atomicExch(ba.sentinel_head, static_cast<int>(row));
+            if (ba.inline_pay) *ba.sentinel_pay = PayloadWord(ba.inline_pay, ba.inline_width, row);
+        } else {
+            unsigned int pos = static_cast<unsigned int>((JoinHome(key, t.mask) - wbase) & t.mask);
+            while (pos < R) {
+                unsigned long long k = static_cast<unsigned long long>(win[pos].key);
+                if (k == kEmptyKey) {
+                    k = atomicCAS(reinterpret_cast<unsigned long long *>(&win[pos].key), kEmptyKey,
+                                  static_cast<unsigned long long>(key));
+                    if (k == kEmptyKey) k = static_cast<unsigned long long>(key);
+                }
+                if (k == static_cast<unsigned long long>(key)) break;
+                ++pos;
+            }
+            if (pos >= R) {  // the run leaves the region: inserted by join_build_overflow
+                const unsigned int at = atomicAdd(ba.ovf_count, 1u);
+                ba.ovf_keys[at] = key;
+                ba.ovf_rows[at] = row;
+                continue;
+            }
+            old = atomicExch(&win[pos].head, static_cast<int>(row));
+            if (ba.inline_pay) win[pos].pay = PayloadWord(ba.inline_pay, ba.inline_width, row);
+        }
+        ba.next[row] = old;
+        if (old >= 0) *ba.has_dup = 1u;
+    }
+    __syncthreads();
+    for (unsigned int i = threadIdx.x; i < R; i += kWinBlock) slots[wbase + i] = win[i];
+}
+
+// Inserts the overflow list into the global table (after every region is written).  Grid-stride over the
+// device count.
+__global__ void __launch_bounds__(kBlock) join_build_overflow(TableDev t, BuildArgs ba, JoinSlot *slots)
+{
+    const unsigned int n = *ba.ovf_count;
+    for (unsigned int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const long long key = ba.ovf_keys[i];
+        const unsigned int row = ba.ovf_rows[i];
+        unsigned long long pos = JoinHome(key, t.mask);
+        while (true) {
+            long long k = slots[pos].key;
+            if (static_cast<unsigned long long>(k) == kEmptyKey) {
+                k = static_cast<long long>(atomicCAS(reinterpret_cast<unsigned long long *>(&slots[pos].key), kEmptyKey,
+                                                     static_cast<unsigned long long>(key)));
+                if (static_cast<unsigned long long>(k) == kEmptyKey) k = key;
+            }
+            if (k == key) break;
+            pos = (pos + 1) & t.mask;
+        }
+        const int old = atomicExch(&slots[pos].head, static_cast<int>(row));
+        if (ba.inline_pay) slots[pos].pay = PayloadWord(ba.inline_pay, ba.inline_width, row);
+        ba.next[row] = old;
+        if (old >= 0) *ba.has_dup = 1u;
+    }
+}
+
 // Continues a probe run that left the LDS window, in the global table from slot `pos`.
 __device__ __forceinline__ int WalkGlobal(const TableDev &t, long long key, unsigned long long pos, unsigned int *pay)
 {
