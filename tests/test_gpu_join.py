@@ -171,6 +171,51 @@ def test_partitioned_probe_parity(partitions, unique, wide):
     assert not info["is_array"] and info["table_slots"] == 131072
 
 
+_FIB = 0x9E3779B97F4A7C15
+_FIB_INV = pow(_FIB, -1, 1 << 64)
+
+
+def keys_with_homes(homes, bits, rng):
+    """int64 keys whose home slot in a table of 2**bits slots is `homes`: the home slot is the top `bits` bits of
+    (k ^ k >> 29) * 0x9E3779B97F4A7C15 mod 2**64, a bijection of k, so a chosen product is inverted (odd-constant
+    inverse, then the inverse of the xor-shift)."""
+    homes = np.asarray(homes, np.uint64)
+    low = rng.integers(0, 1 << (64 - bits), len(homes), dtype=np.uint64)
+    x = (homes << np.uint64(64 - bits)) | low
+    m = x * np.uint64(_FIB_INV)
+    k = m ^ (m >> np.uint64(29)) ^ (m >> np.uint64(58))
+    y = (k ^ (k >> np.uint64(29))) * np.uint64(_FIB)  # forward map, checked
+    assert np.array_equal(y >> np.uint64(64 - bits), homes)
+    return k.view(np.int64)
+
+
+@pytest.mark.parametrize("kind", ["inline", "gathered", "dups"])
+def test_window_probe_fallbacks(kind):
+    """The LDS window probe's two global-walk cases: a region holding more keys than its LDS table takes (4,000
+    keys homed in region 0), and a region whose probe runs fill the slots staged past its end (400 keys homed
+    in the last slot of region 1).  10,000 keys: a 65,536-slot table in 8 regions of 8,192 slots."""
+    rng = np.random.default_rng(5)
+    bits = 16
+    homes = np.concatenate([rng.integers(0, 8192, 4000), np.full(400, 16383), rng.integers(0, 1 << bits, 5600)])
+    bkv = keys_with_homes(homes, bits, rng)
+    assert len(np.unique(bkv)) == len(bkv)
+    miss = keys_with_homes(np.concatenate([rng.integers(0, 8192, 500), np.full(50, 16383)]), bits, rng)
+    if kind == "dups":
+        bkv = np.concatenate([bkv, bkv[rng.integers(0, len(bkv), 3000)]])
+    pkv = np.concatenate([bkv, miss, rng.integers(-2**63, 2**63 - 1, 2000, dtype=np.int64)])
+    pkv = pkv[rng.permutation(len(pkv))]
+    nb = len(bkv)
+    if kind == "inline":
+        pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32))]
+    else:
+        pays = [Col(LONG, rng.integers(-2**63, 2**63 - 1, nb, dtype=np.int64))]
+    bk, pk = Col(LONG, bkv), Col(LONG, pkv)
+    rows, gp, info = gpu_join(bk, pays, pk, 4096, 5000, strategy=1, batched=True, partitions=8)
+    want, _ = oracle_set(bk, pays, pk, 4096, 5000)
+    assert info["table_slots"] == 1 << bits
+    assert as_set(rows, gp) == want
+
+
 def test_sentinel_key_inline_payload():
     """Unique keys with a <= 4-byte payload keep the payload in the slot; the sentinel key's row lives
     outside the table and must still return its own payload."""
