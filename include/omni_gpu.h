@@ -68,6 +68,19 @@ extern "C" {
 #define OMNI_GPU_STRATEGY_GLOBAL 2   /* global open-addressing table, 64-bit CAS insert */
 #define OMNI_GPU_STRATEGY_RADIX 3    /* radix-partition rows by key hash, then per-partition tables */
 
+/* Join types and build sides: the values of OmniOperator's JoinType and BuildSide
+ * (core/src/plannode/planNode.h:406-414 and :28-32). */
+#define OMNI_GPU_JOIN_INNER 0
+#define OMNI_GPU_JOIN_LEFT 1
+#define OMNI_GPU_JOIN_RIGHT 2
+#define OMNI_GPU_JOIN_FULL 3
+#define OMNI_GPU_JOIN_LEFT_SEMI 4
+#define OMNI_GPU_JOIN_LEFT_ANTI 5
+#define OMNI_GPU_JOIN_EXISTENCE 6
+#define OMNI_GPU_BUILD_UNKNOWN 0
+#define OMNI_GPU_BUILD_LEFT 1
+#define OMNI_GPU_BUILD_RIGHT 2
+
 #define OMNI_GPU_MAX_KEYS 4
 #define OMNI_GPU_MAX_AGGS 16
 
@@ -192,7 +205,7 @@ int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_
 int omni_gpu_agg_batches_phases(void *handle, int64_t *phase1, int64_t *phase2, int64_t *phase3);
 int omni_gpu_agg_batches_destroy(void *handle);
 
-/* ---- hash join (inner) --------------------------------------------------------------------------
+/* ---- hash join ------------------------------------------------------------------------------------
  * Replaces HashBuilderOperator + JoinHashTableVariants (core/src/operator/join/hash_builder.cpp:169-213,
  * join_hash_table_variants.cpp:254-283, 695-808) and LookupJoinOperator::ProbeBatchForInnerJoin
  * (core/src/operator/join/lookup_join.cpp:924-1035) with its output gather (:1657-1779). */
@@ -246,6 +259,33 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
                                 int32_t num_batches, int64_t *out_probe_rows,
                                 omni_gpu_out_column *out_payloads, int64_t out_capacity,
                                 int64_t *num_matches);
+/* ---- join types other than inner ----------------------------------------------------------------
+ * Sets the join type before omni_gpu_join_build_finish (HashBuilderOperatorFactory(joinType, buildSide, ...),
+ * core/src/operator/join/hash_builder.cpp:11-43; the default is OMNI_GPU_JOIN_INNER).  The probe entry points
+ * then follow the reference's per-type probe loops (core/src/operator/join/lookup_join.cpp:1037-1438; a NULL
+ * probe key matches nothing):
+ *   LEFT / RIGHT with the build on the other side (or build_side UNKNOWN), FULL
+ *        (ProbeBatchForOppositeSideOuterJoin, ProbeBatchForFullJoin): every match, and one output row with every
+ *        payload NULL for each probe row without a match (payload outputs must carry null bitmaps);
+ *   LEFT_SEMI (ProbeBatchForLeftSemiJoin): one output row per probe row with a match; payloads not written;
+ *   LEFT_ANTI (ProbeBatchForLeftAntiJoin): one output row per probe row without one; payloads not written;
+ *   LEFT / RIGHT with the build on the same side, FULL (ProbeBatchForSameSideOuterJoin, ...ForFullJoin):
+ *        the matched build rows are also marked for omni_gpu_join_outer_output;
+ *   EXISTENCE: probed with omni_gpu_join_probe_exists only.
+ * These types take the direct probe (no radix partitioning). */
+int omni_gpu_join_set_type(void *handle, int32_t join_type, int32_t build_side);
+/* ProbeBatchForExistenceJoin (lookup_join.cpp:1393-1438): out_exists[i] = 1 when probe row i has a matching
+ * build row, else 0 (also for a NULL key); rows bytes of device memory.  Works with any join type.
+ * Synchronizes. */
+int omni_gpu_join_probe_exists(void *handle, const omni_gpu_column *probe_key, int64_t rows, uint8_t *out_exists);
+/* LookupOuterJoinOperator::GetOutput (core/src/operator/join/lookup_outer_join.cpp:155-320): after every probe,
+ * writes each build row that no probe matched (NULL-key rows included) -- its build row id into
+ * out_build_rows[k] (build rows numbered from 0 in build order; may be NULL) and its payloads into
+ * out_payloads, which follow the rules of omni_gpu_join_probe.  The probe-side columns of these rows are NULL.
+ * Only for build-preserving types.  Output order is unspecified; *num_rows receives the row count
+ * (OMNI_GPU_ERR_OUTPUT_FULL beyond out_capacity).  Synchronizes. */
+int omni_gpu_join_outer_output(void *handle, int64_t *out_build_rows, omni_gpu_out_column *out_payloads,
+                               int64_t out_capacity, int64_t *num_rows);
 int omni_gpu_join_destroy(void *handle);
 
 /* ---- learned strategy / capacity advisor ----------------------------------------------------------------

@@ -21,6 +21,9 @@ INT, LONG, DOUBLE, BOOLEAN, SHORT, DECIMAL64, DECIMAL128, DATE32, TIMESTAMP = 1,
 SUM, COUNT_COLUMN, COUNT_ALL, AVG, MAX, MIN = 0, 1, 2, 3, 5, 6
 STRATEGY_AUTO, STRATEGY_LDS, STRATEGY_GLOBAL, STRATEGY_RADIX = 0, 1, 2, 3
 MAX_KEYS, MAX_AGGS = 4, 16
+# JoinType / BuildSide values shared with the reference (core/src/plannode/planNode.h:406-414, 28-32).
+JOIN_INNER, JOIN_LEFT, JOIN_RIGHT, JOIN_FULL, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI, JOIN_EXISTENCE = range(7)
+BUILD_UNKNOWN, BUILD_LEFT, BUILD_RIGHT = 0, 1, 2
 
 ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "OVERFLOW", -5: "OUTPUT_FULL", -6: "STATE"}
 
@@ -104,6 +107,9 @@ SIGNATURES = {
     "omni_gpu_join_probe_info": (_I, [_VP, _P(_I32)]),
     "omni_gpu_join_probe": (_I, [_VP, _P(Column), _I64, _I64, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_probe_batches": (_I, [_VP, _P(Column), _P(_I64), _I32, _VP, _P(OutColumn), _I64, _P(_I64)]),
+    "omni_gpu_join_set_type": (_I, [_VP, _I32, _I32]),
+    "omni_gpu_join_probe_exists": (_I, [_VP, _P(Column), _I64, _VP]),
+    "omni_gpu_join_outer_output": (_I, [_VP, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_destroy": (_I, [_VP]),
     "omni_gpu_sketch_keys": (_I, [_P(Column), _I32, _I64, _I32, _P(Sketch)]),
     "omni_gpu_advise": (_I, [_VP, _P(Column), _I32, _I64, _I32, _P(C.c_float)]),
@@ -427,7 +433,7 @@ class AggBatches:
 
 
 class HashJoin:
-    """GPU inner hash join handle (omni_gpu_join_*)."""
+    """GPU hash join handle (omni_gpu_join_*)."""
 
     def __init__(self, key_type: int, payload_types: Sequence[int] = (), strategy: int = 0):
         init()
@@ -439,6 +445,7 @@ class HashJoin:
         s.strategy = strategy
         self.spec = s
         self.payload_types = list(payload_types)
+        self.build_rows = 0
         h = C.c_void_p()
         check(lib().omni_gpu_join_build_create(C.byref(s), C.byref(h)))
         self.h = h.value
@@ -447,6 +454,28 @@ class HashJoin:
         """payloads: sequence of Columns or a prebuilt ctypes array of them."""
         parr = payloads if isinstance(payloads, C.Array) else (Column * max(1, len(payloads)))(*payloads)
         check(lib().omni_gpu_join_build_add(self.h, C.byref(key), parr, rows))
+        self.build_rows += rows
+
+    def set_type(self, join_type: int, build_side: int = 0):
+        """Join type (OMNI_GPU_JOIN_*) and build side (OMNI_GPU_BUILD_*), before finish."""
+        check(lib().omni_gpu_join_set_type(self.h, join_type, build_side))
+
+    def probe_exists(self, key: Column, rows: int) -> np.ndarray:
+        """Existence flags (uint8, one per probe row) of one batch."""
+        out = DeviceArray(max(1, rows), np.uint8, (rows,))
+        check(lib().omni_gpu_join_probe_exists(self.h, C.byref(key), rows, out.ptr))
+        return out.numpy(rows)
+
+    def outer_output(self, cap: Optional[int] = None, with_nulls: bool = True):
+        """Build rows no probe matched: (build row ids, [(payload values, null mask)])."""
+        if cap is None:
+            cap = self.build_rows
+        out_rows, pays, oarr = self._outs(cap, with_nulls)
+        m = C.c_int64()
+        check(lib().omni_gpu_join_outer_output(self.h, out_rows.ptr, oarr, cap, C.byref(m)))
+        n = m.value
+        res = [(v.numpy(n), unpack_nulls(nb.numpy(), n) if nb is not None else np.zeros(n, bool)) for v, nb in pays]
+        return out_rows.numpy(n), res
 
     def finish(self):
         check(lib().omni_gpu_join_build_finish(self.h))

@@ -45,6 +45,10 @@ struct JoinHandle {
     int probe_group = 1;       // probe width G of join_probe
     int partitions = -1;       // radix-partitioned probe: -1 auto, 0 off, else the partition count
     int last_partitions = 0;   // partition count the last probe used (1 = direct probe)
+    int mode = kModeInner;     // probe-side emission rule of the join type (omni_gpu_join_set_type)
+    bool preserve_build = false;  // build-preserving outer join: probes mark matched build rows in visited
+    bool existence = false;    // existence join: probed with omni_gpu_join_probe_exists only
+    DevBuf visited;            // one byte per build row (preserve_build)
     DevBuf part_keys, part_rows, part_counts, part_meta;
     DevBuf ovf_keys, ovf_rows;  // partitioned build: keys whose probe run leaves their region
     uint64_t slots_n = 0;
@@ -146,7 +150,6 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     OG_TRY(Partition(h, a, t, P, nwg, rows, pa, "join_probe_hist", "join_probe_scan", "join_probe_scatter"));
     int lgP = 0;
     while ((1 << lgP) < P) ++lgP;
-    const int per = P / kXcds;
     int cus = 0;
     OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, CurrentDevice()));
     const uint64_t region = h.slots_n >> lgP;
@@ -236,6 +239,43 @@ void LaunchProbe(const JoinHandle &h, dim3 grid, hipStream_t s, const ProbeArgs 
     }
 }
 
+// Payload output descriptors for `cap` output rows; clears the output NULL bitmaps.
+int SetupPayloadOutputs(JoinHandle &h, omni_gpu_out_column *outs, int64_t cap, PayloadDev *pay)
+{
+    hipStream_t s = h.stream;
+    for (int p = 0; p < h.spec.num_payloads; ++p) {
+        OG_CHECK(outs && outs[p].type_id == h.spec.payload_types[p], OMNI_GPU_ERR_INVALID, "payload output %d type mismatch", p);
+        OG_CHECK(!h.pay_nullable[p] || cap == 0 || !outs[p].values || outs[p].nulls, OMNI_GPU_ERR_INVALID,
+                 "payload %d has NULLs: its output needs a null bitmap", p);
+        pay[p].width = TypeWidth(h.spec.payload_types[p]);
+        pay[p].values = h.pay[p].p;
+        pay[p].nulls = h.pay_nullable[p] ? h.pay_null[p].as<uint8_t>() : nullptr;
+        pay[p].out = outs[p].values;
+        pay[p].out_nulls = outs[p].nulls;
+        if (outs[p].values && outs[p].nulls && cap > 0) {
+            int64_t words = (cap + 31) / 32;
+            KernelScope ks(s, "join_clear_nulls");
+            join_fill_i32<<<Blocks(words), kBlock, 0, s>>>(reinterpret_cast<int *>(outs[p].nulls), 0, words);
+            OG_TRY(CheckLaunch("join_fill_i32"));
+        }
+    }
+    return OMNI_GPU_OK;
+}
+
+TableDev MakeTable(const JoinHandle &h)
+{
+    TableDev t{};
+    t.slots = h.slots.as<JoinSlot>();
+    t.mask = h.slots_n ? h.slots_n - 1 : 0;
+    t.heads = h.heads.as<int>();
+    t.min_key = h.min_key;
+    t.range = h.range;
+    t.next = h.next.as<int>();
+    t.sentinel_head = reinterpret_cast<const int *>(h.scratch.as<uint8_t>());
+    t.sentinel_pay = reinterpret_cast<const unsigned int *>(h.scratch.as<uint8_t>() + 12);
+    return t;
+}
+
 int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blocks, int64_t part_blocks,
              long long *out_rows,
              omni_gpu_out_column *outs, int64_t cap, int64_t *num_matches)
@@ -250,22 +290,15 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
     a.out_rows = out_rows;
     a.cap = cap;
     a.counter = reinterpret_cast<unsigned long long *>(h.scratch.as<uint8_t>() + 32);
-    for (int p = 0; p < a.npay; ++p) {
-        OG_CHECK(outs && outs[p].type_id == h.spec.payload_types[p], OMNI_GPU_ERR_INVALID, "payload output %d type mismatch", p);
-        OG_CHECK(!h.pay_nullable[p] || cap == 0 || !outs[p].values || outs[p].nulls, OMNI_GPU_ERR_INVALID,
-                 "payload %d has NULLs: its output needs a null bitmap", p);
-        a.pay[p].width = TypeWidth(h.spec.payload_types[p]);
-        a.pay[p].values = h.pay[p].p;
-        a.pay[p].nulls = h.pay_nullable[p] ? h.pay_null[p].as<uint8_t>() : nullptr;
-        a.pay[p].out = outs[p].values;
-        a.pay[p].out_nulls = outs[p].nulls;
-        if (outs[p].values && outs[p].nulls && cap > 0) {
-            int64_t words = (cap + 31) / 32;
-            KernelScope ks(s, "join_clear_nulls");
-            join_fill_i32<<<Blocks(words), kBlock, 0, s>>>(reinterpret_cast<int *>(outs[p].nulls), 0, words);
-            OG_TRY(CheckLaunch("join_fill_i32"));
+    OG_CHECK(!h.existence, OMNI_GPU_ERR_STATE, "an existence join is probed with omni_gpu_join_probe_exists");
+    if (h.mode == kModeLeft) {  // unmatched probe rows get NULL payloads
+        for (int p = 0; p < a.npay; ++p) {
+            OG_CHECK(cap == 0 || !outs || !outs[p].values || outs[p].nulls, OMNI_GPU_ERR_INVALID,
+                     "outer join: payload output %d needs a null bitmap", p);
         }
     }
+    if (h.mode == kModeSemi || h.mode == kModeAnti) a.npay = 0;  // the probe rows alone are output
+    else OG_TRY(SetupPayloadOutputs(h, outs, cap, a.pay));
     if (nb == 1) {
         a.single = batches[0];
         a.batches = nullptr;
@@ -275,20 +308,25 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
         a.batches = h.batch_desc.as<ProbeBatch>();
     }
     OG_HIP(hipMemsetAsync(a.counter, 0, 8, s));
-    TableDev t{};
-    t.slots = h.slots.as<JoinSlot>();
-    t.mask = h.slots_n ? h.slots_n - 1 : 0;
-    t.heads = h.heads.as<int>();
-    t.min_key = h.min_key;
-    t.range = h.range;
-    t.next = h.next.as<int>();
-    t.sentinel_head = reinterpret_cast<const int *>(h.scratch.as<uint8_t>());
-    t.sentinel_pay = reinterpret_cast<const unsigned int *>(h.scratch.as<uint8_t>() + 12);
+    const TableDev t = MakeTable(h);
     int64_t total_rows = 0;
     for (int i = 0; i < nb; ++i) total_rows += batches[i].rows;
-    const int P = (total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows) : 1;
+    const bool typed = h.mode != kModeInner || h.preserve_build;
+    const int P = (!typed && total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows) : 1;
     h.last_partitions = P;
-    if (P > 1) {
+    if (typed) {
+        if (total_blocks > 0) {  // also with no build rows: outer and anti joins still emit every probe row
+            TypedArgs ty{h.mode, h.n == 0 ? 1 : 0, h.preserve_build ? h.visited.as<uint8_t>() : nullptr};
+            dim3 grid(static_cast<unsigned int>(total_blocks));
+            KernelScope ks(s, "join_probe_typed");
+            if (h.n == 0) join_probe_typed<false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.is_array && h.has_dup) join_probe_typed<true, true><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.is_array) join_probe_typed<true, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.has_dup) join_probe_typed<false, true><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else join_probe_typed<false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            OG_TRY(CheckLaunch("join_probe_typed"));
+        }
+    } else if (P > 1) {
         OG_TRY(LaunchPartitioned(h, a, t, P, part_blocks, total_rows));
     } else if (total_blocks > 0 && h.n > 0) {
         KernelScope ks(s, "join_probe");
@@ -399,6 +437,10 @@ int omni_gpu_join_build_finish(void *handle)
     static_assert(sizeof(init) == 48, "scratch layout");
     OG_TRY(CopyToDevice(sc, &init, sizeof(init), s));
     h.finished = true;
+    if (h.preserve_build && h.n > 0) {
+        OG_TRY(h.visited.Reserve(static_cast<size_t>(h.n), s));
+        OG_HIP(hipMemsetAsync(h.visited.p, 0, static_cast<size_t>(h.n), s));
+    }
     if (h.n == 0) {
         h.max_dup = 0;
         h.pending.clear();
@@ -600,6 +642,95 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
     OG_CHECK(blocks < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many probe rows for one launch");
     return RunProbe(h, bs.data(), num_batches, blocks, pblocks, reinterpret_cast<long long *>(out_probe_rows),
                     out_payloads, out_capacity, num_matches);
+}
+
+int omni_gpu_join_set_type(void *handle, int32_t join_type, int32_t build_side)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    JoinHandle &h = *static_cast<JoinHandle *>(handle);
+    OG_CHECK(!h.finished, OMNI_GPU_ERR_STATE, "the join type is set before the build side finishes");
+    OG_CHECK(join_type >= OMNI_GPU_JOIN_INNER && join_type <= OMNI_GPU_JOIN_EXISTENCE, OMNI_GPU_ERR_INVALID,
+             "join type %d not supported", join_type);
+    OG_CHECK(build_side >= OMNI_GPU_BUILD_UNKNOWN && build_side <= OMNI_GPU_BUILD_RIGHT, OMNI_GPU_ERR_INVALID,
+             "build side %d not supported", build_side);
+    // lookup_join.cpp:344-367 and hash_builder.cpp:36-40: LEFT / RIGHT keep the build side's rows when the build
+    // is on that side, else the probe side's; FULL keeps both
+    const bool same = (join_type == OMNI_GPU_JOIN_LEFT && build_side == OMNI_GPU_BUILD_LEFT) ||
+                      (join_type == OMNI_GPU_JOIN_RIGHT && build_side == OMNI_GPU_BUILD_RIGHT);
+    h.mode = kModeInner;
+    h.preserve_build = false;
+    h.existence = false;
+    switch (join_type) {
+        case OMNI_GPU_JOIN_LEFT:
+        case OMNI_GPU_JOIN_RIGHT:
+            if (same) h.preserve_build = true;
+            else h.mode = kModeLeft;
+            break;
+        case OMNI_GPU_JOIN_FULL:
+            h.mode = kModeLeft;
+            h.preserve_build = true;
+            break;
+        case OMNI_GPU_JOIN_LEFT_SEMI: h.mode = kModeSemi; break;
+        case OMNI_GPU_JOIN_LEFT_ANTI: h.mode = kModeAnti; break;
+        case OMNI_GPU_JOIN_EXISTENCE: h.existence = true; break;
+        default: break;
+    }
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_join_probe_exists(void *handle, const omni_gpu_column *probe_key, int64_t rows, uint8_t *out_exists)
+{
+    OG_CHECK(handle && probe_key, OMNI_GPU_ERR_INVALID, "null argument");
+    JoinHandle &h = *static_cast<JoinHandle *>(handle);
+    OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "probe before the build side finished");
+    OG_CHECK(rows == 0 || out_exists, OMNI_GPU_ERR_INVALID, "null out_exists");
+    ProbeBatch b{};
+    OG_TRY(FillBatch(b, *probe_key, rows, 0, 0, 0, h.spec.key_type));
+    OG_CHECK(rows < (1LL << 31) * kBlock, OMNI_GPU_ERR_INVALID, "probe batch too large");
+    hipStream_t s = h.stream;
+    if (rows > 0) {
+        const TableDev t = MakeTable(h);
+        const int empty = h.n == 0 ? 1 : 0;
+        KernelScope ks(s, "join_probe_exists");
+        if (h.is_array) join_probe_exists<true><<<Blocks(rows), kBlock, 0, s>>>(b, t, empty, out_exists);
+        else join_probe_exists<false><<<Blocks(rows), kBlock, 0, s>>>(b, t, empty, out_exists);
+        OG_TRY(CheckLaunch("join_probe_exists"));
+    }
+    OG_HIP(hipStreamSynchronize(s));
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_join_outer_output(void *handle, int64_t *out_build_rows, omni_gpu_out_column *out_payloads,
+                               int64_t out_capacity, int64_t *num_rows)
+{
+    OG_CHECK(handle && num_rows, OMNI_GPU_ERR_INVALID, "null argument");
+    JoinHandle &h = *static_cast<JoinHandle *>(handle);
+    OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "outer output before the build side finished");
+    OG_CHECK(h.preserve_build, OMNI_GPU_ERR_STATE, "outer output needs a build-preserving outer join type");
+    OG_CHECK(out_capacity >= 0, OMNI_GPU_ERR_INVALID, "negative out_capacity");
+    hipStream_t s = h.stream;
+    OuterArgs g{};
+    g.visited = h.visited.as<uint8_t>();
+    g.n = h.n;
+    g.out_rows = reinterpret_cast<long long *>(out_build_rows);
+    g.npay = h.spec.num_payloads;
+    g.counter = reinterpret_cast<unsigned long long *>(h.scratch.as<uint8_t>() + 32);
+    g.cap = out_capacity;
+    OG_TRY(SetupPayloadOutputs(h, out_payloads, out_capacity, g.pay));
+    OG_HIP(hipMemsetAsync(g.counter, 0, 8, s));
+    if (h.n > 0) {
+        KernelScope ks(s, "join_outer_emit");
+        join_outer_emit<<<Blocks(h.n), kBlock, 0, s>>>(g);
+        OG_TRY(CheckLaunch("join_outer_emit"));
+    }
+    unsigned long long m = 0;
+    OG_TRY(CopyToHost(s, {{&m, g.counter, 8}}));
+    *num_rows = static_cast<int64_t>(m);
+    if (static_cast<int64_t>(m) > out_capacity) {
+        return SetError(OMNI_GPU_ERR_OUTPUT_FULL, "outer output has %llu rows for an output capacity of %lld", m,
+                        static_cast<long long>(out_capacity));
+    }
+    return OMNI_GPU_OK;
 }
 
 int omni_gpu_join_destroy(void *handle)

@@ -500,6 +500,170 @@ __global__ void __launch_bounds__(kBlock) join_probe(ProbeArgs a, TableDev t)
     EmitTile<INLINE, DUPS>(a, t, head, pay, [&](int j) { return rowBase + j * kBlock; }, sh);
 }
 
+// ---- join types other than inner (lookup_join.cpp:1037-1438, lookup_outer_join.cpp:155-320) ------------------
+// Probe-side emission rules per probe row (a NULL key matches nothing):
+//   kModeInner  one output row per matching build row;
+//   kModeLeft   the same, or one row with NULL payloads when nothing matches (probe-preserving outer joins:
+//               LEFT / RIGHT with the build on the other side, and FULL; ProbeBatchForOppositeSideOuterJoin,
+//               ProbeBatchForFullJoin);
+//   kModeSemi   one row (the probe row alone) when any build row matches (ProbeBatchForLeftSemiJoin);
+//   kModeAnti   one row (the probe row alone) when none matches (ProbeBatchForLeftAntiJoin).
+// With `visited` set, every matching build row is marked (build-preserving outer joins: LEFT / RIGHT with the
+// build on the same side, and FULL; RowRefListWithFlags::visited, join_hash_table_variants.h:129-137), and
+// join_outer_emit later writes the build rows never marked.
+constexpr int kModeInner = 0, kModeLeft = 1, kModeSemi = 2, kModeAnti = 3;
+
+struct TypedArgs {
+    int mode;
+    int empty;           // the build side has no rows: nothing matches (no table to read)
+    uint8_t *visited;    // one byte per build row, or nullptr
+};
+
+// Payload values of build row h into output row o (NULL bits from the build store's NULL bytes).
+__device__ __forceinline__ void CopyPayloads(const PayloadDev *pay, int npay, long long o, long long h)
+{
+    for (int p = 0; p < npay; ++p) {
+        const PayloadDev &pd = pay[p];
+        if (pd.out == nullptr) continue;
+        CopyValue(pd.values, pd.out, pd.width, h, o);
+        if (pd.nulls && pd.out_nulls && pd.nulls[h]) {
+            atomicOr(reinterpret_cast<unsigned int *>(pd.out_nulls) + (o >> 5), 1u << (o & 31));
+        }
+    }
+}
+
+// A probe row without a build row: payload values zero, NULL bits set (the output bitmaps are required).
+__device__ __forceinline__ void WriteNullMatch(const ProbeArgs &a, long long o, long long probeRow)
+{
+    if (o >= a.cap) return;
+    a.out_rows[o] = probeRow;
+    for (int p = 0; p < a.npay; ++p) {
+        const PayloadDev &pd = a.pay[p];
+        if (pd.out == nullptr) continue;
+        switch (pd.width) {
+            case 8: static_cast<unsigned long long *>(pd.out)[o] = 0; break;
+            case 4: static_cast<unsigned int *>(pd.out)[o] = 0; break;
+            case 2: static_cast<unsigned short *>(pd.out)[o] = 0; break;
+            case 16:
+                static_cast<unsigned long long *>(pd.out)[2 * o] = 0;
+                static_cast<unsigned long long *>(pd.out)[2 * o + 1] = 0;
+                break;
+            default: static_cast<uint8_t *>(pd.out)[o] = 0; break;
+        }
+        atomicOr(reinterpret_cast<unsigned int *>(pd.out_nulls) + (o >> 5), 1u << (o & 31));
+    }
+}
+
+// Direct probe of one tile (kProbeTile rows per workgroup, the launch shape of join_probe) under the rules
+// above: per-lane output counts, a wave-level exclusive scan and one atomic per workgroup reserve each lane's
+// output rows, which the lane then writes in row order.
+template <bool ARRAY, bool DUPS>
+__global__ void __launch_bounds__(kBlock) join_probe_typed(ProbeArgs a, TableDev t, TypedArgs ty)
+{
+    __shared__ unsigned long long sh[kBlock / 64 + 1];
+    const ProbeBatch b = BatchOf(a, blockIdx.x);
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
+    int head[kProbeRows];
+    bool inr[kProbeRows];
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        const int64_t row = tile0 + j * kBlock + threadIdx.x;
+        long long key[1];
+        bool live[1];
+        unsigned int pay[1];
+        LoadProbeKeys<1>(b, row, key, live);
+        inr[j] = row < b.rows;
+        head[j] = -1;
+        if (!ty.empty) LookupGroup<ARRAY, 1>(t, key, live, head + j, pay);
+    }
+    unsigned int mine = 0;
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        unsigned int len = head[j] >= 0 ? 1u : 0u;
+        if (DUPS && len) {
+            len = 0;
+            for (int h = head[j]; h >= 0; h = t.next[h]) ++len;
+        }
+        if (ty.mode == kModeInner) mine += len;
+        else if (ty.mode == kModeLeft) mine += inr[j] ? (len ? len : 1u) : 0u;
+        else if (ty.mode == kModeSemi) mine += len ? 1u : 0u;
+        else mine += (inr[j] && !len) ? 1u : 0u;
+    }
+    const int lane = LaneId();
+    unsigned int incl = mine;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        unsigned int up = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += up;
+    }
+    const unsigned int waveTotal = __builtin_amdgcn_readlane(incl, 63);
+    long long o = static_cast<long long>(BlockReserve<kBlock>(a.counter, waveTotal, sh) + (incl - mine));
+    const long long rowBase = b.row_base + tile0 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        const long long pr = rowBase + j * kBlock;
+        if (ty.mode == kModeSemi || ty.mode == kModeAnti) {
+            const bool out = ty.mode == kModeSemi ? head[j] >= 0 : (inr[j] && head[j] < 0);
+            if (out) {
+                if (o < a.cap) a.out_rows[o] = pr;
+                ++o;
+            }
+            continue;
+        }
+        for (int h = head[j]; h >= 0; h = DUPS ? t.next[h] : -1) {
+            if (o < a.cap) {
+                a.out_rows[o] = pr;
+                CopyPayloads(a.pay, a.npay, o, h);
+            }
+            ++o;
+            if (ty.visited) ty.visited[h] = 1;
+        }
+        if (ty.mode == kModeLeft && inr[j] && head[j] < 0) WriteNullMatch(a, o++, pr);
+    }
+}
+
+// Existence join (ProbeBatchForExistenceJoin, lookup_join.cpp:1393-1438): out[i] = 1 when probe row i has a
+// matching build row, else 0 (also for a NULL key).  One row per thread.
+template <bool ARRAY>
+__global__ void __launch_bounds__(kBlock) join_probe_exists(ProbeBatch b, TableDev t, int empty, uint8_t *out)
+{
+    const int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    if (row >= b.rows) return;
+    long long key[1];
+    bool live[1];
+    int head[1] = {-1};
+    unsigned int pay[1];
+    LoadProbeKeys<1>(b, row, key, live);
+    if (!empty) LookupGroup<ARRAY, 1>(t, key, live, head, pay);
+    out[row] = head[0] >= 0 ? 1 : 0;
+}
+
+struct OuterArgs {
+    const uint8_t *visited;
+    int64_t n;                  // build rows
+    long long *out_rows;        // build row ids, or nullptr
+    int npay;
+    PayloadDev pay[OMNI_GPU_MAX_AGGS];
+    unsigned long long *counter;
+    long long cap;
+};
+
+// LookupOuterJoinOperator::GetOutput (lookup_outer_join.cpp:155-320): every build row never marked by a probe,
+// NULL-key rows included (they never match), with its payloads; output order unspecified.  One row per thread.
+__global__ void __launch_bounds__(kBlock) join_outer_emit(OuterArgs g)
+{
+    __shared__ unsigned long long sh[kBlock / 64 + 1];
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x;
+    const bool emit = i < g.n && g.visited[i] == 0;
+    const unsigned long long m = __ballot(emit);
+    const unsigned long long base = BlockReserve<kBlock>(g.counter, static_cast<unsigned int>(__popcll(m)), sh);
+    if (!emit) return;
+    const long long o = static_cast<long long>(base + __popcll(m & LaneMaskLt()));
+    if (o >= g.cap) return;
+    if (g.out_rows) g.out_rows[o] = i;
+    CopyPayloads(g.pay, g.npay, o, i);
+}
+
 // ---- radix-partitioned probe (hash tables larger than the L2 caches) -------------------------------------
 // Partition p of a probe key is the top bits of its home slot (slot >> shift), so partition p only reads
 // table region [p << shift, (p + 1) << shift) (plus the rare probe run past its end).  The probe rows are

@@ -142,6 +142,9 @@ def lib():
             "oracle_join_is_array": (i32, [vp]),
             "oracle_join_probe": (i64, [vp, vp, i64, i64, vp, vp, i64]),
             "oracle_join_gather": (None, [vp, i32, vp, i64, vp, vp]),
+            "oracle_join_probe_typed": (i64, [vp, vp, i64, i64, i32, i32, vp, vp, i64]),
+            "oracle_join_exists": (None, [vp, vp, i64, vp]),
+            "oracle_join_outer": (i64, [vp, vp, i64]),
             "oracle_join_destroy": (None, [vp]),
             "oracle_last_error": (C.c_char_p, []),
         }
@@ -339,5 +342,93 @@ def hash_join(build_key: Col, build_payloads: Sequence[Col], probe_key: Col, bui
             o.nulls = unpack_nulls(bm, m)
             pays.append(o)
         return JoinResult(probe_rows, build_rows, pays, is_array)
+    finally:
+        L.oracle_join_destroy(h)
+
+
+# JoinType / BuildSide values (core/src/plannode/planNode.h:406-414, 28-32).
+JOIN_INNER, JOIN_LEFT, JOIN_RIGHT, JOIN_FULL, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI, JOIN_EXISTENCE = range(7)
+BUILD_UNKNOWN, BUILD_LEFT, BUILD_RIGHT = 0, 1, 2
+
+
+def join_mode(join_type: int, build_side: int):
+    """(probe emission mode, marks visited build rows) of a join type, restated from LookupJoinOperator::ProcessProbe
+    (core/src/operator/join/lookup_join.cpp:344-367: LEFT / RIGHT with the build on the same side take the
+    same-side loop, otherwise the opposite-side loop) and HashBuilderOperatorFactory (hash_builder.cpp:36-40:
+    visited flags for FULL and the same-side outer joins).  Modes: 0 each match, 1 each match or one row without
+    a build row, 2 semi, 3 anti; None for EXISTENCE."""
+    same = (join_type == JOIN_LEFT and build_side == BUILD_LEFT) or \
+        (join_type == JOIN_RIGHT and build_side == BUILD_RIGHT)
+    if join_type in (JOIN_LEFT, JOIN_RIGHT):
+        return (0, True) if same else (1, False)
+    return {JOIN_INNER: (0, False), JOIN_FULL: (1, True), JOIN_LEFT_SEMI: (2, False),
+            JOIN_LEFT_ANTI: (3, False), JOIN_EXISTENCE: (None, False)}[join_type]
+
+
+@dataclass
+class TypedJoinResult:
+    probe_rows: np.ndarray            # probe row of each output row
+    build_rows: np.ndarray            # its build row, -1 when the build columns are NULL
+    payloads: List[Col]               # build payloads of each output row
+    outer_build_rows: np.ndarray      # build rows the outer pass emits (build-preserving types), else empty
+    outer_payloads: List[Col]
+    exists: Optional[np.ndarray]      # EXISTENCE: one flag per probe row
+
+
+def hash_join_typed(build_key: Col, build_payloads: Sequence[Col], probe_key: Col, build_batch: int,
+                    probe_batch: int, join_type: int, build_side: int = BUILD_UNKNOWN) -> TypedJoinResult:
+    """Restated HashBuilderOperator + LookupJoinOperator (+ LookupOuterJoinOperator) for any join type."""
+    mode, mark = join_mode(join_type, build_side)
+    L = lib()
+    h = L.oracle_join_create(build_key.type_id)
+    try:
+        nb = len(build_key)
+        for a in list(range(0, nb, build_batch)) or [0]:  # an empty build side still declares its payload types
+            b = min(nb, a + build_batch)
+            karr, k1 = _ocols([build_key.slice(a, b)], 1)
+            parr, k2 = _ocols([p.slice(a, b) for p in build_payloads], max(1, len(build_payloads)))
+            L.oracle_join_add_batch(h, karr, parr, len(build_payloads), b - a)
+        L.oracle_join_finish(h)
+        npr = len(probe_key)
+        if mode is None:
+            ex = np.zeros(max(1, npr), np.uint8)
+            for a in range(0, npr, probe_batch):
+                b = min(npr, a + probe_batch)
+                karr, k1 = _ocols([probe_key.slice(a, b)], 1)
+                L.oracle_join_exists(h, karr, b - a, _ptr(ex[a:b]))
+            return TypedJoinResult(np.zeros(0, np.int64), np.zeros(0, np.int64), [], np.zeros(0, np.int64), [],
+                                   ex[:npr])
+        np_rows, nb_rows = [], []
+        for a in range(0, npr, probe_batch):
+            b = min(npr, a + probe_batch)
+            karr, k1 = _ocols([probe_key.slice(a, b)], 1)
+            m = L.oracle_join_probe_typed(h, karr, b - a, a, mode, int(mark), None, None, 0)
+            pr = np.zeros(max(m, 1), np.int64)
+            br = np.zeros(max(m, 1), np.int64)
+            # the visited marks of the counting call are the same as the second call's
+            L.oracle_join_probe_typed(h, karr, b - a, a, mode, int(mark), _ptr(pr), _ptr(br), m)
+            np_rows.append(pr[:m])
+            nb_rows.append(br[:m])
+        probe_rows = np.concatenate(np_rows) if np_rows else np.zeros(0, np.int64)
+        build_rows = np.concatenate(nb_rows) if nb_rows else np.zeros(0, np.int64)
+
+        def gather(rows):
+            out = []
+            for p, bp in enumerate(build_payloads):
+                o = alloc_out(bp.type_id, len(rows))
+                bm = np.zeros((len(rows) + 7) // 8 + 1, np.uint8)
+                L.oracle_join_gather(h, p, _ptr(np.ascontiguousarray(rows)), len(rows), _ptr(o.values), _ptr(bm))
+                o.nulls = unpack_nulls(bm, len(rows))
+                out.append(o)
+            return out
+
+        pays = gather(build_rows) if mode in (0, 1) else []
+        outer = np.zeros(0, np.int64)
+        if mark:
+            m = L.oracle_join_outer(h, None, 0)
+            outer = np.zeros(max(1, m), np.int64)
+            L.oracle_join_outer(h, _ptr(outer), m)
+            outer = outer[:m]
+        return TypedJoinResult(probe_rows, build_rows, pays, outer, gather(outer) if mark else [], None)
     finally:
         L.oracle_join_destroy(h)

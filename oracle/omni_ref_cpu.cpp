@@ -1,4 +1,4 @@
-// CPU restatement of OmniOperator's hash-aggregation and inner hash-join hot path.
+// CPU restatement of OmniOperator's hash-aggregation and hash-join hot path.
 //
 // TEST INFRASTRUCTURE ONLY.  This is the oracle the GPU path is checked against and the CPU
 // baseline bench.py times.  Nothing under omnioperator_amd/ links or calls it.
@@ -969,18 +969,103 @@ public:
         return m;
     }
 
-    // ConstructBuildColumn gather (lookup_join.cpp:1667-1779).
+    // Join types other than inner, restated from the per-type probe loops (no join filter, single table):
+    //   mode 1 ProbeBatchForOppositeSideOuterJoin / ProbeBatchForFullJoin (lookup_join.cpp:1037-1108, 1178-1240):
+    //          NULL key -> one row without a build row; each match; no match -> one row without a build row;
+    //   mode 2 ProbeBatchForLeftSemiJoin (:1242-1296): one row at the first match, NULL keys skipped;
+    //   mode 3 ProbeBatchForLeftAntiJoin (:1298-1358): one row when nothing matches, NULL keys included;
+    //   mode 0 ProbeBatchForSameSideOuterJoin (:1110-1176): each match, NULL keys skipped.
+    // outBuild = -1 marks a row without a build row (its build columns are NULL).  markVisited records each
+    // matched build row (RowRefListWithFlags::visited via PositionVisited, join_hash_table_variants.h:129-137).
+    int64_t ProbeTyped(const HostColumn &key, int64_t rows, int64_t rowBase, int32_t mode, bool markVisited,
+                       int64_t *outProbe, int64_t *outBuild, int64_t cap)
+    {
+        if (markVisited && visited_.size() != keys_.size()) visited_.assign(keys_.size(), 0);
+        int64_t m = 0;
+        auto emit = [&](int64_t probe, int64_t build) {
+            if (m < cap) {
+                outProbe[m] = probe;
+                outBuild[m] = build;
+            }
+            ++m;
+        };
+        for (int64_t r = 0; r < rows; ++r) {
+            const int64_t pr = rowBase + r;
+            if (IsNull(key, r)) {
+                if (mode == 1 || mode == 3) emit(pr, -1);
+                continue;
+            }
+            const int64_t head = Lookup(LoadInt(key, r));
+            if (mode == 2) {
+                if (head >= 0) emit(pr, -1);
+                continue;
+            }
+            if (mode == 3) {
+                if (head < 0) emit(pr, -1);
+                continue;
+            }
+            for (int64_t b = head; b >= 0; b = next_[b]) {
+                emit(pr, b);
+                if (markVisited) visited_[b] = 1;
+            }
+            if (mode == 1 && head < 0) emit(pr, -1);
+        }
+        return m;
+    }
+
+    // ProbeBatchForExistenceJoin (lookup_join.cpp:1393-1438): AppendExistenceRow<matched> per probe row.
+    void Exists(const HostColumn &key, int64_t rows, uint8_t *out) const
+    {
+        for (int64_t r = 0; r < rows; ++r) out[r] = (!IsNull(key, r) && Lookup(LoadInt(key, r)) >= 0) ? 1 : 0;
+    }
+
+    // LookupOuterJoinOperator::GetOutput (lookup_outer_join.cpp:155-320, 290-325): the build rows never visited,
+    // including the NULL-key rows, which the build-preserving types keep in the null-key slot
+    // (join_hash_table_variants.cpp:80-82, 286-330) where no probe reaches them.  Build-row order.
+    int64_t Outer(int64_t *outBuild, int64_t cap)
+    {
+        if (visited_.size() != keys_.size()) visited_.assign(keys_.size(), 0);
+        int64_t m = 0;
+        for (size_t b = 0; b < keys_.size(); ++b) {
+            if (visited_[b]) continue;
+            if (m < cap) outBuild[m] = static_cast<int64_t>(b);
+            ++m;
+        }
+        return m;
+    }
+
+    // ConstructBuildColumn gather (lookup_join.cpp:1667-1779); build row -1 gathers a NULL.
     void GatherPayload(int32_t p, const int64_t *buildRows, int64_t m, void *out, uint8_t *nulls) const
     {
         int w = TypeWidth(payTypes_[p]);
         std::memset(nulls, 0, (m + 7) / 8);
         for (int64_t i = 0; i < m; ++i) {
+            if (buildRows[i] < 0) {
+                std::memset(static_cast<uint8_t *>(out) + w * i, 0, w);
+                nulls[i >> 3] |= static_cast<uint8_t>(1u << (i & 7));
+                continue;
+            }
             std::memcpy(static_cast<uint8_t *>(out) + w * i, &pay_[p][static_cast<size_t>(buildRows[i]) * w], w);
             if (payNull_[p][buildRows[i]]) nulls[i >> 3] |= static_cast<uint8_t>(1u << (i & 7));
         }
     }
 
 private:
+    // First build row of a non-NULL key, or -1 (array table or Swiss table, as Finish chose).
+    int64_t Lookup(int64_t k) const
+    {
+        if (isArray_) {
+            int128 idx = static_cast<int128>(k) - min_;
+            if (idx >= 0 && idx < static_cast<int128>(heads_.size())) return heads_[static_cast<size_t>(idx)];
+            return -1;
+        }
+        if (map_) {
+            const uint32_t *v = map_->Find(k);
+            if (v) return listHeads_[*v];
+        }
+        return -1;
+    }
+
     void Link(int64_t &head, int64_t &tail, int64_t row)
     {
         if (head < 0) head = row;
@@ -998,6 +1083,7 @@ private:
     int64_t min_ = 0;
     std::vector<int64_t> heads_, tails_, next_;
     std::unique_ptr<SwissMap<int64_t, JoinIntHasher<int64_t>>> map_;
+    std::vector<uint8_t> visited_;
     std::vector<int64_t> listHeads_, listTails_;
 };
 
@@ -1132,6 +1218,22 @@ int64_t oracle_join_probe(void *h, const oracle_column *key, int64_t rows, int64
 void oracle_join_gather(void *h, int32_t p, const int64_t *build_rows, int64_t m, void *out, uint8_t *nulls)
 {
     static_cast<HashJoin *>(h)->GatherPayload(p, build_rows, m, out, nulls);
+}
+int64_t oracle_join_probe_typed(void *h, const oracle_column *key, int64_t rows, int64_t row_base, int32_t mode,
+                                int32_t mark_visited, int64_t *out_probe, int64_t *out_build, int64_t cap)
+{
+    HostColumn k{key->type_id, key->scale, key->values, key->nulls};
+    return static_cast<HashJoin *>(h)->ProbeTyped(k, rows, row_base, mode, mark_visited != 0, out_probe, out_build,
+                                                  cap);
+}
+void oracle_join_exists(void *h, const oracle_column *key, int64_t rows, uint8_t *out)
+{
+    HostColumn k{key->type_id, key->scale, key->values, key->nulls};
+    static_cast<HashJoin *>(h)->Exists(k, rows, out);
+}
+int64_t oracle_join_outer(void *h, int64_t *out_build, int64_t cap)
+{
+    return static_cast<HashJoin *>(h)->Outer(out_build, cap);
 }
 void oracle_join_destroy(void *h) { delete static_cast<HashJoin *>(h); }
 
