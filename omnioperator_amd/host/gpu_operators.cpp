@@ -228,14 +228,25 @@ GpuHashBuilderOperatorFactory *GpuHashBuilderOperatorFactory::CreateHashBuilderO
     JoinType joinType, const type::DataTypes &buildTypes, const int32_t *buildHashCols, int32_t buildHashColsCount,
     int32_t operatorCount)
 {
-    if (joinType != OMNI_JOIN_TYPE_INNER) throw OmniException("UNSUPPORTED_ERROR", "only inner joins are on the GPU path");
-    if (buildHashColsCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one join key column is supported");
-    if (operatorCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one build partition is supported");
-    return new GpuHashBuilderOperatorFactory(buildTypes, buildHashCols[0]);
+    return CreateHashBuilderOperatorFactory(joinType, OMNI_BUILD_UNKNOWN, buildTypes, buildHashCols,
+                                            buildHashColsCount, operatorCount);
 }
 
-GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(const type::DataTypes &buildTypes, int32_t keyCol)
-    : buildTypes_(buildTypes), keyCol_(keyCol)
+GpuHashBuilderOperatorFactory *GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(
+    JoinType joinType, BuildSide buildSide, const type::DataTypes &buildTypes, const int32_t *buildHashCols,
+    int32_t buildHashColsCount, int32_t operatorCount)
+{
+    if (operatorCount <= 0 || operatorCount > 10000) {  // hash_builder.cpp:33-35
+        throw OmniException("OPERATOR_RUNTIME_ERROR", "operatorCount is not in the acceptable range [1, 10000].");
+    }
+    if (buildHashColsCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one join key column is supported");
+    if (operatorCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one build partition is supported");
+    return new GpuHashBuilderOperatorFactory(joinType, buildSide, buildTypes, buildHashCols[0]);
+}
+
+GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide,
+                                                             const type::DataTypes &buildTypes, int32_t keyCol)
+    : joinType_(joinType), buildSide_(buildSide), buildTypes_(buildTypes), keyCol_(keyCol)
 {
     EnsureInit();
     omni_gpu_join_spec spec{};
@@ -250,6 +261,7 @@ GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(const type::DataTyp
     }
     spec.strategy = 0;
     Check(omni_gpu_join_build_create(&spec, &handle_), "omni_gpu_join_build_create");
+    Check(omni_gpu_join_set_type(handle_, joinType, buildSide), "omni_gpu_join_set_type");
 }
 
 GpuHashBuilderOperatorFactory::~GpuHashBuilderOperatorFactory()
@@ -317,6 +329,10 @@ GpuLookupJoinOperatorFactory *GpuLookupJoinOperatorFactory::CreateLookupJoinOper
             throw OmniException("INVALID_ARGUMENT", "build output type does not match the build column");
         }
     }
+    if (f->builder_->GetJoinType() == OMNI_JOIN_TYPE_LEFT_SEMI && buildOutputColsCount > 0) {
+        // the reference's semi join would emit its first matching build row's columns
+        throw OmniException("UNSUPPORTED_ERROR", "semi joins with build output columns are not on the GPU path");
+    }
     return f;
 }
 
@@ -341,6 +357,19 @@ int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
     Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
     int64_t cap = std::max<int64_t>(1, rows * std::max<int64_t>(1, max_dup));
     omni_gpu_column key = Upload(vecBatch->Get(f_.probeKeyCol_), rows, keyBuf_, nullBuf_);
+    if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) {  // one output row per probe row
+        rowsBuf_.Reserve(static_cast<size_t>(rows));
+        Check(omni_gpu_join_probe_exists(f_.builder_->Table(), &key, rows, static_cast<uint8_t *>(rowsBuf_.Get())),
+              "omni_gpu_join_probe_exists");
+        exists_.resize(rows);
+        Check(omni_gpu_memcpy_d2h(exists_.data(), rowsBuf_.Get(), static_cast<size_t>(rows)), "download flags");
+        matchRows_.resize(rows);
+        for (int32_t r = 0; r < rows; ++r) matchRows_[r] = r;
+        matches_ = rows;
+        return 0;
+    }
+    // outer joins emit a row per unmatched probe row as well
+    cap = std::max<int64_t>(cap, rows);
     rowsBuf_.Reserve(sizeof(int64_t) * cap);
     const auto &btypes = f_.builder_->BuildTypes();
     int32_t nb = btypes.GetSize();
@@ -391,6 +420,7 @@ int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
     int32_t rowSize = 0;
     for (int32_t c : f_.probeOutputCols_) rowSize += f_.probeTypes_.GetType(c)->GetId() ? type::TypeWidth(f_.probeTypes_.GetType(c)->GetId()) : 0;
     for (int32_t c : f_.buildOutputCols_) rowSize += type::TypeWidth(btypes.GetType(c)->GetId());
+    if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) rowSize += 1;
     int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(std::max(rowSize, 1)), matches_ - emitted_));
     auto *out = new vec::VectorBatch(count);
     for (int32_t c : f_.probeOutputCols_) {
@@ -404,18 +434,142 @@ int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
         }
         out->Append(v);
     }
+    const JoinType jt = f_.builder_->GetJoinType();
+    if (jt == OMNI_JOIN_TYPE_EXISTENCE) {  // ConstructExistenceColumn (lookup_join.cpp:1997-2004)
+        auto *v = vec::VectorHelper::CreateFlatVector(type::OMNI_BOOLEAN, count);
+        std::memcpy(v->RawValues(), exists_.data() + emitted_, static_cast<size_t>(count));
+        out->Append(v);
+    } else {
+        for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
+            int32_t tid = btypes.GetType(f_.buildOutputCols_[i])->GetId();
+            auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
+            int w = v->Width();
+            if (jt == OMNI_JOIN_TYPE_LEFT_ANTI) {  // AppendRow(probePosition, nullptr, 0): build columns NULL
+                std::memset(v->RawValues(), 0, static_cast<size_t>(w) * count);
+                for (int32_t r = 0; r < count; ++r) v->SetNull(r);
+                out->Append(v);
+                continue;
+            }
+            std::memcpy(v->RawValues(), matchPay_[i].data() + static_cast<size_t>(w) * emitted_,
+                        static_cast<size_t>(w) * count);
+            for (int32_t r = 0; r < count; ++r) {
+                int64_t m = emitted_ + r;
+                if ((matchPayNulls_[i][m >> 3] >> (m & 7)) & 1) v->SetNull(r);
+            }
+            out->Append(v);
+        }
+        if (f_.builder_->GetBuildSide() == OMNI_BUILD_LEFT) {  // lookup_join.cpp:1959-1962: build columns first
+            out->RotateLeft(static_cast<int32_t>(f_.probeOutputCols_.size()));
+        }
+    }
+    emitted_ += count;
+    *result = out;
+    return 0;
+}
+
+// ================================================================================================
+// Hash join: outer pass over unmatched build rows
+// ================================================================================================
+
+GpuLookupOuterJoinOperatorFactory *GpuLookupOuterJoinOperatorFactory::CreateLookupOuterJoinOperatorFactory(
+    const type::DataTypes &probeTypes, int32_t *probeOutputCols, int32_t probeOutputColsCount,
+    int32_t *buildOutputCols, const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr)
+{
+    auto *f = new GpuLookupOuterJoinOperatorFactory();
+    f->probeTypes_ = probeTypes;
+    f->probeOutputCols_.assign(probeOutputCols, probeOutputCols + probeOutputColsCount);
+    f->builder_ = reinterpret_cast<GpuHashBuilderOperatorFactory *>(hashBuilderFactoryAddr);
+    f->buildOutputCols_.assign(buildOutputCols, buildOutputCols + buildOutputTypes.GetSize());
+    for (int32_t i = 0; i < buildOutputTypes.GetSize(); ++i) {
+        if (buildOutputTypes.GetType(i)->GetId() != f->builder_->BuildTypes().GetType(buildOutputCols[i])->GetId()) {
+            throw OmniException("INVALID_ARGUMENT", "build output type does not match the build column");
+        }
+    }
+    return f;
+}
+
+Operator *GpuLookupOuterJoinOperatorFactory::CreateOperator() { return new GpuLookupOuterJoinOperator(*this); }
+
+// lookup_outer_join.cpp:151: the outer operator takes no input.
+int32_t GpuLookupOuterJoinOperator::AddInput(vec::VectorBatch *vecBatch)
+{
+    vec::VectorHelper::FreeVecBatch(vecBatch);
+    return 0;
+}
+
+// LookupOuterJoinOperator::GetOutput / BuildVecBatch (lookup_outer_join.cpp:155-215): batches of the unmatched
+// build rows, probe output columns NULL, build output columns first when the build side is LEFT.
+int32_t GpuLookupOuterJoinOperator::GetOutput(vec::VectorBatch **result)
+{
+    *result = nullptr;
+    if (!f_.builder_->Finished()) throw OmniException("OPERATOR_RUNTIME_ERROR", "outer output before the build finished");
+    const auto &btypes = f_.builder_->BuildTypes();
+    if (!fetched_) {
+        fetched_ = true;
+        int32_t is_array = 0;
+        int64_t max_dup = 0, slots = 0;
+        Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
+        int32_t nb = btypes.GetSize();
+        std::vector<omni_gpu_out_column> outs(nb);
+        for (int32_t c = 0; c < nb; ++c) outs[c].type_id = btypes.GetType(c)->GetId();
+        // first call with no capacity returns the row count
+        int64_t rows = 0;
+        int rc = omni_gpu_join_outer_output(f_.builder_->Table(), nullptr, outs.data(), 0, &rows);
+        if (rc != OMNI_GPU_OK && rc != OMNI_GPU_ERR_OUTPUT_FULL) Check(rc, "omni_gpu_join_outer_output");
+        rows_ = rows;
+        std::vector<std::unique_ptr<DeviceBuffer>> vals, nulls;
+        for (int32_t col : f_.buildOutputCols_) {
+            vals.emplace_back(new DeviceBuffer());
+            nulls.emplace_back(new DeviceBuffer());
+            vals.back()->Reserve(std::max<size_t>(1, static_cast<size_t>(rows) * type::TypeWidth(outs[col].type_id)));
+            nulls.back()->Reserve(4 * ((rows + 31) / 32) + 4);
+            outs[col].values = vals.back()->Get();
+            outs[col].nulls = static_cast<uint8_t *>(nulls.back()->Get());
+        }
+        if (rows > 0) {
+            Check(omni_gpu_join_outer_output(f_.builder_->Table(), nullptr, outs.data(), rows, &rows),
+                  "omni_gpu_join_outer_output");
+        }
+        pay_.assign(f_.buildOutputCols_.size(), {});
+        payNulls_.assign(f_.buildOutputCols_.size(), {});
+        for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
+            int w = type::TypeWidth(outs[f_.buildOutputCols_[i]].type_id);
+            pay_[i].resize(static_cast<size_t>(rows) * w);
+            payNulls_[i].resize((rows + 7) / 8 + 1);
+            if (rows > 0) {
+                Check(omni_gpu_memcpy_d2h(pay_[i].data(), vals[i]->Get(), pay_[i].size()), "download payload");
+                Check(omni_gpu_memcpy_d2h(payNulls_[i].data(), nulls[i]->Get(), (rows + 7) / 8), "download nulls");
+            }
+        }
+    }
+    if (emitted_ >= rows_) {
+        SetStatus(OMNI_STATUS_FINISHED);
+        return 0;
+    }
+    int32_t rowSize = 0;
+    for (int32_t c : f_.probeOutputCols_) rowSize += type::TypeWidth(f_.probeTypes_.GetType(c)->GetId());
+    for (int32_t c : f_.buildOutputCols_) rowSize += type::TypeWidth(btypes.GetType(c)->GetId());
+    int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(std::max(rowSize, 1)), rows_ - emitted_));
+    auto *out = new vec::VectorBatch(count);
+    for (int32_t c : f_.probeOutputCols_) {
+        auto *v = vec::VectorHelper::CreateFlatVector(f_.probeTypes_.GetType(c)->GetId(), count);
+        std::memset(v->RawValues(), 0, static_cast<size_t>(v->Width()) * count);
+        for (int32_t r = 0; r < count; ++r) v->SetNull(r);
+        out->Append(v);
+    }
     for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
-        int32_t tid = btypes.GetType(f_.buildOutputCols_[i])->GetId();
-        auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
+        auto *v = vec::VectorHelper::CreateFlatVector(btypes.GetType(f_.buildOutputCols_[i])->GetId(), count);
         int w = v->Width();
-        std::memcpy(v->RawValues(), matchPay_[i].data() + static_cast<size_t>(w) * emitted_, static_cast<size_t>(w) * count);
+        std::memcpy(v->RawValues(), pay_[i].data() + static_cast<size_t>(w) * emitted_, static_cast<size_t>(w) * count);
         for (int32_t r = 0; r < count; ++r) {
             int64_t m = emitted_ + r;
-            if ((matchPayNulls_[i][m >> 3] >> (m & 7)) & 1) v->SetNull(r);
+            if ((payNulls_[i][m >> 3] >> (m & 7)) & 1) v->SetNull(r);
         }
         out->Append(v);
     }
+    if (f_.builder_->GetBuildSide() == OMNI_BUILD_LEFT) out->RotateLeft(static_cast<int32_t>(f_.probeOutputCols_.size()));
     emitted_ += count;
+    if (emitted_ >= rows_) SetStatus(OMNI_STATUS_FINISHED);
     *result = out;
     return 0;
 }

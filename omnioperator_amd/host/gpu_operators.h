@@ -1,10 +1,11 @@
-// GPU-backed drop-ins for the reference's hash-aggregation and inner hash-join operators.
+// GPU-backed drop-ins for the reference's hash-aggregation and hash-join operators.
 //
 // The factories keep the reference constructors' argument meaning, so a caller (JNI, the native driver,
 // tests) switches by constructing the Gpu* factory instead of the CPU one:
 //   HashAggregationOperatorFactory  core/src/operator/aggregation/group_aggregation.h:233-245
 //   HashBuilderOperatorFactory      core/src/operator/join/hash_builder.h:28-31
 //   LookupJoinOperatorFactory       core/src/operator/join/lookup_join.h:148-152
+//   LookupOuterJoinOperatorFactory  core/src/operator/join/lookup_outer_join.h
 // Operators follow the reference's ownership rules: aggregation consumes and frees each input batch in
 // AddInput; the builder takes its batches (their data moves to device memory); lookup keeps the current
 // probe batch until all its matches are emitted; the caller owns every output batch.
@@ -89,7 +90,13 @@ private:
 
 class GpuHashBuilderOperatorFactory : public OperatorFactory {
 public:
+    // hash_builder.cpp:122-134: without a build side it is OMNI_BUILD_UNKNOWN (hash_builder.h:48)
     static GpuHashBuilderOperatorFactory *CreateHashBuilderOperatorFactory(JoinType joinType,
+                                                                           const type::DataTypes &buildTypes,
+                                                                           const int32_t *buildHashCols,
+                                                                           int32_t buildHashColsCount,
+                                                                           int32_t operatorCount);
+    static GpuHashBuilderOperatorFactory *CreateHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide,
                                                                            const type::DataTypes &buildTypes,
                                                                            const int32_t *buildHashCols,
                                                                            int32_t buildHashColsCount,
@@ -99,10 +106,15 @@ public:
     void *Table() const { return handle_; }
     const type::DataTypes &BuildTypes() const { return buildTypes_; }
     bool Finished() const { return finished_; }
+    JoinType GetJoinType() const { return joinType_; }
+    BuildSide GetBuildSide() const { return buildSide_; }
 
 private:
     friend class GpuHashBuilderOperator;
-    GpuHashBuilderOperatorFactory(const type::DataTypes &buildTypes, int32_t keyCol);
+    GpuHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide, const type::DataTypes &buildTypes,
+                                  int32_t keyCol);
+    JoinType joinType_;
+    BuildSide buildSide_;
     type::DataTypes buildTypes_;
     int32_t keyCol_;
     void *handle_ = nullptr;
@@ -153,10 +165,42 @@ private:
     std::vector<int64_t> matchRows_;
     std::vector<std::vector<uint8_t>> matchPay_;       // per build output column: values
     std::vector<std::vector<uint8_t>> matchPayNulls_;  // per build output column: bitmap
+    std::vector<uint8_t> exists_;                      // existence join: one flag per probe row
     int64_t matches_ = 0;
     int64_t emitted_ = 0;
     DeviceBuffer keyBuf_, nullBuf_, rowsBuf_;
     std::vector<std::unique_ptr<DeviceBuffer>> payBufs_, payNullBufs_;
+};
+
+// LookupOuterJoinOperatorFactory (core/src/operator/join/lookup_outer_join.h): after every probe of a
+// build-preserving outer join, emits the build rows no probe matched, probe output columns NULL.
+class GpuLookupOuterJoinOperatorFactory : public OperatorFactory {
+public:
+    static GpuLookupOuterJoinOperatorFactory *CreateLookupOuterJoinOperatorFactory(
+        const type::DataTypes &probeTypes, int32_t *probeOutputCols, int32_t probeOutputColsCount,
+        int32_t *buildOutputCols, const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr);
+    Operator *CreateOperator() override;
+
+private:
+    friend class GpuLookupOuterJoinOperator;
+    type::DataTypes probeTypes_;
+    std::vector<int32_t> probeOutputCols_;
+    std::vector<int32_t> buildOutputCols_;
+    GpuHashBuilderOperatorFactory *builder_ = nullptr;
+};
+
+class GpuLookupOuterJoinOperator : public Operator {
+public:
+    explicit GpuLookupOuterJoinOperator(const GpuLookupOuterJoinOperatorFactory &f) : f_(f) {}
+    int32_t AddInput(vec::VectorBatch *vecBatch) override;
+    int32_t GetOutput(vec::VectorBatch **result) override;
+
+private:
+    const GpuLookupOuterJoinOperatorFactory &f_;
+    bool fetched_ = false;
+    int64_t rows_ = 0;
+    int64_t emitted_ = 0;
+    std::vector<std::vector<uint8_t>> pay_, payNulls_;  // per build output column
 };
 
 }  // namespace op

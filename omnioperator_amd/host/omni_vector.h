@@ -7,6 +7,7 @@
 // LSB-first null bitmap (bit = 1 means NULL), batches of such vectors, and the push/pull operator API.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -164,6 +165,8 @@ public:
     BaseVector *Get(int32_t i) const { return vectors_.at(i).get(); }
     int32_t GetVectorCount() const { return static_cast<int32_t>(vectors_.size()); }
     int32_t GetRowCount() const { return rowCount_; }
+    // Moves the first k columns to the end (the reference rotates GetVectors(), lookup_join.cpp:1959-1962).
+    void RotateLeft(int32_t k) { std::rotate(vectors_.begin(), vectors_.begin() + k, vectors_.end()); }
 
 private:
     int32_t rowCount_;
@@ -182,7 +185,17 @@ struct VectorHelper {
 namespace op {
 enum OmniStatus { OMNI_STATUS_NORMAL = 0, OMNI_STATUS_FINISHED = 1, OMNI_STATUS_ERROR = -1 };
 
-enum JoinType { OMNI_JOIN_TYPE_INNER = 0 };
+// core/src/plannode/planNode.h:406-414 and :28-32
+enum JoinType {
+    OMNI_JOIN_TYPE_INNER = 0,
+    OMNI_JOIN_TYPE_LEFT,
+    OMNI_JOIN_TYPE_RIGHT,
+    OMNI_JOIN_TYPE_FULL,
+    OMNI_JOIN_TYPE_LEFT_SEMI,
+    OMNI_JOIN_TYPE_LEFT_ANTI,
+    OMNI_JOIN_TYPE_EXISTENCE
+};
+enum BuildSide { OMNI_BUILD_UNKNOWN = 0, OMNI_BUILD_LEFT, OMNI_BUILD_RIGHT };
 
 class OmniException : public std::runtime_error {
 public:

@@ -36,6 +36,7 @@ static std::string Cell(const BaseVector *v, int32_t r)
         case OMNI_INT:
         case OMNI_DATE32: std::snprintf(buf, sizeof buf, "%d", static_cast<const Vector<int32_t> *>(v)->GetValue(r)); break;
         case OMNI_DOUBLE: std::snprintf(buf, sizeof buf, "%.12g", static_cast<const Vector<double> *>(v)->GetValue(r)); break;
+        case OMNI_BOOLEAN: std::snprintf(buf, sizeof buf, "%s", v->RawValues()[r] ? "true" : "false"); break;
         default: std::snprintf(buf, sizeof buf, "%lld", static_cast<long long>(static_cast<const Vector<int64_t> *>(v)->GetValue(r)));
     }
     return buf;
@@ -287,11 +288,115 @@ static void TestAggDecimalOverflowThrows()
     Operator::DeleteOperator(op);
 }
 
+// Two-column LONG build and probe batches (NULL where `nulls` lists the row), key column `keyCol` on both sides,
+// every column output; returns the lookup operator's rows and, when `outer`, the outer operator's rows.
+static std::pair<Rows, Rows> RunTypedJoin(JoinType jt, BuildSide bs, int32_t keyCol,
+                                          const std::vector<std::vector<int64_t>> &bcols, std::vector<int> bnulls,
+                                          const std::vector<std::vector<int64_t>> &pcols, bool outer)
+{
+    DataTypes types({LongType(), LongType()});
+    auto *build = new VectorBatch(static_cast<int32_t>(bcols[0].size()));
+    for (int c = 0; c < 2; ++c) build->Append(MakeVector<int64_t>(OMNI_LONG, bcols[c]));
+    for (int r : bnulls) build->Get(keyCol)->SetNull(r);
+    int32_t hashCols[1] = {keyCol};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(jt, bs, types, hashCols, 1, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    auto *probe = new VectorBatch(static_cast<int32_t>(pcols[0].size()));
+    for (int c = 0; c < 2; ++c) probe->Append(MakeVector<int64_t>(OMNI_LONG, pcols[c]));
+    const bool probeOnly = jt == OMNI_JOIN_TYPE_LEFT_SEMI || jt == OMNI_JOIN_TYPE_LEFT_ANTI ||
+                           jt == OMNI_JOIN_TYPE_EXISTENCE;
+    int32_t probeOut[2] = {0, 1}, probeHash[1] = {keyCol}, buildOut[2] = {0, 1};
+    DataTypes buildOutTypes = probeOnly ? DataTypes() : DataTypes({LongType(), LongType()});
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        types, probeOut, jt == OMNI_JOIN_TYPE_EXISTENCE ? 0 : 2, probeHash, 1, buildOut, probeOnly ? 0 : 2,
+        buildOutTypes, reinterpret_cast<int64_t>(bf), nullptr, false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    std::pair<Rows, Rows> res;
+    res.first = Drain(lookup);
+    GpuLookupOuterJoinOperatorFactory *of = nullptr;
+    Operator *outerOp = nullptr;
+    if (outer) {
+        of = GpuLookupOuterJoinOperatorFactory::CreateLookupOuterJoinOperatorFactory(
+            types, probeOut, 2, buildOut, buildOutTypes, reinterpret_cast<int64_t>(bf));
+        outerOp = of->CreateOperator();
+        res.second = Drain(outerOp);
+        EXPECT_TRUE(outerOp->GetStatus() == OMNI_STATUS_FINISHED);
+        Operator::DeleteOperator(outerOp);
+        delete of;
+    }
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete lf;
+    delete bf;
+    return res;
+}
+
+static const std::vector<std::vector<int64_t>> kBuild4 = {{1, 2, 3, 4}, {111, 11, 333, 33}};
+static const std::vector<std::vector<int64_t>> kProbe4 = {{1, 2, 3, 4}, {11, 22, 33, 44}};
+
+// join_test.cpp:773-833 (TestLeftEqualityJoin)
+static void TestLeftJoin()
+{
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT, OMNI_BUILD_UNKNOWN, 1, kBuild4, {}, kProbe4, false);
+    Rows want = {{"1", "11", "2", "11"}, {"2", "22", "null", "null"}, {"3", "33", "4", "33"}, {"4", "44", "null", "null"}};
+    EXPECT_TRUE(SameRowSet(r.first, want));
+}
+
+// join_test.cpp:835-918 (TestLeftEqualityJoinAndBuildLeft): build columns come first, the outer operator emits
+// the unmatched build rows
+static void TestLeftJoinBuildLeft()
+{
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT, OMNI_BUILD_LEFT, 1, kBuild4, {}, kProbe4, true);
+    Rows want = {{"2", "11", "1", "11"}, {"4", "33", "3", "33"}};  // (build col0, build col1, probe col0, probe col1)
+    EXPECT_TRUE(SameRowSet(r.first, want));
+    Rows wantOuter = {{"3", "333", "null", "null"}, {"1", "111", "null", "null"}};
+    EXPECT_TRUE(SameRowSet(r.second, wantOuter));
+}
+
+// join_test.cpp:2881-2974 (TestOneKeyFullEqualityJoinWithBuildNull)
+static void TestFullJoinBuildNull()
+{
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_FULL, OMNI_BUILD_UNKNOWN, 1, {{1, 2, 3, 4, 5, 6}, {111, 11, 333, 33, 11, 11}},
+                          {0, 2}, kProbe4, true);
+    Rows want = {{"1", "11", "2", "11"}, {"1", "11", "5", "11"}, {"1", "11", "6", "11"},
+                 {"2", "22", "null", "null"}, {"3", "33", "4", "33"}, {"4", "44", "null", "null"}};
+    EXPECT_TRUE(SameRowSet(r.first, want));
+    Rows wantOuter = {{"null", "null", "1", "null"}, {"null", "null", "3", "null"}};
+    EXPECT_TRUE(SameRowSet(r.second, wantOuter));
+}
+
+// join_test.cpp:4157-4208 (TestLeftAntiEqualityJoin)
+static void TestLeftAntiJoin()
+{
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT_ANTI, OMNI_BUILD_UNKNOWN, 1, kBuild4, {},
+                          {{1, 2, 3, 4, 3, 6}, {11, 22, 33, 44, 22, 44}}, false);
+    Rows want = {{"2", "22"}, {"4", "44"}, {"3", "22"}, {"6", "44"}};
+    EXPECT_TRUE(SameRowSet(r.first, want));
+}
+
+// join_test.cpp:4584-4636 (TestExistenceEqualityJoin): no probe output columns, one BOOLEAN per probe row
+static void TestExistenceJoin()
+{
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_EXISTENCE, OMNI_BUILD_UNKNOWN, 1, kBuild4, {},
+                          {{1, 2, 3, 4, 3, 6}, {11, 22, 33, 44, 22, 44}}, false);
+    Rows want = {{"true"}, {"false"}, {"true"}, {"false"}, {"false"}, {"false"}};
+    EXPECT_TRUE(r.first == want);
+}
+
 int main()
 {
     std::vector<std::pair<const char *, std::function<void()>>> tests = {
         {"InnerJoinOneBuildOp", TestInnerJoinOneBuildOp},
         {"InnerJoinArrayTable", TestInnerJoinArrayTable},
+        {"LeftJoin", TestLeftJoin},
+        {"LeftJoinBuildLeft", TestLeftJoinBuildLeft},
+        {"FullJoinBuildNull", TestFullJoinBuildNull},
+        {"LeftAntiJoin", TestLeftAntiJoin},
+        {"ExistenceJoin", TestExistenceJoin},
         {"AggVerifyCorrectness", TestAggVerifyCorrectness},
         {"AggNullKeys", TestAggNullKeys},
         {"AggOutputBatching", TestAggOutputBatching},
