@@ -41,6 +41,13 @@ def cpu_model():
     return platform.processor()
 
 
+# --join-type: (JoinType, BuildSide) values of the reference (core/src/plannode/planNode.h:406-414, 28-32)
+JOIN_TYPES = {"inner": (0, 0), "left": (1, 0), "right_build_right": (2, 2), "full": (3, 0), "semi": (4, 0),
+              "anti": (5, 0)}
+JOIN_NAMES = {"inner": "inner", "left": "left outer", "right_build_right": "right outer (build right)",
+              "full": "full outer", "semi": "left semi", "anti": "left anti"}
+
+
 def bench_join(args):
     from omnioperator_amd import capi, workloads
 
@@ -54,11 +61,22 @@ def bench_join(args):
     bspans = workloads.batches(nb, w.batch_rows)
     pspans = workloads.batches(npr, w.batch_rows)
     pcols, prows = capi.HashJoin.probe_arrays([probe.c(a) for a, _ in pspans], [b - a for a, b in pspans])
+    jt, bs = JOIN_TYPES[args.join_type]
     cap = npr
     out_rows = capi.DeviceArray(cap * 8, np.int64, (cap,))
     out_pay = capi.DeviceArray(cap * 4, np.int32, (cap,))
+    # outer joins write NULL payloads for unmatched probe rows: their payload output carries a bitmap
+    with_nulls = args.join_type in ("left", "full")
+    out_nulls = capi.DeviceArray(4 * ((cap + 31) // 32), np.uint8) if with_nulls else None
     oarr = (capi.OutColumn * 1)()
-    oarr[0].type_id, oarr[0].values, oarr[0].nulls = capi.INT, out_pay.ptr, None
+    oarr[0].type_id, oarr[0].values = capi.INT, out_pay.ptr
+    oarr[0].nulls = out_nulls.ptr if with_nulls else None
+    keeps_build = args.join_type in ("right_build_right", "full")
+    outer_rows = capi.DeviceArray(nb * 8, np.int64, (nb,)) if keeps_build else None
+    outer_pay = capi.DeviceArray(nb * 4, np.int32, (nb,)) if keeps_build else None
+    oouter = (capi.OutColumn * 1)()
+    if keeps_build:
+        oouter[0].type_id, oouter[0].values, oouter[0].nulls = capi.INT, outer_pay.ptr, None
 
     # per-batch column descriptors prepared once (the C++ operator path has no such host cost)
     bcols = [(bkey.c(a), (capi.Column * 1)(bpay.c(a)), b - a) for a, b in bspans]
@@ -66,16 +84,20 @@ def bench_join(args):
     def step():
         j = capi.HashJoin(capi.LONG, [capi.INT])
         j.tune(args.probe_group, args.partitions)
+        if jt != capi.JOIN_INNER:
+            j.set_type(jt, bs)
         for kc, pc, n in bcols:
             j.add_build(kc, pc, n)
         j.finish()
         m = j.probe_batches_raw(pcols, prows, out_rows, oarr, cap)
+        if keeps_build:  # the outer pass over unmatched build rows belongs to the job
+            outer[0] = j.outer_output_raw(outer_rows, oouter, nb)
         parts[0] = j.last_partitions()
         is_array[0] = j.info()["is_array"]
         j.close()
         return m
 
-    parts, is_array = [0], [False]
+    parts, is_array, outer = [0], [False], [0]
 
     for _ in range(args.warmup):
         m = step()
@@ -91,17 +113,27 @@ def bench_join(args):
     capi.profile_enable(False)
     launches, probe_ms = capi.profile_get("join_probe")
     _, build_ms = capi.profile_get("join_build")
-    assert m == w.hits, f"match count {m} != expected hits {w.hits}"
+    _, outer_ms = capi.profile_get("join_outer")
+    expected = {"inner": w.hits, "left": npr, "right_build_right": w.hits, "full": npr, "semi": w.hits,
+                "anti": npr - w.hits}[args.join_type]
+    assert m == expected, f"output rows {m} != expected {expected}"
+    if keeps_build:  # unique build keys: the build rows no probe row hit
+        assert outer[0] == nb - len(np.unique(w.probe_keys[np.isin(w.probe_keys, w.build_keys)])), "outer rows"
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     rows = nb + npr
     value = rows / (ms_per_step / 1e3)
     # the probe stage of one step: the probe kernel, or the partition chain (histogram, scan, scatter,
     # persistent probe) when the table is radix-partitioned
     avg_probe_ms = probe_ms / args.steps
-    probe_bytes = npr * 8 + m * (8 + 4)
+    # 8 B per probe key; per output row its int64 probe row id, plus the int32 payload (and its NULL bit) for
+    # the types that output build columns
+    out_row_bytes = 8 if args.join_type in ("semi", "anti") else 12
+    probe_bytes = npr * 8 + m * out_row_bytes + (m + 7) // 8 * with_nulls
     achieved = probe_bytes / (avg_probe_ms / 1e3) / 1e9
-    # the profiled configuration is the default one: sparse keys, automatic partitioning (1024 partitions)
-    profiled = not args.dense and parts[0] == 1024 and (nb, npr) == (2_000_000, 64_000_000)
+    # the profiled configuration is the default one: inner join, sparse keys, automatic partitioning
+    # (1024 partitions)
+    profiled = (not args.dense and parts[0] == 1024 and (nb, npr) == (2_000_000, 64_000_000)
+                and args.join_type == "inner")
     traffic = profiled_traffic("probe stage (chain)") if profiled else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -109,17 +141,22 @@ def bench_join(args):
                           f"join_probe_* (radix-partitioned, {parts[0]} partitions: hist+scan+scatter+probe)",
                 "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
     extra = {"build_kernel_ms": round(build_ms / max(1, args.steps), 4), "matches": m, "probe_group": args.probe_group,
-             "probe_partitions": parts[0]}
+             "probe_partitions": parts[0], "join_type": args.join_type}
+    if keeps_build:
+        extra["outer_rows"] = outer[0]
+        extra["outer_kernel_ms"] = round(outer_ms / max(1, args.steps), 4)
     if parts[0] > 1:
         extra["probe_stage_ms"] = {k: round(capi.profile_get("join_probe_" + k)[1] / args.steps, 4)
                                    for k in ("hist", "scan", "scatter", "window", "part")}
-    cpu = cpu_baseline_join(w, args) if not args.no_cpu else None
-    cfg = {"workload": "inner hash join: build 2M unique int64 keys + int32 payload, probe 64M int64 keys "
+    cpu = cpu_baseline_join(w, args, jt, bs) if not args.no_cpu else None
+    name = JOIN_NAMES[args.join_type]
+    cfg = {"workload": f"{name} hash join: build 2M unique int64 keys + int32 payload, probe 64M int64 keys "
                        "(80% hit), VectorBatches of 65536" if (nb, npr) == (2_000_000, 64_000_000) else
-                       f"inner hash join: build {nb} + probe {npr} rows",
+                       f"{name} hash join: build {nb} + probe {npr} rows",
            "build_rows": nb, "probe_rows": npr, "batch_rows": w.batch_rows,
            "keys": "dense permutation" if args.dense else "sparse 64-bit (splitmix64)",
-           "output": "probe rowid int64 + build payload int32"}
+           "output": "probe rowid int64" if args.join_type in ("semi", "anti") else
+                     "probe rowid int64 + build payload int32" + (" (NULL when unmatched)" if with_nulls else "")}
     cfg.update(extra)
     return {"value": value, "ms_per_step": ms_per_step, "config": cfg, "roofline": roofline,
             "cpu_baseline": cpu, "dtype": "int64"}
@@ -359,7 +396,7 @@ def profiled_traffic(row_name: str):
     return None
 
 
-def cpu_baseline_join(w, args):
+def cpu_baseline_join(w, args, jt=0, bs=0):
     """The oracle (C++ restatement of HashBuilderOperator + LookupJoinOperator, one core) on the full
     build side and a bounded probe sample; rows/s = (build + sampled probe rows) / wall time."""
     import oracle
@@ -368,11 +405,14 @@ def cpu_baseline_join(w, args):
     bp = oracle.Col(oracle.INT, w.build_payload)
     pk = oracle.Col(oracle.LONG, w.probe_keys[:sample])
     t0 = time.perf_counter()
-    r = oracle.hash_join(bk, [bp], pk, w.batch_rows, w.batch_rows)
+    if jt == 0:
+        r = oracle.hash_join(bk, [bp], pk, w.batch_rows, w.batch_rows)
+    else:
+        r = oracle.hash_join_typed(bk, [bp], pk, w.batch_rows, w.batch_rows, jt, bs)
     t1 = time.perf_counter()
     return {"value": (len(w.build_keys) + sample) / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
             "sample": f"full build ({len(w.build_keys)} rows) + first {sample} probe rows, "
-                      f"{len(r.probe_rows)} matches; host CPU {cpu_model()}"}
+                      f"{len(r.probe_rows)} output rows; host CPU {cpu_model()}"}
 
 
 def main():
@@ -394,6 +434,8 @@ def main():
     p.add_argument("--agg-form", default="batches", choices=["batches", "per_batch"],
                    help="plain group-by: all batches in one add_batches call, or one add_batch per batch")
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
+    p.add_argument("--join-type", default="inner", choices=sorted(JOIN_TYPES),
+                   help="join workload: join type (right_build_right keeps the build side; left keeps the probe side)")
     p.add_argument("--build-rows", type=int, default=2_000_000)
     p.add_argument("--probe-rows", type=int, default=64_000_000)
     p.add_argument("--cpu-probe-rows", type=int, default=8_000_000)

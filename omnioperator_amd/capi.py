@@ -466,6 +466,12 @@ class HashJoin:
         check(lib().omni_gpu_join_probe_exists(self.h, C.byref(key), rows, out.ptr))
         return out.numpy(rows)
 
+    def outer_output_raw(self, out_rows: "DeviceArray", oarr, cap: int) -> int:
+        """Device-resident outer pass (omni_gpu_join_outer_output); returns the row count."""
+        m = C.c_int64()
+        check(lib().omni_gpu_join_outer_output(self.h, out_rows.ptr, oarr, cap, C.byref(m)))
+        return m.value
+
     def outer_output(self, cap: Optional[int] = None, with_nulls: bool = True):
         """Build rows no probe matched: (build row ids, [(payload values, null mask)])."""
         if cap is None:

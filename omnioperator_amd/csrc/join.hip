@@ -143,7 +143,9 @@ int Partition(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64
     return OMNI_GPU_OK;
 }
 
-int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64_t nwg, int64_t rows)
+// Radix-partitioned probe; `ty` (join types other than inner) needs regions of at most kWinSlots slots.
+int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int P, int64_t nwg, int64_t rows,
+                      const TypedArgs *ty = nullptr)
 {
     hipStream_t s = h.stream;
     PartArgs pa;
@@ -159,21 +161,22 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
         const size_t lds = kWinLdsBytes;
         const unsigned int wgrid = static_cast<unsigned int>(P) * static_cast<unsigned int>(splits);
         KernelScope ks(s, "join_probe_window");
-        if (h.has_dup) {
-            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_probe<false, true>),
+        // template arguments: inline payload, duplicate keys, join-type emission
+        auto launch = [&](auto kernel) -> int {
+            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-            join_window_probe<false, true><<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits);
-        } else if (h.inline_pay) {
-            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_probe<true, false>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-            join_window_probe<true, false><<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits);
-        } else {
-            OG_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&join_window_probe<false, false>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-            join_window_probe<false, false><<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits);
-        }
+            kernel<<<wgrid, kWinBlock, lds, s>>>(a, t, pa, splits, ty ? *ty : TypedArgs{});
+            return OMNI_GPU_OK;
+        };
+        if (ty && h.has_dup) OG_TRY(launch(&join_window_probe<false, true, true>));
+        else if (ty && h.inline_pay) OG_TRY(launch(&join_window_probe<true, false, true>));
+        else if (ty) OG_TRY(launch(&join_window_probe<false, false, true>));
+        else if (h.has_dup) OG_TRY(launch(&join_window_probe<false, true, false>));
+        else if (h.inline_pay) OG_TRY(launch(&join_window_probe<true, false, false>));
+        else OG_TRY(launch(&join_window_probe<false, false, false>));
         return CheckLaunch("join_window_probe");
     }
+    OG_CHECK(ty == nullptr, OMNI_GPU_ERR_INVALID, "join types other than inner need LDS window regions");
     const unsigned int pgrid = static_cast<unsigned int>(std::max(kXcds, (cus * 4 / kXcds) * kXcds));
     KernelScope ks(s, "join_probe_part");
     if (h.has_dup) join_part_probe<false, true><<<pgrid, kBlock, 0, s>>>(a, t, pa);
@@ -312,18 +315,35 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
     int64_t total_rows = 0;
     for (int i = 0; i < nb; ++i) total_rows += batches[i].rows;
     const bool typed = h.mode != kModeInner || h.preserve_build;
-    const int P = (!typed && total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows) : 1;
+    int P = (total_blocks > 0 && h.n > 0) ? ChoosePartitions(h, total_rows) : 1;
+    if (typed && P > 1) {  // the join types take the window probe only (regions of at most kWinSlots slots)
+        int lgP = 0;
+        while ((1 << lgP) < P) ++lgP;
+        if ((h.slots_n >> lgP) > static_cast<uint64_t>(kWinSlots)) P = 1;
+    }
     h.last_partitions = P;
-    if (typed) {
+    if (typed && P > 1) {
+        TypedArgs ty{h.mode, 0, h.preserve_build ? h.visited.as<uint8_t>() : nullptr};
+        OG_TRY(LaunchPartitioned(h, a, t, P, part_blocks, total_rows, &ty));
+        bool nullable = false;
+        for (int i = 0; i < nb; ++i) nullable = nullable || batches[i].nulls != nullptr;
+        if (nullable && (h.mode == kModeLeft || h.mode == kModeAnti)) {
+            KernelScope ks(s, "join_probe_null_keys");
+            join_emit_null_keys<<<static_cast<unsigned int>(total_blocks), kBlock, 0, s>>>(a);
+            OG_TRY(CheckLaunch("join_emit_null_keys"));
+        }
+    } else if (typed) {
         if (total_blocks > 0) {  // also with no build rows: outer and anti joins still emit every probe row
             TypedArgs ty{h.mode, h.n == 0 ? 1 : 0, h.preserve_build ? h.visited.as<uint8_t>() : nullptr};
             dim3 grid(static_cast<unsigned int>(total_blocks));
             KernelScope ks(s, "join_probe_typed");
-            if (h.n == 0) join_probe_typed<false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
-            else if (h.is_array && h.has_dup) join_probe_typed<true, true><<<grid, kBlock, 0, s>>>(a, t, ty);
-            else if (h.is_array) join_probe_typed<true, false><<<grid, kBlock, 0, s>>>(a, t, ty);
-            else if (h.has_dup) join_probe_typed<false, true><<<grid, kBlock, 0, s>>>(a, t, ty);
-            else join_probe_typed<false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            // template arguments: array table, duplicate keys, inline payload
+            if (h.n == 0) join_probe_typed<false, false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.is_array && h.has_dup) join_probe_typed<true, true, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.is_array) join_probe_typed<true, false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.has_dup) join_probe_typed<false, true, false><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else if (h.inline_pay) join_probe_typed<false, false, true><<<grid, kBlock, 0, s>>>(a, t, ty);
+            else join_probe_typed<false, false, false><<<grid, kBlock, 0, s>>>(a, t, ty);
             OG_TRY(CheckLaunch("join_probe_typed"));
         }
     } else if (P > 1) {

@@ -554,36 +554,66 @@ __device__ __forceinline__ void WriteNullMatch(const ProbeArgs &a, long long o, 
     }
 }
 
-// Direct probe of one tile (kProbeTile rows per workgroup, the launch shape of join_probe) under the rules
-// above: per-lane output counts, a wave-level exclusive scan and one atomic per workgroup reserve each lane's
-// output rows, which the lane then writes in row order.
-template <bool ARRAY, bool DUPS>
-__global__ void __launch_bounds__(kBlock) join_probe_typed(ProbeArgs a, TableDev t, TypedArgs ty)
+// Marks build row h matched; the load skips the store once the row is marked (every writer stores 1).
+__device__ __forceinline__ void MarkVisited(uint8_t *visited, int h)
 {
-    __shared__ unsigned long long sh[kBlock / 64 + 1];
-    const ProbeBatch b = BatchOf(a, blockIdx.x);
-    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
-    int head[kProbeRows];
-    bool inr[kProbeRows];
+    if (visited && visited[h] == 0) visited[h] = 1;
+}
+
+// Writes the output rows of one tile under the rules above: lane row j has first build row head[j] (-1: none),
+// inline payload pay[j] (INLINE tables), counts as a probe row when inr[j] (a NULL key is a probe row without a
+// match), and has output probe row id rowid(j).  Reserves the workgroup's output range with one atomic.  Unique
+// keys give every row at most one output row, so row j's output rows are compacted across the wave with a
+// ballot (coalesced stores); with duplicate keys each lane writes its rows in order after a wave-level scan of
+// per-lane counts.  All threads of the workgroup must call it.
+template <bool DUPS, bool INLINE, int BLOCK, int ROWS, typename RowId>
+__device__ __forceinline__ void EmitTileTyped(const ProbeArgs &a, const TableDev &t, const TypedArgs &ty,
+                                              const int *head, const unsigned int *pay, const bool *inr, RowId rowid,
+                                              unsigned long long *sh)
+{
+    const bool probeOnly = ty.mode == kModeSemi || ty.mode == kModeAnti;
+    if (!DUPS) {
+        unsigned long long m[ROWS];
+        unsigned int total = 0;
 #pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
-        const int64_t row = tile0 + j * kBlock + threadIdx.x;
-        long long key[1];
-        bool live[1];
-        unsigned int pay[1];
-        LoadProbeKeys<1>(b, row, key, live);
-        inr[j] = row < b.rows;
-        head[j] = -1;
-        if (!ty.empty) LookupGroup<ARRAY, 1>(t, key, live, head + j, pay);
+        for (int j = 0; j < ROWS; ++j) {
+            const bool out = ty.mode == kModeLeft ? inr[j]
+                           : ty.mode == kModeAnti ? (inr[j] && head[j] < 0)
+                                                  : head[j] >= 0;  // inner, semi
+            m[j] = __ballot(out);
+            total += static_cast<unsigned int>(__popcll(m[j]));
+        }
+        unsigned long long base = BlockReserve<BLOCK>(a.counter, total, sh);
+        const unsigned long long lt = LaneMaskLt();
+        const int lane = LaneId();
+#pragma unroll
+        for (int j = 0; j < ROWS; ++j) {
+            if ((m[j] >> lane) & 1ULL) {
+                const long long o = static_cast<long long>(base + __popcll(m[j] & lt));
+                const long long pr = rowid(j);
+                if (probeOnly) {
+                    if (o < a.cap) a.out_rows[o] = pr;
+                } else if (head[j] >= 0) {
+                    if (INLINE) {
+                        WriteMatch<true>(a, o, pr, head[j], pay[j]);
+                    } else if (o < a.cap) {
+                        a.out_rows[o] = pr;
+                        CopyPayloads(a.pay, a.npay, o, head[j]);
+                    }
+                    MarkVisited(ty.visited, head[j]);
+                } else {
+                    WriteNullMatch(a, o, pr);
+                }
+            }
+            base += __popcll(m[j]);
+        }
+        return;
     }
     unsigned int mine = 0;
 #pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
-        unsigned int len = head[j] >= 0 ? 1u : 0u;
-        if (DUPS && len) {
-            len = 0;
-            for (int h = head[j]; h >= 0; h = t.next[h]) ++len;
-        }
+    for (int j = 0; j < ROWS; ++j) {
+        unsigned int len = 0;
+        for (int h = head[j]; h >= 0; h = t.next[h]) ++len;
         if (ty.mode == kModeInner) mine += len;
         else if (ty.mode == kModeLeft) mine += inr[j] ? (len ? len : 1u) : 0u;
         else if (ty.mode == kModeSemi) mine += len ? 1u : 0u;
@@ -597,12 +627,11 @@ __global__ void __launch_bounds__(kBlock) join_probe_typed(ProbeArgs a, TableDev
         if (lane >= d) incl += up;
     }
     const unsigned int waveTotal = __builtin_amdgcn_readlane(incl, 63);
-    long long o = static_cast<long long>(BlockReserve<kBlock>(a.counter, waveTotal, sh) + (incl - mine));
-    const long long rowBase = b.row_base + tile0 + threadIdx.x;
+    long long o = static_cast<long long>(BlockReserve<BLOCK>(a.counter, waveTotal, sh) + (incl - mine));
 #pragma unroll
-    for (int j = 0; j < kProbeRows; ++j) {
-        const long long pr = rowBase + j * kBlock;
-        if (ty.mode == kModeSemi || ty.mode == kModeAnti) {
+    for (int j = 0; j < ROWS; ++j) {
+        const long long pr = rowid(j);
+        if (probeOnly) {
             const bool out = ty.mode == kModeSemi ? head[j] >= 0 : (inr[j] && head[j] < 0);
             if (out) {
                 if (o < a.cap) a.out_rows[o] = pr;
@@ -610,15 +639,73 @@ __global__ void __launch_bounds__(kBlock) join_probe_typed(ProbeArgs a, TableDev
             }
             continue;
         }
-        for (int h = head[j]; h >= 0; h = DUPS ? t.next[h] : -1) {
+        for (int h = head[j]; h >= 0; h = t.next[h]) {
             if (o < a.cap) {
                 a.out_rows[o] = pr;
                 CopyPayloads(a.pay, a.npay, o, h);
             }
             ++o;
-            if (ty.visited) ty.visited[h] = 1;
+            MarkVisited(ty.visited, h);
         }
         if (ty.mode == kModeLeft && inr[j] && head[j] < 0) WriteNullMatch(a, o++, pr);
+    }
+}
+
+// Direct probe of one tile (kProbeTile rows per workgroup, the launch shape of join_probe) under the rules
+// above.
+template <bool ARRAY, bool DUPS, bool INLINE>
+__global__ void __launch_bounds__(kBlock) join_probe_typed(ProbeArgs a, TableDev t, TypedArgs ty)
+{
+    __shared__ unsigned long long sh[kBlock / 64 + 1];
+    const ProbeBatch b = BatchOf(a, blockIdx.x);
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
+    int head[kProbeRows];
+    unsigned int pay[kProbeRows];
+    bool inr[kProbeRows];
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        const int64_t row = tile0 + j * kBlock + threadIdx.x;
+        long long key[1];
+        bool live[1];
+        LoadProbeKeys<1>(b, row, key, live);
+        inr[j] = row < b.rows;
+        head[j] = -1;
+        pay[j] = 0;
+        if (!ty.empty) LookupGroup<ARRAY, 1>(t, key, live, head + j, pay + j);
+    }
+    const long long rowBase = b.row_base + tile0 + threadIdx.x;
+    EmitTileTyped<DUPS, INLINE, kBlock, kProbeRows>(a, t, ty, head, pay, inr,
+                                                    [&](int j) { return rowBase + j * kBlock; }, sh);
+}
+
+// The NULL-key probe rows of the outer (probe-preserving) and anti joins when the probe is radix-partitioned:
+// the partitioning skips NULL keys, so this launch (the direct probe's tiling) writes each of them as a row
+// without a build row.
+__global__ void __launch_bounds__(kBlock) join_emit_null_keys(ProbeArgs a)
+{
+    __shared__ unsigned long long sh[kBlock / 64 + 1];
+    const ProbeBatch b = BatchOf(a, blockIdx.x);
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kProbeTile;
+    unsigned long long m[kProbeRows];
+    unsigned int total = 0;
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        const int64_t row = tile0 + j * kBlock + threadIdx.x;
+        m[j] = __ballot(b.nulls != nullptr && row < b.rows && NullBit(b.nulls, row));
+        total += static_cast<unsigned int>(__popcll(m[j]));
+    }
+    unsigned long long base = BlockReserve<kBlock>(a.counter, total, sh);
+    const unsigned long long lt = LaneMaskLt();
+    const long long rowBase = b.row_base + tile0 + threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < kProbeRows; ++j) {
+        if ((m[j] >> LaneId()) & 1ULL) {
+            const long long o = static_cast<long long>(base + __popcll(m[j] & lt));
+            const long long pr = rowBase + j * kBlock;
+            if (a.npay > 0) WriteNullMatch(a, o, pr);
+            else if (o < a.cap) a.out_rows[o] = pr;
+        }
+        base += __popcll(m[j]);
     }
 }
 
@@ -1033,11 +1120,13 @@ __device__ __forceinline__ int WalkGlobal(const TableDev &t, long long key, unsi
 }
 
 // Window probe: `splits` workgroups serve each partition (workgroup w: partition w / splits, its share
-// w % splits of the partition's entries).  Each stages the partition's region and the kWinExtra slots after it
+// w % splits of the partition's entries).  TYPED: the join-type emission rules (EmitTileTyped) instead of the
+// inner join's.  Each stages the partition's region and the kWinExtra slots after it
 // (wrapping at the table end) in LDS, then resolves its entries against LDS; a probe run that leaves the
 // window continues in the global table.  Dynamic LDS: kWinLdsBytes (shift <= log2 kWinSlots).
-template <bool INLINE, bool DUPS>
-__global__ void __launch_bounds__(kWinBlock) join_window_probe(ProbeArgs a, TableDev t, PartArgs pa, int splits)
+template <bool INLINE, bool DUPS, bool TYPED>
+__global__ void __launch_bounds__(kWinBlock) join_window_probe(ProbeArgs a, TableDev t, PartArgs pa, int splits,
+                                                               TypedArgs ty)
 {
     OG_DYNAMIC_LDS(JoinSlot, win);
     __shared__ unsigned long long sh[kWinBlock / 64 + 1];
@@ -1106,7 +1195,13 @@ __global__ void __launch_bounds__(kWinBlock) join_window_probe(ProbeArgs a, Tabl
             head[j] = hd;
             pay[j] = py;
         }
-        EmitTile<INLINE, DUPS, kWinBlock>(a, t, head, pay, [&](int j) { return static_cast<long long>(rid[j]); }, sh);
+        if (TYPED) {  // join types other than inner: every partitioned entry is a probe row with a non-NULL key
+            EmitTileTyped<DUPS, INLINE, kWinBlock, kProbeRows>(
+                a, t, ty, head, pay, live, [&](int j) { return static_cast<long long>(rid[j]); }, sh);
+        } else {
+            EmitTile<INLINE, DUPS, kWinBlock>(a, t, head, pay, [&](int j) { return static_cast<long long>(rid[j]); },
+                                              sh);
+        }
     }
 }
 

@@ -17,11 +17,14 @@ def keeps_build(jt, bs):
     return oracle.join_mode(jt, bs)[1]
 
 
-def gpu_typed(bk, bpay, pk, build_batch, probe_batch, jt, bs, strategy=0, batched=False, repeat=1):
+def gpu_typed(bk, bpay, pk, build_batch, probe_batch, jt, bs, strategy=0, batched=False, repeat=1, partitions=None,
+              info=None):
     """GPU output: (probe rows, payload lists), outer (build rows, payload lists) or None, existence flags or None.
     repeat > 1 probes the same batches again (the visited marks accumulate, the outer pass is unchanged)."""
     j = capi.HashJoin(bk.type_id, [p.type_id for p in bpay], strategy)
     j.set_type(jt, bs)
+    if partitions is not None:
+        j.tune(1, partitions)
     keep = []
     for a, b in workloads.batches(len(bk), build_batch):
         kd = capi.DeviceColumn(bk.type_id, bk.values[a:b], None if bk.nulls is None else bk.nulls[a:b])
@@ -52,6 +55,8 @@ def gpu_typed(bk, bpay, pk, build_batch, probe_batch, jt, bs, strategy=0, batche
                 rows.append(r)
                 for i, x in enumerate(p):
                     pays[i].append(x)
+    if info is not None:
+        info["last_partitions"] = j.last_partitions()
     outer = None
     if keeps_build(jt, bs):
         orow, op = j.outer_output()
@@ -105,9 +110,9 @@ def make_inputs(rng, dense, kind, nb=3000, npr=20000):
     return Col(LONG, bkv, bnull), pays, Col(LONG, pkv, rng.random(npr) < 0.03)
 
 
-def check(bk, pays, pk, jt, bs, bb, pbs, strategy=0, batched=False, repeat=1):
+def check(bk, pays, pk, jt, bs, bb, pbs, strategy=0, batched=False, repeat=1, partitions=None, info=None):
     want = oracle.hash_join_typed(bk, pays, pk, bb, pbs, jt, bs)
-    got, outer, flags = gpu_typed(bk, pays, pk, bb, pbs, jt, bs, strategy, batched, repeat)
+    got, outer, flags = gpu_typed(bk, pays, pk, bb, pbs, jt, bs, strategy, batched, repeat, partitions, info)
     if jt == oracle.JOIN_EXISTENCE:
         assert flags.tolist() == want.exists.tolist()
         return
@@ -124,6 +129,32 @@ def test_join_types_parity(jt, bs, dense, kind):
     rng = np.random.default_rng(1000 + 10 * jt + bs + 2 * dense + (kind == "dups"))
     bk, pays, pk = make_inputs(rng, dense, kind)
     check(bk, pays, pk, jt, bs, 1024, 6000, strategy=0, batched=(jt + bs) % 2 == 0)
+
+
+@pytest.mark.parametrize("jt,bs", [t for t in ALL_TYPES if t[0] != oracle.JOIN_EXISTENCE])
+@pytest.mark.parametrize("partitions", [16, 64, 8])
+@pytest.mark.parametrize("dups", [False, True])
+def test_join_types_partitioned(jt, bs, partitions, dups):
+    """The radix-partitioned window probe with the join-type rules, and the separate launch for NULL probe keys.
+    The hash table has 131,072 slots: 16 and 64 partitions (regions of 8,192 and 2,048 slots) take the window
+    probe; 8 (regions of 16,384 slots) falls back to the direct probe (the inner join takes the L2 probe)."""
+    rng = np.random.default_rng(50 + 10 * jt + bs + partitions + dups)
+    nb, npr = 30_000, 250_001
+    if dups:
+        bkv = rng.integers(2**40, 2**40 + 20_000, nb, dtype=np.int64)
+    else:
+        bkv = rng.permutation(np.arange(-2**50, -2**50 + 5 * nb, 5, dtype=np.int64))[:nb]
+    bkv[7] = -2**63
+    pkv = np.where(rng.random(npr) < 0.7, bkv[rng.integers(0, nb, npr)],
+                   rng.integers(-2**63, 2**63 - 1, npr, dtype=np.int64))
+    pkv[[0, 99, npr - 1]] = -2**63
+    bk = Col(LONG, bkv, rng.random(nb) < 0.01)
+    pk = Col(LONG, pkv, rng.random(npr) < 0.02)
+    pays = [Col(INT, rng.integers(-2**31, 2**31, nb, dtype=np.int64).astype(np.int32))]
+    info = {}
+    check(bk, pays, pk, jt, bs, 8192, 65536, strategy=1, batched=True, partitions=partitions, info=info)
+    typed = oracle.join_mode(jt, bs) != (0, False)  # the plain inner join keeps the per-XCD L2 probe at 8
+    assert info["last_partitions"] == (1 if typed and partitions == 8 else partitions)
 
 
 @pytest.mark.parametrize("jt,bs", [(oracle.JOIN_FULL, 0), (oracle.JOIN_LEFT, oracle.BUILD_LEFT),
