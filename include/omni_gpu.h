@@ -123,8 +123,9 @@ int omni_gpu_profile_get(const char *name_prefix, int64_t *launches, double *tot
  * Replaces HashAggregationOperator's Emplace / ProcessGroup / Output path
  * (core/src/operator/aggregation/group_aggregation.cpp:320-406, 811-890, 892-955, 1486-1538).
  * The spec mirrors HashAggregationOperatorFactory's constructor arguments
- * (core/src/operator/aggregation/group_aggregation.h:233-245) for the single-stage
- * (inputRaw = true, outputPartial = false) form. */
+ * (core/src/operator/aggregation/group_aggregation.h:233-245).  Raw input; final output, or the partial
+ * output of agg_partial_out.  A final stage over partial input is composed by the operator layer from these
+ * aggregates (omnioperator_amd/host/gpu_operators.cpp). */
 typedef struct omni_gpu_agg_spec {
     int32_t num_keys;
     int32_t key_types[OMNI_GPU_MAX_KEYS];
@@ -137,6 +138,12 @@ typedef struct omni_gpu_agg_spec {
     int32_t overflow_as_null;                  /* OverflowConfig: NULL instead of error on overflow */
     int32_t strategy;                          /* OMNI_GPU_STRATEGY_* */
     int64_t capacity_hint;                     /* expected number of groups; 0 = unknown */
+    /* Two-stage aggregation (outputPartial = true in the factory, group_aggregation.h:233-245): an AVG with
+     * agg_partial_out[i] = 1 outputs its running double sum -- 0, not NULL, for a group whose inputs were all
+     * NULL -- instead of the mean: the first column of AverageFlatIMAggregator's partial output
+     * (average_flatim_aggregator.h:256-273); a COUNT_COLUMN over the same input gives the second.  The other
+     * functions' partial output equals their final output, and the flag is ignored for them. */
+    int32_t agg_partial_out[OMNI_GPU_MAX_AGGS];
 } omni_gpu_agg_spec;
 
 int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle);

@@ -53,6 +53,7 @@ class AggSpec(C.Structure):
         ("agg_in_types", C.c_int32 * MAX_AGGS), ("agg_out_types", C.c_int32 * MAX_AGGS),
         ("agg_in_scales", C.c_int32 * MAX_AGGS), ("agg_out_scales", C.c_int32 * MAX_AGGS),
         ("overflow_as_null", C.c_int32), ("strategy", C.c_int32), ("capacity_hint", C.c_int64),
+        ("agg_partial_out", C.c_int32 * MAX_AGGS),
     ]
 
 
@@ -234,7 +235,7 @@ class DeviceColumn:
 
 def make_agg_spec(key_types: Sequence[int], aggs: Sequence[tuple], overflow_as_null: bool = True,
                   strategy: int = STRATEGY_AUTO, capacity_hint: int = 0) -> AggSpec:
-    """aggs: tuples (func, in_type, out_type[, in_scale, out_scale])."""
+    """aggs: tuples (func, in_type, out_type[, in_scale, out_scale[, partial_out]])."""
     s = AggSpec()
     s.num_keys = len(key_types)
     for i, t in enumerate(key_types):
@@ -244,6 +245,8 @@ def make_agg_spec(key_types: Sequence[int], aggs: Sequence[tuple], overflow_as_n
         s.agg_funcs[i], s.agg_in_types[i], s.agg_out_types[i] = a[0], a[1], a[2]
         if len(a) > 3:
             s.agg_in_scales[i], s.agg_out_scales[i] = a[3], a[4]
+        if len(a) > 5:
+            s.agg_partial_out[i] = 1 if a[5] else 0
     s.overflow_as_null = 1 if overflow_as_null else 0
     s.strategy = strategy
     s.capacity_hint = capacity_hint

@@ -26,6 +26,7 @@ struct AggDev {
     int kind;
     int in_width;     // bytes per input value
     int in_double;    // input column holds doubles
+    int partial_out;  // AVG: output the running sum (0 when empty) instead of the mean
     int out_width;    // bytes per output value
     const void *values;
     const uint8_t *nulls;
@@ -602,6 +603,10 @@ __device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s
         case K_COUNT_COL: static_cast<u64 *>(o.values)[r] = *s.template p<u64>(i, 0); break;
         case K_AVG: {
             u64 cnt = *s.template p<u64>(i, 1);
+            if (d.partial_out) {  // partial state: the sum, 0 for an empty group (never NULL)
+                static_cast<double *>(o.values)[r] = cnt == 0 ? 0.0 : *s.template p<double>(i, 0);
+                break;
+            }
             isNull = cnt == 0;
             static_cast<double *>(o.values)[r] = isNull ? 0.0 : *s.template p<double>(i, 0) / static_cast<double>(cnt);
             break;

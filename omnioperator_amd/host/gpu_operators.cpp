@@ -64,6 +64,16 @@ void DeviceBuffer::Reserve(size_t bytes)
 
 int32_t MaxRowCount(int32_t rowSize) { return (kMaxVecBatchBytes + rowSize - 1) / rowSize; }
 
+namespace {
+// Value r of a flat vector of T.
+template <typename T> T Cell(const vec::BaseVector *v, int32_t r)
+{
+    T x;
+    std::memcpy(&x, v->RawValues() + sizeof(T) * static_cast<size_t>(r), sizeof(T));
+    return x;
+}
+}  // namespace
+
 // ================================================================================================
 // Hash aggregation
 // ================================================================================================
@@ -83,31 +93,79 @@ GpuHashAggregationOperatorFactory::GpuHashAggregationOperatorFactory(
     if (aggFuncTypes.size() > OMNI_GPU_MAX_AGGS) throw OmniException("UNSUPPORTED_ERROR", "too many aggregates");
     spec_.num_keys = static_cast<int32_t>(groupByCol.size());
     for (size_t i = 0; i < groupByCol.size(); ++i) spec_.key_types[i] = groupInputTypes.GetType(i)->GetId();
-    spec_.num_aggs = static_cast<int32_t>(aggFuncTypes.size());
+    spec_.num_aggs = 0;
     spec_.overflow_as_null = overflowAsNull ? 1 : 0;
-    // COUNT_ALL aggregates take no input column: aggsCols / aggInputTypes list only the others
-    // (HashAggregationOperatorFactory::CreateOperator, group_aggregation.cpp:104-121).
+    // COUNT_ALL aggregates over raw input take no input column: aggsCols / aggInputTypes list only the others
+    // (HashAggregationOperatorFactory::CreateOperator, group_aggregation.cpp:102-125).
+    // Two-stage aggregation (inputRaw / outputPartial per aggregate): SUM, MIN and MAX have partial states equal
+    // to their outputs, so a final stage adds / compares the partial values; a final COUNT adds the partial
+    // counts; AVG's partial output is (double sum, long count) as two flat columns (the non-container layout of
+    // AverageFlatIMAggregator::ExtractValuesBatch, average_flatim_aggregator.h:256-273) and its final stage adds
+    // both and divides (:40-45, 275-290).
     size_t inputIdx = 0;
     for (size_t i = 0; i < aggFuncTypes.size(); ++i) {
-        bool raw = i < inputRaws.size() ? inputRaws[i] : true;
-        bool partial = i < outputPartials.size() ? outputPartials[i] : false;
-        if (!raw || partial) throw OmniException("UNSUPPORTED_ERROR", "only single-stage aggregation is on the GPU path");
-        spec_.agg_funcs[i] = static_cast<int32_t>(aggFuncTypes[i]);
-        const type::DataTypePtr &out = aggOutputTypes.at(i).GetType(0);
-        spec_.agg_out_types[i] = out->GetId();
-        spec_.agg_out_scales[i] = out->GetScale();
-        aggOutTypes_.push_back(out);
-        if (aggFuncTypes[i] == OMNI_AGG_COUNT_ALL) {
-            spec_.agg_in_types[i] = type::OMNI_LONG;
-            aggInputCol_.push_back(-1);
+        const bool raw = i < inputRaws.size() ? inputRaws[i] : true;
+        const bool partial = i < outputPartials.size() ? outputPartials[i] : false;
+        const int32_t func = static_cast<int32_t>(aggFuncTypes[i]);
+        const type::DataTypes &outs = aggOutputTypes.at(i);
+        const int32_t out0 = outs.GetType(0)->GetId();
+        if (func == OMNI_AGG_COUNT_ALL && raw) {
+            plan_.push_back({kDirect, AddSlot(func, type::OMNI_LONG, type::OMNI_LONG, 0, 0, -1, false), {out0}});
             continue;
         }
-        const type::DataTypePtr &in = aggInputTypes.at(inputIdx).GetType(0);
-        spec_.agg_in_types[i] = in->GetId();
-        spec_.agg_in_scales[i] = in->GetScale();
-        aggInputCol_.push_back(static_cast<int32_t>(aggsCols.at(inputIdx).at(0)));
+        const type::DataTypes &ins = aggInputTypes.at(inputIdx);
+        const std::vector<uint32_t> &cols = aggsCols.at(inputIdx);
         ++inputIdx;
+        const int32_t in0 = ins.GetType(0)->GetId();
+        const int32_t col0 = static_cast<int32_t>(cols.at(0));
+        if (func == OMNI_AGG_COUNT_ALL || func == OMNI_AGG_COUNT_COLUMN) {
+            const int32_t slot = raw ? AddSlot(OMNI_AGG_COUNT_COLUMN, in0, type::OMNI_LONG, 0, 0, col0, false)
+                                     : AddSlot(OMNI_AGG_SUM, type::OMNI_LONG, type::OMNI_LONG, 0, 0, col0, false);
+            plan_.push_back({kDirect, slot, {out0}});
+        } else if (func == OMNI_AGG_AVG && raw && !partial) {
+            plan_.push_back({kDirect, AddSlot(func, in0, out0, 0, 0, col0, false), {out0}});
+        } else if (func == OMNI_AGG_AVG && raw) {
+            if (outs.GetSize() != 2 || out0 != type::OMNI_DOUBLE || outs.GetType(1)->GetId() != type::OMNI_LONG) {
+                throw OmniException("UNSUPPORTED_ERROR", "partial AVG output must be (DOUBLE sum, LONG count)");
+            }
+            const int32_t slot = AddSlot(OMNI_AGG_AVG, in0, type::OMNI_DOUBLE, 0, 0, col0, true);
+            AddSlot(OMNI_AGG_COUNT_COLUMN, in0, type::OMNI_LONG, 0, 0, col0, false);
+            plan_.push_back({kAvgPartial, slot, {type::OMNI_DOUBLE, type::OMNI_LONG}});
+        } else if (func == OMNI_AGG_AVG) {
+            if (cols.size() != 2 || ins.GetSize() != 2 || in0 != type::OMNI_DOUBLE ||
+                ins.GetType(1)->GetId() != type::OMNI_LONG) {
+                throw OmniException("UNSUPPORTED_ERROR", "AVG over partial input takes (DOUBLE sum, LONG count) columns");
+            }
+            const int32_t slot = AddSlot(OMNI_AGG_SUM, type::OMNI_DOUBLE, type::OMNI_DOUBLE, 0, 0, col0, false);
+            AddSlot(OMNI_AGG_SUM, type::OMNI_LONG, type::OMNI_LONG, 0, 0, static_cast<int32_t>(cols[1]), false);
+            if (partial) plan_.push_back({kAvgMerge, slot, {type::OMNI_DOUBLE, type::OMNI_LONG}});
+            else plan_.push_back({kAvgFinal, slot, {type::OMNI_DOUBLE}});
+        } else {
+            const bool decimal = in0 == type::OMNI_DECIMAL64 || in0 == type::OMNI_DECIMAL128;
+            if (decimal && func == OMNI_AGG_SUM && (!raw || partial)) {
+                throw OmniException("UNSUPPORTED_ERROR", "two-stage decimal SUM is not on the GPU path");
+            }
+            plan_.push_back({kDirect, AddSlot(func, in0, out0, ins.GetType(0)->GetScale(), outs.GetType(0)->GetScale(),
+                                              col0, false),
+                             {out0}});
+        }
     }
+}
+
+int32_t GpuHashAggregationOperatorFactory::AddSlot(int32_t func, int32_t inType, int32_t outType, int32_t inScale,
+                                                   int32_t outScale, int32_t col, bool partialOut)
+{
+    const int32_t n = spec_.num_aggs;
+    if (n >= OMNI_GPU_MAX_AGGS) throw OmniException("UNSUPPORTED_ERROR", "too many aggregates");
+    spec_.agg_funcs[n] = func;
+    spec_.agg_in_types[n] = inType;
+    spec_.agg_out_types[n] = outType;
+    spec_.agg_in_scales[n] = inScale;
+    spec_.agg_out_scales[n] = outScale;
+    spec_.agg_partial_out[n] = partialOut ? 1 : 0;
+    aggInputCol_.push_back(col);
+    spec_.num_aggs = n + 1;
+    return n;
 }
 
 OmniStatus GpuHashAggregationOperatorFactory::Init() { return OMNI_STATUS_NORMAL; }
@@ -115,18 +173,18 @@ OmniStatus GpuHashAggregationOperatorFactory::Init() { return OMNI_STATUS_NORMAL
 Operator *GpuHashAggregationOperatorFactory::CreateOperator() { return new GpuHashAggregationOperator(*this); }
 
 GpuHashAggregationOperator::GpuHashAggregationOperator(const GpuHashAggregationOperatorFactory &f)
-    : groupByCols_(f.groupByCols_), aggInputCol_(f.aggInputCol_), spec_(f.spec_)
+    : groupByCols_(f.groupByCols_), aggInputCol_(f.aggInputCol_), plan_(f.plan_), spec_(f.spec_)
 {
     EnsureInit();
     Check(omni_gpu_agg_create(&spec_, &handle_), "omni_gpu_agg_create");
-    int32_t rowSize = 0;
+    int32_t rowSize = 0;  // of the output row (group_aggregation.cpp:414-429)
     for (int i = 0; i < spec_.num_keys; ++i) {
         outTypes_.push_back(spec_.key_types[i]);
         rowSize += type::TypeWidth(spec_.key_types[i]);
     }
-    for (int i = 0; i < spec_.num_aggs; ++i) {
-        outTypes_.push_back(spec_.agg_out_types[i]);
-        rowSize += type::TypeWidth(spec_.agg_out_types[i]);
+    for (int i = 0; i < spec_.num_aggs; ++i) outTypes_.push_back(spec_.agg_out_types[i]);
+    for (const auto &p : plan_) {
+        for (int32_t t : p.outTypes) rowSize += type::TypeWidth(t);
     }
     rowsPerBatch_ = MaxRowCount(std::max(rowSize, 1));
 }
@@ -199,12 +257,50 @@ int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
         outs[c].nulls = static_cast<uint8_t *>(nulls[c]->Get());
     }
     Check(omni_gpu_agg_get_output(handle_, emitted_, count, outs.data()), "omni_gpu_agg_get_output");
-    auto *batch = new vec::VectorBatch(count);
+    std::vector<std::unique_ptr<vec::BaseVector>> gpu(ncols);
     for (size_t c = 0; c < ncols; ++c) {
-        auto *v = vec::VectorHelper::CreateFlatVector(outTypes_[c], count);
+        gpu[c].reset(vec::VectorHelper::CreateFlatVector(outTypes_[c], count));
+        auto *v = gpu[c].get();
         Check(omni_gpu_memcpy_d2h(v->RawValues(), vals[c]->Get(), static_cast<size_t>(count) * v->Width()), "download");
         Check(omni_gpu_memcpy_d2h(v->RawNulls(), nulls[c]->Get(), (count + 7) / 8), "download nulls");
-        batch->Append(v);
+    }
+    auto *batch = new vec::VectorBatch(count);
+    const size_t nk = static_cast<size_t>(spec_.num_keys);
+    for (size_t c = 0; c < nk; ++c) batch->Append(gpu[c].release());
+    for (const auto &p : plan_) {
+        auto *first = gpu[nk + p.slot].get();
+        auto *second = p.rule == GpuHashAggregationOperatorFactory::kDirect ? nullptr : gpu[nk + p.slot + 1].get();
+        switch (p.rule) {
+            case GpuHashAggregationOperatorFactory::kDirect: batch->Append(gpu[nk + p.slot].release()); break;
+            case GpuHashAggregationOperatorFactory::kAvgPartial:
+            case GpuHashAggregationOperatorFactory::kAvgMerge: {
+                // (sum, count); a group without a non-NULL input is (0, 0), never NULL
+                auto *os = new vec::Vector<double>(count, type::OMNI_DOUBLE);
+                auto *oc = new vec::Vector<int64_t>(count, type::OMNI_LONG);
+                for (int32_t r = 0; r < count; ++r) {
+                    const int64_t c = second->IsNull(r) ? 0 : Cell<int64_t>(second, r);
+                    os->SetValue(r, c == 0 || first->IsNull(r) ? 0.0 : Cell<double>(first, r));
+                    oc->SetValue(r, c);
+                }
+                batch->Append(os);
+                batch->Append(oc);
+                break;
+            }
+            case GpuHashAggregationOperatorFactory::kAvgFinal: {
+                auto *o = new vec::Vector<double>(count, type::OMNI_DOUBLE);
+                for (int32_t r = 0; r < count; ++r) {
+                    const int64_t c = second->IsNull(r) ? 0 : Cell<int64_t>(second, r);
+                    if (c == 0) {
+                        o->SetValue(r, 0.0);
+                        o->SetNull(r);
+                    } else {
+                        o->SetValue(r, Cell<double>(first, r) / static_cast<double>(c));
+                    }
+                }
+                batch->Append(o);
+                break;
+            }
+        }
     }
     emitted_ += count;
     if (emitted_ >= groups_) SetStatus(OMNI_STATUS_FINISHED);

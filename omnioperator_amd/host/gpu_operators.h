@@ -56,12 +56,27 @@ public:
     void SetCapacityHint(int64_t groups) { spec_.capacity_hint = groups; }
     void SetStrategy(int32_t strategy) { spec_.strategy = strategy; }
 
+    // How one aggregate of the factory (function + stage) reads the GPU aggregates of spec_ from `slot` on.
+    enum AggRule {
+        kDirect = 0,     // one GPU aggregate, one output column
+        kAvgPartial = 1, // AVG with partial output: running sum (partial_out) + COUNT_COLUMN -> (sum, count)
+        kAvgFinal = 2,   // AVG over partial (sum, count) input: SUM + SUM -> sum / count, NULL when count is 0
+        kAvgMerge = 3,   // AVG over partial input with partial output: SUM + SUM -> (sum, count)
+    };
+    struct AggPlan {
+        AggRule rule;
+        int32_t slot;
+        std::vector<int32_t> outTypes;
+    };
+
 private:
     friend class GpuHashAggregationOperator;
+    int32_t AddSlot(int32_t func, int32_t inType, int32_t outType, int32_t inScale, int32_t outScale, int32_t col,
+                    bool partialOut);
     std::vector<uint32_t> groupByCols_;
     type::DataTypes groupTypes_;
-    std::vector<int32_t> aggInputCol_;  // -1 for COUNT_ALL
-    std::vector<type::DataTypePtr> aggOutTypes_;
+    std::vector<int32_t> aggInputCol_;  // per GPU aggregate: input column, -1 for COUNT_ALL
+    std::vector<AggPlan> plan_;         // per aggregate of the factory
     omni_gpu_agg_spec spec_{};
 };
 
@@ -77,7 +92,8 @@ public:
 private:
     std::vector<uint32_t> groupByCols_;
     std::vector<int32_t> aggInputCol_;
-    std::vector<int32_t> outTypes_;
+    std::vector<int32_t> outTypes_;  // GPU output columns: keys, then one per GPU aggregate
+    std::vector<GpuHashAggregationOperatorFactory::AggPlan> plan_;
     omni_gpu_agg_spec spec_;
     void *handle_ = nullptr;
     bool hasInput_ = false;

@@ -482,6 +482,7 @@ inline bool MulChecked128(int128 a, int128 b, int128 &r)  // base_operations.h:3
 
 struct AggDesc {
     int32_t func, inType, outType, inScale, outScale;
+    bool partialOut = false;  // AVG: the partial output's sum column
     size_t offset = 0;  // state offset inside the per-group blob
     size_t size = 0;
 };
@@ -646,6 +647,10 @@ bool ExtractState(const AggDesc &a, const uint8_t *s, OutCol &o, int64_t r, bool
         case OMNI_AGG_COUNT_COLUMN: static_cast<int64_t *>(o.values)[r] = Ld<int64_t>(s); return true;
         case OMNI_AGG_AVG: {
             int64_t cnt = Ld<int64_t>(s + 8);
+            if (a.partialOut) {  // partial output: the sum, 0 for an empty group (average_flatim_aggregator.h:260-273)
+                static_cast<double *>(o.values)[r] = cnt == 0 ? 0.0 : Ld<double>(s);
+                return true;
+            }
             if (cnt == 0) {
                 SetNullBit(o, r);
                 static_cast<double *>(o.values)[r] = 0;
@@ -736,6 +741,7 @@ public:
         for (int i = 0; i < spec.num_aggs; ++i) {
             AggDesc a{spec.agg_funcs[i], spec.agg_in_types[i], spec.agg_out_types[i], spec.agg_in_scales[i],
                       spec.agg_out_scales[i]};
+            a.partialOut = spec.agg_funcs[i] == OMNI_AGG_AVG && spec.agg_partial_out[i] != 0;
             a.size = StateSize(a);
             a.offset = off;
             off += a.size;

@@ -197,6 +197,187 @@ static void TestAggVerifyCorrectness()
     delete f;
 }
 
+// Pushes the batches through an operator and returns its output batches (the caller frees them).
+static std::vector<VectorBatch *> RunAgg(Operator *op, std::vector<VectorBatch *> in)
+{
+    for (auto *b : in) op->AddInput(b);
+    std::vector<VectorBatch *> out;
+    for (int guard = 0; guard < 100000; ++guard) {
+        VectorBatch *o = nullptr;
+        op->GetOutput(&o);
+        if (!o) break;
+        out.push_back(o);
+    }
+    return out;
+}
+
+// aggregation_test.cpp:544-630 (verify_correctness) as written there: two partial operators over 10 batches each,
+// then a final operator over their outputs.  Partial AVG is (DOUBLE sum, LONG count) as two flat columns.
+static void TestAggTwoStageVerifyCorrectness()
+{
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM, OMNI_AGG_AVG, OMNI_AGG_COUNT_COLUMN, OMNI_AGG_MIN, OMNI_AGG_MAX};
+    std::vector<uint32_t> groupCols = {0, 1}, masks;
+    std::vector<std::vector<uint32_t>> pcols = {{2}, {3}, {4}, {5}, {6}};
+    std::vector<DataTypes> pin(5, DataTypes({LongType()}));
+    std::vector<DataTypes> pout = {DataTypes({LongType()}), DataTypes({DoubleType(), LongType()}), DataTypes({LongType()}),
+                                   DataTypes({LongType()}), DataTypes({LongType()})};
+    GpuHashAggregationOperatorFactory pf(groupCols, DataTypes({LongType(), LongType()}), pcols, pin, pout, funcs, masks,
+                                         std::vector<bool>(5, true), std::vector<bool>(5, true), false);
+    std::vector<VectorBatch *> partials;
+    for (int opi = 0; opi < 2; ++opi) {
+        std::vector<VectorBatch *> in;
+        for (int b = 0; b < 10; ++b) {
+            auto *batch = new VectorBatch(2000);
+            std::vector<int64_t> k(2000), one(2000, 1);
+            for (int j = 0; j < 2000; ++j) k[j] = j % 10;
+            batch->Append(MakeVector<int64_t>(OMNI_LONG, k));
+            batch->Append(MakeVector<int64_t>(OMNI_LONG, k));
+            for (int a = 0; a < 5; ++a) batch->Append(MakeVector<int64_t>(OMNI_LONG, one));
+            in.push_back(batch);
+        }
+        Operator *op = pf.CreateOperator();
+        auto out = RunAgg(op, in);
+        EXPECT_TRUE(out.size() == 1 && out[0]->GetVectorCount() == 8);
+        Rows rows;
+        for (auto *o : out) AppendRows(rows, o);
+        Rows want;
+        for (int k = 0; k < 10; ++k) {
+            want.push_back({std::to_string(k), std::to_string(k), "2000", "2000", "2000", "2000", "1", "1"});
+        }
+        EXPECT_TRUE(SameRowSet(rows, want));
+        partials.insert(partials.end(), out.begin(), out.end());
+        Operator::DeleteOperator(op);
+    }
+    std::vector<std::vector<uint32_t>> fcols = {{2}, {3, 4}, {5}, {6}, {7}};
+    std::vector<DataTypes> fin = {DataTypes({LongType()}), DataTypes({DoubleType(), LongType()}), DataTypes({LongType()}),
+                                  DataTypes({LongType()}), DataTypes({LongType()})};
+    std::vector<DataTypes> fout = {DataTypes({LongType()}), DataTypes({DoubleType()}), DataTypes({LongType()}),
+                                   DataTypes({LongType()}), DataTypes({LongType()})};
+    GpuHashAggregationOperatorFactory ff(groupCols, DataTypes({LongType(), LongType()}), fcols, fin, fout, funcs, masks,
+                                         std::vector<bool>(5, false), std::vector<bool>(5, false), false);
+    Operator *fop = ff.CreateOperator();
+    auto out = RunAgg(fop, partials);
+    Rows got;
+    for (auto *o : out) {
+        EXPECT_TRUE(o->GetVectorCount() == 7);
+        AppendRows(got, o);
+        VectorHelper::FreeVecBatch(o);
+    }
+    Rows want;
+    for (int k = 0; k < 10; ++k) want.push_back({std::to_string(k), std::to_string(k), "4000", "1", "4000", "1", "1"});
+    EXPECT_TRUE(SameRowSet(got, want));
+    Operator::DeleteOperator(fop);
+}
+
+// Two stages equal one stage: SUM, AVG(long), COUNT(col), COUNT(*), MIN, MAX, AVG(double) over seeded rows with NULL
+// keys, NULL inputs and one group whose AVG inputs are all NULL (partial (0, 0), final NULL).  Three partial
+// operators take a third of the batches each; the final operator merges their outputs.  AVG inputs are integral,
+// so every double sum is exact in any order and the comparison is exact.
+static void TestAggTwoStageMatchesSingleStage()
+{
+    const int kRows = 30000, kBatch = 4096;
+    uint64_t st = 0x2545F4914F6CDD1DULL;
+    auto rnd = [&st]() {
+        st ^= st << 13;
+        st ^= st >> 7;
+        st ^= st << 17;
+        return st;
+    };
+    std::vector<int64_t> key(kRows), v1(kRows), v2(kRows);
+    std::vector<double> v3(kRows);
+    std::vector<bool> nk(kRows), n1(kRows), n2(kRows), n3(kRows);
+    for (int i = 0; i < kRows; ++i) {
+        key[i] = static_cast<int64_t>(rnd() % 500);
+        nk[i] = rnd() % 50 == 0;
+        v1[i] = static_cast<int64_t>(rnd() % 2000001) - 1000000;
+        n1[i] = rnd() % 20 == 0;
+        v2[i] = static_cast<int64_t>(rnd() % 1000);
+        n2[i] = rnd() % 10 == 0;
+        v3[i] = static_cast<double>(static_cast<int64_t>(rnd() % 2001) - 1000);
+        n3[i] = rnd() % 20 == 0;
+        if (i % 97 == 0) {  // group 777: every AVG input NULL
+            key[i] = 777;
+            nk[i] = false;
+            n1[i] = n3[i] = true;
+        }
+    }
+    auto makeBatches = [&](int lo, int hi) {
+        std::vector<VectorBatch *> in;
+        for (int a = lo; a < hi; a += kBatch) {
+            int b = std::min(hi, a + kBatch), n = b - a;
+            auto *batch = new VectorBatch(n);
+            auto *k = MakeVector<int64_t>(OMNI_LONG, std::vector<int64_t>(key.begin() + a, key.begin() + b));
+            auto *c1 = MakeVector<int64_t>(OMNI_LONG, std::vector<int64_t>(v1.begin() + a, v1.begin() + b));
+            auto *c2 = MakeVector<int64_t>(OMNI_LONG, std::vector<int64_t>(v2.begin() + a, v2.begin() + b));
+            auto *c3 = MakeVector<double>(OMNI_DOUBLE, std::vector<double>(v3.begin() + a, v3.begin() + b));
+            for (int r = 0; r < n; ++r) {
+                if (nk[a + r]) k->SetNull(r);
+                if (n1[a + r]) c1->SetNull(r);
+                if (n2[a + r]) c2->SetNull(r);
+                if (n3[a + r]) c3->SetNull(r);
+            }
+            batch->Append(k);
+            batch->Append(c1);
+            batch->Append(c2);
+            batch->Append(c3);
+            in.push_back(batch);
+        }
+        return in;
+    };
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM, OMNI_AGG_AVG, OMNI_AGG_COUNT_COLUMN, OMNI_AGG_COUNT_ALL,
+                                   OMNI_AGG_MIN, OMNI_AGG_MAX, OMNI_AGG_AVG};
+    std::vector<uint32_t> groupCols = {0}, masks;
+    DataTypes keyTypes({LongType()});
+    // raw input columns (COUNT_ALL over raw input takes none)
+    std::vector<std::vector<uint32_t>> rcols = {{1}, {1}, {2}, {1}, {2}, {3}};
+    std::vector<DataTypes> rin = {DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({LongType()}),
+                                  DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({DoubleType()})};
+    std::vector<DataTypes> single = {DataTypes({LongType()}), DataTypes({DoubleType()}), DataTypes({LongType()}),
+                                     DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({LongType()}),
+                                     DataTypes({DoubleType()})};
+    std::vector<DataTypes> part = {DataTypes({LongType()}), DataTypes({DoubleType(), LongType()}),
+                                   DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({LongType()}),
+                                   DataTypes({LongType()}), DataTypes({DoubleType(), LongType()})};
+    GpuHashAggregationOperatorFactory sf(groupCols, keyTypes, rcols, rin, single, funcs, masks,
+                                         std::vector<bool>(7, true), std::vector<bool>(7, false), false);
+    GpuHashAggregationOperatorFactory pf(groupCols, keyTypes, rcols, rin, part, funcs, masks,
+                                         std::vector<bool>(7, true), std::vector<bool>(7, true), false);
+    // partial output: key, sum, avg (sum, count), count, count(*), min, max, avg (sum, count) = columns 0..9
+    std::vector<std::vector<uint32_t>> fcols = {{1}, {2, 3}, {4}, {5}, {6}, {7}, {8, 9}};
+    std::vector<DataTypes> fin = {DataTypes({LongType()}), DataTypes({DoubleType(), LongType()}), DataTypes({LongType()}),
+                                  DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({LongType()}),
+                                  DataTypes({DoubleType(), LongType()})};
+    GpuHashAggregationOperatorFactory ff(groupCols, keyTypes, fcols, fin, single, funcs, masks,
+                                         std::vector<bool>(7, false), std::vector<bool>(7, false), false);
+    Operator *sop = sf.CreateOperator();
+    Rows want;
+    for (auto *o : RunAgg(sop, makeBatches(0, kRows))) {
+        AppendRows(want, o);
+        VectorHelper::FreeVecBatch(o);
+    }
+    Operator::DeleteOperator(sop);
+    std::vector<VectorBatch *> partials;
+    const int cut[4] = {0, 3 * kBatch, 5 * kBatch, kRows};
+    for (int p = 0; p < 3; ++p) {
+        Operator *op = pf.CreateOperator();
+        auto out = RunAgg(op, makeBatches(cut[p], cut[p + 1]));
+        partials.insert(partials.end(), out.begin(), out.end());
+        Operator::DeleteOperator(op);
+    }
+    Operator *fop = ff.CreateOperator();
+    Rows got;
+    for (auto *o : RunAgg(fop, partials)) {
+        AppendRows(got, o);
+        VectorHelper::FreeVecBatch(o);
+    }
+    Operator::DeleteOperator(fop);
+    EXPECT_TRUE(want.size() == 502);  // 500 keys, the NULL key, key 777
+    EXPECT_TRUE(SameRowSet(got, want));
+    bool sawNullAvg = false;
+    for (const auto &r : got) sawNullAvg = sawNullAvg || (r[0] == "777" && r[2] == "null" && r[7] == "null");
+    EXPECT_TRUE(sawNullAvg);
+}
+
 // aggregation_test.cpp:632-740 (NULL keys form their own group), LONG key
 static void TestAggNullKeys()
 {
@@ -398,6 +579,8 @@ int main()
         {"LeftAntiJoin", TestLeftAntiJoin},
         {"ExistenceJoin", TestExistenceJoin},
         {"AggVerifyCorrectness", TestAggVerifyCorrectness},
+        {"AggTwoStageVerifyCorrectness", TestAggTwoStageVerifyCorrectness},
+        {"AggTwoStageMatchesSingleStage", TestAggTwoStageMatchesSingleStage},
         {"AggNullKeys", TestAggNullKeys},
         {"AggOutputBatching", TestAggOutputBatching},
         {"AggDecimalOverflowThrows", TestAggDecimalOverflowThrows},

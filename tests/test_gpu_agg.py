@@ -297,3 +297,24 @@ def test_add_batches_shared_value_column():
     want = run_oracle([Col(LONG, k)], aggs, [Col(LONG, v), None, Col(LONG, v), Col(LONG, v)], 4096)
     assert len(got) == 1000
     assert_row_sets_equal(got, want)
+
+
+@pytest.mark.parametrize("in_type", [LONG, DOUBLE])
+def test_partial_avg_output(in_type):
+    """agg_partial_out: AVG outputs its running double sum (0, not NULL, for a group of only NULL inputs) beside
+    a COUNT_COLUMN of the same input -- the two columns of the reference's partial AVG state."""
+    rng = np.random.default_rng(17)
+    n = 50_000
+    k = rng.integers(0, 700, n, dtype=np.int64)
+    nulls = rng.random(n) < 0.1
+    nulls[k == 5] = True  # group 5: every input NULL
+    if in_type == DOUBLE:
+        v = Col(DOUBLE, rng.normal(100, 50, n), nulls)
+    else:
+        v = Col(LONG, rng.integers(-10**6, 10**6, n, dtype=np.int64), nulls)
+    aggs = [(AVG, in_type, DOUBLE, 0, 0, True), (COUNT_COLUMN, in_type, LONG), (AVG, in_type, DOUBLE)]
+    got = run_gpu([Col(LONG, k)], aggs, [v, v, v], 4096)
+    want = run_oracle([Col(LONG, k)], aggs, [v, v, v], 4096)
+    assert_row_sets_equal(got, want, float_cols=[1, 3], rtol=DOUBLE_RTOL)
+    row5 = [r for r in got if r[0] == 5][0]
+    assert row5[1] == 0.0 and row5[2] == 0 and row5[3] is None

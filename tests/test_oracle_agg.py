@@ -154,3 +154,21 @@ def test_swiss_map_growth_and_null_slot():
     assert m.size() == thr + 3
     vals = m.values_in_order()
     assert vals[-1] == 123 and sorted(vals[:-1].tolist()) == sorted(list(range(thr + 1)) + [10**6])
+
+
+def test_partial_avg_output_definition():
+    """The oracle's partial AVG column is the sum of the group's non-NULL inputs, 0 when there are none
+    (average_flatim_aggregator.h:256-273), while the final AVG is NULL there."""
+    rng = np.random.default_rng(3)
+    n = 4000
+    k = rng.integers(0, 40, n, dtype=np.int64)
+    v = rng.integers(-1000, 1000, n, dtype=np.int64)
+    nulls = rng.random(n) < 0.2
+    nulls[k == 7] = True
+    spec = oracle.make_agg_spec([LONG], [(AVG, LONG, DOUBLE, 0, 0, True), (AVG, LONG, DOUBLE)])
+    outs = oracle.hash_agg(spec, [Col(LONG, k)], [Col(LONG, v, nulls), Col(LONG, v, nulls)], 512)
+    keys, psum, mean = outs[0].values, outs[1], outs[2]
+    for i, g in enumerate(keys.tolist()):
+        sel = (k == g) & ~nulls
+        assert not psum.nulls[i] and psum.values[i] == float(v[sel].sum())
+        assert mean.nulls[i] == (sel.sum() == 0)
