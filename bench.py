@@ -187,6 +187,44 @@ def kernel_roofline(capi, prefix: str, algorithmic_bytes: float, launches_per_st
             "kernel_ms": round(avg, 4), "algorithmic_bytes": int(per_launch)}
 
 
+def bench_partition(args):
+    """Shuffle partition ids (HashUtil::ComputePartitionIds) of the C2 probe side's 64M int64 keys into 200
+    partitions (Spark's default shuffle partition count), one launch per step."""
+    from omnioperator_amd import capi
+
+    capi.init(0)
+    n = args.probe_rows
+    keys = np.random.default_rng(0xC2).integers(-2**63, 2**63 - 1, n, dtype=np.int64)
+    col = capi.DeviceColumn(capi.LONG, keys)
+    cols = (capi.Column * 1)(col.c())
+    out = capi.DeviceArray(n * 4, np.int32, (n,))
+
+    def step():
+        capi.partition_ids_raw(cols, n, args.partitions_out, out)
+
+    _, ms = timed_steps(step, args, capi)
+    ids = out.numpy()
+    assert ids.min() >= 0 and ids.max() < args.partitions_out
+    nbytes = n * (8 + 4)  # the key in, the int32 id out
+    roof = kernel_roofline(capi, "partition_ids", nbytes, 1, args)
+    if (n, args.partitions_out) == (64_000_000, 200):  # the profiled configuration
+        roof["traffic"] = profiled_traffic("partition_ids", "partition_ids.md")
+    cpu = None
+    if not args.no_cpu:
+        import oracle
+        sample = min(n, args.cpu_probe_rows)
+        t0 = time.perf_counter()
+        oracle.partition_ids([oracle.Col(oracle.LONG, keys[:sample])], args.partitions_out)
+        t1 = time.perf_counter()
+        cpu = {"value": sample / (t1 - t0), "unit": "rows/s", "cores": 1, "kind": "port",
+               "sample": f"first {sample} keys; host CPU {cpu_model()}"}
+    cfg = {"workload": f"shuffle partition ids: {n} int64 keys (uniform 64-bit) into {args.partitions_out} "
+                       "partitions, Murmur3 seed 42 + pmod",
+           "rows": n, "partitions": args.partitions_out, "output": "partition id int32 per row"}
+    return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
+            "dtype": "int64"}
+
+
 def bench_agg_plain(args):
     """1M rows, int64 key over 1,000 values, VectorBatches of 4096; SUM/COUNT/MIN/MAX."""
     from omnioperator_amd import capi, workloads
@@ -381,10 +419,10 @@ def bench_learned(args):
             "cpu_baseline": None, "dtype": "int64"}
 
 
-def profiled_traffic(row_name: str):
-    """HBM bytes per launch of the probe stage from the copy-calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE table in
-    profiles/join_probe_c2.md (the row named `row_name`); None when it was not profiled."""
-    path = os.path.join(ROOT, "profiles", "join_probe_c2.md")
+def profiled_traffic(row_name: str, profile: str = "join_probe_c2.md"):
+    """HBM bytes per launch from the copy-calibrated rocprofv3 FETCH_SIZE / WRITE_SIZE table of a profile under
+    profiles/ (the row named `row_name`); None when it was not profiled."""
+    path = os.path.join(ROOT, "profiles", profile)
     try:
         with open(path) as f:
             for line in f:
@@ -420,7 +458,7 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="join", choices=["join", "agg", "skewed", "batched", "learned"],
+    p.add_argument("--workload", default="join", choices=["join", "agg", "skewed", "batched", "learned", "partition"],
                    help="join: the headline inner-join configuration; agg: 1M-row plain group-by; skewed: 200M-row "
                         "composite-key group-by; batched: 20,000-batch stream; learned: the 512-task advisor set")
     p.add_argument("--skewed-rows", type=int, default=200_000_000)
@@ -434,6 +472,7 @@ def main():
     p.add_argument("--agg-form", default="batches", choices=["batches", "per_batch"],
                    help="plain group-by: all batches in one add_batches call, or one add_batch per batch")
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
+    p.add_argument("--partitions-out", type=int, default=200, help="partition workload: shuffle partitions")
     p.add_argument("--join-type", default="inner", choices=sorted(JOIN_TYPES),
                    help="join workload: join type (right_build_right keeps the build side; left keeps the probe side)")
     p.add_argument("--build-rows", type=int, default=2_000_000)
@@ -460,7 +499,7 @@ def main():
               "described in DESIGN.md and is not run.", file=sys.stderr)
         sys.exit(2)
     res = {"join": bench_join, "agg": bench_agg_plain, "skewed": bench_agg_skewed,
-           "batched": bench_batched, "learned": bench_learned}[args.workload](args)
+           "batched": bench_batched, "learned": bench_learned, "partition": bench_partition}[args.workload](args)
     single = args.workload == "learned"  # one pass over the task set; each task warmed by an untimed run
     line = {"metric": METRIC, "value": res["value"], "unit": "rows/s", "n_gpus": 1,
             "steps": 1 if single else args.steps, "warmup": 0 if single else args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",

@@ -295,6 +295,15 @@ int omni_gpu_join_outer_output(void *handle, int64_t *out_build_rows, omni_gpu_o
                                int64_t out_capacity, int64_t *num_rows);
 int omni_gpu_join_destroy(void *handle);
 
+/* ---- shuffle hash partitioning ----------------------------------------------------------------------------
+ * Replaces HashUtil::ComputePartitionIds (core/src/operator/hash_util.cpp:40-82), the partition id of each row
+ * for a hash exchange: a Murmur3 hash with seed 42 folded over the row's columns (HashInt of INT / DATE32 values,
+ * HashLong of LONG / TIMESTAMP / DECIMAL64 / DOUBLE bits, HashInt of 0 or 1 for BOOLEAN, a NULL leaving the hash
+ * unchanged; core/src/operator/util/mm3_util.h), then out_ids[i] = Pmod(hash, num_partitions).  Other column
+ * types are rejected.  out_ids is a device array of rows int32.  Synchronizes. */
+int omni_gpu_partition_ids(const omni_gpu_column *cols, int32_t num_cols, int64_t rows, int32_t num_partitions,
+                           int32_t *out_ids);
+
 /* ---- learned strategy / capacity advisor ----------------------------------------------------------------
  * No reference counterpart: the reference sizes its tables by growth only (BaseHashMap's Grower,
  * core/src/operator/hashmap/base_hash_map.h:255-292) and picks the table kind by a fixed rule.  These entry

@@ -112,6 +112,7 @@ SIGNATURES = {
     "omni_gpu_join_probe_exists": (_I, [_VP, _P(Column), _I64, _VP]),
     "omni_gpu_join_outer_output": (_I, [_VP, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_destroy": (_I, [_VP]),
+    "omni_gpu_partition_ids": (_I, [_P(Column), _I32, _I64, _I32, _VP]),
     "omni_gpu_sketch_keys": (_I, [_P(Column), _I32, _I64, _I32, _P(Sketch)]),
     "omni_gpu_advise": (_I, [_VP, _P(Column), _I32, _I64, _I32, _P(C.c_float)]),
     "omni_gpu_mlp_create": (_I, [C.c_uint64, _P(_VP)]),
@@ -550,6 +551,19 @@ class HashJoin:
             self.close()
         except Exception:
             pass
+
+
+def partition_ids_raw(cols: Sequence[Column], rows: int, partitions: int, out: "DeviceArray"):
+    """Shuffle partition ids of device columns into a device int32 array (omni_gpu_partition_ids)."""
+    init()
+    arr = cols if isinstance(cols, C.Array) else (Column * len(cols))(*cols)
+    check(lib().omni_gpu_partition_ids(arr, len(arr), rows, partitions, out.ptr))
+
+
+def partition_ids(cols: Sequence[Column], rows: int, partitions: int) -> np.ndarray:
+    out = DeviceArray(max(1, rows) * 4, np.int32, (rows,))
+    partition_ids_raw(cols, rows, partitions, out)
+    return out.numpy(rows)
 
 
 def sketch_keys(keys: Sequence[Column], rows: int, is_join: bool = False) -> Sketch:

@@ -149,6 +149,9 @@ def lib():
             "oracle_join_exists": (None, [vp, vp, i64, vp]),
             "oracle_join_outer": (i64, [vp, vp, i64]),
             "oracle_join_destroy": (None, [vp]),
+            "oracle_mm3_hash_int": (C.c_uint32, [C.c_uint32, C.c_uint32]),
+            "oracle_mm3_hash_long": (C.c_uint32, [C.c_uint64, C.c_uint32]),
+            "oracle_partition_ids": (None, [vp, i32, i64, i32, vp]),
             "oracle_last_error": (C.c_char_p, []),
         }
         for name, (res, args) in sig.items():
@@ -435,3 +438,22 @@ def hash_join_typed(build_key: Col, build_payloads: Sequence[Col], probe_key: Co
         return TypedJoinResult(probe_rows, build_rows, pays, outer, gather(outer) if mark else [], None)
     finally:
         L.oracle_join_destroy(h)
+
+
+def mm3_hash_int(v: int, seed: int = 42) -> int:
+    """Murmur3 HashInt (core/src/operator/util/mm3_util.h) of a 32-bit value, as a signed int32."""
+    return int(np.uint32(lib().oracle_mm3_hash_int(v & 0xFFFFFFFF, seed & 0xFFFFFFFF)).view(np.int32))
+
+
+def mm3_hash_long(v: int, seed: int = 42) -> int:
+    """Murmur3 HashLong of a 64-bit value, as a signed int32."""
+    return int(np.uint32(lib().oracle_mm3_hash_long(v & 0xFFFFFFFFFFFFFFFF, seed & 0xFFFFFFFF)).view(np.int32))
+
+
+def partition_ids(cols: Sequence[Col], partitions: int) -> np.ndarray:
+    """Restated HashUtil::ComputePartitionIds (core/src/operator/hash_util.cpp:40-82): one int32 per row."""
+    n = len(cols[0]) if cols else 0
+    out = np.zeros(max(1, n), np.int32)
+    arr, keep = _ocols(list(cols), max(1, len(cols)))
+    lib().oracle_partition_ids(arr, len(cols), n, partitions, _ptr(out))
+    return out[:n]

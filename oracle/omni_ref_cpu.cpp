@@ -872,6 +872,63 @@ private:
 };
 
 // ================================================================================================
+// Shuffle hash partitioning: HashUtil::ComputePartitionIds (core/src/operator/hash_util.cpp:40-82) with the
+// Murmur3 steps of core/src/operator/util/mm3_util.h (MixK1, MixH1, Fmix, HashInt, HashLong, Pmod).
+// ================================================================================================
+
+inline uint32_t Mm3Rotl(uint32_t i, uint32_t d) { return (i << d) | (i >> (32 - d)); }
+inline uint32_t Mm3MixK1(uint32_t k1) { return Mm3Rotl(k1 * 0xcc9e2d51u, 15) * 0x1b873593u; }
+inline uint32_t Mm3MixH1(uint32_t h1, uint32_t k1) { return Mm3Rotl(h1 ^ k1, 13) * 5u + 0xe6546b64u; }
+inline uint32_t Mm3Fmix(uint32_t h1, uint32_t length)
+{
+    h1 ^= length;
+    h1 ^= h1 >> 16;
+    h1 *= 0x85ebca6bu;
+    h1 ^= h1 >> 13;
+    h1 *= 0xc2b2ae35u;
+    return h1 ^ (h1 >> 16);
+}
+inline uint32_t Mm3HashInt(uint32_t v, uint32_t seed) { return Mm3Fmix(Mm3MixH1(seed, Mm3MixK1(v)), 4); }
+inline uint32_t Mm3HashLong(uint64_t v, uint32_t seed)
+{
+    uint32_t h1 = Mm3MixH1(seed, Mm3MixK1(static_cast<uint32_t>(v)));
+    h1 = Mm3MixH1(h1, Mm3MixK1(static_cast<uint32_t>(v >> 32)));
+    return Mm3Fmix(h1, 8);
+}
+inline int32_t Mm3Pmod(int32_t x, int32_t y)
+{
+    int32_t r = x % y;
+    return r < 0 ? r + y : r;
+}
+
+// Columns: INT / DATE32 (HashInt of the value), LONG / TIMESTAMP / DECIMAL64 / DOUBLE (HashLong of the 8 raw
+// bytes), BOOLEAN (HashInt of 0 or 1).  A NULL leaves the running hash unchanged; the seed is 42
+// (MM3HASH_SEED, hash_util.h:56).
+void PartitionIds(const HostColumn *cols, int32_t ncols, int64_t rows, int32_t partitions, int32_t *out)
+{
+    for (int64_t r = 0; r < rows; ++r) {
+        uint32_t h = 42;
+        for (int32_t c = 0; c < ncols; ++c) {
+            const HostColumn &col = cols[c];
+            if (IsNull(col, r)) continue;
+            switch (col.type_id) {
+                case OMNI_TYPE_BOOLEAN:
+                    h = Mm3HashInt(static_cast<const uint8_t *>(col.values)[r] != 0 ? 1u : 0u, h);
+                    break;
+                case OMNI_TYPE_INT:
+                case OMNI_TYPE_DATE32:
+                    h = Mm3HashInt(static_cast<const uint32_t *>(col.values)[r], h);
+                    break;
+                default:
+                    h = Mm3HashLong(static_cast<const uint64_t *>(col.values)[r], h);
+                    break;
+            }
+        }
+        out[r] = Mm3Pmod(static_cast<int32_t>(h), partitions);
+    }
+}
+
+// ================================================================================================
 // Inner hash join: HashBuilderOperator + JoinHashTableVariants + LookupJoinOperator
 // ================================================================================================
 
@@ -1242,5 +1299,15 @@ int64_t oracle_join_outer(void *h, int64_t *out_build, int64_t cap)
     return static_cast<HashJoin *>(h)->Outer(out_build, cap);
 }
 void oracle_join_destroy(void *h) { delete static_cast<HashJoin *>(h); }
+
+// --- shuffle partitioning ------------------------------------------------------------------------------
+uint32_t oracle_mm3_hash_int(uint32_t v, uint32_t seed) { return Mm3HashInt(v, seed); }
+uint32_t oracle_mm3_hash_long(uint64_t v, uint32_t seed) { return Mm3HashLong(v, seed); }
+void oracle_partition_ids(const oracle_column *cols, int32_t ncols, int64_t rows, int32_t partitions, int32_t *out)
+{
+    std::vector<HostColumn> c;
+    for (int i = 0; i < ncols; ++i) c.push_back({cols[i].type_id, cols[i].scale, cols[i].values, cols[i].nulls});
+    PartitionIds(c.data(), ncols, rows, partitions, out);
+}
 
 }  // extern "C"
