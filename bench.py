@@ -495,8 +495,9 @@ def main():
         print(json.dumps({"copy_bytes": n, "copies": 3}), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU design is "
-              "described in DESIGN.md and is not run.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the path is measured on one GPU's "
+              "share; sharding across GPUs (exchange of partial groups, broadcast build side) is not run.",
+              file=sys.stderr)
         sys.exit(2)
     res = {"join": bench_join, "agg": bench_agg_plain, "skewed": bench_agg_skewed,
            "batched": bench_batched, "learned": bench_learned, "partition": bench_partition}[args.workload](args)

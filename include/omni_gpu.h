@@ -311,7 +311,9 @@ int omni_gpu_partition_ids(const omni_gpu_column *cols, int32_t num_cols, int64_
  * the group count (output 0) and the fastest strategy (outputs 1..3: LDS-local, global-atomic,
  * radix-partitioned).  The prediction only sets a capacity hint and a strategy: results never depend on it. */
 typedef struct omni_gpu_sketch {
-    float features[16];     /* MLP input (see DESIGN) */
+    float features[16];     /* MLP input, each roughly in [0, 1]: log rows, log sample, log distinct, f1 / m,
+                               f2 / m, max / m, heavy / m, distinct / m, log GEE and Chao1 estimates, log key
+                               range, NULL share, key bytes, key columns, join flag, bias (m = sampled rows) */
     int64_t rows, sampled;  /* rows of the column, rows in the 1-in-k sample */
     int64_t distinct, f1, f2, max_count, heavy, nulls;  /* over the sample: distinct keys, keys seen once / twice,
                                                           the top key's count, rows of keys seen >= 4 times */
