@@ -217,7 +217,8 @@ int omni_gpu_agg_batches_destroy(void *handle);
  * join_hash_table_variants.cpp:254-283, 695-808) and LookupJoinOperator::ProbeBatchForInnerJoin
  * (core/src/operator/join/lookup_join.cpp:924-1035) with its output gather (:1657-1779). */
 typedef struct omni_gpu_join_spec {
-    int32_t key_type;                 /* single equality key: INT, DATE32, SHORT, LONG, TIMESTAMP, DECIMAL64 */
+    int32_t key_type;                 /* single equality key: INT, DATE32, SHORT, LONG, TIMESTAMP, DECIMAL64,
+                                         DOUBLE (compared as its 64-bit pattern), BOOLEAN */
     int32_t num_payloads;             /* build-side output columns */
     int32_t payload_types[OMNI_GPU_MAX_AGGS];
     int32_t strategy;                 /* 0 = reference rule (array table if dense), 1 = hash only */

@@ -395,8 +395,11 @@ extern "C" {
 int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle)
 {
     OG_CHECK(spec && handle, OMNI_GPU_ERR_INVALID, "null argument");
-    OG_CHECK(IsIntegerKeyType(spec->key_type), OMNI_GPU_ERR_INVALID,
-             "join key type %d is not supported (integer-like single keys only)", spec->key_type);
+    // single keys the reference keeps in a fixed-width table (HashBuilderOperatorFactory::InitVariant,
+    // hash_builder.cpp:45-80): DOUBLE joins on its 64-bit pattern (the int64 table), BOOLEAN on 0 / 1
+    OG_CHECK(IsIntegerKeyType(spec->key_type) || spec->key_type == OMNI_TYPE_DOUBLE ||
+                 spec->key_type == OMNI_TYPE_BOOLEAN,
+             OMNI_GPU_ERR_INVALID, "join key type %d is not supported (fixed-width single keys only)", spec->key_type);
     OG_CHECK(spec->num_payloads >= 0 && spec->num_payloads <= OMNI_GPU_MAX_AGGS, OMNI_GPU_ERR_INVALID,
              "num_payloads out of range");
     for (int p = 0; p < spec->num_payloads; ++p) {

@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle
-from oracle import INT, LONG, SHORT, Col
+from oracle import BOOLEAN, DOUBLE, INT, LONG, SHORT, Col
 from omnioperator_amd import capi, workloads
 from tests.helpers import load_golden
 
@@ -322,3 +322,38 @@ def test_inner_join_config_full_size(dense, partitions):
     sel = rows < 1_000_000
     assert sorted(zip(rows[sel].tolist(), pay[sel].tolist())) == sorted(zip(r.probe_rows.tolist(),
                                                                              r.payloads[0].values.tolist()))
+
+
+@pytest.mark.parametrize("strategy", [0, 1])
+def test_double_keys_join_on_bit_patterns(strategy):
+    """DOUBLE keys join on their 64-bit pattern, as the reference's int64 table compares them: 0.0 and -0.0 differ
+    (-0.0's pattern is the slot sentinel), NaNs match only the same pattern."""
+    rng = np.random.default_rng(21)
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 1.5, -1.5], np.float64)
+    other_nan = np.array([0x7FF8000000000001], np.int64).view(np.float64)
+    bkv = np.concatenate([special, rng.normal(0, 1e3, 3000).round(2)])
+    bkv = np.concatenate([bkv, bkv[rng.integers(0, len(bkv), 500)]])  # duplicates
+    pkv = np.concatenate([special, other_nan, bkv[rng.integers(0, len(bkv), 20000)],
+                          rng.normal(0, 1e3, 5000).round(2)])
+    bk = Col(DOUBLE, bkv, rng.random(len(bkv)) < 0.02)
+    pk = Col(DOUBLE, pkv, rng.random(len(pkv)) < 0.02)
+    pays = [Col(LONG, np.arange(len(bkv), dtype=np.int64))]
+    rows, gp, _ = gpu_join(bk, pays, pk, 1024, 4096, strategy=strategy)
+    want, _ = oracle_set(bk, pays, pk, 1024, 4096)
+    assert as_set(rows, gp) == want
+    # probe row 1 is -0.0: it never meets a +0.0 build row (payload = build row id)
+    pos0 = {i for i, v in enumerate(bkv) if v == 0 and not np.signbit(v)}
+    assert not ({b for p, b in zip(rows.tolist(), gp[0][0].tolist()) if p == 1} & pos0)
+
+
+def test_boolean_keys():
+    rng = np.random.default_rng(22)
+    bkv = (rng.random(300) < 0.5).astype(np.uint8)
+    pkv = (rng.random(1000) < 0.5).astype(np.uint8)
+    bk = Col(BOOLEAN, bkv, rng.random(300) < 0.1)
+    pk = Col(BOOLEAN, pkv, rng.random(1000) < 0.1)
+    pays = [Col(INT, np.arange(300, dtype=np.int32))]
+    for strategy in (0, 1):
+        rows, gp, _ = gpu_join(bk, pays, pk, 128, 256, strategy=strategy)
+        want, _ = oracle_set(bk, pays, pk, 128, 256)
+        assert as_set(rows, gp) == want
