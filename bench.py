@@ -237,7 +237,8 @@ def bench_agg_plain(args):
     spans = workloads.batches(n, w.batch_rows)
     cols = [([kd.c(a)], [vd.c(a), None, vd.c(a), vd.c(a)], b - a) for a, b in spans]
 
-    spec = capi.make_agg_spec([LONG], aggs)
+    rule = capi.HASH_TABLE_ARRAY if args.agg_rule == "array" else capi.HASH_TABLE_NORMAL
+    spec = capi.make_agg_spec([LONG], aggs, hash_table_rule=rule)
     per_batch = args.agg_form == "per_batch"
     arrays = capi.HashAgg.batch_arrays_for(spec, cols)
 
@@ -269,6 +270,7 @@ def bench_agg_plain(args):
                        "SUM/COUNT(*)/MIN/MAX", "rows": n, "groups": groups, "batch_rows": w.batch_rows,
            "form": "one omni_gpu_agg_add_batch per VectorBatch" if per_batch else
                    "all VectorBatches in one omni_gpu_agg_add_batches call",
+           "hash_table_rule": args.agg_rule,
            "algorithmic_bytes_per_step": alg}
     return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
             "dtype": "int64"}
@@ -473,6 +475,8 @@ def main():
                    help="plain group-by: all batches in one add_batches call, or one add_batch per batch")
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
     p.add_argument("--partitions-out", type=int, default=200, help="partition workload: shuffle partitions")
+    p.add_argument("--agg-rule", default="normal", choices=["normal", "array"],
+                   help="agg workload: AggHashTableRule (array = direct-index map for the single integer key)")
     p.add_argument("--join-type", default="inner", choices=sorted(JOIN_TYPES),
                    help="join workload: join type (right_build_right keeps the build side; left keeps the probe side)")
     p.add_argument("--build-rows", type=int, default=2_000_000)

@@ -81,6 +81,9 @@ extern "C" {
 #define OMNI_GPU_BUILD_LEFT 1
 #define OMNI_GPU_BUILD_RIGHT 2
 
+#define OMNI_GPU_HASH_TABLE_NORMAL 0
+#define OMNI_GPU_HASH_TABLE_ARRAY 1
+
 #define OMNI_GPU_MAX_KEYS 4
 #define OMNI_GPU_MAX_AGGS 16
 
@@ -144,6 +147,12 @@ typedef struct omni_gpu_agg_spec {
      * (average_flatim_aggregator.h:256-273); a COUNT_COLUMN over the same input gives the second.  The other
      * functions' partial output equals their final output, and the flag is ignored for them. */
     int32_t agg_partial_out[OMNI_GPU_MAX_AGGS];
+    /* AggHashTableRule (core/src/util/policy.h:51: NORMAL or ARRAY; group_aggregation.cpp:320-360).  Under
+     * OMNI_GPU_HASH_TABLE_ARRAY a single integer key whose first inserted rows span at most 2^21 - 2 values
+     * gets a direct-index group map over that range widened by max(50, max - min) on each side
+     * (VectorAnalyzer::CheckArrayMap, core/src/operator/hashmap/vector_analyzer.cpp:10-80); keys outside it use
+     * the hash table.  Results do not depend on the rule. */
+    int32_t hash_table_rule;
 } omni_gpu_agg_spec;
 
 int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle);
@@ -171,6 +180,9 @@ int omni_gpu_agg_num_groups(void *handle, int64_t *groups);
  * Synchronizes.  Returns OMNI_GPU_ERR_OVERFLOW if a decimal aggregate overflowed and
  * overflow_as_null == 0. */
 int omni_gpu_agg_get_output(void *handle, int64_t start, int64_t count, omni_gpu_out_column *outs);
+/* The table kind chosen so far: *array_map = 1 when the direct-index map of the ARRAY rule is in use, with its
+ * key range [*range_min, *range_min + *range_cells) (the reference's VectorAnalyzer range). */
+int omni_gpu_agg_table_info(void *handle, int32_t *array_map, int64_t *range_min, int64_t *range_cells);
 int omni_gpu_agg_destroy(void *handle);
 
 /* ---- batched stream of independent group-bys ----------------------------------------------------------

@@ -21,6 +21,7 @@ INT, LONG, DOUBLE, BOOLEAN, SHORT, DECIMAL64, DECIMAL128, DATE32, TIMESTAMP = 1,
 SUM, COUNT_COLUMN, COUNT_ALL, AVG, MAX, MIN = 0, 1, 2, 3, 5, 6
 STRATEGY_AUTO, STRATEGY_LDS, STRATEGY_GLOBAL, STRATEGY_RADIX = 0, 1, 2, 3
 MAX_KEYS, MAX_AGGS = 4, 16
+HASH_TABLE_NORMAL, HASH_TABLE_ARRAY = 0, 1  # AggHashTableRule (core/src/util/policy.h:51)
 # JoinType / BuildSide values shared with the reference (core/src/plannode/planNode.h:406-414, 28-32).
 JOIN_INNER, JOIN_LEFT, JOIN_RIGHT, JOIN_FULL, JOIN_LEFT_SEMI, JOIN_LEFT_ANTI, JOIN_EXISTENCE = range(7)
 BUILD_UNKNOWN, BUILD_LEFT, BUILD_RIGHT = 0, 1, 2
@@ -53,7 +54,7 @@ class AggSpec(C.Structure):
         ("agg_in_types", C.c_int32 * MAX_AGGS), ("agg_out_types", C.c_int32 * MAX_AGGS),
         ("agg_in_scales", C.c_int32 * MAX_AGGS), ("agg_out_scales", C.c_int32 * MAX_AGGS),
         ("overflow_as_null", C.c_int32), ("strategy", C.c_int32), ("capacity_hint", C.c_int64),
-        ("agg_partial_out", C.c_int32 * MAX_AGGS),
+        ("agg_partial_out", C.c_int32 * MAX_AGGS), ("hash_table_rule", C.c_int32),
     ]
 
 
@@ -108,6 +109,7 @@ SIGNATURES = {
     "omni_gpu_join_probe_info": (_I, [_VP, _P(_I32)]),
     "omni_gpu_join_probe": (_I, [_VP, _P(Column), _I64, _I64, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_probe_batches": (_I, [_VP, _P(Column), _P(_I64), _I32, _VP, _P(OutColumn), _I64, _P(_I64)]),
+    "omni_gpu_agg_table_info": (_I, [_VP, _P(_I32), _P(_I64), _P(_I64)]),
     "omni_gpu_join_set_type": (_I, [_VP, _I32, _I32]),
     "omni_gpu_join_probe_exists": (_I, [_VP, _P(Column), _I64, _VP]),
     "omni_gpu_join_outer_output": (_I, [_VP, _VP, _P(OutColumn), _I64, _P(_I64)]),
@@ -235,7 +237,7 @@ class DeviceColumn:
 
 
 def make_agg_spec(key_types: Sequence[int], aggs: Sequence[tuple], overflow_as_null: bool = True,
-                  strategy: int = STRATEGY_AUTO, capacity_hint: int = 0) -> AggSpec:
+                  strategy: int = STRATEGY_AUTO, capacity_hint: int = 0, hash_table_rule: int = 0) -> AggSpec:
     """aggs: tuples (func, in_type, out_type[, in_scale, out_scale[, partial_out]])."""
     s = AggSpec()
     s.num_keys = len(key_types)
@@ -251,6 +253,7 @@ def make_agg_spec(key_types: Sequence[int], aggs: Sequence[tuple], overflow_as_n
     s.overflow_as_null = 1 if overflow_as_null else 0
     s.strategy = strategy
     s.capacity_hint = capacity_hint
+    s.hash_table_rule = hash_table_rule
     return s
 
 
@@ -297,6 +300,12 @@ class HashAgg:
 
     def add_batches_raw(self, karr, aarr, rarr, n: int):
         check(lib().omni_gpu_agg_add_batches(self.h, karr, aarr, rarr, n))
+
+    def table_info(self):
+        """(direct-index map in use, its first key, its cells) -- the ARRAY rule's decision."""
+        a, lo, n = C.c_int32(), C.c_int64(), C.c_int64()
+        check(lib().omni_gpu_agg_table_info(self.h, C.byref(a), C.byref(lo), C.byref(n)))
+        return bool(a.value), lo.value, n.value
 
     def num_groups(self) -> int:
         g = C.c_int64()

@@ -21,6 +21,7 @@
 // (wrapping SUM, COUNT, MIN, MAX and exact 192-bit decimal accumulation); double SUM/AVG depend on the
 // order of atomic additions and are held to a relative tolerance.
 #include <algorithm>
+#include <limits>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -57,13 +58,17 @@ struct AggHandle {
     DevBuf index, tile_counts;      // output compaction: live group ids in ascending order
     DevBuf stage_vals[kStageCols], stage_nulls[kStageCols], stage_src, stage_rows;  // omni_gpu_agg_add_batches
     DevBuf counters;  // [0] ids handed out u64, [8] overflow u32, [12] error u32, [16..24) special u32[2],
-                      // [24] spills u64, [32] live groups u32
+                      // [24] spills u64, [32] live groups u32, [64] key min / max i64[2] (ARRAY rule)
     int64_t known_ids = 0;          // ids handed out at the last synchronisation
     int64_t pending_rows = 0;       // rows added since then
     int64_t live_groups = -1;       // live groups after the last compaction (-1: stale)
     bool use_lds = false;
     LdsLayout lds{};
     int64_t lds_rows = 0;    // rows sent through the LDS kernel since the last spill check
+    bool map_planned = false;       // the direct-index map was considered (first insert)
+    DevBuf amap;                    // direct-index map cells (ARRAY rule)
+    ArrayMapDev am{};
+    long long map_min = 0;          // the map's key range [map_min, map_min + am.range)
     int64_t spills_seen = 0;
 
     unsigned long long *ngroups() const { return reinterpret_cast<unsigned long long *>(counters.as<uint8_t>()); }
@@ -271,8 +276,8 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     for (int i = 0; i < spec->num_aggs; ++i) OG_TRY(ClassifyAgg(*spec, i, h->args.a[i]));
     AssignRowLayout(h->args);
     OG_TRY(AcquireStream(&h->stream));
-    OG_TRY(h->counters.Reserve(64, h->stream));
-    OG_HIP(hipMemsetAsync(h->counters.p, 0, 64, h->stream));
+    OG_TRY(h->counters.Reserve(80, h->stream));  // 64 bytes of counters, then the key min / max of the ARRAY rule
+    OG_HIP(hipMemsetAsync(h->counters.p, 0, 80, h->stream));
     OG_TRY(h->cursors.Reserve(sizeof(unsigned long long) * 2 * kCursorSlots, h->stream));
     OG_HIP(hipMemsetAsync(h->cursors.p, 0, sizeof(unsigned long long) * 2 * kCursorSlots, h->stream));
     std::vector<unsigned long long> tmpl = RowTemplate(h->args);
@@ -293,6 +298,56 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
         h->use_lds = ldsFits && (spec->capacity_hint <= 0 || spec->capacity_hint * 2 <= h->lds.S);
     }
     *handle = h.release();
+    return OMNI_GPU_OK;
+}
+
+// The ARRAY hash-table rule (AggHashTableRule::ARRAY) for one integer key: VectorAnalyzer::CheckArrayMap
+// (core/src/operator/hashmap/vector_analyzer.cpp:10-80) over the first inserted rows -- no array when every key is
+// NULL, when max - min could overflow or exceeds 2^21 - 2; otherwise the range is widened by max(50, max - min)
+// on each side, saturating at the type's limits.  The reference resizes once and then moves to its hash map
+// when later keys leave the range; here such keys go to the hash table directly (results are the same).
+int PlanArrayMap(AggHandle &h, int64_t rows)
+{
+    h.map_planned = true;
+    const int t = h.spec.key_types[0];
+    if (h.spec.hash_table_rule != OMNI_GPU_HASH_TABLE_ARRAY || !h.single || h.kd.num != 1 ||
+        !(t == OMNI_TYPE_SHORT || t == OMNI_TYPE_INT || t == OMNI_TYPE_DATE32 || t == OMNI_TYPE_LONG ||
+          t == OMNI_TYPE_TIMESTAMP || t == OMNI_TYPE_DECIMAL64)) {
+        return OMNI_GPU_OK;
+    }
+    hipStream_t s = h.stream;
+    uint8_t *sc = h.counters.as<uint8_t>() + 64;  // the min / max pair (the counters allocation holds 80 bytes)
+    long long mm[2] = {std::numeric_limits<long long>::max(), std::numeric_limits<long long>::min()};
+    OG_TRY(CopyToDevice(sc, mm, sizeof(mm), s));
+    {
+        KernelScope ks(s, "agg_key_minmax");
+        const unsigned int grid = static_cast<unsigned int>(std::min<int64_t>(Blocks(rows), 1024));
+        agg_key_minmax<<<grid, kBlock, 0, s>>>(h.kd, rows, reinterpret_cast<long long *>(sc));
+        OG_TRY(CheckLaunch("agg_key_minmax"));
+    }
+    OG_TRY(CopyToHost(s, {{mm, sc, sizeof(mm)}}));
+    const long long mn = mm[0], mx = mm[1];
+    if (mx < mn) return OMNI_GPU_OK;                                             // every key NULL
+    if (mx >= 0 && mn < mx - std::numeric_limits<long long>::max()) return OMNI_GPU_OK;  // max - min overflows
+    if (mx - mn > (2LL << 20) - 2) return OMNI_GPU_OK;                           // range above 2M entries
+    long long tmin = std::numeric_limits<long long>::min(), tmax = std::numeric_limits<long long>::max();
+    if (t == OMNI_TYPE_SHORT) {
+        tmin = std::numeric_limits<int16_t>::min();
+        tmax = std::numeric_limits<int16_t>::max();
+    } else if (t == OMNI_TYPE_INT || t == OMNI_TYPE_DATE32) {
+        tmin = std::numeric_limits<int32_t>::min();
+        tmax = std::numeric_limits<int32_t>::max();
+    }
+    const long long reserve = std::max(50LL, mx - mn);
+    const long long hi = mx <= tmax - reserve ? mx + reserve : tmax;
+    const long long lo = mn >= tmin + reserve ? mn - reserve : tmin;
+    const uint64_t cells = static_cast<uint64_t>(hi) - static_cast<uint64_t>(lo) + 1;
+    OG_TRY(h.amap.Reserve(sizeof(unsigned int) * cells, s));
+    OG_TRY(Fill32(s, h.amap.p, 0u, static_cast<int64_t>(cells)));
+    h.am.cells = h.amap.as<unsigned int>();
+    h.am.min = static_cast<unsigned long long>(lo);
+    h.am.range = cells;
+    h.map_min = lo;
     return OMNI_GPU_OK;
 }
 
@@ -337,6 +392,7 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
             OG_TRY(Resize(h, cap, h.known_ids));
         }
     }
+    if (!h.map_planned) OG_TRY(PlanArrayMap(h, rows));
     GroupCtx g = h.Ctx();
     unsigned long long *cur = h.cursors.as<unsigned long long>();
     if (h.use_lds) {
@@ -347,7 +403,7 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         KernelScope ks(h.stream, "agg_insert_lds");
         if (h.single) {
             agg_insert_lds<<<static_cast<unsigned int>(blocks), kLdsBlock, h.lds.bytes, h.stream>>>(
-                h.kd, rows, per, h.slots.as<Slot1>(), h.cap - 1, g, h.args, h.lds, h.spills(), cur);
+                h.kd, rows, per, h.slots.as<Slot1>(), h.cap - 1, g, h.args, h.lds, h.spills(), cur, h.am);
         } else {
             agg_insert_lds_multi<<<static_cast<unsigned int>(blocks), kLdsBlock, h.lds.bytes, h.stream>>>(
                 h.kd, rows, per, h.ctrl.as<unsigned int>(), h.skeys.as<unsigned long long>(), h.cap - 1, g, h.args,
@@ -362,7 +418,8 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         const unsigned int grid = static_cast<unsigned int>(std::min<int64_t>(Blocks(rows), kMaxInsertBlocks));
         KernelScope ks(h.stream, h.single ? "agg_insert_single" : "agg_insert_multi");
         if (h.single) {
-            agg_insert_single<<<grid, kBlock, 0, h.stream>>>(h.kd, rows, h.slots.as<Slot1>(), h.cap - 1, g, h.args, cur);
+            agg_insert_single<<<grid, kBlock, 0, h.stream>>>(h.kd, rows, h.slots.as<Slot1>(), h.cap - 1, g, h.args, cur,
+                                                             h.am);
         } else {
             agg_insert_multi<<<grid, kBlock, 0, h.stream>>>(h.kd, rows, h.ctrl.as<unsigned int>(),
                                                             h.skeys.as<unsigned long long>(), h.cap - 1, g, h.args, cur);
@@ -523,13 +580,24 @@ int omni_gpu_agg_get_output(void *handle, int64_t start, int64_t count, omni_gpu
     return OMNI_GPU_OK;
 }
 
+int omni_gpu_agg_table_info(void *handle, int32_t *array_map, int64_t *range_min, int64_t *range_cells)
+{
+    OG_CHECK(handle && array_map && range_min && range_cells, OMNI_GPU_ERR_INVALID, "null argument");
+    const AggHandle &h = *static_cast<const AggHandle *>(handle);
+    *array_map = h.am.cells != nullptr ? 1 : 0;
+    *range_min = h.am.cells != nullptr ? h.map_min : 0;
+    *range_cells = h.am.cells != nullptr ? static_cast<int64_t>(h.am.range) : 0;
+    return OMNI_GPU_OK;
+}
+
 int omni_gpu_agg_destroy(void *handle)
 {
     if (!handle) return OMNI_GPU_OK;
     AggHandle *h = static_cast<AggHandle *>(handle);
     hipStream_t s = h->stream;
     for (DevBuf *b : {&h->slots, &h->ctrl, &h->skeys, &h->gkeys, &h->gnull, &h->gvalid, &h->states, &h->row_tmpl,
-                      &h->cursors, &h->index, &h->tile_counts, &h->counters, &h->stage_src, &h->stage_rows}) {
+                      &h->cursors, &h->index, &h->tile_counts, &h->counters, &h->stage_src, &h->stage_rows,
+                      &h->amap}) {
         b->Release(s);
     }
     for (int c = 0; c < kStageCols; ++c) {

@@ -168,10 +168,11 @@ __device__ __forceinline__ void StoreWaveChunk(const unsigned long long *lds, un
     }
 }
 
-// Group of a side cell (NULL key, or the key equal to the slot sentinel), created on first use.
-static __device__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned long long key, bool isNull, int W)
+// Group of a cell (0 empty, 1 being created, gid + 2 published), created on first use: a side cell (NULL key,
+// or the key equal to the slot sentinel) or a cell of the direct-index map.
+static __device__ unsigned int CellGroup(const GroupCtx &g, unsigned int *cell, unsigned long long key, bool isNull,
+                                         int W)
 {
-    unsigned int *cell = g.special + which;
     while (true) {
         unsigned int c = LoadRelaxed32(cell);
         if (c >= 2) return c - 2;
@@ -189,6 +190,21 @@ static __device__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned 
         }
     }
 }
+
+static __device__ __forceinline__ unsigned int SideGroup(const GroupCtx &g, int which, unsigned long long key,
+                                                         bool isNull, int W)
+{
+    return CellGroup(g, g.special + which, key, isNull, W);
+}
+
+// Direct-index group map of a single integer key (the ARRAY hash-table rule): key k in [min, min + range) owns
+// cell k - min.  Keys outside the range use the hash table; both hand out ids from the same counter, so every
+// group has one id whichever table found it.
+struct ArrayMapDev {
+    unsigned int *cells;        // nullptr: no map
+    unsigned long long min;     // the range's first key, as a sign-extended word
+    unsigned long long range;   // cells
+};
 
 // Find-or-insert of a single-word key; returns its group id.
 static __device__ unsigned int InsertSingle(Slot1 *slots, unsigned long long mask, unsigned long long key,
@@ -220,6 +236,15 @@ static __device__ unsigned int InsertSingle(Slot1 *slots, unsigned long long mas
         }
         pos = (pos + 1) & mask;
     }
+}
+
+// Group id of a non-NULL single-word key: its cell of the direct-index map when the key is in range, else the
+// side cell of the slot sentinel or the hash table.
+static __device__ __forceinline__ unsigned int KeyGroup(const ArrayMapDev &am, Slot1 *slots, unsigned long long mask,
+                                                        unsigned long long key, const GroupCtx &g)
+{
+    if (am.cells != nullptr && key - am.min < am.range) return CellGroup(g, am.cells + (key - am.min), key, false, 1);
+    return key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g);
 }
 
 // Find-or-insert of a W-word key (ctrl-word protocol).

@@ -13,9 +13,33 @@ constexpr int kCursorSlots = 16384;       // wave slots of the persistent insert
 constexpr int kMaxInsertBlocks = kCursorSlots / kWavesPerBlock;
 
 // Rows are visited grid-stride by a fixed grid, so wave slot w keeps its id chunk across launches.
+// Min / max of the non-NULL values of a single key column (the first batch the direct-index map is planned
+// from): one atomic pair per wave into mm = {min, max} (pre-set to {INT64_MAX, INT64_MIN}).
+__global__ void __launch_bounds__(kBlock) agg_key_minmax(KeyDev kd, int64_t rows, long long *mm)
+{
+    long long lo = 0x7FFFFFFFFFFFFFFFLL, hi = -0x7FFFFFFFFFFFFFFFLL - 1;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < rows; row += stride) {
+        if (NullBit(kd.nulls[0], row)) continue;
+        const long long k = static_cast<long long>(LoadWord(kd.values[0], kd.width[0], row));
+        lo = k < lo ? k : lo;
+        hi = k > hi ? k : hi;
+    }
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+        const long long l2 = __shfl_xor(lo, d, 64), h2 = __shfl_xor(hi, d, 64);
+        lo = l2 < lo ? l2 : lo;
+        hi = h2 > hi ? h2 : hi;
+    }
+    if (LaneId() == 0 && lo <= hi) {
+        atomicMin(&mm[0], lo);
+        atomicMax(&mm[1], hi);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) agg_insert_single(KeyDev kd, int64_t rows, Slot1 *slots,
                                                             unsigned long long mask, GroupCtx g, AggArgs args,
-                                                            unsigned long long *cursors)
+                                                            unsigned long long *cursors, ArrayMapDev am)
 {
     __shared__ unsigned long long wc[kWavesPerBlock][2];
     const int slot = blockIdx.x * kWavesPerBlock + (WaveInBlock());
@@ -27,8 +51,7 @@ __global__ void __launch_bounds__(kBlock) agg_insert_single(KeyDev kd, int64_t r
         if (NullBit(kd.nulls[0], row)) {
             gid = SideGroup(g, 0, 0ULL, true, 1);
         } else {
-            unsigned long long key = LoadWord(kd.values[0], kd.width[0], row);
-            gid = key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g);
+            gid = KeyGroup(am, slots, mask, LoadWord(kd.values[0], kd.width[0], row), g);
         }
         if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
     }
@@ -66,7 +89,7 @@ __global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t r
                                                              Slot1 *slots, unsigned long long mask, GroupCtx g,
                                                              AggArgs args, LdsLayout L,
                                                              unsigned long long *spill_counter,
-                                                             unsigned long long *cursors)
+                                                             unsigned long long *cursors, ArrayMapDev am)
 {
     OG_DYNAMIC_LDS(unsigned long long, lds_words);
     __shared__ unsigned long long wc[kLdsWaves][2];
@@ -118,8 +141,7 @@ __global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t r
             UpdateAggsT(args, LdsRef{base, L, slot}, row);
         } else {
             ++spilled;
-            unsigned int gid = isNull ? SideGroup(g, 0, 0ULL, true, 1)
-                                      : (key == kEmptyKey ? SideGroup(g, 1, key, false, 1) : InsertSingle(slots, mask, key, g));
+            unsigned int gid = isNull ? SideGroup(g, 0, 0ULL, true, 1) : KeyGroup(am, slots, mask, key, g);
             if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
         }
     }
@@ -127,7 +149,7 @@ __global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t r
     for (int i = threadIdx.x; i < S; i += blockDim.x) {
         unsigned long long k = lkeys[i];
         if (k == kEmptyKey) continue;
-        unsigned int gid = InsertSingle(slots, mask, k, g);
+        unsigned int gid = KeyGroup(am, slots, mask, k, g);  // LDS keys are never the slot sentinel
         if (gid != 0xFFFFFFFFu) MergeAggs(args, gid, LdsRef{base, L, i});
     }
     if (spilled) atomicAdd(spill_counter, static_cast<unsigned long long>(spilled));

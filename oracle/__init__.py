@@ -71,7 +71,7 @@ class AggSpec(C.Structure):
         ("agg_in_types", C.c_int32 * MAX_AGGS), ("agg_out_types", C.c_int32 * MAX_AGGS),
         ("agg_in_scales", C.c_int32 * MAX_AGGS), ("agg_out_scales", C.c_int32 * MAX_AGGS),
         ("overflow_as_null", C.c_int32), ("strategy", C.c_int32), ("capacity_hint", C.c_int64),
-        ("agg_partial_out", C.c_int32 * MAX_AGGS),
+        ("agg_partial_out", C.c_int32 * MAX_AGGS), ("hash_table_rule", C.c_int32),
     ]
 
 

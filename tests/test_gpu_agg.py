@@ -4,7 +4,8 @@ import numpy as np
 import pytest
 
 import oracle
-from oracle import (AVG, COUNT_ALL, COUNT_COLUMN, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, MIN, SUM, Col)
+from oracle import (AVG, COUNT_ALL, COUNT_COLUMN, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, MIN, SHORT, SUM,
+                    Col)
 from omnioperator_amd import capi, workloads
 from tests.helpers import assert_row_sets_equal, load_golden, rows_of
 from tests.test_oracle_agg import STD_AGGS, TYPES, _random_case
@@ -318,3 +319,76 @@ def test_partial_avg_output(in_type):
     assert_row_sets_equal(got, want, float_cols=[1, 3], rtol=DOUBLE_RTOL)
     row5 = [r for r in got if r[0] == 5][0]
     assert row5[1] == 0.0 and row5[2] == 0 and row5[3] is None
+
+
+def run_gpu_rule(key_cols, aggs, agg_cols, batch_rows, strategy=capi.STRATEGY_AUTO):
+    """run_gpu under the ARRAY hash-table rule; also returns the table decision."""
+    spec = capi.make_agg_spec([c.type_id for c in key_cols], aggs, True, strategy, 0, capi.HASH_TABLE_ARRAY)
+    h = capi.HashAgg(spec)
+    keep = []
+    for a, b in workloads.batches(len(key_cols[0]), batch_rows):
+        kd = [capi.DeviceColumn(c.type_id, c.values[a:b], None if c.nulls is None else c.nulls[a:b]) for c in key_cols]
+        ad = [None if c is None else capi.DeviceColumn(c.type_id, c.values[a:b], None if c.nulls is None else c.nulls[a:b])
+              for c in agg_cols]
+        keep.append((kd, ad))
+        h.add_batch([d.c() for d in kd], [None if d is None else d.c() for d in ad], b - a)
+    info = h.table_info()
+    outs = h.output()
+    types = [c.type_id for c in key_cols] + [a[2] for a in aggs]
+    h.close()
+    return rows_of([(t, v, nl) for t, (v, nl) in zip(types, outs)]), info
+
+
+def expected_array_range(first, t):
+    """VectorAnalyzer::CheckArrayMap over the first batch's non-NULL keys: (min - reserve, cells), saturated."""
+    lim = {SHORT: (-2**15, 2**15 - 1), INT: (-2**31, 2**31 - 1), LONG: (-2**63, 2**63 - 1)}[t]
+    mn, mx = int(first.min()), int(first.max())
+    reserve = max(50, mx - mn)
+    hi = mx + reserve if mx <= lim[1] - reserve else lim[1]
+    lo = mn - reserve if mn >= lim[0] + reserve else lim[0]
+    return lo, hi - lo + 1
+
+
+@pytest.mark.parametrize("t", [SHORT, INT, LONG])
+@pytest.mark.parametrize("strategy", [capi.STRATEGY_AUTO, capi.STRATEGY_GLOBAL])
+def test_array_rule_parity(t, strategy):
+    """ARRAY rule: keys of the first batch plan the direct-index map; later batches reach keys outside it, which go
+    to the hash table.  The groups equal the oracle's whatever table found them."""
+    rng = np.random.default_rng(31 + t + strategy)
+    dt = oracle.np_dtype(t)
+    n, batch = 60_000, 8192
+    base = -2**15 + 10 if t == SHORT else -700
+    k = (base + rng.integers(0, 1500, n)).astype(dt)
+    far = rng.random(n) < 0.15
+    far[:batch] = False
+    k[far] = rng.integers(np.iinfo(dt).min // 2, np.iinfo(dt).max // 2, int(far.sum())).astype(dt)
+    knull = rng.random(n) < 0.05
+    v = Col(LONG, rng.integers(-10**6, 10**6, n, dtype=np.int64), rng.random(n) < 0.1)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, LONG, LONG), (MAX, LONG, LONG)]
+    got, info = run_gpu_rule([Col(t, k, knull)], aggs, [v, None, v, v], batch, strategy)
+    want = run_oracle([Col(t, k, knull)], aggs, [v, None, v, v], batch)
+    assert_row_sets_equal(got, want)
+    first = k[:batch][~knull[:batch]]
+    lo, cells = expected_array_range(first, t)
+    assert info == (True, lo, cells)
+
+
+def test_array_rule_declines():
+    """No direct-index map when the first batch's keys are all NULL or span more than 2^21 - 2 values; results are
+    unchanged."""
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
+    v = Col(LONG, np.arange(4000, dtype=np.int64))
+    k = np.arange(4000, dtype=np.int64) % 37
+    nulls = np.zeros(4000, bool)
+    nulls[:2048] = True
+    got, info = run_gpu_rule([Col(LONG, k, nulls)], aggs, [v, None], 2048)
+    assert info == (False, 0, 0)
+    assert_row_sets_equal(got, run_oracle([Col(LONG, k, nulls)], aggs, [v, None], 2048))
+    k2 = np.array([0, 2**21 - 1] * 1000, np.int64)
+    v2 = Col(LONG, v.values[:2000])
+    got, info = run_gpu_rule([Col(LONG, k2)], aggs, [v2, None], 2000)
+    assert info == (False, 0, 0)
+    k3 = np.array([0, 2**21 - 2] * 1000, np.int64)
+    got, info = run_gpu_rule([Col(LONG, k3)], aggs, [v2, None], 2000)
+    assert info[0] and info[2] == 3 * (2**21 - 2) + 1
+    assert_row_sets_equal(got, run_oracle([Col(LONG, k3)], aggs, [v2, None], 2000))
