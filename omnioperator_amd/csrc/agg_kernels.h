@@ -12,18 +12,34 @@ constexpr int kWavesPerBlock = kBlock / kWaveSize;
 constexpr int kCursorSlots = 16384;       // wave slots of the persistent insert launches (id chunks live there)
 constexpr int kMaxInsertBlocks = kCursorSlots / kWavesPerBlock;
 
-// Rows are visited grid-stride by a fixed grid, so wave slot w keeps its id chunk across launches.
 // Min / max of the non-NULL values of a single key column (the first batch the direct-index map is planned
-// from): one atomic pair per wave into mm = {min, max} (pre-set to {INT64_MAX, INT64_MIN}).
+// from). Each workgroup stores its pair {min, max} to mm[blockIdx.x] ({INT64_MAX, INT64_MIN} when it saw no
+// key); the caller reads the gridDim.x pairs back and reduces them. No atomics: one contended pair per wave
+// costs far more than the read of the keys. Each thread keeps four loads in flight per iteration.
+constexpr int kMinMaxUnroll = 4;
+constexpr int kMinMaxMaxBlocks = 1024;
 __global__ void __launch_bounds__(kBlock) agg_key_minmax(KeyDev kd, int64_t rows, long long *mm)
 {
+    __shared__ long long red[kWavesPerBlock][2];
     long long lo = 0x7FFFFFFFFFFFFFFFLL, hi = -0x7FFFFFFFFFFFFFFFLL - 1;
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
-    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < rows; row += stride) {
-        if (NullBit(kd.nulls[0], row)) continue;
-        const long long k = static_cast<long long>(LoadWord(kd.values[0], kd.width[0], row));
-        lo = k < lo ? k : lo;
-        hi = k > hi ? k : hi;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock * kMinMaxUnroll;
+    for (int64_t base = static_cast<int64_t>(blockIdx.x) * kBlock * kMinMaxUnroll + threadIdx.x; base < rows;
+         base += stride) {
+        long long k[kMinMaxUnroll];
+        bool ok[kMinMaxUnroll];
+#pragma unroll
+        for (int j = 0; j < kMinMaxUnroll; ++j) {
+            const int64_t row = base + static_cast<int64_t>(j) * kBlock;
+            ok[j] = row < rows && !NullBit(kd.nulls[0], row);
+            k[j] = ok[j] ? static_cast<long long>(LoadWord(kd.values[0], kd.width[0], row)) : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < kMinMaxUnroll; ++j) {
+            if (ok[j]) {
+                lo = k[j] < lo ? k[j] : lo;
+                hi = k[j] > hi ? k[j] : hi;
+            }
+        }
     }
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -31,12 +47,22 @@ __global__ void __launch_bounds__(kBlock) agg_key_minmax(KeyDev kd, int64_t rows
         lo = l2 < lo ? l2 : lo;
         hi = h2 > hi ? h2 : hi;
     }
-    if (LaneId() == 0 && lo <= hi) {
-        atomicMin(&mm[0], lo);
-        atomicMax(&mm[1], hi);
+    if (LaneId() == 0) {
+        red[WaveInBlock()][0] = lo;
+        red[WaveInBlock()][1] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < kWavesPerBlock; ++w) {
+            lo = red[w][0] < lo ? red[w][0] : lo;
+            hi = red[w][1] > hi ? red[w][1] : hi;
+        }
+        mm[2 * blockIdx.x] = lo;
+        mm[2 * blockIdx.x + 1] = hi;
     }
 }
 
+// Rows are visited grid-stride by a fixed grid, so wave slot w keeps its id chunk across launches.
 __global__ void __launch_bounds__(kBlock) agg_insert_single(KeyDev kd, int64_t rows, Slot1 *slots,
                                                             unsigned long long mask, GroupCtx g, AggArgs args,
                                                             unsigned long long *cursors, ArrayMapDev am)
