@@ -21,6 +21,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "join_device.h"
@@ -35,6 +36,8 @@ struct JoinHandle {
     int64_t n = 0;             // build rows
     std::vector<BuildBatch> pending;  // retained batches, gathered at finish
     DevBuf keys, knull;
+    DevBuf mm_parts;           // per-workgroup key min / max pairs of the build gather
+    bool key_nulls = false;    // some build batch has a NULL bitmap on its key column
     DevBuf pay[OMNI_GPU_MAX_AGGS], pay_null[OMNI_GPU_MAX_AGGS];
     bool pay_nullable[OMNI_GPU_MAX_AGGS] = {};
     bool finished = false;
@@ -42,6 +45,11 @@ struct JoinHandle {
     bool inline_pay = false;
     bool has_dup = false;
     int64_t max_dup = 0;
+    // The duplicate-key flag of a finished build is read back without a stream synchronisation: copied into
+    // flag_cell (pinned) before flags_ev; has_dup / inline_pay / max_dup are valid once flags_pending is false.
+    bool flags_pending = false;
+    unsigned int *flag_cell = nullptr;
+    hipEvent_t flags_ev = nullptr;
     int probe_group = 1;       // probe width G of join_probe
     int partitions = -1;       // radix-partitioned probe: -1 auto, 0 off, else the partition count
     int last_partitions = 0;   // partition count the last probe used (1 = direct probe)
@@ -59,6 +67,47 @@ struct JoinHandle {
     DevBuf slots, heads, next, scratch;
     DevBuf batch_desc, build_desc;
 };
+
+namespace {
+// Host-visible words for the duplicate-key flag: pinned pages cut into 64-byte cells, shared by every join
+// handle of the process (pages are kept until the process ends).
+std::mutex g_cell_mu;
+std::vector<unsigned int *> g_free_cells;
+
+int TakeFlagCell(unsigned int **cell)
+{
+    std::lock_guard<std::mutex> g(g_cell_mu);
+    if (g_free_cells.empty()) {
+        void *page = nullptr;
+        OG_HIP(hipHostMalloc(&page, 4096, hipHostMallocDefault));
+        for (int i = 0; i < 4096 / 64; ++i) {
+            g_free_cells.push_back(reinterpret_cast<unsigned int *>(static_cast<uint8_t *>(page) + 64 * i));
+        }
+    }
+    *cell = g_free_cells.back();
+    g_free_cells.pop_back();
+    return OMNI_GPU_OK;
+}
+
+void ReturnFlagCell(unsigned int *cell)
+{
+    std::lock_guard<std::mutex> g(g_cell_mu);
+    g_free_cells.push_back(cell);
+}
+}  // namespace
+
+// Waits for the finished build's duplicate-key flag (only for the work queued before it, not the whole
+// stream) and sets the facts that pick probe kernel variants.
+int ResolveBuildFlags(JoinHandle &h)
+{
+    if (!h.flags_pending) return OMNI_GPU_OK;
+    OG_HIP(hipEventSynchronize(h.flags_ev));
+    h.flags_pending = false;
+    h.has_dup = *h.flag_cell != 0;
+    if (h.has_dup) h.inline_pay = false;
+    h.max_dup = h.has_dup ? -1 : 1;  // longest chain: measured on first request
+    return OMNI_GPU_OK;
+}
 
 // Partition count of the radix-partitioned probe, 1 for the direct probe.  Automatic mode partitions tables
 // above kAutoPartBytes when the probe side has at least half as many rows as the table has slots (so the
@@ -150,6 +199,7 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
     hipStream_t s = h.stream;
     PartArgs pa;
     OG_TRY(Partition(h, a, t, P, nwg, rows, pa, "join_probe_hist", "join_probe_scan", "join_probe_scatter"));
+    OG_TRY(ResolveBuildFlags(h));  // the probe kernel variant; the partitioning above did not need it
     int lgP = 0;
     while ((1 << lgP) < P) ++lgP;
     int cus = 0;
@@ -207,7 +257,7 @@ int BuildPartitioned(JoinHandle &h, int P, unsigned int *has_dup)
     OG_TRY(h.ovf_keys.Reserve(sizeof(long long) * h.n, s));
     OG_TRY(h.ovf_rows.Reserve(sizeof(unsigned int) * h.n, s));
     BuildArgs b{};
-    b.knull = h.knull.as<uint8_t>();
+    b.knull = h.key_nulls ? h.knull.as<uint8_t>() : nullptr;  // no NULL bitmap in any batch: nothing to skip
     b.next = h.next.as<int>();
     b.sentinel_head = reinterpret_cast<int *>(sc);
     b.sentinel_pay = reinterpret_cast<unsigned int *>(sc + 12);
@@ -322,6 +372,7 @@ int RunProbe(JoinHandle &h, const ProbeBatch *batches, int nb, int64_t total_blo
         if ((h.slots_n >> lgP) > static_cast<uint64_t>(kWinSlots)) P = 1;
     }
     h.last_partitions = P;
+    if (P <= 1) OG_TRY(ResolveBuildFlags(h));  // LaunchPartitioned resolves after queueing the partitioning
     if (typed && P > 1) {
         TypedArgs ty{h.mode, 0, h.preserve_build ? h.visited.as<uint8_t>() : nullptr};
         OG_TRY(LaunchPartitioned(h, a, t, P, part_blocks, total_rows, &ty));
@@ -488,7 +539,9 @@ int omni_gpu_join_build_finish(void *handle)
         }
     }
     int64_t blocks = 0;
+    h.key_nulls = false;
     for (BuildBatch &bb : h.pending) {
+        h.key_nulls |= bb.knull != nullptr;
         bb.block_start = blocks;
         blocks += (bb.rows + kGatherTile - 1) / kGatherTile;
     }
@@ -497,11 +550,18 @@ int omni_gpu_join_build_finish(void *handle)
     OG_TRY(CopyToDevice(h.build_desc.p, h.pending.data(), sizeof(BuildBatch) * h.pending.size(), s));
     g.batches = h.build_desc.as<BuildBatch>();
     long long *dmm = reinterpret_cast<long long *>(sc + 16);
-    g.mm = h.spec.strategy == 0 ? dmm : nullptr;
+    if (h.spec.strategy == 0) {
+        OG_TRY(h.mm_parts.Reserve(2 * sizeof(long long) * static_cast<size_t>(blocks), s));
+        g.mm = h.mm_parts.as<long long>();
+    }
     {
         KernelScope ks(s, "join_gather_build");
         join_gather_build<<<static_cast<unsigned int>(blocks), 256, 0, s>>>(g);
         OG_TRY(CheckLaunch("join_gather_build"));
+        if (g.mm) {
+            join_minmax_reduce<<<1, 1024, 0, s>>>(g.mm, blocks, dmm);
+            OG_TRY(CheckLaunch("join_minmax_reduce"));
+        }
     }
     OG_TRY(h.next.Reserve(sizeof(int) * h.n, s));
     // initDegree and the array-table rule of JoinHashTableVariants::BuildHashTable / TryToBuildArrayTable.
@@ -562,12 +622,13 @@ int omni_gpu_join_build_finish(void *handle)
             OG_TRY(CheckLaunch("join_build_hash"));
         }
     }
-    // the probe kernel variant depends on whether any key repeats
-    unsigned int dup = 0;
-    OG_TRY(CopyToHost(s, {{&dup, has_dup, 4}}));
-    h.has_dup = dup != 0;
-    if (h.has_dup) h.inline_pay = false;
-    h.max_dup = h.has_dup ? -1 : 1;  // longest chain: measured on first request
+    // the probe kernel variant depends on whether any key repeats: read back without a stream
+    // synchronisation, so the build kernels overlap the host's set-up of the probe (ResolveBuildFlags)
+    if (!h.flag_cell) OG_TRY(TakeFlagCell(&h.flag_cell));
+    if (!h.flags_ev) OG_HIP(hipEventCreateWithFlags(&h.flags_ev, hipEventDisableTiming));
+    OG_HIP(hipMemcpyAsync(h.flag_cell, has_dup, 4, hipMemcpyDeviceToHost, s));
+    OG_HIP(hipEventRecord(h.flags_ev, s));
+    h.flags_pending = true;
     return OMNI_GPU_OK;
 }
 
@@ -607,6 +668,7 @@ int omni_gpu_join_build_info(void *handle, int32_t *is_array, int64_t *max_dup, 
     OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "build side not finished");
+    OG_TRY(ResolveBuildFlags(h));
     if (max_dup && h.max_dup < 0) OG_TRY(MaxChain(h));
     if (is_array) *is_array = h.is_array ? 1 : 0;
     if (max_dup) *max_dup = h.max_dup;
@@ -707,6 +769,7 @@ int omni_gpu_join_probe_exists(void *handle, const omni_gpu_column *probe_key, i
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
     OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "probe before the build side finished");
     OG_CHECK(rows == 0 || out_exists, OMNI_GPU_ERR_INVALID, "null out_exists");
+    OG_TRY(ResolveBuildFlags(h));
     ProbeBatch b{};
     OG_TRY(FillBatch(b, *probe_key, rows, 0, 0, 0, h.spec.key_type));
     OG_CHECK(rows < (1LL << 31) * kBlock, OMNI_GPU_ERR_INVALID, "probe batch too large");
@@ -731,6 +794,7 @@ int omni_gpu_join_outer_output(void *handle, int64_t *out_build_rows, omni_gpu_o
     OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "outer output before the build side finished");
     OG_CHECK(h.preserve_build, OMNI_GPU_ERR_STATE, "outer output needs a build-preserving outer join type");
     OG_CHECK(out_capacity >= 0, OMNI_GPU_ERR_INVALID, "negative out_capacity");
+    OG_TRY(ResolveBuildFlags(h));
     hipStream_t s = h.stream;
     OuterArgs g{};
     g.visited = h.visited.as<uint8_t>();
@@ -761,8 +825,14 @@ int omni_gpu_join_destroy(void *handle)
     if (!handle) return OMNI_GPU_OK;
     JoinHandle *h = static_cast<JoinHandle *>(handle);
     hipStream_t s = h->stream;
+    if (h->flags_ev) {  // the flag copy into the cell must land before the cell is handed out again
+        (void)hipEventSynchronize(h->flags_ev);
+        (void)hipEventDestroy(h->flags_ev);
+    }
+    if (h->flag_cell) ReturnFlagCell(h->flag_cell);
     h->keys.Release(s);
     h->knull.Release(s);
+    h->mm_parts.Release(s);
     for (int p = 0; p < OMNI_GPU_MAX_AGGS; ++p) {
         h->pay[p].Release(s);
         h->pay_null[p].Release(s);

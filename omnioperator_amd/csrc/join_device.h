@@ -88,7 +88,7 @@ struct GatherArgs {
     uint8_t *knull;
     void *pay[OMNI_GPU_MAX_AGGS];
     uint8_t *pnull[OMNI_GPU_MAX_AGGS];  // nullptr: the column has no NULLs in any batch
-    long long *mm;              // min/max of the non-NULL keys, or nullptr
+    long long *mm;              // per-workgroup min/max pairs of the non-NULL keys, or nullptr
 };
 
 constexpr int kGatherRows = 8;                          // rows per thread
@@ -111,9 +111,23 @@ __device__ __forceinline__ void CopyValue(const void *src, void *dst, int width,
     }
 }
 
+// Stores the low `width` bytes of v as element o of a column of width 1/2/4/8 (the inverse of LoadWord).
+__device__ __forceinline__ void StoreWordRaw(void *dst, int width, int64_t o, unsigned long long v)
+{
+    switch (width) {
+        case 8: static_cast<unsigned long long *>(dst)[o] = v; break;
+        case 4: static_cast<unsigned int *>(dst)[o] = static_cast<unsigned int>(v); break;
+        case 2: static_cast<unsigned short *>(dst)[o] = static_cast<unsigned short>(v); break;
+        default: static_cast<uint8_t *>(dst)[o] = static_cast<uint8_t>(v); break;
+    }
+}
+
 // Concatenates every retained build batch into the build store (keys sign-extended to int64, NULL bytes,
-// payload columns) in one launch, and reduces the min/max of the non-NULL keys with one atomic pair per
-// workgroup.
+// payload columns) in one launch. Each thread loads all of its rows of a column before storing any of them
+// (the stores could alias the loads as far as the compiler knows, so a load-store loop would wait for one
+// round trip per row). With g.mm set, workgroup b stores the min / max of its non-NULL keys to
+// g.mm[2b], g.mm[2b + 1] ({INT64_MAX, INT64_MIN} when it has none), reduced by join_minmax_reduce: one
+// contended atomic pair per workgroup cost about half of this kernel.
 __global__ void __launch_bounds__(256) join_gather_build(GatherArgs g)
 {
     __shared__ long long slo[4], shi[4];
@@ -124,23 +138,56 @@ __global__ void __launch_bounds__(256) join_gather_build(GatherArgs g)
         else hi_i = mid - 1;
     }
     const BuildBatch &b = g.batches[lo_i];
-    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kGatherTile;
-    long long lo = std::numeric_limits<long long>::max(), hi = std::numeric_limits<long long>::min();
+    const int64_t tile0 = (static_cast<int64_t>(blockIdx.x) - b.block_start) * kGatherTile + threadIdx.x;
+    long long k[kGatherRows];
+    bool nul[kGatherRows];
+#pragma unroll
     for (int j = 0; j < kGatherRows; ++j) {
-        int64_t i = tile0 + j * 256 + threadIdx.x;
-        if (i >= b.rows) break;
-        int64_t o = b.row_base + i;
-        bool isNull = NullBit(b.knull, i);
-        long long k = isNull ? 0 : static_cast<long long>(LoadWord(b.key, g.key_width, i));
-        g.keys[o] = k;
-        g.knull[o] = isNull ? 1 : 0;
-        if (!isNull) {
-            lo = min(lo, k);
-            hi = max(hi, k);
+        const int64_t i = tile0 + j * 256;
+        const bool ok = i < b.rows;
+        nul[j] = ok && NullBit(b.knull, i);
+        k[j] = ok && !nul[j] ? static_cast<long long>(LoadWord(b.key, g.key_width, i)) : 0;
+    }
+    long long lo = std::numeric_limits<long long>::max(), hi = std::numeric_limits<long long>::min();
+#pragma unroll
+    for (int j = 0; j < kGatherRows; ++j) {
+        const int64_t i = tile0 + j * 256;
+        if (i < b.rows) {
+            const int64_t o = b.row_base + i;
+            g.keys[o] = k[j];
+            g.knull[o] = nul[j] ? 1 : 0;
+            if (!nul[j]) {
+                lo = min(lo, k[j]);
+                hi = max(hi, k[j]);
+            }
         }
-        for (int p = 0; p < g.npay; ++p) {
-            CopyValue(b.pay[p], g.pay[p], g.pay_width[p], i, o);
-            if (g.pnull[p]) g.pnull[p][o] = NullBit(b.pnull[p], i) ? 1 : 0;
+    }
+    for (int p = 0; p < g.npay; ++p) {
+        const int w = g.pay_width[p];
+        if (w <= 8) {
+            unsigned long long v[kGatherRows];
+#pragma unroll
+            for (int j = 0; j < kGatherRows; ++j) {
+                const int64_t i = tile0 + j * 256;
+                v[j] = i < b.rows ? LoadWord(b.pay[p], w, i) : 0ULL;
+            }
+#pragma unroll
+            for (int j = 0; j < kGatherRows; ++j) {
+                const int64_t i = tile0 + j * 256;
+                if (i < b.rows) StoreWordRaw(g.pay[p], w, b.row_base + i, v[j]);
+            }
+        } else {
+            for (int j = 0; j < kGatherRows; ++j) {
+                const int64_t i = tile0 + j * 256;
+                if (i < b.rows) CopyValue(b.pay[p], g.pay[p], w, i, b.row_base + i);
+            }
+        }
+        if (g.pnull[p]) {
+#pragma unroll
+            for (int j = 0; j < kGatherRows; ++j) {
+                const int64_t i = tile0 + j * 256;
+                if (i < b.rows) g.pnull[p][b.row_base + i] = NullBit(b.pnull[p], i) ? 1 : 0;
+            }
         }
     }
     if (g.mm == nullptr) return;  // uniform across the launch
@@ -159,10 +206,37 @@ __global__ void __launch_bounds__(256) join_gather_build(GatherArgs g)
             lo = min(lo, slo[w]);
             hi = max(hi, shi[w]);
         }
-        if (lo <= hi) {
-            atomicMin(&g.mm[0], lo);
-            atomicMax(&g.mm[1], hi);
+        g.mm[2 * blockIdx.x] = lo;
+        g.mm[2 * blockIdx.x + 1] = hi;
+    }
+}
+
+// One workgroup of 1024 lanes: mm = {min, max} over the n pairs of parts.
+__global__ void __launch_bounds__(1024) join_minmax_reduce(const long long *parts, int64_t n, long long *mm)
+{
+    __shared__ long long slo[16], shi[16];
+    long long lo = std::numeric_limits<long long>::max(), hi = std::numeric_limits<long long>::min();
+    for (int64_t i = threadIdx.x; i < n; i += 1024) {
+        lo = min(lo, parts[2 * i]);
+        hi = max(hi, parts[2 * i + 1]);
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    if (LaneId() == 0) {
+        slo[threadIdx.x >> 6] = lo;
+        shi[threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 16; ++w) {
+            lo = min(lo, slo[w]);
+            hi = max(hi, shi[w]);
         }
+        mm[0] = lo;
+        mm[1] = hi;
     }
 }
 
@@ -1016,7 +1090,7 @@ __global__ void __launch_bounds__(kBlock) join_part_probe(ProbeArgs a, TableDev 
 // is written by its region's workgroup, so the table needs no separate initialisation.
 
 struct BuildArgs {
-    const uint8_t *knull;           // build NULL flags (NULL keys are not inserted)
+    const uint8_t *knull;           // build NULL flags (NULL keys are not inserted), nullptr when none is NULL
     int *next;                      // build row chains
     int *sentinel_head;             // chain of rows whose key is the slot sentinel
     unsigned int *sentinel_pay;
@@ -1046,11 +1120,14 @@ __global__ void __launch_bounds__(kWinBlock) join_window_build(TableDev t, PartA
     for (unsigned int e = lo + threadIdx.x; e < hi; e += kWinBlock) {
         const long long key = pa.pkeys[e];
         const unsigned int row = pa.prows[e];
-        if (ba.knull[row]) continue;
+        // both gathers by row id are issued before either is used, so a row waits for one round trip
+        const bool isNull = ba.knull != nullptr && ba.knull[row];
+        const unsigned int pv = ba.inline_pay ? PayloadWord(ba.inline_pay, ba.inline_width, row) : 0u;
+        if (isNull) continue;
         int old;
         if (static_cast<unsigned long long>(key) == kEmptyKey) {
             old = atomicExch(ba.sentinel_head, static_cast<int>(row));
-            if (ba.inline_pay) *ba.sentinel_pay = PayloadWord(ba.inline_pay, ba.inline_width, row);
+            if (ba.inline_pay) *ba.sentinel_pay = pv;
         } else {
             unsigned int pos = static_cast<unsigned int>((JoinHome(key, t.mask) - wbase) & t.mask);
             while (pos < R) {
@@ -1070,7 +1147,7 @@ __global__ void __launch_bounds__(kWinBlock) join_window_build(TableDev t, PartA
                 continue;
             }
             old = atomicExch(&win[pos].head, static_cast<int>(row));
-            if (ba.inline_pay) win[pos].pay = PayloadWord(ba.inline_pay, ba.inline_width, row);
+            if (ba.inline_pay) win[pos].pay = pv;
         }
         ba.next[row] = old;
         if (old >= 0) *ba.has_dup = 1u;
