@@ -238,10 +238,10 @@ int LaunchPartitioned(JoinHandle &h, const ProbeArgs &a, const TableDev &t, int 
 // Partitioned build of the hash table (P regions of kWinSlots slots): the build store's rows are partitioned
 // like probe rows, each region is built in LDS by one workgroup and written out whole, then the keys whose
 // probe run left their region are inserted into the global table.
-int BuildPartitioned(JoinHandle &h, int P, unsigned int *has_dup)
+// Partitions the build store's rows into the P regions of a table of h.slots_n slots (only the table's size is
+// needed, not its memory).
+int PartitionBuildRows(JoinHandle &h, int P, PartArgs &pa)
 {
-    hipStream_t s = h.stream;
-    uint8_t *sc = h.scratch.as<uint8_t>();
     ProbeArgs a{};
     a.nb = 1;
     a.single.keys = h.keys.p;
@@ -249,11 +249,22 @@ int BuildPartitioned(JoinHandle &h, int P, unsigned int *has_dup)
     a.single.rows = h.n;
     a.single.width = 8;
     TableDev t{};
-    t.slots = h.slots.as<JoinSlot>();
     t.mask = h.slots_n - 1;
     const int64_t nwg = (h.n + kPartTile - 1) / kPartTile;
+    return Partition(h, a, t, P, nwg, h.n, pa, "join_build_hist", "join_build_scan", "join_build_scatter");
+}
+
+// `pre`: the rows were already partitioned by PartitionBuildRows (nullptr: partition them here).
+int BuildPartitioned(JoinHandle &h, int P, unsigned int *has_dup, const PartArgs *pre)
+{
+    hipStream_t s = h.stream;
+    uint8_t *sc = h.scratch.as<uint8_t>();
+    TableDev t{};
+    t.slots = h.slots.as<JoinSlot>();
+    t.mask = h.slots_n - 1;
     PartArgs pa;
-    OG_TRY(Partition(h, a, t, P, nwg, h.n, pa, "join_build_hist", "join_build_scan", "join_build_scatter"));
+    if (pre) pa = *pre;
+    else OG_TRY(PartitionBuildRows(h, P, pa));
     OG_TRY(h.ovf_keys.Reserve(sizeof(long long) * h.n, s));
     OG_TRY(h.ovf_rows.Reserve(sizeof(unsigned int) * h.n, s));
     BuildArgs b{};
@@ -563,12 +574,39 @@ int omni_gpu_join_build_finish(void *handle)
             OG_TRY(CheckLaunch("join_minmax_reduce"));
         }
     }
+    // The handle's pinned cell: [0] the duplicate-key flag, [16] the key min / max. Each read-back is an async
+    // copy before flags_ev, so a wait covers the work queued before it and not the whole stream.
+    if (!h.flag_cell) OG_TRY(TakeFlagCell(&h.flag_cell));
+    if (!h.flags_ev) OG_HIP(hipEventCreateWithFlags(&h.flags_ev, hipEventDisableTiming));
+    const long long *cell_mm = reinterpret_cast<const long long *>(reinterpret_cast<uint8_t *>(h.flag_cell) + 16);
+    if (g.mm) {
+        OG_HIP(hipMemcpyAsync(const_cast<long long *>(cell_mm), dmm, 2 * sizeof(long long), hipMemcpyDeviceToHost,
+                              s));
+        OG_HIP(hipEventRecord(h.flags_ev, s));
+    }
     OG_TRY(h.next.Reserve(sizeof(int) * h.n, s));
+    // The hash table's size depends on n only (load <= 1/4: linear-probing runs stay short, and the window
+    // probe's LDS rounds and the direct probe's line crossings both scale with the longest runs a wave meets).
+    // Its partitioned build starts by partitioning the build rows: queued here, ahead of the min/max read-back
+    // below, so the GPU works while the host waits for the array-table decision (an array table leaves the
+    // partitioning unused).
+    uint64_t hashSlots = 1024;
+    while (hashSlots < static_cast<uint64_t>(4 * h.n)) hashSlots <<= 1;
+    const uint64_t bP = hashSlots / kWinSlots;
+    const bool partBuild = bP >= static_cast<uint64_t>(kXcds) && bP <= static_cast<uint64_t>(kPartMax) &&
+                           h.n < (1LL << 32) - 1;
+    PartArgs bpa;
+    bool prePartitioned = false;
+    if (partBuild && h.spec.strategy == 0) {
+        h.slots_n = hashSlots;
+        OG_TRY(PartitionBuildRows(h, static_cast<int>(bP), bpa));
+        prePartitioned = true;
+    }
     // initDegree and the array-table rule of JoinHashTableVariants::BuildHashTable / TryToBuildArrayTable.
     bool arrayOk = false;
     if (h.spec.strategy == 0) {
-        long long mm[2];
-        OG_TRY(CopyToHost(s, {{mm, dmm, sizeof(mm)}}));
+        OG_HIP(hipEventSynchronize(h.flags_ev));
+        const long long mm[2] = {cell_mm[0], cell_mm[1]};
         if (mm[1] >= mm[0]) {
             double degree = std::ceil(std::log2(static_cast<double>(h.n) / 0.75));
             long double len = std::pow(2.0L, static_cast<long double>(degree));
@@ -584,6 +622,7 @@ int omni_gpu_join_build_finish(void *handle)
     unsigned int *has_dup = reinterpret_cast<unsigned int *>(sc + 4);
     if (arrayOk) {
         h.is_array = true;
+        h.slots_n = 0;
         OG_TRY(h.heads.Reserve(sizeof(int) * h.range, s));
         {
             KernelScope ks(s, "join_init_table");
@@ -596,15 +635,11 @@ int omni_gpu_join_build_finish(void *handle)
                                                         h.heads.as<int>(), h.min_key, h.next.as<int>(), has_dup);
         OG_TRY(CheckLaunch("join_build_array"));
     } else {
-        // load <= 1/4: linear-probing runs stay short (the window probe's LDS rounds and the direct probe's
-        // line crossings both scale with the longest runs a wave meets)
-        h.slots_n = 1024;
-        while (h.slots_n < static_cast<uint64_t>(4 * h.n)) h.slots_n <<= 1;
+        h.slots_n = hashSlots;
         OG_TRY(h.slots.Reserve(sizeof(JoinSlot) * h.slots_n, s));
         h.inline_pay = h.spec.num_payloads == 1 && TypeWidth(h.spec.payload_types[0]) <= 4 && !h.pay_nullable[0];
-        const uint64_t bP = h.slots_n / kWinSlots;
-        if (bP >= static_cast<uint64_t>(kXcds) && bP <= static_cast<uint64_t>(kPartMax) && h.n < (1LL << 32) - 1) {
-            OG_TRY(BuildPartitioned(h, static_cast<int>(bP), has_dup));
+        if (partBuild) {
+            OG_TRY(BuildPartitioned(h, static_cast<int>(bP), has_dup, prePartitioned ? &bpa : nullptr));
         } else {
             {
                 KernelScope ks(s, "join_init_table");
@@ -624,8 +659,6 @@ int omni_gpu_join_build_finish(void *handle)
     }
     // the probe kernel variant depends on whether any key repeats: read back without a stream
     // synchronisation, so the build kernels overlap the host's set-up of the probe (ResolveBuildFlags)
-    if (!h.flag_cell) OG_TRY(TakeFlagCell(&h.flag_cell));
-    if (!h.flags_ev) OG_HIP(hipEventCreateWithFlags(&h.flags_ev, hipEventDisableTiming));
     OG_HIP(hipMemcpyAsync(h.flag_cell, has_dup, 4, hipMemcpyDeviceToHost, s));
     OG_HIP(hipEventRecord(h.flags_ev, s));
     h.flags_pending = true;
