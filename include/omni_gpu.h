@@ -243,7 +243,9 @@ int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle);
  * NULL keys are kept but never match. */
 int omni_gpu_join_build_add(void *handle, const omni_gpu_column *key, const omni_gpu_column *payloads,
                             int64_t rows);
-/* Builds the table (HashBuilderOperator::GetOutput).  Synchronizes. */
+/* Builds the table (HashBuilderOperator::GetOutput).  Returns once every build batch has been copied into
+ * the build storage, so the caller's build buffers may be released; the table build itself can still be
+ * running on the handle's stream, and later calls on the handle are ordered after it. */
 int omni_gpu_join_build_finish(void *handle);
 /* Table facts after finish: *is_array (1 = direct-index array table), *max_dup (largest number of
  * build rows sharing one key), *table_slots. */

@@ -582,8 +582,8 @@ int omni_gpu_join_build_finish(void *handle)
     if (g.mm) {
         OG_HIP(hipMemcpyAsync(const_cast<long long *>(cell_mm), dmm, 2 * sizeof(long long), hipMemcpyDeviceToHost,
                               s));
-        OG_HIP(hipEventRecord(h.flags_ev, s));
     }
+    OG_HIP(hipEventRecord(h.flags_ev, s));  // after the gather: the caller's build buffers are free after it
     OG_TRY(h.next.Reserve(sizeof(int) * h.n, s));
     // The hash table's size depends on n only (load <= 1/4: linear-probing runs stay short, and the window
     // probe's LDS rounds and the direct probe's line crossings both scale with the longest runs a wave meets).
@@ -659,6 +659,9 @@ int omni_gpu_join_build_finish(void *handle)
     }
     // the probe kernel variant depends on whether any key repeats: read back without a stream
     // synchronisation, so the build kernels overlap the host's set-up of the probe (ResolveBuildFlags)
+    // the gather must have read the caller's build buffers before finish returns (the array-table decision
+    // above already waited for it)
+    if (!g.mm) OG_HIP(hipEventSynchronize(h.flags_ev));
     OG_HIP(hipMemcpyAsync(h.flag_cell, has_dup, 4, hipMemcpyDeviceToHost, s));
     OG_HIP(hipEventRecord(h.flags_ev, s));
     h.flags_pending = true;
