@@ -115,6 +115,11 @@ int omni_gpu_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int omni_gpu_memset(void *dst, int value, size_t bytes);
 int omni_gpu_device_synchronize(void);
 
+/* Device memory the library holds: *in_use_bytes in live buffers of open handles, *cached_bytes in freed blocks it
+ * keeps for reuse (returned to the runtime when an allocation would otherwise fail).  No reference counterpart; the
+ * reference reports operator memory to its ThreadMemoryManager (core/src/memory/thread_memory_manager.h:45-58). */
+int omni_gpu_memory_stats(int64_t *in_use_bytes, int64_t *cached_bytes);
+
 /* Kernel timing: when enabled, every kernel the library launches is bracketed by HIP events on
  * the stream it runs on; omni_gpu_profile_get reports the launch count and summed duration (ms)
  * of the kernels whose name starts with `name_prefix`. */

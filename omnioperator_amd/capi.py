@@ -82,6 +82,7 @@ SIGNATURES = {
     "omni_gpu_memcpy_d2d": (_I, [_VP, _VP, _SZ]),
     "omni_gpu_memset": (_I, [_VP, _I, _SZ]),
     "omni_gpu_device_synchronize": (_I, []),
+    "omni_gpu_memory_stats": (_I, [_P(_I64), _P(_I64)]),
     "omni_gpu_profile_enable": (_I, [_I]),
     "omni_gpu_profile_reset": (_I, []),
     "omni_gpu_profile_get": (_I, [C.c_char_p, _P(_I64), _P(_D)]),
@@ -668,6 +669,13 @@ def generate_keys(kind: int, rows: int, distinct: float, zipf_s: float, seed: in
                                        k32.ptr if k32 else None, v.ptr if v else None))
     keys = [k64] if t == 0 else ([k32] if t == 1 else [k32, k64])
     return keys, v
+
+
+def memory_stats():
+    """(bytes in live buffers of open handles, bytes of freed blocks kept for reuse)."""
+    used, cached = C.c_int64(), C.c_int64()
+    check(lib().omni_gpu_memory_stats(C.byref(used), C.byref(cached)))
+    return used.value, cached.value
 
 
 def profile_enable(on: bool = True):

@@ -58,8 +58,12 @@ struct AggHandle {
     DevBuf index, tile_counts;      // output compaction: live group ids in ascending order
     DevBuf stage_vals[kStageCols], stage_nulls[kStageCols], stage_src, stage_rows;  // omni_gpu_agg_add_batches
     DevBuf counters;  // [0] ids handed out u64, [8] overflow u32, [12] error u32, [16..24) special u32[2],
-                      // [24] spills u64, [32] live groups u32, [64] per-workgroup key min / max pairs
-                      // i64[2 * kMinMaxMaxBlocks] (ARRAY rule)
+                      // [24] spills u64, [32] live groups u32, [36] checked decimal groups crossed u32,
+                      // [64] per-workgroup key min / max pairs i64[2 * kMinMaxMaxBlocks] (ARRAY rule)
+    bool checked = false;           // some aggregate is a checked decimal SUM (K_SUM_DEC_CHK): global table only,
+                                    // each launch records its rows' group ids and is settled and replayed
+    DevBuf row_gid;                 // group id of each row of the current launch (checked decimal SUM)
+    int64_t rows_added = 0;         // input rows before the current launch (positions for double MIN/MAX ties)
     int64_t known_ids = 0;          // ids handed out at the last synchronisation
     int64_t pending_rows = 0;       // rows added since then
     int64_t live_groups = -1;       // live groups after the last compaction (-1: stale)
@@ -78,6 +82,7 @@ struct AggHandle {
     unsigned int *special() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 16); }
     unsigned long long *spills() const { return reinterpret_cast<unsigned long long *>(counters.as<uint8_t>() + 24); }
     unsigned int *live() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 32); }
+    unsigned int *crossed() const { return reinterpret_cast<unsigned int *>(counters.as<uint8_t>() + 36); }
 
     // Ids a wave takes at a time: about the groups expected (seen so far, or the capacity hint if larger) over
     // the wave slots, a power of two in [1, 512], so that small group counts leave no unused reserved ids and
@@ -274,7 +279,10 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
         OG_CHECK(kd.W <= kMaxWords, OMNI_GPU_ERR_INVALID, "composite key too wide (%d words)", kd.W);
     }
     h->args.n = spec->num_aggs;
-    for (int i = 0; i < spec->num_aggs; ++i) OG_TRY(ClassifyAgg(*spec, i, h->args.a[i]));
+    for (int i = 0; i < spec->num_aggs; ++i) {
+        OG_TRY(ClassifyAgg(*spec, i, h->args.a[i]));
+        h->checked |= h->args.a[i].kind == K_SUM_DEC_CHK;
+    }
     AssignRowLayout(h->args);
     OG_TRY(AcquireStream(&h->stream));
     // 64 bytes of counters, then the per-workgroup key min / max pairs the ARRAY rule plans its map from
@@ -299,6 +307,9 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
     } else if (spec->strategy == OMNI_GPU_STRATEGY_AUTO) {
         h->use_lds = ldsFits && (spec->capacity_hint <= 0 || spec->capacity_hint * 2 <= h->lds.S);
     }
+    // A checked decimal SUM is replayed in row order from each row's group id, which the global kernels record;
+    // the strategy only moves time, so such a handle takes the global table whatever was asked.
+    if (h->checked) h->use_lds = false;
     *handle = h.release();
     return OMNI_GPU_OK;
 }
@@ -306,7 +317,8 @@ int omni_gpu_agg_create(const omni_gpu_agg_spec *spec, void **handle)
 // The ARRAY hash-table rule (AggHashTableRule::ARRAY) for one integer key: VectorAnalyzer::CheckArrayMap
 // (core/src/operator/hashmap/vector_analyzer.cpp:10-80) over the first inserted rows -- no array when every key is
 // NULL, when max - min could overflow or exceeds 2^21 - 2; otherwise the range is widened by max(50, max - min)
-// on each side, saturating at the type's limits.  The reference resizes once and then moves to its hash map
+// on each side, saturating at the type's limits, and declined when the widened range and its NULL slot exceed
+// 2^21 entries.  The reference resizes once and then moves to its hash map
 // when later keys leave the range; here such keys go to the hash table directly (results are the same).
 int PlanArrayMap(AggHandle &h, int64_t rows)
 {
@@ -350,6 +362,9 @@ int PlanArrayMap(AggHandle &h, int64_t rows)
     const long long hi = mx <= tmax - reserve ? mx + reserve : tmax;
     const long long lo = mn >= tmin + reserve ? mn - reserve : tmin;
     const uint64_t cells = static_cast<uint64_t>(hi) - static_cast<uint64_t>(lo) + 1;
+    // the widened range plus the NULL slot must stay within RANGE_UPPER_BOUND = 2^21 entries (GetRange() check,
+    // vector_analyzer.cpp:74-77)
+    if (cells + 1 > (2ULL << 20)) return OMNI_GPU_OK;
     OG_TRY(h.amap.Reserve(sizeof(unsigned int) * cells, s));
     OG_TRY(Fill32(s, h.amap.p, 0u, static_cast<int64_t>(cells)));
     h.am.cells = h.amap.as<unsigned int>();
@@ -401,6 +416,8 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         }
     }
     if (!h.map_planned) OG_TRY(PlanArrayMap(h, rows));
+    h.args.row_base = h.rows_added;
+    h.rows_added += rows;
     GroupCtx g = h.Ctx();
     unsigned long long *cur = h.cursors.as<unsigned long long>();
     if (h.use_lds) {
@@ -428,17 +445,31 @@ int omni_gpu_agg_add_batch(void *handle, const omni_gpu_column *keys, const omni
         h.pending_rows += rows;
         return OMNI_GPU_OK;
     }
+    unsigned int *rowGid = nullptr;
+    if (h.checked) {
+        OG_TRY(h.row_gid.Reserve(sizeof(unsigned int) * static_cast<size_t>(rows), h.stream));
+        rowGid = h.row_gid.as<unsigned int>();
+    }
+    const unsigned int grid = static_cast<unsigned int>(std::min<int64_t>(Blocks(rows), kMaxInsertBlocks));
     {
-        const unsigned int grid = static_cast<unsigned int>(std::min<int64_t>(Blocks(rows), kMaxInsertBlocks));
         KernelScope ks(h.stream, h.single ? "agg_insert_single" : "agg_insert_multi");
         if (h.single) {
             agg_insert_single<<<grid, kBlock, 0, h.stream>>>(h.kd, rows, h.slots.as<Slot1>(), h.cap - 1, g, h.args, cur,
-                                                             h.am);
+                                                             h.am, rowGid);
         } else {
             agg_insert_multi<<<grid, kBlock, 0, h.stream>>>(h.kd, rows, h.ctrl.as<unsigned int>(),
-                                                            h.skeys.as<unsigned long long>(), h.cap - 1, g, h.args, cur);
+                                                            h.skeys.as<unsigned long long>(), h.cap - 1, g, h.args, cur,
+                                                            rowGid);
         }
         OG_TRY(CheckLaunch("agg_insert"));
+    }
+    if (h.checked) {
+        // checked decimal SUM: mark the groups whose sums passed their limits, then replay their rows in order
+        // (the replay launch returns at once when no group was marked)
+        KernelScope ks(h.stream, "agg_dec_check");
+        agg_dec_settle<<<grid, kBlock, 0, h.stream>>>(rows, rowGid, h.args, h.crossed());
+        agg_dec_replay<<<1, kReplayBlock, 0, h.stream>>>(rows, rowGid, h.args, h.crossed());
+        OG_TRY(CheckLaunch("agg_dec_check"));
     }
     h.pending_rows += rows;
     return OMNI_GPU_OK;
@@ -611,7 +642,7 @@ int omni_gpu_agg_destroy(void *handle)
     hipStream_t s = h->stream;
     for (DevBuf *b : {&h->slots, &h->ctrl, &h->skeys, &h->gkeys, &h->gnull, &h->gvalid, &h->states, &h->row_tmpl,
                       &h->cursors, &h->index, &h->tile_counts, &h->counters, &h->stage_src, &h->stage_rows,
-                      &h->amap}) {
+                      &h->amap, &h->row_gid}) {
         b->Release(s);
     }
     for (int c = 0; c < kStageCols; ++c) {

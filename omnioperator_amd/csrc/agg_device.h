@@ -20,7 +20,21 @@ enum AggKind : int {
     K_MAX_I32,
     K_MIN_F64,
     K_MAX_F64,
+    K_SUM_DEC_CHK,  // decimal SUM whose running sum can leave its limits: DECIMAL64 -> DECIMAL64, DECIMAL128 input
 };
+
+// State of a checked decimal SUM (K_SUM_DEC_CHK).  The reference adds in row order with a checked add and the
+// first add that leaves the limits marks the group OVERFLOWED for good (AddDecimalUseRowIndex / SumOp,
+// sum_spark_decimal_aggregator.h:25-62, sum_aggregator.h:12-54), so the result depends on the order of the rows,
+// not only on their total.  The GPU keeps, per group, the exact 192-bit sums of the positive inputs (P, words
+// 0-2) and of the magnitudes of the negative inputs (N, words 3-5), plus a u32 word of flags:
+//   - while P <= U and N <= L (U, L: the limits' magnitudes), every partial sum in ANY order lies in [-N, P], so
+//     no add of the group overflowed and its value is exactly P - N;
+//   - after each insert launch a group whose P or N passed its limit is CROSSED: the launch's rows of that group
+//     are replayed in row order from the value before the launch (kDecOvf when an add leaves the limits), and
+//     the value is stored back as P = max(v, 0), N = max(-v, 0).
+constexpr unsigned int kDecOvf = 1u;      // a row-order add left the limits (sticky)
+constexpr unsigned int kDecCrossed = 2u;  // P or N passed its limit during the last launch (being replayed)
 
 struct AggDev {
     int kind;
@@ -30,8 +44,9 @@ struct AggDev {
     int out_width;    // bytes per output value
     const void *values;
     const uint8_t *nulls;
-    int off[3];       // byte offsets of the state words s0, s1, s2 within a group's state row (-1 unused)
+    int off[6];       // byte offsets of the state words s0..s5 within a group's state row (-1 unused)
     int flag_off;     // byte offset of the "a non-NULL value was added" flag (-1 unused)
+    int cflag_off;    // byte offset of the u32 kDecOvf / kDecCrossed word of K_SUM_DEC_CHK (-1 unused)
     int dec_scale_up; // 10^(outScale - inScale) exponent for decimal SUM
     int dec_out128;   // decimal SUM output is DECIMAL128
 };
@@ -40,6 +55,7 @@ struct AggArgs {
     int n;
     int row_bytes;        // bytes of one group's state row (a multiple of 8)
     uint8_t *states;      // state rows, indexed by group id (array of structures: one row per group)
+    long long row_base;   // input position of row 0 of the launch (rows added before it): orders double MIN/MAX ties
     AggDev a[OMNI_GPU_MAX_AGGS];
 };
 
@@ -342,9 +358,70 @@ __device__ __forceinline__ void DecInput(const AggDev &d, int64_t row, unsigned 
     vtop = h < 0 ? -1 : 0;
 }
 
+// ---- 192-bit helpers of the checked decimal SUM ---------------------------------------------------------------
+struct U192 {
+    unsigned long long lo, hi;
+    long long top;
+};
+
+__device__ __forceinline__ U192 Neg192(U192 v)
+{
+    U192 r;
+    r.lo = ~v.lo + 1ULL;
+    unsigned long long c = r.lo == 0 ? 1ULL : 0ULL;
+    r.hi = ~v.hi + c;
+    c = (c && r.hi == 0) ? 1ULL : 0ULL;
+    r.top = static_cast<long long>(~static_cast<unsigned long long>(v.top) + c);
+    return r;
+}
+
+__device__ __forceinline__ U192 Add192(U192 a, U192 b)
+{
+    U192 r;
+    r.lo = a.lo + b.lo;
+    unsigned long long c = r.lo < a.lo ? 1ULL : 0ULL;
+    unsigned long long h = a.hi + b.hi;
+    unsigned long long c2 = h < a.hi ? 1ULL : 0ULL;
+    r.hi = h + c;
+    c2 += (r.hi < h) ? 1ULL : 0ULL;
+    r.top = static_cast<long long>(static_cast<unsigned long long>(a.top) + static_cast<unsigned long long>(b.top) + c2);
+    return r;
+}
+
+// a > b as signed 192-bit values
+__device__ __forceinline__ bool Gt192(U192 a, U192 b)
+{
+    if (a.top != b.top) return a.top > b.top;
+    if (a.hi != b.hi) return a.hi > b.hi;
+    return a.lo > b.lo;
+}
+
+// Magnitude limit of a checked decimal SUM: 10^38 - 1 for a DECIMAL128 sum (AddCheckedOverflow,
+// core/src/type/base_operations.h:20-28); INT64_MAX above zero and 2^63 below it for a DECIMAL64 sum
+// (__builtin_add_overflow on int64).
+__device__ __forceinline__ U192 DecUpper(int out128)
+{
+    return out128 ? U192{687399551400673279ULL, 5421010862427522170ULL, 0} : U192{0x7FFFFFFFFFFFFFFFULL, 0ULL, 0};
+}
+__device__ __forceinline__ U192 DecLowerMag(int out128)
+{
+    return out128 ? U192{687399551400673279ULL, 5421010862427522170ULL, 0} : U192{0x8000000000000000ULL, 0ULL, 0};
+}
+
+// Order-preserving image of a double as an unsigned word (IEEE order; -0.0 below +0.0); NaN is never stored.
+__host__ __device__ __forceinline__ unsigned long long OrderedKey(unsigned long long bits)
+{
+    return (bits >> 63) ? ~bits : (bits | 0x8000000000000000ULL);
+}
+__device__ __forceinline__ unsigned long long FromOrderedKey(unsigned long long k)
+{
+    return (k >> 63) ? (k & 0x7FFFFFFFFFFFFFFFULL) : ~k;
+}
+
 // Per-group state addressing: GlobalRef indexes the HBM state arrays by group id, LdsRef the
 // workgroup's LDS table by slot.  The same update code runs against either.
 struct GlobalRef {
+    static constexpr bool kGlobal = true;
     const AggArgs &args;
     unsigned int gid;
     __device__ __forceinline__ uint8_t *row() const
@@ -356,6 +433,14 @@ struct GlobalRef {
         return reinterpret_cast<T *>(row() + args.a[i].off[w]);
     }
     __device__ __forceinline__ uint8_t *flag(int i) const { return row() + args.a[i].flag_off; }
+    __device__ __forceinline__ unsigned int *cflag(int i) const
+    {
+        return reinterpret_cast<unsigned int *>(row() + args.a[i].cflag_off);
+    }
+    __device__ __forceinline__ U192 word3(int i, int w) const
+    {
+        return U192{*p<unsigned long long>(i, w), *p<unsigned long long>(i, w + 1), *p<long long>(i, w + 2)};
+    }
 };
 
 struct LdsLayout {
@@ -368,6 +453,7 @@ struct LdsLayout {
 };
 
 struct LdsRef {
+    static constexpr bool kGlobal = false;
     uint8_t *base;
     const LdsLayout &L;
     int slot;
@@ -430,12 +516,40 @@ template <typename Ref> __device__ __forceinline__ void UpdateAggsT(const AggArg
                 *r.flag(i) = 1;
                 break;
             case K_MIN_F64:
-                atomicMin(r.template p<double>(i, 0), static_cast<const double *>(d.values)[row]);
+            case K_MAX_F64: {
+                // std::min / std::max over the rows in order (MaxOp, max_aggregator.h:39-44): a NaN input never
+                // replaces the state, and among inputs equal to the result the first one stays -- which only shows
+                // for zeros, so the first zero's sign is kept apart (its input position, sign in bit 0)
+                const unsigned long long bits = static_cast<const unsigned long long *>(d.values)[row];
                 *r.flag(i) = 1;
+                const bool nan = (bits & 0x7FFFFFFFFFFFFFFFULL) > 0x7FF0000000000000ULL;
+                if (nan) break;
+                const bool zero = (bits & 0x7FFFFFFFFFFFFFFFULL) == 0;
+                const unsigned long long key = OrderedKey(zero ? 0ULL : bits);
+                if (d.kind == K_MIN_F64) atomicMin(r.template p<u64>(i, 0), key);
+                else atomicMax(r.template p<u64>(i, 0), key);
+                if (zero) {
+                    const u64 pos = static_cast<u64>(args.row_base + row);
+                    atomicMin(r.template p<u64>(i, 1), (pos << 1) | (bits >> 63));
+                }
                 break;
-            case K_MAX_F64:
-                atomicMax(r.template p<double>(i, 0), static_cast<const double *>(d.values)[row]);
-                *r.flag(i) = 1;
+            }
+            case K_SUM_DEC_CHK:
+                if constexpr (Ref::kGlobal) {
+                    u64 vlo, vhi;
+                    long long vtop;
+                    DecInput(d, row, vlo, vhi, vtop);
+                    U192 v{vlo, vhi, vtop};
+                    if (vtop < 0) {  // the magnitude of a negative input into N
+                        v = Neg192(v);
+                        AddDec3(r.template p<u64>(i, 3), r.template p<u64>(i, 4), r.template p<long long>(i, 5), v.lo,
+                                v.hi, v.top);
+                    } else {
+                        AddDec3(r.template p<u64>(i, 0), r.template p<u64>(i, 1), r.template p<long long>(i, 2), v.lo,
+                                v.hi, v.top);
+                    }
+                    *r.flag(i) = 1;
+                }
                 break;
             default: break;
         }
@@ -483,8 +597,14 @@ __device__ __forceinline__ void MergeAggs(const AggArgs &args, unsigned int gid,
                     case K_MAX_I64: atomicMax(g.p<long long>(i, 0), *l.p<long long>(i, 0)); break;
                     case K_MIN_I32: atomicMin(g.p<int>(i, 0), *l.p<int>(i, 0)); break;
                     case K_MAX_I32: atomicMax(g.p<int>(i, 0), *l.p<int>(i, 0)); break;
-                    case K_MIN_F64: atomicMin(g.p<double>(i, 0), *l.p<double>(i, 0)); break;
-                    case K_MAX_F64: atomicMax(g.p<double>(i, 0), *l.p<double>(i, 0)); break;
+                    case K_MIN_F64:
+                    case K_MAX_F64: {
+                        const u64 key = *l.p<u64>(i, 0), zpos = *l.p<u64>(i, 1);
+                        if (d.kind == K_MIN_F64) atomicMin(g.p<u64>(i, 0), key);
+                        else atomicMax(g.p<u64>(i, 0), key);
+                        if (zpos != ~0ULL) atomicMin(g.p<u64>(i, 1), zpos);
+                        break;
+                    }
                     default: break;
                 }
             }
@@ -500,8 +620,26 @@ __host__ __device__ __forceinline__ unsigned long long InitWord(int kind)
         case K_MAX_I64: return 0x8000000000000000ULL;
         case K_MIN_I32: return 0x7FFFFFFFULL;
         case K_MAX_I32: return 0x80000000ULL;
-        case K_MIN_F64: return 0x7FEFFFFFFFFFFFFFULL;  // DBL_MAX
-        case K_MAX_F64: return 0xFFEFFFFFFFFFFFFFULL;  // -DBL_MAX
+        case K_MIN_F64: return OrderedKey(0x7FEFFFFFFFFFFFFFULL);  // DBL_MAX (MinAggregator's initial state)
+        case K_MAX_F64: return OrderedKey(0xFFEFFFFFFFFFFFFFULL);  // lowest() = -DBL_MAX
+        default: return 0;
+    }
+}
+
+// Initial value of an aggregate's second state word (the first zero's position of double MIN/MAX: none yet).
+__host__ __device__ __forceinline__ unsigned long long InitWord1(int kind)
+{
+    return (kind == K_MIN_F64 || kind == K_MAX_F64) ? ~0ULL : 0ULL;
+}
+
+// State words (8 bytes each) an aggregate keeps besides its first word in an LDS or per-slot table.
+__host__ __device__ __forceinline__ int ExtraWords(int kind)
+{
+    switch (kind) {
+        case K_AVG:
+        case K_MIN_F64:
+        case K_MAX_F64: return 1;
+        case K_SUM_DEC: return 2;
         default: return 0;
     }
 }
@@ -524,11 +662,11 @@ inline LdsLayout UnitLayout(const AggArgs &args)
         U.off[i][1] = -1;
         U.off[i][2] = -1;
         U.flag_off[i] = -1;
-        if (k == K_AVG || k == K_SUM_DEC) {
+        if (ExtraWords(k) >= 1) {
             U.off[i][1] = off;
             off += 8;
         }
-        if (k == K_SUM_DEC) {
+        if (ExtraWords(k) >= 2) {
             U.off[i][2] = off;
             off += 8;
         }
@@ -543,6 +681,7 @@ inline LdsLayout UnitLayout(const AggArgs &args)
 
 // State addressing of one slot of a table laid out by UnitLayout with `stride` slots per array.
 struct ScaledRef {
+    static constexpr bool kGlobal = false;
     uint8_t *base;
     const LdsLayout &U;
     long long stride;
@@ -591,24 +730,25 @@ __device__ __forceinline__ void StoreWidth(void *dst, int width, int64_t r, unsi
     }
 }
 
-// Decimal finalisation: range check of the 192-bit total, then x 10^up with overflow checks
-// (SumSparkDecimalAggregator::ExtractValuesBatchInternal, sum_spark_decimal_aggregator.h:246-301).
-static __device__ bool FinishDecimal(unsigned long long lo, unsigned long long hi, long long top, int up, bool out128,
-                              __int128 &res)
+// Decimal finalisation: the 192-bit total must lie within +-(10^38 - 1), then x 10^up checked against the same
+// DECIMAL128 limits whatever the output type (MulCheckedOverflow in ExtractValuesBatchInternal,
+// sum_spark_decimal_aggregator.h:246-301; a DECIMAL64 output keeps the low 64 bits of `res`).  `sum128` is the
+// unchecked DECIMAL64 -> DECIMAL128 sum, whose total cannot leave those limits (|sum| <= rows * 2^63); the checked
+// sums arrive here already inside their own limits.
+static __device__ bool FinishDecimal(unsigned long long lo, unsigned long long hi, long long top, int up, bool sum128,
+                                     __int128 &res)
 {
+    (void)sum128;
     bool fits128 = (top == 0 && static_cast<long long>(hi) >= 0) || (top == -1 && static_cast<long long>(hi) < 0);
     if (!fits128) return false;
     __int128 v = static_cast<__int128>((static_cast<unsigned __int128>(hi) << 64) | lo);
     const __int128 max38 = (static_cast<__int128>(5421010862427522170LL) << 64) + 687399551400673279LL;
-    const __int128 lim = out128 ? max38 : static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL);
-    const __int128 llim = out128 ? -max38 : -static_cast<__int128>(0x7FFFFFFFFFFFFFFFLL) - 1;
-    if (v > lim || v < llim) return false;
+    if (v > max38 || v < -max38) return false;
     for (int i = 0; i < up; ++i) {
         // v * 10 must stay within [-max38, max38]
         if (v > max38 / 10 || v < -(max38 / 10)) return false;
         v = (v << 3) + (v << 1);
     }
-    if (v > lim || v < llim) return false;
     res = v;
     return true;
 }
@@ -656,6 +796,44 @@ __device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s
             }
             break;
         }
+        case K_SUM_DEC_CHK: {
+            isNull = *s.flag(i) == 0;
+            __int128 v = 0;
+            if constexpr (Ref::kGlobal) {
+                if (!isNull) {
+                    bool ok = (*s.cflag(i) & kDecOvf) == 0;
+                    if (ok) {
+                        // not overflowed: P <= U and N <= L, so P - N fits the output's int64 / int128 range
+                        const U192 t = Add192(s.word3(i, 0), Neg192(s.word3(i, 3)));
+                        ok = FinishDecimal(t.lo, t.hi, t.top, d.dec_scale_up, true, v);
+                    }
+                    if (!ok) {
+                        isNull = true;
+                        v = 0;
+                        if (!overflow_as_null) atomicOr(error_flag, 1u);
+                    }
+                }
+            }
+            if (d.dec_out128) {
+                static_cast<u64 *>(o.values)[2 * r] = static_cast<u64>(v);
+                static_cast<u64 *>(o.values)[2 * r + 1] = static_cast<u64>(static_cast<unsigned __int128>(v) >> 64);
+            } else {
+                // SetValToVector: static_cast<int64_t> of the rescaled value (its low 64 bits)
+                static_cast<long long *>(o.values)[r] = static_cast<long long>(v);
+            }
+            break;
+        }
+        case K_MIN_F64:
+        case K_MAX_F64: {
+            isNull = *s.flag(i) == 0;
+            u64 bits = 0;
+            if (!isNull) {
+                bits = FromOrderedKey(*s.template p<u64>(i, 0));
+                if ((bits & 0x7FFFFFFFFFFFFFFFULL) == 0) bits = (*s.template p<u64>(i, 1) & 1ULL) << 63;
+            }
+            static_cast<u64 *>(o.values)[r] = bits;
+            break;
+        }
         case K_MIN_I32:
         case K_MAX_I32: {
             isNull = *s.flag(i) == 0;
@@ -663,7 +841,7 @@ __device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s
             StoreWidth(o.values, o.width, r, static_cast<u64>(static_cast<long long>(v)));
             break;
         }
-        default:  // SUM_I64, SUM_F64, MIN/MAX 64-bit: the 8-byte state is the value
+        default:  // SUM_I64, SUM_F64, integer MIN/MAX 64-bit: the 8-byte state is the value
             isNull = *s.flag(i) == 0;
             static_cast<u64 *>(o.values)[r] = isNull ? 0ULL : *s.template p<u64>(i, 0);
             break;

@@ -63,9 +63,11 @@ __global__ void __launch_bounds__(kBlock) agg_key_minmax(KeyDev kd, int64_t rows
 }
 
 // Rows are visited grid-stride by a fixed grid, so wave slot w keeps its id chunk across launches.
+// row_gid (optional): receives each row's group id (the replay of checked decimal sums reads it).
 __global__ void __launch_bounds__(kBlock) agg_insert_single(KeyDev kd, int64_t rows, Slot1 *slots,
                                                             unsigned long long mask, GroupCtx g, AggArgs args,
-                                                            unsigned long long *cursors, ArrayMapDev am)
+                                                            unsigned long long *cursors, ArrayMapDev am,
+                                                            unsigned int *row_gid)
 {
     __shared__ unsigned long long wc[kWavesPerBlock][2];
     const int slot = blockIdx.x * kWavesPerBlock + (WaveInBlock());
@@ -80,13 +82,15 @@ __global__ void __launch_bounds__(kBlock) agg_insert_single(KeyDev kd, int64_t r
             gid = KeyGroup(am, slots, mask, LoadWord(kd.values[0], kd.width[0], row), g);
         }
         if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+        if (row_gid) row_gid[row] = gid;
     }
     StoreWaveChunk(wc[WaveInBlock()], cursors, slot);
 }
 
 __global__ void __launch_bounds__(kBlock) agg_insert_multi(KeyDev kd, int64_t rows, unsigned int *ctrl,
                                                            unsigned long long *skeys, unsigned long long mask,
-                                                           GroupCtx g, AggArgs args, unsigned long long *cursors)
+                                                           GroupCtx g, AggArgs args, unsigned long long *cursors,
+                                                           unsigned int *row_gid)
 {
     __shared__ unsigned long long wc[kWavesPerBlock][2];
     const int slot = blockIdx.x * kWavesPerBlock + (WaveInBlock());
@@ -98,8 +102,161 @@ __global__ void __launch_bounds__(kBlock) agg_insert_multi(KeyDev kd, int64_t ro
         EncodeKey(kd, row, words);
         unsigned int gid = InsertMulti(ctrl, skeys, mask, words, kd.W, HashWords(words, kd.W), g);
         if (gid != 0xFFFFFFFFu) UpdateAggs(args, gid, row);
+        if (row_gid) row_gid[row] = gid;
     }
     StoreWaveChunk(wc[WaveInBlock()], cursors, slot);
+}
+
+// ---- checked decimal SUM: settle and row-order replay after each insert launch ------------------------------------
+// (the state and its invariant are described at K_SUM_DEC_CHK)
+
+// Marks CROSSED every group of the launch whose P or N passed its limit (and that has not overflowed); sets
+// *crossed when any did.  One thread per row of the launch (a group is checked once per row it has).
+__global__ void __launch_bounds__(kBlock) agg_dec_settle(int64_t rows, const unsigned int *row_gid, AggArgs args,
+                                                         unsigned int *crossed)
+{
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * kBlock;
+    for (int64_t row = static_cast<int64_t>(blockIdx.x) * kBlock + threadIdx.x; row < rows; row += stride) {
+        const unsigned int gid = row_gid[row];
+        if (gid == 0xFFFFFFFFu) continue;
+        GlobalRef g{args, gid};
+        for (int i = 0; i < args.n; ++i) {
+            const AggDev &d = args.a[i];
+            if (d.kind != K_SUM_DEC_CHK || NullBit(d.nulls, row)) continue;
+            const unsigned int cf = *g.cflag(i);
+            if (cf & (kDecOvf | kDecCrossed)) continue;
+            if (Gt192(g.word3(i, 0), DecUpper(d.dec_out128)) || Gt192(g.word3(i, 3), DecLowerMag(d.dec_out128))) {
+                atomicOr(g.cflag(i), kDecCrossed);
+                atomicOr(crossed, 1u);
+            }
+        }
+    }
+}
+
+constexpr int kReplayBlock = 1024;
+
+// One workgroup.  Nothing to do unless agg_dec_settle marked a group.  Then, for the CROSSED groups:
+//  1. subtract the launch's inputs from P / N again (any order: exact sums), which leaves the value before the launch;
+//  2. replay the launch's rows in row order, kReplayBlock rows at a time: the chunk's rows of crossed groups are
+//     sorted by (group, row) in LDS and the first row of each group's run adds the run's inputs in order to the
+//     group's value with the reference's checked add, marking kDecOvf at the first add that leaves the limits;
+//  3. clear CROSSED.
+__global__ void __launch_bounds__(kReplayBlock) agg_dec_replay(int64_t rows, const unsigned int *row_gid, AggArgs args,
+                                                              unsigned int *crossed)
+{
+    __shared__ unsigned long long sk[kReplayBlock];
+    if (*crossed == 0u) return;
+    const int t = threadIdx.x;
+    // State words are read with L1-bypassing loads: the atomics of step 1 and the stores of other threads in step 2
+    // are made at the L2, where a plain load could miss them behind a line this CU's L1 still holds.
+    auto isCrossed = [&](unsigned int gid, int i) {
+        return args.a[i].kind == K_SUM_DEC_CHK &&
+               (LoadRelaxed32(GlobalRef{args, gid}.cflag(i)) & (kDecOvf | kDecCrossed)) == kDecCrossed;
+    };
+    auto load3 = [&](const GlobalRef &g, int i, int w) {
+        return U192{LoadRelaxed64(g.p<unsigned long long>(i, w)), LoadRelaxed64(g.p<unsigned long long>(i, w + 1)),
+                    static_cast<long long>(LoadRelaxed64(g.p<unsigned long long>(i, w + 2)))};
+    };
+    auto store3 = [&](const GlobalRef &g, int i, int w, U192 v) {
+        StoreRelaxed64(g.p<unsigned long long>(i, w), v.lo);
+        StoreRelaxed64(g.p<unsigned long long>(i, w + 1), v.hi);
+        StoreRelaxed64(g.p<unsigned long long>(i, w + 2), static_cast<unsigned long long>(v.top));
+    };
+    // 1. take the launch's inputs out of the crossed groups' sums
+    for (int64_t row = t; row < rows; row += kReplayBlock) {
+        const unsigned int gid = row_gid[row];
+        if (gid == 0xFFFFFFFFu) continue;
+        GlobalRef g{args, gid};
+        for (int i = 0; i < args.n; ++i) {
+            const AggDev &d = args.a[i];
+            if (!isCrossed(gid, i) || NullBit(d.nulls, row)) continue;
+            unsigned long long vlo, vhi;
+            long long vtop;
+            DecInput(d, row, vlo, vhi, vtop);
+            U192 v{vlo, vhi, vtop};
+            const int w = vtop < 0 ? 3 : 0;
+            if (vtop < 0) v = Neg192(v);
+            const U192 m = Neg192(v);
+            AddDec3(g.p<unsigned long long>(i, w), g.p<unsigned long long>(i, w + 1), g.p<long long>(i, w + 2), m.lo,
+                    m.hi, m.top);
+        }
+    }
+    __syncthreads();
+    // 2. row-order replay
+    for (int64_t base = 0; base < rows; base += kReplayBlock) {
+        const int64_t row = base + t;
+        unsigned long long key = ~0ULL;
+        if (row < rows) {
+            const unsigned int gid = row_gid[row];
+            bool any = false;
+            if (gid != 0xFFFFFFFFu) {
+                for (int i = 0; i < args.n; ++i) any |= isCrossed(gid, i);
+            }
+            if (any) key = (static_cast<unsigned long long>(gid) << 10) | static_cast<unsigned long long>(t);
+        }
+        sk[t] = key;
+        __syncthreads();
+        for (int k = 2; k <= kReplayBlock; k <<= 1) {  // bitonic sort, ascending
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const int p = t ^ j;
+                if (p > t) {
+                    const unsigned long long a = sk[t], b = sk[p];
+                    const bool up = (t & k) == 0;
+                    if ((a > b) == up) {
+                        sk[t] = b;
+                        sk[p] = a;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const unsigned long long mine = sk[t];
+        if (mine != ~0ULL && (t == 0 || (sk[t - 1] >> 10) != (mine >> 10))) {
+            const unsigned int gid = static_cast<unsigned int>(mine >> 10);
+            GlobalRef g{args, gid};
+            for (int i = 0; i < args.n; ++i) {
+                const AggDev &d = args.a[i];
+                if (!isCrossed(gid, i)) continue;
+                const U192 up = DecUpper(d.dec_out128), lo = Neg192(DecLowerMag(d.dec_out128));
+                U192 v = Add192(load3(g, i, 0), Neg192(load3(g, i, 3)));
+                bool ovf = false;
+                for (int e = t; e < kReplayBlock && (sk[e] >> 10) == (mine >> 10) && sk[e] != ~0ULL; ++e) {
+                    const int64_t r = base + static_cast<int64_t>(sk[e] & 1023ULL);
+                    if (NullBit(d.nulls, r)) continue;
+                    unsigned long long vlo, vhi;
+                    long long vtop;
+                    DecInput(d, r, vlo, vhi, vtop);
+                    v = Add192(v, U192{vlo, vhi, vtop});
+                    if (Gt192(v, up) || Gt192(lo, v)) {
+                        ovf = true;
+                        break;
+                    }
+                }
+                if (ovf) {
+                    atomicOr(g.cflag(i), kDecOvf);
+                    continue;
+                }
+                const bool neg = v.top < 0;
+                const U192 z{0, 0, 0}, mag = neg ? Neg192(v) : v;
+                store3(g, i, 0, neg ? z : mag);
+                store3(g, i, 3, neg ? mag : z);
+            }
+        }
+        __syncthreads();
+    }
+    // 3. clear CROSSED (the flags are read above for the last time)
+    for (int64_t row = t; row < rows; row += kReplayBlock) {
+        const unsigned int gid = row_gid[row];
+        if (gid == 0xFFFFFFFFu) continue;
+        GlobalRef g{args, gid};
+        for (int i = 0; i < args.n; ++i) {
+            if (args.a[i].kind == K_SUM_DEC_CHK && (LoadRelaxed32(g.cflag(i)) & kDecCrossed)) {
+                atomicAnd(g.cflag(i), ~kDecCrossed);
+            }
+        }
+    }
+    __syncthreads();
+    if (t == 0) *crossed = 0u;
 }
 
 constexpr int kLdsBlock = 1024;
@@ -134,7 +291,7 @@ __global__ void __launch_bounds__(kLdsBlock) agg_insert_lds(KeyDev kd, int64_t r
             } else {
                 reinterpret_cast<unsigned long long *>(base + L.off[a][0])[i] = InitWord(kind);
             }
-            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = 0;
+            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = InitWord1(kind);
             if (L.off[a][2] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][2])[i] = 0;
             if (L.flag_off[a] >= 0) base[L.flag_off[a] + i] = 0;
         }
@@ -209,7 +366,7 @@ __global__ void __launch_bounds__(kLdsBlock) agg_insert_lds_multi(KeyDev kd, int
             } else {
                 reinterpret_cast<unsigned long long *>(base + L.off[a][0])[i] = InitWord(kind);
             }
-            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = 0;
+            if (L.off[a][1] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][1])[i] = InitWord1(kind);
             if (L.off[a][2] >= 0) reinterpret_cast<unsigned long long *>(base + L.off[a][2])[i] = 0;
             if (L.flag_off[a] >= 0) base[L.flag_off[a] + i] = 0;
         }

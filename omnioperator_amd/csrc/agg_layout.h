@@ -34,7 +34,13 @@ inline int ClassifyAgg(const omni_gpu_agg_spec &sp, int i, AggDev &d)
                          "decimal SUM output must be decimal");
                 OG_CHECK(sp.agg_out_scales[i] >= sp.agg_in_scales[i] && sp.agg_out_scales[i] - sp.agg_in_scales[i] <= 38,
                          OMNI_GPU_ERR_INVALID, "decimal SUM output scale must be >= input scale");
-                d.kind = K_SUM_DEC;
+                // SumSparkAggregatorFactory (aggregator_factory.cpp:104-116): DECIMAL128 input is summed into a
+                // DECIMAL128 state only
+                OG_CHECK(!(ti == OMNI_TYPE_DECIMAL128 && to == OMNI_TYPE_DECIMAL64), OMNI_GPU_ERR_INVALID,
+                         "SUM(DECIMAL128) output must be DECIMAL128");
+                // A DECIMAL64 -> DECIMAL128 sum never leaves +-(10^38 - 1) (at most 2^63 per row), so its exact total
+                // is its row-order result; the other two can overflow part-way and are checked in row order.
+                d.kind = (ti == OMNI_TYPE_DECIMAL64 && to == OMNI_TYPE_DECIMAL128) ? K_SUM_DEC : K_SUM_DEC_CHK;
                 d.dec_scale_up = sp.agg_out_scales[i] - sp.agg_in_scales[i];
                 d.dec_out128 = to == OMNI_TYPE_DECIMAL128;
                 return OMNI_GPU_OK;
@@ -76,24 +82,26 @@ inline void AssignRowLayout(AggArgs &args)
     for (int i = 0; i < args.n; ++i) {
         AggDev &d = args.a[i];
         int k = d.kind;
-        d.off[0] = d.off[1] = d.off[2] = -1;
+        for (int w = 0; w < 6; ++w) d.off[w] = -1;
         d.flag_off = -1;
+        d.cflag_off = -1;
         if (!Is32(k)) {
             d.off[0] = off;
             off += 8;
         }
-        if (k == K_AVG || k == K_SUM_DEC) {
-            d.off[1] = off;
-            off += 8;
-        }
-        if (k == K_SUM_DEC) {
-            d.off[2] = off;
+        const int extra = k == K_SUM_DEC_CHK ? 5 : ExtraWords(k);
+        for (int w = 1; w <= extra; ++w) {
+            d.off[w] = off;
             off += 8;
         }
     }
     for (int i = 0; i < args.n; ++i) {
         if (Is32(args.a[i].kind)) {
             args.a[i].off[0] = off;
+            off += 4;
+        }
+        if (args.a[i].kind == K_SUM_DEC_CHK) {
+            args.a[i].cflag_off = off;
             off += 4;
         }
     }
@@ -119,22 +127,25 @@ inline std::vector<unsigned long long> RowTemplate(const AggArgs &args)
         } else {
             std::memcpy(b + d.off[0], &init, 8);
         }
+        if (ExtraWords(d.kind) >= 1 && d.off[1] >= 0) {
+            unsigned long long w1 = InitWord1(d.kind);
+            std::memcpy(b + d.off[1], &w1, 8);
+        }
     }
     return w;
 }
 
 // LDS table layout for the aggregates of a handle within `budget` bytes: per slot `key_words` u64 key words (and a
 // u32 slot state for multi-word keys) plus each aggregate's state words (4 bytes for 32-bit MIN/MAX) and a flag
-// byte where the aggregate needs one.
+// byte where the aggregate needs one.  A checked decimal SUM (K_SUM_DEC_CHK) is never aggregated in LDS: a handle
+// that has one takes the global table.
 inline LdsLayout MakeLdsLayout(const AggArgs &args, int budget, int key_words = 1)
 {
     LdsLayout L{};
     int perSlot = 8 * key_words + (key_words > 1 ? 4 : 0);
     for (int i = 0; i < args.n; ++i) {
         int k = args.a[i].kind;
-        perSlot += Is32(k) ? 4 : 8;
-        if (k == K_AVG || k == K_SUM_DEC) perSlot += 8;
-        if (k == K_SUM_DEC) perSlot += 8;
+        perSlot += (Is32(k) ? 4 : 8) + 8 * ExtraWords(k);
         if (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) perSlot += 1;
     }
     int S = (budget - 64 * (2 * OMNI_GPU_MAX_AGGS + 1)) / perSlot;  // room for 8-byte alignment padding
@@ -151,8 +162,8 @@ inline LdsLayout MakeLdsLayout(const AggArgs &args, int budget, int key_words = 
     for (int i = 0; i < args.n; ++i) {
         int k = args.a[i].kind;
         L.off[i][0] = take((Is32(k) ? 4 : 8) * S);
-        L.off[i][1] = (k == K_AVG || k == K_SUM_DEC) ? take(8 * S) : -1;
-        L.off[i][2] = k == K_SUM_DEC ? take(8 * S) : -1;
+        L.off[i][1] = ExtraWords(k) >= 1 ? take(8 * S) : -1;
+        L.off[i][2] = ExtraWords(k) >= 2 ? take(8 * S) : -1;
         L.flag_off[i] = (k != K_COUNT_ALL && k != K_COUNT_COL && k != K_AVG) ? take(S) : -1;
     }
     L.bytes = off;

@@ -262,7 +262,15 @@ int omni_gpu_agg_batches_create(const omni_gpu_agg_spec *spec, int64_t max_batch
     c.kd.W = 1;
     c.kd.width[0] = TypeWidth(spec->key_types[0]);
     c.args.n = spec->num_aggs;
-    for (int i = 0; i < spec->num_aggs; ++i) OG_TRY(ClassifyAgg(*spec, i, c.args.a[i]));
+    for (int i = 0; i < spec->num_aggs; ++i) {
+        OG_TRY(ClassifyAgg(*spec, i, c.args.a[i]));
+        // the row-order check of a DECIMAL64 -> DECIMAL64 or DECIMAL128 SUM needs the per-batch path
+        OG_CHECK(c.args.a[i].kind != K_SUM_DEC_CHK, OMNI_GPU_ERR_INVALID,
+                 "the batched stream sums decimals into DECIMAL128 from DECIMAL64 only (aggregate %d); other decimal "
+                 "sums can overflow part-way and take omni_gpu_agg_create",
+                 i);
+    }
+    c.args.row_base = 0;
     c.R = UnitLayout(c.args);
     c.overflow_as_null = spec->overflow_as_null;
     OG_TRY(AcquireStream(&h->stream));

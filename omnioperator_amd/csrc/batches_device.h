@@ -78,8 +78,8 @@ template <typename Ref> __device__ __forceinline__ void InitSlot(const AggArgs &
         int kind = args.a[a].kind;
         if (Is32(kind)) *r.template p<int>(a, 0) = static_cast<int>(InitWord(kind));
         else *r.template p<unsigned long long>(a, 0) = InitWord(kind);
-        if (kind == K_AVG || kind == K_SUM_DEC) *r.template p<unsigned long long>(a, 1) = 0;
-        if (kind == K_SUM_DEC) *r.template p<unsigned long long>(a, 2) = 0;
+        if (ExtraWords(kind) >= 1) *r.template p<unsigned long long>(a, 1) = InitWord1(kind);
+        if (ExtraWords(kind) >= 2) *r.template p<unsigned long long>(a, 2) = 0;
         if (kind != K_COUNT_ALL && kind != K_COUNT_COL && kind != K_AVG) *r.flag(a) = 0;
     }
 }

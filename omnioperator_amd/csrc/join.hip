@@ -66,6 +66,19 @@ struct JoinHandle {
     //          [16] min/max i64[2], [32] match counter u64, [40] build overflow count u32
     DevBuf slots, heads, next, scratch;
     DevBuf batch_desc, build_desc;
+
+    // Every device buffer of the handle (destroy releases exactly these; a new member is added here).
+    template <typename F> void ForEachBuffer(F f)
+    {
+        for (DevBuf *b : {&keys, &knull, &mm_parts, &visited, &part_keys, &part_rows, &part_counts, &part_meta,
+                          &ovf_keys, &ovf_rows, &slots, &heads, &next, &scratch, &batch_desc, &build_desc}) {
+            f(*b);
+        }
+        for (int p = 0; p < OMNI_GPU_MAX_AGGS; ++p) {
+            f(pay[p]);
+            f(pay_null[p]);
+        }
+    }
 };
 
 namespace {
@@ -549,6 +562,12 @@ int omni_gpu_join_build_finish(void *handle)
             g.pnull[p] = h.pay_null[p].as<uint8_t>();
         }
     }
+    // JoinHashTableVariants::BuildHashTable tries the array table for INT / DATE32 / SHORT / LONG / TIMESTAMP
+    // keys only (join_hash_table_variants.cpp:700-736); DECIMAL64, DOUBLE and BOOLEAN keys always hash.
+    const int kt = h.spec.key_type;
+    const bool arrayCandidate = h.spec.strategy == 0 &&
+                                (kt == OMNI_TYPE_INT || kt == OMNI_TYPE_DATE32 || kt == OMNI_TYPE_SHORT ||
+                                 kt == OMNI_TYPE_LONG || kt == OMNI_TYPE_TIMESTAMP);
     int64_t blocks = 0;
     h.key_nulls = false;
     for (BuildBatch &bb : h.pending) {
@@ -561,13 +580,24 @@ int omni_gpu_join_build_finish(void *handle)
     OG_TRY(CopyToDevice(h.build_desc.p, h.pending.data(), sizeof(BuildBatch) * h.pending.size(), s));
     g.batches = h.build_desc.as<BuildBatch>();
     long long *dmm = reinterpret_cast<long long *>(sc + 16);
-    if (h.spec.strategy == 0) {
+    if (arrayCandidate) {
         OG_TRY(h.mm_parts.Reserve(2 * sizeof(long long) * static_cast<size_t>(blocks), s));
         g.mm = h.mm_parts.as<long long>();
     }
+    // Once the gather is queued, every return waits for it: the caller may release its build buffers as soon
+    // as finish returns, also after an error.
+    struct GatherWait {
+        hipStream_t s;
+        bool armed = false;
+        ~GatherWait()
+        {
+            if (armed) (void)hipStreamSynchronize(s);
+        }
+    } gatherWait{s};
     {
         KernelScope ks(s, "join_gather_build");
         join_gather_build<<<static_cast<unsigned int>(blocks), 256, 0, s>>>(g);
+        gatherWait.armed = true;
         OG_TRY(CheckLaunch("join_gather_build"));
         if (g.mm) {
             join_minmax_reduce<<<1, 1024, 0, s>>>(g.mm, blocks, dmm);
@@ -597,21 +627,25 @@ int omni_gpu_join_build_finish(void *handle)
                            h.n < (1LL << 32) - 1;
     PartArgs bpa;
     bool prePartitioned = false;
-    if (partBuild && h.spec.strategy == 0) {
+    if (partBuild) {
         h.slots_n = hashSlots;
         OG_TRY(PartitionBuildRows(h, static_cast<int>(bP), bpa));
         prePartitioned = true;
     }
     // initDegree and the array-table rule of JoinHashTableVariants::BuildHashTable / TryToBuildArrayTable.
     bool arrayOk = false;
-    if (h.spec.strategy == 0) {
+    if (arrayCandidate) {
         OG_HIP(hipEventSynchronize(h.flags_ev));
         const long long mm[2] = {cell_mm[0], cell_mm[1]};
         if (mm[1] >= mm[0]) {
             double degree = std::ceil(std::log2(static_cast<double>(h.n) / 0.75));
             long double len = std::pow(2.0L, static_cast<long double>(degree));
             long double range = static_cast<long double>(mm[1]) - static_cast<long double>(mm[0]);
-            arrayOk = range <= 4294967295.0L && range <= 8.0L * len;
+            // max - min must not overflow the key type (TryToBuildArrayTable's `min < 0 && T_MAX + min < max`)
+            long double typeMax = kt == OMNI_TYPE_SHORT ? 32767.0L
+                                  : (kt == OMNI_TYPE_INT || kt == OMNI_TYPE_DATE32) ? 2147483647.0L
+                                                                                    : 9223372036854775807.0L;
+            arrayOk = range <= typeMax && range <= 4294967295.0L && range <= 8.0L * len;
             if (arrayOk) {
                 h.min_key = mm[0];
                 h.range = static_cast<unsigned long long>(mm[1] - mm[0]) + 1;
@@ -662,6 +696,7 @@ int omni_gpu_join_build_finish(void *handle)
     // the gather must have read the caller's build buffers before finish returns (the array-table decision
     // above already waited for it)
     if (!g.mm) OG_HIP(hipEventSynchronize(h.flags_ev));
+    gatherWait.armed = false;
     OG_HIP(hipMemcpyAsync(h.flag_cell, has_dup, 4, hipMemcpyDeviceToHost, s));
     OG_HIP(hipEventRecord(h.flags_ev, s));
     h.flags_pending = true;
@@ -866,23 +901,7 @@ int omni_gpu_join_destroy(void *handle)
         (void)hipEventDestroy(h->flags_ev);
     }
     if (h->flag_cell) ReturnFlagCell(h->flag_cell);
-    h->keys.Release(s);
-    h->knull.Release(s);
-    h->mm_parts.Release(s);
-    for (int p = 0; p < OMNI_GPU_MAX_AGGS; ++p) {
-        h->pay[p].Release(s);
-        h->pay_null[p].Release(s);
-    }
-    h->slots.Release(s);
-    h->heads.Release(s);
-    h->next.Release(s);
-    h->scratch.Release(s);
-    h->batch_desc.Release(s);
-    h->build_desc.Release(s);
-    h->part_keys.Release(s);
-    h->part_rows.Release(s);
-    h->part_counts.Release(s);
-    h->part_meta.Release(s);
+    h->ForEachBuffer([s](DevBuf &b) { b.Release(s); });
     ReleaseStream(s);
     delete h;
     return OMNI_GPU_OK;

@@ -60,6 +60,8 @@ int CheckLaunch(const char *kernel);
 // hipMallocAsync.  DevFree keeps the block for reuse instead of returning it to the runtime.
 int DevAlloc(void **p, size_t bytes, hipStream_t s);
 void DevFree(void *p, hipStream_t s);
+// Bytes of device blocks the library holds for live buffers, and bytes of freed blocks kept for reuse.
+void MemoryStats(int64_t *in_use, int64_t *cached);
 
 // Per-device pool of non-blocking streams (default priority).  Each handle owns one stream while it
 // lives; creating and destroying a HIP stream per operator would cost more host time than the operator's

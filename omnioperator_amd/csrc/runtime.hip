@@ -213,6 +213,15 @@ void DevFree(void *p, hipStream_t s)
     (void)hipFreeAsync(p, s);
 }
 
+void MemoryStats(int64_t *in_use, int64_t *cached)
+{
+    std::lock_guard<std::mutex> g(g_cache.mu);
+    size_t live = 0;
+    for (const auto &kv : g_cache.live) live += kv.second.first;
+    *in_use = static_cast<int64_t>(live - g_cache.cached);
+    *cached = static_cast<int64_t>(g_cache.cached);
+}
+
 namespace {
 // Pinned staging block per stream for small host<->device transfers: a copy from pageable memory costs
 // tens of microseconds of host time on this runtime, one from pinned memory is an ordinary async copy.
@@ -476,6 +485,13 @@ int omni_gpu_memset(void *dst, int value, size_t bytes)
 int omni_gpu_device_synchronize(void)
 {
     OG_HIP(hipDeviceSynchronize());
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_memory_stats(int64_t *in_use_bytes, int64_t *cached_bytes)
+{
+    OG_CHECK(in_use_bytes && cached_bytes, OMNI_GPU_ERR_INVALID, "null argument");
+    MemoryStats(in_use_bytes, cached_bytes);
     return OMNI_GPU_OK;
 }
 

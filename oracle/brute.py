@@ -110,10 +110,13 @@ def group_by(key_cols: Sequence[Col], aggs: Sequence[tuple], agg_cols: Sequence[
                     if acc > lim_hi or acc < lim_lo:
                         ovf = True
                 if seen and not ovf:
+                    # the rescale is checked against the DECIMAL128 limits whatever the output type; a DECIMAL64
+                    # output then keeps the low 64 bits of the result (two's complement)
                     acc *= scale
-                    lim_hi, lim_lo = (DEC128_MAX, -DEC128_MAX) if tout == DECIMAL128 else (I64_MAX, I64_MIN)
-                    if acc > lim_hi or acc < lim_lo:
+                    if acc > DEC128_MAX or acc < -DEC128_MAX:
                         ovf = True
+                    elif tout != DECIMAL128:
+                        acc = (acc + 2**63) % 2**64 - 2**63
                 if ovf and not overflow_as_null:
                     raise OverflowError("decimal overflow")
                 res.append(0 if (ovf or not seen) else acc)

@@ -680,19 +680,18 @@ bool ExtractState(const AggDesc &a, const uint8_t *s, OutCol &o, int64_t r, bool
             bool ovf = flag == static_cast<uint8_t>(VState::OVERFLOWED);
             int128 scaled = 0;
             if (!ovf) {
+                // MulCheckedOverflow(value, 10^(outScale - inScale)) against the DECIMAL128 limits only, for both
+                // output types (sum_spark_decimal_aggregator.h:284-296)
                 int128 p = 1;
                 for (int k = 0; k < a.outScale - a.inScale; ++k) p *= 10;
                 ovf = MulChecked128(v, p, scaled);
-                if (!ovf && width == 8 && (scaled > std::numeric_limits<int64_t>::max() ||
-                                           scaled < std::numeric_limits<int64_t>::min())) {
-                    ovf = true;
-                }
             }
             if (ovf) {
                 SetNullBit(o, r);
                 return overflowAsNull;
             }
             if (width == 8) {
+                // SetValToVector: static_cast<int64_t> of the rescaled int128 (:580-589), i.e. its low 64 bits
                 static_cast<int64_t *>(o.values)[r] = static_cast<int64_t>(scaled);
             } else {
                 std::memcpy(static_cast<uint8_t *>(o.values) + 16 * r, &scaled, 16);
@@ -972,10 +971,19 @@ public:
             mn = std::min(mn, keys_[i]);
             mx = std::max(mx, keys_[i]);
         }
-        bool arrayOk = mx >= mn;
+        // Only INT / DATE32 / SHORT / LONG / TIMESTAMP keys try it (join_hash_table_variants.cpp:700-736), and
+        // max - min must not overflow the key type (TryToBuildArrayTable :792-794).
+        const bool candidate = keyType_ == OMNI_TYPE_INT || keyType_ == OMNI_TYPE_DATE32 ||
+                               keyType_ == OMNI_TYPE_SHORT || keyType_ == OMNI_TYPE_LONG ||
+                               keyType_ == OMNI_TYPE_TIMESTAMP;
+        bool arrayOk = candidate && mx >= mn;
         if (arrayOk) {
             int128 range = static_cast<int128>(mx) - mn;
-            arrayOk = range <= 0xFFFFFFFFLL && range <= static_cast<int128>(8) * len;
+            const int128 typeMax = keyType_ == OMNI_TYPE_SHORT ? 32767
+                                   : (keyType_ == OMNI_TYPE_INT || keyType_ == OMNI_TYPE_DATE32)
+                                       ? 2147483647
+                                       : static_cast<int128>(std::numeric_limits<int64_t>::max());
+            arrayOk = range <= typeMax && range <= 0xFFFFFFFFLL && range <= static_cast<int128>(8) * len;
         }
         if (arrayOk) {
             isArray_ = true;

@@ -92,7 +92,8 @@ def test_decimal_overflow_null_and_error():
     k = np.array([0, 0, 1, 1, 2], np.int64)
     aggs = [(SUM, DECIMAL64, DECIMAL64, 0, 2)]
     got = run_gpu([Col(LONG, k)], aggs, [Col(DECIMAL64, v)], 16)
-    assert sorted(got, key=lambda r: r[0]) == [(0, None), (1, 1200), (2, None)]
+    # group 2: 10^18 rescaled by 10^2 passes the DECIMAL128 check and keeps its low 64 bits (static_cast<int64_t>)
+    assert sorted(got, key=lambda r: r[0]) == [(0, None), (1, 1200), (2, 10**20 - 5 * 2**64)]
     assert_row_sets_equal(got, run_oracle([Col(LONG, k)], aggs, [Col(DECIMAL64, v)], 16))
     with pytest.raises(capi.GpuError) as e:
         run_gpu([Col(LONG, k)], aggs, [Col(DECIMAL64, v)], 16, overflow_as_null=False)
@@ -374,8 +375,9 @@ def test_array_rule_parity(t, strategy):
 
 
 def test_array_rule_declines():
-    """No direct-index map when the first batch's keys are all NULL or span more than 2^21 - 2 values; results are
-    unchanged."""
+    """No direct-index map when the first batch's keys are all NULL, span more than 2^21 - 2 values, or when the
+    range widened by max(50, max - min) on each side plus the NULL slot exceeds 2^21 entries
+    (vector_analyzer.cpp:46-77); results are unchanged."""
     aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
     v = Col(LONG, np.arange(4000, dtype=np.int64))
     k = np.arange(4000, dtype=np.int64) % 37
@@ -388,7 +390,16 @@ def test_array_rule_declines():
     v2 = Col(LONG, v.values[:2000])
     got, info = run_gpu_rule([Col(LONG, k2)], aggs, [v2, None], 2000)
     assert info == (False, 0, 0)
+    # span 2^21 - 2 passes the first check, but its widened range (3 * span + 1 cells + the NULL slot) does not
     k3 = np.array([0, 2**21 - 2] * 1000, np.int64)
     got, info = run_gpu_rule([Col(LONG, k3)], aggs, [v2, None], 2000)
-    assert info[0] and info[2] == 3 * (2**21 - 2) + 1
+    assert info == (False, 0, 0)
     assert_row_sets_equal(got, run_oracle([Col(LONG, k3)], aggs, [v2, None], 2000))
+    # the largest span whose widened range fits: 3 * 699050 + 1 cells + the NULL slot = 2^21 exactly
+    k4 = np.array([0, 699050] * 1000, np.int64)
+    got, info = run_gpu_rule([Col(LONG, k4)], aggs, [v2, None], 2000)
+    assert info == (True, -699050, 3 * 699050 + 1)
+    assert_row_sets_equal(got, run_oracle([Col(LONG, k4)], aggs, [v2, None], 2000))
+    k5 = np.array([0, 699051] * 1000, np.int64)
+    got, info = run_gpu_rule([Col(LONG, k5)], aggs, [v2, None], 2000)
+    assert info == (False, 0, 0)

@@ -159,23 +159,27 @@ def test_graph_replays_match_eager():
 
 
 def test_batches_decimal_overflow_null_and_error():
-    """SUM(decimal64, scale 0 -> 2): a group whose rescaled sum leaves decimal64 is NULL with overflowAsNull and
-    an error without it (the reference's checked decimal add), in each of two batches."""
+    """SUM(decimal64 scale 0 -> decimal128 scale 20): a group whose rescaled sum leaves +-(10^38 - 1) is NULL with
+    overflowAsNull and an error without it (MulCheckedOverflow, sum_spark_decimal_aggregator.h:284-296), in each of
+    two batches; the values equal the oracle's."""
     v = np.array([2**62, 2**62, 5, 7, 10**18] * 2, np.int64)
     k = np.array([0, 0, 1, 1, 2] * 2, np.int64)
     offsets = np.array([0, 5, 10], np.int64)
-    aggs = [(SUM, DECIMAL64, DECIMAL64, 0, 2)]
+    aggs = [(SUM, DECIMAL64, DECIMAL128, 0, 20)]
     kd = capi.DeviceColumn(LONG, k)
     vd = capi.DeviceColumn(DECIMAL64, v)
     s = capi.AggBatches(capi.make_agg_spec([LONG], aggs, overflow_as_null=True), 16)
     groups = s.run(kd.c(), [vd.c()], offsets)
     outs = s.outputs()
     s.close()
+    want = rows_of([(c.type_id, c.values, c.nulls) for c in oracle.hash_agg(
+        oracle.make_agg_spec([LONG], aggs), [Col(LONG, k[:5])], [Col(DECIMAL64, v[:5])], 5)])
+    assert sorted(want, key=lambda r: r[0]) == [(0, None), (1, 12 * 10**20), (2, None)]
     for b in range(2):
         a0, g = int(offsets[b]), int(groups[b])
         got = rows_of([(LONG, outs[0][0][a0:a0 + g], outs[0][1][a0:a0 + g]),
-                       (DECIMAL64, outs[1][0][a0:a0 + g], outs[1][1][a0:a0 + g])])
-        assert sorted(got, key=lambda r: r[0]) == [(0, None), (1, 1200), (2, None)]
+                       (DECIMAL128, outs[1][0][a0:a0 + g], outs[1][1][a0:a0 + g])])
+        assert sorted(got, key=lambda r: r[0]) == sorted(want, key=lambda r: r[0])
     s = capi.AggBatches(capi.make_agg_spec([LONG], aggs, overflow_as_null=False), 16)
     with pytest.raises(capi.GpuError) as e:
         s.run(kd.c(), [vd.c()], offsets)

@@ -86,6 +86,53 @@ def test_join_array_table_fixture(strategy):
     assert got == sorted(tuple(r) for r in f["expected_rows"])
 
 
+@pytest.mark.parametrize("key_type", [DOUBLE, oracle.DECIMAL64, BOOLEAN])
+def test_array_table_only_for_integer_key_types(key_type):
+    """JoinHashTableVariants::BuildHashTable tries the array table for INT / DATE32 / SHORT / LONG / TIMESTAMP keys
+    only (join_hash_table_variants.cpp:700-736): dense DOUBLE, DECIMAL64 and BOOLEAN keys build a hash table, on the
+    GPU as in the oracle, with the same matches."""
+    rng = np.random.default_rng(7 + key_type)
+    if key_type == BOOLEAN:
+        bkv, pkv = rng.integers(0, 2, 500).astype(np.uint8), rng.integers(0, 2, 2000).astype(np.uint8)
+    else:
+        ints = np.arange(1, 3001, dtype=np.int64)
+        pint = rng.integers(0, 3500, 20_000, dtype=np.int64)
+        bkv = ints.view(np.float64) if key_type == DOUBLE else ints
+        pkv = pint.view(np.float64) if key_type == DOUBLE else pint
+    bk = Col(key_type, bkv)
+    pk = Col(key_type, pkv)
+    pays = [Col(INT, np.arange(len(bkv), dtype=np.int32))]
+    rows, gp, info = gpu_join(bk, pays, pk, 1024, 4096)
+    want, is_array = oracle_set(bk, pays, pk, 1024, 4096)
+    assert not is_array and info["is_array"] == 0
+    assert as_set(rows, gp) == want
+
+
+def test_destroy_releases_device_memory():
+    """create / build / probe / destroy in a loop leaves the library's live device memory where it started: the
+    partitioned build (tables of more than 16,384 build rows), its overflow list and the outer join's visited
+    marks are all released."""
+    rng = np.random.default_rng(5)
+    bk = Col(LONG, rng.permutation(np.arange(40_000, dtype=np.int64)) * 7919)
+    pk = Col(LONG, bk.values[rng.integers(0, 40_000, 50_000)])
+    pays = [Col(INT, np.arange(40_000, dtype=np.int32))]
+    kd = capi.DeviceColumn(LONG, bk.values)
+    pd = capi.DeviceColumn(INT, pays[0].values)
+    qd = capi.DeviceColumn(LONG, pk.values)
+    used = []
+    for it in range(4):
+        j = capi.HashJoin(LONG, [INT], 0)
+        if it % 2:
+            j.set_type(capi.JOIN_FULL, capi.BUILD_RIGHT)
+        j.add_build(kd.c(), [pd.c()], len(bk))
+        j.finish()
+        assert j.info()["is_array"] == 0
+        j.probe(qd.c(), len(pk))
+        j.close()
+        used.append(capi.memory_stats()[0])
+    assert used[1] == used[0] and used[2] == used[0] and used[3] == used[0], used
+
+
 @pytest.mark.parametrize("dense", [False, True])
 @pytest.mark.parametrize("nulls", [False, True])
 @pytest.mark.parametrize("batched", [False, True])

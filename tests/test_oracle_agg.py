@@ -91,10 +91,54 @@ def test_decimal_overflow_semantics():
     with pytest.raises(OverflowError):
         oracle.hash_agg(oracle.make_agg_spec([LONG], aggs, overflow_as_null=False), [Col(LONG, k)],
                         [Col(DECIMAL64, v)], 1024)
-    # scale-up overflow at extraction: 10^18 * 10^2 does not fit a decimal64
+    # The rescale at extraction is MulCheckedOverflow against the DECIMAL128 limits only; a DECIMAL64 output then
+    # keeps static_cast<int64_t> of the result (sum_spark_decimal_aggregator.h:284-296, 580-589): 10^18 * 10^2 =
+    # 10^20, whose low 64 bits are 10^20 - 5 * 2^64 = 7766279631452241920.
     out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, DECIMAL64, DECIMAL64, 0, 2)]),
                           [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL64, np.array([10**18], np.int64))], 16)
+    assert cols_rows(out) == [(0, 7766279631452241920)]
+    assert 10**20 - 5 * 2**64 == 7766279631452241920
+    # past the DECIMAL128 limits the rescale overflows: 10^37 * 10^2 > 10^38 - 1
+    big = brute.int_to_dec128([10**37])
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, DECIMAL128, DECIMAL128, 0, 2)]),
+                          [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL128, big)], 16)
     assert cols_rows(out) == [(0, None)]
+
+
+I64_MAX = 2**63 - 1
+DEC38 = 10**38 - 1
+
+
+@pytest.mark.parametrize("tin,tout,seq,expect", [
+    # SUM(decimal64 -> decimal64) adds with __builtin_add_overflow in row order and the OVERFLOWED state sticks
+    # (AddDecimalUseRowIndex / SumOp, sum_spark_decimal_aggregator.h:25-62, sum_aggregator.h:36-54): the same
+    # values in another order may not overflow
+    (DECIMAL64, DECIMAL64, [I64_MAX, 1, -2], None),
+    (DECIMAL64, DECIMAL64, [I64_MAX, -2, 1], I64_MAX - 1),
+    (DECIMAL64, DECIMAL64, [-2, I64_MAX, 1], I64_MAX - 1),
+    (DECIMAL64, DECIMAL64, [-I64_MAX - 1, -1, 5], None),
+    (DECIMAL64, DECIMAL64, [-I64_MAX - 1, 5, -1], -I64_MAX + 3),
+    # SUM(decimal128 -> decimal128): AddCheckedOverflow against +-(10^38 - 1), sticky
+    (DECIMAL128, DECIMAL128, [DEC38, 1, -2], None),
+    (DECIMAL128, DECIMAL128, [DEC38, -2, 1], DEC38 - 1),
+    (DECIMAL128, DECIMAL128, [-DEC38, -1, 3, 7], None),
+    (DECIMAL128, DECIMAL128, [-DEC38, 3, -1, 7], -DEC38 + 9),
+    # decimal64 inputs into a decimal128 sum cannot leave +-(10^38 - 1)
+    (DECIMAL64, DECIMAL128, [I64_MAX, 1, -2], I64_MAX - 1),
+])
+def test_decimal_sum_sticky_overflow_in_row_order(tin, tout, seq, expect):
+    vals = brute.int_to_dec128(seq) if tin == DECIMAL128 else np.array(seq, np.int64)
+    aggs = [(SUM, tin, tout, 0, 0)]
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], aggs, overflow_as_null=True),
+                          [Col(LONG, np.zeros(len(seq), np.int64))], [Col(tin, vals)], 2)
+    assert cols_rows(out) == [(0, expect)]
+    if expect is None:
+        with pytest.raises(OverflowError):
+            oracle.hash_agg(oracle.make_agg_spec([LONG], aggs, overflow_as_null=False),
+                            [Col(LONG, np.zeros(len(seq), np.int64))], [Col(tin, vals)], 2)
+    # the sort-based brute force follows the same row-order rule
+    b = brute.group_by([Col(LONG, np.zeros(len(seq), np.int64))], aggs, [Col(tin, vals)], overflow_as_null=True)
+    assert cols_rows(b) == [(0, expect)]
 
 
 def _random_case(seed, n, key_kind, with_nulls):
@@ -172,3 +216,27 @@ def test_partial_avg_output_definition():
         sel = (k == g) & ~nulls
         assert not psum.nulls[i] and psum.values[i] == float(v[sel].sum())
         assert mean.nulls[i] == (sel.sum() == 0)
+
+
+@pytest.mark.parametrize("tin,tout", [(DECIMAL64, DECIMAL64), (DECIMAL128, DECIMAL128), (DECIMAL64, DECIMAL128)])
+def test_decimal_near_limit_oracle_equals_brute(tin, tout):
+    """Mixed-sign values near the limits over many groups and batches: the oracle's Swiss-table aggregation and the
+    sort-based brute force apply the same row-order checked add and give the same rows."""
+    rng = np.random.default_rng(tin * 10 + tout)
+    n, groups = 6000, 600
+    k = Col(LONG, rng.integers(0, groups, n, dtype=np.int64))
+    sign = np.where(rng.random(n) < 0.5, 1, -1)
+    if tin == DECIMAL64:
+        v = Col(DECIMAL64, np.where(rng.random(n) < 0.8, sign * rng.integers(2**60, 2**62, n, dtype=np.int64),
+                                    rng.integers(-1000, 1000, n, dtype=np.int64)).astype(np.int64),
+                rng.random(n) < 0.1)
+    else:
+        vals = [int(s) * int(x) * 10**19 for s, x in zip(sign, rng.integers(10**17, 4 * 10**18, n))]
+        v = Col(DECIMAL128, brute.int_to_dec128(vals), rng.random(n) < 0.1)
+    aggs = [(SUM, tin, tout, 0, 0), (SUM, tin, tout, 0, 2)]
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], aggs), [k], [v, v], 700)
+    got = cols_rows(out)
+    want = cols_rows(brute.group_by([k], aggs, [v, v]))
+    assert_row_sets_equal(got, want)
+    if tout == DECIMAL64 or tin == DECIMAL128:
+        assert 0 < sum(r[1] is None for r in got) < len(got)
