@@ -344,6 +344,14 @@ typedef struct omni_gpu_sketch {
  * the rows, at least 4,096 (all rows below that) and at most 65,536. */
 int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
                          omni_gpu_sketch *out);
+/* omni_gpu_sketch_keys without any device allocation, for callers that cannot allocate: code under HIP graph
+ * capture, tight serving loops.  No counterpart in the reference.  The same sketch is computed on `stream` (a
+ * hipStream_t; NULL = the null stream) into *out, a DEVICE pointer to an omni_gpu_sketch; nothing is copied to the
+ * host and nothing synchronizes.  The sample's table, counts and statistics live in a fixed device array of the
+ * library (1.5 MiB per GPU, held until the process ends; a sample has at most 65,536 rows, so every row count
+ * fits it), which every call on a GPU reuses and clears with kernels: calls that overlap on one GPU race. */
+int omni_gpu_sketch_keys_noalloc(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
+                                 void *stream, omni_gpu_sketch *out);
 /* Sketch + MLP inference in one call on the MLP handle's stream (one synchronisation): y[4] receives the network's
  * outputs for the key columns (y[0] * 32 = predicted log2(groups + 1); y[1..3] strategy logits). */
 int omni_gpu_advise(void *mlp, const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,

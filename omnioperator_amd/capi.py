@@ -117,6 +117,7 @@ SIGNATURES = {
     "omni_gpu_join_destroy": (_I, [_VP]),
     "omni_gpu_partition_ids": (_I, [_P(Column), _I32, _I64, _I32, _VP]),
     "omni_gpu_sketch_keys": (_I, [_P(Column), _I32, _I64, _I32, _P(Sketch)]),
+    "omni_gpu_sketch_keys_noalloc": (_I, [_P(Column), _I32, _I64, _I32, _VP, _VP]),
     "omni_gpu_advise": (_I, [_VP, _P(Column), _I32, _I64, _I32, _P(C.c_float)]),
     "omni_gpu_mlp_create": (_I, [C.c_uint64, _P(_VP)]),
     "omni_gpu_mlp_num_params": (_I, []),

@@ -9,7 +9,10 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <map>
+#include <mutex>
 #include <numeric>
+#include <optional>
 #include <random>
 #include <vector>
 
@@ -30,61 +33,116 @@ struct SketchBuffers {
 };
 
 constexpr int64_t kMaxSample = 1 << 16;  // sampled rows at most: the sketch's cost stays flat
+constexpr uint64_t kMaxSketchSlots = 2 * kMaxSample;  // table slots of the largest sample
 
-// Launches the sketch of `keys` on stream s: a 1-in-k sample of about 1% of the rows (at least 4096 or all of
-// them, at most 2^21) counted in a hash table, reduced to statistics and turned into the 16 MLP features.
-int RunSketch(hipStream_t s, SketchBuffers &b, const omni_gpu_column *keys, int num_keys, int64_t rows, int is_join,
-              int64_t *sampled)
+// The sketch of one call: its arguments, table slots and the features' inputs.
+struct SketchPlan {
+    SketchArgs a{};
+    uint64_t cap = 0;
+    int key_bytes = 0;
+};
+
+// Validates the key columns and plans a 1-in-k sample of about 1% of the rows (at least 4096 or all of them, at
+// most kMaxSample) counted in a hash table of cap >= 2 * m slots.
+int PlanSketch(const omni_gpu_column *keys, int num_keys, int64_t rows, SketchPlan &p)
 {
     OG_CHECK(keys, OMNI_GPU_ERR_INVALID, "null key columns");
     OG_CHECK(num_keys >= 1 && num_keys <= 2, OMNI_GPU_ERR_INVALID, "the sketch reads one or two key columns");
     OG_CHECK(rows >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
-    SketchArgs a{};
+    SketchArgs &a = p.a;
     a.ncols = num_keys;
-    int keyBytes = 0;
+    p.key_bytes = 0;
     for (int c = 0; c < num_keys; ++c) {
         a.width[c] = TypeWidth(keys[c].type_id);
         OG_CHECK(a.width[c] > 0 && a.width[c] <= 8 && (rows == 0 || keys[c].values), OMNI_GPU_ERR_INVALID,
                  "sketch key %d must be an integer-like column of <= 8 bytes", c);
         a.values[c] = keys[c].values;
         a.nulls[c] = keys[c].nulls;
-        keyBytes += a.width[c];
+        p.key_bytes += a.width[c];
     }
     a.rows = rows;
     int64_t m = std::min<int64_t>(rows, std::max<int64_t>(4096, rows / 100));
     m = std::min<int64_t>(m, kMaxSample);
     a.stride = m > 0 ? std::max<int64_t>(1, rows / m) : 1;
     a.m = m;
-    *sampled = m;
-    uint64_t cap = 1024;
-    while (cap < static_cast<uint64_t>(2 * m)) cap <<= 1;
-    OG_TRY(b.hkeys.Reserve(8 * cap, s));
-    OG_TRY(b.counts.Reserve(4 * cap, s));
-    OG_TRY(b.stats.Reserve(sizeof(SketchStats), s));
-    OG_TRY(b.features.Reserve(sizeof(float) * kIn, s));
-    SketchStats init{};
-    init.kmin = std::numeric_limits<long long>::max();
-    init.kmax = std::numeric_limits<long long>::min();
-    OG_HIP(hipMemsetAsync(b.hkeys.p, 0, 8 * cap, s));
-    OG_HIP(hipMemsetAsync(b.counts.p, 0, 4 * cap, s));
-    OG_TRY(CopyToDevice(b.stats.p, &init, sizeof(init), s));
-    a.hkeys = b.hkeys.as<unsigned long long>();
-    a.counts = b.counts.as<unsigned int>();
-    a.mask = cap - 1;
-    a.stats = b.stats.as<SketchStats>();
-    if (m > 0) {
-        KernelScope ks(s, "sketch_insert");
-        sketch_insert<<<static_cast<unsigned int>((m + 255) / 256), 256, 0, s>>>(a);
+    p.cap = 1024;
+    while (p.cap < static_cast<uint64_t>(2 * m)) p.cap <<= 1;
+    return OMNI_GPU_OK;
+}
+
+// Launches the planned sketch on stream s into the given table and statistics: clear, count the sample, reduce,
+// features.  `timed` brackets the launches with the library's kernel timers.
+int LaunchSketch(hipStream_t s, SketchPlan &p, unsigned long long *hkeys, unsigned int *counts, SketchStats *stats,
+                 float *features, int64_t rows, int num_keys, int is_join, bool timed)
+{
+    SketchArgs &a = p.a;
+    a.hkeys = hkeys;
+    a.counts = counts;
+    a.mask = p.cap - 1;
+    a.stats = stats;
+    const unsigned int tableBlocks = static_cast<unsigned int>((p.cap + 255) / 256);
+    sketch_clear<<<tableBlocks, 256, 0, s>>>(hkeys, counts, p.cap, stats);
+    OG_TRY(CheckLaunch("sketch_clear"));
+    if (a.m > 0) {
+        std::optional<KernelScope> ks;
+        if (timed) ks.emplace(s, "sketch_insert");
+        sketch_insert<<<static_cast<unsigned int>((a.m + 255) / 256), 256, 0, s>>>(a);
         OG_TRY(CheckLaunch("sketch_insert"));
     }
     {
-        KernelScope ks(s, "sketch_reduce");
-        sketch_reduce<<<static_cast<unsigned int>((cap + 255) / 256), 256, 0, s>>>(a, cap);
+        std::optional<KernelScope> ks;
+        if (timed) ks.emplace(s, "sketch_reduce");
+        sketch_reduce<<<tableBlocks, 256, 0, s>>>(a, p.cap);
         OG_TRY(CheckLaunch("sketch_reduce"));
     }
-    sketch_features<<<1, 64, 0, s>>>(b.stats.as<SketchStats>(), rows, m, keyBytes, num_keys, is_join,
-                                     b.features.as<float>());
+    sketch_features<<<1, 64, 0, s>>>(stats, rows, a.m, p.key_bytes, num_keys, is_join, features);
     return CheckLaunch("sketch_features");
+}
+
+// Launches the sketch of `keys` on stream s into the buffers b (grown as needed).
+int RunSketch(hipStream_t s, SketchBuffers &b, const omni_gpu_column *keys, int num_keys, int64_t rows, int is_join,
+              int64_t *sampled)
+{
+    SketchPlan p;
+    OG_TRY(PlanSketch(keys, num_keys, rows, p));
+    *sampled = p.a.m;
+    OG_TRY(b.hkeys.Reserve(8 * p.cap, s));
+    OG_TRY(b.counts.Reserve(4 * p.cap, s));
+    OG_TRY(b.stats.Reserve(sizeof(SketchStats), s));
+    OG_TRY(b.features.Reserve(sizeof(float) * kIn, s));
+    return LaunchSketch(s, p, b.hkeys.as<unsigned long long>(), b.counts.as<unsigned int>(), b.stats.as<SketchStats>(),
+                        b.features.as<float>(), rows, num_keys, is_join, true);
+}
+
+// ---- the allocation-free sketch (omni_gpu_sketch_keys_noalloc) ------------------------------------------------
+// Its table, counts, statistics and features live in `scratch`, a file-scope device array of fixed size that
+// holds the largest sample's table (kMaxSketchSlots slots): 8 + 4 bytes per slot, then the statistics and the
+// features.  Each GPU the library runs on holds its own copy until the process ends.  Host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol): the device address of each GPU's
+// copy comes from hipGetSymbolAddress, once per device, with that device current.
+constexpr size_t kScratchStatsOff = 12 * kMaxSketchSlots;
+constexpr size_t kScratchFeatOff = kScratchStatsOff + 256;
+constexpr size_t kScratchBytes = kScratchFeatOff + 256;
+static_assert(sizeof(SketchStats) <= 256 && sizeof(float) * kIn <= 256, "sketch scratch layout");
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+std::mutex g_scratch_mu;
+std::map<int, uint8_t *> g_scratch_ptr;  // device -> address of that device's scratch
+
+int ScratchOf(int dev, uint8_t **p)
+{
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) {
+        *p = it->second;
+        return OMNI_GPU_OK;
+    }
+    void *ptr = nullptr;
+    OG_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    OG_CHECK(ptr != nullptr, OMNI_GPU_ERR_HIP, "no device address for the sketch scratch");
+    g_scratch_ptr[dev] = static_cast<uint8_t *>(ptr);
+    *p = static_cast<uint8_t *>(ptr);
+    return OMNI_GPU_OK;
 }
 
 struct MlpHandle {
@@ -168,6 +226,27 @@ int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t 
     out->key_min = st.kmin;
     out->key_max = st.kmax;
     return OMNI_GPU_OK;
+}
+
+int omni_gpu_sketch_keys_noalloc(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
+                                 void *stream, omni_gpu_sketch *out)
+{
+    OG_CHECK(out, OMNI_GPU_ERR_INVALID, "null output");
+    SketchPlan p;
+    OG_TRY(PlanSketch(keys, num_keys, rows, p));
+    OG_CHECK(p.cap <= kMaxSketchSlots, OMNI_GPU_ERR_INVALID,
+             "sketch table of %llu slots exceeds the scratch's %llu slots", static_cast<unsigned long long>(p.cap),
+             static_cast<unsigned long long>(kMaxSketchSlots));
+    uint8_t *sc = nullptr;
+    OG_TRY(ScratchOf(CurrentDevice(), &sc));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    unsigned long long *hkeys = reinterpret_cast<unsigned long long *>(sc);
+    unsigned int *counts = reinterpret_cast<unsigned int *>(sc + 8 * kMaxSketchSlots);
+    SketchStats *stats = reinterpret_cast<SketchStats *>(sc + kScratchStatsOff);
+    float *features = reinterpret_cast<float *>(sc + kScratchFeatOff);
+    OG_TRY(LaunchSketch(s, p, hkeys, counts, stats, features, rows, num_keys, is_join, false));
+    sketch_publish<<<1, 64, 0, s>>>(stats, features, rows, p.a.m, out);
+    return CheckLaunch("sketch_publish");
 }
 
 int omni_gpu_advise(void *mlp, const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,

@@ -312,6 +312,43 @@ __global__ void __launch_bounds__(256) sketch_reduce(SketchArgs a, unsigned long
     }
 }
 
+// Empties a sketch table of `cap` slots and seeds the statistics.  A kernel rather than memsets, so that the
+// sketch's launch sequence holds no memset node when a caller captures it in a graph.
+__global__ void __launch_bounds__(256) sketch_clear(unsigned long long *hkeys, unsigned int *counts,
+                                                    unsigned long long cap, SketchStats *stats)
+{
+    const unsigned long long i = static_cast<unsigned long long>(blockIdx.x) * 256 + threadIdx.x;
+    if (i < cap) {
+        hkeys[i] = 0;
+        counts[i] = 0;
+    }
+    if (i == 0) {
+        SketchStats init{};
+        init.kmin = std::numeric_limits<long long>::max();
+        init.kmax = std::numeric_limits<long long>::min();
+        *stats = init;
+    }
+}
+
+// Copies the sketch of `rows` rows (m sampled) into a device-resident omni_gpu_sketch.
+__global__ void sketch_publish(const SketchStats *st, const float *f, int64_t rows, int64_t m, omni_gpu_sketch *out)
+{
+    const int t = threadIdx.x;
+    if (t < kIn) out->features[t] = f[t];
+    if (t == 0) {
+        out->rows = rows;
+        out->sampled = m;
+        out->distinct = static_cast<int64_t>(st->distinct);
+        out->f1 = static_cast<int64_t>(st->f1);
+        out->f2 = static_cast<int64_t>(st->f2);
+        out->max_count = static_cast<int64_t>(st->maxc);
+        out->heavy = static_cast<int64_t>(st->heavy);
+        out->nulls = static_cast<int64_t>(st->nulls);
+        out->key_min = st->kmin;
+        out->key_max = st->kmax;
+    }
+}
+
 // The MLP's 16 input features from the sketch statistics (all roughly in [0, 1]).  One thread; shared by the
 // sketch entry point and the advisor so that training and inference see identical features.
 __global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, int key_bytes, int num_keys,
