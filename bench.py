@@ -1,16 +1,19 @@
 #!/usr/bin/env python3
-"""Benchmark of the MI355X hash-join / hash-aggregation hot path.
+"""Benchmark of the MI355X hash-aggregation / hash-join hot path.
 
-Default workload (the configuration the headline metric is quoted on): inner hash join, build side
-2,000,000 unique int64 keys + int32 payload, probe side 64,000,000 int64 keys (80% hit build keys,
-20% miss), VectorBatches of 65,536 rows.  One step = build (31 build batches appended + table build)
-+ probe of all 977 probe batches in one launch, writing (probe row id int64, build payload int32)
-per match into device-resident output buffers.  Inputs are resident in HBM before timing starts.
+Default workload (the configuration the headline metric, "learned+batched vs plain GPU agg/join", is quoted on):
+the batched stream of 20,000 independent group-bys -- VectorBatches of lognormal row counts (median 2048, clipped
+to [64, 65536], 102,826,994 rows), int64 keys with a per-batch distinct count log-uniform in [8, 1e6], SUM(int64) +
+COUNT(*) -- in its learned + batched form: a per-batch sketch through the stream advisor network picks each batch's
+first phase, the autotuner picks the tiles, and the whole run is one hipGraph launch.  One step = one run over all
+20,000 batches with every group written to device-resident output columns.  Inputs are resident in HBM before
+timing starts.  The same process times the plain form (one launch per batch and phase) and the fixed-strategy
+batched forms for the speed-ups in `config`.
 
-Prints one JSON line with rows/s (build + probe rows per second), the probe kernel's roofline figure
-(algorithmic bytes / average kernel duration from HIP events on the kernel's stream) and a CPU
-baseline: the oracle (the C++ restatement of the reference CPU operators, one core) on a bounded
-sample of the same workload.
+Prints one JSON line with rows/s, the run's roofline figure (algorithmic bytes / the graph launch's duration from
+HIP events on its stream) and a CPU baseline: the oracle (the C++ restatement of the reference operator, one
+operator per batch) over the same stream on all host cores and on one core.  --workload selects the other
+configurations (join, agg, skewed, learned, partition), each a line of the same form.
 """
 from __future__ import annotations
 
@@ -228,7 +231,7 @@ def bench_partition(args):
 def bench_agg_plain(args):
     """1M rows, int64 key over 1,000 values, VectorBatches of 4096; SUM/COUNT/MIN/MAX."""
     from omnioperator_amd import capi, workloads
-    from oracle import COUNT_ALL, LONG, MAX, MIN, SUM
+    from omnioperator_amd.capi import COUNT_ALL, LONG, MAX, MIN, SUM
     capi.init(0)
     w = workloads.agg_plain()
     n = len(w.keys)
@@ -285,7 +288,7 @@ def bench_agg_plain(args):
 def bench_agg_skewed(args):
     """200M rows, key (int32 U[1e4], int64 Zipf 1.1 over 5e7); SUM(dec64->dec128), SUM/AVG(double), COUNT, MAX(int32)."""
     from omnioperator_amd import capi, workloads
-    from oracle import AVG, COUNT_ALL, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, SUM
+    from omnioperator_amd.capi import AVG, COUNT_ALL, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, SUM
     capi.init(0)
     w = workloads.agg_skewed(rows=args.skewed_rows)
     n = len(w.k1)
@@ -334,51 +337,133 @@ def bench_agg_skewed(args):
 
 
 def bench_batched(args):
-    """20,000 independent group-bys (lognormal rows, median 2048), SUM + COUNT(*), one run over the stream."""
-    from omnioperator_amd import capi, workloads
-    from oracle import COUNT_ALL, LONG, SUM
+    """The headline configuration: 20,000 independent group-bys (lognormal rows, median 2048, clipped to
+    [64, 65536]; per-batch distinct keys log-uniform in [8, 1e6]), SUM(int64) + COUNT(*), one run over the stream.
+    Default form "learned": the learned + batched form -- the stream advisor (a per-batch sketch through the MLP,
+    on the device) picks each batch's first phase, tiles from the autotuner, the whole run one hipGraph launch.
+    It is compared in the same process with the plain form (one launch per batch and phase, the reference's one
+    operator per VectorBatch) and with the fixed-strategy batched forms (every batch LDS-first, every batch
+    partitioned) at the same tiles."""
+    from omnioperator_amd import autotune, batch_advisor, capi, workloads
     capi.init(0)
     w = workloads.batched_stream()
     offsets = np.concatenate([[0], np.cumsum(w.row_counts)]).astype(np.int64)
+    nb = len(w.row_counts)
     n = int(offsets[-1])
-    kd, vd = capi.DeviceColumn(LONG, w.keys), capi.DeviceColumn(LONG, w.values)
-    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
-    s = capi.AggBatches(capi.make_agg_spec([LONG], aggs), 65536)
-    flags = {"graph": capi.AggBatches.FLAG_GRAPH, "two_launches": 0, "per_batch": capi.AggBatches.FLAG_PER_BATCH}[args.batched_form]
+    kd, vd = capi.DeviceColumn(capi.LONG, w.keys), capi.DeviceColumn(capi.LONG, w.values)
+    aggs = [(capi.SUM, capi.LONG, capi.LONG), (capi.COUNT_ALL, capi.LONG, capi.LONG)]
+    spec = capi.make_agg_spec([capi.LONG], aggs)
+    s = capi.AggBatches(spec, 65536)
+    F = capi.AggBatches
+    forms = {"learned": F.FLAG_GRAPH | F.FLAG_ADVISOR, "graph": F.FLAG_GRAPH, "two_launches": 0,
+             "per_batch": F.FLAG_PER_BATCH}
+    flags = forms[args.batched_form]
+    net = None
+    if args.batched_form == "learned":
+        net = batch_advisor.network()
+        s.set_advisor(net)
+    all_part = np.full(nb, 2, np.int32)
+
+    def run(fl, modes=None):
+        return s.run(kd.c(), [vd.c(), None], offsets, modes, fl)
 
     def step():
-        g = s.run(kd.c(), [vd.c(), None], offsets, None, flags)
-        return int(g.sum())
+        return int(run(flags).sum())
 
     tuned = None
-    from omnioperator_amd import autotune
     if args.tiles:  # fixed tiles, e.g. for a profile of one configuration
         b, l, p_, q = (int(x) for x in args.tiles.split(","))
         tuned = autotune.StreamConfig(b, l, p_, q)
         autotune.apply(s, tuned)
     elif args.batched_form != "per_batch":  # the plain one-launch-per-batch form keeps the default tiles
-        key = autotune.key_of(LONG, aggs, autotune.cardinality_bucket(float(np.mean(w.distinct))))
+        key = autotune.key_of(capi.LONG, aggs, autotune.cardinality_bucket(float(np.mean(w.distinct)))) + \
+            f"|{args.batched_form}"
         tuned = autotune.tune_stream(s, lambda: (step(), capi.synchronize()), key, use_cache=False)
 
     groups, ms = timed_steps(step, args, capi)
-    st = s.stats()
+    prof = {"learned": "agg_batches_graph", "graph": "agg_batches_graph", "two_launches": "agg_batches",
+            "per_batch": "agg_batches_per_batch"}[args.batched_form]
+    n_launch, gpu_ms = capi.profile_get(prof)
+    run_ms = gpu_ms / max(1, n_launch)
     ph = s.phases()
+    assert groups == 41_697_991, f"groups {groups}"
+
+    compare = {}
+    if not args.no_compare and args.batched_form == "learned":
+        def timed(fl, modes=None, steps=args.steps):
+            run(fl, modes)
+            capi.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                g = int(run(fl, modes).sum())
+            capi.synchronize()
+            assert g == groups
+            return (time.perf_counter() - t0) * 1e3 / steps
+        compare["fixed_lds_first_ms"] = timed(F.FLAG_GRAPH)
+        compare["fixed_partitioned_ms"] = timed(F.FLAG_GRAPH, all_part)
+        best_fixed = min(compare["fixed_lds_first_ms"], compare["fixed_partitioned_ms"])
+        autotune.apply(s, autotune.DEFAULT)
+        compare["plain_one_launch_per_batch_ms"] = timed(F.FLAG_PER_BATCH, steps=1)
+        if tuned is not None:
+            autotune.apply(s, tuned)
+        compare["speedup_vs_plain"] = compare["plain_one_launch_per_batch_ms"] / ms
+        compare["speedup_vs_best_fixed_batched"] = best_fixed / ms
+        compare = {k: round(v, 4) for k, v in compare.items()}
     s.close()
+    if net is not None:
+        net.close()
     alg = n * 16 + groups * 24
-    cfg = {"workload": f"batched group-by: {len(w.row_counts)} independent VectorBatches, {n} rows (lognormal rows, "
-                       "median 2048, [64, 65536]), per-batch distinct keys log-uniform in [8, 1e6]; SUM + COUNT(*)",
-           "form": args.batched_form, "rows": n, "groups": groups, "lds_batches": st["lds_batches"],
-           "hbm_batches": st["hbm_batches"],
+    achieved = alg / (run_ms / 1e3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": profiled_traffic("stream (all kernels)", "batched_groupby_c4.md")
+            if args.batched_form == "learned" and not args.tiles else None,
+            "kernel": "the stream's launch sequence (one hipGraph launch: advisor sketch + MLP, per-batch LDS table, "
+                      "scan, split, partition tables, exact fallback)" if flags & F.FLAG_GRAPH else prof,
+            "kernel_ms": round(run_ms, 4), "algorithmic_bytes": alg}
+    cpu = cpu_baseline_stream(w, offsets, spec) if not args.no_cpu else None
+    cfg = {"workload": f"batched group-by: {nb} independent VectorBatches, {n} rows (lognormal rows, median 2048, "
+                       "[64, 65536]), per-batch distinct keys log-uniform in [8, 1e6]; SUM(int64) + COUNT(*)",
+           "form": {"learned": "learned + batched (stream advisor, autotuned tiles, hipGraph)",
+                    "graph": "batched, every batch LDS-first (hipGraph)", "two_launches": "batched, no graph",
+                    "per_batch": "plain: one launch per batch and phase"}[args.batched_form],
+           "rows": n, "batches": nb, "groups": groups,
            "batches_by_phase": {"lds_table": ph[0], "lds_partitions": ph[1], "hbm_table": ph[2]},
            "tiles": None if tuned is None else {"block": tuned.block, "lds_bytes": tuned.lds_bytes,
                                                 "max_probe": tuned.max_probe, "part_lds_bytes": tuned.part_lds_bytes},
-           "algorithmic_bytes_per_step": alg}
-    achieved = alg / (ms / 1e3) / 1e9
-    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "batch_lds + batch_hbm (whole run)",
-            "kernel_ms": round(ms, 4), "algorithmic_bytes": alg}
-    return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": None,
+           "algorithmic_bytes_per_step": alg, "bytes_per_row_in": 16, "bytes_per_group_out": 24}
+    cfg.update(compare)
+    return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
             "dtype": "int64"}
+
+
+def cpu_baseline_stream(w, offsets, spec):
+    """The oracle (the C++ restatement of HashAggregationOperator, one operator per batch) over the same stream:
+    all host cores (P threads taking batches from a shared counter) on the full stream, and one core on the first
+    2,000 batches; each a warm-up run, then the best of 3."""
+    import oracle
+    key, val = oracle.Col(oracle.LONG, w.keys), oracle.Col(oracle.LONG, w.values)
+    ospec = oracle.make_agg_spec([oracle.LONG], [(oracle.SUM, oracle.LONG, oracle.LONG),
+                                                 (oracle.COUNT_ALL, oracle.LONG, oracle.LONG)])
+
+    def best(offs, threads):
+        times = []
+        for i in range(4):
+            t0 = time.perf_counter()
+            oracle.agg_stream(ospec, key, [val, None], offs, threads)
+            if i:
+                times.append(time.perf_counter() - t0)
+        return min(times)
+
+    P = max(1, min(16, os.cpu_count() or 1))
+    t_all = best(offsets, P)
+    sub = offsets[:2001]
+    t_one = best(sub, 1)
+    return {"value": float(offsets[-1]) / t_all, "unit": "rows/s", "cores": P, "kind": "port",
+            "sample": f"the full stream ({len(offsets) - 1} batches, {offsets[-1]} rows) on {P} threads, one "
+                      f"operator per batch, warm-up + best of 3; host CPU {cpu_model()}",
+            "single_core": {"value": float(sub[-1]) / t_one, "unit": "rows/s", "cores": 1,
+                            "sample": f"first 2000 batches ({sub[-1]} rows), warm-up + best of 3"}}
 
 
 def bench_learned(args):
@@ -466,12 +551,16 @@ def main():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
-    p.add_argument("--workload", default="join", choices=["join", "agg", "skewed", "batched", "learned", "partition"],
-                   help="join: the headline inner-join configuration; agg: 1M-row plain group-by; skewed: 200M-row "
-                        "composite-key group-by; batched: 20,000-batch stream; learned: the 512-task advisor set")
+    p.add_argument("--workload", default="batched",
+                   choices=["join", "agg", "skewed", "batched", "learned", "partition"],
+                   help="batched (default): the 20,000-batch mixed-cardinality stream, learned + batched form; join: "
+                        "2M x 64M inner join; agg: 1M-row plain group-by; skewed: 200M-row composite-key group-by; "
+                        "learned: the 512-task advisor set; partition: shuffle partition ids")
     p.add_argument("--skewed-rows", type=int, default=200_000_000)
     p.add_argument("--cpu-skewed-rows", type=int, default=4_000_000)
-    p.add_argument("--batched-form", default="graph", choices=["graph", "two_launches", "per_batch"])
+    p.add_argument("--batched-form", default="learned", choices=["learned", "graph", "two_launches", "per_batch"])
+    p.add_argument("--no-compare", action="store_true",
+                   help="batched: skip the plain and fixed-strategy runs beside the learned form")
     p.add_argument("--tiles", default=None, help="batched: fixed block,lds_bytes,max_probe,part_lds_bytes "
                                                  "instead of the autotuner")
     p.add_argument("--learned-tasks", type=int, default=512)

@@ -214,13 +214,27 @@ int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_by
  * at phase 2 (e.g. from a predicted group count).  flags: bit 0 = one launch per batch and phase instead of
  * one launch per phase; bit 1 = capture the launch sequence in a hipGraph and replay it while the stream's
  * shape stays the same; bit 2 (without batch_modes, not with bit 0) = a sketch launch estimates each batch's
- * distinct keys from its first 1,024 rows and starts the batches estimated above the LDS table at phase 2.  Batch
- * b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
+ * distinct keys from its first 1,024 rows and starts the batches estimated above the LDS table at phase 2;
+ * bit 3 (same conditions) = the advisor network picks each batch's first phase
+ * (omni_gpu_agg_batches_set_advisor).  Batch b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
  * then aggregates); outs hold batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.
  * Synchronizes. */
 int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,
                              const int64_t *batch_offsets, int32_t num_batches, const int32_t *batch_modes,
                              int32_t flags, omni_gpu_out_column *outs, int64_t *groups_per_batch);
+/* The learned form of the stream: with flag bit 3 in omni_gpu_agg_batches_run, a per-batch sketch (an evenly
+ * strided sample of at most 512 rows of each batch, the 16 features of omni_gpu_sketch) goes through the advisor
+ * network `mlp` (an omni_gpu_mlp_create handle, trained on runs of this stream) predicts its group count, and a
+ * batch predicted above half the per-batch LDS table's slots starts in the hash-partitioned passes.  The
+ * launches stay on the device (no synchronisation), so they are captured with the rest under flag bit 1.  Results
+ * do not depend on the choice.  mlp NULL detaches the advisor. */
+int omni_gpu_agg_batches_set_advisor(void *handle, void *mlp);
+/* Phase that finished each batch of the last run (1, 2 or 3; phase[num_batches], host). */
+int omni_gpu_agg_batches_phase_of(void *handle, int32_t *phase);
+/* The advisor's features of each batch of a stream (the sketch of flag bit 3): features[num_batches * 16] (host)
+ * for the key column `key` (device) cut by batch_offsets (host).  Synchronizes. */
+int omni_gpu_batch_features(const omni_gpu_column *key, const int64_t *batch_offsets, int32_t num_batches,
+                            float *features);
 /* Batches of the last run finished by the per-batch LDS table (phase 1) / by a later phase. */
 int omni_gpu_agg_batches_stats(void *handle, int64_t *lds_batches, int64_t *hbm_batches);
 /* Batches of the last run finished by each phase: 1 the per-batch LDS table, 2 the hash-partitioned LDS
@@ -299,7 +313,8 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
  *   LEFT / RIGHT with the build on the same side, FULL (ProbeBatchForSameSideOuterJoin, ...ForFullJoin):
  *        the matched build rows are also marked for omni_gpu_join_outer_output;
  *   EXISTENCE: probed with omni_gpu_join_probe_exists only.
- * These types take the direct probe (no radix partitioning). */
+ * They are probed like the inner join: the radix-partitioned window probe for large hash tables (omni_gpu_join_tune),
+ * the direct probe for array tables and small ones. */
 int omni_gpu_join_set_type(void *handle, int32_t join_type, int32_t build_side);
 /* ProbeBatchForExistenceJoin (lookup_join.cpp:1393-1438): out_exists[i] = 1 when probe row i has a matching
  * build row, else 0 (also for a NULL key); rows bytes of device memory.  Works with any join type.

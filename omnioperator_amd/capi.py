@@ -102,6 +102,9 @@ SIGNATURES = {
     "omni_gpu_agg_batches_phases": (_I, [_VP, _P(_I64), _P(_I64), _P(_I64)]),
     "omni_gpu_agg_batches_tune_partitions": (_I, [_VP, _I32, _I32]),
     "omni_gpu_agg_batches_destroy": (_I, [_VP]),
+    "omni_gpu_agg_batches_set_advisor": (_I, [_VP, _VP]),
+    "omni_gpu_agg_batches_phase_of": (_I, [_VP, _VP]),
+    "omni_gpu_batch_features": (_I, [_P(Column), _VP, _I32, _VP]),
     "omni_gpu_join_build_create": (_I, [_P(JoinSpec), _P(_VP)]),
     "omni_gpu_join_build_add": (_I, [_VP, _P(Column), _P(Column), _I64]),
     "omni_gpu_join_build_finish": (_I, [_VP]),
@@ -364,7 +367,7 @@ class HashAgg:
 class AggBatches:
     """Batched stream of independent group-bys (omni_gpu_agg_batches_*)."""
 
-    FLAG_PER_BATCH, FLAG_GRAPH, FLAG_SKETCH = 1, 2, 4
+    FLAG_PER_BATCH, FLAG_GRAPH, FLAG_SKETCH, FLAG_ADVISOR = 1, 2, 4, 8
 
     def __init__(self, spec: AggSpec, max_batch_rows: int):
         init()
@@ -423,6 +426,17 @@ class AggBatches:
         """[(values, null mask)] of the whole output buffer (rows past each batch's group count are junk)."""
         types, bufs, _, total = self._outs
         return [(v.numpy(), unpack_nulls(nb.numpy(), total)) for v, nb in bufs]
+
+    def set_advisor(self, mlp: Optional["Mlp"]):
+        """Attaches the advisor network used under FLAG_ADVISOR (None detaches it)."""
+        check(lib().omni_gpu_agg_batches_set_advisor(self.h, mlp.h if mlp is not None else None))
+        self._advisor = mlp  # keeps the network alive while attached
+
+    def phase_of(self, nb: int) -> np.ndarray:
+        """The phase (1, 2, 3) that finished each batch of the last run."""
+        out = np.zeros(max(1, nb), np.int32)
+        check(lib().omni_gpu_agg_batches_phase_of(self.h, out.ctypes.data))
+        return out[:nb]
 
     def stats(self):
         a, b = C.c_int64(), C.c_int64()
@@ -584,6 +598,16 @@ def sketch_keys(keys: Sequence[Column], rows: int, is_join: bool = False) -> Ske
     out = Sketch()
     check(lib().omni_gpu_sketch_keys(karr, len(keys), rows, 1 if is_join else 0, C.byref(out)))
     return out
+
+
+def batch_features(key: Column, offsets: np.ndarray) -> np.ndarray:
+    """The advisor's 16 features of each batch of a stream (omni_gpu_batch_features): float32 [nb, 16]."""
+    init()
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    nb = len(offsets) - 1
+    out = np.zeros((max(1, nb), 16), np.float32)
+    check(lib().omni_gpu_batch_features(C.byref(key), offsets.ctypes.data, nb, out.ctypes.data))
+    return out[:nb]
 
 
 def _fptr(a: np.ndarray):

@@ -17,6 +17,7 @@
 namespace omni_gpu {
 
 constexpr int kScanBlock = 1024;
+constexpr float kAdvisorSlotShare = 0.5f;  // the advisor sends a batch past phase 1 above this share of its slots
 constexpr int kXcdCount = 8;  // XCDs of an MI355X; workgroup w runs on XCD w mod 8
 
 // order (optional): workgroup i takes batch order[i] -- the batches by decreasing rows, so that the longest
@@ -105,7 +106,10 @@ struct BatchesHandle {
     DevBuf dcfg, offsets, tiles, modes, spilled, groups, region_off, pool, counters;  // counters: [0] error, [4] redo
     DevBuf items, parts, parts_used;  // phase-2 work items, partitions launched / used per batch
     DevBuf order;                     // batches by decreasing rows (phase 1 and 2a workgroup order)
-    DevBuf sketch_modes;              // per-batch first phase from the sketch launch
+    DevBuf sketch_modes;              // per-batch first phase from the sketch launch or the advisor
+    void *advisor = nullptr;          // MLP handle (omni_gpu_agg_batches_set_advisor)
+    bool use_advisor = false;         // this run's first phases come from the advisor
+    DevBuf adv_x, adv_y;              // the advisor's per-batch features and outputs
     int64_t nitems = 0;
     bool last_per_batch = false;
     std::vector<int64_t> h_tiles;  // phase-3 tile prefix over the batches
@@ -191,7 +195,11 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
         }
     } else {
         const int32_t *order = h.order.as<int32_t>();
-        if (h.cfg.sketch_modes) {
+        if (h.use_advisor) {
+            OG_TRY(LaunchBatchAdvisor(s, h.advisor, h.cfg.kd.values[0], h.cfg.kd.nulls[0], h.cfg.kd.width[0],
+                                      h.offsets.as<int64_t>(), nb, h.adv_x.as<float>(), h.adv_y.as<float>(),
+                                      h.cfg.sketch_modes, kAdvisorSlotShare * static_cast<float>(h.cfg.L.S)));
+        } else if (h.cfg.sketch_modes) {
             batch_sketch<256><<<nb, 256, 0, s>>>(dc);
             OG_TRY(CheckLaunch("batch_sketch"));
         }
@@ -427,10 +435,16 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.redo = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>() + 4);
     c.nitems = h.nitems;
     c.hbm_flag = (flags & 1) ? 1 : 2;  // the per-batch form has no phase 2
-    // flag bit 2: the per-batch sketch picks each batch's first phase (not with caller modes or the per-batch
-    // form).  Off by default: on the C4 stream it moved as much work into phase 2 as it saved in phase 1.
-    const bool sketch = !(flags & 1) && (flags & 4) && batch_modes == nullptr;
+    // flag bit 3: the advisor network picks each batch's first phase from a per-batch sketch; flag bit 2: a
+    // linear-counting estimate does (neither with caller modes or the per-batch form; bit 3 wins)
+    h.use_advisor = !(flags & 1) && (flags & 8) && batch_modes == nullptr;
+    OG_CHECK(!h.use_advisor || h.advisor, OMNI_GPU_ERR_STATE, "flag bit 3 needs omni_gpu_agg_batches_set_advisor");
+    const bool sketch = !(flags & 1) && ((flags & 4) || h.use_advisor) && batch_modes == nullptr;
     if (sketch) OG_TRY(h.sketch_modes.Reserve(sizeof(int32_t) * num_batches, s));
+    if (h.use_advisor) {
+        OG_TRY(h.adv_x.Reserve(sizeof(float) * 16 * num_batches, s));
+        OG_TRY(h.adv_y.Reserve(sizeof(float) * 4 * num_batches, s));
+    }
     c.sketch_modes = sketch ? h.sketch_modes.as<int32_t>() : nullptr;
     h.cfg.sketch_modes = c.sketch_modes;
     h.last_per_batch = (flags & 1) != 0;
@@ -444,7 +458,11 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         // the launch shapes; it is re-captured when any of them changes
         std::vector<uint64_t> key = {static_cast<uint64_t>(num_batches), static_cast<uint64_t>(total),
                                      static_cast<uint64_t>(h.block), static_cast<uint64_t>(c.L.bytes),
-                                     reinterpret_cast<uint64_t>(h.dcfg.p), c.sketch_modes ? 1ULL : 0ULL};
+                                     reinterpret_cast<uint64_t>(h.dcfg.p), c.sketch_modes ? 1ULL : 0ULL,
+                                     h.use_advisor ? reinterpret_cast<uint64_t>(h.advisor) : 0ULL,
+                                     reinterpret_cast<uint64_t>(h.adv_x.p), reinterpret_cast<uint64_t>(h.adv_y.p),
+                                     reinterpret_cast<uint64_t>(h.sketch_modes.p), reinterpret_cast<uint64_t>(keys[0].values),
+                                     reinterpret_cast<uint64_t>(keys[0].nulls)};
         for (int k = 0; k < 1 + c.args.n; ++k) key.push_back(reinterpret_cast<uint64_t>(outs[k].nulls));
         if (!h.graph_exec || key != h.graph_key) {
             DropGraph(h);
@@ -469,6 +487,29 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     OG_TRY(CopyToHost(s, {{g.data(), h.groups.p, sizeof(unsigned long long) * num_batches}, {&err, c.error_flag, 4}}));
     for (int b = 0; b < num_batches; ++b) groups_per_batch[b] = static_cast<int64_t>(g[b]);
     if (err) return SetError(OMNI_GPU_ERR_OVERFLOW, "decimal overflow in SUM (overflowAsNull is false)");
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_batches_set_advisor(void *handle, void *mlp)
+{
+    OG_CHECK(handle, OMNI_GPU_ERR_INVALID, "null handle");
+    BatchesHandle &h = *static_cast<BatchesHandle *>(handle);
+    if (h.advisor != mlp) DropGraph(h);
+    h.advisor = mlp;
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_agg_batches_phase_of(void *handle, int32_t *phase)
+{
+    OG_CHECK(handle && phase, OMNI_GPU_ERR_INVALID, "null argument");
+    BatchesHandle &h = *static_cast<BatchesHandle *>(handle);
+    int64_t nb = h.have_desc ? static_cast<int64_t>(h.h_offsets.size()) - 1 : 0;
+    std::vector<unsigned int> sp(static_cast<size_t>(std::max<int64_t>(nb, 1)));
+    if (nb > 0) OG_TRY(CopyToHost(h.stream, {{sp.data(), h.spilled.p, sizeof(unsigned int) * nb}}));
+    for (int64_t b = 0; b < nb; ++b) {
+        const unsigned int f = sp[b];
+        phase[b] = f == 0 ? 1 : (f == 1 && !h.last_per_batch ? 2 : 3);
+    }
     return OMNI_GPU_OK;
 }
 
@@ -524,6 +565,8 @@ int omni_gpu_agg_batches_destroy(void *handle)
     h->parts_used.Release(s);
     h->order.Release(s);
     h->sketch_modes.Release(s);
+    h->adv_x.Release(s);
+    h->adv_y.Release(s);
     h->offsets.Release(s);
     h->tiles.Release(s);
     h->modes.Release(s);

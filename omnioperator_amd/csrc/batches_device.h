@@ -109,11 +109,18 @@ template <int BLOCK, bool CLEAR = false, typename MakeRef>
 __device__ __forceinline__ void EmitTable(const BatchConfig &c, const unsigned long long *keys, int S, int64_t start,
                                           unsigned int *counter, MakeRef ref)
 {
-    for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
-        unsigned long long k = keys[i];
-        if (k == kEmptyKey) continue;
-        unsigned int r = atomicAdd(counter, 1u);
-        bool side = i >= S;
+    // whole waves walk the slots so that each wave reserves its groups' output rows with one LDS atomic
+    for (int base = threadIdx.x - LaneId(); base < S + 2; base += BLOCK) {
+        const int i = base + LaneId();
+        const unsigned long long k = i < S + 2 ? keys[i] : kEmptyKey;
+        const bool used = k != kEmptyKey;
+        const unsigned long long mask = __ballot(used);
+        unsigned int first = 0;
+        if (LaneId() == 0 && mask) first = atomicAdd(counter, static_cast<unsigned int>(__popcll(mask)));
+        first = __shfl(first, 0, 64);
+        if (!used) continue;
+        const unsigned int r = first + static_cast<unsigned int>(__popcll(mask & LaneMaskLt()));
+        const bool side = i >= S;
         EmitGroup<CLEAR>(c, ref(i), side ? (i == S ? 0ULL : kEmptyKey) : k, i == S, start + r);
     }
 }

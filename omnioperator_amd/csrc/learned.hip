@@ -189,11 +189,60 @@ int GradStep(MlpHandle &h, const int *idx, int B, float lr, bool keep_grad)
     return CheckLaunch("mlp_reduce_adam");
 }
 
+int LaunchBatchAdvisor(hipStream_t s, void *mlp, const void *keys, const uint8_t *nulls, int width,
+                       const int64_t *offsets, int nb, float *X, float *Y, int32_t *modes, float limit)
+{
+    if (nb <= 0) return OMNI_GPU_OK;
+    batch_features<<<nb, kBatchSketchBlock, 0, s>>>(keys, nulls, width, offsets, X);
+    OG_TRY(CheckLaunch("batch_features"));
+    if (!mlp) return OMNI_GPU_OK;
+    const MlpHandle &h = *static_cast<const MlpHandle *>(mlp);
+    mlp_forward<<<(nb + kMlpFwdBlock - 1) / kMlpFwdBlock, kMlpFwdBlock, 0, s>>>(h.params.as<float>(), X, nb, Y);
+    OG_TRY(CheckLaunch("mlp_forward"));
+    if (!modes) return OMNI_GPU_OK;
+    batch_plan<<<(nb + 255) / 256, 256, 0, s>>>(Y, nb, limit, modes);
+    return CheckLaunch("batch_plan");
+}
+
 }  // namespace omni_gpu
 
 using namespace omni_gpu;
 
 extern "C" {
+
+int omni_gpu_batch_features(const omni_gpu_column *key, const int64_t *batch_offsets, int32_t num_batches,
+                            float *features)
+{
+    OG_CHECK(key && batch_offsets && features && num_batches >= 0, OMNI_GPU_ERR_INVALID, "null argument");
+    OG_CHECK(IsIntegerKeyType(key->type_id) && key->values, OMNI_GPU_ERR_INVALID, "key must be an integer column");
+    OG_CHECK(batch_offsets[0] == 0, OMNI_GPU_ERR_INVALID, "batch_offsets[0] must be 0");
+    for (int b = 0; b < num_batches; ++b) {
+        OG_CHECK(batch_offsets[b + 1] >= batch_offsets[b], OMNI_GPU_ERR_INVALID, "batch %d has a negative size", b);
+    }
+    if (num_batches == 0) return OMNI_GPU_OK;
+    hipStream_t s;
+    OG_TRY(AcquireStream(&s));
+    DevBuf off, X;
+    int rc = off.Reserve(sizeof(int64_t) * (num_batches + 1), s);
+    if (rc == OMNI_GPU_OK) rc = X.Reserve(sizeof(float) * kIn * num_batches, s);
+    if (rc == OMNI_GPU_OK && hipMemcpyAsync(off.p, batch_offsets, sizeof(int64_t) * (num_batches + 1),
+                                            hipMemcpyHostToDevice, s) != hipSuccess) {
+        rc = SetError(OMNI_GPU_ERR_HIP, "offsets upload failed");
+    }
+    if (rc == OMNI_GPU_OK) {
+        rc = LaunchBatchAdvisor(s, nullptr, key->values, key->nulls, TypeWidth(key->type_id), off.as<int64_t>(),
+                                num_batches, X.as<float>(), nullptr, nullptr, 0.f);
+    }
+    if (rc == OMNI_GPU_OK && (hipMemcpyAsync(features, X.p, sizeof(float) * kIn * num_batches, hipMemcpyDeviceToHost,
+                                             s) != hipSuccess ||
+                              hipStreamSynchronize(s) != hipSuccess)) {
+        rc = SetError(OMNI_GPU_ERR_HIP, "feature readback failed");
+    }
+    off.Release(s);
+    X.Release(s);
+    ReleaseStream(s);
+    return rc;
+}
 
 int omni_gpu_sketch_keys(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, int32_t is_join,
                          omni_gpu_sketch *out)

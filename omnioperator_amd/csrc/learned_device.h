@@ -351,10 +351,9 @@ __global__ void sketch_publish(const SketchStats *st, const float *f, int64_t ro
 
 // The MLP's 16 input features from the sketch statistics (all roughly in [0, 1]).  One thread; shared by the
 // sketch entry point and the advisor so that training and inference see identical features.
-__global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, int key_bytes, int num_keys,
-                                int is_join, float *f)
+__device__ __forceinline__ void SketchFeatures(const SketchStats *st, int64_t rows, int64_t m, int key_bytes,
+                                               int num_keys, int is_join, float *f)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double md = m > 0 ? static_cast<double>(m) : 1.0;
     const double d = static_cast<double>(st->distinct), f1 = static_cast<double>(st->f1),
                  f2 = static_cast<double>(st->f2);
@@ -377,6 +376,131 @@ __global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, 
     f[13] = static_cast<float>(num_keys / 4.0);
     f[14] = is_join ? 1.f : 0.f;
     f[15] = 1.f;
+}
+
+__global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, int key_bytes, int num_keys,
+                                int is_join, float *f)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    SketchFeatures(st, rows, m, key_bytes, num_keys, is_join, f);
+}
+
+// ---- per-batch sketch of a batched stream ------------------------------------------------------------------
+// One workgroup per batch: a sample of at most kBatchSample rows of the batch's key column -- every row of a small
+// batch, else 8 runs of 64 consecutive rows evenly spread over the batch (coalesced reads) -- is counted in an LDS
+// table and reduced to the same statistics and 16 features as the column sketch (so the advisor network reads
+// features of one kind), into X[b * kIn ..].
+constexpr int kBatchSample = 512;
+constexpr int kBatchSketchSlots = 2 * kBatchSample;
+constexpr int kBatchSketchBlock = 256;
+
+__device__ __forceinline__ unsigned long long WaveSum(unsigned long long v)
+{
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(kBatchSketchBlock) batch_features(const void *keys, const uint8_t *nulls, int width,
+                                                                    const int64_t *offsets, float *X)
+{
+    __shared__ unsigned long long hk[kBatchSketchSlots];
+    __shared__ unsigned int cnt[kBatchSketchSlots];
+    __shared__ SketchStats st;
+    const int b = blockIdx.x;
+    const int64_t start = offsets[b], rows = offsets[b + 1] - offsets[b];
+    const int64_t m = rows < kBatchSample ? rows : kBatchSample;
+    for (int i = threadIdx.x; i < kBatchSketchSlots; i += kBatchSketchBlock) {
+        hk[i] = 0;
+        cnt[i] = 0;
+    }
+    if (threadIdx.x == 0) {
+        st = SketchStats{};
+        st.kmin = std::numeric_limits<long long>::max();
+        st.kmax = std::numeric_limits<long long>::min();
+    }
+    __syncthreads();
+    long long lo = std::numeric_limits<long long>::max(), hi = std::numeric_limits<long long>::min();
+    unsigned long long nn = 0;
+    for (int64_t i = threadIdx.x; i < m; i += kBatchSketchBlock) {
+        const int64_t r = rows <= kBatchSample ? i : (i >> 6) * (rows >> 3) + (i & 63);
+        const int64_t row = start + r;
+        const bool isNull = NullBit(nulls, row);
+        const unsigned long long w = isNull ? 0x9E3779B97F4A7C15ULL : LoadWord(keys, width, row);
+        unsigned long long h = Mix64(0x243F6A8885A308D3ULL ^ (w + 0x632BE59BD9B4E019ULL + (isNull ? 1ULL : 0ULL)));
+        if (h == 0) h = 1;
+        if (isNull) {
+            ++nn;
+        } else {
+            lo = min(lo, static_cast<long long>(w));
+            hi = max(hi, static_cast<long long>(w));
+        }
+        unsigned int pos = static_cast<unsigned int>(h) & (kBatchSketchSlots - 1);
+        while (true) {
+            unsigned long long k = hk[pos];
+            if (k == 0) {
+                k = atomicCAS(&hk[pos], 0ULL, h);
+                if (k == 0) k = h;
+            }
+            if (k == h) {
+                atomicAdd(&cnt[pos], 1u);
+                break;
+            }
+            pos = (pos + 1) & (kBatchSketchSlots - 1);
+        }
+    }
+#pragma unroll
+    for (int d = 32; d > 0; d >>= 1) {
+        lo = min(lo, __shfl_xor(lo, d, 64));
+        hi = max(hi, __shfl_xor(hi, d, 64));
+    }
+    nn = WaveSum(nn);
+    if (LaneId() == 0) {
+        if (lo <= hi) {
+            atomicMin(&st.kmin, lo);
+            atomicMax(&st.kmax, hi);
+        }
+        if (nn) atomicAdd(&st.nulls, nn);
+    }
+    __syncthreads();
+    unsigned long long d = 0, f1 = 0, f2 = 0, heavy = 0, mx = 0;
+    for (int i = threadIdx.x; i < kBatchSketchSlots; i += kBatchSketchBlock) {
+        const unsigned int c = cnt[i];
+        d += c ? 1 : 0;
+        f1 += c == 1 ? 1 : 0;
+        f2 += c == 2 ? 1 : 0;
+        heavy += c >= 4 ? c : 0;
+        mx = c > mx ? c : mx;
+    }
+    d = WaveSum(d);
+    f1 = WaveSum(f1);
+    f2 = WaveSum(f2);
+    heavy = WaveSum(heavy);
+#pragma unroll
+    for (int s = 32; s > 0; s >>= 1) {
+        const unsigned long long o = __shfl_xor(mx, s, 64);
+        mx = o > mx ? o : mx;
+    }
+    if (LaneId() == 0) {
+        atomicAdd(&st.distinct, d);
+        atomicAdd(&st.f1, f1);
+        atomicAdd(&st.f2, f2);
+        atomicAdd(&st.heavy, heavy);
+        atomicMax(&st.maxc, mx);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) SketchFeatures(&st, rows, m, width, 1, 0, X + static_cast<int64_t>(b) * kIn);
+}
+
+// Per-batch first phase from the network's outputs Y[b][4]: the predicted group count 2^(32 y0) - 1 against
+// `limit` (a share of the per-batch LDS table's slots): at most that many start in the per-batch LDS table (0),
+// more go straight to the hash-partitioned passes (2).
+__global__ void batch_plan(const float *Y, int nb, float limit, int32_t *modes)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const float groups = exp2f(Y[static_cast<int64_t>(b) * kOut] * 32.f) - 1.f;
+    modes[b] = groups > limit ? 2 : 0;
 }
 
 // ---- seeded key generator ----------------------------------------------------------------------------------

@@ -139,6 +139,7 @@ def lib():
             "oracle_agg_num_groups": (i64, [vp]),
             "oracle_agg_get_output": (i32, [vp, vp]),
             "oracle_agg_destroy": (None, [vp]),
+            "oracle_agg_stream": (i64, [vp, vp, vp, vp, i32, i32, vp]),
             "oracle_join_create": (vp, [i32]),
             "oracle_join_add_batch": (i32, [vp, vp, vp, i32, i64]),
             "oracle_join_finish": (i32, [vp]),
@@ -301,6 +302,22 @@ def hash_agg(spec: AggSpec, key_cols: Sequence[Col], agg_cols: Sequence[Optional
         return outs
     finally:
         L.oracle_agg_destroy(h)
+
+
+def agg_stream(spec: AggSpec, key_col: Col, agg_cols: Sequence[Optional[Col]], offsets: np.ndarray,
+               threads: int = 1) -> np.ndarray:
+    """One restated HashAggregationOperator per batch (rows [offsets[b], offsets[b + 1])): AddInput, then the
+    output of every group; batches spread over `threads` host threads.  Returns the group count of each batch."""
+    L = lib()
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    nb = len(offsets) - 1
+    karr, k1 = _ocols([key_col], MAX_KEYS)
+    aarr, k2 = _ocols(list(agg_cols), MAX_AGGS)
+    groups = np.zeros(max(1, nb), np.int64)
+    total = L.oracle_agg_stream(C.byref(spec), karr, aarr, _ptr(offsets), nb, threads, _ptr(groups))
+    if total < 0:
+        raise OverflowError(L.oracle_last_error().decode())
+    return groups[:nb]
 
 
 # ---- join ---------------------------------------------------------------------------------------------------

@@ -18,12 +18,14 @@
 #include <emmintrin.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <limits>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../include/omni_gpu.h"
@@ -1262,6 +1264,69 @@ int32_t oracle_agg_get_output(void *h, omni_gpu_out_column *outs)
     return 0;
 }
 void oracle_agg_destroy(void *h) { delete static_cast<HashAgg *>(h); }
+
+// A stream of independent group-bys: for each VectorBatch b (rows [offsets[b], offsets[b + 1]) of the columns)
+// one HashAggregationOperator takes the batch (AddInput) and emits its groups (GetOutput), as an engine runs one
+// operator per batch (group_aggregation.cpp:320-406, 454-474).  `threads` host threads take batches from a shared
+// counter (threads = 1: the reference's one operator per thread).  Writes each batch's group count; returns the
+// total, or -1 on a decimal overflow that raises.
+int64_t oracle_agg_stream(const omni_gpu_agg_spec *spec, const oracle_column *keys, const oracle_column *aggs,
+                          const int64_t *offsets, int32_t nb, int32_t threads, int64_t *groups_per_batch)
+{
+    std::atomic<int32_t> next{0};
+    std::atomic<bool> failed{false};
+    auto shifted = [](const oracle_column &c, int64_t start, int64_t rows, std::vector<uint8_t> &bits) {
+        HostColumn h{c.type_id, c.scale, nullptr, nullptr};
+        if (c.values) h.values = static_cast<const uint8_t *>(c.values) + start * TypeWidth(c.type_id);
+        if (c.nulls) {
+            bits.assign(static_cast<size_t>(rows + 7) / 8 + 1, 0);
+            for (int64_t r = 0; r < rows; ++r) {
+                if ((c.nulls[(start + r) >> 3] >> ((start + r) & 7)) & 1) bits[r >> 3] |= 1u << (r & 7);
+            }
+            h.nulls = bits.data();
+        }
+        return h;
+    };
+    auto work = [&]() {
+        std::vector<std::vector<uint8_t>> kbits(OMNI_GPU_MAX_KEYS), abits(OMNI_GPU_MAX_AGGS);
+        std::vector<std::vector<uint8_t>> vals, nulls;
+        while (true) {
+            const int32_t b = next.fetch_add(1);
+            if (b >= nb || failed.load()) break;
+            const int64_t start = offsets[b], rows = offsets[b + 1] - offsets[b];
+            HostColumn k[OMNI_GPU_MAX_KEYS], v[OMNI_GPU_MAX_AGGS];
+            for (int i = 0; i < OMNI_GPU_MAX_KEYS; ++i) k[i] = shifted(keys[i], start, rows, kbits[i]);
+            for (int i = 0; i < OMNI_GPU_MAX_AGGS; ++i) v[i] = shifted(aggs[i], start, rows, abits[i]);
+            HashAgg a(*spec);
+            a.AddBatch(k, v, rows);
+            const int64_t g = a.NumGroups();
+            const int ncols = spec->num_keys + spec->num_aggs;
+            vals.resize(ncols);
+            nulls.resize(ncols);
+            OutCol outs[OMNI_GPU_MAX_KEYS + OMNI_GPU_MAX_AGGS];
+            for (int c = 0; c < ncols; ++c) {
+                const int32_t t = c < spec->num_keys ? spec->key_types[c] : spec->agg_out_types[c - spec->num_keys];
+                vals[c].resize(static_cast<size_t>(std::max<int64_t>(1, g)) * TypeWidth(t));
+                nulls[c].assign(static_cast<size_t>(g + 7) / 8 + 1, 0);
+                outs[c] = OutCol{t, 0, vals[c].data(), nulls[c].data()};
+            }
+            if (!a.Output(outs)) failed.store(true);
+            groups_per_batch[b] = g;
+        }
+    };
+    const int nt = std::max(1, threads);
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    if (failed.load()) {
+        g_err = "decimal overflow";
+        return -1;
+    }
+    int64_t total = 0;
+    for (int32_t b = 0; b < nb; ++b) total += groups_per_batch[b];
+    return total;
+}
 
 // --- join -----------------------------------------------------------------------------------------------
 void *oracle_join_create(int32_t key_type) { return new HashJoin(key_type); }

@@ -138,6 +138,40 @@ def test_batches_match_oracle(form):
         assert stats["lds_batches"] > 0
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_advisor_form_matches_oracle(graph):
+    """The learned form (flag bit 3: the stream advisor sketches every batch and picks its first phase on the
+    device) gives every batch the oracle's rows, eager and under hipGraph replay, and does route batches both
+    ways; the advisor's features equal omni_gpu_batch_features."""
+    from omnioperator_amd import batch_advisor
+    offsets, key, cols = make_stream(4321)
+    nb = len(offsets) - 1
+    net = batch_advisor.network()
+    spec = capi.make_agg_spec([LONG], AGGS)
+    s = capi.AggBatches(spec, int(np.max(np.diff(offsets))))
+    s.set_advisor(net)
+    kd = capi.DeviceColumn(LONG, key.values, key.nulls)
+    ad = [None if c is None else capi.DeviceColumn(c.type_id, c.values, c.nulls) for c in cols]
+    ac = [None if d is None else d.c() for d in ad]
+    flags = capi.AggBatches.FLAG_ADVISOR | (capi.AggBatches.FLAG_GRAPH if graph else 0)
+    want = oracle_batches(offsets, key, cols, range(nb))
+    for _ in range(3 if graph else 1):
+        groups = s.run(kd.c(), ac, offsets, None, flags)
+        got = per_batch_rows(offsets, groups, s.outputs())
+        for b in range(nb):
+            assert_row_sets_equal(got[b], want[b], float_cols=FLOAT_COLS, rtol=1e-12)
+    ph = s.phase_of(nb)
+    assert (ph == 1).any() and (ph == 2).any()
+    # batches the network predicts above half the LDS table start in the partitioned passes
+    X = capi.batch_features(kd.c(), offsets)
+    y = net.forward(X)
+    limit = 0.5 * s.info()["lds_slots"]
+    skip = np.exp2(y[:, 0] * 32.0) - 1.0 > 1.05 * limit  # clear of the float rounding at the limit
+    assert np.all(ph[skip] >= 2)
+    s.close()
+    net.close()
+
+
 def test_graph_replays_match_eager():
     """hipGraph replays of the steady state, with eager runs between them, give the eager result each time."""
     offsets, key, cols = make_stream(77, nb=120)

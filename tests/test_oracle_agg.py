@@ -240,3 +240,21 @@ def test_decimal_near_limit_oracle_equals_brute(tin, tout):
     assert_row_sets_equal(got, want)
     if tout == DECIMAL64 or tin == DECIMAL128:
         assert 0 < sum(r[1] is None for r in got) < len(got)
+
+
+def test_agg_stream_equals_one_operator_per_batch():
+    """oracle.agg_stream (one operator per batch, batches shared by host threads) gives each batch the group count of
+    its own hash_agg run, with one thread and with several; NULL keys and values at unaligned batch starts."""
+    rng = np.random.default_rng(44)
+    rows = rng.integers(0, 700, 60)
+    rows[:2] = [0, 1]
+    offsets = np.concatenate([[0], np.cumsum(rows)]).astype(np.int64)
+    n = int(offsets[-1])
+    key = Col(LONG, rng.integers(0, 90, n, dtype=np.int64), rng.random(n) < 0.05)
+    val = Col(LONG, rng.integers(-10**6, 10**6, n, dtype=np.int64), rng.random(n) < 0.1)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
+    spec = oracle.make_agg_spec([LONG], aggs)
+    want = [len(oracle.hash_agg(spec, [key.slice(a, b)], [val.slice(a, b), None], max(1, b - a))[0].values)
+            if b > a else 0 for a, b in zip(offsets[:-1], offsets[1:])]
+    for threads in (1, 4):
+        assert oracle.agg_stream(spec, key, [val, None], offsets, threads).tolist() == want
