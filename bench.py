@@ -57,13 +57,15 @@ def bench_join(args):
     capi.init(0)
     w = workloads.join_inner(build_rows=args.build_rows, probe_rows=args.probe_rows, dense=args.dense)
     nb, npr = len(w.build_keys), len(w.probe_keys)
-    # device-resident inputs, cut into VectorBatches
-    bkey = capi.DeviceColumn(capi.LONG, w.build_keys)
+    # device-resident inputs, cut into VectorBatches; the key as one int64 column, or as the key columns of
+    # --join-keys (each a function of the int64 key, so the same rows match)
+    ktypes, bkcols, pkcols = join_key_columns(capi, w, args.join_keys)
+    bkey = [capi.DeviceColumn(t, v) for t, v in zip(ktypes, bkcols)]
     bpay = capi.DeviceColumn(capi.INT, w.build_payload)
-    probe = capi.DeviceColumn(capi.LONG, w.probe_keys)
+    probe = [capi.DeviceColumn(t, v) for t, v in zip(ktypes, pkcols)]
     bspans = workloads.batches(nb, w.batch_rows)
     pspans = workloads.batches(npr, w.batch_rows)
-    pcols, prows = capi.HashJoin.probe_arrays([probe.c(a) for a, _ in pspans], [b - a for a, b in pspans])
+    pcols, prows = capi.HashJoin.probe_arrays([k.c(a) for a, _ in pspans for k in probe], [b - a for a, b in pspans])
     jt, bs = JOIN_TYPES[args.join_type]
     cap = npr
     out_rows = capi.DeviceArray(cap * 8, np.int64, (cap,))
@@ -82,10 +84,11 @@ def bench_join(args):
         oouter[0].type_id, oouter[0].values, oouter[0].nulls = capi.INT, outer_pay.ptr, None
 
     # per-batch column descriptors prepared once (the C++ operator path has no such host cost)
-    bcols = [(bkey.c(a), (capi.Column * 1)(bpay.c(a)), b - a) for a, b in bspans]
+    bcols = [((capi.Column * len(bkey))(*[k.c(a) for k in bkey]), (capi.Column * 1)(bpay.c(a)), b - a)
+             for a, b in bspans]
 
     def step():
-        j = capi.HashJoin(capi.LONG, [capi.INT])
+        j = capi.HashJoin(ktypes if len(ktypes) > 1 else ktypes[0], [capi.INT])
         j.tune(args.probe_group, args.partitions)
         if jt != capi.JOIN_INNER:
             j.set_type(jt, bs)
@@ -116,6 +119,8 @@ def bench_join(args):
     capi.profile_enable(False)
     launches, probe_ms = capi.profile_get("join_probe")
     _, build_ms = capi.profile_get("join_build")
+    _, tbuild_ms = capi.profile_get("join_translate_build")
+    _, tprobe_ms = capi.profile_get("join_translate_probe")
     _, outer_ms = capi.profile_get("join_outer")
     expected = {"inner": w.hits, "left": npr, "right_build_right": w.hits, "full": npr, "semi": w.hits,
                 "anti": npr - w.hits}[args.join_type]
@@ -151,11 +156,25 @@ def bench_join(args):
     if parts[0] > 1:
         extra["probe_stage_ms"] = {k: round(capi.profile_get("join_probe_" + k)[1] / args.steps, 4)
                                    for k in ("hist", "scan", "scatter", "window", "part")}
-    cpu = cpu_baseline_join(w, args, jt, bs) if not args.no_cpu else None
+    if args.join_keys != "single":
+        # key translation: per probe row its key columns read, the int64 join key and its NULL bit written, and
+        # for a dictionary mode one dictionary slot (state word + key words) read
+        kbytes = sum(capi.WIDTH[t] for t in ktypes)
+        dict_bytes = 0 if args.join_keys == "int_int" else 4 + kbytes_words(ktypes) * 8
+        tbytes = npr * (kbytes + 8 + dict_bytes) + npr // 8
+        tms = tprobe_ms / args.steps
+        extra["key_translation"] = {
+            "key_types": [TYPE_NAMES[t] for t in ktypes],
+            "build_translate_ms": round(tbuild_ms / args.steps, 4), "probe_translate_ms": round(tms, 4),
+            "probe_translate_algorithmic_bytes": tbytes,
+            "probe_translate_gbs": round(tbytes / (tms / 1e3) / 1e9, 1) if tms > 0 else None}
+    cpu = cpu_baseline_join(w, args, jt, bs, ktypes, bkcols, pkcols) if not args.no_cpu else None
     name = JOIN_NAMES[args.join_type]
-    cfg = {"workload": f"{name} hash join: build 2M unique int64 keys + int32 payload, probe 64M int64 keys "
+    keys_desc = {"single": "int64 keys", "int_int": "(INT, INT) keys", "long_long": "(LONG, LONG) keys",
+                 "decimal128": "DECIMAL128 keys"}[args.join_keys]
+    cfg = {"workload": f"{name} hash join: build 2M unique {keys_desc} + int32 payload, probe 64M {keys_desc} "
                        "(80% hit), VectorBatches of 65536" if (nb, npr) == (2_000_000, 64_000_000) else
-                       f"{name} hash join: build {nb} + probe {npr} rows",
+                       f"{name} hash join on {keys_desc}: build {nb} + probe {npr} rows",
            "build_rows": nb, "probe_rows": npr, "batch_rows": w.batch_rows,
            "keys": "dense permutation" if args.dense else "sparse 64-bit (splitmix64)",
            "output": "probe rowid int64" if args.join_type in ("semi", "anti") else
@@ -527,14 +546,54 @@ def profiled_traffic(row_name: str, profile: str = "join_probe_c2.md"):
     return None
 
 
-def cpu_baseline_join(w, args, jt=0, bs=0):
+TYPE_NAMES = {1: "INT", 2: "LONG", 7: "DECIMAL128"}
+
+
+def kbytes_words(types):
+    return sum(2 if t == 7 else 1 for t in types)
+
+
+def join_key_columns(capi, w, mode: str):
+    """(key types, build key columns, probe key columns) of the join workload for --join-keys: the int64 key
+    itself; (INT, INT) = its high and low halves, the low half masked to 31 bits so that the packed int64 key
+    (high << 32 | low) is the key with bit 31 cleared; (LONG, LONG) = the key and splitmix64 of it (a 128-bit
+    packed key, joined through the key dictionary); DECIMAL128 = the key sign-extended to 128 bits."""
+    def split(k):
+        return [(k >> 32).astype(np.int32), (k & 0x7FFFFFFF).astype(np.int32)]
+
+    def mix(k):
+        z = k.view(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        return (z ^ (z >> np.uint64(31))).view(np.int64)
+
+    def dec(k):
+        return np.stack([k.view(np.uint64), (k >> 63).view(np.uint64)], 1)
+
+    with np.errstate(over="ignore"):
+        if mode == "single":
+            return [capi.LONG], [w.build_keys], [w.probe_keys]
+        if mode == "int_int":
+            return [capi.INT, capi.INT], split(w.build_keys), split(w.probe_keys)
+        if mode == "long_long":
+            return [capi.LONG, capi.LONG], [w.build_keys, mix(w.build_keys)], [w.probe_keys, mix(w.probe_keys)]
+        return [capi.DECIMAL128], [dec(w.build_keys)], [dec(w.probe_keys)]
+
+
+def cpu_baseline_join(w, args, jt=0, bs=0, ktypes=None, bkcols=None, pkcols=None):
     """The oracle (C++ restatement of HashBuilderOperator + LookupJoinOperator, one core) on the full
     build side and a bounded probe sample; rows/s = (build + sampled probe rows) / wall time."""
     import oracle
     sample = min(len(w.probe_keys), args.cpu_probe_rows)
-    bk = oracle.Col(oracle.LONG, w.build_keys)
+    if ktypes is None or len(ktypes) == 1 and ktypes[0] == oracle.LONG:
+        bk = oracle.Col(oracle.LONG, w.build_keys)
+        pk = oracle.Col(oracle.LONG, w.probe_keys[:sample])
+    else:
+        bk = [oracle.Col(t, v) for t, v in zip(ktypes, bkcols)]
+        pk = [oracle.Col(t, v[:sample]) for t, v in zip(ktypes, pkcols)]
+        if len(bk) == 1:
+            bk, pk = bk[0], pk[0]
     bp = oracle.Col(oracle.INT, w.build_payload)
-    pk = oracle.Col(oracle.LONG, w.probe_keys[:sample])
     t0 = time.perf_counter()
     if jt == 0:
         r = oracle.hash_join(bk, [bp], pk, w.batch_rows, w.batch_rows)
@@ -579,6 +638,8 @@ def main():
     p.add_argument("--cpu-probe-rows", type=int, default=8_000_000)
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--probe-group", type=int, default=1, help="probe width (lookups in flight per lane)")
+    p.add_argument("--join-keys", default="single", choices=["single", "int_int", "long_long", "decimal128"],
+                   help="join workload: the key as one int64 column or as several key columns / a DECIMAL128")
     p.add_argument("--partitions", type=int, default=-1, help="radix-partitioned probe: -1 auto, 0 off, or P")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="only run 3 device-to-device copies of this many MiB (HBM counter calibration)")

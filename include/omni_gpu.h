@@ -248,11 +248,22 @@ int omni_gpu_agg_batches_destroy(void *handle);
  * join_hash_table_variants.cpp:254-283, 695-808) and LookupJoinOperator::ProbeBatchForInnerJoin
  * (core/src/operator/join/lookup_join.cpp:924-1035) with its output gather (:1657-1779). */
 typedef struct omni_gpu_join_spec {
-    int32_t key_type;                 /* single equality key: INT, DATE32, SHORT, LONG, TIMESTAMP, DECIMAL64,
-                                         DOUBLE (compared as its 64-bit pattern), BOOLEAN */
+    int32_t key_type;                 /* single equality key (num_keys <= 1): INT, DATE32, SHORT, LONG, TIMESTAMP,
+                                         DECIMAL64, DOUBLE (compared as its 64-bit pattern), BOOLEAN, DECIMAL128 */
     int32_t num_payloads;             /* build-side output columns */
     int32_t payload_types[OMNI_GPU_MAX_AGGS];
     int32_t strategy;                 /* 0 = reference rule (array table if dense), 1 = hash only */
+    /* Several equality key columns (buildHashCols of HashBuilderOperatorFactory, hash_builder.cpp:44-93): num_keys
+     * in [2, OMNI_GPU_MAX_KEYS] fixed-width columns of key_types (0 or 1: the single key_type).  Keys match as the
+     * reference's table compares them: SHORT / INT / LONG / DECIMAL64 columns of at most 32, 64 or 128 bits in
+     * total by their ComputeMultiColKey packing (join_hash_table_variants.cpp:105-146; each value is sign-extended
+     * into the key before it is OR-ed in, so a negative later column overwrites the earlier ones and such tuples
+     * match), other combinations by the bytes of all their values.  With num_keys >= 2 or a DECIMAL128 key, the
+     * `key` argument of omni_gpu_join_build_add and the probe entry points points to num_keys columns (in
+     * omni_gpu_join_probe_batches, batch b's columns are keys[b * num_keys ..]); a row with a NULL in any key column
+     * matches nothing; the table is never an array table. */
+    int32_t num_keys;
+    int32_t key_types[OMNI_GPU_MAX_KEYS];
 } omni_gpu_join_spec;
 
 int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle);
@@ -294,8 +305,9 @@ int omni_gpu_join_probe(void *handle, const omni_gpu_column *probe_key, int64_t 
                         int64_t probe_row_base, int64_t *out_probe_rows, omni_gpu_out_column *out_payloads,
                         int64_t out_capacity, int64_t *num_matches);
 
-/* Probes many batches in ONE launch.  keys[b] holds batch b's key column with rows[b] rows;
- * probe row ids are numbered consecutively across batches starting at 0 in batch order. */
+/* Probes many batches in ONE launch.  keys[b] holds batch b's key column with rows[b] rows (with several key
+ * columns or a DECIMAL128 key: keys[b * num_keys ..]); probe row ids are numbered consecutively across batches
+ * starting at 0 in batch order. */
 int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const int64_t *rows,
                                 int32_t num_batches, int64_t *out_probe_rows,
                                 omni_gpu_out_column *out_payloads, int64_t out_capacity,

@@ -60,7 +60,7 @@ class AggSpec(C.Structure):
 
 class JoinSpec(C.Structure):
     _fields_ = [("key_type", C.c_int32), ("num_payloads", C.c_int32), ("payload_types", C.c_int32 * MAX_AGGS),
-                ("strategy", C.c_int32)]
+                ("strategy", C.c_int32), ("num_keys", C.c_int32), ("key_types", C.c_int32 * MAX_KEYS)]
 
 
 class Sketch(C.Structure):
@@ -461,13 +461,28 @@ class AggBatches:
             pass
 
 
+def _key_arg(key):
+    """A key Column, a sequence of them (several key columns) or a prebuilt ctypes array of them, as the pointer
+    the C ABI takes."""
+    if isinstance(key, C.Array):
+        return key
+    if isinstance(key, (list, tuple)):
+        return (Column * len(key))(*key)
+    return C.byref(key)
+
+
 class HashJoin:
     """GPU hash join handle (omni_gpu_join_*)."""
 
-    def __init__(self, key_type: int, payload_types: Sequence[int] = (), strategy: int = 0):
+    def __init__(self, key_type, payload_types: Sequence[int] = (), strategy: int = 0):
+        """key_type: one key type, or a sequence of them (several key columns)."""
         init()
         s = JoinSpec()
-        s.key_type = key_type
+        types = list(key_type) if isinstance(key_type, (list, tuple)) else [key_type]
+        s.key_type = types[0]
+        s.num_keys = len(types)
+        for i, t in enumerate(types):
+            s.key_types[i] = t
         s.num_payloads = len(payload_types)
         for i, t in enumerate(payload_types):
             s.payload_types[i] = t
@@ -482,7 +497,7 @@ class HashJoin:
     def add_build(self, key: Column, payloads, rows: int):
         """payloads: sequence of Columns or a prebuilt ctypes array of them."""
         parr = payloads if isinstance(payloads, C.Array) else (Column * max(1, len(payloads)))(*payloads)
-        check(lib().omni_gpu_join_build_add(self.h, C.byref(key), parr, rows))
+        check(lib().omni_gpu_join_build_add(self.h, _key_arg(key), parr, rows))
         self.build_rows += rows
 
     def set_type(self, join_type: int, build_side: int = 0):
@@ -492,7 +507,7 @@ class HashJoin:
     def probe_exists(self, key: Column, rows: int) -> np.ndarray:
         """Existence flags (uint8, one per probe row) of one batch."""
         out = DeviceArray(max(1, rows), np.uint8, (rows,))
-        check(lib().omni_gpu_join_probe_exists(self.h, C.byref(key), rows, out.ptr))
+        check(lib().omni_gpu_join_probe_exists(self.h, _key_arg(key), rows, out.ptr))
         return out.numpy(rows)
 
     def outer_output_raw(self, out_rows: "DeviceArray", oarr, cap: int) -> int:
@@ -546,7 +561,7 @@ class HashJoin:
             cap = rows * max(1, self.info()["max_dup"])
         out_rows, pays, oarr = self._outs(cap, with_nulls)
         m = C.c_int64()
-        check(lib().omni_gpu_join_probe(self.h, C.byref(key), rows, row_base, out_rows.ptr, oarr, cap, C.byref(m)))
+        check(lib().omni_gpu_join_probe(self.h, _key_arg(key), rows, row_base, out_rows.ptr, oarr, cap, C.byref(m)))
         n = m.value
         res = [(v.numpy(n), unpack_nulls(nb.numpy(), n) if nb is not None else np.zeros(n, bool)) for v, nb in pays]
         return out_rows.numpy(n), res
@@ -558,7 +573,7 @@ class HashJoin:
         karr = keys if isinstance(keys, C.Array) else (Column * len(keys))(*keys)
         rarr = rows if isinstance(rows, C.Array) else (C.c_int64 * len(rows))(*rows)
         m = C.c_int64()
-        check(lib().omni_gpu_join_probe_batches(self.h, karr, rarr, len(karr), out_rows.ptr, oarr, cap, C.byref(m)))
+        check(lib().omni_gpu_join_probe_batches(self.h, karr, rarr, len(rarr), out_rows.ptr, oarr, cap, C.byref(m)))
         return m.value
 
     @staticmethod

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "join_device.h"
+#include "join_keys.h"
 
 namespace omni_gpu {
 
@@ -66,12 +67,20 @@ struct JoinHandle {
     //          [16] min/max i64[2], [32] match counter u64, [40] build overflow count u32
     DevBuf slots, heads, next, scratch;
     DevBuf batch_desc, build_desc;
+    // keys other than one fixed-width column (join_keys.h): their translation into int64 join keys
+    JoinKeyCols kc{};
+    int32_t jkey_type = 0;             // type of the join key the kernels see (LONG for translated keys)
+    std::vector<KeyBatch> pending_keys;  // the key columns of each retained build batch
+    DevBuf dict;                       // dictionary of the build side's wide keys (KeyDict records)
+    uint64_t dict_slots = 0;           // its slots (a power of two)
+    DevBuf bkeys, bnulls, pkeys, pnulls, kdesc;  // translated build / probe keys and their NULL bitmaps
 
     // Every device buffer of the handle (destroy releases exactly these; a new member is added here).
     template <typename F> void ForEachBuffer(F f)
     {
         for (DevBuf *b : {&keys, &knull, &mm_parts, &visited, &part_keys, &part_rows, &part_counts, &part_meta,
-                          &ovf_keys, &ovf_rows, &slots, &heads, &next, &scratch, &batch_desc, &build_desc}) {
+                          &ovf_keys, &ovf_rows, &slots, &heads, &next, &scratch, &batch_desc, &build_desc,
+                          &dict, &bkeys, &bnulls, &pkeys, &pnulls, &kdesc}) {
             f(*b);
         }
         for (int p = 0; p < OMNI_GPU_MAX_AGGS; ++p) {
@@ -461,6 +470,134 @@ int FillBatch(ProbeBatch &b, const omni_gpu_column &c, int64_t rows, int64_t row
     return OMNI_GPU_OK;
 }
 
+// The reference's key variant for the spec's key columns (HashBuilderOperatorFactory::InitVariant,
+// hash_builder.cpp:44-119) as a translation mode of join_keys.h.
+int PlanJoinKeys(const omni_gpu_join_spec &sp, JoinKeyCols &kc)
+{
+    const int n = sp.num_keys <= 1 ? 1 : sp.num_keys;
+    OG_CHECK(n <= OMNI_GPU_MAX_KEYS, OMNI_GPU_ERR_INVALID, "num_keys must be at most %d", OMNI_GPU_MAX_KEYS);
+    kc = JoinKeyCols{};
+    kc.n = n;
+    for (int c = 0; c < n; ++c) {
+        kc.type[c] = n == 1 ? sp.key_type : sp.key_types[c];
+        kc.width[c] = TypeWidth(kc.type[c]);
+        OG_CHECK(kc.width[c] > 0, OMNI_GPU_ERR_INVALID, "join key type %d is not a fixed-width type", kc.type[c]);
+    }
+    if (n == 1) {
+        const int t = kc.type[0];
+        if (IsIntegerKeyType(t) || t == OMNI_TYPE_DOUBLE || t == OMNI_TYPE_BOOLEAN) {
+            kc.mode = kKeySingle;
+        } else {
+            OG_CHECK(t == OMNI_TYPE_DECIMAL128, OMNI_GPU_ERR_INVALID, "join key type %d is not supported", t);
+            kc.mode = kKeyDecimal128;
+            kc.W = 2;
+        }
+        return OMNI_GPU_OK;
+    }
+    // GetTypeLength: SHORT 16, INT 32, LONG and DECIMAL64 64 bits; any other type serialises the key
+    int bits = 0;
+    bool packable = true;
+    for (int c = 0; c < n; ++c) {
+        switch (kc.type[c]) {
+            case OMNI_TYPE_SHORT: kc.bits[c] = 16; break;
+            case OMNI_TYPE_INT: kc.bits[c] = 32; break;
+            case OMNI_TYPE_LONG:
+            case OMNI_TYPE_DECIMAL64: kc.bits[c] = 64; break;
+            default: packable = false; break;
+        }
+        bits += kc.bits[c];
+    }
+    if (packable && bits <= 32) {
+        kc.mode = kKeyPacked32;
+    } else if (packable && bits <= 64) {
+        kc.mode = kKeyPacked64;
+    } else if (packable && bits <= 128) {
+        kc.mode = kKeyPacked128;
+        kc.W = 2;
+    } else {
+        kc.mode = kKeySerialized;
+        kc.W = 0;
+        for (int c = 0; c < n; ++c) kc.W += kc.width[c] == 16 ? 2 : 1;
+    }
+    return OMNI_GPU_OK;
+}
+
+// Translates nb batches of key columns (cols[b * kc.n + c]) into int64 join keys: out_keys gets the keys of all
+// batches in order, out_nulls a NULL bitmap per batch (starting at its row 0); `cols` returns each batch's
+// translated key column.  INSERT adds the keys to the dictionary (build side).
+template <bool INSERT>
+int TranslateKeys(JoinHandle &h, const omni_gpu_column *in, const int64_t *rows, int nb, DevBuf &out_keys,
+                  DevBuf &out_nulls, std::vector<omni_gpu_column> &cols)
+{
+    hipStream_t s = h.stream;
+    std::vector<KeyBatch> kb(nb);
+    int64_t total = 0, words = 0, blocks = 0;
+    for (int b = 0; b < nb; ++b) {
+        OG_CHECK(rows[b] >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
+        for (int c = 0; c < h.kc.n; ++c) {
+            const omni_gpu_column &col = in[static_cast<size_t>(b) * h.kc.n + c];
+            OG_CHECK(col.type_id == h.kc.type[c], OMNI_GPU_ERR_INVALID, "key column %d type %d != %d", c, col.type_id,
+                     h.kc.type[c]);
+            OG_CHECK(rows[b] == 0 || col.values, OMNI_GPU_ERR_INVALID, "key column %d has no values", c);
+            kb[b].values[c] = col.values;
+            kb[b].nulls[c] = col.nulls;
+        }
+        kb[b].rows = rows[b];
+        kb[b].out_base = total;
+        kb[b].null_word_base = words;
+        kb[b].block_start = blocks;
+        total += rows[b];
+        words += (rows[b] + 31) / 32;
+        blocks += (rows[b] + kKeyTile - 1) / kKeyTile;
+    }
+    OG_CHECK(blocks < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "too many key rows for one launch");
+    OG_TRY(out_keys.Reserve(sizeof(long long) * std::max<int64_t>(1, total), s));
+    OG_TRY(out_nulls.Reserve(sizeof(unsigned int) * std::max<int64_t>(1, words), s));
+    // descriptors, then the compact array of their first workgroups that the kernel searches
+    std::vector<uint8_t> desc(sizeof(KeyBatch) * nb + sizeof(int64_t) * nb);
+    std::memcpy(desc.data(), kb.data(), sizeof(KeyBatch) * nb);
+    for (int b = 0; b < nb; ++b) {
+        std::memcpy(desc.data() + sizeof(KeyBatch) * nb + sizeof(int64_t) * b, &kb[b].block_start, sizeof(int64_t));
+    }
+    OG_TRY(h.kdesc.Reserve(desc.size(), s));
+    OG_TRY(CopyToDevice(h.kdesc.p, desc.data(), desc.size(), s));
+    KeyDict d{};
+    if (h.kc.mode != kKeyPacked32 && h.kc.mode != kKeyPacked64) {
+        d.recs = h.dict.as<unsigned long long>();
+        OG_CHECK(h.dict_slots > 0, OMNI_GPU_ERR_STATE, "probe before the build side finished");
+        d.mask = h.dict_slots - 1;
+        d.W = h.kc.W;
+        d.S = DictStride(h.kc.W);
+    }
+    if (blocks > 0) {
+        KernelScope ks(s, INSERT ? "join_translate_build" : "join_translate_probe");
+        const int64_t *starts = reinterpret_cast<const int64_t *>(h.kdesc.as<uint8_t>() + sizeof(KeyBatch) * nb);
+        // equal batches (the last may be shorter) let the kernel find its batch without a search
+        int64_t uniform = nb > 0 && rows[0] > 0 ? (rows[0] + kKeyTile - 1) / kKeyTile : 0;
+        for (int b = 0; b < nb && uniform > 0; ++b) {
+            if (b + 1 < nb ? rows[b] != rows[0] : rows[b] > rows[0] || rows[b] == 0) uniform = 0;
+        }
+        if (h.kc.W <= 2) {
+            join_translate_keys<INSERT, 2><<<static_cast<unsigned int>(blocks), kKeyThreads, 0, s>>>(
+                h.kc, h.kdesc.as<KeyBatch>(), starts, nb, uniform, d, out_keys.as<long long>(),
+                out_nulls.as<unsigned int>());
+        } else {
+            join_translate_keys<INSERT, kJoinKeyMaxWords><<<static_cast<unsigned int>(blocks), kKeyThreads, 0, s>>>(
+                h.kc, h.kdesc.as<KeyBatch>(), starts, nb, uniform, d, out_keys.as<long long>(),
+                out_nulls.as<unsigned int>());
+        }
+        OG_TRY(CheckLaunch("join_translate_keys"));
+    }
+    cols.resize(nb);
+    for (int b = 0; b < nb; ++b) {
+        cols[b] = omni_gpu_column{};
+        cols[b].type_id = OMNI_TYPE_LONG;
+        cols[b].values = out_keys.as<long long>() + kb[b].out_base;
+        cols[b].nulls = reinterpret_cast<const uint8_t *>(out_nulls.as<unsigned int>() + kb[b].null_word_base);
+    }
+    return OMNI_GPU_OK;
+}
+
 }  // namespace omni_gpu
 
 using namespace omni_gpu;
@@ -471,10 +608,10 @@ int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle)
 {
     OG_CHECK(spec && handle, OMNI_GPU_ERR_INVALID, "null argument");
     // single keys the reference keeps in a fixed-width table (HashBuilderOperatorFactory::InitVariant,
-    // hash_builder.cpp:45-80): DOUBLE joins on its 64-bit pattern (the int64 table), BOOLEAN on 0 / 1
-    OG_CHECK(IsIntegerKeyType(spec->key_type) || spec->key_type == OMNI_TYPE_DOUBLE ||
-                 spec->key_type == OMNI_TYPE_BOOLEAN,
-             OMNI_GPU_ERR_INVALID, "join key type %d is not supported (fixed-width single keys only)", spec->key_type);
+    // hash_builder.cpp:45-80) are joined directly -- DOUBLE on its 64-bit pattern (the int64 table), BOOLEAN on
+    // 0 / 1 --, the other key variants through their translation (join_keys.h)
+    JoinKeyCols kc{};
+    OG_TRY(PlanJoinKeys(*spec, kc));
     OG_CHECK(spec->num_payloads >= 0 && spec->num_payloads <= OMNI_GPU_MAX_AGGS, OMNI_GPU_ERR_INVALID,
              "num_payloads out of range");
     for (int p = 0; p < spec->num_payloads; ++p) {
@@ -483,6 +620,9 @@ int omni_gpu_join_build_create(const omni_gpu_join_spec *spec, void **handle)
     }
     std::unique_ptr<JoinHandle> h(new JoinHandle());
     h->spec = *spec;
+    h->kc = kc;
+    h->jkey_type = kc.mode == kKeySingle ? spec->key_type : OMNI_TYPE_LONG;
+    h->spec.key_type = h->jkey_type;
     OG_TRY(AcquireStream(&h->stream));
     OG_TRY(h->scratch.Reserve(64, h->stream));
     *handle = h.release();
@@ -497,8 +637,20 @@ int omni_gpu_join_build_add(void *handle, const omni_gpu_column *key, const omni
     OG_CHECK(rows >= 0, OMNI_GPU_ERR_INVALID, "negative row count");
     if (rows == 0) return OMNI_GPU_OK;
     OG_CHECK(h.n + rows < 0x7FFFFFFF, OMNI_GPU_ERR_INVALID, "build side exceeds 2^31-1 rows");
-    OG_CHECK(key->type_id == h.spec.key_type, OMNI_GPU_ERR_INVALID, "build key type mismatch");
-    OG_CHECK(key->values, OMNI_GPU_ERR_INVALID, "build key has no values");
+    if (h.kc.mode == kKeySingle) {
+        OG_CHECK(key->type_id == h.spec.key_type, OMNI_GPU_ERR_INVALID, "build key type mismatch");
+        OG_CHECK(key->values, OMNI_GPU_ERR_INVALID, "build key has no values");
+    } else {  // `key` holds kc.n columns, translated at finish
+        KeyBatch kb{};
+        for (int c = 0; c < h.kc.n; ++c) {
+            OG_CHECK(key[c].type_id == h.kc.type[c] && key[c].values, OMNI_GPU_ERR_INVALID,
+                     "build key column %d: type %d (expected %d) or no values", c, key[c].type_id, h.kc.type[c]);
+            kb.values[c] = key[c].values;
+            kb.nulls[c] = key[c].nulls;
+        }
+        kb.rows = rows;
+        h.pending_keys.push_back(kb);
+    }
     OG_CHECK(h.spec.num_payloads == 0 || payloads, OMNI_GPU_ERR_INVALID, "null payload columns");
     BuildBatch bb{};
     bb.key = key->values;
@@ -539,6 +691,40 @@ int omni_gpu_join_build_finish(void *handle)
         OG_TRY(h.visited.Reserve(static_cast<size_t>(h.n), s));
         OG_HIP(hipMemsetAsync(h.visited.p, 0, static_cast<size_t>(h.n), s));
     }
+    if (h.kc.mode != kKeySingle) {
+        // every build batch's key columns into int64 join keys (dictionary ids for wide keys); the gather below
+        // reads them as the batches' key columns
+        if (h.kc.mode != kKeyPacked32 && h.kc.mode != kKeyPacked64) {
+            uint64_t cap = 1024;
+            while (cap < static_cast<uint64_t>(2 * h.n)) cap <<= 1;
+            const int S = DictStride(h.kc.W);
+            OG_TRY(h.dict.Reserve(sizeof(unsigned long long) * S * cap, s));
+            h.dict_slots = cap;
+            KernelScope ks(s, "join_init_table");
+            join_dict_clear<<<Blocks(static_cast<int64_t>(cap)), kBlock, 0, s>>>(h.dict.as<unsigned long long>(), S,
+                                                                                  h.kc.W, static_cast<int64_t>(cap));
+            OG_TRY(CheckLaunch("join_dict_clear"));
+        }
+        const int nb = static_cast<int>(h.pending.size());
+        std::vector<int64_t> rows(nb);
+        std::vector<omni_gpu_column> in(static_cast<size_t>(nb) * h.kc.n);
+        for (int b = 0; b < nb; ++b) {
+            rows[b] = h.pending[b].rows;
+            for (int c = 0; c < h.kc.n; ++c) {
+                omni_gpu_column &col = in[static_cast<size_t>(b) * h.kc.n + c];
+                col.type_id = h.kc.type[c];
+                col.values = h.pending_keys[b].values[c];
+                col.nulls = h.pending_keys[b].nulls[c];
+            }
+        }
+        std::vector<omni_gpu_column> cols;
+        if (nb > 0) OG_TRY(TranslateKeys<true>(h, in.data(), rows.data(), nb, h.bkeys, h.bnulls, cols));
+        for (int b = 0; b < nb; ++b) {
+            h.pending[b].key = cols[b].values;
+            h.pending[b].knull = cols[b].nulls;
+        }
+        h.pending_keys.clear();
+    }
     if (h.n == 0) {
         h.max_dup = 0;
         h.pending.clear();
@@ -565,7 +751,7 @@ int omni_gpu_join_build_finish(void *handle)
     // JoinHashTableVariants::BuildHashTable tries the array table for INT / DATE32 / SHORT / LONG / TIMESTAMP
     // keys only (join_hash_table_variants.cpp:700-736); DECIMAL64, DOUBLE and BOOLEAN keys always hash.
     const int kt = h.spec.key_type;
-    const bool arrayCandidate = h.spec.strategy == 0 &&
+    const bool arrayCandidate = h.spec.strategy == 0 && h.kc.mode == kKeySingle &&
                                 (kt == OMNI_TYPE_INT || kt == OMNI_TYPE_DATE32 || kt == OMNI_TYPE_SHORT ||
                                  kt == OMNI_TYPE_LONG || kt == OMNI_TYPE_TIMESTAMP);
     int64_t blocks = 0;
@@ -775,6 +961,12 @@ int omni_gpu_join_probe(void *handle, const omni_gpu_column *probe_key, int64_t 
 {
     OG_CHECK(handle && probe_key, OMNI_GPU_ERR_INVALID, "null argument");
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
+    std::vector<omni_gpu_column> tk;
+    if (h.kc.mode != kKeySingle) {  // the key columns into int64 join keys (a key not in the build side: NULL)
+        OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "probe before the build side finished");
+        OG_TRY(TranslateKeys<false>(h, probe_key, &rows, 1, h.pkeys, h.pnulls, tk));
+        probe_key = tk.data();
+    }
     ProbeBatch b{};
     OG_TRY(FillBatch(b, *probe_key, rows, probe_row_base, 0, 0, h.spec.key_type));
     return RunProbe(h, &b, 1, (rows + kProbeTile - 1) / kProbeTile, (rows + kPartTile - 1) / kPartTile,
@@ -787,6 +979,12 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
 {
     OG_CHECK(handle && keys && rows && num_batches > 0, OMNI_GPU_ERR_INVALID, "null argument or no batches");
     JoinHandle &h = *static_cast<JoinHandle *>(handle);
+    std::vector<omni_gpu_column> tk;
+    if (h.kc.mode != kKeySingle) {  // batch b's key columns are keys[b * num_keys ..]
+        OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "probe before the build side finished");
+        OG_TRY(TranslateKeys<false>(h, keys, rows, num_batches, h.pkeys, h.pnulls, tk));
+        keys = tk.data();
+    }
     std::vector<ProbeBatch> bs(num_batches);
     int64_t blocks = 0, pblocks = 0, base = 0;
     for (int i = 0; i < num_batches; ++i) {
@@ -841,6 +1039,11 @@ int omni_gpu_join_probe_exists(void *handle, const omni_gpu_column *probe_key, i
     OG_CHECK(h.finished, OMNI_GPU_ERR_STATE, "probe before the build side finished");
     OG_CHECK(rows == 0 || out_exists, OMNI_GPU_ERR_INVALID, "null out_exists");
     OG_TRY(ResolveBuildFlags(h));
+    std::vector<omni_gpu_column> tk;
+    if (h.kc.mode != kKeySingle) {
+        OG_TRY(TranslateKeys<false>(h, probe_key, &rows, 1, h.pkeys, h.pnulls, tk));
+        probe_key = tk.data();
+    }
     ProbeBatch b{};
     OG_TRY(FillBatch(b, *probe_key, rows, 0, 0, 0, h.spec.key_type));
     OG_CHECK(rows < (1LL << 31) * kBlock, OMNI_GPU_ERR_INVALID, "probe batch too large");

@@ -335,18 +335,32 @@ GpuHashBuilderOperatorFactory *GpuHashBuilderOperatorFactory::CreateHashBuilderO
     if (operatorCount <= 0 || operatorCount > 10000) {  // hash_builder.cpp:33-35
         throw OmniException("OPERATOR_RUNTIME_ERROR", "operatorCount is not in the acceptable range [1, 10000].");
     }
-    if (buildHashColsCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one join key column is supported");
+    if (buildHashColsCount < 1 || buildHashColsCount > OMNI_GPU_MAX_KEYS) {
+        throw OmniException("UNSUPPORTED_ERROR", "1 to " + std::to_string(OMNI_GPU_MAX_KEYS) +
+                                                     " join key columns are supported");
+    }
     if (operatorCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one build partition is supported");
-    return new GpuHashBuilderOperatorFactory(joinType, buildSide, buildTypes, buildHashCols[0]);
+    for (int32_t i = 0; i < buildHashColsCount; ++i) {
+        if (buildHashCols[i] < 0 || buildHashCols[i] >= buildTypes.GetSize()) {
+            throw OmniException("INVALID_ARGUMENT", "join key column out of range");
+        }
+    }
+    return new GpuHashBuilderOperatorFactory(joinType, buildSide, buildTypes,
+                                             std::vector<int32_t>(buildHashCols, buildHashCols + buildHashColsCount));
 }
 
+// The key variant follows the key columns as in HashBuilderOperatorFactory::InitVariant (hash_builder.cpp:44-93);
+// several columns or a DECIMAL128 column go through the library's key translation.
 GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide,
-                                                             const type::DataTypes &buildTypes, int32_t keyCol)
-    : joinType_(joinType), buildSide_(buildSide), buildTypes_(buildTypes), keyCol_(keyCol)
+                                                             const type::DataTypes &buildTypes,
+                                                             std::vector<int32_t> keyCols)
+    : joinType_(joinType), buildSide_(buildSide), buildTypes_(buildTypes), keyCols_(std::move(keyCols))
 {
     EnsureInit();
     omni_gpu_join_spec spec{};
-    spec.key_type = buildTypes.GetType(keyCol)->GetId();
+    spec.key_type = buildTypes.GetType(keyCols_[0])->GetId();
+    spec.num_keys = static_cast<int32_t>(keyCols_.size());
+    for (size_t i = 0; i < keyCols_.size(); ++i) spec.key_types[i] = buildTypes.GetType(keyCols_[i])->GetId();
     spec.num_payloads = buildTypes.GetSize();  // every build column is retained, like the reference's batches
     if (spec.num_payloads > OMNI_GPU_MAX_AGGS) throw OmniException("UNSUPPORTED_ERROR", "too many build columns");
     for (int32_t i = 0; i < spec.num_payloads; ++i) {
@@ -382,7 +396,9 @@ int32_t GpuHashBuilderOperator::AddInput(vec::VectorBatch *vecBatch)
             auto &nb = *f_.retained_.back();
             cols[c] = Upload(vecBatch->Get(c), rows, vb, nb);
         }
-        Check(omni_gpu_join_build_add(f_.handle_, &cols[f_.keyCol_], cols.data(), rows), "omni_gpu_join_build_add");
+        std::vector<omni_gpu_column> keys;
+        for (int32_t k : f_.keyCols_) keys.push_back(cols[k]);
+        Check(omni_gpu_join_build_add(f_.handle_, keys.data(), cols.data(), rows), "omni_gpu_join_build_add");
     }
     vec::VectorHelper::FreeVecBatch(vecBatch);
     return 0;
@@ -412,14 +428,25 @@ GpuLookupJoinOperatorFactory *GpuLookupJoinOperatorFactory::CreateLookupJoinOper
     bool isShuffleExchangeBuildPlan, void *overflowConfig)
 {
     if (filterExpr != nullptr) throw OmniException("UNSUPPORTED_ERROR", "join filters are not on the GPU path");
-    if (probeHashColsCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one join key column is supported");
+    auto *builder = reinterpret_cast<GpuHashBuilderOperatorFactory *>(hashBuilderFactoryAddr);
+    if (probeHashColsCount != static_cast<int32_t>(builder->KeyCols().size())) {
+        throw OmniException("INVALID_ARGUMENT", "probe and build sides have different numbers of join keys");
+    }
+    for (int32_t i = 0; i < probeHashColsCount; ++i) {
+        // the probe's keys are translated as the build's: their types must agree column by column
+        if (probeHashCols[i] < 0 || probeHashCols[i] >= probeTypes.GetSize() ||
+            probeTypes.GetType(probeHashCols[i])->GetId() !=
+                builder->BuildTypes().GetType(builder->KeyCols()[i])->GetId()) {
+            throw OmniException("UNSUPPORTED_ERROR", "probe join key types must equal the build join key types");
+        }
+    }
     auto *f = new GpuLookupJoinOperatorFactory();
     f->probeTypes_ = probeTypes;
     f->probeOutputCols_.assign(probeOutputCols, probeOutputCols + probeOutputColsCount);
-    f->probeKeyCol_ = probeHashCols[0];
+    f->probeKeyCols_.assign(probeHashCols, probeHashCols + probeHashColsCount);
     f->buildOutputCols_.assign(buildOutputCols, buildOutputCols + buildOutputColsCount);
     // the build side is shared by raw factory address (lookup_join.cpp:84-95)
-    f->builder_ = reinterpret_cast<GpuHashBuilderOperatorFactory *>(hashBuilderFactoryAddr);
+    f->builder_ = builder;
     for (int32_t i = 0; i < buildOutputColsCount; ++i) {
         if (buildOutputTypes.GetType(i)->GetId() != f->builder_->BuildTypes().GetType(buildOutputCols[i])->GetId()) {
             throw OmniException("INVALID_ARGUMENT", "build output type does not match the build column");
@@ -452,10 +479,18 @@ int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
     int64_t max_dup = 0, slots = 0;
     Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
     int64_t cap = std::max<int64_t>(1, rows * std::max<int64_t>(1, max_dup));
-    omni_gpu_column key = Upload(vecBatch->Get(f_.probeKeyCol_), rows, keyBuf_, nullBuf_);
+    std::vector<omni_gpu_column> keys(f_.probeKeyCols_.size());
+    while (keyBufs_.size() < keys.size()) {
+        keyBufs_.emplace_back(new DeviceBuffer());
+        keyNullBufs_.emplace_back(new DeviceBuffer());
+    }
+    for (size_t k = 0; k < keys.size(); ++k) {
+        keys[k] = Upload(vecBatch->Get(f_.probeKeyCols_[k]), rows, *keyBufs_[k], *keyNullBufs_[k]);
+    }
     if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) {  // one output row per probe row
         rowsBuf_.Reserve(static_cast<size_t>(rows));
-        Check(omni_gpu_join_probe_exists(f_.builder_->Table(), &key, rows, static_cast<uint8_t *>(rowsBuf_.Get())),
+        Check(omni_gpu_join_probe_exists(f_.builder_->Table(), keys.data(), rows,
+                                         static_cast<uint8_t *>(rowsBuf_.Get())),
               "omni_gpu_join_probe_exists");
         exists_.resize(rows);
         Check(omni_gpu_memcpy_d2h(exists_.data(), rowsBuf_.Get(), static_cast<size_t>(rows)), "download flags");
@@ -481,7 +516,7 @@ int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
         outs[col].values = payBufs_[col]->Get();
         outs[col].nulls = static_cast<uint8_t *>(payNullBufs_[col]->Get());
     }
-    Check(omni_gpu_join_probe(f_.builder_->Table(), &key, rows, 0, static_cast<int64_t *>(rowsBuf_.Get()), outs.data(),
+    Check(omni_gpu_join_probe(f_.builder_->Table(), keys.data(), rows, 0, static_cast<int64_t *>(rowsBuf_.Get()), outs.data(),
                               cap, &matches_),
           "omni_gpu_join_probe");
     matchRows_.resize(matches_);

@@ -124,15 +124,16 @@ public:
     bool Finished() const { return finished_; }
     JoinType GetJoinType() const { return joinType_; }
     BuildSide GetBuildSide() const { return buildSide_; }
+    const std::vector<int32_t> &KeyCols() const { return keyCols_; }
 
 private:
     friend class GpuHashBuilderOperator;
     GpuHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide, const type::DataTypes &buildTypes,
-                                  int32_t keyCol);
+                                  std::vector<int32_t> keyCols);
     JoinType joinType_;
     BuildSide buildSide_;
     type::DataTypes buildTypes_;
-    int32_t keyCol_;
+    std::vector<int32_t> keyCols_;  // buildHashCols
     void *handle_ = nullptr;
     bool finished_ = false;
     std::vector<std::unique_ptr<DeviceBuffer>> retained_;  // build batches in device memory until finish
@@ -163,7 +164,7 @@ private:
     friend class GpuLookupJoinOperator;
     type::DataTypes probeTypes_;
     std::vector<int32_t> probeOutputCols_;
-    int32_t probeKeyCol_ = 0;
+    std::vector<int32_t> probeKeyCols_;  // probeHashCols
     std::vector<int32_t> buildOutputCols_;
     GpuHashBuilderOperatorFactory *builder_ = nullptr;
 };
@@ -184,7 +185,8 @@ private:
     std::vector<uint8_t> exists_;                      // existence join: one flag per probe row
     int64_t matches_ = 0;
     int64_t emitted_ = 0;
-    DeviceBuffer keyBuf_, nullBuf_, rowsBuf_;
+    DeviceBuffer rowsBuf_;
+    std::vector<std::unique_ptr<DeviceBuffer>> keyBufs_, keyNullBufs_;  // per probe key column
     std::vector<std::unique_ptr<DeviceBuffer>> payBufs_, payNullBufs_;
 };
 

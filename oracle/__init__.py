@@ -142,6 +142,8 @@ def lib():
             "oracle_agg_stream": (i64, [vp, vp, vp, vp, i32, i32, vp]),
             "oracle_join_create": (vp, [i32]),
             "oracle_join_add_batch": (i32, [vp, vp, vp, i32, i64]),
+            "oracle_join_create_multi": (vp, [vp, i32]),
+            "oracle_join_translate": (None, [vp, vp, i32, i64, i32, vp, vp]),
             "oracle_join_finish": (i32, [vp]),
             "oracle_join_is_array": (i32, [vp]),
             "oracle_join_probe": (i64, [vp, vp, i64, i64, vp, vp, i64]),
@@ -329,25 +331,56 @@ class JoinResult:
     is_array: bool
 
 
-def hash_join(build_key: Col, build_payloads: Sequence[Col], probe_key: Col, build_batch: int,
+def _is_wide(key) -> bool:
+    return isinstance(key, (list, tuple)) or key.type_id == DECIMAL128
+
+
+def _join_create(L, build_key):
+    """A restated join for a single fixed-width key, or for several key columns / one DECIMAL128 key (their keys
+    translated into int64 join keys by the oracle's key codec)."""
+    if not _is_wide(build_key):
+        return L.oracle_join_create(build_key.type_id)
+    cols = list(build_key) if isinstance(build_key, (list, tuple)) else [build_key]
+    types = np.array([c.type_id for c in cols], np.int32)
+    return L.oracle_join_create_multi(_ptr(types), len(cols))
+
+
+def _key_slice(L, h, key, a: int, b: int, insert: bool):
+    """The key of rows [a, b) as one oracle column array: the column itself, or the translated int64 join key."""
+    if not _is_wide(key):
+        return _ocols([key.slice(a, b)], 1)
+    cols = list(key) if isinstance(key, (list, tuple)) else [key]
+    karr, keep = _ocols([c.slice(a, b) for c in cols], len(cols))
+    out = np.zeros(max(1, b - a), np.int64)
+    bm = np.zeros((b - a + 7) // 8 + 1, np.uint8)
+    L.oracle_join_translate(h, karr, len(cols), b - a, 1 if insert else 0, _ptr(out), _ptr(bm))
+    return _ocols([Col(LONG, out[:b - a], unpack_nulls(bm, b - a))], 1)
+
+
+def _key_len(key) -> int:
+    return len(key[0]) if isinstance(key, (list, tuple)) else len(key)
+
+
+def hash_join(build_key, build_payloads: Sequence[Col], probe_key, build_batch: int,
               probe_batch: int) -> JoinResult:
-    """Restated HashBuilderOperator + LookupJoinOperator (inner join, single key)."""
+    """Restated HashBuilderOperator + LookupJoinOperator (inner join).  build_key / probe_key: one key column, or a
+    list of key columns."""
     L = lib()
-    h = L.oracle_join_create(build_key.type_id)
+    h = _join_create(L, build_key)
     try:
-        nb = len(build_key)
+        nb = _key_len(build_key)
         for a in range(0, nb, build_batch):
             b = min(nb, a + build_batch)
-            karr, k1 = _ocols([build_key.slice(a, b)], 1)
+            karr, k1 = _key_slice(L, h, build_key, a, b, True)
             parr, k2 = _ocols([p.slice(a, b) for p in build_payloads], max(1, len(build_payloads)))
             L.oracle_join_add_batch(h, karr, parr, len(build_payloads), b - a)
         L.oracle_join_finish(h)
         is_array = bool(L.oracle_join_is_array(h))
         np_rows, nb_rows = [], []
-        npr = len(probe_key)
+        npr = _key_len(probe_key)
         for a in range(0, npr, probe_batch):
             b = min(npr, a + probe_batch)
-            karr, k1 = _ocols([probe_key.slice(a, b)], 1)
+            karr, k1 = _key_slice(L, h, probe_key, a, b, False)
             m = L.oracle_join_probe(h, karr, b - a, a, None, None, 0)
             pr = np.zeros(max(m, 1), np.int64)
             br = np.zeros(max(m, 1), np.int64)
@@ -398,33 +431,34 @@ class TypedJoinResult:
     exists: Optional[np.ndarray]      # EXISTENCE: one flag per probe row
 
 
-def hash_join_typed(build_key: Col, build_payloads: Sequence[Col], probe_key: Col, build_batch: int,
+def hash_join_typed(build_key, build_payloads: Sequence[Col], probe_key, build_batch: int,
                     probe_batch: int, join_type: int, build_side: int = BUILD_UNKNOWN) -> TypedJoinResult:
-    """Restated HashBuilderOperator + LookupJoinOperator (+ LookupOuterJoinOperator) for any join type."""
+    """Restated HashBuilderOperator + LookupJoinOperator (+ LookupOuterJoinOperator) for any join type; keys as in
+    hash_join."""
     mode, mark = join_mode(join_type, build_side)
     L = lib()
-    h = L.oracle_join_create(build_key.type_id)
+    h = _join_create(L, build_key)
     try:
-        nb = len(build_key)
+        nb = _key_len(build_key)
         for a in list(range(0, nb, build_batch)) or [0]:  # an empty build side still declares its payload types
             b = min(nb, a + build_batch)
-            karr, k1 = _ocols([build_key.slice(a, b)], 1)
+            karr, k1 = _key_slice(L, h, build_key, a, b, True)
             parr, k2 = _ocols([p.slice(a, b) for p in build_payloads], max(1, len(build_payloads)))
             L.oracle_join_add_batch(h, karr, parr, len(build_payloads), b - a)
         L.oracle_join_finish(h)
-        npr = len(probe_key)
+        npr = _key_len(probe_key)
         if mode is None:
             ex = np.zeros(max(1, npr), np.uint8)
             for a in range(0, npr, probe_batch):
                 b = min(npr, a + probe_batch)
-                karr, k1 = _ocols([probe_key.slice(a, b)], 1)
+                karr, k1 = _key_slice(L, h, probe_key, a, b, False)
                 L.oracle_join_exists(h, karr, b - a, _ptr(ex[a:b]))
             return TypedJoinResult(np.zeros(0, np.int64), np.zeros(0, np.int64), [], np.zeros(0, np.int64), [],
                                    ex[:npr])
         np_rows, nb_rows = [], []
         for a in range(0, npr, probe_batch):
             b = min(npr, a + probe_batch)
-            karr, k1 = _ocols([probe_key.slice(a, b)], 1)
+            karr, k1 = _key_slice(L, h, probe_key, a, b, False)
             m = L.oracle_join_probe_typed(h, karr, b - a, a, mode, int(mark), None, None, 0)
             pr = np.zeros(max(m, 1), np.int64)
             br = np.zeros(max(m, 1), np.int64)

@@ -9,7 +9,12 @@ with the GPU kernels.  It encodes only the result semantics the reference pins:
   - SUM/AVG/MIN/MAX over only NULLs give NULL, COUNT gives 0; a NULL key forms its own group
     (sum_flatim_aggregator.h:243-257, max_aggregator.cpp:34-55, column_marshaller.h:225-234);
   - the inner join skips NULL keys on both sides (join_hash_table_variants.cpp:254-283,
-    lookup_join.cpp:973-976).
+    lookup_join.cpp:973-976);
+  - several join key columns compare as the reference's key variant does (hash_builder.cpp:44-119): SHORT / INT /
+    LONG / DECIMAL64 columns of at most 128 bits in total as one integer, key = (key * 2^bits OR (value mod 2^W))
+    mod 2^W with W the key width 32, 64 or 128 (ComputeMultiColKey, join_hash_table_variants.cpp:105-146, where the
+    value is sign-extended into the key), other combinations and a single DECIMAL128 key by the tuple of their
+    stored values; a NULL in any key column matches nothing (:340-400).
 """
 from __future__ import annotations
 
@@ -17,7 +22,9 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from . import (AVG, COUNT_ALL, COUNT_COLUMN, DECIMAL128, DECIMAL64, DOUBLE, MAX, MIN, SUM, Col, np_dtype)
+from . import (AVG, BUILD_LEFT, BUILD_RIGHT, COUNT_ALL, COUNT_COLUMN, DECIMAL128, DECIMAL64, DOUBLE, INT,
+               JOIN_EXISTENCE, JOIN_FULL, JOIN_LEFT, JOIN_LEFT_ANTI, JOIN_LEFT_SEMI, JOIN_RIGHT, LONG, MAX, MIN, SHORT,
+               SUM, Col, np_dtype)
 
 DEC128_MAX = 10**38 - 1
 I64_MIN, I64_MAX = -(2**63), 2**63 - 1
@@ -179,3 +186,71 @@ def inner_join(build_key: Col, probe_key: Col):
     build_rows = order[starts + offs].astype(np.int64)
     o = np.lexsort((build_rows, probe_rows))
     return probe_rows[o], build_rows[o]
+
+
+_KEY_BITS = {SHORT: 16, INT: 32, LONG: 64, DECIMAL64: 64}
+
+
+def join_keys(cols: Sequence[Col]) -> List[Optional[object]]:
+    """The value a row's key columns compare by (None: a NULL key column), for one or several key columns."""
+    n = len(cols[0])
+    null = np.zeros(n, bool)
+    for c in cols:
+        if c.nulls is not None:
+            null |= c.nulls
+    bits = [_KEY_BITS.get(c.type_id) for c in cols]
+    total = sum(b for b in bits if b is not None)
+    width = None
+    if len(cols) > 1 and None not in bits:
+        width = 32 if total <= 32 else 64 if total <= 64 else 128 if total <= 128 else None
+    if width is not None:
+        mod = 1 << width
+        vals = [c.values.astype(np.int64).tolist() for c in cols]
+        out = []
+        for r in range(n):
+            if null[r]:
+                out.append(None)
+                continue
+            k = 0
+            for b, v in zip(bits, vals):
+                k = ((k << b) | (v[r] % mod)) % mod  # the value sign-extended to `width` bits, then OR-ed in
+            out.append(k)
+        return out
+    stored = []
+    for c in cols:
+        if c.type_id == DECIMAL128:
+            stored.append([tuple(x) for x in c.values.astype(np.uint64).tolist()])
+        elif c.type_id == DOUBLE:
+            stored.append(c.values.view(np.int64).tolist())
+        else:
+            stored.append(c.values.tolist())
+    return [None if null[r] else tuple(s[r] for s in stored) for r in range(n)]
+
+
+def typed_join(build_keys: Sequence[Optional[object]], probe_keys: Sequence[Optional[object]], join_type: int,
+               build_side: int = 0):
+    """Each join type from its definition over hashable keys (None matches nothing).  Returns (pairs, outer):
+    sorted (probe, build) pairs with build -1 for a kept unmatched probe row and the unmatched build rows when the
+    type keeps them (else None); semi / anti give the sorted probe rows, existence one flag per probe row."""
+    index = {}
+    for b, k in enumerate(build_keys):
+        if k is not None:
+            index.setdefault(k, []).append(b)
+    matches = [[] if k is None else index.get(k, []) for k in probe_keys]
+    same = (join_type, build_side) in ((JOIN_LEFT, BUILD_LEFT), (JOIN_RIGHT, BUILD_RIGHT))
+    keep_probe = join_type == JOIN_FULL or (join_type in (JOIN_LEFT, JOIN_RIGHT) and not same)
+    keep_build = join_type == JOIN_FULL or same
+    if join_type == JOIN_EXISTENCE:
+        return [bool(m) for m in matches], None
+    if join_type == JOIN_LEFT_SEMI:
+        return sorted(p for p, m in enumerate(matches) if m), None
+    if join_type == JOIN_LEFT_ANTI:
+        return sorted(p for p, m in enumerate(matches) if not m), None
+    pairs = [(p, b) for p, m in enumerate(matches) for b in m]
+    if keep_probe:
+        pairs += [(p, -1) for p, m in enumerate(matches) if not m]
+    outer = None
+    if keep_build:
+        seen = {b for m in matches for b in m}
+        outer = [b for b in range(len(build_keys)) if b not in seen]
+    return sorted(pairs), outer

@@ -26,6 +26,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "../include/omni_gpu.h"
@@ -933,9 +934,119 @@ void PartitionIds(const HostColumn *cols, int32_t ncols, int64_t rows, int32_t p
 // Inner hash join: HashBuilderOperator + JoinHashTableVariants + LookupJoinOperator
 // ================================================================================================
 
+// Join keys other than one fixed-width column: the key the reference's table compares, as an int64 join key.
+// HashBuilderOperatorFactory::InitVariant (hash_builder.cpp:44-93) with GetTypeLength (:96-119): several SHORT (16
+// bits) / INT (32) / LONG, DECIMAL64 (64) columns of at most 32, 64 or 128 bits in total are packed by
+// ComputeMultiColKey (join_hash_table_variants.cpp:105-146) into int32 / int64 / int128; a single DECIMAL128 column
+// is a Decimal128 key; other combinations are serialised (a StringRef key of the values' bytes).  A row with a NULL
+// in any key column has no key (EmplaceMultiKey..., :340-400).  int32 / int64 keys are the join key itself; wider
+// keys are numbered by a dictionary of the build side's keys, and a probe key outside it has no key.
+class JoinKeyCodec {
+public:
+    explicit JoinKeyCodec(const std::vector<int32_t> &types) : types_(types)
+    {
+        const size_t n = types.size();
+        if (n == 1) {
+            kind_ = Kind::bytes;  // Decimal128: equality of both words
+            return;
+        }
+        int bits = 0;
+        bool fixedInt = true;
+        for (int32_t t : types) {
+            switch (t) {
+                case OMNI_TYPE_SHORT: bits += 16; break;
+                case OMNI_TYPE_INT: bits += 32; break;
+                case OMNI_TYPE_LONG:
+                case OMNI_TYPE_DECIMAL64: bits += 64; break;
+                default: fixedInt = false;
+            }
+        }
+        if (fixedInt && bits <= 32) kind_ = Kind::int32;
+        else if (fixedInt && bits <= 64) kind_ = Kind::int64;
+        else if (fixedInt && bits <= 128) kind_ = Kind::int128;
+        else kind_ = Kind::bytes;
+    }
+
+    // The join key of row r; false when the row has no key (a NULL column, or a probe key not in the build side).
+    bool Encode(const HostColumn *cols, int64_t r, bool insert, int64_t &key)
+    {
+        for (size_t c = 0; c < types_.size(); ++c) {
+            if (IsNull(cols[c], r)) return false;
+        }
+        std::string bytes;
+        switch (kind_) {
+            case Kind::int32: {  // key = key << bits | (int32_t)value
+                int32_t k = 0;
+                for (size_t c = 0; c < types_.size(); ++c) {
+                    const int shift = types_[c] == OMNI_TYPE_SHORT ? 16 : 32;
+                    k = static_cast<int32_t>((static_cast<uint32_t>(k) << shift) |
+                                             static_cast<uint32_t>(static_cast<int32_t>(LoadInt(cols[c], r))));
+                }
+                key = k;
+                return true;
+            }
+            case Kind::int64: {
+                int64_t k = 0;
+                for (size_t c = 0; c < types_.size(); ++c) {
+                    const int shift = types_[c] == OMNI_TYPE_SHORT ? 16 : (types_[c] == OMNI_TYPE_INT ? 32 : 64);
+                    k = static_cast<int64_t>((static_cast<uint64_t>(k) << shift) |
+                                             static_cast<uint64_t>(LoadInt(cols[c], r)));
+                }
+                key = k;
+                return true;
+            }
+            case Kind::int128: {
+                int128 k = 0;
+                for (size_t c = 0; c < types_.size(); ++c) {
+                    const int shift = types_[c] == OMNI_TYPE_SHORT ? 16 : (types_[c] == OMNI_TYPE_INT ? 32 : 64);
+                    k = static_cast<int128>((static_cast<unsigned __int128>(k) << shift) |
+                                            static_cast<unsigned __int128>(static_cast<int128>(LoadInt(cols[c], r))));
+                }
+                bytes.assign(reinterpret_cast<const char *>(&k), sizeof(k));
+                break;
+            }
+            case Kind::bytes:
+                for (size_t c = 0; c < types_.size(); ++c) {
+                    const int w = TypeWidth(types_[c]);
+                    bytes.append(static_cast<const char *>(cols[c].values) + static_cast<int64_t>(w) * r, w);
+                }
+                break;
+        }
+        auto it = dict_.find(bytes);
+        if (it != dict_.end()) {
+            key = it->second;
+            return true;
+        }
+        if (!insert) return false;
+        key = static_cast<int64_t>(dict_.size());
+        dict_.emplace(std::move(bytes), key);
+        return true;
+    }
+
+private:
+    enum class Kind { int32, int64, int128, bytes };
+    std::vector<int32_t> types_;
+    Kind kind_ = Kind::bytes;
+    std::unordered_map<std::string, int64_t> dict_;
+};
+
 class HashJoin {
 public:
     explicit HashJoin(int32_t keyType) : keyType_(keyType) {}
+    explicit HashJoin(const std::vector<int32_t> &keyTypes) : keyType_(OMNI_TYPE_LONG), codec_(new JoinKeyCodec(keyTypes))
+    {
+    }
+
+    // The key columns of a batch (several columns or one DECIMAL128) as one int64 key column with its bitmap.
+    void Translate(const HostColumn *cols, int64_t rows, bool insert, int64_t *out, uint8_t *bitmap)
+    {
+        std::memset(bitmap, 0, static_cast<size_t>(rows + 7) / 8);
+        for (int64_t r = 0; r < rows; ++r) {
+            int64_t k = 0;
+            if (!codec_->Encode(cols, r, insert, k)) bitmap[r >> 3] |= static_cast<uint8_t>(1u << (r & 7));
+            out[r] = k;
+        }
+    }
 
     // HashBuilderOperator::AddInput retains the batch (hash_builder.cpp:169-180).
     void AddBatch(const HostColumn &key, const HostColumn *payloads, int32_t npay, int64_t rows)
@@ -975,9 +1086,9 @@ public:
         }
         // Only INT / DATE32 / SHORT / LONG / TIMESTAMP keys try it (join_hash_table_variants.cpp:700-736), and
         // max - min must not overflow the key type (TryToBuildArrayTable :792-794).
-        const bool candidate = keyType_ == OMNI_TYPE_INT || keyType_ == OMNI_TYPE_DATE32 ||
+        const bool candidate = !codec_ && (keyType_ == OMNI_TYPE_INT || keyType_ == OMNI_TYPE_DATE32 ||
                                keyType_ == OMNI_TYPE_SHORT || keyType_ == OMNI_TYPE_LONG ||
-                               keyType_ == OMNI_TYPE_TIMESTAMP;
+                               keyType_ == OMNI_TYPE_TIMESTAMP);
         bool arrayOk = candidate && mx >= mn;
         if (arrayOk) {
             int128 range = static_cast<int128>(mx) - mn;
@@ -1147,6 +1258,7 @@ private:
     }
 
     int32_t keyType_;
+    std::unique_ptr<JoinKeyCodec> codec_;
     std::vector<int64_t> keys_;
     std::vector<uint8_t> keyNull_;
     std::vector<int32_t> payTypes_;
@@ -1338,6 +1450,19 @@ int32_t oracle_join_add_batch(void *h, const oracle_column *key, const oracle_co
     for (int i = 0; i < npay; ++i) p.push_back({payloads[i].type_id, payloads[i].scale, payloads[i].values, payloads[i].nulls});
     static_cast<HashJoin *>(h)->AddBatch(k, p.data(), npay, rows);
     return 0;
+}
+void *oracle_join_create_multi(const int32_t *types, int32_t n)
+{
+    return new HashJoin(std::vector<int32_t>(types, types + n));
+}
+// Key columns keys[0..n) of `rows` rows into int64 join keys and their bitmap (insert: the build side numbers its
+// wide keys; the probe side only looks them up).
+void oracle_join_translate(void *h, const oracle_column *keys, int32_t n, int64_t rows, int32_t insert, int64_t *out,
+                           uint8_t *bitmap)
+{
+    std::vector<HostColumn> k;
+    for (int i = 0; i < n; ++i) k.push_back({keys[i].type_id, keys[i].scale, keys[i].values, keys[i].nulls});
+    static_cast<HashJoin *>(h)->Translate(k.data(), rows, insert != 0, out, bitmap);
 }
 int32_t oracle_join_finish(void *h)
 {

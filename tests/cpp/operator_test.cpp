@@ -469,18 +469,19 @@ static void TestAggDecimalOverflowThrows()
     Operator::DeleteOperator(op);
 }
 
-// Two-column LONG build and probe batches (NULL where `nulls` lists the row), key column `keyCol` on both sides,
-// every column output; returns the lookup operator's rows and, when `outer`, the outer operator's rows.
-static std::pair<Rows, Rows> RunTypedJoin(JoinType jt, BuildSide bs, int32_t keyCol,
+// Two-column LONG build and probe batches (NULL where `nulls` lists the row of the first key column), key columns
+// `keyCols` on both sides, every column output; returns the lookup operator's rows and, when `outer`, the outer
+// operator's rows.
+static std::pair<Rows, Rows> RunTypedJoin(JoinType jt, BuildSide bs, std::vector<int32_t> keyCols,
                                           const std::vector<std::vector<int64_t>> &bcols, std::vector<int> bnulls,
                                           const std::vector<std::vector<int64_t>> &pcols, bool outer)
 {
     DataTypes types({LongType(), LongType()});
     auto *build = new VectorBatch(static_cast<int32_t>(bcols[0].size()));
     for (int c = 0; c < 2; ++c) build->Append(MakeVector<int64_t>(OMNI_LONG, bcols[c]));
-    for (int r : bnulls) build->Get(keyCol)->SetNull(r);
-    int32_t hashCols[1] = {keyCol};
-    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(jt, bs, types, hashCols, 1, 1);
+    for (int r : bnulls) build->Get(keyCols[0])->SetNull(r);
+    const int32_t nk = static_cast<int32_t>(keyCols.size());
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(jt, bs, types, keyCols.data(), nk, 1);
     Operator *builder = bf->CreateOperator();
     builder->AddInput(build);
     VectorBatch *none = nullptr;
@@ -489,10 +490,10 @@ static std::pair<Rows, Rows> RunTypedJoin(JoinType jt, BuildSide bs, int32_t key
     for (int c = 0; c < 2; ++c) probe->Append(MakeVector<int64_t>(OMNI_LONG, pcols[c]));
     const bool probeOnly = jt == OMNI_JOIN_TYPE_LEFT_SEMI || jt == OMNI_JOIN_TYPE_LEFT_ANTI ||
                            jt == OMNI_JOIN_TYPE_EXISTENCE;
-    int32_t probeOut[2] = {0, 1}, probeHash[1] = {keyCol}, buildOut[2] = {0, 1};
+    int32_t probeOut[2] = {0, 1}, buildOut[2] = {0, 1};
     DataTypes buildOutTypes = probeOnly ? DataTypes() : DataTypes({LongType(), LongType()});
     auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
-        types, probeOut, jt == OMNI_JOIN_TYPE_EXISTENCE ? 0 : 2, probeHash, 1, buildOut, probeOnly ? 0 : 2,
+        types, probeOut, jt == OMNI_JOIN_TYPE_EXISTENCE ? 0 : 2, keyCols.data(), nk, buildOut, probeOnly ? 0 : 2,
         buildOutTypes, reinterpret_cast<int64_t>(bf), nullptr, false, nullptr);
     Operator *lookup = lf->CreateOperator();
     lookup->AddInput(probe);
@@ -522,7 +523,7 @@ static const std::vector<std::vector<int64_t>> kProbe4 = {{1, 2, 3, 4}, {11, 22,
 // join_test.cpp:773-833 (TestLeftEqualityJoin)
 static void TestLeftJoin()
 {
-    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT, OMNI_BUILD_UNKNOWN, 1, kBuild4, {}, kProbe4, false);
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT, OMNI_BUILD_UNKNOWN, {1}, kBuild4, {}, kProbe4, false);
     Rows want = {{"1", "11", "2", "11"}, {"2", "22", "null", "null"}, {"3", "33", "4", "33"}, {"4", "44", "null", "null"}};
     EXPECT_TRUE(SameRowSet(r.first, want));
 }
@@ -531,7 +532,7 @@ static void TestLeftJoin()
 // the unmatched build rows
 static void TestLeftJoinBuildLeft()
 {
-    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT, OMNI_BUILD_LEFT, 1, kBuild4, {}, kProbe4, true);
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT, OMNI_BUILD_LEFT, {1}, kBuild4, {}, kProbe4, true);
     Rows want = {{"2", "11", "1", "11"}, {"4", "33", "3", "33"}};  // (build col0, build col1, probe col0, probe col1)
     EXPECT_TRUE(SameRowSet(r.first, want));
     Rows wantOuter = {{"3", "333", "null", "null"}, {"1", "111", "null", "null"}};
@@ -541,7 +542,7 @@ static void TestLeftJoinBuildLeft()
 // join_test.cpp:2881-2974 (TestOneKeyFullEqualityJoinWithBuildNull)
 static void TestFullJoinBuildNull()
 {
-    auto r = RunTypedJoin(OMNI_JOIN_TYPE_FULL, OMNI_BUILD_UNKNOWN, 1, {{1, 2, 3, 4, 5, 6}, {111, 11, 333, 33, 11, 11}},
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_FULL, OMNI_BUILD_UNKNOWN, {1}, {{1, 2, 3, 4, 5, 6}, {111, 11, 333, 33, 11, 11}},
                           {0, 2}, kProbe4, true);
     Rows want = {{"1", "11", "2", "11"}, {"1", "11", "5", "11"}, {"1", "11", "6", "11"},
                  {"2", "22", "null", "null"}, {"3", "33", "4", "33"}, {"4", "44", "null", "null"}};
@@ -553,7 +554,7 @@ static void TestFullJoinBuildNull()
 // join_test.cpp:4157-4208 (TestLeftAntiEqualityJoin)
 static void TestLeftAntiJoin()
 {
-    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT_ANTI, OMNI_BUILD_UNKNOWN, 1, kBuild4, {},
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_LEFT_ANTI, OMNI_BUILD_UNKNOWN, {1}, kBuild4, {},
                           {{1, 2, 3, 4, 3, 6}, {11, 22, 33, 44, 22, 44}}, false);
     Rows want = {{"2", "22"}, {"4", "44"}, {"3", "22"}, {"6", "44"}};
     EXPECT_TRUE(SameRowSet(r.first, want));
@@ -562,10 +563,81 @@ static void TestLeftAntiJoin()
 // join_test.cpp:4584-4636 (TestExistenceEqualityJoin): no probe output columns, one BOOLEAN per probe row
 static void TestExistenceJoin()
 {
-    auto r = RunTypedJoin(OMNI_JOIN_TYPE_EXISTENCE, OMNI_BUILD_UNKNOWN, 1, kBuild4, {},
+    auto r = RunTypedJoin(OMNI_JOIN_TYPE_EXISTENCE, OMNI_BUILD_UNKNOWN, {1}, kBuild4, {},
                           {{1, 2, 3, 4, 3, 6}, {11, 22, 33, 44, 22, 44}}, false);
     Rows want = {{"true"}, {"false"}, {"true"}, {"false"}, {"false"}, {"false"}};
     EXPECT_TRUE(r.first == want);
+}
+
+// join_test.cpp:2975-3072 (TestFixedKeysFullEqualityJoinWithBuildNull): two LONG keys (an int128 key), NULL in the
+// second key column of build rows 0 and 2
+static void TestFixedKeysFullJoinBuildNull()
+{
+    std::vector<std::vector<int64_t>> bcols = {{1, 1, 3, 3, 5, 6}, {111, 11, 333, 33, 11, 11}};
+    DataTypes types({LongType(), LongType()});
+    auto *build = new VectorBatch(6);
+    for (int c = 0; c < 2; ++c) build->Append(MakeVector<int64_t>(OMNI_LONG, bcols[c]));
+    build->Get(1)->SetNull(0);
+    build->Get(1)->SetNull(2);
+    int32_t keys[2] = {0, 1}, out[2] = {0, 1};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(OMNI_JOIN_TYPE_FULL, types, keys, 2, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    auto *probe = new VectorBatch(4);
+    probe->Append(MakeVector<int64_t>(OMNI_LONG, {1, 2, 3, 4}));
+    probe->Append(MakeVector<int64_t>(OMNI_LONG, {11, 22, 33, 44}));
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        types, out, 2, keys, 2, out, 2, types, reinterpret_cast<int64_t>(bf), nullptr, false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    Rows want = {{"1", "11", "1", "11"}, {"2", "22", "null", "null"}, {"3", "33", "3", "33"},
+                 {"4", "44", "null", "null"}};
+    EXPECT_TRUE(SameRowSet(Drain(lookup), want));
+    auto *of = GpuLookupOuterJoinOperatorFactory::CreateLookupOuterJoinOperatorFactory(types, out, 2, out, types,
+                                                                                      reinterpret_cast<int64_t>(bf));
+    Operator *outerOp = of->CreateOperator();
+    Rows wantOuter = {{"null", "null", "1", "null"}, {"null", "null", "3", "null"}, {"null", "null", "5", "11"},
+                      {"null", "null", "6", "11"}};
+    EXPECT_TRUE(SameRowSet(Drain(outerOp), wantOuter));
+    Operator::DeleteOperator(outerOp);
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete of;
+    delete lf;
+    delete bf;
+}
+
+// join_test.cpp:4637-4705 (TestMultiKeysFullEqualityJoinInteger): SHORT, INT, LONG keys (112 bits, an int128 key)
+static void TestMultiKeysFullJoinInteger()
+{
+    DataTypes types({ShortType(), IntType(), LongType()});
+    auto *build = new VectorBatch(6);
+    build->Append(MakeVector<int16_t>(OMNI_SHORT, {1, 2, 3, 4, 5, 6}));
+    build->Append(MakeVector<int32_t>(OMNI_INT, {11, 22, 33, 44, 55, 66}));
+    build->Append(MakeVector<int64_t>(OMNI_LONG, {111, 222, 333, 444, 555, 666}));
+    int32_t keys[3] = {0, 1, 2};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(OMNI_JOIN_TYPE_FULL, types, keys, 3, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    auto *probe = new VectorBatch(4);
+    probe->Append(MakeVector<int16_t>(OMNI_SHORT, {1, 3, 5, 7}));
+    probe->Append(MakeVector<int32_t>(OMNI_INT, {11, 33, 55, 77}));
+    probe->Append(MakeVector<int64_t>(OMNI_LONG, {111, 333, 555, 777}));
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        types, keys, 3, keys, 3, keys, 3, types, reinterpret_cast<int64_t>(bf), nullptr, false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    Rows want = {{"1", "11", "111", "1", "11", "111"}, {"3", "33", "333", "3", "33", "333"},
+                 {"5", "55", "555", "5", "55", "555"}, {"7", "77", "777", "null", "null", "null"}};
+    EXPECT_TRUE(SameRowSet(Drain(lookup), want));
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete lf;
+    delete bf;
 }
 
 int main()
@@ -578,6 +650,8 @@ int main()
         {"FullJoinBuildNull", TestFullJoinBuildNull},
         {"LeftAntiJoin", TestLeftAntiJoin},
         {"ExistenceJoin", TestExistenceJoin},
+        {"FixedKeysFullJoinBuildNull", TestFixedKeysFullJoinBuildNull},
+        {"MultiKeysFullJoinInteger", TestMultiKeysFullJoinInteger},
         {"AggVerifyCorrectness", TestAggVerifyCorrectness},
         {"AggTwoStageVerifyCorrectness", TestAggTwoStageVerifyCorrectness},
         {"AggTwoStageMatchesSingleStage", TestAggTwoStageMatchesSingleStage},
