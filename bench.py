@@ -320,8 +320,11 @@ def bench_agg_skewed(args):
     cols = [([k1.c(a), k2.c(a)], [vd.c(a), vf.c(a), vf.c(a), None, vi.c(a)], b - a) for a, b in spans]
     outs = {}
 
+    strategy = {"radix": capi.STRATEGY_RADIX, "auto": capi.STRATEGY_AUTO, "global": capi.STRATEGY_GLOBAL,
+                "lds": capi.STRATEGY_LDS}[args.agg_strategy]
+
     def step():
-        h = capi.HashAgg(capi.make_agg_spec([INT, LONG], aggs))
+        h = capi.HashAgg(capi.make_agg_spec([INT, LONG], aggs, strategy=strategy))
         for kc, ac, m in cols:
             h.add_batch(kc, ac, m)
         g = h.num_groups()
@@ -334,7 +337,24 @@ def bench_agg_skewed(args):
     groups, ms = timed_steps(step, args, capi)
     in_bytes = n * (4 + 8 + 8 + 8 + 4)
     out_bytes = groups * (4 + 8 + 16 + 8 + 8 + 8 + 4)
-    roof = kernel_roofline(capi, "agg_insert_multi", in_bytes, len(spans), args)
+    if args.agg_strategy == "radix":
+        # the aggregation proper is the radix pipeline: its input rows read once and each group written once (as
+        # many bytes as its output row) are the algorithmic bytes; the output pass is timed apart
+        stages = {k: capi.profile_get("agg_radix_" + k) for k in ("hist", "scan", "scatter", "pieces", "final",
+                                                                   "gather")}
+        pipe_ms = sum(v[1] for v in stages.values()) / args.steps
+        state_bytes = out_bytes
+        achieved = (in_bytes + state_bytes) / (pipe_ms / 1e3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": profiled_traffic("radix pipeline (all kernels)", "agg_radix_c3.md"),
+                "kernel": "radix pipeline: agg_radix_hist + scan + scatter (two levels) + agg_radix_final",
+                "kernel_ms": round(pipe_ms, 4), "algorithmic_bytes": in_bytes + state_bytes,
+                "stage_ms": {k: round(v[1] / args.steps, 4) for k, v in stages.items()},
+                "output_ms": round((capi.profile_get("agg_output")[1] + capi.profile_get("agg_compact")[1]) /
+                                   args.steps, 4)}
+    else:
+        roof = kernel_roofline(capi, "agg_insert_multi", in_bytes, len(spans), args)
     cpu = None
     if not args.no_cpu:
         import oracle
@@ -350,7 +370,7 @@ def bench_agg_skewed(args):
     cfg = {"workload": f"skewed group-by: {n} rows, key (int32 U[0,1e4), int64 Zipf(1.1) over 5e7), VectorBatches of "
                        f"{batch}; SUM(decimal64->decimal128), SUM/AVG(double), COUNT(*), MAX(int32); output of "
                        "every group included", "rows": n, "groups": groups,
-           "algorithmic_bytes_per_step": in_bytes + out_bytes}
+           "algorithmic_bytes_per_step": in_bytes + out_bytes, "strategy": args.agg_strategy}
     return {"value": n / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof, "cpu_baseline": cpu,
             "dtype": "mixed: int32/int64 keys, decimal64->decimal128, f64, int32"}
 
@@ -629,6 +649,8 @@ def main():
                    help="plain group-by: all batches in one add_batches call, or one add_batch per batch")
     p.add_argument("--dense", action="store_true", help="dense build keys (array-table variant)")
     p.add_argument("--partitions-out", type=int, default=200, help="partition workload: shuffle partitions")
+    p.add_argument("--agg-strategy", default="radix", choices=["radix", "auto", "global", "lds"],
+                   help="skewed workload: the group-by strategy (radix = partitioned, LDS tables per piece)")
     p.add_argument("--agg-rule", default="normal", choices=["normal", "array"],
                    help="agg workload: AggHashTableRule (array = direct-index map for the single integer key)")
     p.add_argument("--join-type", default="inner", choices=sorted(JOIN_TYPES),

@@ -67,6 +67,14 @@ extern "C" {
 #define OMNI_GPU_STRATEGY_LDS 1      /* workgroup-local LDS table, spill to the global table */
 #define OMNI_GPU_STRATEGY_GLOBAL 2   /* global open-addressing table, 64-bit CAS insert */
 #define OMNI_GPU_STRATEGY_RADIX 3    /* radix-partition rows by key hash, then per-partition tables */
+/* OMNI_GPU_STRATEGY_RADIX (many groups): the batches are recorded, and read when the groups are first asked for
+ * (omni_gpu_agg_num_groups / omni_gpu_agg_get_output / omni_gpu_agg_wait), so their columns must stay valid until
+ * one of those returns.  The rows are partitioned twice by bits of their key hash (2^8, then up to 2^11 children
+ * each, chosen from a HyperLogLog estimate of the groups) and each final piece is aggregated in one workgroup's LDS
+ * table, its groups written once -- no device-wide atomic per row.  It reorders rows, so it is refused for a
+ * checked decimal SUM (an input or output that can leave its limits) and for double MIN / MAX, whose results
+ * depend on the row order.  AUTO takes it for a capacity hint of 2^21 groups or more when it applies.  Batches
+ * added after the groups were read go to the hash table (every existing group is inserted into it first). */
 
 /* Join types and build sides: the values of OmniOperator's JoinType and BuildSide
  * (core/src/plannode/planNode.h:406-414 and :28-32). */

@@ -312,6 +312,10 @@ class HashAgg:
         check(lib().omni_gpu_agg_table_info(self.h, C.byref(a), C.byref(lo), C.byref(n)))
         return bool(a.value), lo.value, n.value
 
+    def wait(self):
+        """omni_gpu_agg_wait: every added batch consumed (the caller's columns may be released afterwards)."""
+        check(lib().omni_gpu_agg_wait(self.h))
+
     def num_groups(self) -> int:
         g = C.c_int64()
         check(lib().omni_gpu_agg_num_groups(self.h, C.byref(g)))

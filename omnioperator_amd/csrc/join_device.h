@@ -835,7 +835,6 @@ __global__ void __launch_bounds__(kBlock) join_outer_emit(OuterArgs g)
 //    partitions x, x + 8, ... in order, so the region it reads stays in its own L2.
 
 constexpr int kPartMax = 4096;
-constexpr int kXcds = 8;
 constexpr int kPartTile = 8192;                     // probe rows per workgroup of the histogram / scatter launches
 constexpr int kHistRows = kPartTile / kBlock;       // rows per thread of the histogram
 constexpr int kScatterBlock = 1024;                 // threads of the scatter

@@ -29,6 +29,7 @@
 namespace omni_gpu {
 
 constexpr int kWaveSize = OG_WAVE_SIZE;
+constexpr int kXcds = 8;  // MI355X: workgroup b of a launch runs on XCD b mod 8, each XCD with its own L2
 
 // ---- errors ---------------------------------------------------------------------------------------------
 int SetError(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
