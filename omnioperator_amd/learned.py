@@ -5,9 +5,10 @@ and run by the library's own HIP kernels).  Output 0 predicts log2 of the group 
 hash table once instead of growing it x4 several times; outputs 1..3 pick the strategy:
   class 0  LDS-local       group-by: LDS partial tables merged into the global table;
   class 1  global-atomic   group-by: global table only; join: direct probe;
-  class 2  radix-partitioned  join: partitioned probe with LDS-staged table regions.
-A group-by chooses between classes 0 and 1 (it has no radix kernel), a join between 1 and 2 (its table is sized
-from the exact build row count at finish, so only the strategy is predicted).  The plain method is the fallback
+  class 2  radix-partitioned  group-by: hash partitions aggregated in LDS (OMNI_GPU_STRATEGY_RADIX); join:
+                              partitioned probe with LDS-staged table regions.
+A group-by chooses among the three classes, a join between 1 and 2 (its table is sized from the exact build row
+count at finish, so only the strategy is predicted).  The plain method is the fallback
 and the judge: a capacity that is too small only costs a grow-and-rehash, and every strategy gives the same
 result rows, so results never depend on the prediction.
 
@@ -29,7 +30,7 @@ from . import capi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 MODEL_PATH = os.path.join(HERE, "models", "advisor_mlp.npz")
-STRATEGIES = (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL)   # group-by kernels of classes 0 and 1
+STRATEGIES = (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL, capi.STRATEGY_RADIX)   # group-by kernels of classes 0-2
 KIND_TYPES = {0: [capi.LONG], 1: [capi.INT], 2: [capi.INT, capi.LONG]}
 T_SCALE = 32.0          # the regression target is log2(groups + 1) / T_SCALE
 CAPACITY_MARGIN = 1.25  # capacity hint = predicted groups x margin
@@ -121,7 +122,7 @@ def join_features(probe_rows: int, build_rows: int, key_bytes: int) -> np.ndarra
 
 AGGS = [(capi.SUM, capi.LONG, capi.LONG), (capi.COUNT_ALL, capi.LONG, capi.LONG)]
 CLASS_LDS, CLASS_GLOBAL, CLASS_RADIX = 0, 1, 2
-AGG_CLASSES = (CLASS_LDS, CLASS_GLOBAL)
+AGG_CLASSES = (CLASS_LDS, CLASS_GLOBAL, CLASS_RADIX)
 JOIN_CLASSES = (CLASS_GLOBAL, CLASS_RADIX)
 BATCH_ROWS = 1 << 22
 
@@ -279,8 +280,8 @@ class Advisor:
 
     @staticmethod
     def _classes(y: np.ndarray, is_join: np.ndarray) -> np.ndarray:
-        """Strategy class per row: argmax over the classes the operator has (group-by 0/1, join 1/2)."""
-        agg = np.argmax(y[:, 1:3], axis=1)
+        """Strategy class per row: argmax over the classes the operator has (group-by 0/1/2, join 1/2)."""
+        agg = np.argmax(y[:, 1:4], axis=1)
         join = 1 + np.argmax(y[:, 2:4], axis=1)
         return np.where(is_join, join, agg)
 
@@ -321,8 +322,7 @@ CHECK_ROWS = 20_000_000  # result sets up to this size are compared by checksum,
 def evaluate(tasks: Sequence[Task], advisor: "Advisor", log=None):
     """Runs every task with each fixed strategy, the library's automatic rule and the advisor (its sketch and
     inference timed with it).  Fixed strategies per operator: lds = LDS-local group-by / direct join, global =
-    global-atomic group-by / direct join, radix = global-atomic group-by (no radix group-by kernel) /
-    partitioned join.  Group-by results are written to device outputs inside the timed region.  Returns
+    global-atomic group-by / direct join, radix = radix-partitioned group-by / partitioned join.  Group-by results are written to device outputs inside the timed region.  Returns
     {config: seconds}, input rows, and per-task records; raises if any configuration's result rows differ."""
     secs = {k: 0.0 for k in CONFIGS}
     rows_total = 0
@@ -366,7 +366,7 @@ def evaluate(tasks: Sequence[Task], advisor: "Advisor", log=None):
             t_plan = time.perf_counter() - t
             results = {}
             for k, s, c in (("lds", capi.STRATEGY_LDS, 0), ("global", capi.STRATEGY_GLOBAL, 0),
-                            ("radix", capi.STRATEGY_GLOBAL, 0), ("auto", capi.STRATEGY_AUTO, 0),
+                            ("radix", capi.STRATEGY_RADIX, 0), ("auto", capi.STRATEGY_AUTO, 0),
                             ("advisor", strat, cap)):
                 g, dt = run_agg_full(data, s, c, bufs)
                 if k == "advisor":

@@ -4,7 +4,7 @@
 
 Each run generates a seeded key distribution on the device (rows log-uniform in [1e4, 1.6e7], distinct keys
 log-uniform in [10, 1e8] capped at the rows, Zipf s uniform in [0, 1.5], int32 / int64 / (int32, int64) keys),
-sketches it, and runs either SUM + COUNT(*) GROUP BY key (LDS-local and global-atomic kernels) or, for half of
+sketches it, and runs either SUM + COUNT(*) GROUP BY key (LDS-local, global-atomic and radix-partitioned) or, for half of
 the runs, an inner join of unique build keys with the rows as probe side (direct and partitioned probes), to
 record the true group count (build rows for a join) and the fastest strategy.  The MLP is then trained by the library's own HIP kernels (Adam, mini-batches of 128); 10% of the
 runs are held out to report the prediction error.

@@ -152,43 +152,49 @@ def test_advisor_training_pipeline_reduces_loss():
 
 
 def test_advisor_gains_over_the_plain_method():
-    """The committed advisor (trained on the GPU from 4096 runs by omnioperator_amd.train_advisor) against the plain
-    fixed-strategy method on 32 held-out tasks (seeds never used in training): the global-atomic kernel without a
-    capacity (grow-and-rehash) versus the advisor's capacity + strategy, sketch and inference time included.  Each
-    configuration runs once untimed first.  Every task's groups are identical both ways; the advisor's total time
-    is lower.  The adaptive automatic rule is timed alongside for reference."""
+    """The committed advisor (trained on the GPU by omnioperator_amd.train_advisor) on 32 held-out tasks (seeds
+    never used in training) against each fixed strategy without a capacity (grow-and-rehash): the global-atomic
+    kernel -- the plain method -- and the best of the fixed strategies (LDS-local, global-atomic, radix), sketch
+    and inference time charged to the advisor.  Each configuration runs once untimed first.  Every task's groups
+    are identical in every configuration; the advisor beats the plain method, and stays within 40% of the best
+    fixed strategy over the set: on tasks this small (0.4 ms each with the LDS kernel) the sketch and inference
+    (about 0.1 ms a task) are a large share, which the task set at full scale amortises."""
     adv = learned.Advisor()
     test_tasks = learned.sample_tasks(32, seed=777, min_rows=2e5, max_rows=8e6, max_distinct=8e6)
     assert all(tk.op == "agg" for tk in test_tasks)
-    plain_t = adv_t = auto_t = 0.0
+    fixed = {"lds": capi.STRATEGY_LDS, "global": capi.STRATEGY_GLOBAL, "radix": capi.STRATEGY_RADIX}
+    tot = {k: 0.0 for k in list(fixed) + ["advisor"]}
     errs = []
     for task in test_tasks:
         data = learned.TaskData(task)
         cap, strategy = adv.plan(data)
-        learned.run_agg(data, capi.STRATEGY_GLOBAL, 0)
-        learned.run_agg(data, strategy, cap)
-        learned.run_agg(data, capi.STRATEGY_AUTO, 0)
+        for s in list(fixed.values()) + [None]:
+            learned.run_agg(data, strategy if s is None else s, cap if s is None else 0)
         g0, dt0, out0 = learned.run_agg(data, capi.STRATEGY_GLOBAL, 0, output=True)
+        tot["global"] += dt0
+        order0 = np.lexsort([o[0] for o in out0[:len(data.types)]][::-1])
+        for k in ("lds", "radix"):
+            g, dt = learned.run_agg(data, fixed[k], 0)
+            assert g == g0
+            tot[k] += dt
         t0 = time.perf_counter()
         cap, strategy = adv.plan(data)  # sketch + inference, charged to the advisor
         t_plan = time.perf_counter() - t0
         g1, dt1, out1 = learned.run_agg(data, strategy, cap, output=True)
-        dt1 += t_plan
-        _, dta = learned.run_agg(data, capi.STRATEGY_AUTO, 0)
+        tot["advisor"] += dt1 + t_plan
         assert g0 == g1
-        order0 = np.lexsort([o[0] for o in out0[:len(data.types)]][::-1])
         order1 = np.lexsort([o[0] for o in out1[:len(data.types)]][::-1])
         for (v0, n0), (v1, n1) in zip(out0, out1):
             assert np.array_equal(v0[order0], v1[order1]) and np.array_equal(n0[order0], n1[order1])
-        plain_t += dt0
-        adv_t += dt1
-        auto_t += dta
         errs.append(abs(math.log2((cap / learned.CAPACITY_MARGIN) + 1) - math.log2(g0 + 1)))
     adv.close()
-    print(f"plain fixed global {plain_t * 1e3:.1f} ms, advisor {adv_t * 1e3:.1f} ms ({plain_t / adv_t:.2f}x), "
-          f"automatic rule {auto_t * 1e3:.1f} ms, median |log2 error| {np.median(errs):.2f}")
+    best = min(tot[k] for k in fixed)
+    print(", ".join(f"{k} {v * 1e3:.1f} ms" for k, v in tot.items()) +
+          f"; advisor {tot['global'] / tot['advisor']:.2f}x the global-atomic kernel, "
+          f"{best / tot['advisor']:.2f}x the best fixed strategy; median |log2 error| {np.median(errs):.2f}")
     assert np.median(errs) < 1.0
-    assert adv_t < plain_t
+    assert tot["advisor"] < tot["global"]
+    assert tot["advisor"] <= 1.4 * best
 
 
 def test_capacity_below_the_groups_falls_back_to_growth():
@@ -214,4 +220,4 @@ def test_advisor_evaluation_small_set():
         if r["op"] == "join":
             assert r["choice"] in learned.JOIN_CLASSES
         else:
-            assert r["choice"] in (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL)
+            assert r["choice"] in (capi.STRATEGY_LDS, capi.STRATEGY_GLOBAL, capi.STRATEGY_RADIX)
