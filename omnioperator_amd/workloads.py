@@ -9,7 +9,7 @@ Every generator is deterministic for a given seed, so the GPU path and the CPU o
 from __future__ import annotations
 
 from dataclasses import dataclass
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 
@@ -95,10 +95,11 @@ class AggSkewed:
     v_dec: np.ndarray    # int64 (decimal64, scale 2)
     v_dbl: np.ndarray    # float64
     v_i32: np.ndarray    # int32
+    ranks: Optional[np.ndarray] = None  # int64 Zipf rank of k2 (k2 = splitmix64(rank)), when asked for
 
 
 def agg_skewed(rows: int = 200_000_000, k1_distinct: int = 10_000, k2_ranks: int = 50_000_000, s: float = 1.1,
-               seed: int = 0xC3) -> AggSkewed:
+               seed: int = 0xC3, keep_ranks: bool = False) -> AggSkewed:
     rng = np.random.Generator(np.random.PCG64(seed))
     k1 = rng.integers(0, k1_distinct, rows, dtype=np.int64).astype(np.int32)
     ranks = zipf_ranks(rng, rows, s, k2_ranks)
@@ -106,7 +107,7 @@ def agg_skewed(rows: int = 200_000_000, k1_distinct: int = 10_000, k2_ranks: int
     v_dec = rng.integers(0, 1_000_000_001, rows, dtype=np.int64)
     v_dbl = rng.normal(100.0, 50.0, rows)
     v_i32 = rng.integers(-2**31, 2**31, rows, dtype=np.int64).astype(np.int32)
-    return AggSkewed(k1, k2, v_dec, v_dbl, v_i32)
+    return AggSkewed(k1, k2, v_dec, v_dbl, v_i32, ranks if keep_ranks else None)
 
 
 @dataclass
