@@ -394,11 +394,12 @@ def bench_batched(args):
     spec = capi.make_agg_spec([capi.LONG], aggs)
     s = capi.AggBatches(spec, 65536)
     F = capi.AggBatches
-    forms = {"learned": F.FLAG_GRAPH | F.FLAG_ADVISOR, "graph": F.FLAG_GRAPH, "two_launches": 0,
+    forms = {"learned": F.FLAG_GRAPH | F.FLAG_ADVISOR | F.FLAG_FUSED, "fused": F.FLAG_GRAPH | F.FLAG_FUSED,
+             "learned_phases": F.FLAG_GRAPH | F.FLAG_ADVISOR, "graph": F.FLAG_GRAPH, "two_launches": 0,
              "per_batch": F.FLAG_PER_BATCH}
     flags = forms[args.batched_form]
     net = None
-    if args.batched_form == "learned":
+    if flags & F.FLAG_ADVISOR:
         net = batch_advisor.network()
         s.set_advisor(net)
     all_part = np.full(nb, 2, np.int32)
@@ -420,8 +421,8 @@ def bench_batched(args):
         tuned = autotune.tune_stream(s, lambda: (step(), capi.synchronize()), key, use_cache=False)
 
     groups, ms = timed_steps(step, args, capi)
-    prof = {"learned": "agg_batches_graph", "graph": "agg_batches_graph", "two_launches": "agg_batches",
-            "per_batch": "agg_batches_per_batch"}[args.batched_form]
+    prof = "agg_batches_graph" if flags & F.FLAG_GRAPH else \
+        ("agg_batches_per_batch" if flags & F.FLAG_PER_BATCH else "agg_batches")
     n_launch, gpu_ms = capi.profile_get(prof)
     run_ms = gpu_ms / max(1, n_launch)
     ph = s.phases()
@@ -637,7 +638,7 @@ def main():
                         "learned: the 512-task advisor set; partition: shuffle partition ids")
     p.add_argument("--skewed-rows", type=int, default=200_000_000)
     p.add_argument("--cpu-skewed-rows", type=int, default=4_000_000)
-    p.add_argument("--batched-form", default="learned", choices=["learned", "graph", "two_launches", "per_batch"])
+    p.add_argument("--batched-form", default="learned", choices=["learned", "fused", "learned_phases", "graph", "two_launches", "per_batch"])
     p.add_argument("--no-compare", action="store_true",
                    help="batched: skip the plain and fixed-strategy runs beside the learned form")
     p.add_argument("--tiles", default=None, help="batched: fixed block,lds_bytes,max_probe,part_lds_bytes "

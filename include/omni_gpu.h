@@ -117,6 +117,8 @@ int omni_gpu_init(int device);
 const char *omni_gpu_last_error(void);
 int omni_gpu_malloc(void **ptr, size_t bytes);
 int omni_gpu_free(void *ptr);
+/* Copies and memset on device memory; each returns when the device has finished it (so that work queued on any
+ * handle afterwards sees the result). */
 int omni_gpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int omni_gpu_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int omni_gpu_memcpy_d2d(void *dst, const void *src, size_t bytes);
@@ -213,7 +215,9 @@ int omni_gpu_agg_batches_create(const omni_gpu_agg_spec *spec, int64_t max_batch
 int omni_gpu_agg_batches_tune(void *handle, int32_t block_threads, int32_t lds_bytes, int32_t max_probe);
 /* Phase-2 tuning: LDS bytes of a partition table ([8 KiB, 64 KiB], default 64 KiB) and rows per partition
  * (0, the default: half the table's slots, so a partition overflows only far above its mean share; more
- * rows mean fewer passes over each batch and more batches left to phase 3). */
+ * rows mean fewer passes over each batch and more batches left to phase 3).  Under flag bit 4 the table is the
+ * workgroup's, and part_rows, when not 0, is the number of distinct keys a partition is sized for (default: 45%
+ * of the table's slots, and a batch whose keys fit that many goes direct). */
 int omni_gpu_agg_batches_tune_partitions(void *handle, int32_t part_lds_bytes, int32_t part_rows);
 int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_bytes);
 /* Runs the stream.  keys[0] / aggs[i] are the concatenated device columns of all batches; batch b owns rows
@@ -224,7 +228,10 @@ int omni_gpu_agg_batches_info(void *handle, int32_t *lds_slots, int64_t *pool_by
  * shape stays the same; bit 2 (without batch_modes, not with bit 0) = a sketch launch estimates each batch's
  * distinct keys from its first 1,024 rows and starts the batches estimated above the LDS table at phase 2;
  * bit 3 (same conditions) = the advisor network picks each batch's first phase
- * (omni_gpu_agg_batches_set_advisor).  Batch b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
+ * (omni_gpu_agg_batches_set_advisor); bit 4 (not with bit 0) = one workgroup per batch does the batch's whole
+ * group-by in LDS (a batch that fits one table directly, a larger one by hash partitions it writes in partition
+ * order and aggregates one after another; no device-wide atomics), in place of phases 1 and 2; with bit 3 the
+ * advisor's predicted group count picks direct or partitioned and the partition count.  Batch b's groups go to output rows [batch_offsets[b], batch_offsets[b] + groups_per_batch[b]) of outs (key,
  * then aggregates); outs hold batch_offsets[num_batches] rows and NULL bitmaps of 4 * ceil(rows / 32) bytes.
  * Synchronizes. */
 int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const omni_gpu_column *aggs,

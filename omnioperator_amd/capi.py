@@ -371,7 +371,7 @@ class HashAgg:
 class AggBatches:
     """Batched stream of independent group-bys (omni_gpu_agg_batches_*)."""
 
-    FLAG_PER_BATCH, FLAG_GRAPH, FLAG_SKETCH, FLAG_ADVISOR = 1, 2, 4, 8
+    FLAG_PER_BATCH, FLAG_GRAPH, FLAG_SKETCH, FLAG_ADVISOR, FLAG_FUSED = 1, 2, 4, 8, 16
 
     def __init__(self, spec: AggSpec, max_batch_rows: int):
         init()

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "agg_layout.h"
+#include "batch_fused.h"
 #include "batches_device.h"
 
 namespace omni_gpu {
@@ -52,6 +53,39 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const B
     extern __shared__ unsigned long long lds_words[];
     __shared__ unsigned int sh[4];
     BatchPart<BLOCK>(*cfg, static_cast<int64_t>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
+}
+
+// One workgroup per batch (batch order[blockIdx.x]) does the batch's whole group-by (BatchFused), with the
+// generic table (NA = 0) or the lite table of NA aggregates.
+template <int BLOCK, int NA>
+__global__ void __launch_bounds__(BLOCK) batch_fused(const BatchConfig *__restrict__ cfg, const int32_t *__restrict__ order)
+{
+    extern __shared__ unsigned long long lds_words[];
+    __shared__ unsigned int cnt[kFusedParts + 1];
+    __shared__ unsigned int aux[kFusedParts + 1];
+    __shared__ unsigned int ccnt[kFusedParts + 1];
+    __shared__ unsigned int wsum[BLOCK / kWaveSize];
+    __shared__ unsigned int sh[4];
+    const BatchConfig &c = *cfg;
+    uint8_t *lds = reinterpret_cast<uint8_t *>(lds_words);
+    if constexpr (NA == 0) {
+        BatchFused<BLOCK>(c, GenericTable{c, lds}, order[blockIdx.x], lds, c.L2.S, c.L2.bytes, cnt, aux, ccnt, wsum, sh);
+    } else {
+        BatchFused<BLOCK>(c, MakeLiteTable<NA>(c, lds), order[blockIdx.x], lds, c.lite_smax, c.lite_bytes, cnt, aux,
+                          ccnt, wsum, sh);
+    }
+}
+
+template <int BLOCK>
+void LaunchFused(const BatchConfig &hc, const BatchConfig *dc, const int32_t *order, int nb, hipStream_t s)
+{
+    switch (hc.lite_na) {
+        case 1: batch_fused<BLOCK, 1><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        case 2: batch_fused<BLOCK, 2><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        case 3: batch_fused<BLOCK, 3><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        case 4: batch_fused<BLOCK, 4><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        default: batch_fused<BLOCK, 0><<<nb, BLOCK, hc.L2.bytes, s>>>(dc, order); break;
+    }
 }
 
 // Phase 3 launches are grid-stride over their tiles / batches (grid capped at kPhase3Grid) and return at once
@@ -107,6 +141,8 @@ struct BatchesHandle {
     DevBuf items, parts, parts_used;  // phase-2 work items, partitions launched / used per batch
     DevBuf order;                     // batches by decreasing rows (phase 1 and 2a workgroup order)
     DevBuf sketch_modes;              // per-batch first phase from the sketch launch or the advisor
+    DevBuf perm_keys, perm_vals;      // partition-ordered rows of the one-workgroup-per-batch form
+    bool fused = false;               // this run takes the one-workgroup-per-batch form (flag bit 4)
     void *advisor = nullptr;          // MLP handle (omni_gpu_agg_batches_set_advisor)
     bool use_advisor = false;         // this run's first phases come from the advisor
     DevBuf adv_x, adv_y;              // the advisor's per-batch features and outputs
@@ -188,7 +224,20 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
         OG_TRY(CheckLaunch("batch_clear"));
     }
     int lds_bytes = h.cfg.L.bytes;
-    if (per_batch) {
+    if (h.fused && !per_batch) {
+        // the advisor's predictions (no per-batch plan: the fused kernel reads the predicted group counts)
+        if (h.use_advisor) {
+            OG_TRY(LaunchBatchAdvisor(s, h.advisor, h.cfg.kd.values[0], h.cfg.kd.nulls[0], h.cfg.kd.width[0],
+                                      h.offsets.as<int64_t>(), nb, h.adv_x.as<float>(), h.adv_y.as<float>(), nullptr,
+                                      0.f));
+        }
+        const int32_t *order = h.order.as<int32_t>();
+        if (h.block == 512) LaunchFused<512>(h.cfg, dc, order, nb, s);
+        else LaunchFused<256>(h.cfg, dc, order, nb, s);
+        OG_TRY(CheckLaunch("batch_fused"));
+        batch_scan<<<1, kScanBlock, 0, s>>>(dc);
+        OG_TRY(CheckLaunch("batch_scan"));
+    } else if (per_batch) {
         for (int b = 0; b < nb; ++b) {
             if (h.block == 512) batch_lds<512><<<1, 512, lds_bytes, s>>>(dc, b, nullptr);
             else batch_lds<256><<<1, 256, lds_bytes, s>>>(dc, b, nullptr);
@@ -206,10 +255,12 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
         if (h.block == 512) batch_lds<512><<<nb, 512, lds_bytes, s>>>(dc, 0, order);
         else batch_lds<256><<<nb, 256, lds_bytes, s>>>(dc, 0, order);
     }
-    OG_TRY(CheckLaunch("batch_lds"));
-    batch_scan<<<1, kScanBlock, 0, s>>>(dc);
-    OG_TRY(CheckLaunch("batch_scan"));
-    if (!per_batch && h.nitems > 0) {
+    if (!h.fused || per_batch) {
+        OG_TRY(CheckLaunch("batch_lds"));
+        batch_scan<<<1, kScanBlock, 0, s>>>(dc);
+        OG_TRY(CheckLaunch("batch_scan"));
+    }
+    if (!per_batch && !h.fused && h.nitems > 0) {
         const unsigned int g = static_cast<unsigned int>(h.nitems);
         if (h.block == 512) {
             batch_split<512><<<nb, 512, 0, s>>>(dc, h.order.as<int32_t>());
@@ -435,18 +486,71 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.redo = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>() + 4);
     c.nitems = h.nitems;
     c.hbm_flag = (flags & 1) ? 1 : 2;  // the per-batch form has no phase 2
+    // flag bit 4: one workgroup per batch does the whole group-by (BatchFused)
+    h.fused = !(flags & 1) && (flags & 16);
+    c.carry = 0;
+    c.carry_agg = -1;
+    c.part_fit = h.part_rows;
+    c.lite_na = 0;
+    c.perm_keys = nullptr;
+    c.perm_vals = nullptr;
+    c.perm_rows = nullptr;
+    if (h.fused) {
+        // the rows carry their value word when every aggregate is COUNT(*) or reads one column of at most 8 bytes
+        // without NULLs (and the key has no NULLs); otherwise they carry their row number
+        bool carry = c.kd.nulls[0] == nullptr;
+        for (int i = 0; i < c.args.n && carry; ++i) {
+            const AggDev &d = c.args.a[i];
+            if (d.kind == K_COUNT_ALL) continue;
+            const bool kindOk = d.kind == K_COUNT_COL || d.kind == K_SUM_I64 || d.kind == K_SUM_F64 || d.kind == K_AVG ||
+                                d.kind == K_SUM_DEC || d.kind == K_MIN_I64 || d.kind == K_MAX_I64 ||
+                                d.kind == K_MIN_I32 || d.kind == K_MAX_I32;
+            carry = kindOk && d.nulls == nullptr && d.in_width <= 8;
+            if (carry && c.carry_agg >= 0) {
+                const AggDev &f = c.args.a[c.carry_agg];
+                carry = f.values == d.values && f.in_width == d.in_width;
+            }
+            if (carry && c.carry_agg < 0) c.carry_agg = i;
+        }
+        if (!carry) c.carry_agg = -1;
+        c.carry = carry ? 1 : 0;
+        // the lite table: at most kLiteAggs aggregates, each COUNT or an int64 SUM / MIN / MAX with an 8-byte output
+        bool lite = carry && c.args.n >= 1 && c.args.n <= kLiteAggs;
+        for (int i = 0; i < c.args.n && lite; ++i) {
+            const int k = c.args.a[i].kind;
+            lite = (k == K_COUNT_ALL || k == K_COUNT_COL || k == K_SUM_I64 || k == K_MIN_I64 || k == K_MAX_I64) &&
+                   c.out.o[1 + i].width == 8;
+            if (lite) {
+                c.lite_op[i] = k == K_SUM_I64 ? kLiteSum : k == K_MIN_I64 ? kLiteMin : k == K_MAX_I64 ? kLiteMax
+                                                                                                   : kLiteCount;
+            }
+        }
+        c.lite_na = lite ? c.args.n : 0;
+        if (lite) {
+            const int slots = (h.part_budget / (8 * (1 + c.args.n))) & ~63;  // keys + one word per aggregate
+            c.lite_smax = slots - 2;
+            c.lite_bytes = 8 * (1 + c.args.n) * slots;
+        }
+        const size_t n = static_cast<size_t>(std::max<int64_t>(1, total));
+        OG_TRY(h.perm_keys.Reserve(sizeof(unsigned long long) * n, s));
+        OG_TRY(h.perm_vals.Reserve((carry ? sizeof(unsigned long long) : sizeof(unsigned int)) * n, s));
+        c.perm_keys = h.perm_keys.as<unsigned long long>();
+        if (carry) c.perm_vals = h.perm_vals.as<unsigned long long>();
+        else c.perm_rows = h.perm_vals.as<unsigned int>();
+    }
     // flag bit 3: the advisor network picks each batch's first phase from a per-batch sketch; flag bit 2: a
     // linear-counting estimate does (neither with caller modes or the per-batch form; bit 3 wins)
     h.use_advisor = !(flags & 1) && (flags & 8) && batch_modes == nullptr;
     OG_CHECK(!h.use_advisor || h.advisor, OMNI_GPU_ERR_STATE, "flag bit 3 needs omni_gpu_agg_batches_set_advisor");
     const bool sketch = !(flags & 1) && ((flags & 4) || h.use_advisor) && batch_modes == nullptr;
-    if (sketch) OG_TRY(h.sketch_modes.Reserve(sizeof(int32_t) * num_batches, s));
+    if (sketch && !h.fused) OG_TRY(h.sketch_modes.Reserve(sizeof(int32_t) * num_batches, s));
     if (h.use_advisor) {
         OG_TRY(h.adv_x.Reserve(sizeof(float) * 16 * num_batches, s));
         OG_TRY(h.adv_y.Reserve(sizeof(float) * 4 * num_batches, s));
     }
-    c.sketch_modes = sketch ? h.sketch_modes.as<int32_t>() : nullptr;
+    c.sketch_modes = sketch && !h.fused ? h.sketch_modes.as<int32_t>() : nullptr;
     h.cfg.sketch_modes = c.sketch_modes;
+    c.pred = h.fused && h.use_advisor ? h.adv_y.as<float>() : nullptr;
     h.last_per_batch = (flags & 1) != 0;
     c.error_flag = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>());
     OG_TRY(CopyToDevice(h.dcfg.p, &c, sizeof(BatchConfig), s));
@@ -462,7 +566,8 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
                                      h.use_advisor ? reinterpret_cast<uint64_t>(h.advisor) : 0ULL,
                                      reinterpret_cast<uint64_t>(h.adv_x.p), reinterpret_cast<uint64_t>(h.adv_y.p),
                                      reinterpret_cast<uint64_t>(h.sketch_modes.p), reinterpret_cast<uint64_t>(keys[0].values),
-                                     reinterpret_cast<uint64_t>(keys[0].nulls)};
+                                     reinterpret_cast<uint64_t>(keys[0].nulls), h.fused ? 1ULL : 0ULL,
+                                     static_cast<uint64_t>(c.L2.bytes), static_cast<uint64_t>(c.lite_na)};
         for (int k = 0; k < 1 + c.args.n; ++k) key.push_back(reinterpret_cast<uint64_t>(outs[k].nulls));
         if (!h.graph_exec || key != h.graph_key) {
             DropGraph(h);

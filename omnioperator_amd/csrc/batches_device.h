@@ -50,6 +50,20 @@ struct BatchConfig {
     unsigned long long *region_off;  // per batch byte offset into the region pool (phase 2)
     uint8_t *region_pool;
     unsigned int *error_flag;
+    // one-workgroup-per-batch form (BatchFused): the rows of each partitioned batch in partition order, at the
+    // batch's own row positions -- their keys and either the value word every aggregate reads (carry = 1) or
+    // the batch-relative row number (carry = 0)
+    unsigned long long *perm_keys;
+    unsigned long long *perm_vals;
+    unsigned int *perm_rows;
+    int32_t carry;
+    int32_t carry_agg;         // the aggregate whose column is carried (-1: every aggregate is COUNT(*))
+    int32_t part_fit;          // distinct keys aimed at per partition (0: kFusedLoad of the table's slots)
+    int32_t lite_na;           // aggregates of the lite table (0: the generic table)
+    int32_t lite_smax;         // probe slots of the lite table
+    int32_t lite_bytes;        // its LDS bytes
+    int32_t lite_op[4];        // per aggregate: COUNT, SUM, MIN or MAX of int64 (LiteOp)
+    const float *pred;         // advisor outputs (4 per batch; [0] = log2(groups + 1) / 32), or nullptr
 };
 
 // Probe slots of the phase-2 table of a batch with `rows` rows: next power of two >= 2 * rows (at least 64).

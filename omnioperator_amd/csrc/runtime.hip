@@ -458,27 +458,42 @@ int omni_gpu_free(void *ptr)
     return OMNI_GPU_OK;
 }
 
+// The copies and the memset run on the null stream, which does not order the library's own streams (they are
+// non-blocking), and hipMemset / pageable hipMemcpy may return before the device has finished.  Each call
+// therefore waits for the null stream, so that work a handle queues afterwards never races with it.
 int omni_gpu_memcpy_h2d(void *dst, const void *src, size_t bytes)
 {
-    if (bytes) OG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    if (bytes) {
+        OG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        OG_HIP(hipStreamSynchronize(nullptr));
+    }
     return OMNI_GPU_OK;
 }
 
 int omni_gpu_memcpy_d2h(void *dst, const void *src, size_t bytes)
 {
-    if (bytes) OG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    if (bytes) {
+        OG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        OG_HIP(hipStreamSynchronize(nullptr));
+    }
     return OMNI_GPU_OK;
 }
 
 int omni_gpu_memcpy_d2d(void *dst, const void *src, size_t bytes)
 {
-    if (bytes) OG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+    if (bytes) {
+        OG_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+        OG_HIP(hipStreamSynchronize(nullptr));
+    }
     return OMNI_GPU_OK;
 }
 
 int omni_gpu_memset(void *dst, int value, size_t bytes)
 {
-    if (bytes) OG_HIP(hipMemset(dst, value, bytes));
+    if (bytes) {
+        OG_HIP(hipMemset(dst, value, bytes));
+        OG_HIP(hipStreamSynchronize(nullptr));
+    }
     return OMNI_GPU_OK;
 }
 

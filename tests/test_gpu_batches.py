@@ -86,11 +86,15 @@ def run_batches(offsets, key, cols, modes=None, flags=0, tune=None, repeat=1, pa
 
 
 @pytest.mark.parametrize("form", ["two_launches", "sketch", "per_batch", "graph", "hbm_modes", "small_lds",
-                                  "block512", "small_partitions", "partition_overflow"])
+                                  "block512", "small_partitions", "partition_overflow", "fused", "fused_graph",
+                                  "fused_block512", "fused_small", "fused_forced", "fused_overflow"])
 def test_batches_match_oracle(form):
     """Every form of the batched group-by against the oracle, batch by batch.  small_partitions: 8 KiB phase-2
     tables (many partitions per batch); partition_overflow: partitions sized 16x above their table, so phase-2
-    partitions overflow, the batch's partial phase-2 output is replaced by phase 3 (NULL bits included)."""
+    partitions overflow, the batch's partial phase-2 output is replaced by phase 3 (NULL bits included).  fused*:
+    one workgroup per batch (flag bit 4; rows carry their row numbers here, as the aggregates read several columns
+    with NULLs): fused_small with 8 KiB tables (many partitions per batch), fused_forced with every batch sent to
+    the partitioned path, fused_overflow with partitions sized far above their table (phase 3 replaces them)."""
     offsets, key, cols = make_stream(1234)
     nb = len(offsets) - 1
     kwargs = {}
@@ -113,6 +117,19 @@ def test_batches_match_oracle(form):
     elif form == "partition_overflow":
         kwargs["tune"] = {"block": 256, "lds_bytes": 8192, "max_probe": 4}
         kwargs["parts"] = {"part_lds_bytes": 8192, "part_rows": 4096}
+    elif form.startswith("fused"):
+        kwargs["flags"] = capi.AggBatches.FLAG_FUSED
+        if form == "fused_graph":
+            kwargs["flags"] |= capi.AggBatches.FLAG_GRAPH
+            kwargs["repeat"] = 3
+        elif form == "fused_block512":
+            kwargs["tune"] = {"block": 512, "lds_bytes": 32768, "max_probe": 32}
+        elif form == "fused_small":
+            kwargs["parts"] = {"part_lds_bytes": 8192, "part_rows": 0}
+        elif form == "fused_forced":
+            kwargs["modes"] = np.full(nb, 2, np.int32)
+        elif form == "fused_overflow":
+            kwargs["parts"] = {"part_lds_bytes": 8192, "part_rows": 4096}
     results, stats = run_batches(offsets, key, cols, **kwargs)
     want = oracle_batches(offsets, key, cols, range(nb))
     for groups, outs in results:
@@ -136,6 +153,12 @@ def test_batches_match_oracle(form):
         assert stats["hbm_batches"] >= (nb + 1) // 2
     if form == "block512":
         assert stats["lds_batches"] > 0
+    if form in ("fused", "fused_graph", "fused_block512", "fused_small"):
+        assert p1 > 0 and p2 > 0 and p3 == 0  # direct and partitioned batches, no overflow
+    if form == "fused_forced":
+        assert p2 > 0 and p3 == 0
+    if form == "fused_overflow":
+        assert p3 > 0
 
 
 @pytest.mark.parametrize("graph", [False, True])
@@ -170,6 +193,57 @@ def test_advisor_form_matches_oracle(graph):
     assert np.all(ph[skip] >= 2)
     s.close()
     net.close()
+
+
+@pytest.mark.parametrize("form", ["fused", "fused_graph", "fused_advisor", "fused_advisor_graph", "fused_small"])
+def test_fused_carried_values_match_oracle(form):
+    """One workgroup per batch with the rows carrying their value word (SUM(int64) and COUNT(*) over NULL-free
+    columns, the batched stream's aggregates) against the oracle, batch by batch; with the stream advisor's
+    predicted group counts choosing direct or partitioned and the partition count (fused_advisor*)."""
+    from omnioperator_amd import batch_advisor
+    rng = np.random.default_rng(99)
+    nb = 400
+    rows = np.clip(np.round(np.exp(np.log(1024) + 1.4 * rng.standard_normal(nb))), 0, 20000).astype(np.int64)
+    rows[:3] = [20000, 1, 0]
+    offsets = np.concatenate([[0], np.cumsum(rows)])
+    n = int(offsets[-1])
+    keys = np.empty(n, np.int64)
+    for b in range(nb):
+        d = max(1, int(np.exp(rng.uniform(0, np.log(30000)))))
+        keys[offsets[b]:offsets[b + 1]] = rng.integers(0, d, rows[b]) * 2654435761 + b
+    keys[rng.random(n) < 0.0005] = -2**63
+    vals = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+    aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
+    s = capi.AggBatches(capi.make_agg_spec([LONG], aggs), int(rows.max()))
+    net = None
+    flags = capi.AggBatches.FLAG_FUSED
+    if "graph" in form:
+        flags |= capi.AggBatches.FLAG_GRAPH
+    if "advisor" in form:
+        net = batch_advisor.network()
+        s.set_advisor(net)
+        flags |= capi.AggBatches.FLAG_ADVISOR
+    if form == "fused_small":
+        s.tune_partitions(part_lds_bytes=8192, part_rows=0)
+    kd, vd = capi.DeviceColumn(LONG, keys), capi.DeviceColumn(LONG, vals)
+    spec_o = oracle.make_agg_spec([LONG], aggs)
+    want = []
+    for b in range(nb):
+        a0, a1 = int(offsets[b]), int(offsets[b + 1])
+        ref = oracle.hash_agg(spec_o, [Col(LONG, keys[a0:a1])], [Col(LONG, vals[a0:a1]), None], max(1, a1 - a0))
+        want.append(rows_of([(c.type_id, c.values, c.nulls) for c in ref]))
+    for _ in range(3 if "graph" in form else 1):
+        groups = s.run(kd.c(), [vd.c(), None], offsets, None, flags)
+        outs = s.outputs()
+        for b in range(nb):
+            a0, g = int(offsets[b]), int(groups[b])
+            got = rows_of([(LONG, v[a0:a0 + g], nl[a0:a0 + g]) for v, nl in outs])
+            assert_row_sets_equal(got, want[b])
+    p1, p2, p3 = s.phases()
+    assert p1 > 0 and p2 > 0 and p3 == 0
+    s.close()
+    if net is not None:
+        net.close()
 
 
 def test_graph_replays_match_eager():
@@ -221,11 +295,14 @@ def test_batches_decimal_overflow_null_and_error():
     s.close()
 
 
-def test_batched_stream_config_full_size():
+@pytest.mark.parametrize("form", ["phases", "fused_learned"])
+def test_batched_stream_config_full_size(form):
     """20,000 batches (lognormal rows, median 2048, clipped to [64, 65536]), per-batch distinct keys
-    log-uniform in [8, 1e6]; SUM(int64) + COUNT.  Size-independent checks on every batch (COUNT sums to the
-    batch's rows, SUM equals the batch's wrapping total, group count equals the batch's distinct keys) and
-    bit-exact parity with the oracle on a sample of batches."""
+    log-uniform in [8, 1e6]; SUM(int64) + COUNT, in the phased form and in the one-workgroup-per-batch form with
+    the stream advisor (the benchmarked learned + batched form).  Size-independent checks on every batch (COUNT
+    sums to the batch's rows, SUM equals the batch's wrapping total, group count equals the batch's distinct keys)
+    and bit-exact parity with the oracle on a sample of batches."""
+    from omnioperator_amd import batch_advisor
     w = workloads.batched_stream()
     offsets = np.concatenate([[0], np.cumsum(w.row_counts)])
     aggs = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG)]
@@ -233,9 +310,18 @@ def test_batched_stream_config_full_size():
     s = capi.AggBatches(spec, 65536)
     kd = capi.DeviceColumn(LONG, w.keys)
     vd = capi.DeviceColumn(LONG, w.values)
-    groups = s.run(kd.c(), [vd.c(), None], offsets, None, capi.AggBatches.FLAG_GRAPH)
+    flags = capi.AggBatches.FLAG_GRAPH
+    net = None
+    if form == "fused_learned":
+        net = batch_advisor.network()
+        s.set_advisor(net)
+        flags |= capi.AggBatches.FLAG_FUSED | capi.AggBatches.FLAG_ADVISOR
+    for _ in range(2):  # captured, then replayed
+        groups = s.run(kd.c(), [vd.c(), None], offsets, None, flags)
     outs = s.outputs()
     s.close()
+    if net is not None:
+        net.close()
     kv, sv, cv = outs[0][0], outs[1][0], outs[2][0]
     nb = len(w.row_counts)
     owner = np.repeat(np.arange(nb), w.row_counts)
