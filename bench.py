@@ -379,10 +379,11 @@ def bench_batched(args):
     """The headline configuration: 20,000 independent group-bys (lognormal rows, median 2048, clipped to
     [64, 65536]; per-batch distinct keys log-uniform in [8, 1e6]), SUM(int64) + COUNT(*), one run over the stream.
     Default form "learned": the learned + batched form -- the stream advisor (a per-batch sketch through the MLP,
-    on the device) picks each batch's first phase, tiles from the autotuner, the whole run one hipGraph launch.
-    It is compared in the same process with the plain form (one launch per batch and phase, the reference's one
-    operator per VectorBatch) and with the fixed-strategy batched forms (every batch LDS-first, every batch
-    partitioned) at the same tiles."""
+    on the device) predicts each batch's group count, which decides direct LDS aggregation or hash partitions
+    (and their number) inside the one-workgroup-per-batch kernel; tiles from the autotuner; the whole run one
+    hipGraph launch.  It is compared in the same process with the plain form (one launch per batch and phase, the
+    reference's one operator per VectorBatch), with the same kernel without the advisor (its own distinct
+    estimate) and with the earlier phased batched forms (every batch LDS-first, every batch partitioned)."""
     from omnioperator_amd import autotune, batch_advisor, capi, workloads
     capi.init(0)
     w = workloads.batched_stream()
@@ -410,15 +411,18 @@ def bench_batched(args):
     def step():
         return int(run(flags).sum())
 
+    def tune(fl, form):
+        key = autotune.key_of(capi.LONG, aggs, autotune.cardinality_bucket(float(np.mean(w.distinct)))) + f"|{form}"
+        cands = autotune.FUSED_CANDIDATES if fl & F.FLAG_FUSED else autotune.CANDIDATES
+        return autotune.tune_stream(s, lambda: (run(fl), capi.synchronize()), key, cands, use_cache=False)
+
     tuned = None
     if args.tiles:  # fixed tiles, e.g. for a profile of one configuration
         b, l, p_, q = (int(x) for x in args.tiles.split(","))
         tuned = autotune.StreamConfig(b, l, p_, q)
         autotune.apply(s, tuned)
     elif args.batched_form != "per_batch":  # the plain one-launch-per-batch form keeps the default tiles
-        key = autotune.key_of(capi.LONG, aggs, autotune.cardinality_bucket(float(np.mean(w.distinct)))) + \
-            f"|{args.batched_form}"
-        tuned = autotune.tune_stream(s, lambda: (step(), capi.synchronize()), key, use_cache=False)
+        tuned = tune(flags, args.batched_form)
 
     groups, ms = timed_steps(step, args, capi)
     prof = "agg_batches_graph" if flags & F.FLAG_GRAPH else \
@@ -439,36 +443,50 @@ def bench_batched(args):
             capi.synchronize()
             assert g == groups
             return (time.perf_counter() - t0) * 1e3 / steps
-        compare["fixed_lds_first_ms"] = timed(F.FLAG_GRAPH)
-        compare["fixed_partitioned_ms"] = timed(F.FLAG_GRAPH, all_part)
-        best_fixed = min(compare["fixed_lds_first_ms"], compare["fixed_partitioned_ms"])
+        compare["fused_without_advisor_ms"] = timed(F.FLAG_GRAPH | F.FLAG_FUSED)
+        phased_tiles = tune(F.FLAG_GRAPH, "graph")
+        compare["phased_lds_first_ms"] = timed(F.FLAG_GRAPH)
+        compare["phased_partitioned_ms"] = timed(F.FLAG_GRAPH, all_part)
+        compare["phased_learned_ms"] = timed(F.FLAG_GRAPH | F.FLAG_ADVISOR)
+        best_fixed = min(compare["fused_without_advisor_ms"], compare["phased_lds_first_ms"],
+                         compare["phased_partitioned_ms"])
         autotune.apply(s, autotune.DEFAULT)
         compare["plain_one_launch_per_batch_ms"] = timed(F.FLAG_PER_BATCH, steps=1)
         if tuned is not None:
             autotune.apply(s, tuned)
         compare["speedup_vs_plain"] = compare["plain_one_launch_per_batch_ms"] / ms
         compare["speedup_vs_best_fixed_batched"] = best_fixed / ms
+        compare["speedup_vs_fused_without_advisor"] = compare["fused_without_advisor_ms"] / ms
         compare = {k: round(v, 4) for k, v in compare.items()}
+        compare["phased_tiles"] = {"block": phased_tiles.block, "lds_bytes": phased_tiles.lds_bytes,
+                                   "max_probe": phased_tiles.max_probe, "part_lds_bytes": phased_tiles.part_lds_bytes}
     s.close()
     if net is not None:
         net.close()
     alg = n * 16 + groups * 24
     achieved = alg / (run_ms / 1e3) / 1e9
+    fused = bool(flags & F.FLAG_FUSED)
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": profiled_traffic("stream (all kernels)", "batched_groupby_c4.md")
             if args.batched_form == "learned" and not args.tiles else None,
-            "kernel": "the stream's launch sequence (one hipGraph launch: advisor sketch + MLP, per-batch LDS table, "
-                      "scan, split, partition tables, exact fallback)" if flags & F.FLAG_GRAPH else prof,
+            "kernel": ("the stream's launch sequence (one hipGraph launch: advisor sketch + MLP, one workgroup per batch "
+                       "doing its whole group-by in LDS, region scan, exact fallback)" if fused else
+                       "the stream's launch sequence (one hipGraph launch: advisor sketch + MLP, per-batch LDS table, "
+                       "scan, split, partition tables, exact fallback)") if flags & F.FLAG_GRAPH else prof,
             "kernel_ms": round(run_ms, 4), "algorithmic_bytes": alg}
     cpu = cpu_baseline_stream(w, offsets, spec) if not args.no_cpu else None
     cfg = {"workload": f"batched group-by: {nb} independent VectorBatches, {n} rows (lognormal rows, median 2048, "
                        "[64, 65536]), per-batch distinct keys log-uniform in [8, 1e6]; SUM(int64) + COUNT(*)",
-           "form": {"learned": "learned + batched (stream advisor, autotuned tiles, hipGraph)",
-                    "graph": "batched, every batch LDS-first (hipGraph)", "two_launches": "batched, no graph",
+           "form": {"learned": "learned + batched (stream advisor predicting each batch's groups, one workgroup per "
+                               "batch, autotuned tiles, hipGraph)",
+                    "fused": "batched, one workgroup per batch, own distinct estimate (hipGraph)",
+                    "learned_phases": "learned + batched, phased kernels (hipGraph)",
+                    "graph": "batched, phased, every batch LDS-first (hipGraph)", "two_launches": "batched, no graph",
                     "per_batch": "plain: one launch per batch and phase"}[args.batched_form],
            "rows": n, "batches": nb, "groups": groups,
-           "batches_by_phase": {"lds_table": ph[0], "lds_partitions": ph[1], "hbm_table": ph[2]},
+           "batches_by_phase": ({"direct": ph[0], "partitioned": ph[1], "hbm_table": ph[2]} if fused else
+                                {"lds_table": ph[0], "lds_partitions": ph[1], "hbm_table": ph[2]}),
            "tiles": None if tuned is None else {"block": tuned.block, "lds_bytes": tuned.lds_bytes,
                                                 "max_probe": tuned.max_probe, "part_lds_bytes": tuned.part_lds_bytes},
            "algorithmic_bytes_per_step": alg, "bytes_per_row_in": 16, "bytes_per_group_out": 24}

@@ -1,5 +1,5 @@
 """Autotuner for the batched group-by: picks threads per workgroup, LDS table bytes, the LDS probe limit and the
-phase-2 partition table bytes by
+phase-2 partition table bytes (for the one-workgroup-per-batch form: threads and its table bytes) by
 timing candidate runs on the stream itself, per (key type, aggregate signature, cardinality bucket); winners
 are cached in a JSON file (OMNI_GPU_AUTOTUNE_CACHE, default ~/.cache/omnioperator_amd/autotune.json) so later
 runs of the same kind skip the sweep.
@@ -38,6 +38,12 @@ CANDIDATES: List[StreamConfig] = [
     for l in (16384, 32768, 65536)
     for p in (8, 32)
     for q in (16384, 32768, 65536)
+]
+
+
+# the one-workgroup-per-batch form (flag bit 4) reads only the block size and the table bytes
+FUSED_CANDIDATES: List[StreamConfig] = [
+    StreamConfig(b, 32768, 32, q) for b in (256, 512) for q in (24576, 32768, 40960, 49152, 65536)
 ]
 
 

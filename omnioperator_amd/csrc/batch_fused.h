@@ -11,9 +11,9 @@
 //   3. aggregates partition after partition in its LDS table (sized to the partition's rows), emitting each
 //      partition's groups to the batch's output rows through a counter in LDS; the records of the next partition
 //      are loaded into registers while the current one is aggregated and emitted.
-// With an advisor, the predicted group count (times kFusedMargin) replaces the estimate: a batch predicted to fit
-// skips step 1 and goes direct (a prediction that is too low shows as a full table, and the batch then takes
-// steps 1-3 with its own estimate); otherwise P follows the prediction.  A partition whose keys overflow the largest table (adversarial
+// With an advisor, a batch predicted to fill at most kDirectLoad of the table skips step 1 and goes direct (a
+// prediction that is too low shows as a full table, and the batch then takes steps 1-3 with its own estimate);
+// otherwise P follows the larger of the estimate and the prediction times kFusedMargin.  A partition whose keys overflow the largest table (adversarial
 // hash skew or a far too low prediction) flags the batch for the exact HBM-table pass (spilled = 2), which
 // re-aggregates it from its columns and overwrites its output rows.
 #pragma once
@@ -26,7 +26,8 @@ constexpr int kFusedParts = 256;      // partitions counted in step 1 (P divides
 constexpr float kFusedLoad = 0.45f;   // distinct keys per table slot aimed at
 constexpr int kFusedAhead = 4;        // records per thread loaded ahead
 constexpr int kFusedRows = 4;         // rows per thread whose loads are issued together in the row loops
-constexpr float kFusedMargin = 1.5f;  // an advisor prediction is scaled by this before it sizes tables
+constexpr float kFusedMargin = 1.25f; // an advisor prediction is scaled by this before it sizes partitions
+constexpr float kDirectLoad = 0.6f;   // a batch with at most this share of the largest table's slots goes direct
 constexpr int kLiteAggs = 4;          // aggregates of the lite table at most
 enum LiteOp : int { kLiteCount = 0, kLiteSum = 1, kLiteMin = 2, kLiteMax = 3 };
 
@@ -176,6 +177,18 @@ __device__ __forceinline__ void FusedScan(unsigned int *v, int n, unsigned int *
     __syncthreads();
 }
 
+// Loads and stores through global-address-space pointers: the pointers come from the configuration in memory, so
+// the compiler would otherwise emit flat accesses, which also count against the LDS wait counter and make every
+// wait for a load a wait for the LDS operations in flight as well.
+template <typename T> __device__ __forceinline__ T LdG(const T *p, int64_t i)
+{
+    return ((const __attribute__((address_space(1))) T *)(p))[i];
+}
+template <typename T> __device__ __forceinline__ void StG(T *p, int64_t i, T v)
+{
+    ((__attribute__((address_space(1))) T *)(p))[i] = v;
+}
+
 // Everything the workgroup reads of the configuration's columns, loaded into registers once (the configuration
 // lives in global memory, and the workgroup's own global stores would otherwise have it read again).
 struct FusedIn {
@@ -210,6 +223,16 @@ __device__ __forceinline__ FusedIn LoadFusedIn(const BatchConfig &c)
 struct GenericTable {
     const BatchConfig &c;
     uint8_t *lds;
+    FusedIn in;
+    __device__ __forceinline__ unsigned long long Key(int64_t row, bool &nul) const
+    {
+        nul = NullBit(in.knulls, row);
+        return nul ? 0ULL : LoadWord(in.kvals, in.kwidth, row);
+    }
+    __device__ __forceinline__ unsigned long long Val(int64_t row) const
+    {
+        return in.cvals ? LoadWord(in.cvals, in.cwidth, row) : 0ULL;
+    }
     __device__ __forceinline__ unsigned long long *Keys() const
     {
         return reinterpret_cast<unsigned long long *>(lds + c.L2.keys_off);
@@ -220,33 +243,61 @@ struct GenericTable {
         if (carry) UpdateAggsCarried(c.args, LdsRef{lds, c.L2, slot}, v);
         else UpdateAggsT(c.args, LdsRef{lds, c.L2, slot}, row);
     }
-    template <int BLOCK> __device__ __forceinline__ void Emit(int S, int64_t start, unsigned int *counter) const
+    // Emits the used slots of [0, S + 2); with RESET, also returns each of them to its initial state (the
+    // other slots are untouched since the last reset, so the whole table is clean afterwards).
+    template <int BLOCK, bool RESET = false>
+    __device__ __forceinline__ void Emit(int S, int64_t start, unsigned int *counter) const
     {
-        EmitTable<BLOCK>(c, Keys(), S, start, counter, [&](int i) { return LdsRef{lds, c.L2, i}; });
+        unsigned long long *keys = Keys();
+        for (int base = threadIdx.x - LaneId(); base < S + 2; base += BLOCK) {
+            const int i = base + LaneId();
+            const unsigned long long k = i < S + 2 ? keys[i] : kEmptyKey;
+            const bool used = k != kEmptyKey;
+            const unsigned long long mask = __ballot(used);
+            unsigned int first = 0;
+            if (LaneId() == 0 && mask) first = atomicAdd(counter, static_cast<unsigned int>(__popcll(mask)));
+            first = __shfl(first, 0, 64);
+            if (!used) continue;
+            const unsigned int r = first + static_cast<unsigned int>(__popcll(mask & LaneMaskLt()));
+            const bool side = i >= S;
+            EmitGroup(c, LdsRef{lds, c.L2, i}, side ? (i == S ? 0ULL : kEmptyKey) : k, i == S, start + r);
+            if (RESET) {
+                keys[i] = kEmptyKey;
+                InitSlot(c.args, LdsRef{lds, c.L2, i});
+            }
+        }
     }
 };
 
 // The table of the lite form: NA <= kLiteAggs aggregates, each COUNT / SUM / MIN / MAX into one int64 state word,
 // over a carried NULL-free column, with 8-byte outputs.  LDS: the u64 keys, then one u64 state array per
 // aggregate, `stride` slots each; the aggregate codes and the output columns stay in registers.
-template <int NA> struct LiteTable {
+template <int NA, int KW> struct LiteTable {
     unsigned long long *lkeys;
     int stride;
-    int kwidth;
     int op[NA];
+    const void *kvals;
+    const unsigned long long *cvals;  // the carried int64 column (nullptr: every aggregate is COUNT(*))
     void *okey;
     unsigned long long *oval[NA];
+    __device__ __forceinline__ unsigned long long Key(int64_t row, bool &nul) const
+    {
+        nul = false;
+        if constexpr (KW == 8) return LdG(static_cast<const unsigned long long *>(kvals), row);
+        else return static_cast<unsigned long long>(static_cast<long long>(LdG(static_cast<const int *>(kvals), row)));
+    }
+    __device__ __forceinline__ unsigned long long Val(int64_t row) const { return cvals ? LdG(cvals, row) : 0ULL; }
     __device__ __forceinline__ unsigned long long *Keys() const { return lkeys; }
+    __device__ __forceinline__ unsigned long long Init(int a) const
+    {
+        return op[a] == kLiteMin ? 0x7FFFFFFFFFFFFFFFULL : op[a] == kLiteMax ? 0x8000000000000000ULL : 0ULL;
+    }
     template <int BLOCK> __device__ __forceinline__ void Clear(int S) const
     {
         for (int i = threadIdx.x; i < S + 2; i += BLOCK) {
             lkeys[i] = kEmptyKey;
 #pragma unroll
-            for (int a = 0; a < NA; ++a) {
-                lkeys[(a + 1) * stride + i] = op[a] == kLiteMin   ? 0x7FFFFFFFFFFFFFFFULL
-                                              : op[a] == kLiteMax ? 0x8000000000000000ULL
-                                                                  : 0ULL;
-            }
+            for (int a = 0; a < NA; ++a) lkeys[(a + 1) * stride + i] = Init(a);
         }
     }
     __device__ __forceinline__ void Update(int slot, int64_t, unsigned long long v, int) const
@@ -262,7 +313,8 @@ template <int NA> struct LiteTable {
             }
         }
     }
-    template <int BLOCK> __device__ __forceinline__ void Emit(int S, int64_t start, unsigned int *counter) const
+    template <int BLOCK, bool RESET = false>
+    __device__ __forceinline__ void Emit(int S, int64_t start, unsigned int *counter) const
     {
         for (int base = threadIdx.x - LaneId(); base < S + 2; base += BLOCK) {
             const int i = base + LaneId();
@@ -274,19 +326,28 @@ template <int NA> struct LiteTable {
             first = __shfl(first, 0, 64);
             if (!used) continue;
             const int64_t r = start + first + static_cast<unsigned int>(__popcll(mask & LaneMaskLt()));
-            StoreWidth(okey, kwidth, r, i == S + 1 ? kEmptyKey : k);  // slot S + 1: the sentinel's own key
+            const unsigned long long kout = i == S + 1 ? kEmptyKey : k;  // slot S + 1: the sentinel's own key
+            if constexpr (KW == 8) StG(static_cast<unsigned long long *>(okey), r, kout);
+            else StG(static_cast<unsigned int *>(okey), r, static_cast<unsigned int>(kout));
 #pragma unroll
-            for (int a = 0; a < NA; ++a) oval[a][r] = lkeys[(a + 1) * stride + i];
+            for (int a = 0; a < NA; ++a) {
+                unsigned long long *w = lkeys + (a + 1) * stride + i;
+                StG(oval[a], r, *w);
+                if (RESET) *w = Init(a);
+            }
+            if (RESET) lkeys[i] = kEmptyKey;
         }
     }
 };
 
-template <int NA> __device__ __forceinline__ LiteTable<NA> MakeLiteTable(const BatchConfig &c, uint8_t *lds)
+template <int NA, int KW>
+__device__ __forceinline__ LiteTable<NA, KW> MakeLiteTable(const BatchConfig &c, uint8_t *lds)
 {
-    LiteTable<NA> t;
+    LiteTable<NA, KW> t;
     t.lkeys = reinterpret_cast<unsigned long long *>(lds);
     t.stride = c.lite_smax + 2;
-    t.kwidth = c.out.o[0].width;
+    t.kvals = c.kd.values[0];
+    t.cvals = c.carry_agg >= 0 ? static_cast<const unsigned long long *>(c.args.a[c.carry_agg].values) : nullptr;
     t.okey = c.out.o[0].values;
 #pragma unroll
     for (int a = 0; a < NA; ++a) {
@@ -316,10 +377,13 @@ __device__ __forceinline__ bool FusedDirect(const BatchConfig &c, const FusedIn 
 #pragma unroll
         for (int u = 0; u < kFusedRows; ++u) {  // every load of the chunk in flight before the first insert
             const int64_t row = base + u * BLOCK;
-            const bool inb = row < end;
-            nul[u] = inb && NullBit(in.knulls, row);
-            key[u] = inb && !nul[u] ? LoadWord(in.kvals, in.kwidth, row) : 0ULL;
-            val[u] = inb && in.cvals ? LoadWord(in.cvals, in.cwidth, row) : 0ULL;
+            nul[u] = false;
+            key[u] = 0;
+            val[u] = 0;
+            if (row < end) {
+                key[u] = t.Key(row, nul[u]);
+                if (in.carry) val[u] = t.Val(row);
+            }
         }
 #pragma unroll
         for (int u = 0; u < kFusedRows; ++u) {
@@ -370,8 +434,8 @@ __device__ __forceinline__ void FusedLoad(const FusedIn &in, int64_t start, unsi
     for (int k = 0; k < kFusedAhead; ++k) {
         const unsigned int e = e0 + threadIdx.x + k * BLOCK;
         if (e < e1) {
-            r.key[k] = in.pkeys[start + e];
-            r.val[k] = in.carry ? in.pvals[start + e] : static_cast<unsigned long long>(in.prows[start + e]);
+            r.key[k] = LdG(in.pkeys, start + e);
+            r.val[k] = in.carry ? LdG(in.pvals, start + e) : static_cast<unsigned long long>(LdG(in.prows, start + e));
         }
     }
 }
@@ -397,6 +461,35 @@ __device__ __forceinline__ void FusedInsert(const FusedIn &in, const Table &t, i
     }
 }
 
+// The rows of one staging chunk held in registers (row base + threadIdx.x + u * BLOCK): key, carried value or
+// (partition << 32) | batch-relative row, partition.
+struct FusedChunkRegs {
+    unsigned long long key[kFusedRows];
+    unsigned long long val[kFusedRows];
+    unsigned int part[kFusedRows];
+};
+
+template <int BLOCK, typename Table>
+__device__ __forceinline__ void FusedLoadChunk(const FusedIn &in, const Table &t, int64_t start, int64_t end,
+                                               int64_t base, int P, FusedChunkRegs &r)
+{
+#pragma unroll
+    for (int u = 0; u < kFusedRows; ++u) {
+        const int64_t row = base + threadIdx.x + u * BLOCK;
+        bool nul = false;
+        r.key[u] = 0;
+        r.val[u] = 0;
+        if (row < end) {
+            r.key[u] = t.Key(row, nul);
+            if (in.carry) r.val[u] = t.Val(row);
+        }
+        r.part[u] = static_cast<unsigned int>(PartOfKey(r.key[u], nul, P));
+        if (!in.carry && row < end) {
+            r.val[u] = (static_cast<unsigned long long>(r.part[u]) << 32) | static_cast<unsigned long long>(row - start);
+        }
+    }
+}
+
 // The whole group-by of batch b in one workgroup.  smax: the table's probe slots at most; lds_bytes: its LDS.
 template <int BLOCK, typename Table>
 __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t, int b, uint8_t *lds, int smax,
@@ -406,7 +499,8 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
     const FusedIn in = LoadFusedIn(c);
     const int64_t start = c.offsets[b], end = c.offsets[b + 1];
     const int n = static_cast<int>(end - start);
-    const int fit = c.part_fit > 0 ? c.part_fit : static_cast<int>(kFusedLoad * static_cast<float>(smax));
+    const bool part_fit_set = c.part_fit > 0;
+    const int fit = part_fit_set ? c.part_fit : static_cast<int>(kFusedLoad * static_cast<float>(smax));
     const bool forced = c.modes != nullptr && c.modes[b] == 2;
     const float *pred = c.pred;
     // 1. plan: direct when the rows (or the predicted groups) fit one table
@@ -416,20 +510,22 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
         if (FusedDirect<BLOCK>(c, in, t, b, start, end, FusedSlots(static_cast<float>(n), smax), sh)) return;
         triedDirect = true;
     }
-    float est = -1.f;
+    // a batch whose keys fill at most kDirectLoad of the largest table goes direct (a probe run is still short)
+    const float directMax = part_fit_set ? static_cast<float>(fit) : kDirectLoad * static_cast<float>(smax);
+    float predicted = 0.f;  // the advisor's group count (0: none)
     if (!forced && !triedDirect && pred != nullptr) {
         const float y = pred[static_cast<int64_t>(b) * 4];
-        est = fminf(static_cast<float>(n),
-                    kFusedMargin * fmaxf(1.f, exp2f(fminf(fmaxf(y, 0.f), 1.f) * 32.f) - 1.f));
-        if (est <= static_cast<float>(fit)) {
+        predicted = fminf(static_cast<float>(n), fmaxf(1.f, exp2f(fminf(fmaxf(y, 0.f), 1.f) * 32.f) - 1.f));
+        if (predicted <= directMax) {
             if (FusedDirect<BLOCK>(c, in, t, b, start, end, smax, sh)) return;
             triedDirect = true;  // the prediction was too low: count and estimate below
-            est = -1.f;
+            predicted = 0.f;
         }
     }
-    // 2. counts per partition (and the distinct estimate when there is no prediction, its bitmap in the table's
-    //    LDS, unused until step 4)
-    const bool estimate = est < 0.f;
+    // 2. counts per partition and the distinct estimate (its bitmap in the table's LDS, unused until step 4); a
+    //    prediction raises the estimate, never lowers it, so a low one cannot make partitions overflow
+    const bool estimate = true;
+    float est = -1.f;
     unsigned int *lcbits = reinterpret_cast<unsigned int *>(lds);
     for (int i = threadIdx.x; i < kFusedParts; i += BLOCK) cnt[i] = 0;
     if (estimate) {
@@ -443,17 +539,18 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
 #pragma unroll
         for (int u = 0; u < kFusedRows; ++u) {
             const int64_t row = base + u * BLOCK;
-            nul[u] = row < end && NullBit(in.knulls, row);
-            key[u] = row < end && !nul[u] ? LoadWord(in.kvals, in.kwidth, row) : 0ULL;
+            nul[u] = false;
+            key[u] = row < end ? t.Key(row, nul[u]) : 0ULL;
         }
 #pragma unroll
         for (int u = 0; u < kFusedRows; ++u) {
             if (base + u * BLOCK >= end) break;
-            atomicAdd(&cnt[PartOfKey(key[u], nul[u], kFusedParts)], 1u);
+            // one hash for both: the partition (PartOfKey's low bits) and the counting bit (bits 20..34)
+            const unsigned long long h = Mix64(key[u]);
+            const bool side = nul[u] || key[u] == kEmptyKey;
+            atomicAdd(&cnt[side ? 0 : static_cast<int>(h & (kFusedParts - 1))], 1u);
             if (estimate) {
-                const unsigned int bit =
-                    static_cast<unsigned int>(Mix64(key[u] ^ (nul[u] ? 0x5bd1e995ULL : 0ULL)) >> 20) &
-                    (kLcWords * 32 - 1);
+                const unsigned int bit = nul[u] ? 0u : static_cast<unsigned int>(h >> 20) & (kLcWords * 32 - 1);
                 atomicOr(&lcbits[bit >> 5], 1u << (bit & 31));
             }
         }
@@ -467,8 +564,9 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
         const float m = kLcWords * 32.f;
         const float zeros = m - static_cast<float>(sh[2]);
         est = zeros >= 1.f ? fminf(static_cast<float>(n), -m * logf(zeros / m)) : static_cast<float>(n);
+        est = fmaxf(est, fminf(static_cast<float>(n), kFusedMargin * predicted));
     }
-    int P = FusedParts(est, fit);
+    int P = est <= directMax ? 1 : FusedParts(est, fit);
     if (P == 1) {
         if (!triedDirect && FusedDirect<BLOCK>(c, in, t, b, start, end, smax, sh)) return;
         P = 2;  // the keys did not fit one table after all
@@ -492,34 +590,22 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
     for (int64_t base = start; base < end; base += cap) {
         const int m = end - base < cap ? static_cast<int>(end - base) : cap;
         for (int p = threadIdx.x; p <= P; p += BLOCK) ccnt[p] = 0;
+        FusedChunkRegs chunk;
+        FusedLoadChunk<BLOCK>(in, t, start, end, base, P, chunk);
         __syncthreads();
-        unsigned long long key[kFusedRows], val[kFusedRows];
-        unsigned int part[kFusedRows], rank[kFusedRows];
+        unsigned int rank[kFusedRows];
 #pragma unroll
         for (int u = 0; u < kFusedRows; ++u) {
-            const int li = static_cast<int>(threadIdx.x) + u * BLOCK;
-            const int64_t row = base + li;
-            const bool inb = li < m;
-            const bool nul = inb && NullBit(in.knulls, row);
-            key[u] = inb && !nul ? LoadWord(in.kvals, in.kwidth, row) : 0ULL;
-            part[u] = static_cast<unsigned int>(PartOfKey(key[u], nul, P));
-            val[u] = !inb     ? 0ULL
-                     : in.carry ? (in.cvals ? LoadWord(in.cvals, in.cwidth, row) : 0ULL)
-                                : (static_cast<unsigned long long>(part[u]) << 32) |
-                                      static_cast<unsigned long long>(row - start);
-        }
-#pragma unroll
-        for (int u = 0; u < kFusedRows; ++u) {
-            if (static_cast<int>(threadIdx.x) + u * BLOCK < m) rank[u] = atomicAdd(&ccnt[part[u]], 1u);
+            if (static_cast<int>(threadIdx.x) + u * BLOCK < m) rank[u] = atomicAdd(&ccnt[chunk.part[u]], 1u);
         }
         __syncthreads();
         FusedScan<BLOCK>(ccnt, P, wsum);  // chunk offsets per partition; ccnt[P] = m
 #pragma unroll
         for (int u = 0; u < kFusedRows; ++u) {
             if (static_cast<int>(threadIdx.x) + u * BLOCK < m) {
-                const unsigned int pos = ccnt[part[u]] + rank[u];
-                skeys[pos] = key[u];
-                svals[pos] = val[u];
+                const unsigned int pos = ccnt[chunk.part[u]] + rank[u];
+                skeys[pos] = chunk.key[u];
+                svals[pos] = chunk.val[u];
             }
         }
         __syncthreads();
@@ -529,9 +615,9 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
             const unsigned int p =
                 in.carry ? static_cast<unsigned int>(PartOfKey(k, false, P)) : static_cast<unsigned int>(v >> 32);
             const unsigned int at = cnt[p] + static_cast<unsigned int>(i) - ccnt[p];
-            in.pkeys[start + at] = k;
-            if (in.carry) in.pvals[start + at] = v;
-            else in.prows[start + at] = static_cast<unsigned int>(v);
+            StG(in.pkeys, start + at, k);
+            if (in.carry) StG(in.pvals, start + at, v);
+            else StG(in.prows, start + at, static_cast<unsigned int>(v));
         }
         __syncthreads();
         for (int p = threadIdx.x; p < P; p += BLOCK) cnt[p] += ccnt[p + 1] - ccnt[p];
@@ -545,12 +631,12 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
     }
     FusedRecs cur;
     FusedLoad<BLOCK>(in, start, aux[0], aux[1], cur);
+    t.template Clear<BLOCK>(smax);  // once: each partition's emission resets the slots it used
+    __syncthreads();
     bool full = false;
     for (int p = 0; p < P; ++p) {
         const unsigned int e0 = aux[p], e1 = aux[p + 1];
         const int S = FusedSlots(static_cast<float>(e1 - e0), smax);
-        t.template Clear<BLOCK>(S);
-        __syncthreads();
         FusedInsert<BLOCK>(in, t, start, e0, e1, cur, S, sh);
         // records past the ones held in registers (a partition of many duplicate keys), kFusedAhead per thread at a time
         for (unsigned int base = e0 + kFusedAhead * BLOCK; base < e1; base += kFusedAhead * BLOCK) {
@@ -562,7 +648,7 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
         __syncthreads();
         full = sh[0] != 0u;
         if (full) break;
-        t.template Emit<BLOCK>(S, start, &sh[1]);
+        t.template Emit<BLOCK, true>(S, start, &sh[1]);
         __syncthreads();
     }
     (void)lkeys;

@@ -56,8 +56,8 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const B
 }
 
 // One workgroup per batch (batch order[blockIdx.x]) does the batch's whole group-by (BatchFused), with the
-// generic table (NA = 0) or the lite table of NA aggregates.
-template <int BLOCK, int NA>
+// generic table (NA = 0) or the lite table of NA aggregates over KW-byte keys.
+template <int BLOCK, int NA, int KW>
 __global__ void __launch_bounds__(BLOCK) batch_fused(const BatchConfig *__restrict__ cfg, const int32_t *__restrict__ order)
 {
     extern __shared__ unsigned long long lds_words[];
@@ -69,23 +69,31 @@ __global__ void __launch_bounds__(BLOCK) batch_fused(const BatchConfig *__restri
     const BatchConfig &c = *cfg;
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_words);
     if constexpr (NA == 0) {
-        BatchFused<BLOCK>(c, GenericTable{c, lds}, order[blockIdx.x], lds, c.L2.S, c.L2.bytes, cnt, aux, ccnt, wsum, sh);
-    } else {
-        BatchFused<BLOCK>(c, MakeLiteTable<NA>(c, lds), order[blockIdx.x], lds, c.lite_smax, c.lite_bytes, cnt, aux,
+        BatchFused<BLOCK>(c, GenericTable{c, lds, LoadFusedIn(c)}, order[blockIdx.x], lds, c.L2.S, c.L2.bytes, cnt, aux,
                           ccnt, wsum, sh);
+    } else {
+        BatchFused<BLOCK>(c, MakeLiteTable<NA, KW>(c, lds), order[blockIdx.x], lds, c.lite_smax, c.lite_bytes, cnt,
+                          aux, ccnt, wsum, sh);
+    }
+}
+
+template <int BLOCK, int KW>
+void LaunchFusedLite(const BatchConfig &hc, const BatchConfig *dc, const int32_t *order, int nb, hipStream_t s)
+{
+    switch (hc.lite_na) {
+        case 1: batch_fused<BLOCK, 1, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        case 2: batch_fused<BLOCK, 2, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        case 3: batch_fused<BLOCK, 3, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+        default: batch_fused<BLOCK, 4, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
     }
 }
 
 template <int BLOCK>
 void LaunchFused(const BatchConfig &hc, const BatchConfig *dc, const int32_t *order, int nb, hipStream_t s)
 {
-    switch (hc.lite_na) {
-        case 1: batch_fused<BLOCK, 1><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        case 2: batch_fused<BLOCK, 2><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        case 3: batch_fused<BLOCK, 3><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        case 4: batch_fused<BLOCK, 4><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        default: batch_fused<BLOCK, 0><<<nb, BLOCK, hc.L2.bytes, s>>>(dc, order); break;
-    }
+    if (hc.lite_na == 0) batch_fused<BLOCK, 0, 8><<<nb, BLOCK, hc.L2.bytes, s>>>(dc, order);
+    else if (hc.kd.width[0] == 8) LaunchFusedLite<BLOCK, 8>(hc, dc, order, nb, s);
+    else LaunchFusedLite<BLOCK, 4>(hc, dc, order, nb, s);
 }
 
 // Phase 3 launches are grid-stride over their tiles / batches (grid capped at kPhase3Grid) and return at once
@@ -227,9 +235,13 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     if (h.fused && !per_batch) {
         // the advisor's predictions (no per-batch plan: the fused kernel reads the predicted group counts)
         if (h.use_advisor) {
+            // batches of at most a partition's worth of rows go direct without a prediction: not sketched
+            const int smax = h.cfg.lite_na ? h.cfg.lite_smax : h.cfg.L2.S;
+            const int64_t fit = h.cfg.part_fit > 0 ? h.cfg.part_fit
+                                                   : static_cast<int64_t>(kFusedLoad * static_cast<float>(smax));
             OG_TRY(LaunchBatchAdvisor(s, h.advisor, h.cfg.kd.values[0], h.cfg.kd.nulls[0], h.cfg.kd.width[0],
                                       h.offsets.as<int64_t>(), nb, h.adv_x.as<float>(), h.adv_y.as<float>(), nullptr,
-                                      0.f));
+                                      0.f, fit));
         }
         const int32_t *order = h.order.as<int32_t>();
         if (h.block == 512) LaunchFused<512>(h.cfg, dc, order, nb, s);
@@ -247,7 +259,7 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
         if (h.use_advisor) {
             OG_TRY(LaunchBatchAdvisor(s, h.advisor, h.cfg.kd.values[0], h.cfg.kd.nulls[0], h.cfg.kd.width[0],
                                       h.offsets.as<int64_t>(), nb, h.adv_x.as<float>(), h.adv_y.as<float>(),
-                                      h.cfg.sketch_modes, kAdvisorSlotShare * static_cast<float>(h.cfg.L.S)));
+                                      h.cfg.sketch_modes, kAdvisorSlotShare * static_cast<float>(h.cfg.L.S), -1));
         } else if (h.cfg.sketch_modes) {
             batch_sketch<256><<<nb, 256, 0, s>>>(dc);
             OG_TRY(CheckLaunch("batch_sketch"));
@@ -515,7 +527,8 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
         if (!carry) c.carry_agg = -1;
         c.carry = carry ? 1 : 0;
         // the lite table: at most kLiteAggs aggregates, each COUNT or an int64 SUM / MIN / MAX with an 8-byte output
-        bool lite = carry && c.args.n >= 1 && c.args.n <= kLiteAggs;
+        bool lite = carry && c.args.n >= 1 && c.args.n <= kLiteAggs && (c.kd.width[0] == 4 || c.kd.width[0] == 8) &&
+                    (c.carry_agg < 0 || c.args.a[c.carry_agg].in_width == 8);
         for (int i = 0; i < c.args.n && lite; ++i) {
             const int k = c.args.a[i].kind;
             lite = (k == K_COUNT_ALL || k == K_COUNT_COL || k == K_SUM_I64 || k == K_MIN_I64 || k == K_MAX_I64) &&

@@ -190,15 +190,17 @@ int GradStep(MlpHandle &h, const int *idx, int B, float lr, bool keep_grad)
 }
 
 int LaunchBatchAdvisor(hipStream_t s, void *mlp, const void *keys, const uint8_t *nulls, int width,
-                       const int64_t *offsets, int nb, float *X, float *Y, int32_t *modes, float limit)
+                       const int64_t *offsets, int nb, float *X, float *Y, int32_t *modes, float limit, int64_t min_rows)
 {
     if (nb <= 0) return OMNI_GPU_OK;
-    batch_features<<<nb, kBatchSketchBlock, 0, s>>>(keys, nulls, width, offsets, X);
+    batch_features<<<nb, kBatchSketchBlock, 0, s>>>(keys, nulls, width, offsets, X, min_rows);
     OG_TRY(CheckLaunch("batch_features"));
     if (!mlp) return OMNI_GPU_OK;
     const MlpHandle &h = *static_cast<const MlpHandle *>(mlp);
-    mlp_forward<<<(nb + kMlpFwdBlock - 1) / kMlpFwdBlock, kMlpFwdBlock, 0, s>>>(h.params.as<float>(), X, nb, Y);
-    OG_TRY(CheckLaunch("mlp_forward"));
+    const int waves = kMlpWaveBlock / kWaveSize;
+    const int grid = std::max(1, std::min((nb + waves - 1) / waves, 1024));
+    mlp_forward_waves<<<grid, kMlpWaveBlock, 0, s>>>(h.params.as<float>(), X, nb, Y);
+    OG_TRY(CheckLaunch("mlp_forward_waves"));
     if (!modes) return OMNI_GPU_OK;
     batch_plan<<<(nb + 255) / 256, 256, 0, s>>>(Y, nb, limit, modes);
     return CheckLaunch("batch_plan");
@@ -231,7 +233,7 @@ int omni_gpu_batch_features(const omni_gpu_column *key, const int64_t *batch_off
     }
     if (rc == OMNI_GPU_OK) {
         rc = LaunchBatchAdvisor(s, nullptr, key->values, key->nulls, TypeWidth(key->type_id), off.as<int64_t>(),
-                                num_batches, X.as<float>(), nullptr, nullptr, 0.f);
+                                num_batches, X.as<float>(), nullptr, nullptr, 0.f, -1);
     }
     if (rc == OMNI_GPU_OK && (hipMemcpyAsync(features, X.p, sizeof(float) * kIn * num_batches, hipMemcpyDeviceToHost,
                                              s) != hipSuccess ||

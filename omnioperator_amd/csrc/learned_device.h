@@ -54,6 +54,39 @@ __global__ void __launch_bounds__(kMlpFwdBlock) mlp_forward(const float *__restr
     for (int j = 0; j < kOut; ++j) Y[b * kOut + j] = y[j];
 }
 
+// Inference with one wave per sample (many samples, latency-bound otherwise): lane j computes hidden unit j of
+// both hidden layers, reading the previous layer's outputs lane by lane (v_readlane), and the four outputs are
+// reduced across the wave.  Same network and parameter layout as MlpForward.
+constexpr int kMlpWaveBlock = 256;
+
+__global__ void __launch_bounds__(kMlpWaveBlock) mlp_forward_waves(const float *__restrict__ params,
+                                                                   const float *__restrict__ X, int B,
+                                                                   float *__restrict__ Y)
+{
+    static_assert(kH == kWaveSize, "one lane per hidden unit");
+    __shared__ float P[kParams];
+    for (int i = threadIdx.x; i < kParams; i += kMlpWaveBlock) P[i] = params[i];
+    __syncthreads();
+    const int lane = LaneId();
+    constexpr int kWaves = kMlpWaveBlock / kWaveSize;
+    for (int b = blockIdx.x * kWaves + WaveInBlock(); b < B; b += gridDim.x * kWaves) {
+        float s = P[kOffB1 + lane];
+        for (int i = 0; i < kIn; ++i) s += X[static_cast<int64_t>(b) * kIn + i] * P[kOffW1 + i * kH + lane];
+        const float h1 = fmaxf(s, 0.f);
+        float t = P[kOffB2 + lane];
+#pragma unroll 16
+        for (int i = 0; i < kH; ++i) t += __shfl(h1, i, kWaveSize) * P[kOffW2 + i * kH + lane];
+        const float h2 = fmaxf(t, 0.f);
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) {
+            float v = h2 * P[kOffW3 + lane * kOut + o];
+#pragma unroll
+            for (int d = kWaveSize / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWaveSize);
+            if (lane == 0) Y[static_cast<int64_t>(b) * kOut + o] = v + P[kOffB3 + o];
+        }
+    }
+}
+
 // Forward + backward of up to kMlpBlock samples per workgroup (sample k of the batch is row idx[k] of X / T / C,
 // or row k when idx is null); writes the workgroup's summed parameter gradient (already divided by B) to
 // partial[blockIdx.x][kParams] and its share of the loss to loss[blockIdx.x].
@@ -402,13 +435,17 @@ __device__ __forceinline__ unsigned long long WaveSum(unsigned long long v)
 }
 
 __global__ void __launch_bounds__(kBatchSketchBlock) batch_features(const void *keys, const uint8_t *nulls, int width,
-                                                                    const int64_t *offsets, float *X)
+                                                                    const int64_t *offsets, float *X, int64_t min_rows)
 {
     __shared__ unsigned long long hk[kBatchSketchSlots];
     __shared__ unsigned int cnt[kBatchSketchSlots];
     __shared__ SketchStats st;
     const int b = blockIdx.x;
     const int64_t start = offsets[b], rows = offsets[b + 1] - offsets[b];
+    if (rows <= min_rows) {  // the caller decides such batches without the network: zero features
+        if (threadIdx.x < kIn) X[static_cast<int64_t>(b) * kIn + threadIdx.x] = 0.f;
+        return;
+    }
     const int64_t m = rows < kBatchSample ? rows : kBatchSample;
     for (int i = threadIdx.x; i < kBatchSketchSlots; i += kBatchSketchBlock) {
         hk[i] = 0;
