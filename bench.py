@@ -524,6 +524,62 @@ def cpu_baseline_stream(w, offsets, spec):
                             "sample": f"first 2000 batches ({sub[-1]} rows), warm-up + best of 3"}}
 
 
+def bench_operator(args, mode: str):
+    """C1 (mode "agg") or C2 (mode "join") end to end through the GPU operators' Operator surface: host VectorBatches
+    in, host VectorBatches out, every host<->device copy inside the timed region (omni_operator_bench, a C++ driver
+    of GpuHashAggregationOperator / GpuHashBuilderOperator + GpuLookupJoinOperator)."""
+    import subprocess
+    exe = os.path.join(ROOT, "omnioperator_amd", "lib", "omni_operator_bench")
+    if not os.path.exists(exe):
+        raise SystemExit("omni_operator_bench is not built (python -c 'import __graft_entry__ as g; g.build()')")
+    r = subprocess.run([exe, mode, str(args.steps), str(args.warmup)], stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, timeout=1200)
+    if r.returncode != 0:
+        raise SystemExit(f"omni_operator_bench {mode} failed ({r.returncode}): {r.stderr.strip()} {r.stdout.strip()}")
+    d = json.loads(r.stdout.strip().splitlines()[-1])
+    ms = d["ms_per_step"]
+    if mode == "agg":
+        alg = d["rows"] * 16
+        work = "group-by through GpuHashAggregationOperator: 1M rows, int64 key over 1,000 values, int64 value, " \
+               "VectorBatches of 4096; SUM/COUNT(*)/MIN/MAX; host batches in and out, copies included"
+        cfg_extra = {"groups": d["groups"], "groups_ok": d["groups_ok"]}
+    else:
+        alg = d["probe_rows"] * 8 + int(d["matches"]) * 12
+        work = "inner join through GpuHashBuilderOperator + GpuLookupJoinOperator: build 2M unique int64 keys + int32 " \
+               "payload, probe 64M int64 keys (80% hit), VectorBatches of 65536; output probe key + build payload as " \
+               "host batches, copies included"
+        cfg_extra = {"matches": d["matches"], "matches_ok": d["matches_ok"],
+                     "build_kernel_ms_per_step": d["build_kernel_ms_per_step"]}
+    kms = d["kernel_ms_per_step"]
+    achieved = alg / (kms / 1e3) / 1e9 if kms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": d["kernel"] + " (all launches of a step, HIP events)", "kernel_ms": round(kms, 4),
+            "algorithmic_bytes": alg}
+    cfg = {"workload": work, "rows": d["rows"], "end_to_end": True,
+           "kernel_share": round(kms / ms, 4) if ms > 0 else None}
+    cfg.update(cfg_extra)
+    cpu = None
+    if not args.no_cpu:  # the oracle on the same configuration (the Python generator's rows of it)
+        from omnioperator_amd import workloads
+        if mode == "agg":
+            import oracle
+            w = workloads.agg_plain()
+            spec = oracle.make_agg_spec([oracle.LONG], [(oracle.SUM, oracle.LONG, oracle.LONG),
+                                                        (oracle.COUNT_ALL, oracle.LONG, oracle.LONG),
+                                                        (oracle.MIN, oracle.LONG, oracle.LONG),
+                                                        (oracle.MAX, oracle.LONG, oracle.LONG)])
+            vc = oracle.Col(oracle.LONG, w.values)
+            t0 = time.perf_counter()
+            oracle.hash_agg(spec, [oracle.Col(oracle.LONG, w.keys)], [vc, None, vc, vc], w.batch_rows)
+            cpu = {"value": len(w.keys) / (time.perf_counter() - t0), "unit": "rows/s", "cores": 1, "kind": "port",
+                   "sample": f"the full configuration ({len(w.keys)} rows); host CPU {cpu_model()}"}
+        else:
+            cpu = cpu_baseline_join(workloads.join_inner(), args)
+    return {"value": d["rows"] / (ms / 1e3), "ms_per_step": ms, "config": cfg, "roofline": roof,
+            "cpu_baseline": cpu, "dtype": "int64"}
+
+
 def bench_learned(args):
     """The learned advisor on the C5 task set: 512 seeded group-by and join tasks (rows log-uniform in
     [1e5, 2e8]), each run with every fixed strategy, the library's automatic rule and the advisor."""
@@ -650,7 +706,8 @@ def main():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--workload", default="batched",
-                   choices=["join", "agg", "skewed", "batched", "learned", "partition"],
+                   choices=["join", "agg", "skewed", "batched", "learned", "partition", "agg_operator",
+                            "join_operator"],
                    help="batched (default): the 20,000-batch mixed-cardinality stream, learned + batched form; join: "
                         "2M x 64M inner join; agg: 1M-row plain group-by; skewed: 200M-row composite-key group-by; "
                         "learned: the 512-task advisor set; partition: shuffle partition ids")
@@ -701,7 +758,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     res = {"join": bench_join, "agg": bench_agg_plain, "skewed": bench_agg_skewed,
-           "batched": bench_batched, "learned": bench_learned, "partition": bench_partition}[args.workload](args)
+           "batched": bench_batched, "learned": bench_learned, "partition": bench_partition,
+           "agg_operator": lambda a: bench_operator(a, "agg"),
+           "join_operator": lambda a: bench_operator(a, "join")}[args.workload](args)
     single = args.workload == "learned"  # one pass over the task set; each task warmed by an untimed run
     line = {"metric": METRIC, "value": res["value"], "unit": "rows/s", "n_gpus": 1,
             "steps": 1 if single else args.steps, "warmup": 0 if single else args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",

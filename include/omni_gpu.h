@@ -123,6 +123,10 @@ int omni_gpu_memcpy_h2d(void *dst, const void *src, size_t bytes);
 int omni_gpu_memcpy_d2h(void *dst, const void *src, size_t bytes);
 int omni_gpu_memcpy_d2d(void *dst, const void *src, size_t bytes);
 int omni_gpu_memset(void *dst, int value, size_t bytes);
+/* Page-locked host memory (hipHostMalloc), for staging many small VectorBatches into one large copy; no reference
+ * counterpart (the reference's operators never leave host memory). */
+int omni_gpu_host_alloc(void **ptr, size_t bytes);
+int omni_gpu_host_free(void *ptr);
 int omni_gpu_device_synchronize(void);
 
 /* Device memory the library holds: *in_use_bytes in live buffers of open handles, *cached_bytes in freed blocks it

@@ -3,6 +3,7 @@
   omnioperator_amd/lib/libomni_gpu.so   HIP kernels + the C ABI of include/omni_gpu.h (hipcc, gfx950)
                                         + the GPU-backed Operator classes (omnioperator_amd/host)
   omnioperator_amd/lib/omni_operator_test   C++ operator-level test driver (tests/cpp/operator_test.cpp)
+  omnioperator_amd/lib/omni_operator_bench  end-to-end operator benchmark (omnioperator_amd/drivers/operator_bench.cpp)
 
 Usage: python -m omnioperator_amd.build   (or build() from Python).  Every call recompiles.
 """
@@ -64,12 +65,13 @@ def build(verbose: bool = False) -> str:
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + [j[1] for j in jobs] +
          ["-Wl,-soname,libomni_gpu.so"])
     os.replace(tmp, lib)
-    test_src = os.path.join(ROOT, "tests", "cpp", "operator_test.cpp")
-    if os.path.exists(test_src):
-        exe = os.path.join(LIB_DIR, "omni_operator_test")
-        _run([CXX] + CXX_FLAGS + [test_src, "-o", exe, "-L", LIB_DIR, "-lomni_gpu",
-                                  f"-Wl,-rpath,{LIB_DIR}", "-Wl,-rpath,$ORIGIN",
-                                  "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,/opt/rocm/lib"])
+    # executables over the operator surface: the C++ operator tests and the end-to-end operator benchmark
+    for src, name in ((os.path.join(ROOT, "tests", "cpp", "operator_test.cpp"), "omni_operator_test"),
+                      (os.path.join(HERE, "drivers", "operator_bench.cpp"), "omni_operator_bench")):
+        if os.path.exists(src):
+            _run([CXX] + CXX_FLAGS + ["-O3", src, "-o", os.path.join(LIB_DIR, name), "-L", LIB_DIR, "-lomni_gpu",
+                                      "-Wl,-rpath,$ORIGIN", "-L/opt/rocm/lib", "-lamdhip64",
+                                      "-Wl,-rpath,/opt/rocm/lib"])
     return lib
 
 

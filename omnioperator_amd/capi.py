@@ -81,6 +81,8 @@ SIGNATURES = {
     "omni_gpu_memcpy_d2h": (_I, [_VP, _VP, _SZ]),
     "omni_gpu_memcpy_d2d": (_I, [_VP, _VP, _SZ]),
     "omni_gpu_memset": (_I, [_VP, _I, _SZ]),
+    "omni_gpu_host_alloc": (_I, [_P(_VP), _SZ]),
+    "omni_gpu_host_free": (_I, [_VP]),
     "omni_gpu_device_synchronize": (_I, []),
     "omni_gpu_memory_stats": (_I, [_P(_I64), _P(_I64)]),
     "omni_gpu_profile_enable": (_I, [_I]),

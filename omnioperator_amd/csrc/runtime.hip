@@ -497,6 +497,19 @@ int omni_gpu_memset(void *dst, int value, size_t bytes)
     return OMNI_GPU_OK;
 }
 
+int omni_gpu_host_alloc(void **ptr, size_t bytes)
+{
+    OG_CHECK(ptr != nullptr, OMNI_GPU_ERR_INVALID, "null output pointer");
+    OG_HIP(hipHostMalloc(ptr, bytes == 0 ? 1 : bytes, hipHostMallocDefault));
+    return OMNI_GPU_OK;
+}
+
+int omni_gpu_host_free(void *ptr)
+{
+    if (ptr) OG_HIP(hipHostFree(ptr));
+    return OMNI_GPU_OK;
+}
+
 int omni_gpu_device_synchronize(void)
 {
     OG_HIP(hipDeviceSynchronize());

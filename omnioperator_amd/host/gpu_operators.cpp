@@ -28,24 +28,107 @@ void EnsureInit()
     }
 }
 
-// Copies a flat vector to device memory: values into `vals`, the null bitmap into `nulls` when it has NULLs.
-omni_gpu_column Upload(const vec::BaseVector *v, int32_t rows, DeviceBuffer &vals, DeviceBuffer &nulls)
+// Storage address of row 0 of a flat vector (the slice's own row 0, UnsafeVector::GetRawValues); the storage type
+// follows the type id as VectorHelper::CreateFlatVector lays it out.
+void *FlatValues(vec::BaseVector *v)
+{
+    using vec::Vector;
+    using vec::unsafe::UnsafeVector;
+    switch (v->GetTypeId()) {
+        case type::OMNI_DOUBLE: return UnsafeVector::GetRawValues(static_cast<Vector<double> *>(v));
+        case type::OMNI_BOOLEAN: return UnsafeVector::GetRawValues(static_cast<Vector<uint8_t> *>(v));
+        case type::OMNI_SHORT: return UnsafeVector::GetRawValues(static_cast<Vector<int16_t> *>(v));
+        case type::OMNI_INT:
+        case type::OMNI_DATE32: return UnsafeVector::GetRawValues(static_cast<Vector<int32_t> *>(v));
+        case type::OMNI_DECIMAL128: return UnsafeVector::GetRawValues(static_cast<Vector<type::Decimal128> *>(v));
+        default: return UnsafeVector::GetRawValues(static_cast<Vector<int64_t> *>(v));
+    }
+}
+
+// Rows [0, rows) of a dictionary vector gathered through its ids into `out` (NULL rows zero).
+template <typename T> void GatherDictionary(vec::BaseVector *v, int32_t rows, std::vector<uint8_t> &out)
+{
+    using vec::unsafe::UnsafeDictionaryVector;
+    auto *dv = static_cast<vec::Vector<vec::DictionaryContainer<T>> *>(v);
+    const T *dict = UnsafeDictionaryVector::GetDictionary(dv);
+    const int32_t *ids = UnsafeDictionaryVector::GetIds(dv);
+    out.assign(static_cast<size_t>(rows) * sizeof(T), 0);
+    T *o = reinterpret_cast<T *>(out.data());
+    for (int32_t r = 0; r < rows; ++r) {
+        if (!v->IsNull(r)) o[r] = dict[ids[r]];
+    }
+}
+
+// Host address of rows [0, rows) of a column as flat values: a flat vector (or slice) in place, a dictionary
+// vector gathered into `tmp`; other encodings are not on the GPU path.
+const void *ColumnValues(vec::BaseVector *v, int32_t rows, std::vector<uint8_t> &tmp)
+{
+    switch (v->GetEncoding()) {
+        case vec::OMNI_FLAT: return FlatValues(v);
+        case vec::OMNI_DICTIONARY:
+            switch (v->GetTypeId()) {
+                case type::OMNI_DOUBLE: GatherDictionary<double>(v, rows, tmp); break;
+                case type::OMNI_BOOLEAN: GatherDictionary<uint8_t>(v, rows, tmp); break;
+                case type::OMNI_SHORT: GatherDictionary<int16_t>(v, rows, tmp); break;
+                case type::OMNI_INT:
+                case type::OMNI_DATE32: GatherDictionary<int32_t>(v, rows, tmp); break;
+                case type::OMNI_DECIMAL128: GatherDictionary<type::Decimal128>(v, rows, tmp); break;
+                default: GatherDictionary<int64_t>(v, rows, tmp); break;
+            }
+            return tmp.data();
+        default:
+            throw OmniException("UNSUPPORTED_ERROR",
+                                "vector encoding " + std::to_string(v->GetEncoding()) + " is not on the GPU path");
+    }
+}
+
+// Copies a column to device memory: values into `vals`, the null bitmap into `nulls` when it has NULLs.
+omni_gpu_column Upload(vec::BaseVector *v, int32_t rows, DeviceBuffer &vals, DeviceBuffer &nulls)
 {
     omni_gpu_column c{};
     c.type_id = v->GetTypeId();
-    size_t bytes = static_cast<size_t>(rows) * v->Width();
+    std::vector<uint8_t> tmp;
+    const void *src = ColumnValues(v, rows, tmp);
+    size_t bytes = static_cast<size_t>(rows) * type::TypeWidth(v->GetTypeId());
     vals.Reserve(bytes);
-    Check(omni_gpu_memcpy_h2d(vals.Get(), v->RawValues(), bytes), "upload values");
+    Check(omni_gpu_memcpy_h2d(vals.Get(), src, bytes), "upload values");
     c.values = vals.Get();
     if (v->HasNull()) {
         size_t nb = (rows + 7) / 8;
         nulls.Reserve(nb);
-        Check(omni_gpu_memcpy_h2d(nulls.Get(), v->RawNulls(), nb), "upload nulls");
+        Check(omni_gpu_memcpy_h2d(nulls.Get(), vec::unsafe::UnsafeBaseVector::GetNulls(v), nb), "upload nulls");
         c.nulls = static_cast<const uint8_t *>(nulls.Get());
     }
     return c;
 }
+
+// Copies a downloaded NULL bitmap of `rows` rows into a new flat vector and sets its NULL flag.
+void SetNullsFrom(vec::BaseVector *v, const uint8_t *bits, int32_t rows)
+{
+    uint8_t *dst = vec::unsafe::UnsafeBaseVector::GetNulls(v);
+    const size_t nb = (rows + 7) / 8;
+    std::memcpy(dst, bits, nb);
+    if (rows % 8) dst[nb - 1] &= static_cast<uint8_t>((1u << (rows % 8)) - 1);
+    bool any = false;
+    for (size_t i = 0; i < nb && !any; ++i) any = dst[i] != 0;
+    v->SetNullFlag(any);
+}
 }  // namespace
+
+HostBuffer::~HostBuffer()
+{
+    if (ptr_) omni_gpu_host_free(ptr_);
+}
+
+void HostBuffer::Reserve(size_t bytes)
+{
+    if (bytes <= bytes_ && ptr_) return;
+    if (ptr_) omni_gpu_host_free(ptr_);
+    ptr_ = nullptr;
+    bytes_ = 0;
+    Check(omni_gpu_host_alloc(&ptr_, bytes ? bytes : 1), "omni_gpu_host_alloc");
+    bytes_ = bytes ? bytes : 1;
+}
 
 DeviceBuffer::~DeviceBuffer()
 {
@@ -66,12 +149,7 @@ int32_t MaxRowCount(int32_t rowSize) { return (kMaxVecBatchBytes + rowSize - 1) 
 
 namespace {
 // Value r of a flat vector of T.
-template <typename T> T Cell(const vec::BaseVector *v, int32_t r)
-{
-    T x;
-    std::memcpy(&x, v->RawValues() + sizeof(T) * static_cast<size_t>(r), sizeof(T));
-    return x;
-}
+template <typename T> T Cell(vec::BaseVector *v, int32_t r) { return static_cast<vec::Vector<T> *>(v)->GetValue(r); }
 }  // namespace
 
 // ================================================================================================
@@ -200,34 +278,96 @@ OmniStatus GpuHashAggregationOperator::Close()
     return OMNI_STATUS_NORMAL;
 }
 
-// AddInput: group_aggregation.cpp:320-406 -- the batch is consumed and freed here.
+// AddInput: group_aggregation.cpp:320-406 -- the batch is consumed and freed here.  Its columns are appended to
+// page-locked host staging; once kFlushRows rows are staged (or at output), the run goes to the device in one copy
+// per column and one omni_gpu_agg_add_batch, so many small batches cost one launch sequence and a few large copies
+// instead of several small copies and launches each.  The groups do not depend on where the runs are cut.
 int32_t GpuHashAggregationOperator::AddInput(vec::VectorBatch *vecBatch)
 {
     hasInput_ = true;
-    int32_t rows = vecBatch->GetRowCount();
+    const int32_t rows = vecBatch->GetRowCount();
     if (rows <= 0) {
         vec::VectorHelper::FreeVecBatch(vecBatch);
         return 0;
     }
-    if (pending_) Check(omni_gpu_agg_wait(handle_), "omni_gpu_agg_wait");
-    size_t nk = groupByCols_.size(), na = aggInputCol_.size();
+    const size_t nk = groupByCols_.size(), na = aggInputCol_.size();
+    if (stage_.empty()) {
+        for (size_t i = 0; i < nk + na; ++i) stage_.emplace_back(new StagedColumn());
+        for (size_t i = 0; i < nk; ++i) stage_[i]->col = static_cast<int32_t>(groupByCols_[i]);
+        for (size_t i = 0; i < na; ++i) stage_[nk + i]->col = aggInputCol_[i];
+    }
+    if (stagedRows_ > 0 && stagedRows_ + rows > stageCap_) Flush();
+    if (rows > stageCap_ || stage_[0]->values.Bytes() == 0) {
+        stageCap_ = std::max<int64_t>(stageCap_, rows);
+        for (auto &st : stage_) {
+            if (st->col < 0) continue;
+            const int32_t w = type::TypeWidth(vecBatch->Get(st->col)->GetTypeId());
+            st->width = w;
+            st->type = vecBatch->Get(st->col)->GetTypeId();
+            st->values.Reserve(static_cast<size_t>(stageCap_) * w);
+            st->nulls.Reserve(static_cast<size_t>(stageCap_ + 7) / 8 + 8);
+            std::memset(st->nulls.Get(), 0, static_cast<size_t>(stageCap_ + 7) / 8 + 8);
+        }
+    }
+    std::vector<uint8_t> tmp;
+    for (auto &st : stage_) {
+        if (st->col < 0) continue;
+        vec::BaseVector *v = vecBatch->Get(st->col);
+        const void *src = ColumnValues(v, rows, tmp);
+        std::memcpy(static_cast<uint8_t *>(st->values.Get()) + static_cast<size_t>(stagedRows_) * st->width, src,
+                    static_cast<size_t>(rows) * st->width);
+        if (v->HasNull()) {  // the batch's bits go to staged rows [stagedRows_, stagedRows_ + rows)
+            const uint8_t *bits = vec::unsafe::UnsafeBaseVector::GetNulls(v);
+            uint8_t *dst = static_cast<uint8_t *>(st->nulls.Get());
+            for (int32_t r = 0; r < rows; ++r) {
+                if ((bits[r >> 3] >> (r & 7)) & 1) {
+                    const int64_t q = stagedRows_ + r;
+                    dst[q >> 3] |= static_cast<uint8_t>(1u << (q & 7));
+                    st->anyNull = true;
+                }
+            }
+        }
+    }
+    stagedRows_ += rows;
+    vec::VectorHelper::FreeVecBatch(vecBatch);
+    if (stagedRows_ >= kFlushRows) Flush();
+    return 0;
+}
+
+// One copy per staged column and one insert for every batch staged since the last flush.
+void GpuHashAggregationOperator::Flush()
+{
+    if (stagedRows_ == 0) return;
+    if (pending_) Check(omni_gpu_agg_wait(handle_), "omni_gpu_agg_wait");  // the last insert still reads them
+    const size_t nk = groupByCols_.size(), na = aggInputCol_.size();
     while (staging_.size() < 2 * (nk + na)) staging_.emplace_back(new DeviceBuffer());
     omni_gpu_column keys[OMNI_GPU_MAX_KEYS] = {};
     omni_gpu_column aggs[OMNI_GPU_MAX_AGGS] = {};
-    for (size_t i = 0; i < nk; ++i) {
-        keys[i] = Upload(vecBatch->Get(groupByCols_[i]), rows, *staging_[2 * i], *staging_[2 * i + 1]);
-    }
-    for (size_t i = 0; i < na; ++i) {
-        if (aggInputCol_[i] < 0) {
-            aggs[i].type_id = type::OMNI_LONG;
+    for (size_t i = 0; i < nk + na; ++i) {
+        StagedColumn &st = *stage_[i];
+        omni_gpu_column &c = i < nk ? keys[i] : aggs[i - nk];
+        if (st.col < 0) {  // COUNT(*): no input column
+            c.type_id = type::OMNI_LONG;
             continue;
         }
-        aggs[i] = Upload(vecBatch->Get(aggInputCol_[i]), rows, *staging_[2 * (nk + i)], *staging_[2 * (nk + i) + 1]);
+        DeviceBuffer &dv = *staging_[2 * i], &dn = *staging_[2 * i + 1];
+        const size_t bytes = static_cast<size_t>(stagedRows_) * st.width;
+        dv.Reserve(bytes);
+        Check(omni_gpu_memcpy_h2d(dv.Get(), st.values.Get(), bytes), "upload values");
+        c.type_id = st.type;
+        c.values = dv.Get();
+        if (st.anyNull) {
+            const size_t nb = static_cast<size_t>(stagedRows_ + 7) / 8;
+            dn.Reserve(nb);
+            Check(omni_gpu_memcpy_h2d(dn.Get(), st.nulls.Get(), nb), "upload nulls");
+            c.nulls = static_cast<const uint8_t *>(dn.Get());
+            std::memset(st.nulls.Get(), 0, nb);
+            st.anyNull = false;
+        }
     }
-    vec::VectorHelper::FreeVecBatch(vecBatch);
-    Check(omni_gpu_agg_add_batch(handle_, keys, aggs, rows), "omni_gpu_agg_add_batch");
+    Check(omni_gpu_agg_add_batch(handle_, keys, aggs, stagedRows_), "omni_gpu_agg_add_batch");
     pending_ = true;
-    return 0;
+    stagedRows_ = 0;
 }
 
 // GetOutput: group_aggregation.cpp:454-474, 1486-1538 -- batches of ceil(1 MiB / row size) rows.
@@ -236,6 +376,7 @@ int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
     *result = nullptr;
     if (!hasInput_) return 0;
     if (groups_ < 0) {
+        Flush();
         Check(omni_gpu_agg_num_groups(handle_, &groups_), "omni_gpu_agg_num_groups");
         pending_ = false;
     }
@@ -258,11 +399,14 @@ int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
     }
     Check(omni_gpu_agg_get_output(handle_, emitted_, count, outs.data()), "omni_gpu_agg_get_output");
     std::vector<std::unique_ptr<vec::BaseVector>> gpu(ncols);
+    std::vector<uint8_t> bits((count + 7) / 8);
     for (size_t c = 0; c < ncols; ++c) {
         gpu[c].reset(vec::VectorHelper::CreateFlatVector(outTypes_[c], count));
         auto *v = gpu[c].get();
-        Check(omni_gpu_memcpy_d2h(v->RawValues(), vals[c]->Get(), static_cast<size_t>(count) * v->Width()), "download");
-        Check(omni_gpu_memcpy_d2h(v->RawNulls(), nulls[c]->Get(), (count + 7) / 8), "download nulls");
+        Check(omni_gpu_memcpy_d2h(FlatValues(v), vals[c]->Get(), static_cast<size_t>(count) * type::TypeWidth(outTypes_[c])),
+              "download");
+        Check(omni_gpu_memcpy_d2h(bits.data(), nulls[c]->Get(), bits.size()), "download nulls");
+        SetNullsFrom(v, bits.data(), count);
     }
     auto *batch = new vec::VectorBatch(count);
     const size_t nk = static_cast<size_t>(spec_.num_keys);
@@ -310,6 +454,7 @@ int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
 
 uint64_t GpuHashAggregationOperator::GetHashMapUniqueKeys()
 {
+    Flush();
     int64_t g = 0;
     Check(omni_gpu_agg_num_groups(handle_, &g), "omni_gpu_agg_num_groups");
     pending_ = false;
@@ -424,8 +569,8 @@ int32_t GpuHashBuilderOperator::GetOutput(vec::VectorBatch **result)
 GpuLookupJoinOperatorFactory *GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
     const type::DataTypes &probeTypes, int32_t *probeOutputCols, int32_t probeOutputColsCount, int32_t *probeHashCols,
     int32_t probeHashColsCount, int32_t *buildOutputCols, int32_t buildOutputColsCount,
-    const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr, void *filterExpr,
-    bool isShuffleExchangeBuildPlan, void *overflowConfig)
+    const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr, expressions::Expr *filterExpr,
+    bool isShuffleExchangeBuildPlan, OverflowConfig *overflowConfig)
 {
     if (filterExpr != nullptr) throw OmniException("UNSUPPORTED_ERROR", "join filters are not on the GPU path");
     auto *builder = reinterpret_cast<GpuHashBuilderOperatorFactory *>(hashBuilderFactoryAddr);
@@ -554,13 +699,16 @@ int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
     if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) rowSize += 1;
     int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(std::max(rowSize, 1)), matches_ - emitted_));
     auto *out = new vec::VectorBatch(count);
+    std::vector<uint8_t> tmp;
     for (int32_t c : f_.probeOutputCols_) {
-        const vec::BaseVector *src = probe_->Get(c);
+        vec::BaseVector *src = probe_->Get(c);
+        const auto *sv = static_cast<const uint8_t *>(ColumnValues(src, probe_->GetRowCount(), tmp));
         auto *v = vec::VectorHelper::CreateFlatVector(src->GetTypeId(), count);
-        int w = src->Width();
+        auto *dv = static_cast<uint8_t *>(FlatValues(v));
+        const int w = type::TypeWidth(src->GetTypeId());
         for (int32_t r = 0; r < count; ++r) {
             int64_t pr = matchRows_[emitted_ + r];
-            std::memcpy(v->RawValues() + static_cast<size_t>(w) * r, src->RawValues() + static_cast<size_t>(w) * pr, w);
+            std::memcpy(dv + static_cast<size_t>(w) * r, sv + static_cast<size_t>(w) * pr, w);
             if (src->IsNull(static_cast<int32_t>(pr))) v->SetNull(r);
         }
         out->Append(v);
@@ -568,20 +716,20 @@ int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
     const JoinType jt = f_.builder_->GetJoinType();
     if (jt == OMNI_JOIN_TYPE_EXISTENCE) {  // ConstructExistenceColumn (lookup_join.cpp:1997-2004)
         auto *v = vec::VectorHelper::CreateFlatVector(type::OMNI_BOOLEAN, count);
-        std::memcpy(v->RawValues(), exists_.data() + emitted_, static_cast<size_t>(count));
+        std::memcpy(FlatValues(v), exists_.data() + emitted_, static_cast<size_t>(count));
         out->Append(v);
     } else {
         for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
             int32_t tid = btypes.GetType(f_.buildOutputCols_[i])->GetId();
             auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
-            int w = v->Width();
+            const int w = type::TypeWidth(tid);
             if (jt == OMNI_JOIN_TYPE_LEFT_ANTI) {  // AppendRow(probePosition, nullptr, 0): build columns NULL
-                std::memset(v->RawValues(), 0, static_cast<size_t>(w) * count);
+                std::memset(FlatValues(v), 0, static_cast<size_t>(w) * count);
                 for (int32_t r = 0; r < count; ++r) v->SetNull(r);
                 out->Append(v);
                 continue;
             }
-            std::memcpy(v->RawValues(), matchPay_[i].data() + static_cast<size_t>(w) * emitted_,
+            std::memcpy(FlatValues(v), matchPay_[i].data() + static_cast<size_t>(w) * emitted_,
                         static_cast<size_t>(w) * count);
             for (int32_t r = 0; r < count; ++r) {
                 int64_t m = emitted_ + r;
@@ -683,15 +831,17 @@ int32_t GpuLookupOuterJoinOperator::GetOutput(vec::VectorBatch **result)
     int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(std::max(rowSize, 1)), rows_ - emitted_));
     auto *out = new vec::VectorBatch(count);
     for (int32_t c : f_.probeOutputCols_) {
-        auto *v = vec::VectorHelper::CreateFlatVector(f_.probeTypes_.GetType(c)->GetId(), count);
-        std::memset(v->RawValues(), 0, static_cast<size_t>(v->Width()) * count);
+        const int32_t tid = f_.probeTypes_.GetType(c)->GetId();
+        auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
+        std::memset(FlatValues(v), 0, static_cast<size_t>(type::TypeWidth(tid)) * count);
         for (int32_t r = 0; r < count; ++r) v->SetNull(r);
         out->Append(v);
     }
     for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
-        auto *v = vec::VectorHelper::CreateFlatVector(btypes.GetType(f_.buildOutputCols_[i])->GetId(), count);
-        int w = v->Width();
-        std::memcpy(v->RawValues(), pay_[i].data() + static_cast<size_t>(w) * emitted_, static_cast<size_t>(w) * count);
+        const int32_t tid = btypes.GetType(f_.buildOutputCols_[i])->GetId();
+        auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
+        const int w = type::TypeWidth(tid);
+        std::memcpy(FlatValues(v), pay_[i].data() + static_cast<size_t>(w) * emitted_, static_cast<size_t>(w) * count);
         for (int32_t r = 0; r < count; ++r) {
             int64_t m = emitted_ + r;
             if ((payNulls_[i][m >> 3] >> (m & 7)) & 1) v->SetNull(r);

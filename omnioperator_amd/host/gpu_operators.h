@@ -20,7 +20,11 @@
 #include "omni_vector.h"
 
 namespace omniruntime {
+namespace expressions {
+class Expr;  // core/src/expression/expressions.h:78 (join filters are not on the GPU path)
+}
 namespace op {
+class OverflowConfig;  // core/src/operator/config/operator_config.h:174
 
 // Owning device buffer allocated through the C ABI.
 class DeviceBuffer {
@@ -31,6 +35,22 @@ public:
     DeviceBuffer &operator=(const DeviceBuffer &) = delete;
     void Reserve(size_t bytes);  // contents not kept
     void *Get() const { return ptr_; }
+
+private:
+    void *ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Owning page-locked host buffer allocated through the C ABI.
+class HostBuffer {
+public:
+    HostBuffer() = default;
+    ~HostBuffer();
+    HostBuffer(const HostBuffer &) = delete;
+    HostBuffer &operator=(const HostBuffer &) = delete;
+    void Reserve(size_t bytes);  // contents not kept
+    void *Get() const { return ptr_; }
+    size_t Bytes() const { return bytes_; }
 
 private:
     void *ptr_ = nullptr;
@@ -89,7 +109,18 @@ public:
     OmniStatus Close() override;
     uint64_t GetHashMapUniqueKeys() override;
 
+    static constexpr int64_t kFlushRows = 1 << 20;  // staged rows that trigger a copy and an insert
+
 private:
+    // One input column's staged rows in page-locked host memory (col < 0: COUNT(*), nothing staged).
+    struct StagedColumn {
+        int32_t col = -1;
+        int32_t type = 0;
+        int32_t width = 0;
+        bool anyNull = false;
+        HostBuffer values, nulls;
+    };
+    void Flush();
     std::vector<uint32_t> groupByCols_;
     std::vector<int32_t> aggInputCol_;
     std::vector<int32_t> outTypes_;  // GPU output columns: keys, then one per GPU aggregate
@@ -102,6 +133,9 @@ private:
     int64_t emitted_ = 0;
     int32_t rowsPerBatch_ = 0;
     std::vector<std::unique_ptr<DeviceBuffer>> staging_;
+    std::vector<std::unique_ptr<StagedColumn>> stage_;  // keys, then aggregate inputs
+    int64_t stagedRows_ = 0;
+    int64_t stageCap_ = kFlushRows;
 };
 
 class GpuHashBuilderOperatorFactory : public OperatorFactory {
@@ -151,13 +185,13 @@ private:
 
 class GpuLookupJoinOperatorFactory : public OperatorFactory {
 public:
-    // filterExpr must be null (join filters are not on the GPU path); isShuffle and overflowConfig are
-    // accepted for signature compatibility.
+    // The signature of lookup_join.h:148-152.  filterExpr must be null (join filters are not on the GPU path);
+    // isShuffleExchangeBuildPlan and overflowConfig are accepted and unused.
     static GpuLookupJoinOperatorFactory *CreateLookupJoinOperatorFactory(
         const type::DataTypes &probeTypes, int32_t *probeOutputCols, int32_t probeOutputColsCount,
         int32_t *probeHashCols, int32_t probeHashColsCount, int32_t *buildOutputCols, int32_t buildOutputColsCount,
-        const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr, void *filterExpr,
-        bool isShuffleExchangeBuildPlan, void *overflowConfig);
+        const type::DataTypes &buildOutputTypes, int64_t hashBuilderFactoryAddr, expressions::Expr *filterExpr,
+        bool isShuffleExchangeBuildPlan, OverflowConfig *overflowConfig);
     Operator *CreateOperator() override;
 
 private:

@@ -3,8 +3,11 @@
 // core/src/vector/nulls_buffer.h:16-117, core/src/type/data_type.h:40-65, core/src/type/decimal128.h:21-148,
 // core/src/operator/operator.h:22-264, core/src/operator/operator_factory.h:12-27, core/src/operator/status.h:9-13).
 //
-// Only the subset the hash-aggregation / hash-join path needs: flat vectors of fixed-width types with an
-// LSB-first null bitmap (bit = 1 means NULL), batches of such vectors, and the push/pull operator API.
+// Only the subset the hash-aggregation / hash-join path needs, under the reference's names: flat and dictionary
+// vectors of fixed-width types with slices (an offset into shared buffers), an LSB-first null bitmap (bit = 1 means
+// NULL), the unsafe raw accessors (vector/unsafe_vector.h:17-90), batches of vectors, and the push/pull operator
+// API.  The reference's own headers cannot be compiled in this image (vector.h includes jemalloc), so this subset
+// stands in for them; the GPU operators use only accessors the reference's headers have under the same names.
 #pragma once
 
 #include <algorithm>
@@ -106,56 +109,174 @@ inline int32_t TypeWidth(int32_t id)
 namespace vec {
 using type::DataTypeId;
 
-enum Encoding { OMNI_FLAT = 0, OMNI_DICTIONARY = 1 };
+// vector.h:34-41
+enum Encoding { OMNI_FLAT = 0, OMNI_DICTIONARY = 1, OMNI_ENCODING_CONTAINER = 2 };
 
-// A flat vector: contiguous fixed-width values plus an LSB-first null bitmap.
+class BaseVector;
+template <typename T> class Vector;
+template <typename T> class DictionaryContainer;
+namespace unsafe {
+class UnsafeBaseVector;
+class UnsafeVector;
+class UnsafeDictionaryVector;
+}  // namespace unsafe
+
+// LSB-first NULL bitmap (bit = 1 is NULL) of `size` rows (nulls_buffer.h:16-80).  A slice shares its parent's
+// bitmap when the slice starts on a byte boundary and copies its bits otherwise, so GetNulls() always addresses
+// the slice's own row 0.
+class NullsBuffer {
+public:
+    explicit NullsBuffer(int32_t size, NullsBuffer *input = nullptr, int32_t sliceOffset = 0) : size_(size)
+    {
+        if (input != nullptr && sliceOffset % 8 == 0) {
+            buf_ = input->buf_;
+            nulls_ = input->nulls_ + sliceOffset / 8;
+            hasNull_ = input->hasNull_;
+            return;
+        }
+        buf_ = std::make_shared<std::vector<uint8_t>>(static_cast<size_t>((size + 7) / 8 + 8), 0);
+        nulls_ = buf_->data();
+        if (input != nullptr) {
+            for (int32_t i = 0; i < size; ++i) SetNull(i, input->IsNull(sliceOffset + i));
+        }
+    }
+    bool IsNull(int32_t i) const { return (nulls_[i >> 3] >> (i & 7)) & 1; }
+    void SetNull(int32_t i, bool isNull)
+    {
+        if (isNull) {
+            nulls_[i >> 3] |= static_cast<uint8_t>(1u << (i & 7));
+            hasNull_ = true;
+        } else {
+            nulls_[i >> 3] &= static_cast<uint8_t>(~(1u << (i & 7)));
+        }
+    }
+    bool HasNull() const { return hasNull_; }
+    void SetNullFlag(bool hasNull) { hasNull_ = hasNull; }
+    uint8_t *GetNulls() const { return nulls_; }
+
+private:
+    int32_t size_;
+    std::shared_ptr<std::vector<uint8_t>> buf_;
+    uint8_t *nulls_ = nullptr;
+    bool hasNull_ = false;
+};
+
+// vector.h:45-200: size, encoding, type, slice offset and the NULL bitmap.
 class BaseVector {
 public:
-    BaseVector(DataTypeId id, int32_t size)
-        : id_(id), size_(size), width_(type::TypeWidth(id)), values_(static_cast<size_t>(size) * type::TypeWidth(id)),
-          nulls_((size + 7) / 8 + 8, 0) {}
-    virtual ~BaseVector() = default;
-    DataTypeId GetTypeId() const { return id_; }
-    int32_t GetSize() const { return size_; }
-    Encoding GetEncoding() const { return OMNI_FLAT; }
-    bool IsNull(int32_t i) const { return (nulls_[i >> 3] >> (i & 7)) & 1; }
-    void SetNull(int32_t i) { nulls_[i >> 3] |= static_cast<uint8_t>(1u << (i & 7)); }
-    void SetNotNull(int32_t i) { nulls_[i >> 3] &= static_cast<uint8_t>(~(1u << (i & 7))); }
-    bool HasNull() const
+    BaseVector(int32_t size, Encoding encoding, DataTypeId typeId, NullsBuffer *nulls = nullptr, int32_t sliceOffset = 0)
+        : size(size), encoding(encoding), dataTypeId(typeId),
+          nullsBuffer(std::make_shared<NullsBuffer>(size, nulls, sliceOffset))
     {
-        for (int32_t i = 0; i < size_; ++i)
-            if (IsNull(i)) return true;
-        return false;
     }
-    uint8_t *RawValues() { return values_.data(); }
-    const uint8_t *RawValues() const { return values_.data(); }
-    uint8_t *RawNulls() { return nulls_.data(); }
-    const uint8_t *RawNulls() const { return nulls_.data(); }
-    int32_t Width() const { return width_; }
+    virtual ~BaseVector() = default;
+    DataTypeId GetTypeId() const { return dataTypeId; }
+    int32_t GetSize() const { return size; }
+    Encoding GetEncoding() const { return encoding; }
+    int32_t GetOffset() const { return offset; }
+    bool IsNull(int32_t i) const { return nullsBuffer->IsNull(i); }
+    void SetNull(int32_t i) { nullsBuffer->SetNull(i, true); }
+    void SetNotNull(int32_t i) { nullsBuffer->SetNull(i, false); }
+    bool HasNull() const { return nullsBuffer->HasNull(); }
+    // Whether the vector may hold NULLs, for a bitmap written through UnsafeBaseVector::GetNulls (vector.h:117-120).
+    void SetNullFlag(bool hasNull) { nullsBuffer->SetNullFlag(hasNull); }
+    // A view of rows [positionOffset, positionOffset + length) sharing this vector's buffers (vector.h:152-155).
+    virtual BaseVector *Slice(int32_t positionOffset, int32_t length) = 0;
 
 protected:
-    DataTypeId id_;
-    int32_t size_;
-    int32_t width_;
-    std::vector<uint8_t> values_;
-    std::vector<uint8_t> nulls_;
+    friend class unsafe::UnsafeBaseVector;
+    int32_t size;
+    Encoding encoding;
+    DataTypeId dataTypeId;
+    int32_t offset = 0;
+    std::shared_ptr<NullsBuffer> nullsBuffer;
 };
 
+// A flat vector of T (vector.h:202-426): values[offset + i] is row i.
 template <typename T> class Vector : public BaseVector {
 public:
-    Vector(int32_t size, DataTypeId id) : BaseVector(id, size) {}
-    T GetValue(int32_t i) const
+    Vector(int32_t size, DataTypeId typeId)
+        : BaseVector(size, OMNI_FLAT, typeId), valuesBuffer(std::make_shared<std::vector<T>>(static_cast<size_t>(size)))
     {
-        T v;
-        std::memcpy(&v, values_.data() + sizeof(T) * i, sizeof(T));
-        return v;
+        values = valuesBuffer->data();
     }
+    T GetValue(int32_t i) const { return values[offset + i]; }
     void SetValue(int32_t i, T v)
     {
-        std::memcpy(values_.data() + sizeof(T) * i, &v, sizeof(T));
+        values[offset + i] = v;
         SetNotNull(i);
     }
+    BaseVector *Slice(int32_t positionOffset, int32_t length) override
+    {
+        auto *v = new Vector<T>(*this, positionOffset, length);
+        return v;
+    }
+
+private:
+    friend class unsafe::UnsafeVector;
+    Vector(const Vector<T> &parent, int32_t positionOffset, int32_t length)
+        : BaseVector(length, OMNI_FLAT, parent.dataTypeId, parent.nullsBuffer.get(), positionOffset),
+          valuesBuffer(parent.valuesBuffer), values(parent.values)
+    {
+        offset = parent.offset + positionOffset;
+    }
+    std::shared_ptr<std::vector<T>> valuesBuffer;
+    T *values = nullptr;
 };
+
+// Row i of a dictionary vector is dictionary[ids[offset + i]] (dictionary_container.h, vector.h:434-555).
+template <typename T> class DictionaryContainer {
+public:
+    DictionaryContainer(std::vector<int32_t> ids, std::shared_ptr<std::vector<T>> dictionary)
+        : ids(std::move(ids)), dictionary(std::move(dictionary))
+    {
+    }
+    std::vector<int32_t> ids;
+    std::shared_ptr<std::vector<T>> dictionary;
+};
+
+template <typename T> class Vector<DictionaryContainer<T>> : public BaseVector {
+public:
+    Vector(int32_t size, std::shared_ptr<DictionaryContainer<T>> container, DataTypeId typeId)
+        : BaseVector(size, OMNI_DICTIONARY, typeId), container(std::move(container))
+    {
+    }
+    T GetValue(int32_t i) const { return (*container->dictionary)[container->ids[offset + i]]; }
+    BaseVector *Slice(int32_t positionOffset, int32_t length) override
+    {
+        auto *v = new Vector<DictionaryContainer<T>>(length, container, dataTypeId);
+        v->offset = offset + positionOffset;
+        v->nullsBuffer = std::make_shared<NullsBuffer>(length, nullsBuffer.get(), positionOffset);
+        return v;
+    }
+
+private:
+    friend class unsafe::UnsafeDictionaryVector;
+    std::shared_ptr<DictionaryContainer<T>> container;
+};
+
+// The raw accessors the reference's operators use (unsafe_vector.h:17-90).
+namespace unsafe {
+class UnsafeBaseVector {
+public:
+    static uint8_t *GetNulls(BaseVector *vector) { return vector->nullsBuffer->GetNulls(); }
+};
+class UnsafeVector {
+public:
+    template <typename T> static T *GetRawValues(Vector<T> *vector) { return vector->values + vector->offset; }
+};
+class UnsafeDictionaryVector {
+public:
+    template <typename T> static T *GetDictionary(Vector<DictionaryContainer<T>> *vector)
+    {
+        return vector->container->dictionary->data();
+    }
+    template <typename T> static int32_t *GetIds(Vector<DictionaryContainer<T>> *vector)
+    {
+        return vector->container->ids.data() + vector->offset;
+    }
+};
+}  // namespace unsafe
 
 class VectorBatch {
 public:
@@ -173,11 +294,22 @@ private:
     std::vector<std::unique_ptr<BaseVector>> vectors_;
 };
 
+// A flat vector of the type's storage (vector_helper.h CreateFlatVector): double, a byte for BOOLEAN, int16, int32,
+// Decimal128, else int64.
 struct VectorHelper {
     static void FreeVecBatch(VectorBatch *b) { delete b; }
     static BaseVector *CreateFlatVector(int32_t typeId, int32_t size)
     {
-        return new BaseVector(static_cast<DataTypeId>(typeId), size);
+        const auto id = static_cast<DataTypeId>(typeId);
+        switch (typeId) {
+            case type::OMNI_DOUBLE: return new Vector<double>(size, id);
+            case type::OMNI_BOOLEAN: return new Vector<uint8_t>(size, id);
+            case type::OMNI_SHORT: return new Vector<int16_t>(size, id);
+            case type::OMNI_INT:
+            case type::OMNI_DATE32: return new Vector<int32_t>(size, id);
+            case type::OMNI_DECIMAL128: return new Vector<type::Decimal128>(size, id);
+            default: return new Vector<int64_t>(size, id);
+        }
     }
 };
 }  // namespace vec

@@ -4,6 +4,7 @@
 // Exit status 0 when every case passes.
 #include <algorithm>
 #include <cstdio>
+#include <map>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -36,7 +37,7 @@ static std::string Cell(const BaseVector *v, int32_t r)
         case OMNI_INT:
         case OMNI_DATE32: std::snprintf(buf, sizeof buf, "%d", static_cast<const Vector<int32_t> *>(v)->GetValue(r)); break;
         case OMNI_DOUBLE: std::snprintf(buf, sizeof buf, "%.12g", static_cast<const Vector<double> *>(v)->GetValue(r)); break;
-        case OMNI_BOOLEAN: std::snprintf(buf, sizeof buf, "%s", v->RawValues()[r] ? "true" : "false"); break;
+        case OMNI_BOOLEAN: std::snprintf(buf, sizeof buf, "%s", static_cast<const Vector<uint8_t> *>(v)->GetValue(r) ? "true" : "false"); break;
         default: std::snprintf(buf, sizeof buf, "%lld", static_cast<long long>(static_cast<const Vector<int64_t> *>(v)->GetValue(r)));
     }
     return buf;
@@ -640,6 +641,230 @@ static void TestMultiKeysFullJoinInteger()
     delete bf;
 }
 
+// A dictionary vector of int64 values: row i is dict[ids[i]]; NULL where nullAt(i).
+static BaseVector *MakeDictVector(const std::vector<int32_t> &ids, const std::vector<int64_t> &dict,
+                                  const std::function<bool(int)> &nullAt)
+{
+    auto container = std::make_shared<DictionaryContainer<int64_t>>(ids, std::make_shared<std::vector<int64_t>>(dict));
+    auto *v = new Vector<DictionaryContainer<int64_t>>(static_cast<int32_t>(ids.size()), container, OMNI_LONG);
+    for (size_t i = 0; i < ids.size(); ++i) {
+        if (nullAt(static_cast<int>(i))) v->SetNull(static_cast<int32_t>(i));
+    }
+    return v;
+}
+
+// A flat int64 vector of `n + off` rows sliced to its rows [off, off + n): row i of the slice is vals[i], NULL where
+// nullAt(i) (the slice's NULL bitmap is a bit-shifted copy unless off is a multiple of 8).
+static BaseVector *MakeSlicedVector(const std::vector<int64_t> &vals, int32_t off, const std::function<bool(int)> &nullAt)
+{
+    const int32_t n = static_cast<int32_t>(vals.size());
+    Vector<int64_t> parent(n + off + 5, OMNI_LONG);
+    for (int32_t i = 0; i < n + off + 5; ++i) parent.SetValue(i, -777);  // rows outside the slice
+    for (int32_t i = 0; i < off; ++i) parent.SetNull(i);
+    for (int32_t i = 0; i < n; ++i) {
+        parent.SetValue(off + i, vals[i]);
+        if (nullAt(i)) parent.SetNull(off + i);
+    }
+    return parent.Slice(off, n);
+}
+
+// SUM / COUNT(col) / MIN / MAX over int64 values grouped by an int64 key, as a std::map (NULL key -> "null").
+static Rows ExpectedAgg(const std::vector<int64_t> &k, const std::vector<bool> &kn, const std::vector<int64_t> &v,
+                        const std::vector<bool> &vn)
+{
+    struct Acc {
+        int64_t sum = 0, cnt = 0, mn = 0, mx = 0;
+        bool any = false;
+    };
+    std::map<std::string, Acc> m;
+    for (size_t i = 0; i < k.size(); ++i) {
+        Acc &a = m[kn[i] ? "null" : std::to_string(k[i])];
+        if (vn[i]) continue;
+        a.sum = static_cast<int64_t>(static_cast<uint64_t>(a.sum) + static_cast<uint64_t>(v[i]));
+        a.mn = a.any ? std::min(a.mn, v[i]) : v[i];
+        a.mx = a.any ? std::max(a.mx, v[i]) : v[i];
+        a.cnt += 1;
+        a.any = true;
+    }
+    Rows want;
+    for (auto &e : m) {
+        const Acc &a = e.second;
+        auto s = [&](int64_t x) { return a.any ? std::to_string(x) : std::string("null"); };
+        want.push_back({e.first, s(a.sum), std::to_string(a.cnt), s(a.mn), s(a.mx)});
+    }
+    return want;
+}
+
+static GpuHashAggregationOperatorFactory *SumCountMinMaxFactory()
+{
+    std::vector<uint32_t> groupCols = {0};
+    std::vector<std::vector<uint32_t>> aggCols = {{1}, {1}, {1}, {1}};
+    std::vector<DataTypes> inTypes(4, DataTypes({LongType()})), outTypes(4, DataTypes({LongType()}));
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM, OMNI_AGG_COUNT_COLUMN, OMNI_AGG_MIN, OMNI_AGG_MAX};
+    std::vector<uint32_t> masks;
+    return new GpuHashAggregationOperatorFactory(groupCols, DataTypes({LongType()}), aggCols, inTypes, outTypes, funcs,
+                                                 masks, std::vector<bool>(4, true), std::vector<bool>(4, false), false);
+}
+
+// Sliced key columns (byte-aligned and not) and dictionary-encoded value columns read through the reference's
+// accessors (GetRawValues / GetNulls of the slice, GetIds / GetDictionary) give the groups of their logical rows.
+static void TestAggSlicedAndDictionaryInputs()
+{
+    auto *f = SumCountMinMaxFactory();
+    Operator *op = f->CreateOperator();
+    std::vector<int64_t> allK, allV;
+    std::vector<bool> allKn, allVn;
+    const int32_t offsets[3] = {3, 8, 13};
+    for (int b = 0; b < 3; ++b) {
+        const int32_t n = 1000 + 37 * b;
+        std::vector<int64_t> k(n), dict = {-5, 7, 1LL << 40, -(1LL << 50), 0};
+        std::vector<int32_t> ids(n);
+        for (int32_t i = 0; i < n; ++i) {
+            k[i] = (i * 7 + b) % 23;
+            ids[i] = (i * 3 + b) % 5;
+        }
+        auto kNull = [b](int i) { return (i + b) % 17 == 0; };
+        auto vNull = [b](int i) { return (i + 2 * b) % 11 == 0; };
+        auto *batch = new VectorBatch(n);
+        batch->Append(MakeSlicedVector(k, offsets[b], kNull));
+        batch->Append(MakeDictVector(ids, dict, vNull));
+        for (int32_t i = 0; i < n; ++i) {
+            allK.push_back(k[i]);
+            allKn.push_back(kNull(i));
+            allV.push_back(dict[ids[i]]);
+            allVn.push_back(vNull(i));
+        }
+        op->AddInput(batch);
+    }
+    Rows got = Drain(op);
+    EXPECT_TRUE(SameRowSet(got, ExpectedAgg(allK, allKn, allV, allVn)));
+    Operator::DeleteOperator(op);
+    delete f;
+}
+
+// 300 batches of 4,097 rows (1.2M rows: the host staging flushes to the device part-way, and batch boundaries fall
+// inside NULL-bitmap bytes) give the groups of adding them one by one.
+static void TestAggStagingAcrossFlushes()
+{
+    auto *f = SumCountMinMaxFactory();
+    Operator *op = f->CreateOperator();
+    std::vector<int64_t> allK, allV;
+    std::vector<bool> allKn, allVn;
+    for (int b = 0; b < 300; ++b) {
+        const int32_t n = 4097;
+        std::vector<int64_t> k(n), v(n);
+        auto *kv = new Vector<int64_t>(n, OMNI_LONG);
+        auto *vv = new Vector<int64_t>(n, OMNI_LONG);
+        for (int32_t i = 0; i < n; ++i) {
+            const int64_t row = static_cast<int64_t>(b) * n + i;
+            k[i] = (row * 2654435761LL) % 5003;
+            v[i] = (row % 1000) - 500;
+            kv->SetValue(i, k[i]);
+            vv->SetValue(i, v[i]);
+            const bool kn = row % 1009 == 0, vn = row % 13 == 5;
+            if (kn) kv->SetNull(i);
+            if (vn) vv->SetNull(i);
+            allK.push_back(k[i]);
+            allKn.push_back(kn);
+            allV.push_back(v[i]);
+            allVn.push_back(vn);
+        }
+        auto *batch = new VectorBatch(n);
+        batch->Append(kv);
+        batch->Append(vv);
+        op->AddInput(batch);
+    }
+    Rows got = Drain(op);
+    EXPECT_TRUE(SameRowSet(got, ExpectedAgg(allK, allKn, allV, allVn)));
+    Operator::DeleteOperator(op);
+    delete f;
+}
+
+// An encoding the GPU path does not read (the AVG partial container) is refused with OmniException.
+class ContainerVector : public BaseVector {
+public:
+    explicit ContainerVector(int32_t n) : BaseVector(n, OMNI_ENCODING_CONTAINER, OMNI_LONG) {}
+    BaseVector *Slice(int32_t, int32_t) override { return nullptr; }
+};
+
+static void TestUnsupportedEncodingThrows()
+{
+    auto *f = SumCountMinMaxFactory();
+    Operator *op = f->CreateOperator();
+    auto *batch = new VectorBatch(4);
+    batch->Append(MakeVector<int64_t>(OMNI_LONG, {1, 2, 3, 4}));
+    batch->Append(new ContainerVector(4));
+    bool threw = false;
+    try {
+        op->AddInput(batch);
+    } catch (const OmniException &e) {
+        threw = e.Code() == "UNSUPPORTED_ERROR";
+    }
+    EXPECT_TRUE(threw);
+    Operator::DeleteOperator(op);
+    delete f;
+}
+
+// Inner join whose build key column is dictionary-encoded and whose probe columns are a slice (key) and a
+// dictionary vector (output), against a nested-loop join of the logical rows.
+static void TestJoinSlicedAndDictionaryInputs()
+{
+    const int32_t nb = 300, np = 1000;
+    std::vector<int32_t> bids(nb);
+    std::vector<int64_t> bdict, bpay(nb);
+    for (int i = 0; i < 60; ++i) bdict.push_back(i * 1000003LL);
+    for (int32_t i = 0; i < nb; ++i) {
+        bids[i] = (i * 7) % 60;
+        bpay[i] = 5000 + i;
+    }
+    auto bKeyNull = [](int i) { return i % 29 == 3; };
+    DataTypes buildTypes({LongType(), LongType()});
+    auto *build = new VectorBatch(nb);
+    build->Append(MakeDictVector(bids, bdict, bKeyNull));
+    build->Append(MakeVector<int64_t>(OMNI_LONG, bpay));
+    int32_t hashCols[1] = {0};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(OMNI_JOIN_TYPE_INNER, buildTypes, hashCols, 1, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+
+    std::vector<int64_t> pk(np), pdictv = {11, 22, 33};
+    std::vector<int32_t> pids(np);
+    for (int32_t i = 0; i < np; ++i) {
+        pk[i] = ((i * 13) % 80) * 1000003LL;
+        pids[i] = i % 3;
+    }
+    auto pKeyNull = [](int i) { return i % 31 == 7; };
+    auto pOutNull = [](int i) { return i % 10 == 4; };
+    DataTypes probeTypes({LongType(), LongType()});
+    auto *probe = new VectorBatch(np);
+    probe->Append(MakeSlicedVector(pk, 5, pKeyNull));
+    probe->Append(MakeDictVector(pids, pdictv, pOutNull));
+    int32_t probeOut[1] = {1}, probeHash[1] = {0}, buildOut[1] = {1};
+    DataTypes buildOutTypes({LongType()});
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        probeTypes, probeOut, 1, probeHash, 1, buildOut, 1, buildOutTypes, reinterpret_cast<int64_t>(bf), nullptr,
+        false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    Rows got = Drain(lookup);
+    Rows want;
+    for (int32_t p = 0; p < np; ++p) {
+        if (pKeyNull(p)) continue;
+        for (int32_t b = 0; b < nb; ++b) {
+            if (bKeyNull(b) || bdict[bids[b]] != pk[p]) continue;
+            want.push_back({pOutNull(p) ? "null" : std::to_string(pdictv[pids[p]]), std::to_string(bpay[b])});
+        }
+    }
+    EXPECT_TRUE(!want.empty());
+    EXPECT_TRUE(SameRowSet(got, want));
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete lf;
+    delete bf;
+}
+
 int main()
 {
     std::vector<std::pair<const char *, std::function<void()>>> tests = {
@@ -658,6 +883,10 @@ int main()
         {"AggNullKeys", TestAggNullKeys},
         {"AggOutputBatching", TestAggOutputBatching},
         {"AggDecimalOverflowThrows", TestAggDecimalOverflowThrows},
+        {"AggSlicedAndDictionaryInputs", TestAggSlicedAndDictionaryInputs},
+        {"AggStagingAcrossFlushes", TestAggStagingAcrossFlushes},
+        {"UnsupportedEncodingThrows", TestUnsupportedEncodingThrows},
+        {"JoinSlicedAndDictionaryInputs", TestJoinSlicedAndDictionaryInputs},
     };
     for (auto &t : tests) {
         int before = g_failures;
