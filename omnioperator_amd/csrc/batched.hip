@@ -55,10 +55,14 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_part(const B
     BatchPart<BLOCK>(*cfg, static_cast<int64_t>(blockIdx.x), reinterpret_cast<uint8_t *>(lds_words), sh);
 }
 
-// One workgroup per batch (batch order[blockIdx.x]) does the batch's whole group-by (BatchFused), with the
-// generic table (NA = 0) or the lite table of NA aggregates over KW-byte keys.
+// One workgroup per batch does the batch's whole group-by (BatchFused), with the generic table (NA = 0) or the lite
+// table of NA aggregates over KW-byte keys.  The grid is persistent (about the workgroups the chip holds at once):
+// each workgroup takes the next batch from the device queue head (*queue, zero at launch), in the order of
+// `order` (batches by decreasing rows), until every batch is taken.  512-lane workgroups are held to 80 VGPRs
+// (six waves per SIMD: three workgroups per CU, as many as their LDS tables allow).
 template <int BLOCK, int NA, int KW>
-__global__ void __launch_bounds__(BLOCK) batch_fused(const BatchConfig *__restrict__ cfg, const int32_t *__restrict__ order)
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(BLOCK == 512 ? 6 : 1)))
+batch_fused(const BatchConfig *__restrict__ cfg, const int32_t *__restrict__ order, unsigned int *__restrict__ queue)
 {
     extern __shared__ unsigned long long lds_words[];
     __shared__ unsigned int cnt[kFusedParts + 1];
@@ -66,34 +70,67 @@ __global__ void __launch_bounds__(BLOCK) batch_fused(const BatchConfig *__restri
     __shared__ unsigned int ccnt[kFusedParts + 1];
     __shared__ unsigned int wsum[BLOCK / kWaveSize];
     __shared__ unsigned int sh[4];
-    const BatchConfig &c = *cfg;
+    __shared__ unsigned int head;
+    const unsigned int nb = static_cast<unsigned int>(cfg->nbatches);
     uint8_t *lds = reinterpret_cast<uint8_t *>(lds_words);
-    if constexpr (NA == 0) {
-        BatchFused<BLOCK>(c, GenericTable{c, lds, LoadFusedIn(c)}, order[blockIdx.x], lds, c.L2.S, c.L2.bytes, cnt, aux,
-                          ccnt, wsum, sh);
-    } else {
-        BatchFused<BLOCK>(c, MakeLiteTable<NA, KW>(c, lds), order[blockIdx.x], lds, c.lite_smax, c.lite_bytes, cnt,
-                          aux, ccnt, wsum, sh);
+    while (true) {
+        if (threadIdx.x == 0) head = atomicAdd(queue, 1u);
+        __syncthreads();
+        const unsigned int i = head;
+        if (i >= nb) break;  // the same value in every thread: the whole workgroup leaves
+        // the configuration is read again for each batch: an index the compiler cannot see is zero keeps it from
+        // holding every field in registers across batches
+        unsigned int zero;
+        __asm__ volatile("s_mov_b32 %0, 0" : "=s"(zero));
+        const BatchConfig &c = cfg[zero];
+        if constexpr (NA == 0) {
+            BatchFused<BLOCK>(c, GenericTable{c, lds, LoadFusedIn(c)}, order[i], lds, c.L2.S, c.L2.bytes, cnt, aux,
+                              ccnt, wsum, sh);
+        } else {
+            BatchFused<BLOCK>(c, MakeLiteTable<NA, KW>(c, lds), order[i], lds, c.lite_smax, c.lite_bytes, cnt, aux,
+                              ccnt, wsum, sh);
+        }
+        __syncthreads();  // the batch's LDS and every read of `head` are done before the next batch
     }
 }
+
+typedef void (*FusedKernelFn)(const BatchConfig *, const int32_t *, unsigned int *);
 
 template <int BLOCK, int KW>
-void LaunchFusedLite(const BatchConfig &hc, const BatchConfig *dc, const int32_t *order, int nb, hipStream_t s)
+FusedKernelFn FusedLiteKernel(int na)
 {
-    switch (hc.lite_na) {
-        case 1: batch_fused<BLOCK, 1, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        case 2: batch_fused<BLOCK, 2, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        case 3: batch_fused<BLOCK, 3, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
-        default: batch_fused<BLOCK, 4, KW><<<nb, BLOCK, hc.lite_bytes, s>>>(dc, order); break;
+    switch (na) {
+        case 1: return batch_fused<BLOCK, 1, KW>;
+        case 2: return batch_fused<BLOCK, 2, KW>;
+        case 3: return batch_fused<BLOCK, 3, KW>;
+        default: return batch_fused<BLOCK, 4, KW>;
     }
 }
 
+// The fused kernel for this configuration and its dynamic LDS bytes.
 template <int BLOCK>
-void LaunchFused(const BatchConfig &hc, const BatchConfig *dc, const int32_t *order, int nb, hipStream_t s)
+FusedKernelFn FusedKernel(const BatchConfig &hc, int *lds)
 {
-    if (hc.lite_na == 0) batch_fused<BLOCK, 0, 8><<<nb, BLOCK, hc.L2.bytes, s>>>(dc, order);
-    else if (hc.kd.width[0] == 8) LaunchFusedLite<BLOCK, 8>(hc, dc, order, nb, s);
-    else LaunchFusedLite<BLOCK, 4>(hc, dc, order, nb, s);
+    if (hc.lite_na == 0) {
+        *lds = hc.L2.bytes;
+        return batch_fused<BLOCK, 0, 8>;
+    }
+    *lds = hc.lite_bytes;
+    return hc.kd.width[0] == 8 ? FusedLiteKernel<BLOCK, 8>(hc.lite_na) : FusedLiteKernel<BLOCK, 4>(hc.lite_na);
+}
+
+// Persistent grid of the fused kernel: the workgroups the device holds at once (at most one per batch).
+int FusedGrid(const BatchConfig &hc, int block, int nb, int *grid)
+{
+    int lds = 0;
+    const FusedKernelFn k = block == 512 ? FusedKernel<512>(hc, &lds) : FusedKernel<256>(hc, &lds);
+    int per_cu = 0, dev = 0, cus = 0;
+    OG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void *>(k), block,
+                                                        static_cast<size_t>(lds)));
+    OG_HIP(hipGetDevice(&dev));
+    OG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    *grid = std::max(1, std::min(nb, std::max(1, per_cu) * std::max(1, cus)));
+    return OMNI_GPU_OK;
 }
 
 // Phase 3 launches are grid-stride over their tiles / batches (grid capped at kPhase3Grid) and return at once
@@ -123,10 +160,14 @@ template <int BLOCK> __global__ void __launch_bounds__(BLOCK) batch_hbm_emit(con
     for (int b = blockIdx.x; b < count; b += gridDim.x) BatchHbmEmit<BLOCK>(*cfg, first + b, sh);
 }
 
-__global__ void __launch_bounds__(kScanBlock) batch_scan(const BatchConfig *__restrict__ cfg)
+// HBM-table region offsets of the batches flagged for phase 2 / 3.  only_redo: the regions are needed only when a
+// batch was flagged for the exact HBM pass (the one-workgroup-per-batch form), so the kernel returns at once
+// otherwise.
+__global__ void __launch_bounds__(kScanBlock) batch_scan(const BatchConfig *__restrict__ cfg, int only_redo)
 {
-    __shared__ unsigned long long partial[kScanBlock];
-    ScanRegions<kScanBlock>(*cfg, partial);
+    __shared__ unsigned long long wsum[kScanBlock / kWaveSize];
+    if (only_redo && *cfg->redo == 0u) return;
+    ScanRegions<kScanBlock>(*cfg, wsum);
 }
 
 __global__ void batch_clear(unsigned int *words, int64_t nwords)
@@ -151,6 +192,7 @@ struct BatchesHandle {
     DevBuf sketch_modes;              // per-batch first phase from the sketch launch or the advisor
     DevBuf perm_keys, perm_vals;      // partition-ordered rows of the one-workgroup-per-batch form
     bool fused = false;               // this run takes the one-workgroup-per-batch form (flag bit 4)
+    int fused_grid = 1;               // its persistent grid
     void *advisor = nullptr;          // MLP handle (omni_gpu_agg_batches_set_advisor)
     bool use_advisor = false;         // this run's first phases come from the advisor
     DevBuf adv_x, adv_y;              // the advisor's per-batch features and outputs
@@ -244,10 +286,14 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
                                       0.f, fit));
         }
         const int32_t *order = h.order.as<int32_t>();
-        if (h.block == 512) LaunchFused<512>(h.cfg, dc, order, nb, s);
-        else LaunchFused<256>(h.cfg, dc, order, nb, s);
+        unsigned int *queue = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>() + 8);
+        batch_clear<<<1, 64, 0, s>>>(queue, 1);  // a kernel, not a memset: the sequence may be captured
+        OG_TRY(CheckLaunch("batch_clear"));
+        int lds = 0;
+        const FusedKernelFn k = h.block == 512 ? FusedKernel<512>(h.cfg, &lds) : FusedKernel<256>(h.cfg, &lds);
+        k<<<h.fused_grid, h.block, lds, s>>>(dc, order, queue);
         OG_TRY(CheckLaunch("batch_fused"));
-        batch_scan<<<1, kScanBlock, 0, s>>>(dc);
+        batch_scan<<<1, kScanBlock, 0, s>>>(dc, 1);
         OG_TRY(CheckLaunch("batch_scan"));
     } else if (per_batch) {
         for (int b = 0; b < nb; ++b) {
@@ -269,7 +315,7 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     }
     if (!h.fused || per_batch) {
         OG_TRY(CheckLaunch("batch_lds"));
-        batch_scan<<<1, kScanBlock, 0, s>>>(dc);
+        batch_scan<<<1, kScanBlock, 0, s>>>(dc, 0);
         OG_TRY(CheckLaunch("batch_scan"));
     }
     if (!per_batch && !h.fused && h.nitems > 0) {
@@ -564,6 +610,7 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
     c.sketch_modes = sketch && !h.fused ? h.sketch_modes.as<int32_t>() : nullptr;
     h.cfg.sketch_modes = c.sketch_modes;
     c.pred = h.fused && h.use_advisor ? h.adv_y.as<float>() : nullptr;
+    if (h.fused) OG_TRY(FusedGrid(c, h.block, num_batches, &h.fused_grid));
     h.last_per_batch = (flags & 1) != 0;
     c.error_flag = reinterpret_cast<unsigned int *>(h.counters.as<uint8_t>());
     OG_TRY(CopyToDevice(h.dcfg.p, &c, sizeof(BatchConfig), s));
@@ -580,7 +627,8 @@ int omni_gpu_agg_batches_run(void *handle, const omni_gpu_column *keys, const om
                                      reinterpret_cast<uint64_t>(h.adv_x.p), reinterpret_cast<uint64_t>(h.adv_y.p),
                                      reinterpret_cast<uint64_t>(h.sketch_modes.p), reinterpret_cast<uint64_t>(keys[0].values),
                                      reinterpret_cast<uint64_t>(keys[0].nulls), h.fused ? 1ULL : 0ULL,
-                                     static_cast<uint64_t>(c.L2.bytes), static_cast<uint64_t>(c.lite_na)};
+                                     static_cast<uint64_t>(c.L2.bytes), static_cast<uint64_t>(c.lite_na),
+                                     static_cast<uint64_t>(h.fused_grid)};
         for (int k = 0; k < 1 + c.args.n; ++k) key.push_back(reinterpret_cast<uint64_t>(outs[k].nulls));
         if (!h.graph_exec || key != h.graph_key) {
             DropGraph(h);

@@ -535,31 +535,43 @@ __device__ __forceinline__ void BatchHbmEmit(const BatchConfig &c, int b, unsign
 
 // Region offsets of the flagged batches: exclusive scan of their table bytes (one workgroup of BLOCK threads).
 template <int BLOCK>
-__device__ __forceinline__ void ScanRegions(const BatchConfig &c, unsigned long long *partial)
+__device__ __forceinline__ void ScanRegions(const BatchConfig &c, unsigned long long *wsum)
 {
     const int nb = c.nbatches;
     const int per = (nb + BLOCK - 1) / BLOCK;
     const int lo = threadIdx.x * per;
     const int hi = lo + per < nb ? lo + per : nb;
     unsigned long long sum = 0;
+#pragma unroll 8
     for (int b = lo; b < hi; ++b) {
-        if (c.spilled[b]) sum += RegionBytes(c.R, c.offsets[b + 1] - c.offsets[b]);
+        const unsigned long long r = RegionBytes(c.R, c.offsets[b + 1] - c.offsets[b]);
+        sum += c.spilled[b] ? r : 0ULL;
     }
-    partial[threadIdx.x] = sum;
+    // exclusive scan of the per-thread sums: within each wave by shuffles, then over the waves' totals
+    const int lane = LaneId(), wave = WaveInBlock();
+    unsigned long long incl = sum;
+#pragma unroll
+    for (int d = 1; d < kWaveSize; d <<= 1) {
+        const unsigned long long up = __shfl_up(incl, d, kWaveSize);
+        if (lane >= d) incl += up;
+    }
+    if (lane == kWaveSize - 1) wsum[wave] = incl;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long run = 0;
-        for (int t = 0; t < BLOCK; ++t) {
-            unsigned long long v = partial[t];
-            partial[t] = run;
+        for (int w = 0; w < BLOCK / kWaveSize; ++w) {
+            const unsigned long long v = wsum[w];
+            wsum[w] = run;
             run += v;
         }
     }
     __syncthreads();
-    unsigned long long off = partial[threadIdx.x];
+    unsigned long long off = wsum[wave] + incl - sum;
+#pragma unroll 8
     for (int b = lo; b < hi; ++b) {
         c.region_off[b] = off;
-        if (c.spilled[b]) off += RegionBytes(c.R, c.offsets[b + 1] - c.offsets[b]);
+        const unsigned long long r = RegionBytes(c.R, c.offsets[b + 1] - c.offsets[b]);
+        off += c.spilled[b] ? r : 0ULL;
     }
 }
 

@@ -193,15 +193,12 @@ int LaunchBatchAdvisor(hipStream_t s, void *mlp, const void *keys, const uint8_t
                        const int64_t *offsets, int nb, float *X, float *Y, int32_t *modes, float limit, int64_t min_rows)
 {
     if (nb <= 0) return OMNI_GPU_OK;
-    batch_features<<<nb, kBatchSketchBlock, 0, s>>>(keys, nulls, width, offsets, X, min_rows);
+    // the sketch and the network's inference of each batch in one workgroup
+    const float *params = mlp ? static_cast<const MlpHandle *>(mlp)->params.as<float>() : nullptr;
+    batch_features<<<nb, kBatchSketchBlock, 0, s>>>(keys, nulls, width, offsets, X, min_rows, params,
+                                                    mlp ? Y : nullptr);
     OG_TRY(CheckLaunch("batch_features"));
-    if (!mlp) return OMNI_GPU_OK;
-    const MlpHandle &h = *static_cast<const MlpHandle *>(mlp);
-    const int waves = kMlpWaveBlock / kWaveSize;
-    const int grid = std::max(1, std::min((nb + waves - 1) / waves, 1024));
-    mlp_forward_waves<<<grid, kMlpWaveBlock, 0, s>>>(h.params.as<float>(), X, nb, Y);
-    OG_TRY(CheckLaunch("mlp_forward_waves"));
-    if (!modes) return OMNI_GPU_OK;
+    if (!mlp || !modes) return OMNI_GPU_OK;
     batch_plan<<<(nb + 255) / 256, 256, 0, s>>>(Y, nb, limit, modes);
     return CheckLaunch("batch_plan");
 }

@@ -54,36 +54,27 @@ __global__ void __launch_bounds__(kMlpFwdBlock) mlp_forward(const float *__restr
     for (int j = 0; j < kOut; ++j) Y[b * kOut + j] = y[j];
 }
 
-// Inference with one wave per sample (many samples, latency-bound otherwise): lane j computes hidden unit j of
-// both hidden layers, reading the previous layer's outputs lane by lane (v_readlane), and the four outputs are
-// reduced across the wave.  Same network and parameter layout as MlpForward.
-constexpr int kMlpWaveBlock = 256;
-
-__global__ void __launch_bounds__(kMlpWaveBlock) mlp_forward_waves(const float *__restrict__ params,
-                                                                   const float *__restrict__ X, int B,
-                                                                   float *__restrict__ Y)
+// Inference of one sample by one wave (many samples, latency-bound otherwise): lane j computes hidden unit j of
+// both hidden layers, reading the previous layer's outputs lane by lane (shuffles), and the four outputs are
+// reduced across the wave; lane 0 writes y[0..kOut).  Same network and parameter layout as MlpForward; P and x may
+// be in LDS or global memory.
+__device__ __forceinline__ void MlpWave(const float *__restrict__ P, const float *__restrict__ x, float *__restrict__ y)
 {
     static_assert(kH == kWaveSize, "one lane per hidden unit");
-    __shared__ float P[kParams];
-    for (int i = threadIdx.x; i < kParams; i += kMlpWaveBlock) P[i] = params[i];
-    __syncthreads();
     const int lane = LaneId();
-    constexpr int kWaves = kMlpWaveBlock / kWaveSize;
-    for (int b = blockIdx.x * kWaves + WaveInBlock(); b < B; b += gridDim.x * kWaves) {
-        float s = P[kOffB1 + lane];
-        for (int i = 0; i < kIn; ++i) s += X[static_cast<int64_t>(b) * kIn + i] * P[kOffW1 + i * kH + lane];
-        const float h1 = fmaxf(s, 0.f);
-        float t = P[kOffB2 + lane];
+    float s = P[kOffB1 + lane];
+    for (int i = 0; i < kIn; ++i) s += x[i] * P[kOffW1 + i * kH + lane];
+    const float h1 = fmaxf(s, 0.f);
+    float t = P[kOffB2 + lane];
 #pragma unroll 16
-        for (int i = 0; i < kH; ++i) t += __shfl(h1, i, kWaveSize) * P[kOffW2 + i * kH + lane];
-        const float h2 = fmaxf(t, 0.f);
+    for (int i = 0; i < kH; ++i) t += __shfl(h1, i, kWaveSize) * P[kOffW2 + i * kH + lane];
+    const float h2 = fmaxf(t, 0.f);
 #pragma unroll
-        for (int o = 0; o < kOut; ++o) {
-            float v = h2 * P[kOffW3 + lane * kOut + o];
+    for (int o = 0; o < kOut; ++o) {
+        float v = h2 * P[kOffW3 + lane * kOut + o];
 #pragma unroll
-            for (int d = kWaveSize / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWaveSize);
-            if (lane == 0) Y[static_cast<int64_t>(b) * kOut + o] = v + P[kOffB3 + o];
-        }
+        for (int d = kWaveSize / 2; d > 0; d >>= 1) v += __shfl_xor(v, d, kWaveSize);
+        if (lane == 0) y[o] = v + P[kOffB3 + o];
     }
 }
 
@@ -382,33 +373,51 @@ __global__ void sketch_publish(const SketchStats *st, const float *f, int64_t ro
     }
 }
 
-// The MLP's 16 input features from the sketch statistics (all roughly in [0, 1]).  One thread; shared by the
-// sketch entry point and the advisor so that training and inference see identical features.
-__device__ __forceinline__ void SketchFeatures(const SketchStats *st, int64_t rows, int64_t m, int key_bytes,
-                                               int num_keys, int is_join, float *f)
+// Feature i of the MLP's 16 input features from the sketch statistics (all roughly in [0, 1]): log2(a + 1) / den
+// or a / den.  The cases only pick a and den, so that lanes computing different features of one sketch share the
+// one logarithm and the one division.  Shared by the sketch entry point and the advisor so that training and
+// inference see identical features.
+__device__ __forceinline__ float SketchFeature(const SketchStats *st, int64_t rows, int64_t m, int key_bytes,
+                                               int num_keys, int is_join, int i)
 {
     const double md = m > 0 ? static_cast<double>(m) : 1.0;
     const double d = static_cast<double>(st->distinct), f1 = static_cast<double>(st->f1),
                  f2 = static_cast<double>(st->f2);
-    const double gee = sqrt(static_cast<double>(rows > 0 ? rows : 1) / md) * f1 + (d - f1);
-    const double chao = d + f1 * f1 / (2.0 * (f2 > 1.0 ? f2 : 1.0));
-    const double range = st->kmax >= st->kmin ? static_cast<double>(st->kmax) - static_cast<double>(st->kmin) : 0.0;
-    f[0] = static_cast<float>(log2(static_cast<double>(rows) + 1.0) / 32.0);
-    f[1] = static_cast<float>(log2(md + 1.0) / 32.0);
-    f[2] = static_cast<float>(log2(d + 1.0) / 32.0);
-    f[3] = static_cast<float>(f1 / md);
-    f[4] = static_cast<float>(f2 / md);
-    f[5] = static_cast<float>(static_cast<double>(st->maxc) / md);
-    f[6] = static_cast<float>(static_cast<double>(st->heavy) / md);
-    f[7] = static_cast<float>(d / md);
-    f[8] = static_cast<float>(log2(gee + 1.0) / 32.0);
-    f[9] = static_cast<float>(log2(chao + 1.0) / 32.0);
-    f[10] = static_cast<float>(log2(range + 1.0) / 64.0);
-    f[11] = static_cast<float>(static_cast<double>(st->nulls) / md);
-    f[12] = static_cast<float>(key_bytes / 16.0);
-    f[13] = static_cast<float>(num_keys / 4.0);
-    f[14] = is_join ? 1.f : 0.f;
-    f[15] = 1.f;
+    double a = 1.0, den = 1.0;
+    bool lg = false;
+    switch (i) {
+        case 0: a = static_cast<double>(rows), den = 32.0, lg = true; break;
+        case 1: a = md, den = 32.0, lg = true; break;
+        case 2: a = d, den = 32.0, lg = true; break;
+        case 3: a = f1, den = md; break;
+        case 4: a = f2, den = md; break;
+        case 5: a = static_cast<double>(st->maxc), den = md; break;
+        case 6: a = static_cast<double>(st->heavy), den = md; break;
+        case 7: a = d, den = md; break;
+        case 8:  // GEE estimate of the distinct keys
+            a = sqrt(static_cast<double>(rows > 0 ? rows : 1) / md) * f1 + (d - f1), den = 32.0, lg = true;
+            break;
+        case 9:  // Chao1 estimate
+            a = d + f1 * f1 / (2.0 * (f2 > 1.0 ? f2 : 1.0)), den = 32.0, lg = true;
+            break;
+        case 10:  // key range
+            a = st->kmax >= st->kmin ? static_cast<double>(st->kmax) - static_cast<double>(st->kmin) : 0.0;
+            den = 64.0, lg = true;
+            break;
+        case 11: a = static_cast<double>(st->nulls), den = md; break;
+        case 12: a = static_cast<double>(key_bytes), den = 16.0; break;
+        case 13: a = static_cast<double>(num_keys), den = 4.0; break;
+        case 14: a = is_join ? 1.0 : 0.0; break;
+        default: break;
+    }
+    return static_cast<float>((lg ? log2(a + 1.0) : a) / den);
+}
+
+// All 16 features in one thread.
+__device__ __forceinline__ void SketchFeatures(const SketchStats *st, int64_t rows, int64_t m, int key_bytes,
+                                               int num_keys, int is_join, float *f)
+{
+    for (int i = 0; i < kIn; ++i) f[i] = SketchFeature(st, rows, m, key_bytes, num_keys, is_join, i);
 }
 
 __global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, int key_bytes, int num_keys,
@@ -422,7 +431,8 @@ __global__ void sketch_features(const SketchStats *st, int64_t rows, int64_t m, 
 // One workgroup per batch: a sample of at most kBatchSample rows of the batch's key column -- every row of a small
 // batch, else 8 runs of 64 consecutive rows evenly spread over the batch (coalesced reads) -- is counted in an LDS
 // table and reduced to the same statistics and 16 features as the column sketch (so the advisor network reads
-// features of one kind), into X[b * kIn ..].
+// features of one kind), into X[b * kIn ..]; given the network's parameters, wave 0 then runs the network on them
+// into Y[b * kOut ..].  Batches of at most min_rows rows get zero features and no outputs.
 constexpr int kBatchSample = 512;
 constexpr int kBatchSketchSlots = 2 * kBatchSample;
 constexpr int kBatchSketchBlock = 256;
@@ -435,14 +445,16 @@ __device__ __forceinline__ unsigned long long WaveSum(unsigned long long v)
 }
 
 __global__ void __launch_bounds__(kBatchSketchBlock) batch_features(const void *keys, const uint8_t *nulls, int width,
-                                                                    const int64_t *offsets, float *X, int64_t min_rows)
+                                                                    const int64_t *offsets, float *X, int64_t min_rows,
+                                                                    const float *params, float *Y)
 {
     __shared__ unsigned long long hk[kBatchSketchSlots];
     __shared__ unsigned int cnt[kBatchSketchSlots];
     __shared__ SketchStats st;
+    __shared__ float xs[kIn];
     const int b = blockIdx.x;
     const int64_t start = offsets[b], rows = offsets[b + 1] - offsets[b];
-    if (rows <= min_rows) {  // the caller decides such batches without the network: zero features
+    if (rows <= min_rows) {  // the caller decides such batches without the network: zero features, no outputs
         if (threadIdx.x < kIn) X[static_cast<int64_t>(b) * kIn + threadIdx.x] = 0.f;
         return;
     }
@@ -526,7 +538,11 @@ __global__ void __launch_bounds__(kBatchSketchBlock) batch_features(const void *
         atomicMax(&st.maxc, mx);
     }
     __syncthreads();
-    if (threadIdx.x == 0) SketchFeatures(&st, rows, m, width, 1, 0, X + static_cast<int64_t>(b) * kIn);
+    if (threadIdx.x < kIn) xs[threadIdx.x] = SketchFeature(&st, rows, m, width, 1, 0, threadIdx.x);
+    __syncthreads();
+    if (threadIdx.x < kIn) X[static_cast<int64_t>(b) * kIn + threadIdx.x] = xs[threadIdx.x];
+    // the network's outputs for this batch (wave 0), when the caller passes its parameters
+    if (Y && threadIdx.x < kWaveSize) MlpWave(params, xs, Y + static_cast<int64_t>(b) * kOut);
 }
 
 // Per-batch first phase from the network's outputs Y[b][4]: the predicted group count 2^(32 y0) - 1 against

@@ -113,7 +113,8 @@ struct DevBuf {
 // [offsets[b], offsets[b + 1]), device offsets) into X[nb][16]; when `mlp` (an omni_gpu_mlp_create handle) is given,
 // the network's outputs into Y[nb][4] and, when `modes` is given, each batch's first phase (0 = per-batch LDS
 // table when at most `limit` groups are predicted, else 2 = hash-partitioned passes).  Batches of at most
-// `min_rows` rows get zero features (their callers decide them without the network).  Kernels only (no memset,
+// `min_rows` rows get zero features and no outputs (their callers decide them without the network).  The sketch
+// and the network's inference of a batch run in one workgroup of one kernel.  Kernels only (no memset,
 // copy or synchronisation): capturable in a graph.
 int LaunchBatchAdvisor(hipStream_t s, void *mlp, const void *keys, const uint8_t *nulls, int width,
                        const int64_t *offsets, int nb, float *X, float *Y, int32_t *modes, float limit,
