@@ -108,14 +108,19 @@ def bench_join(args):
     for _ in range(args.warmup):
         m = step()
     capi.synchronize()
-    capi.profile_reset()
-    capi.profile_enable(True)
-    capi.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         m = step()
     capi.synchronize()
     t1 = time.perf_counter()
+    # stage durations from HIP events in separate steps: the events around each stage's launches hold the
+    # stream for a few microseconds each, so the timed steps above run without them
+    prof_steps = max(1, min(args.steps, 5))
+    capi.profile_reset()
+    capi.profile_enable(True)
+    for _ in range(prof_steps):
+        step()
+    capi.synchronize()
     capi.profile_enable(False)
     launches, probe_ms = capi.profile_get("join_probe")
     _, build_ms = capi.profile_get("join_build")
@@ -132,7 +137,7 @@ def bench_join(args):
     value = rows / (ms_per_step / 1e3)
     # the probe stage of one step: the probe kernel, or the partition chain (histogram, scan, scatter,
     # persistent probe) when the table is radix-partitioned
-    avg_probe_ms = probe_ms / args.steps
+    avg_probe_ms = probe_ms / prof_steps
     # 8 B per probe key; per output row its int64 probe row id, plus the int32 payload (and its NULL bit) for
     # the types that output build columns
     out_row_bytes = 8 if args.join_type in ("semi", "anti") else 12
@@ -148,13 +153,13 @@ def bench_join(args):
                 "kernel": "join_probe" if parts[0] <= 1 else
                           f"join_probe_* (radix-partitioned, {parts[0]} partitions: hist+scan+scatter+probe)",
                 "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
-    extra = {"build_kernel_ms": round(build_ms / max(1, args.steps), 4), "matches": m, "probe_group": args.probe_group,
+    extra = {"build_kernel_ms": round(build_ms / prof_steps, 4), "matches": m, "probe_group": args.probe_group,
              "probe_partitions": parts[0], "join_type": args.join_type}
     if keeps_build:
         extra["outer_rows"] = outer[0]
-        extra["outer_kernel_ms"] = round(outer_ms / max(1, args.steps), 4)
+        extra["outer_kernel_ms"] = round(outer_ms / prof_steps, 4)
     if parts[0] > 1:
-        extra["probe_stage_ms"] = {k: round(capi.profile_get("join_probe_" + k)[1] / args.steps, 4)
+        extra["probe_stage_ms"] = {k: round(capi.profile_get("join_probe_" + k)[1] / prof_steps, 4)
                                    for k in ("hist", "scan", "scatter", "window", "part")}
     if args.join_keys != "single":
         # key translation: per probe row its key columns read, the int64 join key and its NULL bit written, and
@@ -162,10 +167,10 @@ def bench_join(args):
         kbytes = sum(capi.WIDTH[t] for t in ktypes)
         dict_bytes = 0 if args.join_keys == "int_int" else 4 + kbytes_words(ktypes) * 8
         tbytes = npr * (kbytes + 8 + dict_bytes) + npr // 8
-        tms = tprobe_ms / args.steps
+        tms = tprobe_ms / prof_steps
         extra["key_translation"] = {
             "key_types": [TYPE_NAMES[t] for t in ktypes],
-            "build_translate_ms": round(tbuild_ms / args.steps, 4), "probe_translate_ms": round(tms, 4),
+            "build_translate_ms": round(tbuild_ms / prof_steps, 4), "probe_translate_ms": round(tms, 4),
             "probe_translate_algorithmic_bytes": tbytes,
             "probe_translate_gbs": round(tbytes / (tms / 1e3) / 1e9, 1) if tms > 0 else None}
     cpu = cpu_baseline_join(w, args, jt, bs, ktypes, bkcols, pkcols) if not args.no_cpu else None
@@ -185,16 +190,22 @@ def bench_join(args):
 
 
 def timed_steps(step, args, capi):
+    """Warm-up, then args.steps timed steps, then args.steps more with the per-launch HIP events on (the events
+    hold the stream for a few microseconds each, so the timed steps run without them); returns the last step's
+    result and the timed milliseconds per step."""
     for _ in range(args.warmup):
         step()
     capi.synchronize()
-    capi.profile_reset()
-    capi.profile_enable(True)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         r = step()
     capi.synchronize()
     t1 = time.perf_counter()
+    capi.profile_reset()
+    capi.profile_enable(True)
+    for _ in range(args.steps):
+        step()
+    capi.synchronize()
     capi.profile_enable(False)
     return r, (t1 - t0) * 1e3 / args.steps
 
