@@ -163,8 +163,12 @@ typedef struct omni_gpu_agg_spec {
     /* Two-stage aggregation (outputPartial = true in the factory, group_aggregation.h:233-245): an AVG with
      * agg_partial_out[i] = 1 outputs its running double sum -- 0, not NULL, for a group whose inputs were all
      * NULL -- instead of the mean: the first column of AverageFlatIMAggregator's partial output
-     * (average_flatim_aggregator.h:256-273); a COUNT_COLUMN over the same input gives the second.  The other
-     * functions' partial output equals their final output, and the flag is ignored for them. */
+     * (average_flatim_aggregator.h:256-273); a COUNT_COLUMN over the same input gives the second.  A decimal SUM
+     * with the flag outputs the sum column of SumSparkDecimalAggregator's partial output
+     * (sum_spark_decimal_aggregator.h:246-301 with OUTPUT_PARTIAL): 0 for an empty group, NULL for an overflowed
+     * one (the per-add or the rescale overflow) and never an error whatever overflow_as_null says; the isEmpty
+     * column is a COUNT_COLUMN over the same input being 0.  The other functions' partial output equals their
+     * final output, and the flag is ignored for them. */
     int32_t agg_partial_out[OMNI_GPU_MAX_AGGS];
     /* AggHashTableRule (core/src/util/policy.h:51: NORMAL or ARRAY; group_aggregation.cpp:320-360).  Under
      * OMNI_GPU_HASH_TABLE_ARRAY a single integer key whose first inserted rows span at most 2^21 - 2 values

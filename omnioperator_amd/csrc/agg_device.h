@@ -40,7 +40,8 @@ struct AggDev {
     int kind;
     int in_width;     // bytes per input value
     int in_double;    // input column holds doubles
-    int partial_out;  // AVG: output the running sum (0 when empty) instead of the mean
+    int partial_out;  // partial output: AVG its running sum (0 when empty) instead of the mean; decimal SUM 0 when
+                      // empty and NULL (never an error) when overflowed
     int out_width;    // bytes per output value
     const void *values;
     const uint8_t *nulls;
@@ -777,15 +778,17 @@ __device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s
             break;
         }
         case K_SUM_DEC: {
-            isNull = *s.flag(i) == 0;
+            // partial output (sum_spark_decimal_aggregator.h:257-296 with OUTPUT_PARTIAL): 0 for an empty group,
+            // NULL for an overflowed one, never an error
+            isNull = *s.flag(i) == 0 && !d.partial_out;
             __int128 v = 0;
-            if (!isNull) {
+            if (*s.flag(i) != 0) {
                 bool ok = FinishDecimal(*s.template p<u64>(i, 0), *s.template p<u64>(i, 1),
                                         *s.template p<long long>(i, 2), d.dec_scale_up, d.dec_out128, v);
                 if (!ok) {
                     isNull = true;
                     v = 0;
-                    if (!overflow_as_null) atomicOr(error_flag, 1u);
+                    if (!overflow_as_null && !d.partial_out) atomicOr(error_flag, 1u);
                 }
             }
             if (d.dec_out128) {
@@ -797,10 +800,10 @@ __device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s
             break;
         }
         case K_SUM_DEC_CHK: {
-            isNull = *s.flag(i) == 0;
+            isNull = *s.flag(i) == 0 && !d.partial_out;
             __int128 v = 0;
             if constexpr (Ref::kGlobal) {
-                if (!isNull) {
+                if (*s.flag(i) != 0) {
                     bool ok = (*s.cflag(i) & kDecOvf) == 0;
                     if (ok) {
                         // not overflowed: P <= U and N <= L, so P - N fits the output's int64 / int128 range
@@ -810,7 +813,7 @@ __device__ __forceinline__ bool FinalizeAgg(const AggDev &d, int i, const Ref &s
                     if (!ok) {
                         isNull = true;
                         v = 0;
-                        if (!overflow_as_null) atomicOr(error_flag, 1u);
+                        if (!overflow_as_null && !d.partial_out) atomicOr(error_flag, 1u);
                     }
                 }
             }

@@ -14,7 +14,8 @@ inline int ClassifyAgg(const omni_gpu_agg_spec &sp, int i, AggDev &d)
     int f = sp.agg_funcs[i], ti = sp.agg_in_types[i], to = sp.agg_out_types[i];
     d.in_width = TypeWidth(ti);
     d.in_double = ti == OMNI_TYPE_DOUBLE;
-    d.partial_out = (f == OMNI_AGG_AVG && sp.agg_partial_out[i] != 0) ? 1 : 0;
+    const bool decIn = ti == OMNI_TYPE_DECIMAL64 || ti == OMNI_TYPE_DECIMAL128;
+    d.partial_out = ((f == OMNI_AGG_AVG || (f == OMNI_AGG_SUM && decIn)) && sp.agg_partial_out[i] != 0) ? 1 : 0;
     d.out_width = TypeWidth(to);
     d.dec_scale_up = 0;
     d.dec_out128 = 0;

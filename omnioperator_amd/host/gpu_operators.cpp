@@ -189,6 +189,8 @@ GpuHashAggregationOperatorFactory::GpuHashAggregationOperatorFactory(
         const int32_t out0 = outs.GetType(0)->GetId();
         if (func == OMNI_AGG_COUNT_ALL && raw) {
             plan_.push_back({kDirect, AddSlot(func, type::OMNI_LONG, type::OMNI_LONG, 0, 0, -1, false), {out0}});
+            plan_.back().func = func;
+            plan_.back().partial = partial;
             continue;
         }
         const type::DataTypes &ins = aggInputTypes.at(inputIdx);
@@ -196,6 +198,7 @@ GpuHashAggregationOperatorFactory::GpuHashAggregationOperatorFactory(
         ++inputIdx;
         const int32_t in0 = ins.GetType(0)->GetId();
         const int32_t col0 = static_cast<int32_t>(cols.at(0));
+        const size_t planned = plan_.size();
         if (func == OMNI_AGG_COUNT_ALL || func == OMNI_AGG_COUNT_COLUMN) {
             const int32_t slot = raw ? AddSlot(OMNI_AGG_COUNT_COLUMN, in0, type::OMNI_LONG, 0, 0, col0, false)
                                      : AddSlot(OMNI_AGG_SUM, type::OMNI_LONG, type::OMNI_LONG, 0, 0, col0, false);
@@ -220,18 +223,52 @@ GpuHashAggregationOperatorFactory::GpuHashAggregationOperatorFactory(
             else plan_.push_back({kAvgFinal, slot, {type::OMNI_DOUBLE}});
         } else {
             const bool decimal = in0 == type::OMNI_DECIMAL64 || in0 == type::OMNI_DECIMAL128;
+            const int32_t inScale = ins.GetType(0)->GetScale(), outScale = outs.GetType(0)->GetScale();
             if (decimal && func == OMNI_AGG_SUM && (!raw || partial)) {
-                throw OmniException("UNSUPPORTED_ERROR", "two-stage decimal SUM is not on the GPU path");
+                // SumSparkDecimalAggregator's partial state is (sum, isEmpty) (sum_spark_decimal_aggregator.h:68-72)
+                if (partial && (outs.GetSize() != 2 || outs.GetType(1)->GetId() != type::OMNI_BOOLEAN)) {
+                    throw OmniException("UNSUPPORTED_ERROR", "partial decimal SUM output must be (DECIMAL, BOOLEAN)");
+                }
+                if (!raw && (cols.size() != 2 || ins.GetSize() != 2 || ins.GetType(1)->GetId() != type::OMNI_BOOLEAN)) {
+                    throw OmniException("UNSUPPORTED_ERROR", "decimal SUM over partial input takes (DECIMAL, BOOLEAN)");
+                }
+                if (raw) {
+                    // ExtractValuesBatchInternal<OUTPUT_PARTIAL> (:246-301): the sum (0 when empty, NULL when
+                    // overflowed) and isEmpty = no non-NULL input
+                    const int32_t slot = AddSlot(func, in0, out0, inScale, outScale, col0, true);
+                    AddSlot(OMNI_AGG_COUNT_COLUMN, in0, type::OMNI_LONG, 0, 0, col0, false);
+                    plan_.push_back({kDecPartial, slot, {out0, type::OMNI_BOOLEAN}});
+                } else {
+                    // ProcessGroupInternalFinal / AddDecimalUseRowIndex (:16-66, 329-345): a NULL partial sum
+                    // marks the group overflowed for good, an empty one is skipped, the others are added with the
+                    // raw stage's checked add in row order
+                    const int32_t empty = static_cast<int32_t>(cols[1]);
+                    const int32_t slot = AddSlot(func, in0, out0, inScale, outScale, col0, partial, kNonEmpty, empty);
+                    AddSlot(OMNI_AGG_COUNT_COLUMN, in0, type::OMNI_LONG, 0, 0, col0, false, kNullMarks);
+                    if (partial) {
+                        AddSlot(OMNI_AGG_COUNT_COLUMN, in0, type::OMNI_LONG, 0, 0, col0, false, kNonEmpty, empty);
+                        plan_.push_back({kDecMerge, slot, {out0, type::OMNI_BOOLEAN}});
+                    } else {
+                        plan_.push_back({kDecFinal, slot, {out0}});
+                    }
+                }
+            } else {
+                plan_.push_back({kDirect, AddSlot(func, in0, out0, inScale, outScale, col0, false), {out0}});
             }
-            plan_.push_back({kDirect, AddSlot(func, in0, out0, ins.GetType(0)->GetScale(), outs.GetType(0)->GetScale(),
-                                              col0, false),
-                             {out0}});
+        }
+        for (size_t k = planned; k < plan_.size(); ++k) {
+            plan_[k].func = func;
+            plan_[k].col = col0;
+            plan_[k].inType = in0;
+            plan_[k].raw = raw;
+            plan_[k].partial = partial;
         }
     }
 }
 
 int32_t GpuHashAggregationOperatorFactory::AddSlot(int32_t func, int32_t inType, int32_t outType, int32_t inScale,
-                                                   int32_t outScale, int32_t col, bool partialOut)
+                                                   int32_t outScale, int32_t col, bool partialOut, InputMode mode,
+                                                   int32_t maskCol)
 {
     const int32_t n = spec_.num_aggs;
     if (n >= OMNI_GPU_MAX_AGGS) throw OmniException("UNSUPPORTED_ERROR", "too many aggregates");
@@ -242,6 +279,8 @@ int32_t GpuHashAggregationOperatorFactory::AddSlot(int32_t func, int32_t inType,
     spec_.agg_out_scales[n] = outScale;
     spec_.agg_partial_out[n] = partialOut ? 1 : 0;
     aggInputCol_.push_back(col);
+    aggInputMode_.push_back(mode);
+    aggMaskCol_.push_back(maskCol);
     spec_.num_aggs = n + 1;
     return n;
 }
@@ -251,7 +290,8 @@ OmniStatus GpuHashAggregationOperatorFactory::Init() { return OMNI_STATUS_NORMAL
 Operator *GpuHashAggregationOperatorFactory::CreateOperator() { return new GpuHashAggregationOperator(*this); }
 
 GpuHashAggregationOperator::GpuHashAggregationOperator(const GpuHashAggregationOperatorFactory &f)
-    : groupByCols_(f.groupByCols_), aggInputCol_(f.aggInputCol_), plan_(f.plan_), spec_(f.spec_)
+    : groupByCols_(f.groupByCols_), aggInputCol_(f.aggInputCol_), aggInputMode_(f.aggInputMode_),
+      aggMaskCol_(f.aggMaskCol_), plan_(f.plan_), spec_(f.spec_)
 {
     EnsureInit();
     Check(omni_gpu_agg_create(&spec_, &handle_), "omni_gpu_agg_create");
@@ -294,7 +334,11 @@ int32_t GpuHashAggregationOperator::AddInput(vec::VectorBatch *vecBatch)
     if (stage_.empty()) {
         for (size_t i = 0; i < nk + na; ++i) stage_.emplace_back(new StagedColumn());
         for (size_t i = 0; i < nk; ++i) stage_[i]->col = static_cast<int32_t>(groupByCols_[i]);
-        for (size_t i = 0; i < na; ++i) stage_[nk + i]->col = aggInputCol_[i];
+        for (size_t i = 0; i < na; ++i) {
+            stage_[nk + i]->col = aggInputCol_[i];
+            stage_[nk + i]->mode = aggInputMode_[i];
+            stage_[nk + i]->mask = aggMaskCol_[i];
+        }
     }
     if (stagedRows_ > 0 && stagedRows_ + rows > stageCap_) Flush();
     if (rows > stageCap_ || stage_[0]->values.Bytes() == 0) {
@@ -316,11 +360,30 @@ int32_t GpuHashAggregationOperator::AddInput(vec::VectorBatch *vecBatch)
         const void *src = ColumnValues(v, rows, tmp);
         std::memcpy(static_cast<uint8_t *>(st->values.Get()) + static_cast<size_t>(stagedRows_) * st->width, src,
                     static_cast<size_t>(rows) * st->width);
-        if (v->HasNull()) {  // the batch's bits go to staged rows [stagedRows_, stagedRows_ + rows)
-            const uint8_t *bits = vec::unsafe::UnsafeBaseVector::GetNulls(v);
-            uint8_t *dst = static_cast<uint8_t *>(st->nulls.Get());
+        uint8_t *dst = static_cast<uint8_t *>(st->nulls.Get());
+        if (st->mode == GpuHashAggregationOperatorFactory::kPlain) {
+            if (v->HasNull()) {  // the batch's bits go to staged rows [stagedRows_, stagedRows_ + rows)
+                const uint8_t *bits = vec::unsafe::UnsafeBaseVector::GetNulls(v);
+                for (int32_t r = 0; r < rows; ++r) {
+                    if ((bits[r >> 3] >> (r & 7)) & 1) {
+                        const int64_t q = stagedRows_ + r;
+                        dst[q >> 3] |= static_cast<uint8_t>(1u << (q & 7));
+                        st->anyNull = true;
+                    }
+                }
+            }
+        } else {
+            // a decimal partial sum column: NULL also where isEmpty is true (kNonEmpty), or NULL exactly where it
+            // is not NULL (kNullMarks); the isEmpty column's raw values are read, as AddDecimalUseRowIndex does
+            const uint8_t *bits = v->HasNull() ? vec::unsafe::UnsafeBaseVector::GetNulls(v) : nullptr;
+            std::vector<uint8_t> tmpMask;
+            const uint8_t *empty = st->mode == GpuHashAggregationOperatorFactory::kNonEmpty
+                                       ? static_cast<const uint8_t *>(ColumnValues(vecBatch->Get(st->mask), rows, tmpMask))
+                                       : nullptr;
             for (int32_t r = 0; r < rows; ++r) {
-                if ((bits[r >> 3] >> (r & 7)) & 1) {
+                const bool isNull = bits && ((bits[r >> 3] >> (r & 7)) & 1);
+                const bool out = empty ? (isNull || empty[r] != 0) : !isNull;
+                if (out) {
                     const int64_t q = stagedRows_ + r;
                     dst[q >> 3] |= static_cast<uint8_t>(1u << (q & 7));
                     st->anyNull = true;
@@ -408,7 +471,7 @@ int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
         Check(omni_gpu_memcpy_d2h(bits.data(), nulls[c]->Get(), bits.size()), "download nulls");
         SetNullsFrom(v, bits.data(), count);
     }
-    auto *batch = new vec::VectorBatch(count);
+    std::unique_ptr<vec::VectorBatch> batch(new vec::VectorBatch(count));  // freed if an overflow raises
     const size_t nk = static_cast<size_t>(spec_.num_keys);
     for (size_t c = 0; c < nk; ++c) batch->Append(gpu[c].release());
     for (const auto &p : plan_) {
@@ -444,11 +507,41 @@ int32_t GpuHashAggregationOperator::GetOutput(vec::VectorBatch **result)
                 batch->Append(o);
                 break;
             }
+            case GpuHashAggregationOperatorFactory::kDecPartial: {
+                // (sum, isEmpty): the GPU's partial sum column as it is, isEmpty = no non-NULL input
+                auto *e = new vec::Vector<uint8_t>(count, type::OMNI_BOOLEAN);
+                for (int32_t r = 0; r < count; ++r) e->SetValue(r, Cell<int64_t>(second, r) == 0 ? 1 : 0);
+                batch->Append(gpu[nk + p.slot].release());
+                batch->Append(e);
+                break;
+            }
+            case GpuHashAggregationOperatorFactory::kDecFinal:
+            case GpuHashAggregationOperatorFactory::kDecMerge: {
+                // a NULL partial sum among the group's inputs: the group is overflowed (SetNullOrThrowException in
+                // the final output, NULL in a partial one)
+                const bool merge = p.rule == GpuHashAggregationOperatorFactory::kDecMerge;
+                vec::BaseVector *third = merge ? gpu[nk + p.slot + 2].get() : nullptr;
+                vec::Vector<uint8_t> *e = merge ? new vec::Vector<uint8_t>(count, type::OMNI_BOOLEAN) : nullptr;
+                std::unique_ptr<vec::Vector<uint8_t>> eOwn(e);
+                for (int32_t r = 0; r < count; ++r) {
+                    const int64_t marks = Cell<int64_t>(second, r);
+                    if (marks > 0) {
+                        if (!merge && !spec_.overflow_as_null) {
+                            throw OmniException("OPERATOR_RUNTIME_ERROR", "decimal SUM overflow");
+                        }
+                        first->SetNull(r);
+                    }
+                    if (merge) e->SetValue(r, marks == 0 && Cell<int64_t>(third, r) == 0 ? 1 : 0);
+                }
+                batch->Append(gpu[nk + p.slot].release());
+                if (merge) batch->Append(eOwn.release());
+                break;
+            }
         }
     }
     emitted_ += count;
     if (emitted_ >= groups_) SetStatus(OMNI_STATUS_FINISHED);
-    *result = batch;
+    *result = batch.release();
     return 1;
 }
 
@@ -459,6 +552,114 @@ uint64_t GpuHashAggregationOperator::GetHashMapUniqueKeys()
     Check(omni_gpu_agg_num_groups(handle_, &g), "omni_gpu_agg_num_groups");
     pending_ = false;
     return static_cast<uint64_t>(g);
+}
+
+namespace {
+// Row r of a flat column's values (as ColumnValues returns them) widened to int64 or to double.
+int64_t IntAt(const void *vals, int32_t typeId, int32_t r)
+{
+    switch (type::TypeWidth(typeId)) {
+        case 1: return static_cast<const uint8_t *>(vals)[r];
+        case 2: return static_cast<const int16_t *>(vals)[r];
+        case 4: return static_cast<const int32_t *>(vals)[r];
+        default: return static_cast<const int64_t *>(vals)[r];
+    }
+}
+double DoubleAt(const void *vals, int32_t typeId, int32_t r)
+{
+    return typeId == type::OMNI_DOUBLE ? static_cast<const double *>(vals)[r]
+                                       : static_cast<double>(IntAt(vals, typeId, r));
+}
+// Stores an int64 (or, for DOUBLE, a double) into row r of a new flat vector of typeId.
+void StoreAt(vec::BaseVector *v, int32_t typeId, int32_t r, int64_t i, double d)
+{
+    switch (typeId) {
+        case type::OMNI_DOUBLE: static_cast<vec::Vector<double> *>(v)->SetValue(r, d); break;
+        case type::OMNI_BOOLEAN: static_cast<vec::Vector<uint8_t> *>(v)->SetValue(r, static_cast<uint8_t>(i)); break;
+        case type::OMNI_SHORT: static_cast<vec::Vector<int16_t> *>(v)->SetValue(r, static_cast<int16_t>(i)); break;
+        case type::OMNI_INT:
+        case type::OMNI_DATE32: static_cast<vec::Vector<int32_t> *>(v)->SetValue(r, static_cast<int32_t>(i)); break;
+        case type::OMNI_DECIMAL128:
+            static_cast<vec::Vector<type::Decimal128> *>(v)->SetValue(
+                r, type::Decimal128(i < 0 ? -1 : 0, static_cast<uint64_t>(i)));
+            break;
+        default: static_cast<vec::Vector<int64_t> *>(v)->SetValue(r, i); break;
+    }
+}
+}  // namespace
+
+// Each aggregate's ProcessAlignAggSchema (typed_aggregator.h:243-249): COUNT(*) 1 (count_all_aggregator.h:38-47);
+// COUNT(col) 0 / 1 (count_column_aggregator.cpp:186-205); SUM / MIN / MAX the value, NULL for a NULL row, as a
+// slice when the output type is the input's (sum_flatim_aggregator.h:259-300, max_aggregator.cpp:197-240,
+// min_aggregator.cpp); AVG (sum, count) = (value, 1), (0, 0) for a NULL row, the sum a slice when it is a DOUBLE
+// input without NULLs (average_flatim_aggregator.h:292-347); decimal SUM (sum, isEmpty) = (value, NULL row), the
+// sum a slice keeping the NULL rows when the output type is the input's, else 0 for a NULL row
+// (sum_spark_decimal_aggregator.h:496-560).
+vec::VectorBatch *GpuHashAggregationOperator::AlignSchema(vec::VectorBatch *inputVecBatch)
+{
+    const int32_t rows = inputVecBatch->GetRowCount();
+    std::unique_ptr<vec::VectorBatch> in(inputVecBatch);
+    std::unique_ptr<vec::VectorBatch> out(new vec::VectorBatch(rows));
+    for (uint32_t c : groupByCols_) out->Append(vec::VectorHelper::SliceVector(in->Get(static_cast<int32_t>(c)), 0, rows));
+    std::vector<uint8_t> tmp;
+    for (const auto &p : plan_) {
+        if (!p.raw || !p.partial) {
+            throw OmniException("UNSUPPORTED_ERROR", "AlignSchema reshapes raw input into a partial output only");
+        }
+        vec::BaseVector *v = p.col >= 0 ? in->Get(p.col) : nullptr;
+        const int32_t out0 = p.outTypes[0];
+        if (p.func == OMNI_AGG_COUNT_ALL || p.func == OMNI_AGG_COUNT_COLUMN) {
+            auto *c = new vec::Vector<int64_t>(rows, type::OMNI_LONG);
+            for (int32_t r = 0; r < rows; ++r) c->SetValue(r, v == nullptr || !v->IsNull(r) ? 1 : 0);
+            out->Append(c);
+            continue;
+        }
+        const void *vals = ColumnValues(v, rows, tmp);
+        if (p.func == OMNI_AGG_AVG) {
+            if (p.inType == type::OMNI_DOUBLE && !v->HasNull()) {
+                out->Append(vec::VectorHelper::SliceVector(v, 0, rows));
+            } else {
+                auto *sm = new vec::Vector<double>(rows, type::OMNI_DOUBLE);
+                for (int32_t r = 0; r < rows; ++r) sm->SetValue(r, v->IsNull(r) ? 0.0 : DoubleAt(vals, p.inType, r));
+                out->Append(sm);
+            }
+            auto *cn = new vec::Vector<int64_t>(rows, type::OMNI_LONG);
+            for (int32_t r = 0; r < rows; ++r) cn->SetValue(r, v->IsNull(r) ? 0 : 1);
+            out->Append(cn);
+            continue;
+        }
+        const bool decimal = p.inType == type::OMNI_DECIMAL64 || p.inType == type::OMNI_DECIMAL128;
+        if (decimal && p.func == OMNI_AGG_SUM) {
+            if (out0 == p.inType) {
+                out->Append(vec::VectorHelper::SliceVector(v, 0, rows));
+            } else {  // DECIMAL64 into a DECIMAL128 sum
+                auto *sm = vec::VectorHelper::CreateFlatVector(out0, rows);
+                for (int32_t r = 0; r < rows; ++r) {
+                    StoreAt(sm, out0, r, v->IsNull(r) ? 0 : IntAt(vals, p.inType, r), 0.0);
+                }
+                out->Append(sm);
+            }
+            auto *e = new vec::Vector<uint8_t>(rows, type::OMNI_BOOLEAN);
+            for (int32_t r = 0; r < rows; ++r) e->SetValue(r, v->IsNull(r) ? 1 : 0);
+            out->Append(e);
+            continue;
+        }
+        // SUM, MIN, MAX
+        if (out0 == p.inType) {
+            out->Append(vec::VectorHelper::SliceVector(v, 0, rows));
+            continue;
+        }
+        auto *o = vec::VectorHelper::CreateFlatVector(out0, rows);
+        for (int32_t r = 0; r < rows; ++r) {
+            if (v->IsNull(r)) {
+                o->SetNull(r);
+            } else {
+                StoreAt(o, out0, r, IntAt(vals, p.inType, r), DoubleAt(vals, p.inType, r));
+            }
+        }
+        out->Append(o);
+    }
+    return out.release();
 }
 
 // ================================================================================================

@@ -82,20 +82,40 @@ public:
         kAvgPartial = 1, // AVG with partial output: running sum (partial_out) + COUNT_COLUMN -> (sum, count)
         kAvgFinal = 2,   // AVG over partial (sum, count) input: SUM + SUM -> sum / count, NULL when count is 0
         kAvgMerge = 3,   // AVG over partial input with partial output: SUM + SUM -> (sum, count)
+        // decimal SUM (SumSparkDecimalAggregator, partial state (sum, isEmpty), sum_spark_decimal_aggregator.h):
+        kDecPartial = 4, // raw input, partial output: SUM (partial_out) + COUNT_COLUMN -> (sum, count == 0)
+        kDecFinal = 5,   // partial input, final output: SUM over the non-empty sums + COUNT of NULL sums (a NULL
+                         // partial sum marks an overflow, which sticks) -> sum, or the overflow when that count > 0
+        kDecMerge = 6,   // partial input, partial output: kDecFinal's two + COUNT of the non-empty sums
+    };
+    // How a GPU aggregate's input column is derived from the batch (decimal SUM over partial input).
+    enum InputMode {
+        kPlain = 0,      // the column as it is
+        kNonEmpty = 1,   // the sum column, NULL also where the isEmpty column is true
+        kNullMarks = 2,  // non-NULL exactly where the sum column is NULL
     };
     struct AggPlan {
         AggRule rule;
         int32_t slot;
         std::vector<int32_t> outTypes;
+        // the factory's aggregate, for AlignSchema: function, input column (-1: COUNT(*)), input type and
+        // whether its input is raw and its output partial
+        int32_t func = 0;
+        int32_t col = -1;
+        int32_t inType = 0;
+        bool raw = true;
+        bool partial = false;
     };
 
 private:
     friend class GpuHashAggregationOperator;
     int32_t AddSlot(int32_t func, int32_t inType, int32_t outType, int32_t inScale, int32_t outScale, int32_t col,
-                    bool partialOut);
+                    bool partialOut, InputMode mode = kPlain, int32_t maskCol = -1);
     std::vector<uint32_t> groupByCols_;
     type::DataTypes groupTypes_;
     std::vector<int32_t> aggInputCol_;  // per GPU aggregate: input column, -1 for COUNT_ALL
+    std::vector<int32_t> aggInputMode_; // per GPU aggregate: InputMode
+    std::vector<int32_t> aggMaskCol_;   // per GPU aggregate: the isEmpty column of kNonEmpty, else -1
     std::vector<AggPlan> plan_;         // per aggregate of the factory
     omni_gpu_agg_spec spec_{};
 };
@@ -108,6 +128,10 @@ public:
     int32_t GetOutput(vec::VectorBatch **result) override;
     OmniStatus Close() override;
     uint64_t GetHashMapUniqueKeys() override;
+    // HashAggregationOperator::AlignSchema (group_aggregation.cpp:1182-1218): a raw input batch reshaped into the
+    // partial output schema row by row (the group-by columns, then each aggregate's partial columns from that
+    // row alone), for a partial stage that passes rows through instead of aggregating them.  Frees the input.
+    vec::VectorBatch *AlignSchema(vec::VectorBatch *inputVecBatch) override;
 
     static constexpr int64_t kFlushRows = 1 << 20;  // staged rows that trigger a copy and an insert
 
@@ -115,6 +139,8 @@ private:
     // One input column's staged rows in page-locked host memory (col < 0: COUNT(*), nothing staged).
     struct StagedColumn {
         int32_t col = -1;
+        int32_t mode = 0;   // GpuHashAggregationOperatorFactory::InputMode
+        int32_t mask = -1;  // the isEmpty column of kNonEmpty
         int32_t type = 0;
         int32_t width = 0;
         bool anyNull = false;
@@ -123,6 +149,8 @@ private:
     void Flush();
     std::vector<uint32_t> groupByCols_;
     std::vector<int32_t> aggInputCol_;
+    std::vector<int32_t> aggInputMode_;
+    std::vector<int32_t> aggMaskCol_;
     std::vector<int32_t> outTypes_;  // GPU output columns: keys, then one per GPU aggregate
     std::vector<GpuHashAggregationOperatorFactory::AggPlan> plan_;
     omni_gpu_agg_spec spec_;

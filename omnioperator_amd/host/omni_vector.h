@@ -68,6 +68,7 @@ inline DataTypePtr Date32Type() { return std::make_shared<DataType>(OMNI_DATE32)
 inline DataTypePtr TimestampType() { return std::make_shared<DataType>(OMNI_TIMESTAMP); }
 inline DataTypePtr Decimal64Type(int32_t p = 18, int32_t s = 0) { return std::make_shared<DataType>(OMNI_DECIMAL64, p, s); }
 inline DataTypePtr Decimal128Type(int32_t p = 38, int32_t s = 0) { return std::make_shared<DataType>(OMNI_DECIMAL128, p, s); }
+inline DataTypePtr BooleanType() { return std::make_shared<DataType>(OMNI_BOOLEAN); }
 
 class DataTypes {
 public:
@@ -298,6 +299,8 @@ private:
 // Decimal128, else int64.
 struct VectorHelper {
     static void FreeVecBatch(VectorBatch *b) { delete b; }
+    // vector_helper.h SliceVector: a view of rows [offset, offset + length) sharing the vector's buffers
+    static BaseVector *SliceVector(BaseVector *v, int32_t offset, int32_t length) { return v->Slice(offset, length); }
     static BaseVector *CreateFlatVector(int32_t typeId, int32_t size)
     {
         const auto id = static_cast<DataTypeId>(typeId);
@@ -347,6 +350,8 @@ public:
     virtual OmniStatus Init() { return OMNI_STATUS_NORMAL; }
     virtual OmniStatus Close() { return OMNI_STATUS_NORMAL; }
     virtual uint64_t GetHashMapUniqueKeys() { return 0; }
+    // operator.h:92-95: the identity unless the operator reshapes raw input into its output schema
+    virtual vec::VectorBatch *AlignSchema(vec::VectorBatch *inputVecBatch) { return inputVecBatch; }
     static void DeleteOperator(Operator *op)
     {
         op->Close();
@@ -355,6 +360,7 @@ public:
     OmniStatus GetStatus() const { return status_; }
     void SetStatus(OmniStatus s) { status_ = s; }
     virtual void noMoreInput() { noMoreInput_ = true; }
+    virtual void setNoMoreInput(bool noMoreInput) { noMoreInput_ = noMoreInput; }
     virtual bool needsInput() { return status_ != OMNI_STATUS_FINISHED && !noMoreInput_; }
     bool isFinished() const { return status_ == OMNI_STATUS_FINISHED; }
 

@@ -485,7 +485,7 @@ inline bool MulChecked128(int128 a, int128 b, int128 &r)  // base_operations.h:3
 
 struct AggDesc {
     int32_t func, inType, outType, inScale, outScale;
-    bool partialOut = false;  // AVG: the partial output's sum column
+    bool partialOut = false;  // partial output: AVG its sum column; decimal SUM 0 when empty, NULL on overflow
     size_t offset = 0;  // state offset inside the per-group blob
     size_t size = 0;
 };
@@ -673,10 +673,12 @@ bool ExtractState(const AggDesc &a, const uint8_t *s, OutCol &o, int64_t r, bool
                 }
                 return true;
             }
+            // ExtractValuesBatchInternal<OUTPUT_PARTIAL> (sum_spark_decimal_aggregator.h:246-301): the partial
+            // output is 0 for an empty group and NULL for an overflowed one, without raising
             int width = a.outType == OMNI_TYPE_DECIMAL128 ? 16 : 8;
             std::memset(static_cast<uint8_t *>(o.values) + width * r, 0, width);
             if (flag == static_cast<uint8_t>(VState::EMPTY)) {
-                SetNullBit(o, r);
+                if (!a.partialOut) SetNullBit(o, r);
                 return true;
             }
             int128 v = a.outType == OMNI_TYPE_DECIMAL128 ? Ld<int128>(s) : static_cast<int128>(Ld<int64_t>(s));
@@ -691,7 +693,7 @@ bool ExtractState(const AggDesc &a, const uint8_t *s, OutCol &o, int64_t r, bool
             }
             if (ovf) {
                 SetNullBit(o, r);
-                return overflowAsNull;
+                return overflowAsNull || a.partialOut;
             }
             if (width == 8) {
                 // SetValToVector: static_cast<int64_t> of the rescaled int128 (:580-589), i.e. its low 64 bits
@@ -743,7 +745,9 @@ public:
         for (int i = 0; i < spec.num_aggs; ++i) {
             AggDesc a{spec.agg_funcs[i], spec.agg_in_types[i], spec.agg_out_types[i], spec.agg_in_scales[i],
                       spec.agg_out_scales[i]};
-            a.partialOut = spec.agg_funcs[i] == OMNI_AGG_AVG && spec.agg_partial_out[i] != 0;
+            a.partialOut = (spec.agg_funcs[i] == OMNI_AGG_AVG ||
+                            (spec.agg_funcs[i] == OMNI_AGG_SUM && IsDecimal(spec.agg_in_types[i]))) &&
+                           spec.agg_partial_out[i] != 0;
             a.size = StateSize(a);
             a.offset = off;
             off += a.size;

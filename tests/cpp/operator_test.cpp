@@ -38,6 +38,18 @@ static std::string Cell(const BaseVector *v, int32_t r)
         case OMNI_DATE32: std::snprintf(buf, sizeof buf, "%d", static_cast<const Vector<int32_t> *>(v)->GetValue(r)); break;
         case OMNI_DOUBLE: std::snprintf(buf, sizeof buf, "%.12g", static_cast<const Vector<double> *>(v)->GetValue(r)); break;
         case OMNI_BOOLEAN: std::snprintf(buf, sizeof buf, "%s", static_cast<const Vector<uint8_t> *>(v)->GetValue(r) ? "true" : "false"); break;
+        case OMNI_DECIMAL128: {
+            const Decimal128 d = static_cast<const Vector<Decimal128> *>(v)->GetValue(r);
+            __int128 x = static_cast<__int128>((static_cast<unsigned __int128>(static_cast<uint64_t>(d.high)) << 64) | d.low);
+            std::string out;
+            const bool neg = x < 0;
+            unsigned __int128 u = neg ? -static_cast<unsigned __int128>(x) : static_cast<unsigned __int128>(x);
+            do {
+                out.insert(out.begin(), static_cast<char>('0' + static_cast<int>(u % 10)));
+                u /= 10;
+            } while (u != 0);
+            return neg ? "-" + out : out;
+        }
         default: std::snprintf(buf, sizeof buf, "%lld", static_cast<long long>(static_cast<const Vector<int64_t> *>(v)->GetValue(r)));
     }
     return buf;
@@ -865,6 +877,239 @@ static void TestJoinSlicedAndDictionaryInputs()
     delete bf;
 }
 
+
+// ---- two-stage decimal SUM (SumSparkDecimalAggregator: partial state (sum, isEmpty)) --------------------------
+
+static Decimal128 D128(__int128 v)
+{
+    return Decimal128(static_cast<int64_t>(v >> 64), static_cast<uint64_t>(static_cast<unsigned __int128>(v)));
+}
+static const __int128 kMax38 = static_cast<__int128>(5421010862427522170LL) * (static_cast<__int128>(1) << 64) +
+                               static_cast<__int128>(687399551400673279ULL);  // 10^38 - 1
+
+// A one-key (LONG) decimal SUM operator factory: raw or partial (sum, isEmpty) input, final or partial output.
+static GpuHashAggregationOperatorFactory *DecSumFactory(bool raw, bool partial, DataTypePtr inT, DataTypePtr outT,
+                                                        bool overflowAsNull)
+{
+    std::vector<uint32_t> groupCols = {0};
+    std::vector<std::vector<uint32_t>> aggCols = {raw ? std::vector<uint32_t>{1} : std::vector<uint32_t>{1, 2}};
+    std::vector<DataTypes> inTypes = {raw ? DataTypes({inT}) : DataTypes({inT, BooleanType()})};
+    std::vector<DataTypes> outTypes = {partial ? DataTypes({outT, BooleanType()}) : DataTypes({outT})};
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM}, masks;
+    return new GpuHashAggregationOperatorFactory(groupCols, DataTypes({LongType()}), aggCols, inTypes, outTypes, funcs,
+                                                 masks, {raw}, {partial}, overflowAsNull);
+}
+
+// A batch (key, decimal sum[, isEmpty]); sumNull lists the NULL sums.
+static VectorBatch *DecBatch(const std::vector<int64_t> &keys, BaseVector *sums, const std::vector<int> &sumNull,
+                             const std::vector<uint8_t> *empty)
+{
+    auto *b = new VectorBatch(static_cast<int32_t>(keys.size()));
+    b->Append(MakeVector<int64_t>(OMNI_LONG, keys));
+    for (int r : sumNull) sums->SetNull(r);
+    b->Append(sums);
+    if (empty) b->Append(MakeVector<uint8_t>(OMNI_BOOLEAN, *empty));
+    return b;
+}
+
+static Rows RunOne(GpuHashAggregationOperatorFactory *f, std::vector<VectorBatch *> in)
+{
+    Operator *op = f->CreateOperator();
+    for (auto *b : in) op->AddInput(b);
+    Rows rows = Drain(op);
+    Operator::DeleteOperator(op);
+    delete f;
+    return rows;
+}
+
+// aggregation_test.cpp:3680-3734 (spark_sum_decimal64_normal): DECIMAL64(18, 6) into DECIMAL128(28, 6); the
+// partial output of one row of 999999999999.999999 is that value with isEmpty false, and the three rows sum to
+// 2999999999999.999997 -- in one stage, and in two (two partial operators, then a final one).
+static void TestAggDecimalTwoStageKnownAnswers()
+{
+    const int64_t v = 999999999999999999LL;
+    Rows p1 = RunOne(DecSumFactory(true, true, Decimal64Type(18, 6), Decimal128Type(28, 6), false),
+                     {DecBatch({7}, MakeVector<int64_t>(OMNI_DECIMAL64, {v}), {}, nullptr)});
+    EXPECT_TRUE(SameRowSet(p1, {{"7", "999999999999999999", "false"}}));
+    Rows one = RunOne(DecSumFactory(true, false, Decimal64Type(18, 6), Decimal128Type(28, 6), false),
+                      {DecBatch({7, 7, 7}, MakeVector<int64_t>(OMNI_DECIMAL64, {v, v, v}), {}, nullptr)});
+    EXPECT_TRUE(SameRowSet(one, {{"7", "2999999999999999997"}}));
+    Rows p2 = RunOne(DecSumFactory(true, true, Decimal64Type(18, 6), Decimal128Type(28, 6), false),
+                     {DecBatch({7, 7}, MakeVector<int64_t>(OMNI_DECIMAL64, {v, v}), {}, nullptr)});
+    EXPECT_TRUE(SameRowSet(p2, {{"7", "1999999999999999998", "false"}}));
+    const std::vector<uint8_t> no = {0, 0};
+    Rows fin = RunOne(DecSumFactory(false, false, Decimal128Type(28, 6), Decimal128Type(28, 6), false),
+                      {DecBatch({7, 7}, MakeVector<Decimal128>(OMNI_DECIMAL128, {D128(v), D128(2 * static_cast<__int128>(v))}),
+                                {}, &no)});
+    EXPECT_TRUE(SameRowSet(fin, {{"7", "2999999999999999997"}}));
+}
+
+// aggregation_test.cpp:3790-3902 (spark_sum_decimal128_overflow_*): DECIMAL128(38, 0); the partial output of
+// 10^38 - 1 is itself, and a final stage adding three such partial sums overflows: an error when overflowAsNull is
+// false, a NULL when it is true.
+static void TestAggDecimalTwoStageOverflow()
+{
+    Rows p = RunOne(DecSumFactory(true, true, Decimal128Type(38, 0), Decimal128Type(38, 0), false),
+                    {DecBatch({1}, MakeVector<Decimal128>(OMNI_DECIMAL128, {D128(kMax38)}), {}, nullptr)});
+    EXPECT_TRUE(SameRowSet(p, {{"1", "99999999999999999999999999999999999999", "false"}}));
+    const std::vector<uint8_t> no = {0, 0, 0};
+    auto batch = [&]() {
+        return DecBatch({1, 1, 1}, MakeVector<Decimal128>(OMNI_DECIMAL128, {D128(kMax38), D128(kMax38), D128(kMax38)}),
+                        {}, &no);
+    };
+    bool threw = false;
+    try {
+        RunOne(DecSumFactory(false, false, Decimal128Type(38, 0), Decimal128Type(38, 0), false), {batch()});
+    } catch (const OmniException &e) {
+        threw = e.Code() == "OPERATOR_RUNTIME_ERROR";
+    }
+    EXPECT_TRUE(threw);
+    Rows n = RunOne(DecSumFactory(false, false, Decimal128Type(38, 0), Decimal128Type(38, 0), true), {batch()});
+    EXPECT_TRUE(SameRowSet(n, {{"1", "null"}}));
+}
+
+// The partial state's rules (sum_spark_decimal_aggregator.h:16-66, 246-301): a raw partial stage outputs 0 with
+// isEmpty true for a group of NULL inputs, and NULL with isEmpty false for an overflowed group (never an error);
+// over partial input, a NULL sum overflows the group whatever follows (even with isEmpty set), empty inputs are
+// skipped, and a group of empty inputs only is NULL (final) or (0, true) (partial).
+static void TestAggDecimalPartialStateRules()
+{
+    Rows p = RunOne(DecSumFactory(true, true, Decimal64Type(18, 0), Decimal64Type(18, 0), false),
+                    {DecBatch({1, 1, 2, 2, 3}, MakeVector<int64_t>(OMNI_DECIMAL64, {INT64_MAX, 1, 0, 0, 5}), {2, 3},
+                              nullptr)});
+    EXPECT_TRUE(SameRowSet(p, {{"1", "null", "false"}, {"2", "0", "true"}, {"3", "5", "false"}}));
+    // keys: 1 NULL sum then 5; 2 empty only; 3 a sum and an empty; 4 a NULL sum flagged empty; 5 two sums
+    const std::vector<int64_t> keys = {1, 1, 2, 2, 3, 3, 4, 5, 5};
+    const std::vector<uint8_t> empty = {0, 0, 1, 1, 0, 1, 1, 0, 0};
+    auto batch = [&]() {
+        return DecBatch(keys, MakeVector<int64_t>(OMNI_DECIMAL64, {0, 5, 0, 0, 3, 0, 0, 3, 4}), {0, 6}, &empty);
+    };
+    Rows fin = RunOne(DecSumFactory(false, false, Decimal64Type(18, 0), Decimal64Type(18, 0), true), {batch()});
+    EXPECT_TRUE(SameRowSet(fin, {{"1", "null"}, {"2", "null"}, {"3", "3"}, {"4", "null"}, {"5", "7"}}));
+    Rows merge = RunOne(DecSumFactory(false, true, Decimal64Type(18, 0), Decimal64Type(18, 0), false), {batch()});
+    EXPECT_TRUE(SameRowSet(merge, {{"1", "null", "false"}, {"2", "0", "true"}, {"3", "3", "false"},
+                                   {"4", "null", "false"}, {"5", "7", "false"}}));
+    bool threw = false;
+    try {
+        RunOne(DecSumFactory(false, false, Decimal64Type(18, 0), Decimal64Type(18, 0), false), {batch()});
+    } catch (const OmniException &e) {
+        threw = e.Code() == "OPERATOR_RUNTIME_ERROR";
+    }
+    EXPECT_TRUE(threw);
+}
+
+// Two stages equal one on seeded rows that do not overflow: DECIMAL64(18, 2) into DECIMAL128(38, 2), keys over 50
+// values, 20% NULL inputs, split over three partial operators, their outputs into one final operator.
+static void TestAggDecimalTwoStageMatchesSingleStage()
+{
+    uint64_t st = 0xDEC2;
+    auto next = [&]() {
+        st += 0x9E3779B97F4A7C15ULL;
+        uint64_t z = st;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    };
+    std::vector<VectorBatch *> one, parts[3];
+    for (int b = 0; b < 6; ++b) {
+        std::vector<int64_t> k(1000), v(1000);
+        std::vector<int> nulls;
+        for (int r = 0; r < 1000; ++r) {
+            k[r] = static_cast<int64_t>(next() % 50);
+            v[r] = static_cast<int64_t>(next() % 2000000001ULL) - 1000000000LL;
+            if (next() % 5 == 0) nulls.push_back(r);
+        }
+        one.push_back(DecBatch(k, MakeVector<int64_t>(OMNI_DECIMAL64, v), nulls, nullptr));
+        parts[b % 3].push_back(DecBatch(k, MakeVector<int64_t>(OMNI_DECIMAL64, v), nulls, nullptr));
+    }
+    // a key whose inputs are all NULL: NULL in one stage, (0, true) partials and NULL again in two
+    one.push_back(DecBatch({99, 99}, MakeVector<int64_t>(OMNI_DECIMAL64, {1, 2}), {0, 1}, nullptr));
+    parts[0].push_back(DecBatch({99, 99}, MakeVector<int64_t>(OMNI_DECIMAL64, {1, 2}), {0, 1}, nullptr));
+    Rows single = RunOne(DecSumFactory(true, false, Decimal64Type(18, 2), Decimal128Type(38, 2), false), one);
+    auto *ff = DecSumFactory(false, false, Decimal128Type(38, 2), Decimal128Type(38, 2), false);
+    Operator *fin = ff->CreateOperator();
+    for (auto &ps : parts) {
+        auto *pf = DecSumFactory(true, true, Decimal64Type(18, 2), Decimal128Type(38, 2), false);
+        Operator *op = pf->CreateOperator();
+        for (auto *o : RunAgg(op, ps)) fin->AddInput(o);
+        Operator::DeleteOperator(op);
+        delete pf;
+    }
+    Rows two = Drain(fin);
+    Operator::DeleteOperator(fin);
+    delete ff;
+    EXPECT_TRUE(single.size() == 51);
+    EXPECT_TRUE(SameRowSet(single, two));
+}
+
+// HashAggregationOperator::AlignSchema (group_aggregation.cpp:1182-1218): raw rows reshaped into the partial schema
+// one by one; a final stage over aligned batches equals one stage over the raw rows (SUM / COUNT(col) / AVG / MAX
+// of LONG, COUNT(*), SUM of DECIMAL64 into DECIMAL128), and the aligned columns follow each aggregator's rule.
+static void TestAggAlignSchema()
+{
+    std::vector<uint32_t> groupCols = {0};
+    std::vector<std::vector<uint32_t>> aggCols = {{1}, {1}, {1}, {1}, {2}};
+    std::vector<DataTypes> inTypes = {DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({LongType()}),
+                                      DataTypes({LongType()}), DataTypes({Decimal64Type(18, 2)})};
+    std::vector<DataTypes> partOut = {DataTypes({LongType()}), DataTypes({LongType()}),
+                                      DataTypes({DoubleType(), LongType()}), DataTypes({LongType()}),
+                                      DataTypes({LongType()}), DataTypes({Decimal128Type(38, 2), BooleanType()})};
+    std::vector<uint32_t> funcs = {OMNI_AGG_SUM, OMNI_AGG_COUNT_COLUMN, OMNI_AGG_AVG, OMNI_AGG_MAX,
+                                   OMNI_AGG_COUNT_ALL, OMNI_AGG_SUM}, masks;
+    // COUNT(*) over raw input takes no input column: aggCols / inTypes list the other aggregates only
+    GpuHashAggregationOperatorFactory pf(groupCols, DataTypes({LongType()}), aggCols, inTypes, partOut, funcs, masks,
+                                         {true, true, true, true, true, true}, {true, true, true, true, true, true});
+    auto raw = [](int salt) {
+        auto *b = new VectorBatch(5);
+        b->Append(MakeVector<int64_t>(OMNI_LONG, {1, 2, 1, 3, 2}));
+        auto *v = MakeVector<int64_t>(OMNI_LONG, {10 + salt, 20, 30, 40, 50});
+        v->SetNull(3);
+        b->Append(v);
+        auto *d = MakeVector<int64_t>(OMNI_DECIMAL64, {100, 200, 300 + salt, 400, 500});
+        d->SetNull(1);
+        b->Append(d);
+        return b;
+    };
+    Operator *op = pf.CreateOperator();
+    VectorBatch *aligned = op->AlignSchema(raw(0));
+    Rows got;
+    AppendRows(got, aligned);
+    EXPECT_TRUE(SameRowSet(got, {{"1", "10", "1", "10", "1", "10", "1", "100", "false"},
+                                 {"2", "20", "1", "20", "1", "20", "1", "0", "true"},
+                                 {"1", "30", "1", "30", "1", "30", "1", "300", "false"},
+                                 {"3", "null", "0", "0", "0", "null", "1", "400", "false"},
+                                 {"2", "50", "1", "50", "1", "50", "1", "500", "false"}}));
+    // the final stage over two aligned batches equals one stage over the raw rows
+    std::vector<std::vector<uint32_t>> finCols = {{1}, {2}, {3, 4}, {5}, {6}, {7, 8}};
+    std::vector<DataTypes> finIn = {DataTypes({LongType()}), DataTypes({LongType()}),
+                                    DataTypes({DoubleType(), LongType()}), DataTypes({LongType()}),
+                                    DataTypes({LongType()}), DataTypes({Decimal128Type(38, 2), BooleanType()})};
+    std::vector<DataTypes> finOut = {DataTypes({LongType()}), DataTypes({LongType()}), DataTypes({DoubleType()}),
+                                     DataTypes({LongType()}), DataTypes({LongType()}),
+                                     DataTypes({Decimal128Type(38, 2)})};
+    GpuHashAggregationOperatorFactory ff(groupCols, DataTypes({LongType()}), finCols, finIn, finOut, funcs, masks,
+                                         {false, false, false, false, false, false},
+                                         {false, false, false, false, false, false});
+    Operator *fin = ff.CreateOperator();
+    fin->AddInput(aligned);
+    fin->AddInput(op->AlignSchema(raw(1)));
+    Rows two = Drain(fin);
+    std::vector<std::vector<uint32_t>> oneCols = {{1}, {1}, {1}, {1}, {2}};
+    std::vector<DataTypes> oneOut = finOut;
+    GpuHashAggregationOperatorFactory of(groupCols, DataTypes({LongType()}), oneCols, inTypes, oneOut, funcs, masks,
+                                         {true, true, true, true, true, true},
+                                         {false, false, false, false, false, false});
+    Operator *single = of.CreateOperator();
+    single->AddInput(raw(0));
+    single->AddInput(raw(1));
+    Rows expect = Drain(single);
+    EXPECT_TRUE(expect.size() == 3);
+    EXPECT_TRUE(SameRowSet(two, expect));
+    Operator::DeleteOperator(single);
+    Operator::DeleteOperator(fin);
+    Operator::DeleteOperator(op);
+}
+
 int main()
 {
     std::vector<std::pair<const char *, std::function<void()>>> tests = {
@@ -887,6 +1132,11 @@ int main()
         {"AggStagingAcrossFlushes", TestAggStagingAcrossFlushes},
         {"UnsupportedEncodingThrows", TestUnsupportedEncodingThrows},
         {"JoinSlicedAndDictionaryInputs", TestJoinSlicedAndDictionaryInputs},
+        {"AggDecimalTwoStageKnownAnswers", TestAggDecimalTwoStageKnownAnswers},
+        {"AggDecimalTwoStageOverflow", TestAggDecimalTwoStageOverflow},
+        {"AggDecimalPartialStateRules", TestAggDecimalPartialStateRules},
+        {"AggDecimalTwoStageMatchesSingleStage", TestAggDecimalTwoStageMatchesSingleStage},
+        {"AggAlignSchema", TestAggAlignSchema},
     };
     for (auto &t : tests) {
         int before = g_failures;

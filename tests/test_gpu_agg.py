@@ -4,11 +4,12 @@ import numpy as np
 import pytest
 
 import oracle
+from oracle import brute
 from oracle import (AVG, COUNT_ALL, COUNT_COLUMN, DECIMAL64, DECIMAL128, DOUBLE, INT, LONG, MAX, MIN, SHORT, SUM,
                     Col)
 from omnioperator_amd import capi, workloads
 from tests.helpers import assert_row_sets_equal, load_golden, rows_of
-from tests.test_oracle_agg import STD_AGGS, TYPES, _random_case
+from tests.test_oracle_agg import I64_MAX, STD_AGGS, TYPES, _random_case
 
 pytestmark = pytest.mark.gpu
 FIX = load_golden("operator_fixtures.json")
@@ -320,6 +321,34 @@ def test_partial_avg_output(in_type):
     assert_row_sets_equal(got, want, float_cols=[1, 3], rtol=DOUBLE_RTOL)
     row5 = [r for r in got if r[0] == 5][0]
     assert row5[1] == 0.0 and row5[2] == 0 and row5[3] is None
+
+
+@pytest.mark.parametrize("tin,tout", [(DECIMAL64, DECIMAL64), (DECIMAL64, DECIMAL128), (DECIMAL128, DECIMAL128)])
+def test_partial_decimal_sum_output(tin, tout):
+    """agg_partial_out on a decimal SUM equals the oracle's partial sum column (0 for a group of NULL inputs, NULL
+    for an overflowed group) and raises nothing even with overflow_as_null off; near-limit values of both signs
+    make row-order overflows in some groups."""
+    rng = np.random.default_rng(23)
+    n = 40_000
+    k = rng.integers(0, 400, n, dtype=np.int64)
+    lim = I64_MAX if tout == DECIMAL64 else 10**38 - 1
+    big = rng.random(n) < 0.02
+    small = rng.integers(-10**6, 10**6, n, dtype=np.int64)
+    if tin == DECIMAL64:
+        vals = np.where(big, np.where(rng.random(n) < 0.5, I64_MAX // 2, -(I64_MAX // 2)), small).astype(np.int64)
+    else:
+        ints = [int(lim // 2) * (1 if rng.random() < 0.5 else -1) if b else int(x) for b, x in zip(big, small)]
+        vals = brute.int_to_dec128(ints)
+    nulls = rng.random(n) < 0.05
+    nulls[k == 7] = True  # group 7: every input NULL
+    col = Col(tin, vals, nulls)
+    aggs = [(SUM, tin, tout, 0, 0, True), (COUNT_COLUMN, tin, LONG)]
+    got = run_gpu([Col(LONG, k)], aggs, [col, col], 4096, overflow_as_null=False)
+    want = run_oracle([Col(LONG, k)], aggs, [col, col], 4096)
+    assert_row_sets_equal(got, want)
+    assert [r for r in got if r[0] == 7][0][1:] == (0, 0)
+    # some groups overflowed (DECIMAL64 inputs cannot leave a DECIMAL128 sum's limits)
+    assert any(r[1] is None for r in got) == (tin == tout)
 
 
 def run_gpu_rule(key_cols, aggs, agg_cols, batch_rows, strategy=capi.STRATEGY_AUTO):

@@ -258,3 +258,29 @@ def test_agg_stream_equals_one_operator_per_batch():
             if b > a else 0 for a, b in zip(offsets[:-1], offsets[1:])]
     for threads in (1, 4):
         assert oracle.agg_stream(spec, key, [val, None], offsets, threads).tolist() == want
+
+
+def test_partial_decimal_sum_output():
+    """agg_partial_out on a decimal SUM: the sum column of SumSparkDecimalAggregator's partial state
+    (sum_spark_decimal_aggregator.h:246-301 with OUTPUT_PARTIAL) -- the value itself for the reference's
+    spark_sum_decimal64_normal row (aggregation_test.cpp:3680-3715: 999999999999.999999 from DECIMAL64(18, 6) into
+    DECIMAL128(28, 6)), 0 (not NULL) for a group of NULL inputs, NULL for an overflowed group or rescale, and never
+    an error."""
+    f = FIX["sum_decimal64"]
+    v1 = np.array(f["values"][:1], np.int64)
+    spec = oracle.make_agg_spec([LONG], [(SUM, DECIMAL64, DECIMAL128, 6, 6, True)])
+    out = oracle.hash_agg(spec, [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL64, v1, scale=6)], 16)
+    assert cols_rows(out) == [(0, f["sum_first_row"])]
+    k = np.array([0, 0, 1, 1, 2], np.int64)
+    v = np.array([I64_MAX, 1, 3, 4, 9], np.int64)
+    nulls = np.array([False, False, True, True, False])
+    for ovf_null in (False, True):
+        out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, DECIMAL64, DECIMAL64, 0, 0, True)],
+                                                   overflow_as_null=ovf_null),
+                              [Col(LONG, k)], [Col(DECIMAL64, v, nulls)], 16)
+        assert sorted(cols_rows(out), key=lambda r: r[0]) == [(0, None), (1, 0), (2, 9)]
+    # the rescale by 10^(out - in) overflowing the DECIMAL128 limits: NULL in the partial output, no error
+    big = brute.int_to_dec128([10**37])
+    out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, DECIMAL128, DECIMAL128, 0, 2, True)]),
+                          [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL128, big)], 16)
+    assert cols_rows(out) == [(0, None)]
