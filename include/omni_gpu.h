@@ -351,6 +351,12 @@ int omni_gpu_join_probe_batches(void *handle, const omni_gpu_column *keys, const
  * They are probed like the inner join: the radix-partitioned window probe for large hash tables (omni_gpu_join_tune),
  * the direct probe for array tables and small ones. */
 int omni_gpu_join_set_type(void *handle, int32_t join_type, int32_t build_side);
+/* LookupJoinOutputBuilder::ConstructProbeColumns (core/src/operator/join/lookup_join.cpp:1853-1877) on the
+ * device: out[c] row k = cols[c] row rows[k] (value and NULL bit) for k < n, rows being device int64 row ids
+ * into the columns (the out_probe_rows of a probe over them).  Fixed-width columns; out[c].nulls, when not NULL,
+ * must hold 4 * ceil(n / 32) bytes and is written in full as 32-bit words.  Synchronizes. */
+int omni_gpu_join_gather_rows(const omni_gpu_column *cols, int32_t num_cols, const int64_t *rows, int64_t n,
+                              omni_gpu_out_column *out);
 /* ProbeBatchForExistenceJoin (lookup_join.cpp:1393-1438): out_exists[i] = 1 when probe row i has a matching
  * build row, else 0 (also for a NULL key); rows bytes of device memory.  Works with any join type.
  * Synchronizes. */

@@ -598,6 +598,55 @@ int TranslateKeys(JoinHandle &h, const omni_gpu_column *in, const int64_t *rows,
     return OMNI_GPU_OK;
 }
 
+// Probe-side output columns of join output rows (LookupJoinOutputBuilder::ConstructProbeColumns,
+// lookup_join.cpp:1853-1877), gathered on the device: column c of output row k is probe column c's row rows[k].
+constexpr int kRowGatherCols = 8;  // columns per launch
+struct RowGatherCol {
+    const void *src;
+    const uint8_t *src_nulls;  // nullptr: no NULLs
+    void *dst;
+    unsigned int *dst_nulls;   // 32-bit words of the output rows' bits, nullptr: not written
+    int width;
+};
+struct RowGatherArgs {
+    RowGatherCol c[kRowGatherCols];
+    int ncols;
+    const long long *rows;
+    int64_t n;
+};
+
+__global__ void __launch_bounds__(256) join_gather_rows(RowGatherArgs g)
+{
+    const int64_t k = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;  // rows of a wave: 64 aligned ones
+    const bool live = k < g.n;
+    const int64_t r = live ? g.rows[k] : 0;
+    for (int c = 0; c < g.ncols; ++c) {
+        const RowGatherCol &gc = g.c[c];
+        if (live) {
+            switch (gc.width) {
+                case 16: {
+                    const auto *s2 = static_cast<const unsigned long long *>(gc.src) + 2 * r;
+                    auto *d2 = static_cast<unsigned long long *>(gc.dst) + 2 * k;
+                    d2[0] = s2[0];
+                    d2[1] = s2[1];
+                    break;
+                }
+                case 8: static_cast<unsigned long long *>(gc.dst)[k] = static_cast<const unsigned long long *>(gc.src)[r]; break;
+                case 4: static_cast<unsigned int *>(gc.dst)[k] = static_cast<const unsigned int *>(gc.src)[r]; break;
+                case 2: static_cast<unsigned short *>(gc.dst)[k] = static_cast<const unsigned short *>(gc.src)[r]; break;
+                default: static_cast<uint8_t *>(gc.dst)[k] = static_cast<const uint8_t *>(gc.src)[r]; break;
+            }
+        }
+        if (gc.dst_nulls) {
+            const bool isNull = live && gc.src_nulls && ((gc.src_nulls[r >> 3] >> (r & 7)) & 1);
+            const unsigned long long m = __ballot(isNull);
+            const int lane = LaneId();
+            const int64_t word = ((k - lane) >> 5) + (lane >> 5);  // lanes 0 and 32 write the wave's two words
+            if ((lane & 31) == 0 && word * 32 < g.n) gc.dst_nulls[word] = static_cast<unsigned int>(m >> lane);
+        }
+    }
+}
+
 }  // namespace omni_gpu
 
 using namespace omni_gpu;
@@ -1092,6 +1141,39 @@ int omni_gpu_join_outer_output(void *handle, int64_t *out_build_rows, omni_gpu_o
                         static_cast<long long>(out_capacity));
     }
     return OMNI_GPU_OK;
+}
+
+int omni_gpu_join_gather_rows(const omni_gpu_column *cols, int32_t num_cols, const int64_t *rows, int64_t n,
+                              omni_gpu_out_column *out)
+{
+    OG_CHECK(num_cols >= 0 && n >= 0 && (num_cols == 0 || (cols && out)) && (n == 0 || rows), OMNI_GPU_ERR_INVALID,
+             "null argument");
+    if (n == 0 || num_cols == 0) return OMNI_GPU_OK;
+    for (int32_t c = 0; c < num_cols; ++c) {
+        const int w = TypeWidth(cols[c].type_id);
+        OG_CHECK(w > 0 && cols[c].values && out[c].values, OMNI_GPU_ERR_INVALID, "column %d: fixed-width values needed", c);
+        OG_CHECK(TypeWidth(out[c].type_id) == w, OMNI_GPU_ERR_INVALID, "column %d: output width differs", c);
+    }
+    hipStream_t s;
+    OG_TRY(AcquireStream(&s));
+    int rc = OMNI_GPU_OK;
+    const unsigned int grid = static_cast<unsigned int>((n + 255) / 256);
+    for (int32_t c0 = 0; c0 < num_cols && rc == OMNI_GPU_OK; c0 += kRowGatherCols) {
+        RowGatherArgs g{};
+        g.ncols = std::min<int32_t>(kRowGatherCols, num_cols - c0);
+        g.rows = reinterpret_cast<const long long *>(rows);
+        g.n = n;
+        for (int i = 0; i < g.ncols; ++i) {
+            const omni_gpu_column &src = cols[c0 + i];
+            g.c[i] = RowGatherCol{src.values, src.nulls, out[c0 + i].values,
+                               reinterpret_cast<unsigned int *>(out[c0 + i].nulls), TypeWidth(src.type_id)};
+        }
+        join_gather_rows<<<grid, 256, 0, s>>>(g);
+        rc = CheckLaunch("join_gather_rows");
+    }
+    if (rc == OMNI_GPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = SetError(OMNI_GPU_ERR_HIP, "gather failed");
+    ReleaseStream(s);
+    return rc;
 }
 
 int omni_gpu_join_destroy(void *handle)

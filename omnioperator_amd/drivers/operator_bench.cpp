@@ -175,9 +175,11 @@ int RunJoin(int steps, int warmup)
             probeTypes, probeOut, 1, probeHash, 1, buildOut, 1, buildOutTypes, reinterpret_cast<int64_t>(bf), nullptr,
             false, nullptr);
         Operator *lookup = lf->CreateOperator();
+        // as LocalPlanner creates operators (local_planner.cpp:30-40): more input announced, noMoreInput() at the
+        // end -- the lookup probes buffered runs of batches
+        lookup->setNoMoreInput(false);
         int64_t m = 0;
-        for (auto *b : pin) {
-            lookup->AddInput(b);
+        auto drain = [&]() {
             for (;;) {
                 VectorBatch *out = nullptr;
                 lookup->GetOutput(&out);
@@ -185,7 +187,13 @@ int RunJoin(int steps, int warmup)
                 m += out->GetRowCount();
                 VectorHelper::FreeVecBatch(out);
             }
+        };
+        for (auto *b : pin) {
+            lookup->AddInput(b);
+            drain();
         }
+        lookup->noMoreInput();
+        drain();
         Operator::DeleteOperator(lookup);
         Operator::DeleteOperator(builder);
         delete lf;

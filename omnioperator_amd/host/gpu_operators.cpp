@@ -810,11 +810,277 @@ Operator *GpuLookupJoinOperatorFactory::CreateOperator() { return new GpuLookupJ
 GpuLookupJoinOperator::~GpuLookupJoinOperator()
 {
     if (probe_) vec::VectorHelper::FreeVecBatch(probe_);
+    for (auto *b : pending_) vec::VectorHelper::FreeVecBatch(b);
+    for (auto *b : emitting_) vec::VectorHelper::FreeVecBatch(b);
+}
+
+bool GpuLookupJoinOperator::needsInput()
+{
+    return GetStatus() != OMNI_STATUS_FINISHED && !noMoreInput_ && !inRun_ && probe_ == nullptr;
+}
+
+// Bytes of one output row: ceil(1 MiB / this) rows per output batch (lookup_join.cpp:211-244).
+int32_t GpuLookupJoinOperator::OutputRowSize() const
+{
+    const auto &btypes = f_.builder_->BuildTypes();
+    int32_t rowSize = 0;
+    for (int32_t c : f_.probeOutputCols_) rowSize += type::TypeWidth(f_.probeTypes_.GetType(c)->GetId());
+    for (int32_t c : f_.buildOutputCols_) rowSize += type::TypeWidth(btypes.GetType(c)->GetId());
+    if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) rowSize += 1;
+    return std::max(rowSize, 1);
 }
 
 int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
 {
     if (!f_.builder_->Finished()) throw OmniException("OPERATOR_RUNTIME_ERROR", "probe before the build finished");
+    if (noMoreInput_ || f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) return ProbeOne(vecBatch);
+    SetStatus(OMNI_STATUS_NORMAL);
+    if (vecBatch->GetRowCount() <= 0) {
+        vec::VectorHelper::FreeVecBatch(vecBatch);
+        return 0;
+    }
+    Stage(vecBatch);
+    return 0;
+}
+
+// Appends a probe batch's key and probe output columns to the staged run (values, and NULL bits at the run's
+// row positions); the batch is kept until the run's rows are emitted.
+void GpuLookupJoinOperator::Stage(vec::VectorBatch *vecBatch)
+{
+    const int32_t rows = vecBatch->GetRowCount();
+    if (stage_.empty()) {
+        int32_t is_array = 0;
+        int64_t max_dup = 0, slots = 0;
+        Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
+        flushRows_ = std::min<int64_t>(std::max<int64_t>(slots / 2, 1 << 20), 1 << 24);
+        for (int32_t c : f_.probeKeyCols_) {
+            stage_.emplace_back(new StagedProbeColumn());
+            stage_.back()->col = c;
+        }
+        for (int32_t c : f_.probeOutputCols_) {
+            int32_t at = -1;
+            for (size_t i = 0; i < stage_.size(); ++i) {
+                if (stage_[i]->col == c) at = static_cast<int32_t>(i);
+            }
+            if (at < 0) {
+                at = static_cast<int32_t>(stage_.size());
+                stage_.emplace_back(new StagedProbeColumn());
+                stage_.back()->col = c;
+            }
+            outStage_.push_back(at);
+        }
+        for (auto &st : stage_) {
+            st->type = vecBatch->Get(st->col)->GetTypeId();
+            st->width = type::TypeWidth(st->type);
+        }
+    }
+    if (stagedRows_ + rows > stageCap_) {
+        // grow (a run ends at the first batch past flushRows_): new buffers, the staged rows copied over
+        const int64_t cap = std::max<int64_t>(flushRows_ + (1 << 17), stagedRows_ + rows);
+        for (auto &st : stage_) {
+            HostBuffer v, n;
+            v.Reserve(static_cast<size_t>(cap) * st->width);
+            n.Reserve(static_cast<size_t>(cap + 7) / 8 + 8);
+            std::memset(n.Get(), 0, static_cast<size_t>(cap + 7) / 8 + 8);
+            if (stagedRows_ > 0) {
+                std::memcpy(v.Get(), st->values.Get(), static_cast<size_t>(stagedRows_) * st->width);
+                std::memcpy(n.Get(), st->nulls.Get(), static_cast<size_t>(stagedRows_ + 7) / 8);
+            }
+            st->values.Swap(v);
+            st->nulls.Swap(n);
+        }
+        stageCap_ = cap;
+    }
+    std::vector<uint8_t> tmp;
+    for (auto &st : stage_) {
+        vec::BaseVector *v = vecBatch->Get(st->col);
+        const void *src = ColumnValues(v, rows, tmp);
+        std::memcpy(static_cast<uint8_t *>(st->values.Get()) + static_cast<size_t>(stagedRows_) * st->width, src,
+                    static_cast<size_t>(rows) * st->width);
+        if (v->HasNull()) {
+            const uint8_t *bits = vec::unsafe::UnsafeBaseVector::GetNulls(v);
+            uint8_t *dst = static_cast<uint8_t *>(st->nulls.Get());
+            for (int32_t r = 0; r < rows; ++r) {
+                if ((bits[r >> 3] >> (r & 7)) & 1) {
+                    const int64_t q = stagedRows_ + r;
+                    dst[q >> 3] |= static_cast<uint8_t>(1u << (q & 7));
+                    st->anyNull = true;
+                }
+            }
+        }
+    }
+    stagedRows_ += rows;
+    pending_.push_back(vecBatch);
+}
+
+// Probes the staged run in one call, gathers its probe output columns on the device and downloads the output
+// columns into page-locked host memory; the run's batches move to emitting_.
+void GpuLookupJoinOperator::FlushRun()
+{
+    const int64_t rows = stagedRows_;
+    const size_t ns = stage_.size(), nk = f_.probeKeyCols_.size(), npo = f_.probeOutputCols_.size();
+    while (keyBufs_.size() < ns) {
+        keyBufs_.emplace_back(new DeviceBuffer());
+        keyNullBufs_.emplace_back(new DeviceBuffer());
+    }
+    std::vector<omni_gpu_column> cols(ns);
+    for (size_t i = 0; i < ns; ++i) {
+        StagedProbeColumn &st = *stage_[i];
+        keyBufs_[i]->Reserve(static_cast<size_t>(rows) * st.width);
+        Check(omni_gpu_memcpy_h2d(keyBufs_[i]->Get(), st.values.Get(), static_cast<size_t>(rows) * st.width),
+              "upload probe column");
+        cols[i].type_id = st.type;
+        cols[i].values = keyBufs_[i]->Get();
+        if (st.anyNull) {
+            const size_t nb = static_cast<size_t>(rows + 7) / 8;
+            keyNullBufs_[i]->Reserve(nb);
+            Check(omni_gpu_memcpy_h2d(keyNullBufs_[i]->Get(), st.nulls.Get(), nb), "upload probe nulls");
+            cols[i].nulls = static_cast<const uint8_t *>(keyNullBufs_[i]->Get());
+            std::memset(st.nulls.Get(), 0, nb);
+            st.anyNull = false;
+        }
+    }
+    int32_t is_array = 0;
+    int64_t max_dup = 0, slots = 0;
+    Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
+    // outer joins emit a row per unmatched probe row as well
+    const int64_t cap = std::max<int64_t>(rows, rows * std::max<int64_t>(1, max_dup));
+    rowsBuf_.Reserve(sizeof(int64_t) * cap);
+    const auto &btypes = f_.builder_->BuildTypes();
+    const int32_t nb = btypes.GetSize();
+    std::vector<omni_gpu_out_column> outs(nb);
+    while (payBufs_.size() < static_cast<size_t>(nb)) {
+        payBufs_.emplace_back(new DeviceBuffer());
+        payNullBufs_.emplace_back(new DeviceBuffer());
+    }
+    for (int32_t c = 0; c < nb; ++c) outs[c].type_id = btypes.GetType(c)->GetId();
+    for (int32_t col : f_.buildOutputCols_) {
+        payBufs_[col]->Reserve(static_cast<size_t>(cap) * type::TypeWidth(btypes.GetType(col)->GetId()));
+        payNullBufs_[col]->Reserve(4 * ((cap + 31) / 32));
+        outs[col].values = payBufs_[col]->Get();
+        outs[col].nulls = static_cast<uint8_t *>(payNullBufs_[col]->Get());
+    }
+    Check(omni_gpu_join_probe(f_.builder_->Table(), cols.data(), rows, 0, static_cast<int64_t *>(rowsBuf_.Get()),
+                              outs.data(), cap, &matches_),
+          "omni_gpu_join_probe");
+    emitted_ = 0;
+    // the probe output columns of the output rows, gathered on the device by probe row
+    while (gatherBufs_.size() < npo) {
+        gatherBufs_.emplace_back(new DeviceBuffer());
+        gatherNullBufs_.emplace_back(new DeviceBuffer());
+    }
+    std::vector<omni_gpu_column> src(npo);
+    std::vector<omni_gpu_out_column> gout(npo);
+    for (size_t i = 0; i < npo; ++i) {
+        src[i] = cols[outStage_[i]];
+        gatherBufs_[i]->Reserve(static_cast<size_t>(std::max<int64_t>(matches_, 1)) * stage_[outStage_[i]]->width);
+        gatherNullBufs_[i]->Reserve(4 * ((std::max<int64_t>(matches_, 1) + 31) / 32));
+        gout[i].type_id = src[i].type_id;
+        gout[i].values = gatherBufs_[i]->Get();
+        gout[i].nulls = static_cast<uint8_t *>(gatherNullBufs_[i]->Get());
+    }
+    if (matches_ > 0) {
+        Check(omni_gpu_join_gather_rows(src.data(), static_cast<int32_t>(npo), static_cast<const int64_t *>(rowsBuf_.Get()),
+                                        matches_, gout.data()),
+              "omni_gpu_join_gather_rows");
+    }
+    // download: probe output columns, then build output columns
+    const size_t nout = npo + f_.buildOutputCols_.size();
+    while (runVals_.size() < nout) {
+        runVals_.emplace_back(new HostBuffer());
+        runNulls_.emplace_back(new HostBuffer());
+    }
+    runAnyNull_.assign(nout, false);
+    const size_t nbytes = static_cast<size_t>(matches_ + 7) / 8;
+    for (size_t i = 0; i < nout; ++i) {
+        const bool probeSide = i < npo;
+        const int32_t col = probeSide ? 0 : f_.buildOutputCols_[i - npo];
+        const int w = probeSide ? stage_[outStage_[i]]->width : type::TypeWidth(btypes.GetType(col)->GetId());
+        const void *dv = probeSide ? gatherBufs_[i]->Get() : payBufs_[col]->Get();
+        const void *dn = probeSide ? gatherNullBufs_[i]->Get() : payNullBufs_[col]->Get();
+        runVals_[i]->Reserve(static_cast<size_t>(std::max<int64_t>(matches_, 1)) * w);
+        runNulls_[i]->Reserve(nbytes + 8);
+        if (matches_ == 0) continue;
+        Check(omni_gpu_memcpy_d2h(runVals_[i]->Get(), dv, static_cast<size_t>(matches_) * w), "download output");
+        Check(omni_gpu_memcpy_d2h(runNulls_[i]->Get(), dn, nbytes), "download output nulls");
+        const uint8_t *bits = static_cast<const uint8_t *>(runNulls_[i]->Get());
+        bool any = false;
+        for (size_t b = 0; b < nbytes && !any; ++b) {
+            uint8_t m = bits[b];
+            if (b == nbytes - 1 && (matches_ & 7)) m &= static_cast<uint8_t>((1u << (matches_ & 7)) - 1);
+            any = m != 0;
+        }
+        runAnyNull_[i] = any;
+    }
+    emitting_.swap(pending_);
+    stagedRows_ = 0;
+    inRun_ = true;
+}
+
+// The next output batch of the flushed run: copies out of the downloaded columns.
+vec::VectorBatch *GpuLookupJoinOperator::EmitRun()
+{
+    const auto &btypes = f_.builder_->BuildTypes();
+    const int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(OutputRowSize()), matches_ - emitted_));
+    std::unique_ptr<vec::VectorBatch> out(new vec::VectorBatch(count));
+    const size_t npo = f_.probeOutputCols_.size();
+    auto column = [&](size_t i, int32_t tid, bool allNull) {
+        auto *v = vec::VectorHelper::CreateFlatVector(tid, count);
+        const int w = type::TypeWidth(tid);
+        if (allNull) {
+            for (int32_t r = 0; r < count; ++r) v->SetNull(r);
+            return v;
+        }
+        std::memcpy(FlatValues(v), static_cast<const uint8_t *>(runVals_[i]->Get()) + static_cast<size_t>(w) * emitted_,
+                    static_cast<size_t>(w) * count);
+        if (runAnyNull_[i]) {
+            const uint8_t *bits = static_cast<const uint8_t *>(runNulls_[i]->Get());
+            for (int32_t r = 0; r < count; ++r) {
+                const int64_t m = emitted_ + r;
+                if ((bits[m >> 3] >> (m & 7)) & 1) v->SetNull(r);
+            }
+        }
+        return v;
+    };
+    for (size_t i = 0; i < npo; ++i) out->Append(column(i, stage_[outStage_[i]]->type, false));
+    const JoinType jt = f_.builder_->GetJoinType();
+    for (size_t j = 0; j < f_.buildOutputCols_.size(); ++j) {
+        // LEFT_ANTI: AppendRow(probePosition, nullptr, 0), build columns NULL
+        out->Append(column(npo + j, btypes.GetType(f_.buildOutputCols_[j])->GetId(), jt == OMNI_JOIN_TYPE_LEFT_ANTI));
+    }
+    if (f_.builder_->GetBuildSide() == OMNI_BUILD_LEFT) {  // lookup_join.cpp:1959-1962: build columns first
+        out->RotateLeft(static_cast<int32_t>(npo));
+    }
+    emitted_ += count;
+    return out.release();
+}
+
+int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
+{
+    *result = nullptr;
+    if (probe_) return EmitOne(result);
+    if (inRun_ && emitted_ >= matches_) {
+        for (auto *b : emitting_) vec::VectorHelper::FreeVecBatch(b);
+        emitting_.clear();
+        inRun_ = false;
+    }
+    if (!inRun_ && stagedRows_ > 0 && (noMoreInput_ || stagedRows_ >= flushRows_)) FlushRun();
+    if (inRun_ && emitted_ < matches_) {
+        *result = EmitRun();
+        return 0;
+    }
+    if (inRun_) {  // a run without output rows
+        for (auto *b : emitting_) vec::VectorHelper::FreeVecBatch(b);
+        emitting_.clear();
+        inRun_ = false;
+    }
+    if (noMoreInput_ && stagedRows_ == 0) SetStatus(OMNI_STATUS_FINISHED);
+    return 0;
+}
+
+// One batch probed on its own (the reference's flow, lookup_join.cpp:420-475).
+int32_t GpuLookupJoinOperator::ProbeOne(vec::VectorBatch *vecBatch)
+{
     if (probe_) vec::VectorHelper::FreeVecBatch(probe_);
     probe_ = vecBatch;
     SetStatus(OMNI_STATUS_NORMAL);
@@ -883,7 +1149,7 @@ int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
 
 // Output batch = probe output columns (gathered by probe row) ++ build output columns
 // (LookupJoinOutputBuilder::BuildOutput, lookup_join.cpp:1924-1985), at most ceil(1 MiB / row size) rows.
-int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
+int32_t GpuLookupJoinOperator::EmitOne(vec::VectorBatch **result)
 {
     *result = nullptr;
     if (!probe_) return 0;
@@ -894,11 +1160,7 @@ int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
         return 0;
     }
     const auto &btypes = f_.builder_->BuildTypes();
-    int32_t rowSize = 0;
-    for (int32_t c : f_.probeOutputCols_) rowSize += f_.probeTypes_.GetType(c)->GetId() ? type::TypeWidth(f_.probeTypes_.GetType(c)->GetId()) : 0;
-    for (int32_t c : f_.buildOutputCols_) rowSize += type::TypeWidth(btypes.GetType(c)->GetId());
-    if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) rowSize += 1;
-    int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(std::max(rowSize, 1)), matches_ - emitted_));
+    int32_t count = static_cast<int32_t>(std::min<int64_t>(MaxRowCount(OutputRowSize()), matches_ - emitted_));
     auto *out = new vec::VectorBatch(count);
     std::vector<uint8_t> tmp;
     for (int32_t c : f_.probeOutputCols_) {

@@ -51,6 +51,11 @@ public:
     void Reserve(size_t bytes);  // contents not kept
     void *Get() const { return ptr_; }
     size_t Bytes() const { return bytes_; }
+    void Swap(HostBuffer &o)
+    {
+        std::swap(ptr_, o.ptr_);
+        std::swap(bytes_, o.bytes_);
+    }
 
 private:
     void *ptr_ = nullptr;
@@ -231,15 +236,46 @@ private:
     GpuHashBuilderOperatorFactory *builder_ = nullptr;
 };
 
+// Probes one batch at a time, as the reference does, unless the caller announced more input
+// (setNoMoreInput(false), as LocalPlanner does for every operator, local_planner.cpp:30-40): then AddInput stages
+// the probe batches' columns in page-locked host memory and every run of about half the table's slots (at least
+// 2^20, at most 2^24 rows) -- or the rest, once noMoreInput() is called -- is probed in one call (the
+// radix-partitioned chain for large tables), the probe output columns are gathered on the device, and GetOutput
+// emits the run's rows before the operator takes more input (needsInput() is false meanwhile).  Outputs are the
+// same rows either way; only their grouping into batches differs.
 class GpuLookupJoinOperator : public Operator {
 public:
     explicit GpuLookupJoinOperator(const GpuLookupJoinOperatorFactory &f) : f_(f) {}
     ~GpuLookupJoinOperator() override;
     int32_t AddInput(vec::VectorBatch *vecBatch) override;
     int32_t GetOutput(vec::VectorBatch **result) override;
+    bool needsInput() override;
 
 private:
+    // One probe column's staged rows (key or probe output column).
+    struct StagedProbeColumn {
+        int32_t col = -1;
+        int32_t type = 0;
+        int32_t width = 0;
+        bool anyNull = false;
+        HostBuffer values, nulls;
+    };
+    int32_t ProbeOne(vec::VectorBatch *vecBatch);
+    void Stage(vec::VectorBatch *vecBatch);
+    void FlushRun();
+    vec::VectorBatch *EmitRun();
     const GpuLookupJoinOperatorFactory &f_;
+    // buffered probe runs
+    std::vector<vec::VectorBatch *> pending_;   // staged probe batches (freed once the run is emitted)
+    std::vector<vec::VectorBatch *> emitting_;  // the batches of the run being emitted
+    std::vector<std::unique_ptr<StagedProbeColumn>> stage_;  // key columns, then the other probe output columns
+    std::vector<int32_t> outStage_;             // per probe output column: its index in stage_
+    int64_t stagedRows_ = 0;
+    int64_t stageCap_ = 0;
+    int64_t flushRows_ = 0;
+    bool inRun_ = false;                        // the rows of a flushed run are being emitted
+    std::vector<std::unique_ptr<HostBuffer>> runVals_, runNulls_;  // probe output columns, then build output columns
+    std::vector<bool> runAnyNull_;
     vec::VectorBatch *probe_ = nullptr;
     std::vector<int64_t> matchRows_;
     std::vector<std::vector<uint8_t>> matchPay_;       // per build output column: values
@@ -250,6 +286,9 @@ private:
     DeviceBuffer rowsBuf_;
     std::vector<std::unique_ptr<DeviceBuffer>> keyBufs_, keyNullBufs_;  // per probe key column
     std::vector<std::unique_ptr<DeviceBuffer>> payBufs_, payNullBufs_;
+    std::vector<std::unique_ptr<DeviceBuffer>> gatherBufs_, gatherNullBufs_;  // gathered probe output columns
+    int32_t OutputRowSize() const;
+    int32_t EmitOne(vec::VectorBatch **result);
 };
 
 // LookupOuterJoinOperatorFactory (core/src/operator/join/lookup_outer_join.h): after every probe of a

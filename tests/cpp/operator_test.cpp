@@ -3,6 +3,8 @@
 // AddInput, pull GetOutput, compare as row sets (VecBatchMatchIgnoreOrder semantics).
 // Exit status 0 when every case passes.
 #include <algorithm>
+#include <array>
+#include <climits>
 #include <cstdio>
 #include <map>
 #include <cstring>
@@ -1110,6 +1112,125 @@ static void TestAggAlignSchema()
     Operator::DeleteOperator(op);
 }
 
+
+// ---- buffered probe runs (setNoMoreInput(false) ... noMoreInput()) -------------------------------------------
+
+// A join of `probeRows` probe rows (batches of 65,536, a LONG key with NULLs and a LONG payload column with NULLs)
+// against 3,000 build rows (keys 0..999 three times each for the inner / left joins: duplicate keys), run once
+// batch by batch and once as buffered runs; both give the same rows, the buffered operator takes no input while
+// it emits a run and finishes after noMoreInput().
+static void BufferedVersusPerBatch(JoinType jt, int64_t probeRows)
+{
+    DataTypes btypes({LongType(), LongType()});
+    std::vector<int64_t> bk(3000), bv(3000);
+    for (int i = 0; i < 3000; ++i) {
+        bk[i] = i % 1000;
+        bv[i] = 7 * i;
+    }
+    int32_t keyCols[1] = {0};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(jt, OMNI_BUILD_UNKNOWN, btypes, keyCols, 1, 1);
+    Operator *builder = bf->CreateOperator();
+    auto *build = new VectorBatch(3000);
+    build->Append(MakeVector<int64_t>(OMNI_LONG, bk));
+    build->Append(MakeVector<int64_t>(OMNI_LONG, bv));
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    const bool probeOnly = jt == OMNI_JOIN_TYPE_LEFT_ANTI;
+    DataTypes ptypes({LongType(), LongType()});
+    int32_t probeOut[2] = {0, 1}, buildOut[1] = {1};
+    DataTypes buildOutTypes = probeOnly ? DataTypes() : DataTypes({LongType()});
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        ptypes, probeOut, 2, keyCols, 1, buildOut, probeOnly ? 0 : 1, buildOutTypes, reinterpret_cast<int64_t>(bf),
+        nullptr, false, nullptr);
+    auto batches = [&]() {
+        std::vector<VectorBatch *> out;
+        uint64_t st = 0xB0F;
+        for (int64_t a = 0; a < probeRows; a += 65536) {
+            const int32_t n = static_cast<int32_t>(std::min<int64_t>(65536, probeRows - a));
+            std::vector<int64_t> k(n), v(n);
+            for (int32_t r = 0; r < n; ++r) {
+                st = st * 6364136223846793005ULL + 1442695040888963407ULL;
+                k[r] = static_cast<int64_t>((st >> 33) % 1500);  // a third of the keys miss
+                v[r] = a + r;
+            }
+            auto *b = new VectorBatch(n);
+            auto *kv = MakeVector<int64_t>(OMNI_LONG, k);
+            auto *vv = MakeVector<int64_t>(OMNI_LONG, v);
+            for (int32_t r = 0; r < n; r += 97) kv->SetNull(r);
+            for (int32_t r = 5; r < n; r += 89) vv->SetNull(r);
+            b->Append(kv);
+            b->Append(vv);
+            out.push_back(b);
+        }
+        return out;
+    };
+    // rows as int64 triples (INT64_MIN for NULL), sorted: cheaper than text rows at a million rows
+    using Row3 = std::array<int64_t, 3>;
+    auto collect = [](std::vector<Row3> &rows, VectorBatch *o) {
+        for (int32_t r = 0; r < o->GetRowCount(); ++r) {
+            Row3 t = {INT64_MIN, INT64_MIN, INT64_MIN};
+            for (int32_t c = 0; c < o->GetVectorCount() && c < 3; ++c) {
+                const BaseVector *v = o->Get(c);
+                if (!v->IsNull(r)) t[c] = static_cast<const Vector<int64_t> *>(v)->GetValue(r);
+            }
+            rows.push_back(t);
+        }
+        VectorHelper::FreeVecBatch(o);
+    };
+    auto drain = [&](Operator *op, std::vector<Row3> &rows) {
+        int n = 0;
+        for (;;) {
+            VectorBatch *o = nullptr;
+            op->GetOutput(&o);
+            if (!o) break;
+            collect(rows, o);
+            ++n;
+        }
+        return n;
+    };
+    Operator *one = lf->CreateOperator();
+    std::vector<Row3> perBatch;
+    for (auto *b : batches()) {
+        one->AddInput(b);
+        drain(one, perBatch);
+    }
+    Operator *buf = lf->CreateOperator();
+    buf->setNoMoreInput(false);
+    std::vector<Row3> buffered;
+    int runsSeen = 0;
+    for (auto *b : batches()) {
+        EXPECT_TRUE(buf->needsInput());
+        buf->AddInput(b);
+        VectorBatch *o = nullptr;
+        buf->GetOutput(&o);
+        if (o) {
+            ++runsSeen;
+            EXPECT_TRUE(!buf->needsInput());  // a run is being emitted
+            collect(buffered, o);
+            drain(buf, buffered);
+            EXPECT_TRUE(buf->needsInput());
+        }
+    }
+    buf->noMoreInput();
+    drain(buf, buffered);
+    EXPECT_TRUE(buf->GetStatus() == OMNI_STATUS_FINISHED);
+    EXPECT_TRUE(!perBatch.empty());
+    std::sort(perBatch.begin(), perBatch.end());
+    std::sort(buffered.begin(), buffered.end());
+    EXPECT_TRUE(perBatch == buffered);
+    if (probeRows > (1 << 20)) EXPECT_TRUE(runsSeen >= 1);  // a run flushed before noMoreInput()
+    Operator::DeleteOperator(one);
+    Operator::DeleteOperator(buf);
+    Operator::DeleteOperator(builder);
+    delete lf;
+    delete bf;
+}
+
+static void TestJoinBufferedRunsInner() { BufferedVersusPerBatch(OMNI_JOIN_TYPE_INNER, 1300000); }
+static void TestJoinBufferedRunsLeft() { BufferedVersusPerBatch(OMNI_JOIN_TYPE_LEFT, 200000); }
+static void TestJoinBufferedRunsAnti() { BufferedVersusPerBatch(OMNI_JOIN_TYPE_LEFT_ANTI, 200000); }
+
 int main()
 {
     std::vector<std::pair<const char *, std::function<void()>>> tests = {
@@ -1137,6 +1258,9 @@ int main()
         {"AggDecimalPartialStateRules", TestAggDecimalPartialStateRules},
         {"AggDecimalTwoStageMatchesSingleStage", TestAggDecimalTwoStageMatchesSingleStage},
         {"AggAlignSchema", TestAggAlignSchema},
+        {"JoinBufferedRunsInner", TestJoinBufferedRunsInner},
+        {"JoinBufferedRunsLeft", TestJoinBufferedRunsLeft},
+        {"JoinBufferedRunsAnti", TestJoinBufferedRunsAnti},
     };
     for (auto &t : tests) {
         int before = g_failures;
