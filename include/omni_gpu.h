@@ -357,6 +357,13 @@ int omni_gpu_join_set_type(void *handle, int32_t join_type, int32_t build_side);
  * must hold 4 * ceil(n / 32) bytes and is written in full as 32-bit words.  Synchronizes. */
 int omni_gpu_join_gather_rows(const omni_gpu_column *cols, int32_t num_cols, const int64_t *rows, int64_t n,
                               omni_gpu_out_column *out);
+/* Hash-table partition of each probe row of a join over several builder partitions (operatorCount > 1 in
+ * HashBuilderOperatorFactory, hash_builder.cpp:155-160): PopulateProbeHashes (lookup_join.cpp:1572-1612; per row
+ * h = 0, then h = 31 * h + HashUtil::HashValue(value) for each non-NULL key column) and
+ * HashUtil::GetRawHashPartition(h, mask) (hash_util.h:146-154), into out_partitions[rows] (device).  Key columns
+ * of SHORT, INT, DATE32, LONG, TIMESTAMP, DECIMAL64, DOUBLE, BOOLEAN or DECIMAL128.  Synchronizes. */
+int omni_gpu_join_partition_ids(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, uint32_t mask,
+                                uint32_t *out_partitions);
 /* ProbeBatchForExistenceJoin (lookup_join.cpp:1393-1438): out_exists[i] = 1 when probe row i has a matching
  * build row, else 0 (also for a NULL key); rows bytes of device memory.  Works with any join type.
  * Synchronizes. */

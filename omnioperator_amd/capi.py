@@ -119,6 +119,7 @@ SIGNATURES = {
     "omni_gpu_join_set_type": (_I, [_VP, _I32, _I32]),
     "omni_gpu_join_probe_exists": (_I, [_VP, _P(Column), _I64, _VP]),
     "omni_gpu_join_gather_rows": (_I, [_P(Column), _I32, _VP, _I64, _P(OutColumn)]),
+    "omni_gpu_join_partition_ids": (_I, [_P(Column), _I32, _I64, C.c_uint32, _VP]),
     "omni_gpu_join_outer_output": (_I, [_VP, _VP, _P(OutColumn), _I64, _P(_I64)]),
     "omni_gpu_join_destroy": (_I, [_VP]),
     "omni_gpu_partition_ids": (_I, [_P(Column), _I32, _I64, _I32, _VP]),

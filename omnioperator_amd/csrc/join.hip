@@ -647,6 +647,75 @@ __global__ void __launch_bounds__(256) join_gather_rows(RowGatherArgs g)
     }
 }
 
+// Probe rows' hash-table partitions of a multi-partition join (operatorCount > 1): LookupJoinOperator's
+// PopulateProbeHashes (lookup_join.cpp:1572-1612, CalculateColHashes et al.) -- per row h = 0, then per non-NULL
+// key column h = 31 * h + HashValue(value) -- and HashUtil::GetRawHashPartition(h, mask) (hash_util.h:146-154).
+// HashValue: int16 / int32 / int64 sign-extended, rotl(v * P2, 31) * P1 (hash_util.h:74-82); DOUBLE through
+// DoubleToLongBits (NaNs as 0x7ff8000000000000, :84-87, 270-279); BOOLEAN 1231 / 1237 (:89-92); DECIMAL64 the
+// value (:94-97); DECIMAL128 the xxhash of the low word XOR that of the high word without its sign bit (:99-103).
+constexpr unsigned long long kXxP1 = 0x9E3779B185EBCA87ULL, kXxP2 = 0xC2B2AE3D27D4EB4FULL,
+                             kXxP3 = 0x165667B19E3779F9ULL, kXxP4 = 0x85EBCA77C2B2AE63ULL,
+                             kXxP5 = 0x27D4EB2F165667C5ULL;
+
+__device__ __forceinline__ unsigned long long Rotl64(unsigned long long v, int r) { return (v << r) | (v >> (64 - r)); }
+
+// xxhash64 of one 64-bit word with seed 0 (XxHash64UpdateTail + XxHash64FinalShuffle from P5 + 8)
+__device__ __forceinline__ unsigned long long XxWord(unsigned long long v)
+{
+    unsigned long long h = kXxP5 + 8;
+    h = Rotl64(h ^ (Rotl64(v * kXxP2, 31) * kXxP1), 27) * kXxP1 + kXxP4;
+    h ^= h >> 33;
+    h *= kXxP2;
+    h ^= h >> 29;
+    h *= kXxP3;
+    h ^= h >> 32;
+    return h;
+}
+
+__device__ __forceinline__ unsigned long long HashUtilValue(unsigned long long v) { return Rotl64(v * kXxP2, 31) * kXxP1; }
+
+struct PartKeyCols {
+    int n;
+    int types[OMNI_GPU_MAX_KEYS];
+    const void *values[OMNI_GPU_MAX_KEYS];
+    const uint8_t *nulls[OMNI_GPU_MAX_KEYS];
+};
+
+__global__ void __launch_bounds__(256) join_partition_ids(PartKeyCols kc, int64_t rows, unsigned int mask,
+                                                          unsigned int *__restrict__ out)
+{
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * 256 + threadIdx.x;
+    if (r >= rows) return;
+    unsigned long long h = 0;
+    for (int c = 0; c < kc.n; ++c) {
+        if (NullBit(kc.nulls[c], r)) continue;
+        unsigned long long hv;
+        switch (kc.types[c]) {
+            case OMNI_TYPE_SHORT: hv = HashUtilValue(static_cast<long long>(static_cast<const short *>(kc.values[c])[r])); break;
+            case OMNI_TYPE_INT:
+            case OMNI_TYPE_DATE32: hv = HashUtilValue(static_cast<long long>(static_cast<const int *>(kc.values[c])[r])); break;
+            case OMNI_TYPE_BOOLEAN: hv = static_cast<const uint8_t *>(kc.values[c])[r] ? 1231ULL : 1237ULL; break;
+            case OMNI_TYPE_DECIMAL64: hv = static_cast<const unsigned long long *>(kc.values[c])[r]; break;
+            case OMNI_TYPE_DOUBLE: {
+                unsigned long long bits = static_cast<const unsigned long long *>(kc.values[c])[r];
+                if ((bits & 0x7FF0000000000000ULL) == 0x7FF0000000000000ULL && (bits & 0x000FFFFFFFFFFFFFULL) != 0) {
+                    bits = 0x7FF8000000000000ULL;
+                }
+                hv = HashUtilValue(bits);
+                break;
+            }
+            case OMNI_TYPE_DECIMAL128: {
+                const auto *w = static_cast<const unsigned long long *>(kc.values[c]) + 2 * r;
+                hv = XxWord(w[0]) ^ XxWord(w[1] & 0x7FFFFFFFFFFFFFFFULL);
+                break;
+            }
+            default: hv = HashUtilValue(static_cast<const unsigned long long *>(kc.values[c])[r]); break;
+        }
+        h = 31ULL * h + hv;
+    }
+    out[r] = static_cast<unsigned int>(XxWord(__brevll(h))) & mask;
+}
+
 }  // namespace omni_gpu
 
 using namespace omni_gpu;
@@ -1172,6 +1241,34 @@ int omni_gpu_join_gather_rows(const omni_gpu_column *cols, int32_t num_cols, con
         rc = CheckLaunch("join_gather_rows");
     }
     if (rc == OMNI_GPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = SetError(OMNI_GPU_ERR_HIP, "gather failed");
+    ReleaseStream(s);
+    return rc;
+}
+
+int omni_gpu_join_partition_ids(const omni_gpu_column *keys, int32_t num_keys, int64_t rows, uint32_t mask,
+                                uint32_t *out_partitions)
+{
+    OG_CHECK(keys && out_partitions && num_keys >= 1 && num_keys <= OMNI_GPU_MAX_KEYS && rows >= 0,
+             OMNI_GPU_ERR_INVALID, "bad argument");
+    if (rows == 0) return OMNI_GPU_OK;
+    PartKeyCols kc{};
+    kc.n = num_keys;
+    for (int32_t c = 0; c < num_keys; ++c) {
+        const int t = keys[c].type_id;
+        OG_CHECK(t == OMNI_TYPE_SHORT || t == OMNI_TYPE_INT || t == OMNI_TYPE_DATE32 || t == OMNI_TYPE_LONG ||
+                     t == OMNI_TYPE_TIMESTAMP || t == OMNI_TYPE_DECIMAL64 || t == OMNI_TYPE_DOUBLE ||
+                     t == OMNI_TYPE_BOOLEAN || t == OMNI_TYPE_DECIMAL128,
+                 OMNI_GPU_ERR_INVALID, "key column %d: type %d has no partition hash here", c, t);
+        OG_CHECK(keys[c].values, OMNI_GPU_ERR_INVALID, "key column %d has no values", c);
+        kc.types[c] = t;
+        kc.values[c] = keys[c].values;
+        kc.nulls[c] = keys[c].nulls;
+    }
+    hipStream_t s;
+    OG_TRY(AcquireStream(&s));
+    join_partition_ids<<<static_cast<unsigned int>((rows + 255) / 256), 256, 0, s>>>(kc, rows, mask, out_partitions);
+    int rc = CheckLaunch("join_partition_ids");
+    if (rc == OMNI_GPU_OK && hipStreamSynchronize(s) != hipSuccess) rc = SetError(OMNI_GPU_ERR_HIP, "partition ids failed");
     ReleaseStream(s);
     return rc;
 }

@@ -685,21 +685,25 @@ GpuHashBuilderOperatorFactory *GpuHashBuilderOperatorFactory::CreateHashBuilderO
         throw OmniException("UNSUPPORTED_ERROR", "1 to " + std::to_string(OMNI_GPU_MAX_KEYS) +
                                                      " join key columns are supported");
     }
-    if (operatorCount != 1) throw OmniException("UNSUPPORTED_ERROR", "one build partition is supported");
+    if (operatorCount & (operatorCount - 1)) {
+        // the probe masks its partition hash with operatorCount - 1 (lookup_join.cpp:496-498)
+        throw OmniException("UNSUPPORTED_ERROR", "operatorCount must be a power of two on the GPU path");
+    }
     for (int32_t i = 0; i < buildHashColsCount; ++i) {
         if (buildHashCols[i] < 0 || buildHashCols[i] >= buildTypes.GetSize()) {
             throw OmniException("INVALID_ARGUMENT", "join key column out of range");
         }
     }
     return new GpuHashBuilderOperatorFactory(joinType, buildSide, buildTypes,
-                                             std::vector<int32_t>(buildHashCols, buildHashCols + buildHashColsCount));
+                                             std::vector<int32_t>(buildHashCols, buildHashCols + buildHashColsCount),
+                                             operatorCount);
 }
 
 // The key variant follows the key columns as in HashBuilderOperatorFactory::InitVariant (hash_builder.cpp:44-93);
 // several columns or a DECIMAL128 column go through the library's key translation.
 GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide,
                                                              const type::DataTypes &buildTypes,
-                                                             std::vector<int32_t> keyCols)
+                                                             std::vector<int32_t> keyCols, int32_t operatorCount)
     : joinType_(joinType), buildSide_(buildSide), buildTypes_(buildTypes), keyCols_(std::move(keyCols))
 {
     EnsureInit();
@@ -716,16 +720,27 @@ GpuHashBuilderOperatorFactory::GpuHashBuilderOperatorFactory(JoinType joinType, 
         }
     }
     spec.strategy = 0;
-    Check(omni_gpu_join_build_create(&spec, &handle_), "omni_gpu_join_build_create");
-    Check(omni_gpu_join_set_type(handle_, joinType, buildSide), "omni_gpu_join_set_type");
+    tables_.assign(static_cast<size_t>(operatorCount), nullptr);
+    partFinished_.assign(static_cast<size_t>(operatorCount), false);
+    retained_.resize(static_cast<size_t>(operatorCount));
+    for (auto &t : tables_) {
+        Check(omni_gpu_join_build_create(&spec, &t), "omni_gpu_join_build_create");
+        Check(omni_gpu_join_set_type(t, joinType, buildSide), "omni_gpu_join_set_type");
+    }
 }
 
 GpuHashBuilderOperatorFactory::~GpuHashBuilderOperatorFactory()
 {
-    if (handle_) omni_gpu_join_destroy(handle_);
+    for (void *t : tables_) {
+        if (t) omni_gpu_join_destroy(t);
+    }
 }
 
-Operator *GpuHashBuilderOperatorFactory::CreateOperator() { return new GpuHashBuilderOperator(*this); }
+// HashBuilderOperatorFactory::CreateOperator (hash_builder.cpp:155-160): partitions round robin.
+Operator *GpuHashBuilderOperatorFactory::CreateOperator()
+{
+    return new GpuHashBuilderOperator(*this, nextOperator_++ % Partitions());
+}
 
 // HashBuilderOperator::AddInput (hash_builder.cpp:169-180): the batch is retained -- here its columns move to
 // device memory and the host batch is freed.
@@ -735,16 +750,18 @@ int32_t GpuHashBuilderOperator::AddInput(vec::VectorBatch *vecBatch)
     int32_t ncols = f_.buildTypes_.GetSize();
     if (rows > 0) {
         std::vector<omni_gpu_column> cols(ncols);
+        auto &retained = f_.retained_[partition_];
         for (int32_t c = 0; c < ncols; ++c) {
-            f_.retained_.emplace_back(new DeviceBuffer());
-            f_.retained_.emplace_back(new DeviceBuffer());
-            auto &vb = *f_.retained_[f_.retained_.size() - 2];
-            auto &nb = *f_.retained_.back();
+            retained.emplace_back(new DeviceBuffer());
+            retained.emplace_back(new DeviceBuffer());
+            auto &vb = *retained[retained.size() - 2];
+            auto &nb = *retained.back();
             cols[c] = Upload(vecBatch->Get(c), rows, vb, nb);
         }
         std::vector<omni_gpu_column> keys;
         for (int32_t k : f_.keyCols_) keys.push_back(cols[k]);
-        Check(omni_gpu_join_build_add(f_.handle_, keys.data(), cols.data(), rows), "omni_gpu_join_build_add");
+        Check(omni_gpu_join_build_add(f_.tables_[partition_], keys.data(), cols.data(), rows),
+              "omni_gpu_join_build_add");
     }
     vec::VectorHelper::FreeVecBatch(vecBatch);
     return 0;
@@ -754,10 +771,11 @@ int32_t GpuHashBuilderOperator::AddInput(vec::VectorBatch *vecBatch)
 int32_t GpuHashBuilderOperator::GetOutput(vec::VectorBatch **result)
 {
     *result = nullptr;
-    if (!f_.finished_) {
-        Check(omni_gpu_join_build_finish(f_.handle_), "omni_gpu_join_build_finish");
-        f_.finished_ = true;
-        f_.retained_.clear();
+    if (!f_.partFinished_[partition_]) {
+        Check(omni_gpu_join_build_finish(f_.tables_[partition_]), "omni_gpu_join_build_finish");
+        f_.partFinished_[partition_] = true;
+        ++f_.finishedCount_;
+        f_.retained_[partition_].clear();
     }
     SetStatus(OMNI_STATUS_FINISHED);
     return 0;
@@ -833,7 +851,9 @@ int32_t GpuLookupJoinOperator::OutputRowSize() const
 int32_t GpuLookupJoinOperator::AddInput(vec::VectorBatch *vecBatch)
 {
     if (!f_.builder_->Finished()) throw OmniException("OPERATOR_RUNTIME_ERROR", "probe before the build finished");
-    if (noMoreInput_ || f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) return ProbeOne(vecBatch);
+    if (noMoreInput_ || f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE || f_.builder_->Partitions() > 1) {
+        return ProbeOne(vecBatch);
+    }
     SetStatus(OMNI_STATUS_NORMAL);
     if (vecBatch->GetRowCount() <= 0) {
         vec::VectorHelper::FreeVecBatch(vecBatch);
@@ -1078,41 +1098,36 @@ int32_t GpuLookupJoinOperator::GetOutput(vec::VectorBatch **result)
     return 0;
 }
 
-// One batch probed on its own (the reference's flow, lookup_join.cpp:420-475).
-int32_t GpuLookupJoinOperator::ProbeOne(vec::VectorBatch *vecBatch)
+namespace {
+// Appends bits [0, n) of src after the first `at` bits of dst.
+void AppendBits(std::vector<uint8_t> &dst, int64_t at, const uint8_t *src, int64_t n)
 {
-    if (probe_) vec::VectorHelper::FreeVecBatch(probe_);
-    probe_ = vecBatch;
-    SetStatus(OMNI_STATUS_NORMAL);
-    int32_t rows = vecBatch->GetRowCount();
-    matches_ = emitted_ = 0;
-    if (rows <= 0) return 0;
+    dst.resize(static_cast<size_t>((at + n + 7) / 8 + 1), 0);
+    for (int64_t i = 0; i < n; ++i) {
+        if ((src[i >> 3] >> (i & 7)) & 1) dst[(at + i) >> 3] |= static_cast<uint8_t>(1u << ((at + i) & 7));
+    }
+}
+}  // namespace
+
+// Probes `rows` rows of device key columns against one table and appends the output rows to matchRows_ (probe
+// row ids, through rowMap when it is given) and matchPay_ / matchPayNulls_; EXISTENCE sets exists_ instead.
+void GpuLookupJoinOperator::ProbeTable(void *table, const std::vector<omni_gpu_column> &keys, int64_t rows,
+                                       const int64_t *rowMap)
+{
     int32_t is_array = 0;
     int64_t max_dup = 0, slots = 0;
-    Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
-    int64_t cap = std::max<int64_t>(1, rows * std::max<int64_t>(1, max_dup));
-    std::vector<omni_gpu_column> keys(f_.probeKeyCols_.size());
-    while (keyBufs_.size() < keys.size()) {
-        keyBufs_.emplace_back(new DeviceBuffer());
-        keyNullBufs_.emplace_back(new DeviceBuffer());
-    }
-    for (size_t k = 0; k < keys.size(); ++k) {
-        keys[k] = Upload(vecBatch->Get(f_.probeKeyCols_[k]), rows, *keyBufs_[k], *keyNullBufs_[k]);
-    }
+    Check(omni_gpu_join_build_info(table, &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
     if (f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE) {  // one output row per probe row
         rowsBuf_.Reserve(static_cast<size_t>(rows));
-        Check(omni_gpu_join_probe_exists(f_.builder_->Table(), keys.data(), rows,
-                                         static_cast<uint8_t *>(rowsBuf_.Get())),
+        Check(omni_gpu_join_probe_exists(table, keys.data(), rows, static_cast<uint8_t *>(rowsBuf_.Get())),
               "omni_gpu_join_probe_exists");
-        exists_.resize(rows);
-        Check(omni_gpu_memcpy_d2h(exists_.data(), rowsBuf_.Get(), static_cast<size_t>(rows)), "download flags");
-        matchRows_.resize(rows);
-        for (int32_t r = 0; r < rows; ++r) matchRows_[r] = r;
-        matches_ = rows;
-        return 0;
+        std::vector<uint8_t> flags(static_cast<size_t>(rows));
+        Check(omni_gpu_memcpy_d2h(flags.data(), rowsBuf_.Get(), static_cast<size_t>(rows)), "download flags");
+        for (int64_t r = 0; r < rows; ++r) exists_[rowMap ? rowMap[r] : r] = flags[r];
+        return;
     }
     // outer joins emit a row per unmatched probe row as well
-    cap = std::max<int64_t>(cap, rows);
+    const int64_t cap = std::max<int64_t>(std::max<int64_t>(1, rows), rows * std::max<int64_t>(1, max_dup));
     rowsBuf_.Reserve(sizeof(int64_t) * cap);
     const auto &btypes = f_.builder_->BuildTypes();
     int32_t nb = btypes.GetSize();
@@ -1128,21 +1143,101 @@ int32_t GpuLookupJoinOperator::ProbeOne(vec::VectorBatch *vecBatch)
         outs[col].values = payBufs_[col]->Get();
         outs[col].nulls = static_cast<uint8_t *>(payNullBufs_[col]->Get());
     }
-    Check(omni_gpu_join_probe(f_.builder_->Table(), keys.data(), rows, 0, static_cast<int64_t *>(rowsBuf_.Get()), outs.data(),
-                              cap, &matches_),
+    int64_t m = 0;
+    Check(omni_gpu_join_probe(table, keys.data(), rows, 0, static_cast<int64_t *>(rowsBuf_.Get()), outs.data(), cap, &m),
           "omni_gpu_join_probe");
-    matchRows_.resize(matches_);
-    Check(omni_gpu_memcpy_d2h(matchRows_.data(), rowsBuf_.Get(), sizeof(int64_t) * matches_), "download rows");
-    matchPay_.assign(f_.buildOutputCols_.size(), {});
-    matchPayNulls_.assign(f_.buildOutputCols_.size(), {});
+    const int64_t at = matches_;
+    matchRows_.resize(static_cast<size_t>(at + m));
+    Check(omni_gpu_memcpy_d2h(matchRows_.data() + at, rowsBuf_.Get(), sizeof(int64_t) * m), "download rows");
+    if (rowMap) {
+        for (int64_t k = at; k < at + m; ++k) matchRows_[k] = rowMap[matchRows_[k]];
+    }
+    std::vector<uint8_t> bits;
     for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
         int32_t col = f_.buildOutputCols_[i];
         int w = type::TypeWidth(btypes.GetType(col)->GetId());
-        matchPay_[i].resize(static_cast<size_t>(matches_) * w);
-        matchPayNulls_[i].resize((matches_ + 7) / 8 + 1);
-        Check(omni_gpu_memcpy_d2h(matchPay_[i].data(), payBufs_[col]->Get(), matchPay_[i].size()), "download payload");
-        Check(omni_gpu_memcpy_d2h(matchPayNulls_[i].data(), payNullBufs_[col]->Get(), (matches_ + 7) / 8),
-              "download payload nulls");
+        matchPay_[i].resize(static_cast<size_t>(at + m) * w);
+        Check(omni_gpu_memcpy_d2h(matchPay_[i].data() + static_cast<size_t>(at) * w, payBufs_[col]->Get(),
+                                  static_cast<size_t>(m) * w),
+              "download payload");
+        bits.assign(static_cast<size_t>((m + 7) / 8 + 1), 0);
+        if (m > 0) {
+            Check(omni_gpu_memcpy_d2h(bits.data(), payNullBufs_[col]->Get(), (m + 7) / 8), "download payload nulls");
+        }
+        AppendBits(matchPayNulls_[i], at, bits.data(), m);
+    }
+    matches_ = at + m;
+}
+
+// One batch probed on its own (the reference's flow, lookup_join.cpp:420-475).  With several builder partitions
+// each row probes the table of its hash partition (ProbeBatchForInnerJoin with partitionMask, :950-1000): the
+// rows are grouped by partition, each group's keys gathered on the device and probed against its table.
+int32_t GpuLookupJoinOperator::ProbeOne(vec::VectorBatch *vecBatch)
+{
+    if (probe_) vec::VectorHelper::FreeVecBatch(probe_);
+    probe_ = vecBatch;
+    SetStatus(OMNI_STATUS_NORMAL);
+    int32_t rows = vecBatch->GetRowCount();
+    matches_ = emitted_ = 0;
+    matchRows_.clear();
+    matchPay_.assign(f_.buildOutputCols_.size(), {});
+    matchPayNulls_.assign(f_.buildOutputCols_.size(), {});
+    if (rows <= 0) return 0;
+    const size_t nk = f_.probeKeyCols_.size();
+    std::vector<omni_gpu_column> keys(nk);
+    while (keyBufs_.size() < 2 * nk) {
+        keyBufs_.emplace_back(new DeviceBuffer());
+        keyNullBufs_.emplace_back(new DeviceBuffer());
+    }
+    for (size_t k = 0; k < nk; ++k) {
+        keys[k] = Upload(vecBatch->Get(f_.probeKeyCols_[k]), rows, *keyBufs_[k], *keyNullBufs_[k]);
+    }
+    const bool existence = f_.builder_->GetJoinType() == OMNI_JOIN_TYPE_EXISTENCE;
+    if (existence) exists_.assign(static_cast<size_t>(rows), 0);
+    const int32_t P = f_.builder_->Partitions();
+    if (P == 1) {
+        ProbeTable(f_.builder_->Table(), keys, rows, nullptr);
+    } else {
+        DeviceBuffer parts;
+        parts.Reserve(sizeof(uint32_t) * rows);
+        Check(omni_gpu_join_partition_ids(keys.data(), static_cast<int32_t>(nk), rows, static_cast<uint32_t>(P - 1),
+                                          static_cast<uint32_t *>(parts.Get())),
+              "omni_gpu_join_partition_ids");
+        std::vector<uint32_t> part(static_cast<size_t>(rows));
+        Check(omni_gpu_memcpy_d2h(part.data(), parts.Get(), sizeof(uint32_t) * rows), "download partitions");
+        std::vector<int64_t> start(static_cast<size_t>(P) + 1, 0), idx(static_cast<size_t>(rows));
+        for (uint32_t q : part) ++start[q + 1];
+        for (int32_t q = 0; q < P; ++q) start[q + 1] += start[q];
+        std::vector<int64_t> fill(start.begin(), start.end() - 1);
+        for (int32_t r = 0; r < rows; ++r) idx[fill[part[r]]++] = r;
+        DeviceBuffer didx;
+        didx.Reserve(sizeof(int64_t) * rows);
+        Check(omni_gpu_memcpy_h2d(didx.Get(), idx.data(), sizeof(int64_t) * rows), "upload row groups");
+        std::vector<omni_gpu_column> sub(nk);
+        std::vector<omni_gpu_out_column> gout(nk);
+        for (int32_t q = 0; q < P; ++q) {
+            const int64_t n = start[q + 1] - start[q];
+            if (n == 0) continue;
+            for (size_t k = 0; k < nk; ++k) {
+                const int w = type::TypeWidth(keys[k].type_id);
+                keyBufs_[nk + k]->Reserve(static_cast<size_t>(n) * w);
+                keyNullBufs_[nk + k]->Reserve(4 * ((n + 31) / 32));
+                gout[k].type_id = keys[k].type_id;
+                gout[k].values = keyBufs_[nk + k]->Get();
+                gout[k].nulls = static_cast<uint8_t *>(keyNullBufs_[nk + k]->Get());
+                sub[k] = omni_gpu_column{keys[k].type_id, 0, keyBufs_[nk + k]->Get(),
+                                         static_cast<const uint8_t *>(keyNullBufs_[nk + k]->Get())};
+            }
+            Check(omni_gpu_join_gather_rows(keys.data(), static_cast<int32_t>(nk),
+                                            static_cast<const int64_t *>(didx.Get()) + start[q], n, gout.data()),
+                  "omni_gpu_join_gather_rows");
+            ProbeTable(f_.builder_->Table(q), sub, n, idx.data() + start[q]);
+        }
+    }
+    if (existence) {
+        matchRows_.resize(rows);
+        for (int32_t r = 0; r < rows; ++r) matchRows_[r] = r;
+        matches_ = rows;
     }
     return 0;
 }
@@ -1247,41 +1342,46 @@ int32_t GpuLookupOuterJoinOperator::GetOutput(vec::VectorBatch **result)
     if (!f_.builder_->Finished()) throw OmniException("OPERATOR_RUNTIME_ERROR", "outer output before the build finished");
     const auto &btypes = f_.builder_->BuildTypes();
     if (!fetched_) {
+        // every builder partition's unmatched build rows, partition after partition
         fetched_ = true;
-        int32_t is_array = 0;
-        int64_t max_dup = 0, slots = 0;
-        Check(omni_gpu_join_build_info(f_.builder_->Table(), &is_array, &max_dup, &slots), "omni_gpu_join_build_info");
-        int32_t nb = btypes.GetSize();
-        std::vector<omni_gpu_out_column> outs(nb);
-        for (int32_t c = 0; c < nb; ++c) outs[c].type_id = btypes.GetType(c)->GetId();
-        // first call with no capacity returns the row count
-        int64_t rows = 0;
-        int rc = omni_gpu_join_outer_output(f_.builder_->Table(), nullptr, outs.data(), 0, &rows);
-        if (rc != OMNI_GPU_OK && rc != OMNI_GPU_ERR_OUTPUT_FULL) Check(rc, "omni_gpu_join_outer_output");
-        rows_ = rows;
-        std::vector<std::unique_ptr<DeviceBuffer>> vals, nulls;
-        for (int32_t col : f_.buildOutputCols_) {
-            vals.emplace_back(new DeviceBuffer());
-            nulls.emplace_back(new DeviceBuffer());
-            vals.back()->Reserve(std::max<size_t>(1, static_cast<size_t>(rows) * type::TypeWidth(outs[col].type_id)));
-            nulls.back()->Reserve(4 * ((rows + 31) / 32) + 4);
-            outs[col].values = vals.back()->Get();
-            outs[col].nulls = static_cast<uint8_t *>(nulls.back()->Get());
-        }
-        if (rows > 0) {
-            Check(omni_gpu_join_outer_output(f_.builder_->Table(), nullptr, outs.data(), rows, &rows),
-                  "omni_gpu_join_outer_output");
-        }
+        rows_ = 0;
         pay_.assign(f_.buildOutputCols_.size(), {});
         payNulls_.assign(f_.buildOutputCols_.size(), {});
-        for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
-            int w = type::TypeWidth(outs[f_.buildOutputCols_[i]].type_id);
-            pay_[i].resize(static_cast<size_t>(rows) * w);
-            payNulls_[i].resize((rows + 7) / 8 + 1);
-            if (rows > 0) {
-                Check(omni_gpu_memcpy_d2h(pay_[i].data(), vals[i]->Get(), pay_[i].size()), "download payload");
-                Check(omni_gpu_memcpy_d2h(payNulls_[i].data(), nulls[i]->Get(), (rows + 7) / 8), "download nulls");
+        int32_t nb = btypes.GetSize();
+        std::vector<uint8_t> bits;
+        for (int32_t q = 0; q < f_.builder_->Partitions(); ++q) {
+            void *table = f_.builder_->Table(q);
+            std::vector<omni_gpu_out_column> outs(nb);
+            for (int32_t c = 0; c < nb; ++c) outs[c].type_id = btypes.GetType(c)->GetId();
+            // first call with no capacity returns the row count
+            int64_t rows = 0;
+            int rc = omni_gpu_join_outer_output(table, nullptr, outs.data(), 0, &rows);
+            if (rc != OMNI_GPU_OK && rc != OMNI_GPU_ERR_OUTPUT_FULL) Check(rc, "omni_gpu_join_outer_output");
+            std::vector<std::unique_ptr<DeviceBuffer>> vals, nulls;
+            for (int32_t col : f_.buildOutputCols_) {
+                vals.emplace_back(new DeviceBuffer());
+                nulls.emplace_back(new DeviceBuffer());
+                vals.back()->Reserve(std::max<size_t>(1, static_cast<size_t>(rows) * type::TypeWidth(outs[col].type_id)));
+                nulls.back()->Reserve(4 * ((rows + 31) / 32) + 4);
+                outs[col].values = vals.back()->Get();
+                outs[col].nulls = static_cast<uint8_t *>(nulls.back()->Get());
             }
+            if (rows > 0) {
+                Check(omni_gpu_join_outer_output(table, nullptr, outs.data(), rows, &rows), "omni_gpu_join_outer_output");
+            }
+            for (size_t i = 0; i < f_.buildOutputCols_.size(); ++i) {
+                int w = type::TypeWidth(outs[f_.buildOutputCols_[i]].type_id);
+                pay_[i].resize(static_cast<size_t>(rows_ + rows) * w);
+                bits.assign(static_cast<size_t>((rows + 7) / 8 + 1), 0);
+                if (rows > 0) {
+                    Check(omni_gpu_memcpy_d2h(pay_[i].data() + static_cast<size_t>(rows_) * w, vals[i]->Get(),
+                                              static_cast<size_t>(rows) * w),
+                          "download payload");
+                    Check(omni_gpu_memcpy_d2h(bits.data(), nulls[i]->Get(), (rows + 7) / 8), "download nulls");
+                }
+                AppendBits(payNulls_[i], rows_, bits.data(), rows);
+            }
+            rows_ += rows;
         }
     }
     if (emitted_ >= rows_) {

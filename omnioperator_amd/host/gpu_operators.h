@@ -186,9 +186,12 @@ public:
                                                                            int32_t operatorCount);
     ~GpuHashBuilderOperatorFactory() override;
     Operator *CreateOperator() override;
-    void *Table() const { return handle_; }
+    // One GPU table per builder partition (operatorCount, hash_builder.cpp:155-160): builder operator i fills
+    // table i % operatorCount; a probe row looks only in the table of its hash partition.
+    void *Table(int32_t partition = 0) const { return tables_.at(partition); }
+    int32_t Partitions() const { return static_cast<int32_t>(tables_.size()); }
     const type::DataTypes &BuildTypes() const { return buildTypes_; }
-    bool Finished() const { return finished_; }
+    bool Finished() const { return finishedCount_ == Partitions(); }
     JoinType GetJoinType() const { return joinType_; }
     BuildSide GetBuildSide() const { return buildSide_; }
     const std::vector<int32_t> &KeyCols() const { return keyCols_; }
@@ -196,24 +199,28 @@ public:
 private:
     friend class GpuHashBuilderOperator;
     GpuHashBuilderOperatorFactory(JoinType joinType, BuildSide buildSide, const type::DataTypes &buildTypes,
-                                  std::vector<int32_t> keyCols);
+                                  std::vector<int32_t> keyCols, int32_t operatorCount);
     JoinType joinType_;
     BuildSide buildSide_;
     type::DataTypes buildTypes_;
     std::vector<int32_t> keyCols_;  // buildHashCols
-    void *handle_ = nullptr;
-    bool finished_ = false;
-    std::vector<std::unique_ptr<DeviceBuffer>> retained_;  // build batches in device memory until finish
+    std::vector<void *> tables_;
+    std::vector<bool> partFinished_;
+    int32_t finishedCount_ = 0;
+    int32_t nextOperator_ = 0;
+    // per partition: its build batches in device memory until its table is built
+    std::vector<std::vector<std::unique_ptr<DeviceBuffer>>> retained_;
 };
 
 class GpuHashBuilderOperator : public Operator {
 public:
-    explicit GpuHashBuilderOperator(GpuHashBuilderOperatorFactory &f) : f_(f) {}
+    GpuHashBuilderOperator(GpuHashBuilderOperatorFactory &f, int32_t partition) : f_(f), partition_(partition) {}
     int32_t AddInput(vec::VectorBatch *vecBatch) override;
     int32_t GetOutput(vec::VectorBatch **result) override;
 
 private:
     GpuHashBuilderOperatorFactory &f_;
+    int32_t partition_;
 };
 
 class GpuLookupJoinOperatorFactory : public OperatorFactory {
@@ -242,7 +249,8 @@ private:
 // 2^20, at most 2^24 rows) -- or the rest, once noMoreInput() is called -- is probed in one call (the
 // radix-partitioned chain for large tables), the probe output columns are gathered on the device, and GetOutput
 // emits the run's rows before the operator takes more input (needsInput() is false meanwhile).  Outputs are the
-// same rows either way; only their grouping into batches differs.
+// same rows either way; only their grouping into batches differs.  With several builder partitions each probe row
+// probes the table of its hash partition, one batch at a time.
 class GpuLookupJoinOperator : public Operator {
 public:
     explicit GpuLookupJoinOperator(const GpuLookupJoinOperatorFactory &f) : f_(f) {}
@@ -261,6 +269,7 @@ private:
         HostBuffer values, nulls;
     };
     int32_t ProbeOne(vec::VectorBatch *vecBatch);
+    void ProbeTable(void *table, const std::vector<omni_gpu_column> &keys, int64_t rows, const int64_t *rowMap);
     void Stage(vec::VectorBatch *vecBatch);
     void FlushRun();
     vec::VectorBatch *EmitRun();

@@ -1231,6 +1231,150 @@ static void TestJoinBufferedRunsInner() { BufferedVersusPerBatch(OMNI_JOIN_TYPE_
 static void TestJoinBufferedRunsLeft() { BufferedVersusPerBatch(OMNI_JOIN_TYPE_LEFT, 200000); }
 static void TestJoinBufferedRunsAnti() { BufferedVersusPerBatch(OMNI_JOIN_TYPE_LEFT_ANTI, 200000); }
 
+
+// ---- several builder partitions (operatorCount > 1) ------------------------------------------------------------
+
+// Hash-table partitions of LONG keys (NULL where null[i]) from the library (pinned against the oracle's HashUtil
+// restatement by tests/test_gpu_join.py::test_join_partition_ids_match_oracle).
+static std::vector<uint32_t> KeyPartitions(const std::vector<int64_t> &keys, const std::vector<bool> &null, uint32_t mask)
+{
+    const int64_t n = static_cast<int64_t>(keys.size());
+    void *dv = nullptr, *dn = nullptr, *dp = nullptr;
+    std::vector<uint8_t> bits((n + 7) / 8 + 1, 0);
+    for (int64_t i = 0; i < n; ++i) {
+        if (null[i]) bits[i >> 3] |= static_cast<uint8_t>(1u << (i & 7));
+    }
+    omni_gpu_malloc(&dv, 8 * n);
+    omni_gpu_malloc(&dn, bits.size());
+    omni_gpu_malloc(&dp, 4 * n);
+    omni_gpu_memcpy_h2d(dv, keys.data(), 8 * n);
+    omni_gpu_memcpy_h2d(dn, bits.data(), bits.size());
+    omni_gpu_column c{OMNI_LONG, 0, dv, static_cast<const uint8_t *>(dn)};
+    std::vector<uint32_t> out(n);
+    EXPECT_TRUE(omni_gpu_join_partition_ids(&c, 1, n, mask, static_cast<uint32_t *>(dp)) == 0);
+    omni_gpu_memcpy_d2h(out.data(), dp, 4 * n);
+    omni_gpu_free(dv);
+    omni_gpu_free(dn);
+    omni_gpu_free(dp);
+    return out;
+}
+
+// A LONG-key join over P builder operators (HashBuilderOperatorFactory with operatorCount = P): build rows
+// either routed to the builder of their key's hash partition or dealt out by batch regardless of the key; every
+// probe row only sees the build rows of its own partition's builder (lookup_join.cpp:950-1000), checked against
+// a brute force over the same assignment, and -- when routed -- against the one-table join.
+static Rows MultiPartitionJoin(JoinType jt, int32_t P, bool routed, Rows *expected)
+{
+    const int32_t nb = 900, np = 2500;
+    std::vector<int64_t> bk(nb), bv(nb), pk(np), pv(np);
+    std::vector<bool> bnull(nb, false), pnull(np, false);
+    for (int32_t i = 0; i < nb; ++i) {
+        bk[i] = (i * 37) % 400;  // keys with two or three build rows each
+        bv[i] = 1000 + i;
+        bnull[i] = i % 53 == 0;
+    }
+    for (int32_t i = 0; i < np; ++i) {
+        pk[i] = (i * 7919) % 600;
+        pv[i] = i;
+        pnull[i] = i % 41 == 0;
+    }
+    std::vector<int32_t> owner(nb);
+    if (routed) {
+        std::vector<uint32_t> q = KeyPartitions(bk, bnull, static_cast<uint32_t>(P - 1));
+        for (int32_t i = 0; i < nb; ++i) owner[i] = static_cast<int32_t>(q[i]);
+    } else {
+        for (int32_t i = 0; i < nb; ++i) owner[i] = (i / 50) % P;
+    }
+    DataTypes types({LongType(), LongType()});
+    int32_t keyCols[1] = {0};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(jt, types, keyCols, 1, P);
+    std::vector<Operator *> builders;
+    for (int32_t q = 0; q < P; ++q) builders.push_back(bf->CreateOperator());  // partitions 0..P-1
+    for (int32_t q = 0; q < P; ++q) {
+        std::vector<int64_t> k, v;
+        std::vector<int> nulls;
+        for (int32_t i = 0; i < nb; ++i) {
+            if (owner[i] != q) continue;
+            if (bnull[i]) nulls.push_back(static_cast<int>(k.size()));
+            k.push_back(bk[i]);
+            v.push_back(bv[i]);
+        }
+        if (!k.empty()) {
+            auto *b = new VectorBatch(static_cast<int32_t>(k.size()));
+            auto *kv = MakeVector<int64_t>(OMNI_LONG, k);
+            for (int r : nulls) kv->SetNull(r);
+            b->Append(kv);
+            b->Append(MakeVector<int64_t>(OMNI_LONG, v));
+            builders[q]->AddInput(b);
+        }
+        VectorBatch *none = nullptr;
+        EXPECT_TRUE(!bf->Finished() || q == P - 1);
+        builders[q]->GetOutput(&none);
+    }
+    EXPECT_TRUE(bf->Finished());
+    int32_t probeOut[2] = {0, 1}, buildOut[1] = {1};
+    DataTypes buildOutTypes({LongType()});
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        types, probeOut, 2, keyCols, 1, buildOut, 1, buildOutTypes, reinterpret_cast<int64_t>(bf), nullptr, false,
+        nullptr);
+    Operator *lookup = lf->CreateOperator();
+    Rows got;
+    for (int32_t a = 0; a < np; a += 1000) {
+        const int32_t n = std::min(1000, np - a);
+        auto *b = new VectorBatch(n);
+        auto *kv = MakeVector<int64_t>(OMNI_LONG, std::vector<int64_t>(pk.begin() + a, pk.begin() + a + n));
+        for (int32_t r = 0; r < n; ++r) {
+            if (pnull[a + r]) kv->SetNull(r);
+        }
+        b->Append(kv);
+        b->Append(MakeVector<int64_t>(OMNI_LONG, std::vector<int64_t>(pv.begin() + a, pv.begin() + a + n)));
+        lookup->AddInput(b);
+        Rows r = Drain(lookup);
+        got.insert(got.end(), r.begin(), r.end());
+    }
+    if (expected) {
+        std::vector<uint32_t> pq = KeyPartitions(pk, pnull, static_cast<uint32_t>(P - 1));
+        expected->clear();
+        for (int32_t i = 0; i < np; ++i) {
+            bool any = false;
+            for (int32_t j = 0; j < nb && !pnull[i]; ++j) {
+                if (!bnull[j] && bk[j] == pk[i] && owner[j] == static_cast<int32_t>(pq[i])) {
+                    expected->push_back({std::to_string(pk[i]), std::to_string(pv[i]), std::to_string(bv[j])});
+                    any = true;
+                }
+            }
+            if (!any && jt == OMNI_JOIN_TYPE_LEFT) {
+                expected->push_back({pnull[i] ? "null" : std::to_string(pk[i]), std::to_string(pv[i]), "null"});
+            }
+        }
+    }
+    Operator::DeleteOperator(lookup);
+    for (auto *b : builders) Operator::DeleteOperator(b);
+    delete lf;
+    delete bf;
+    return got;
+}
+
+static void TestJoinMultiPartitionRouted()
+{
+    Rows want, one = MultiPartitionJoin(OMNI_JOIN_TYPE_INNER, 1, false, nullptr);
+    for (int32_t P : {2, 4}) {
+        Rows got = MultiPartitionJoin(OMNI_JOIN_TYPE_INNER, P, true, &want);
+        EXPECT_TRUE(SameRowSet(got, want));
+        EXPECT_TRUE(SameRowSet(got, one));  // rows routed by the probe's own partition function: one table's rows
+    }
+}
+
+static void TestJoinMultiPartitionUnrouted()
+{
+    for (JoinType jt : {OMNI_JOIN_TYPE_INNER, OMNI_JOIN_TYPE_LEFT}) {
+        Rows want;
+        Rows got = MultiPartitionJoin(jt, 4, false, &want);
+        EXPECT_TRUE(!want.empty());
+        EXPECT_TRUE(SameRowSet(got, want));
+    }
+}
+
 int main()
 {
     std::vector<std::pair<const char *, std::function<void()>>> tests = {
@@ -1261,6 +1405,8 @@ int main()
         {"JoinBufferedRunsInner", TestJoinBufferedRunsInner},
         {"JoinBufferedRunsLeft", TestJoinBufferedRunsLeft},
         {"JoinBufferedRunsAnti", TestJoinBufferedRunsAnti},
+        {"JoinMultiPartitionRouted", TestJoinMultiPartitionRouted},
+        {"JoinMultiPartitionUnrouted", TestJoinMultiPartitionUnrouted},
     };
     for (auto &t : tests) {
         int before = g_failures;

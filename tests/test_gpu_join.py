@@ -436,3 +436,64 @@ def test_gather_rows_matches_numpy():
         bits = np.unpackbits(keep[2 * i + 1].numpy(), bitorder="little")[:n].astype(bool)
         np.testing.assert_array_equal(bits, nulls[rows])
         np.testing.assert_array_equal(got, vals[rows])
+
+
+def _probe_partitions_oracle(cols, mask):
+    """PopulateProbeHashes + GetRawHashPartition from the oracle's KAT-pinned HashUtil primitives
+    (hash_value_long / _double / _decimal128, raw_hash_partition): h = 0, then h = 31 h + HashValue(v) for each
+    non-NULL key column, wrapping in 64 bits."""
+    n = len(cols[0][1])
+    out = np.zeros(n, np.uint32)
+    for r in range(n):
+        h = 0
+        for t, vals, nulls in cols:
+            if nulls[r]:
+                continue
+            if t == BOOLEAN:
+                hv = 1231 if vals[r] else 1237
+            elif t == oracle.DECIMAL64:
+                hv = int(vals[r])
+            elif t == DOUBLE:
+                hv = oracle.hash_value_double(float(vals[r]))
+            elif t == oracle.DECIMAL128:
+                hv = oracle.hash_value_decimal128(int(vals[r][0]), int(vals[r][1]))
+            else:
+                hv = oracle.hash_value_i64(int(vals[r]))
+            h = (31 * h + hv) & (2**64 - 1)
+        out[r] = oracle.raw_hash_partition(h - 2**64 if h >= 2**63 else h, mask)
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("types", [(LONG,), (INT, LONG), (SHORT, DOUBLE, BOOLEAN), (oracle.DECIMAL64,),
+                                   (oracle.DECIMAL128, INT)])
+def test_join_partition_ids_match_oracle(types):
+    """omni_gpu_join_partition_ids equals the oracle's composition of the reference's hash primitives, NULLs
+    skipped, for 1 to 3 key columns (DOUBLE with NaN payloads and -0.0)."""
+    rng = np.random.default_rng(41 + len(types))
+    n = 3000
+    cols, dcols = [], []
+    for t in types:
+        if t == DOUBLE:
+            v = rng.normal(0, 1e6, n)
+            v[::17] = np.nan
+            v[5::19] = -0.0
+            v[7::23] = np.frombuffer(np.array([0x7FF0000000000001], np.uint64).tobytes(), np.float64)[0]
+        elif t == BOOLEAN:
+            v = rng.integers(0, 2, n).astype(np.uint8)
+        elif t == SHORT:
+            v = rng.integers(-2**15, 2**15, n).astype(np.int16)
+        elif t == INT:
+            v = rng.integers(-2**31, 2**31, n).astype(np.int32)
+        elif t == oracle.DECIMAL128:
+            v = rng.integers(-2**62, 2**62, (n, 2), dtype=np.int64)
+        else:
+            v = rng.integers(-2**62, 2**62, n, dtype=np.int64)
+        nulls = rng.random(n) < 0.1
+        cols.append((t, v, nulls))
+        dcols.append(capi.DeviceColumn(t, v, nulls))
+    out = capi.DeviceArray(4 * n)
+    arr = (capi.Column * len(types))(*[c.c() for c in dcols])
+    capi.check(capi.lib().omni_gpu_join_partition_ids(arr, len(types), n, 7, out.ptr))
+    got = out.numpy().view(np.uint32)[:n]
+    np.testing.assert_array_equal(got, _probe_partitions_oracle(cols, 7))
