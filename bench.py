@@ -87,9 +87,11 @@ def bench_join(args):
     bcols = [((capi.Column * len(bkey))(*[k.c(a) for k in bkey]), (capi.Column * 1)(bpay.c(a)), b - a)
              for a, b in bspans]
 
+    tuned = {"probe_group": args.probe_group, "partitions": args.partitions}
+
     def step():
         j = capi.HashJoin(ktypes if len(ktypes) > 1 else ktypes[0], [capi.INT])
-        j.tune(args.probe_group, args.partitions)
+        j.tune(tuned["probe_group"], tuned["partitions"])
         if jt != capi.JOIN_INNER:
             j.set_type(jt, bs)
         for kc, pc, n in bcols:
@@ -104,6 +106,18 @@ def bench_join(args):
         return m
 
     parts, is_array, outer = [0], [False], [0]
+
+    if args.join_autotune:
+        # probe width and partition count from the autotuner (cached per key types and size buckets)
+        from omnioperator_amd import autotune
+
+        def run(cfg):
+            tuned.update(probe_group=cfg.probe_group, partitions=cfg.partitions)
+            step()
+            capi.synchronize()
+
+        best = autotune.tune_join(run, autotune.join_key_of(ktypes, nb, npr))
+        tuned.update(probe_group=best.probe_group, partitions=best.partitions)
 
     for _ in range(args.warmup):
         m = step()
@@ -153,7 +167,8 @@ def bench_join(args):
                 "kernel": "join_probe" if parts[0] <= 1 else
                           f"join_probe_* (radix-partitioned, {parts[0]} partitions: hist+scan+scatter+probe)",
                 "kernel_ms": round(avg_probe_ms, 4), "algorithmic_bytes": probe_bytes}
-    extra = {"build_kernel_ms": round(build_ms / prof_steps, 4), "matches": m, "probe_group": args.probe_group,
+    extra = {"build_kernel_ms": round(build_ms / prof_steps, 4), "matches": m, "probe_group": tuned["probe_group"],
+             "partitions_setting": tuned["partitions"], "autotuned": bool(args.join_autotune),
              "probe_partitions": parts[0], "join_type": args.join_type}
     if keeps_build:
         extra["outer_rows"] = outer[0]
@@ -750,6 +765,8 @@ def main():
     p.add_argument("--join-keys", default="single", choices=["single", "int_int", "long_long", "decimal128"],
                    help="join workload: the key as one int64 column or as several key columns / a DECIMAL128")
     p.add_argument("--partitions", type=int, default=-1, help="radix-partitioned probe: -1 auto, 0 off, or P")
+    p.add_argument("--join-autotune", action="store_true",
+                   help="join workload: probe width and partition count from the autotuner (cached sweep)")
     p.add_argument("--calibrate-copy-mib", type=int, default=0,
                    help="only run 3 device-to-device copies of this many MiB (HBM counter calibration)")
     args = p.parse_args()

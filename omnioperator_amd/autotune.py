@@ -1,11 +1,14 @@
-"""Autotuner for the batched group-by: picks threads per workgroup, LDS table bytes, the LDS probe limit and the
-phase-2 partition table bytes (for the one-workgroup-per-batch form: threads and its table bytes) by
-timing candidate runs on the stream itself, per (key type, aggregate signature, cardinality bucket); winners
-are cached in a JSON file (OMNI_GPU_AUTOTUNE_CACHE, default ~/.cache/omnioperator_amd/autotune.json) so later
-runs of the same kind skip the sweep.
+"""Autotuner.  For the batched group-by it picks threads per workgroup, LDS table bytes, the LDS probe limit and
+the phase-2 partition table bytes (for the one-workgroup-per-batch form: threads and its table bytes), per (key
+type, aggregate signature, cardinality bucket); for the hash join the probe width (lookups in flight per lane of
+the direct probe) and the radix partition count of the probe (the window probe's region size), per (key types,
+build-rows bucket, probe-to-build ratio bucket).  Both time candidate runs of the workload itself; winners are
+cached in a JSON file (OMNI_GPU_AUTOTUNE_CACHE, default ~/.cache/omnioperator_amd/autotune.json) so later runs of
+the same kind skip the sweep.
 
 Tuning never changes results: every candidate is an exact configuration of the same kernels (a batch that
-does not fit the LDS table is aggregated exactly in HBM).
+does not fit the LDS table is aggregated exactly in HBM; every probe width and partition count finds the same
+matches).
 """
 from __future__ import annotations
 
@@ -114,3 +117,58 @@ def tune_stream(stream, run, key: str, candidates: List[StreamConfig] = CANDIDAT
     d[key] = {"config": asdict(best), "seconds": best_t, "swept": results}
     _store(d)
     return best
+
+
+@dataclass(frozen=True)
+class JoinConfig:
+    probe_group: int = 1   # lookups in flight per lane of the direct probe (omni_gpu_join_tune)
+    partitions: int = -1   # radix partitions of the probe: -1 the library's rule, 0 none, else P
+
+
+# the direct probe at every width, the library's rule, and the partition counts whose regions fit the LDS window
+# of tables from 2^20 to 2^26 slots
+JOIN_CANDIDATES: List[JoinConfig] = (
+    [JoinConfig(g, 0) for g in (1, 2, 4, 8, 16)] + [JoinConfig(1, -1)] +
+    [JoinConfig(1, p) for p in (256, 512, 1024, 2048, 4096)]
+)
+
+
+def join_key_of(key_types: Iterable[int], build_rows: int, probe_rows: int) -> str:
+    """Cache key of a join: its key types, build rows (power-of-two bucket) and probe / build ratio bucket."""
+    b = int(math.log2(max(1, build_rows)))
+    r = int(math.log2(max(1, probe_rows // max(1, build_rows)) + 1))
+    return "join|" + ",".join(str(t) for t in key_types) + f"|b{b}|r{r}"
+
+
+def tune_join(run, key: str, candidates: List[JoinConfig] = JOIN_CANDIDATES, repeats: int = 2,
+              use_cache: bool = True) -> JoinConfig:
+    """run(cfg): one build + probe of the workload with the probe tuned to cfg (synchronizing); raising for a
+    candidate the table cannot take (e.g. a partition count with regions smaller than 64 slots) skips it.  Returns
+    the fastest configuration."""
+    if use_cache:
+        e = _load().get(key)
+        if e:
+            return JoinConfig(**e["config"])
+    best, best_t, results = None, float("inf"), []
+    for cfg in candidates:
+        try:
+            run(cfg)  # warm
+            ts = []
+            for _ in range(repeats):
+                t0 = time.perf_counter()
+                run(cfg)
+                ts.append(time.perf_counter() - t0)
+        except Exception as e:  # a candidate the library rejects for this table
+            results.append((asdict(cfg), str(e)))
+            continue
+        t = float(np.median(ts))
+        results.append((asdict(cfg), t))
+        if t < best_t:
+            best, best_t = cfg, t
+    if best is None:
+        raise RuntimeError("no join candidate ran")
+    d = _load()
+    d[key] = {"config": asdict(best), "seconds": best_t, "swept": results}
+    _store(d)
+    return best
+
