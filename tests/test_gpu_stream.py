@@ -14,7 +14,8 @@ pytestmark = [
     pytest.mark.skip(reason="the first persistent-queue stream kernel (agg_stream) took a GPU memory fault on its "
                             "first run; its indexing is clean in a concurrent CPU build under AddressSanitizer and no "
                             "cause is shown, so that kernel and its entry points (omni_gpu_agg_stream_*) were "
-                            "removed; the batched path is rebuilt as agg_batch_* (tests/test_gpu_batches.py)"),
+                            "removed; the batched path is rebuilt as agg_batch_* and its persistent queue-draining "
+                            "kernel as batch_fused (tests/test_gpu_batches.py)"),
 ]
 
 AGGS = [(SUM, LONG, LONG), (COUNT_ALL, LONG, LONG), (MIN, INT, INT), (AVG, DOUBLE, DOUBLE),
