@@ -503,15 +503,16 @@ __device__ __forceinline__ void BatchFused(const BatchConfig &c, const Table &t,
     const int fit = part_fit_set ? c.part_fit : static_cast<int>(kFusedLoad * static_cast<float>(smax));
     const bool forced = c.modes != nullptr && c.modes[b] == 2;
     const float *pred = c.pred;
-    // 1. plan: direct when the rows (or the predicted groups) fit one table
+    // 1. plan: direct when the rows (or the predicted groups) fit one table; a batch whose keys fill at most
+    //    kDirectLoad of the largest table goes direct (a probe run is still short), and one of at most that many
+    //    rows has at most that many keys, so it needs no estimate
+    const float directMax = part_fit_set ? static_cast<float>(fit) : kDirectLoad * static_cast<float>(smax);
     bool triedDirect = false;
-    if (!forced && n <= fit) {
+    if (!forced && static_cast<float>(n) <= fmaxf(static_cast<float>(fit), directMax)) {
         // at most n keys: the table cannot overflow unless part_fit sized partitions above the table
         if (FusedDirect<BLOCK>(c, in, t, b, start, end, FusedSlots(static_cast<float>(n), smax), sh)) return;
         triedDirect = true;
     }
-    // a batch whose keys fill at most kDirectLoad of the largest table goes direct (a probe run is still short)
-    const float directMax = part_fit_set ? static_cast<float>(fit) : kDirectLoad * static_cast<float>(smax);
     float predicted = 0.f;  // the advisor's group count (0: none)
     if (!forced && !triedDirect && pred != nullptr) {
         const float y = pred[static_cast<int64_t>(b) * 4];

@@ -277,10 +277,13 @@ int LaunchBatches(BatchesHandle &h, int64_t total_rows, int nb, bool per_batch)
     if (h.fused && !per_batch) {
         // the advisor's predictions (no per-batch plan: the fused kernel reads the predicted group counts)
         if (h.use_advisor) {
-            // batches of at most a partition's worth of rows go direct without a prediction: not sketched
+            // batches that go direct on their row count alone are not sketched
             const int smax = h.cfg.lite_na ? h.cfg.lite_smax : h.cfg.L2.S;
-            const int64_t fit = h.cfg.part_fit > 0 ? h.cfg.part_fit
-                                                   : static_cast<int64_t>(kFusedLoad * static_cast<float>(smax));
+            // the batch_fused kernel's rows-only direct rule: max(partition fit, kDirectLoad of the table's slots)
+            const int64_t fit = h.cfg.part_fit > 0
+                                    ? h.cfg.part_fit
+                                    : std::max(static_cast<int64_t>(kFusedLoad * static_cast<float>(smax)),
+                                               static_cast<int64_t>(kDirectLoad * static_cast<float>(smax)));
             OG_TRY(LaunchBatchAdvisor(s, h.advisor, h.cfg.kd.values[0], h.cfg.kd.nulls[0], h.cfg.kd.width[0],
                                       h.offsets.as<int64_t>(), nb, h.adv_x.as<float>(), h.adv_y.as<float>(), nullptr,
                                       0.f, fit));
