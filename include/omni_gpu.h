@@ -54,6 +54,7 @@ extern "C" {
 #define OMNI_TYPE_DECIMAL128 7
 #define OMNI_TYPE_DATE32 8
 #define OMNI_TYPE_TIMESTAMP 12
+#define OMNI_TYPE_BYTE 18       /* int8; a group-by or join key column only */
 
 #define OMNI_AGG_SUM 0
 #define OMNI_AGG_COUNT_COLUMN 1
@@ -150,7 +151,7 @@ int omni_gpu_profile_get(const char *name_prefix, int64_t *launches, double *tot
  * aggregates (omnioperator_amd/host/gpu_operators.cpp). */
 typedef struct omni_gpu_agg_spec {
     int32_t num_keys;
-    int32_t key_types[OMNI_GPU_MAX_KEYS];
+    int32_t key_types[OMNI_GPU_MAX_KEYS];   /* any fixed-width type id above, BYTE included */
     int32_t num_aggs;
     int32_t agg_funcs[OMNI_GPU_MAX_AGGS];      /* OMNI_AGG_* */
     int32_t agg_in_types[OMNI_GPU_MAX_AGGS];   /* ignored for COUNT_ALL */
@@ -272,13 +273,14 @@ int omni_gpu_agg_batches_destroy(void *handle);
  * (core/src/operator/join/lookup_join.cpp:924-1035) with its output gather (:1657-1779). */
 typedef struct omni_gpu_join_spec {
     int32_t key_type;                 /* single equality key (num_keys <= 1): INT, DATE32, SHORT, LONG, TIMESTAMP,
-                                         DECIMAL64, DOUBLE (compared as its 64-bit pattern), BOOLEAN, DECIMAL128 */
+                                         DECIMAL64, DOUBLE (compared as its 64-bit pattern), BOOLEAN (0 or 1),
+                                         BYTE, DECIMAL128 */
     int32_t num_payloads;             /* build-side output columns */
     int32_t payload_types[OMNI_GPU_MAX_AGGS];
     int32_t strategy;                 /* 0 = reference rule (array table if dense), 1 = hash only */
     /* Several equality key columns (buildHashCols of HashBuilderOperatorFactory, hash_builder.cpp:44-93): num_keys
      * in [2, OMNI_GPU_MAX_KEYS] fixed-width columns of key_types (0 or 1: the single key_type).  Keys match as the
-     * reference's table compares them: SHORT / INT / LONG / DECIMAL64 columns of at most 32, 64 or 128 bits in
+     * reference's table compares them: BYTE / SHORT / INT / LONG / DECIMAL64 columns of at most 32, 64 or 128 bits in
      * total by their ComputeMultiColKey packing (join_hash_table_variants.cpp:105-146; each value is sign-extended
      * into the key before it is OR-ed in, so a negative later column overwrites the earlier ones and such tuples
      * match), other combinations by the bytes of all their values.  With num_keys >= 2 or a DECIMAL128 key, the

@@ -18,6 +18,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libomni_gpu.so")
 # DataTypeId / FunctionType values shared with the reference (core/src/type/data_type.h:40-65,
 # core/src/operator/util/function_type.h:11-27).
 INT, LONG, DOUBLE, BOOLEAN, SHORT, DECIMAL64, DECIMAL128, DATE32, TIMESTAMP = 1, 2, 3, 4, 5, 6, 7, 8, 12
+BYTE = 18
 SUM, COUNT_COLUMN, COUNT_ALL, AVG, MAX, MIN = 0, 1, 2, 3, 5, 6
 STRATEGY_AUTO, STRATEGY_LDS, STRATEGY_GLOBAL, STRATEGY_RADIX = 0, 1, 2, 3
 MAX_KEYS, MAX_AGGS = 4, 16
@@ -28,9 +29,9 @@ BUILD_UNKNOWN, BUILD_LEFT, BUILD_RIGHT = 0, 1, 2
 
 ERRORS = {-1: "INVALID", -2: "HIP", -3: "NOMEM", -4: "OVERFLOW", -5: "OUTPUT_FULL", -6: "STATE"}
 
-WIDTH = {SHORT: 2, INT: 4, DATE32: 4, LONG: 8, TIMESTAMP: 8, DECIMAL64: 8, DOUBLE: 8, DECIMAL128: 16, BOOLEAN: 1}
+WIDTH = {SHORT: 2, INT: 4, DATE32: 4, LONG: 8, TIMESTAMP: 8, DECIMAL64: 8, DOUBLE: 8, DECIMAL128: 16, BOOLEAN: 1, BYTE: 1}
 NP = {INT: np.int32, DATE32: np.int32, LONG: np.int64, TIMESTAMP: np.int64, DECIMAL64: np.int64,
-      DOUBLE: np.float64, SHORT: np.int16, BOOLEAN: np.uint8, DECIMAL128: np.uint64}
+      DOUBLE: np.float64, SHORT: np.int16, BOOLEAN: np.uint8, BYTE: np.int8, DECIMAL128: np.uint64}
 
 
 class GpuError(RuntimeError):

@@ -485,7 +485,7 @@ int PlanJoinKeys(const omni_gpu_join_spec &sp, JoinKeyCols &kc)
     }
     if (n == 1) {
         const int t = kc.type[0];
-        if (IsIntegerKeyType(t) || t == OMNI_TYPE_DOUBLE || t == OMNI_TYPE_BOOLEAN) {
+        if (IsIntegerKeyType(t) || t == OMNI_TYPE_DOUBLE || t == OMNI_TYPE_BOOLEAN || t == OMNI_TYPE_BYTE) {
             kc.mode = kKeySingle;
         } else {
             OG_CHECK(t == OMNI_TYPE_DECIMAL128, OMNI_GPU_ERR_INVALID, "join key type %d is not supported", t);
@@ -494,11 +494,12 @@ int PlanJoinKeys(const omni_gpu_join_spec &sp, JoinKeyCols &kc)
         }
         return OMNI_GPU_OK;
     }
-    // GetTypeLength: SHORT 16, INT 32, LONG and DECIMAL64 64 bits; any other type serialises the key
+    // GetTypeLength: BYTE 8, SHORT 16, INT 32, LONG and DECIMAL64 64 bits; any other type serialises the key
     int bits = 0;
     bool packable = true;
     for (int c = 0; c < n; ++c) {
         switch (kc.type[c]) {
+            case OMNI_TYPE_BYTE: kc.bits[c] = 8; break;
             case OMNI_TYPE_SHORT: kc.bits[c] = 16; break;
             case OMNI_TYPE_INT: kc.bits[c] = 32; break;
             case OMNI_TYPE_LONG:

@@ -359,10 +359,8 @@ __device__ __forceinline__ void LoadProbeKeys(const ProbeBatch &b, int64_t first
         case 8: LoadKeysT<G, long long, STRIDE>(b.keys, b.rows, first, key, live); break;
         case 4: LoadKeysT<G, int, STRIDE>(b.keys, b.rows, first, key, live); break;
         case 2: LoadKeysT<G, short, STRIDE>(b.keys, b.rows, first, key, live); break;
-        default:  // BOOLEAN: any non-zero byte is true (1), as LoadWord reads it on the build side
-            LoadKeysT<G, uint8_t, STRIDE>(b.keys, b.rows, first, key, live);
-#pragma unroll
-            for (int j = 0; j < G; ++j) key[j] = key[j] != 0 ? 1 : 0;
+        default:  // BYTE, or BOOLEAN (0 or 1): int8, as LoadWord reads it on the build side
+            LoadKeysT<G, int8_t, STRIDE>(b.keys, b.rows, first, key, live);
             break;
     }
     if (b.nulls != nullptr) {

@@ -211,14 +211,14 @@ __global__ void __launch_bounds__(kKeyThreads) join_translate_keys(JoinKeyCols k
         const void *v = b.values[c];
         const uint8_t *nl = b.nulls[c];
         const int width = kc.width[c];
-        unsigned long long x[R], y[R];  // value: sign-extended (width 2 / 4 / 8), zero-extended (1); y = high word
+        unsigned long long x[R], y[R];  // value sign-extended (BYTE and BOOLEAN read as int8); y = high word
 #pragma unroll
         for (int k = 0; k < R; ++k) {
             y[k] = 0;
             if (width == 8) x[k] = static_cast<const unsigned long long *>(v)[ci[k]];
             else if (width == 4) x[k] = static_cast<unsigned long long>(static_cast<long long>(static_cast<const int *>(v)[ci[k]]));
             else if (width == 2) x[k] = static_cast<unsigned long long>(static_cast<long long>(static_cast<const short *>(v)[ci[k]]));
-            else if (width == 1) x[k] = static_cast<const uint8_t *>(v)[ci[k]];
+            else if (width == 1) x[k] = static_cast<unsigned long long>(static_cast<long long>(static_cast<const int8_t *>(v)[ci[k]]));
             else {
                 x[k] = static_cast<const unsigned long long *>(v)[2 * ci[k]];
                 y[k] = static_cast<const unsigned long long *>(v)[2 * ci[k] + 1];

@@ -147,7 +147,8 @@ inline int TypeWidth(int32_t t)
         case OMNI_TYPE_DECIMAL64:
         case OMNI_TYPE_DOUBLE: return 8;
         case OMNI_TYPE_DECIMAL128: return 16;
-        case OMNI_TYPE_BOOLEAN: return 1;
+        case OMNI_TYPE_BOOLEAN:
+        case OMNI_TYPE_BYTE: return 1;
         default: return 0;
     }
 }
@@ -167,14 +168,15 @@ __device__ __forceinline__ bool NullBit(const uint8_t *nulls, int64_t i)
     return nulls != nullptr && ((nulls[i >> 3] >> (i & 7)) & 1);
 }
 
-// Loads element i of an integer-like column of width 1/2/4/8 as a sign-extended 64-bit word.
+// Loads element i of an integer-like column of width 1/2/4/8 as a sign-extended 64-bit word.  Width 1 is BYTE
+// (int8) or BOOLEAN, whose values are 0 or 1 (the reference's Vector<bool>), so both read as int8.
 __device__ __forceinline__ unsigned long long LoadWord(const void *v, int width, int64_t i)
 {
     switch (width) {
         case 8: return static_cast<const unsigned long long *>(v)[i];
         case 4: return static_cast<unsigned long long>(static_cast<long long>(static_cast<const int32_t *>(v)[i]));
         case 2: return static_cast<unsigned long long>(static_cast<long long>(static_cast<const int16_t *>(v)[i]));
-        default: return static_cast<const uint8_t *>(v)[i] ? 1ULL : 0ULL;
+        default: return static_cast<unsigned long long>(static_cast<long long>(static_cast<const int8_t *>(v)[i]));
     }
 }
 

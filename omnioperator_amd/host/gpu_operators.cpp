@@ -37,6 +37,7 @@ void *FlatValues(vec::BaseVector *v)
     switch (v->GetTypeId()) {
         case type::OMNI_DOUBLE: return UnsafeVector::GetRawValues(static_cast<Vector<double> *>(v));
         case type::OMNI_BOOLEAN: return UnsafeVector::GetRawValues(static_cast<Vector<uint8_t> *>(v));
+        case type::OMNI_BYTE: return UnsafeVector::GetRawValues(static_cast<Vector<int8_t> *>(v));
         case type::OMNI_SHORT: return UnsafeVector::GetRawValues(static_cast<Vector<int16_t> *>(v));
         case type::OMNI_INT:
         case type::OMNI_DATE32: return UnsafeVector::GetRawValues(static_cast<Vector<int32_t> *>(v));
@@ -69,6 +70,7 @@ const void *ColumnValues(vec::BaseVector *v, int32_t rows, std::vector<uint8_t> 
             switch (v->GetTypeId()) {
                 case type::OMNI_DOUBLE: GatherDictionary<double>(v, rows, tmp); break;
                 case type::OMNI_BOOLEAN: GatherDictionary<uint8_t>(v, rows, tmp); break;
+                case type::OMNI_BYTE: GatherDictionary<int8_t>(v, rows, tmp); break;
                 case type::OMNI_SHORT: GatherDictionary<int16_t>(v, rows, tmp); break;
                 case type::OMNI_INT:
                 case type::OMNI_DATE32: GatherDictionary<int32_t>(v, rows, tmp); break;
@@ -576,6 +578,7 @@ void StoreAt(vec::BaseVector *v, int32_t typeId, int32_t r, int64_t i, double d)
     switch (typeId) {
         case type::OMNI_DOUBLE: static_cast<vec::Vector<double> *>(v)->SetValue(r, d); break;
         case type::OMNI_BOOLEAN: static_cast<vec::Vector<uint8_t> *>(v)->SetValue(r, static_cast<uint8_t>(i)); break;
+        case type::OMNI_BYTE: static_cast<vec::Vector<int8_t> *>(v)->SetValue(r, static_cast<int8_t>(i)); break;
         case type::OMNI_SHORT: static_cast<vec::Vector<int16_t> *>(v)->SetValue(r, static_cast<int16_t>(i)); break;
         case type::OMNI_INT:
         case type::OMNI_DATE32: static_cast<vec::Vector<int32_t> *>(v)->SetValue(r, static_cast<int32_t>(i)); break;

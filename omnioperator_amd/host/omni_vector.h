@@ -34,6 +34,7 @@ enum DataTypeId {
     OMNI_DATE64 = 9,
     OMNI_TIMESTAMP = 12,
     OMNI_VARCHAR = 15,
+    OMNI_BYTE = 18,
 };
 
 // Decimal128 memory layout of the reference: little-endian {uint64 low; int64 high}.
@@ -69,6 +70,7 @@ inline DataTypePtr TimestampType() { return std::make_shared<DataType>(OMNI_TIME
 inline DataTypePtr Decimal64Type(int32_t p = 18, int32_t s = 0) { return std::make_shared<DataType>(OMNI_DECIMAL64, p, s); }
 inline DataTypePtr Decimal128Type(int32_t p = 38, int32_t s = 0) { return std::make_shared<DataType>(OMNI_DECIMAL128, p, s); }
 inline DataTypePtr BooleanType() { return std::make_shared<DataType>(OMNI_BOOLEAN); }
+inline DataTypePtr ByteType() { return std::make_shared<DataType>(OMNI_BYTE); }
 
 class DataTypes {
 public:
@@ -101,7 +103,8 @@ inline int32_t TypeWidth(int32_t id)
         case OMNI_DOUBLE:
         case OMNI_DATE64: return 8;
         case OMNI_DECIMAL128: return 16;
-        case OMNI_BOOLEAN: return 1;
+        case OMNI_BOOLEAN:
+        case OMNI_BYTE: return 1;
         default: return 0;
     }
 }
@@ -307,6 +310,7 @@ struct VectorHelper {
         switch (typeId) {
             case type::OMNI_DOUBLE: return new Vector<double>(size, id);
             case type::OMNI_BOOLEAN: return new Vector<uint8_t>(size, id);
+            case type::OMNI_BYTE: return new Vector<int8_t>(size, id);
             case type::OMNI_SHORT: return new Vector<int16_t>(size, id);
             case type::OMNI_INT:
             case type::OMNI_DATE32: return new Vector<int32_t>(size, id);

@@ -23,12 +23,13 @@ LIB_PATH = os.path.join(_HERE, "lib", "liboracle.so")
 # DataTypeId values (core/src/type/data_type.h:40-65) and FunctionType values
 # (core/src/operator/util/function_type.h:11-27).
 INT, LONG, DOUBLE, BOOLEAN, SHORT, DECIMAL64, DECIMAL128, DATE32, TIMESTAMP = 1, 2, 3, 4, 5, 6, 7, 8, 12
+BYTE = 18
 SUM, COUNT_COLUMN, COUNT_ALL, AVG, MAX, MIN = 0, 1, 2, 3, 5, 6
 MAX_KEYS, MAX_AGGS = 4, 16
 
 NP_DTYPE = {
     INT: np.int32, DATE32: np.int32, LONG: np.int64, TIMESTAMP: np.int64, DECIMAL64: np.int64,
-    DOUBLE: np.float64, SHORT: np.int16, BOOLEAN: np.uint8,
+    DOUBLE: np.float64, SHORT: np.int16, BOOLEAN: np.uint8, BYTE: np.int8,
 }
 
 
@@ -268,7 +269,7 @@ class SwissMap:
 # ---- aggregation ------------------------------------------------------------------------------------------
 def type_width(type_id: int) -> int:
     return {SHORT: 2, INT: 4, DATE32: 4, LONG: 8, TIMESTAMP: 8, DECIMAL64: 8, DOUBLE: 8, DECIMAL128: 16,
-            BOOLEAN: 1}[type_id]
+            BOOLEAN: 1, BYTE: 1}[type_id]
 
 
 def alloc_out(type_id: int, n: int) -> Col:

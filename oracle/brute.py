@@ -22,7 +22,7 @@ from typing import List, Optional, Sequence
 
 import numpy as np
 
-from . import (AVG, BUILD_LEFT, BUILD_RIGHT, COUNT_ALL, COUNT_COLUMN, DECIMAL128, DECIMAL64, DOUBLE, INT,
+from . import (AVG, BUILD_LEFT, BUILD_RIGHT, BYTE, COUNT_ALL, COUNT_COLUMN, DECIMAL128, DECIMAL64, DOUBLE, INT,
                JOIN_EXISTENCE, JOIN_FULL, JOIN_LEFT, JOIN_LEFT_ANTI, JOIN_LEFT_SEMI, JOIN_RIGHT, LONG, MAX, MIN, SHORT,
                SUM, Col, np_dtype)
 
@@ -188,7 +188,7 @@ def inner_join(build_key: Col, probe_key: Col):
     return probe_rows[o], build_rows[o]
 
 
-_KEY_BITS = {SHORT: 16, INT: 32, LONG: 64, DECIMAL64: 64}
+_KEY_BITS = {BYTE: 8, SHORT: 16, INT: 32, LONG: 64, DECIMAL64: 64}
 
 
 def join_keys(cols: Sequence[Col]) -> List[Optional[object]]:

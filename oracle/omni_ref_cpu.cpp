@@ -213,7 +213,8 @@ inline int TypeWidth(int32_t t)
         case OMNI_TYPE_DECIMAL64:
         case OMNI_TYPE_DOUBLE: return 8;
         case OMNI_TYPE_DECIMAL128: return 16;
-        case OMNI_TYPE_BOOLEAN: return 1;
+        case OMNI_TYPE_BOOLEAN:
+        case OMNI_TYPE_BYTE: return 1;
         default: return 0;
     }
 }
@@ -225,6 +226,7 @@ inline int64_t LoadInt(const HostColumn &c, int64_t i)
         case OMNI_TYPE_INT:
         case OMNI_TYPE_DATE32: return static_cast<const int32_t *>(c.values)[i];
         case OMNI_TYPE_BOOLEAN: return static_cast<const uint8_t *>(c.values)[i] ? 1 : 0;
+        case OMNI_TYPE_BYTE: return static_cast<const int8_t *>(c.values)[i];
         default: return static_cast<const int64_t *>(c.values)[i];
     }
 }
@@ -947,6 +949,20 @@ void PartitionIds(const HostColumn *cols, int32_t ncols, int64_t rows, int32_t p
 // keys are numbered by a dictionary of the build side's keys, and a probe key outside it has no key.
 class JoinKeyCodec {
 public:
+    // GetTypeLength (hash_builder.cpp:96-119): the bits a key column takes in a packed key, 0 when the type
+    // makes the key serialised.
+    static int PackBits(int32_t t)
+    {
+        switch (t) {
+            case OMNI_TYPE_BYTE: return 8;
+            case OMNI_TYPE_SHORT: return 16;
+            case OMNI_TYPE_INT: return 32;
+            case OMNI_TYPE_LONG:
+            case OMNI_TYPE_DECIMAL64: return 64;
+            default: return 0;
+        }
+    }
+
     explicit JoinKeyCodec(const std::vector<int32_t> &types) : types_(types)
     {
         const size_t n = types.size();
@@ -957,13 +973,9 @@ public:
         int bits = 0;
         bool fixedInt = true;
         for (int32_t t : types) {
-            switch (t) {
-                case OMNI_TYPE_SHORT: bits += 16; break;
-                case OMNI_TYPE_INT: bits += 32; break;
-                case OMNI_TYPE_LONG:
-                case OMNI_TYPE_DECIMAL64: bits += 64; break;
-                default: fixedInt = false;
-            }
+            const int b = PackBits(t);
+            if (b == 0) fixedInt = false;
+            bits += b;
         }
         if (fixedInt && bits <= 32) kind_ = Kind::int32;
         else if (fixedInt && bits <= 64) kind_ = Kind::int64;
@@ -982,7 +994,7 @@ public:
             case Kind::int32: {  // key = key << bits | (int32_t)value
                 int32_t k = 0;
                 for (size_t c = 0; c < types_.size(); ++c) {
-                    const int shift = types_[c] == OMNI_TYPE_SHORT ? 16 : 32;
+                    const int shift = PackBits(types_[c]);
                     k = static_cast<int32_t>((static_cast<uint32_t>(k) << shift) |
                                              static_cast<uint32_t>(static_cast<int32_t>(LoadInt(cols[c], r))));
                 }
@@ -992,7 +1004,7 @@ public:
             case Kind::int64: {
                 int64_t k = 0;
                 for (size_t c = 0; c < types_.size(); ++c) {
-                    const int shift = types_[c] == OMNI_TYPE_SHORT ? 16 : (types_[c] == OMNI_TYPE_INT ? 32 : 64);
+                    const int shift = PackBits(types_[c]);
                     k = static_cast<int64_t>((static_cast<uint64_t>(k) << shift) |
                                              static_cast<uint64_t>(LoadInt(cols[c], r)));
                 }
@@ -1002,7 +1014,7 @@ public:
             case Kind::int128: {
                 int128 k = 0;
                 for (size_t c = 0; c < types_.size(); ++c) {
-                    const int shift = types_[c] == OMNI_TYPE_SHORT ? 16 : (types_[c] == OMNI_TYPE_INT ? 32 : 64);
+                    const int shift = PackBits(types_[c]);
                     k = static_cast<int128>((static_cast<unsigned __int128>(k) << shift) |
                                             static_cast<unsigned __int128>(static_cast<int128>(LoadInt(cols[c], r))));
                 }

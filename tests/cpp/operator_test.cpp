@@ -40,6 +40,7 @@ static std::string Cell(const BaseVector *v, int32_t r)
         case OMNI_DATE32: std::snprintf(buf, sizeof buf, "%d", static_cast<const Vector<int32_t> *>(v)->GetValue(r)); break;
         case OMNI_DOUBLE: std::snprintf(buf, sizeof buf, "%.12g", static_cast<const Vector<double> *>(v)->GetValue(r)); break;
         case OMNI_BOOLEAN: std::snprintf(buf, sizeof buf, "%s", static_cast<const Vector<uint8_t> *>(v)->GetValue(r) ? "true" : "false"); break;
+        case OMNI_BYTE: std::snprintf(buf, sizeof buf, "%d", static_cast<int>(static_cast<const Vector<int8_t> *>(v)->GetValue(r))); break;
         case OMNI_DECIMAL128: {
             const Decimal128 d = static_cast<const Vector<Decimal128> *>(v)->GetValue(r);
             __int128 x = static_cast<__int128>((static_cast<unsigned __int128>(static_cast<uint64_t>(d.high)) << 64) | d.low);

@@ -432,3 +432,21 @@ def test_array_rule_declines():
     k5 = np.array([0, 699051] * 1000, np.int64)
     got, info = run_gpu_rule([Col(LONG, k5)], aggs, [v2, None], 2000)
     assert info == (False, 0, 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("strategy", [capi.STRATEGY_AUTO, capi.STRATEGY_RADIX])
+def test_byte_key_parity(strategy):
+    """BYTE (int8) group-by keys, negative values and NULLs included, alone and in a composite key."""
+    from oracle import BYTE
+    rng = np.random.default_rng(180 + strategy)
+    n = 60_000
+    b = rng.integers(-128, 128, n).astype(np.int8)
+    bn = rng.random(n) < 0.05
+    i = rng.integers(-40, 40, n).astype(np.int32)
+    v = rng.integers(-10**9, 10**9, n).astype(np.int64)
+    cols = [Col(LONG, v)] * 5
+    for keys in ([Col(BYTE, b, bn)], [Col(INT, i), Col(BYTE, b, bn)], [Col(BYTE, b), Col(LONG, v % 7)]):
+        got = run_gpu(keys, STD_AGGS, cols, 7000, strategy=strategy)
+        want = run_oracle(keys, STD_AGGS, cols, 7000)
+        assert_row_sets_equal(got, want, float_cols=float_cols(len(keys), STD_AGGS), rtol=DOUBLE_RTOL)

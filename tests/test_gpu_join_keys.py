@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle
-from oracle import DATE32, INT, LONG, SHORT, Col
+from oracle import BYTE, DATE32, INT, LONG, SHORT, Col
 from omnioperator_amd import capi, workloads
 from tests.helpers import load_golden
 from tests.test_gpu_join_types import gpu_set, keeps_build, oracle_outer_set, oracle_set, outer_set
@@ -155,7 +155,9 @@ def test_gpu_sign_extension_collisions():
     cases = [(cols(INT, [5, 5], SHORT, [-1, 1]), cols(INT, [7, 7], SHORT, [-1, 1]), [(0, 0)]),
              (cols(SHORT, [5], SHORT, [-1]), cols(SHORT, [9], SHORT, [-1]), [(0, 0)]),
              (cols(LONG, [3], LONG, [-1]), cols(LONG, [4], LONG, [-1]), [(0, 0)]),
-             (cols(DATE32, [5], SHORT, [-1]), cols(DATE32, [7], SHORT, [-1]), [])]
+             (cols(DATE32, [5], SHORT, [-1]), cols(DATE32, [7], SHORT, [-1]), []),
+             (cols(SHORT, [3, 3], BYTE, [-1, 4]), cols(SHORT, [8, 8], BYTE, [-1, 4]), [(0, 0)]),
+             (cols(BYTE, [3, -7], INT, [-1, 9]), cols(BYTE, [5, -7], INT, [-1, 9]), [(0, 0), (1, 1)])]
     for bk, pk, want in cases:
         n = len(bk[0])
         got, _, _ = gpu_multi(bk, [Col(LONG, np.arange(n, dtype=np.int64))], pk, n, n, oracle.JOIN_INNER, 0)

@@ -284,3 +284,25 @@ def test_partial_decimal_sum_output():
     out = oracle.hash_agg(oracle.make_agg_spec([LONG], [(SUM, DECIMAL128, DECIMAL128, 0, 2, True)]),
                           [Col(LONG, np.zeros(1, np.int64))], [Col(DECIMAL128, big)], 16)
     assert cols_rows(out) == [(0, None)]
+
+
+def test_byte_keys_group_as_their_values():
+    """BYTE (int8) key columns group by their values: the oracle's groups over BYTE keys, alone and beside a LONG
+    key, are its groups over the same values stored as SHORT (pinned by the fixtures above), NULL keys included."""
+    from oracle import BYTE
+    from tests.helpers import rows_of
+    rng = np.random.default_rng(18)
+    n = 5000
+    b = rng.integers(-128, 128, n).astype(np.int8)
+    bn = rng.random(n) < 0.05
+    l = rng.integers(-3, 3, n).astype(np.int64)
+    v = rng.integers(-10**6, 10**6, n).astype(np.int64)
+    cols = [Col(LONG, v)] * 5
+    for keys_b, keys_s in (([Col(BYTE, b, bn)], [Col(SHORT, b.astype(np.int16), bn)]),
+                           ([Col(BYTE, b, bn), Col(LONG, l)], [Col(SHORT, b.astype(np.int16), bn), Col(LONG, l)])):
+        got = [oracle.hash_agg(oracle.make_agg_spec([c.type_id for c in k], STD_AGGS, True), k, cols, 1000)
+               for k in (keys_b, keys_s)]
+        rb, rs = (rows_of([(c.type_id, c.values, c.nulls) for c in outs]) for outs in got)
+        assert got[0][0].values.dtype == np.int8
+        assert sorted(rb, key=repr) == sorted(rs, key=repr)
+        assert len(rb) > 250

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from oracle import (BOOLEAN, DATE32, DECIMAL128, DECIMAL64, DOUBLE, INT, LONG, SHORT, TIMESTAMP, Col, brute)
+from oracle import (BOOLEAN, BYTE, DATE32, DECIMAL128, DECIMAL64, DOUBLE, INT, LONG, SHORT, TIMESTAMP, Col, brute)
 from tests.helpers import load_golden
 
 FIX = load_golden("operator_fixtures.json")["join_multi_key"]
@@ -79,6 +79,13 @@ def test_sign_extension_collisions():
     assert _pairs(cols(LONG, [3], LONG, [-1]), cols(LONG, [4], LONG, [-1])) == [(0, 0)]
     # the serialised variant compares the stored values: no such collision for (DATE32, SHORT)
     assert _pairs(cols(DATE32, [5], SHORT, [-1]), cols(DATE32, [7], SHORT, [-1])) == []
+    # BYTE packs in 8 bits (GetTypeLength, hash_builder.cpp:96-119), sign-extended like the others: (SHORT, BYTE)
+    # (3, -1) and (8, -1) are the same int32 key; (INT, INT, BYTE) -- 72 bits, an int128 key -- (1, 2, -1) and
+    # (6, 7, -1) too; with a non-negative BYTE the SHORT survives
+    assert _pairs(cols(SHORT, [3, 3], BYTE, [-1, 4]), cols(SHORT, [8, 8], BYTE, [-1, 4])) == [(0, 0)]
+    three = [Col(INT, np.array([1], np.int32)), Col(INT, np.array([2], np.int32)), Col(BYTE, np.array([-1], np.int8))]
+    other = [Col(INT, np.array([6], np.int32)), Col(INT, np.array([7], np.int32)), Col(BYTE, np.array([-1], np.int8))]
+    assert _pairs(three, other) == [(0, 0)]
     # brute force agrees on each of them
     assert brute.join_keys(cols(INT, [5], SHORT, [-1])) == brute.join_keys(cols(INT, [7], SHORT, [-1]))
     assert brute.join_keys(cols(DATE32, [5], SHORT, [-1])) != brute.join_keys(cols(DATE32, [7], SHORT, [-1]))
@@ -98,6 +105,11 @@ KEY_SETS = {
     "decimal128_int": [DECIMAL128, INT],           # serialised
     "decimal128": [DECIMAL128],                    # one Decimal128 key
     "four_shorts": [SHORT, SHORT, SHORT, SHORT],   # int64 key, 64 bits
+    "byte": [BYTE],                                # one int8 key
+    "short_byte": [SHORT, BYTE],                   # int32 key, 24 bits
+    "byte_int": [BYTE, INT],                       # int64 key, 40 bits
+    "long_int_short_byte": [LONG, INT, SHORT, BYTE],  # int128 key, 120 bits
+    "byte_double": [BYTE, DOUBLE],                 # serialised
 }
 
 
