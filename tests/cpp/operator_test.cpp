@@ -1366,6 +1366,47 @@ static void TestJoinMultiPartitionRouted()
     }
 }
 
+// BYTE (int8) keys through the drop-in operators: a group-by over negative and extreme values, and an inner join
+// on (BYTE, INT) keys, whose packed key (8 + 32 bits) lets (3, -1) and (8, -1) match as ComputeMultiColKey's
+// sign extension makes them (join_hash_table_variants.cpp:105-146).
+static void TestByteKeysThroughOperators()
+{
+    auto *f = StdAggFactory({ByteType()}, 1);
+    Operator *op = f->CreateOperator();
+    auto *batch = new VectorBatch(6);
+    batch->Append(MakeVector<int8_t>(OMNI_BYTE, {-1, -128, 127, -1, 127, -1}));
+    for (int a = 0; a < 5; ++a) batch->Append(MakeVector<int64_t>(OMNI_LONG, {2, 5, 7, 4, 9, 6}));
+    op->AddInput(batch);
+    Rows want = {{"-1", "12", "4", "3", "2", "6"}, {"-128", "5", "5", "1", "5", "5"}, {"127", "16", "8", "2", "7", "9"}};
+    EXPECT_TRUE(SameRowSet(Drain(op), want));
+    Operator::DeleteOperator(op);
+    delete f;
+
+    DataTypes types({ByteType(), IntType()});
+    auto *build = new VectorBatch(3);
+    build->Append(MakeVector<int8_t>(OMNI_BYTE, {3, -7, 100}));
+    build->Append(MakeVector<int32_t>(OMNI_INT, {-1, 9, 10}));
+    int32_t keys[2] = {0, 1};
+    auto *bf = GpuHashBuilderOperatorFactory::CreateHashBuilderOperatorFactory(OMNI_JOIN_TYPE_INNER, types, keys, 2, 1);
+    Operator *builder = bf->CreateOperator();
+    builder->AddInput(build);
+    VectorBatch *none = nullptr;
+    builder->GetOutput(&none);
+    auto *probe = new VectorBatch(3);
+    probe->Append(MakeVector<int8_t>(OMNI_BYTE, {8, -7, 101}));
+    probe->Append(MakeVector<int32_t>(OMNI_INT, {-1, 9, 10}));
+    auto *lf = GpuLookupJoinOperatorFactory::CreateLookupJoinOperatorFactory(
+        types, keys, 2, keys, 2, keys, 2, types, reinterpret_cast<int64_t>(bf), nullptr, false, nullptr);
+    Operator *lookup = lf->CreateOperator();
+    lookup->AddInput(probe);
+    Rows jwant = {{"8", "-1", "3", "-1"}, {"-7", "9", "-7", "9"}};
+    EXPECT_TRUE(SameRowSet(Drain(lookup), jwant));
+    Operator::DeleteOperator(builder);
+    Operator::DeleteOperator(lookup);
+    delete lf;
+    delete bf;
+}
+
 static void TestJoinMultiPartitionUnrouted()
 {
     for (JoinType jt : {OMNI_JOIN_TYPE_INNER, OMNI_JOIN_TYPE_LEFT}) {
@@ -1408,6 +1449,7 @@ int main()
         {"JoinBufferedRunsAnti", TestJoinBufferedRunsAnti},
         {"JoinMultiPartitionRouted", TestJoinMultiPartitionRouted},
         {"JoinMultiPartitionUnrouted", TestJoinMultiPartitionUnrouted},
+        {"ByteKeysThroughOperators", TestByteKeysThroughOperators},
     };
     for (auto &t : tests) {
         int before = g_failures;
